@@ -1,0 +1,247 @@
+"""CPU oracle for the 6502 search core -- TEST INFRASTRUCTURE ONLY.
+
+A ctypes front end over ``liboracle.so`` (built from ``sx_oracle.c``), which
+restates the Go CPU algorithm of oisee/6502-optimizer (pkg/cpu, pkg/inst,
+pkg/search).  Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+CPU-baseline leg import this package, and only as the checker.  The product
+path (``6502-optimizer_amd``) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Iterable, List, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+# Opcode ids, pkg/inst/instruction.go:42-122.
+OPS = [
+    "TAX", "TAY", "TXA", "TYA", "TSX", "TXS", "INX", "INY", "DEX", "DEY",
+    "ASL_A", "LSR_A", "ROL_A", "ROR_A", "CLC", "SEC", "CLD", "SED", "CLI", "SEI",
+    "CLV", "PHA", "PLA", "PHP", "PLP", "NOP", "LDA_IMM", "LDX_IMM", "LDY_IMM",
+    "ADC_IMM", "SBC_IMM", "AND_IMM", "ORA_IMM", "EOR_IMM", "CMP_IMM", "CPX_IMM",
+    "CPY_IMM", "LDA_M", "LDX_M", "LDY_M", "STA_M", "STX_M", "STY_M", "ADC_M",
+    "SBC_M", "AND_M", "ORA_M", "EOR_M", "CMP_M", "CPX_M", "CPY_M", "INC_M",
+    "DEC_M", "ASL_M", "LSR_M", "ROL_M", "ROR_M", "BIT_M",
+]
+OP = {name: i for i, name in enumerate(OPS)}
+globals().update({name: i for i, name in enumerate(OPS)})
+
+REG_A, REG_P, REG_X, REG_Y, REG_S, REG_M, REG_S0, REG_S1 = 1, 2, 4, 8, 16, 32, 64, 128
+
+
+class Rule(C.Structure):
+    _fields_ = [
+        ("src", C.c_uint32 * 4), ("rep", C.c_uint32 * 4), ("target_index", C.c_uint64),
+        ("src_len", C.c_uint8), ("rep_len", C.c_uint8), ("dead_flags", C.c_uint8),
+        ("pass_", C.c_uint8), ("bytes_saved", C.c_int16), ("cycles_saved", C.c_int16),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [("checked", C.c_uint64), ("qc_hits", C.c_uint64), ("mid_pass", C.c_uint64),
+                ("exh_calls", C.c_uint64), ("found", C.c_uint64)]
+
+
+class SearchCfg(C.Structure):
+    _fields_ = [("target_len", C.c_int), ("max_cand_len", C.c_int), ("bytes_model", C.c_int),
+                ("hits", C.c_int), ("exh_mode", C.c_int), ("use_mid", C.c_int),
+                ("nthreads", C.c_int), ("dead", C.c_uint8)]
+
+
+RULE_DTYPE = np.dtype([
+    ("src", "<u4", (4,)), ("rep", "<u4", (4,)), ("target_index", "<u8"),
+    ("src_len", "u1"), ("rep_len", "u1"), ("dead_flags", "u1"), ("pass_", "u1"),
+    ("bytes_saved", "<i2"), ("cycles_saved", "<i2"),
+])
+assert RULE_DTYPE.itemsize == C.sizeof(Rule) == 48
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library missing: build it with `make -C {_HERE}`")
+        L = C.CDLL(_LIB_PATH)
+        u32p = C.POINTER(C.c_uint32)
+        u8p = C.POINTER(C.c_uint8)
+        L.sxo_exec.argtypes = [u8p, C.c_int, C.c_uint16]
+        L.sxo_exec.restype = C.c_int
+        L.sxo_exec_seq.argtypes = [u8p, u32p, C.c_int]
+        L.sxo_exec_many.argtypes = [u8p, u32p, C.c_int64]
+        for fn in ("sxo_quick_check", "sxo_mid_check"):
+            getattr(L, fn).argtypes = [u32p, C.c_int, u32p, C.c_int, C.c_uint8]
+            getattr(L, fn).restype = C.c_int
+        L.sxo_exhaustive_check.argtypes = [u32p, C.c_int, u32p, C.c_int, C.c_uint8, C.c_int]
+        L.sxo_exhaustive_check.restype = C.c_int
+        L.sxo_flag_diff.argtypes = [u32p, C.c_int, u32p, C.c_int]
+        L.sxo_flag_diff.restype = C.c_uint8
+        L.sxo_fingerprint.argtypes = [u32p, C.c_int, u8p]
+        L.sxo_should_prune.argtypes = [u32p, C.c_int]
+        L.sxo_should_prune.restype = C.c_int
+        L.sxo_enumerate_pruned.argtypes = [C.c_int, u32p, C.c_int64]
+        L.sxo_enumerate_pruned.restype = C.c_int64
+        L.sxo_instruction_table.argtypes = [u32p, C.c_int]
+        L.sxo_instruction_table.restype = C.c_int
+        L.sxo_disasm.argtypes = [C.c_uint32, C.c_char_p, C.c_int]
+        L.sxo_disasm.restype = C.c_int
+        L.sxo_qc_vectors.restype = u8p
+        L.sxo_mid_vectors.restype = u8p
+        L.sxo_search.argtypes = [C.POINTER(SearchCfg), u32p, C.c_int64, C.POINTER(Rule),
+                                 C.c_int64, C.POINTER(C.c_int64), C.POINTER(Stats)]
+        L.sxo_search.restype = C.c_int
+        L.sxo_states_equal_masked.argtypes = [u8p, u8p, C.c_uint8]
+        L.sxo_states_equal_masked.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def word(op: int, imm: int = 0) -> int:
+    return (op & 0xFFFF) | ((imm & 0xFFFF) << 16)
+
+
+def _seq(seq: Sequence[int]):
+    arr = (C.c_uint32 * max(1, len(seq)))(*[int(w) for w in seq])
+    return arr, len(seq)
+
+
+def state(a=0, x=0, y=0, p=0, s=0, m=0, s0=0, s1=0) -> List[int]:
+    return [a, x, y, p, s, m, s0, s1]
+
+
+def exec_op(st: Sequence[int], op: int, imm: int = 0):
+    buf = (C.c_uint8 * 8)(*st)
+    cyc = lib().sxo_exec(buf, op, imm)
+    return list(buf), cyc
+
+
+def exec_seq(st: Sequence[int], seq: Sequence[int]) -> List[int]:
+    buf = (C.c_uint8 * 8)(*st)
+    a, n = _seq(seq)
+    lib().sxo_exec_seq(buf, a, n)
+    return list(buf)
+
+
+def exec_many(states: np.ndarray, words: np.ndarray) -> np.ndarray:
+    """Run words[i] on states[i] (shape (n, 8) uint8); returns the new states."""
+    st = np.ascontiguousarray(states, dtype=np.uint8).copy()
+    w = np.ascontiguousarray(np.broadcast_to(np.asarray(words, dtype=np.uint32), (len(st),)))
+    lib().sxo_exec_many(st.ctypes.data_as(C.POINTER(C.c_uint8)),
+                        w.ctypes.data_as(C.POINTER(C.c_uint32)), len(st))
+    return st
+
+
+def quick_check(t, c, dead: int = 0) -> bool:
+    ta, tn = _seq(t)
+    ca, cn = _seq(c)
+    return bool(lib().sxo_quick_check(ta, tn, ca, cn, dead))
+
+
+def mid_check(t, c, dead: int = 0) -> bool:
+    ta, tn = _seq(t)
+    ca, cn = _seq(c)
+    return bool(lib().sxo_mid_check(ta, tn, ca, cn, dead))
+
+
+def exhaustive_check(t, c, dead: int = 0, mode: int = 0) -> bool:
+    """mode 0 = Go (reduced sweep for >= 3 extra registers), 1 = complete."""
+    ta, tn = _seq(t)
+    ca, cn = _seq(c)
+    return bool(lib().sxo_exhaustive_check(ta, tn, ca, cn, dead, mode))
+
+
+def flag_diff(t, c) -> int:
+    ta, tn = _seq(t)
+    ca, cn = _seq(c)
+    return int(lib().sxo_flag_diff(ta, tn, ca, cn))
+
+
+def fingerprint(seq) -> bytes:
+    a, n = _seq(seq)
+    out = (C.c_uint8 * 64)()
+    lib().sxo_fingerprint(a, n, out)
+    return bytes(out)
+
+
+def should_prune(seq) -> bool:
+    a, n = _seq(seq)
+    return bool(lib().sxo_should_prune(a, n))
+
+
+def op_reads(op: int) -> int:
+    return lib().sxo_op_reads(op)
+
+
+def op_writes(op: int) -> int:
+    return lib().sxo_op_writes(op)
+
+
+def byte_size(op: int, model: int = 0) -> int:
+    return lib().sxo_byte_size(op, model)
+
+
+def cycles(op: int) -> int:
+    return lib().sxo_cycles(op)
+
+
+def disasm(w: int) -> str:
+    buf = C.create_string_buffer(32)
+    lib().sxo_disasm(w, buf, 32)
+    return buf.value.decode()
+
+
+def instruction_table(order: int = 0) -> np.ndarray:
+    out = np.zeros(2863, dtype=np.uint32)
+    n = lib().sxo_instruction_table(out.ctypes.data_as(C.POINTER(C.c_uint32)), order)
+    assert n == 2863
+    return out
+
+
+def qc_vectors() -> np.ndarray:
+    p = lib().sxo_qc_vectors()
+    return np.ctypeslib.as_array(p, shape=(64,)).reshape(8, 8).copy()
+
+
+def mid_vectors() -> np.ndarray:
+    p = lib().sxo_mid_vectors()
+    return np.ctypeslib.as_array(p, shape=(256,)).reshape(32, 8).copy()
+
+
+def enumerate_pruned(length: int) -> np.ndarray:
+    """Pruned sequences of the given length (1 or 2) in Go order, shape (n, length)."""
+    n = lib().sxo_enumerate_pruned(length, None, 0)
+    out = np.zeros((n, length), dtype=np.uint32)
+    m = lib().sxo_enumerate_pruned(length, out.ctypes.data_as(C.POINTER(C.c_uint32)), n)
+    assert m == n
+    return out
+
+
+def search(targets: np.ndarray, *, max_cand_len: int = 1, dead: int = 0, bytes_model: int = 0,
+           hits: int = 0, exh_mode: int = 0, use_mid: int = 0, nthreads: int | None = None):
+    """Run the restated worker pool over ``targets`` (shape (n, len) uint32).
+
+    Returns (rules as a RULE_DTYPE array sorted by target index then pass, Stats).
+    """
+    targets = np.ascontiguousarray(targets, dtype=np.uint32)
+    n, tl = targets.shape
+    cfg = SearchCfg(tl, max_cand_len, bytes_model, hits, exh_mode, use_mid,
+                    nthreads or os.cpu_count() or 1, dead)
+    st = Stats()
+    nout = C.c_int64(0)
+    tp = targets.ctypes.data_as(C.POINTER(C.c_uint32))
+    cap = 2 * n + 1024 if hits == 0 else 8 * n + 1024
+    while True:
+        out = np.zeros(cap, dtype=RULE_DTYPE)
+        rc = lib().sxo_search(C.byref(cfg), tp, n, out.ctypes.data_as(C.POINTER(Rule)), cap,
+                              C.byref(nout), C.byref(st))
+        if rc == -3:  # capacity too small: the required count is in nout
+            cap = nout.value
+            continue
+        if rc != 0:
+            raise RuntimeError(f"sxo_search failed: {rc}")
+        return out[:nout.value].copy(), st

@@ -1,0 +1,878 @@
+/*
+ * sx_oracle.c -- CPU oracle for the 6502 search core (TEST INFRASTRUCTURE).
+ *
+ * A plain-C restatement of the Go algorithm of oisee/6502-optimizer, written
+ * from the Go sources read as text.  Only tests/, the smoke check and the
+ * bench's CPU-baseline leg load it, always as the checker.  The product
+ * library (6502-optimizer_amd/csrc) never links or calls this file.
+ *
+ * Citations are path:line in the reference tree.
+ */
+#include "sx_oracle.h"
+
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ ISA -- */
+
+/* pkg/inst/instruction.go:13-21 */
+int sxo_has_imm(int op) { return op >= SXO_LDA_IMM && op <= SXO_CPY_IMM; }
+
+/* pkg/inst/catalog.go:105-201 (mnemonics, encodings, cycles). */
+static const char *const k_mnem[SXO_OPCOUNT] = {
+  "TAX", "TAY", "TXA", "TYA", "TSX", "TXS",
+  "INX", "INY", "DEX", "DEY",
+  "ASL A", "LSR A", "ROL A", "ROR A",
+  "CLC", "SEC", "CLD", "SED", "CLI", "SEI", "CLV",
+  "PHA", "PLA", "PHP", "PLP", "NOP",
+  "LDA #n", "LDX #n", "LDY #n", "ADC #n", "SBC #n",
+  "AND #n", "ORA #n", "EOR #n", "CMP #n", "CPX #n", "CPY #n",
+  "LDA M", "LDX M", "LDY M", "STA M", "STX M", "STY M",
+  "ADC M", "SBC M", "AND M", "ORA M", "EOR M",
+  "CMP M", "CPX M", "CPY M", "INC M", "DEC M",
+  "ASL M", "LSR M", "ROL M", "ROR M", "BIT M",
+};
+
+const char *sxo_mnemonic(int op) {
+  return (op >= 0 && op < SXO_OPCOUNT) ? k_mnem[op] : NULL;
+}
+
+/* Go ByteSize (catalog.go:50-56): one encoding byte, plus one for an
+ * immediate, so the 21 M ops count 1 byte.  The CUDA tool counts every
+ * non-implied op as 2 bytes (cuda/6502_search_v2.cu:437-440). */
+int sxo_byte_size(int op, int model) {
+  if (model == 1) return op <= SXO_NOP ? 1 : 2;
+  return sxo_has_imm(op) ? 2 : 1;
+}
+
+/* catalog.go:108-200 cycle column. */
+int sxo_cycles(int op) {
+  switch (op) {
+  case SXO_PHA: case SXO_PHP: return 3;
+  case SXO_PLA: case SXO_PLP: return 4;
+  case SXO_INC_M: case SXO_DEC_M: case SXO_ASL_M: case SXO_LSR_M:
+  case SXO_ROL_M: case SXO_ROR_M: return 5;
+  default: break;
+  }
+  if (op >= SXO_LDA_M && op <= SXO_BIT_M) return 3;
+  return 2;
+}
+
+/* catalog.go:59-85: the 'n' placeholder becomes $XX in upper-case hex. */
+int sxo_disasm(uint32_t w, char *buf, int cap) {
+  int op = SXO_OP(w);
+  if (op >= SXO_OPCOUNT) return snprintf(buf, cap, "OP(%d)", op);
+  const char *m = k_mnem[op];
+  if (!sxo_has_imm(op)) return snprintf(buf, cap, "%s", m);
+  static const char hx[] = "0123456789ABCDEF";
+  char tmp[32];
+  int o = 0;
+  for (int i = 0; m[i]; i++) {
+    if (m[i] == 'n' && (i == 0 || m[i - 1] == '#' || m[i - 1] == ' ')) {
+      uint8_t v = (uint8_t)SXO_IMM(w);
+      tmp[o++] = '$';
+      tmp[o++] = hx[v >> 4];
+      tmp[o++] = hx[v & 15];
+    } else {
+      tmp[o++] = m[i];
+    }
+  }
+  tmp[o] = 0;
+  return snprintf(buf, cap, "%s", tmp);
+}
+
+/* Go order: enumerator.go:14-38 (non-immediate ops in opcode order, then each
+ * immediate op with imm 0..255).  CUDA order: cuda/6502_search_v2.cu:741-752
+ * (opcode order with immediates inline). */
+int sxo_instruction_table(uint32_t *out, int order) {
+  int n = 0;
+  if (order == 0) {
+    for (int op = 0; op < SXO_OPCOUNT; op++)
+      if (!sxo_has_imm(op)) out[n++] = SXO_WORD(op, 0);
+    for (int op = 0; op < SXO_OPCOUNT; op++)
+      if (sxo_has_imm(op))
+        for (int imm = 0; imm < 256; imm++) out[n++] = SXO_WORD(op, imm);
+  } else {
+    for (int op = 0; op < SXO_OPCOUNT; op++) {
+      if (sxo_has_imm(op)) {
+        for (int imm = 0; imm < 256; imm++) out[n++] = SXO_WORD(op, imm);
+      } else {
+        out[n++] = SXO_WORD(op, 0);
+      }
+    }
+  }
+  return n;
+}
+
+/* ------------------------------------------------------------- Executor -- */
+
+#define FC 0x01
+#define FZ 0x02
+#define FI 0x04
+#define FD 0x08
+#define FB 0x10
+#define FU 0x20
+#define FV 0x40
+#define FN 0x80
+#define PMASK 0xCF
+
+/* pkg/cpu/flags.go:17-30 */
+static inline uint8_t nz(uint8_t v) { return (uint8_t)((v == 0 ? FZ : 0) | (v & 0x80 ? FN : 0)); }
+
+/* exec.go:194-197 */
+static inline void set_nz(uint8_t *s, uint8_t v) {
+  s[SXO_P] = (uint8_t)((s[SXO_P] & ~(FN | FZ)) | nz(v));
+}
+
+/* exec.go:202-220 */
+static inline void do_adc(uint8_t *s, uint8_t m) {
+  unsigned a = s[SXO_A], c = s[SXO_P] & FC;
+  unsigned r = a + m + c;
+  uint8_t v = (uint8_t)((a ^ (uint8_t)r) & (m ^ (uint8_t)r) & 0x80);
+  s[SXO_A] = (uint8_t)r;
+  s[SXO_P] &= (uint8_t)~(FC | FZ | FN | FV);
+  s[SXO_P] |= nz(s[SXO_A]);
+  if (r > 0xFF) s[SXO_P] |= FC;
+  if (v) s[SXO_P] |= FV;
+}
+
+/* exec.go:224-242: uint16 arithmetic, so a borrow wraps above 0xFF. */
+static inline void do_sbc(uint8_t *s, uint8_t m) {
+  uint16_t a = s[SXO_A], c = s[SXO_P] & FC;
+  uint16_t r = (uint16_t)(a - m - (1 - c));
+  uint8_t v = (uint8_t)((s[SXO_A] ^ (uint8_t)r) & (s[SXO_A] ^ m) & 0x80);
+  s[SXO_A] = (uint8_t)r;
+  s[SXO_P] &= (uint8_t)~(FC | FZ | FN | FV);
+  s[SXO_P] |= nz(s[SXO_A]);
+  if (r <= 0xFF) s[SXO_P] |= FC;
+  if (v) s[SXO_P] |= FV;
+}
+
+/* exec.go:246-253 */
+static inline void do_cmp(uint8_t *s, uint8_t reg, uint8_t m) {
+  uint16_t r = (uint16_t)((uint16_t)reg - (uint16_t)m);
+  s[SXO_P] &= (uint8_t)~(FC | FZ | FN);
+  s[SXO_P] |= nz((uint8_t)r);
+  if (reg >= m) s[SXO_P] |= FC;
+}
+
+/* exec.go:256-294 */
+static inline void do_asl(uint8_t *s, uint8_t *v) {
+  s[SXO_P] &= (uint8_t)~FC;
+  if (*v & 0x80) s[SXO_P] |= FC;
+  *v = (uint8_t)(*v << 1);
+  set_nz(s, *v);
+}
+static inline void do_lsr(uint8_t *s, uint8_t *v) {
+  s[SXO_P] &= (uint8_t)~FC;
+  if (*v & 0x01) s[SXO_P] |= FC;
+  *v = (uint8_t)(*v >> 1);
+  set_nz(s, *v);
+}
+static inline void do_rol(uint8_t *s, uint8_t *v) {
+  uint8_t oc = s[SXO_P] & FC;
+  s[SXO_P] &= (uint8_t)~FC;
+  if (*v & 0x80) s[SXO_P] |= FC;
+  *v = (uint8_t)((*v << 1) | oc);
+  set_nz(s, *v);
+}
+static inline void do_ror(uint8_t *s, uint8_t *v) {
+  uint8_t oc = s[SXO_P] & FC;
+  s[SXO_P] &= (uint8_t)~FC;
+  if (*v & 0x01) s[SXO_P] |= FC;
+  *v = (uint8_t)((*v >> 1) | (oc << 7));
+  set_nz(s, *v);
+}
+
+/* exec.go:298-310 */
+static inline void do_bit(uint8_t *s, uint8_t m) {
+  s[SXO_P] &= (uint8_t)~(FN | FV | FZ);
+  if (m & 0x80) s[SXO_P] |= FN;
+  if (m & 0x40) s[SXO_P] |= FV;
+  if ((s[SXO_A] & m) == 0) s[SXO_P] |= FZ;
+}
+
+/* exec.go:314-327: two-deep virtual stack. */
+static inline void do_push(uint8_t *s, uint8_t v) {
+  s[SXO_S1] = s[SXO_S0];
+  s[SXO_S0] = v;
+  s[SXO_S]--;
+}
+static inline uint8_t do_pull(uint8_t *s) {
+  uint8_t v = s[SXO_S0];
+  s[SXO_S0] = s[SXO_S1];
+  s[SXO_S1] = 0;
+  s[SXO_S]++;
+  return v;
+}
+
+/* pkg/cpu/exec.go:8-189 */
+static inline __attribute__((always_inline)) int exec_one(uint8_t *s, int op, uint16_t imm16) {
+  uint8_t imm = (uint8_t)imm16;
+  switch (op) {
+  case SXO_TAX: s[SXO_X] = s[SXO_A]; set_nz(s, s[SXO_X]); break;
+  case SXO_TAY: s[SXO_Y] = s[SXO_A]; set_nz(s, s[SXO_Y]); break;
+  case SXO_TXA: s[SXO_A] = s[SXO_X]; set_nz(s, s[SXO_A]); break;
+  case SXO_TYA: s[SXO_A] = s[SXO_Y]; set_nz(s, s[SXO_A]); break;
+  case SXO_TSX: s[SXO_X] = s[SXO_S]; set_nz(s, s[SXO_X]); break;
+  case SXO_TXS: s[SXO_S] = s[SXO_X]; break;
+  case SXO_INX: s[SXO_X]++; set_nz(s, s[SXO_X]); break;
+  case SXO_INY: s[SXO_Y]++; set_nz(s, s[SXO_Y]); break;
+  case SXO_DEX: s[SXO_X]--; set_nz(s, s[SXO_X]); break;
+  case SXO_DEY: s[SXO_Y]--; set_nz(s, s[SXO_Y]); break;
+  case SXO_ASL_A: do_asl(s, &s[SXO_A]); break;
+  case SXO_LSR_A: do_lsr(s, &s[SXO_A]); break;
+  case SXO_ROL_A: do_rol(s, &s[SXO_A]); break;
+  case SXO_ROR_A: do_ror(s, &s[SXO_A]); break;
+  case SXO_CLC: s[SXO_P] &= (uint8_t)~FC; break;
+  case SXO_SEC: s[SXO_P] |= FC; break;
+  case SXO_CLD: s[SXO_P] &= (uint8_t)~FD; break;
+  case SXO_SED: s[SXO_P] |= FD; break;
+  case SXO_CLI: s[SXO_P] &= (uint8_t)~FI; break;
+  case SXO_SEI: s[SXO_P] |= FI; break;
+  case SXO_CLV: s[SXO_P] &= (uint8_t)~FV; break;
+  case SXO_PHA: do_push(s, s[SXO_A]); break;
+  case SXO_PLA: s[SXO_A] = do_pull(s); set_nz(s, s[SXO_A]); break;
+  case SXO_PHP: do_push(s, (uint8_t)(s[SXO_P] | FB | FU)); break;
+  case SXO_PLP: s[SXO_P] = do_pull(s); break;
+  case SXO_NOP: break;
+  case SXO_LDA_IMM: s[SXO_A] = imm; set_nz(s, s[SXO_A]); break;
+  case SXO_LDX_IMM: s[SXO_X] = imm; set_nz(s, s[SXO_X]); break;
+  case SXO_LDY_IMM: s[SXO_Y] = imm; set_nz(s, s[SXO_Y]); break;
+  case SXO_ADC_IMM: do_adc(s, imm); break;
+  case SXO_SBC_IMM: do_sbc(s, imm); break;
+  case SXO_AND_IMM: s[SXO_A] &= imm; set_nz(s, s[SXO_A]); break;
+  case SXO_ORA_IMM: s[SXO_A] |= imm; set_nz(s, s[SXO_A]); break;
+  case SXO_EOR_IMM: s[SXO_A] ^= imm; set_nz(s, s[SXO_A]); break;
+  case SXO_CMP_IMM: do_cmp(s, s[SXO_A], imm); break;
+  case SXO_CPX_IMM: do_cmp(s, s[SXO_X], imm); break;
+  case SXO_CPY_IMM: do_cmp(s, s[SXO_Y], imm); break;
+  case SXO_LDA_M: s[SXO_A] = s[SXO_M]; set_nz(s, s[SXO_A]); break;
+  case SXO_LDX_M: s[SXO_X] = s[SXO_M]; set_nz(s, s[SXO_X]); break;
+  case SXO_LDY_M: s[SXO_Y] = s[SXO_M]; set_nz(s, s[SXO_Y]); break;
+  case SXO_STA_M: s[SXO_M] = s[SXO_A]; break;
+  case SXO_STX_M: s[SXO_M] = s[SXO_X]; break;
+  case SXO_STY_M: s[SXO_M] = s[SXO_Y]; break;
+  case SXO_ADC_M: do_adc(s, s[SXO_M]); break;
+  case SXO_SBC_M: do_sbc(s, s[SXO_M]); break;
+  case SXO_AND_M: s[SXO_A] &= s[SXO_M]; set_nz(s, s[SXO_A]); break;
+  case SXO_ORA_M: s[SXO_A] |= s[SXO_M]; set_nz(s, s[SXO_A]); break;
+  case SXO_EOR_M: s[SXO_A] ^= s[SXO_M]; set_nz(s, s[SXO_A]); break;
+  case SXO_CMP_M: do_cmp(s, s[SXO_A], s[SXO_M]); break;
+  case SXO_CPX_M: do_cmp(s, s[SXO_X], s[SXO_M]); break;
+  case SXO_CPY_M: do_cmp(s, s[SXO_Y], s[SXO_M]); break;
+  case SXO_INC_M: s[SXO_M]++; set_nz(s, s[SXO_M]); break;
+  case SXO_DEC_M: s[SXO_M]--; set_nz(s, s[SXO_M]); break;
+  case SXO_ASL_M: do_asl(s, &s[SXO_M]); break;
+  case SXO_LSR_M: do_lsr(s, &s[SXO_M]); break;
+  case SXO_ROL_M: do_rol(s, &s[SXO_M]); break;
+  case SXO_ROR_M: do_ror(s, &s[SXO_M]); break;
+  case SXO_BIT_M: do_bit(s, s[SXO_M]); break;
+  default: return -1; /* Go panics here (exec.go:185-186) */
+  }
+  return sxo_cycles(op);
+}
+
+int sxo_exec(uint8_t *s, int op, uint16_t imm16) { return exec_one(s, op, imm16); }
+
+void sxo_exec_many(uint8_t *states, const uint32_t *words, int64_t n) {
+  for (int64_t i = 0; i < n; i++) exec_one(states + 8 * i, SXO_OP(words[i]), SXO_IMM(words[i]));
+}
+
+/* verifier.go:70-76 */
+void sxo_exec_seq(uint8_t *s, const uint32_t *seq, int n) {
+  for (int i = 0; i < n; i++) exec_one(s, SXO_OP(seq[i]), SXO_IMM(seq[i]));
+}
+
+/* --------------------------------------------------------------- Vectors -- */
+
+/* verifier.go:18-27, A X Y P S M S0 S1. */
+static const uint8_t k_qc[8][8] = {
+  {0x00, 0x00, 0x00, 0x00, 0xFD, 0x42, 0x00, 0x00},
+  {0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xBD, 0xFF, 0xFF},
+  {0x01, 0x02, 0x03, 0x00, 0xFD, 0x13, 0x10, 0x20},
+  {0x80, 0x40, 0x20, 0x01, 0xFC, 0x7E, 0x30, 0x40},
+  {0x55, 0xAA, 0x55, 0x00, 0xFE, 0xA5, 0x55, 0xAA},
+  {0xAA, 0x55, 0xAA, 0x01, 0xFD, 0x5A, 0xAA, 0x55},
+  {0x0F, 0xF0, 0x0F, 0x00, 0xFB, 0xE1, 0x0F, 0xF0},
+  {0x7F, 0x80, 0x7F, 0x01, 0xFD, 0x33, 0x80, 0x7F},
+};
+
+/* verifier.go:30-67 (the first 8 equal the QuickCheck vectors). */
+static const uint8_t k_mid[32][8] = {
+  {0x00, 0x00, 0x00, 0x00, 0xFD, 0x42, 0x00, 0x00},
+  {0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xBD, 0xFF, 0xFF},
+  {0x01, 0x02, 0x03, 0x00, 0xFD, 0x13, 0x10, 0x20},
+  {0x80, 0x40, 0x20, 0x01, 0xFC, 0x7E, 0x30, 0x40},
+  {0x55, 0xAA, 0x55, 0x00, 0xFE, 0xA5, 0x55, 0xAA},
+  {0xAA, 0x55, 0xAA, 0x01, 0xFD, 0x5A, 0xAA, 0x55},
+  {0x0F, 0xF0, 0x0F, 0x00, 0xFB, 0xE1, 0x0F, 0xF0},
+  {0x7F, 0x80, 0x7F, 0x01, 0xFD, 0x33, 0x80, 0x7F},
+  {0x02, 0x01, 0x04, 0x00, 0xFD, 0x91, 0x11, 0x22},
+  {0x04, 0x08, 0x10, 0x01, 0xFD, 0x48, 0x33, 0x44},
+  {0x08, 0x10, 0x20, 0x00, 0xFD, 0x24, 0x55, 0x66},
+  {0x10, 0x20, 0x40, 0x01, 0xFD, 0xD7, 0x77, 0x88},
+  {0x20, 0x40, 0x80, 0x00, 0xFD, 0x6B, 0x99, 0xBB},
+  {0x40, 0x80, 0x01, 0x01, 0xFD, 0xB2, 0xCC, 0xDD},
+  {0x03, 0x05, 0x09, 0x00, 0xFD, 0x0F, 0xEE, 0x11},
+  {0x05, 0x0A, 0x14, 0x01, 0xFD, 0xF0, 0x22, 0x33},
+  {0xFE, 0x7F, 0x80, 0x00, 0xFD, 0x01, 0x44, 0x55},
+  {0x81, 0xFE, 0x01, 0x01, 0xFD, 0xFE, 0x66, 0x77},
+  {0x7E, 0x81, 0x42, 0x00, 0xFD, 0x80, 0x88, 0x99},
+  {0xBF, 0x40, 0xBF, 0x01, 0xFD, 0x40, 0xAA, 0xBB},
+  {0xC0, 0x3F, 0xC0, 0x00, 0xFD, 0x1F, 0xCC, 0xDD},
+  {0xE0, 0x1F, 0xE0, 0x01, 0xFD, 0xEF, 0xEE, 0x00},
+  {0xF0, 0x0F, 0xF0, 0x00, 0xFD, 0x55, 0x11, 0x22},
+  {0x0E, 0xE0, 0x0E, 0x01, 0xFD, 0xAA, 0x33, 0x44},
+  {0x11, 0x22, 0x33, 0x00, 0xFE, 0xC3, 0x00, 0x00},
+  {0x22, 0x33, 0x44, 0x01, 0xFF, 0x3C, 0x55, 0x66},
+  {0x44, 0x55, 0x66, 0x00, 0xFC, 0x87, 0x77, 0x88},
+  {0x88, 0x99, 0xAA, 0x01, 0xFB, 0x69, 0x99, 0xAA},
+  {0x33, 0x44, 0x55, 0x00, 0xFD, 0xAE, 0xBB, 0xCC},
+  {0xCC, 0xDD, 0xEE, 0x01, 0xFD, 0x51, 0xDD, 0xEE},
+  {0x66, 0x77, 0x88, 0x00, 0xFD, 0xDA, 0x12, 0x34},
+  {0x99, 0xAA, 0xBB, 0x01, 0xFD, 0x25, 0x56, 0x78},
+};
+
+const uint8_t *sxo_qc_vectors(void) { return &k_qc[0][0]; }
+const uint8_t *sxo_mid_vectors(void) { return &k_mid[0][0]; }
+
+/* -------------------------------------------------------------- Verifier -- */
+
+/* verifier.go:418-424 (dead = 0 reduces to State.Equal, state.go:23-28). */
+int sxo_states_equal_masked(const uint8_t *a, const uint8_t *b, uint8_t dead) {
+  uint8_t pm = (uint8_t)(PMASK & ~dead);
+  return a[SXO_A] == b[SXO_A] && a[SXO_X] == b[SXO_X] && a[SXO_Y] == b[SXO_Y] &&
+         (a[SXO_P] & pm) == (b[SXO_P] & pm) && a[SXO_S] == b[SXO_S] &&
+         a[SXO_M] == b[SXO_M] && a[SXO_S0] == b[SXO_S0] && a[SXO_S1] == b[SXO_S1];
+}
+
+static int check_vectors(const uint8_t (*vec)[8], int nv, const uint32_t *t, int tn,
+                         const uint32_t *c, int cn, uint8_t dead) {
+  for (int i = 0; i < nv; i++) {
+    uint8_t ts[8], cs[8];
+    memcpy(ts, vec[i], 8);
+    memcpy(cs, vec[i], 8);
+    sxo_exec_seq(ts, t, tn);
+    sxo_exec_seq(cs, c, cn);
+    if (!sxo_states_equal_masked(ts, cs, dead)) return 0;
+  }
+  return 1;
+}
+
+/* verifier.go:84-93 and :427-439 */
+int sxo_quick_check(const uint32_t *t, int tn, const uint32_t *c, int cn, uint8_t dead) {
+  return check_vectors(k_qc, 8, t, tn, c, cn, dead);
+}
+
+/* verifier.go:96-105 and :442-454 */
+int sxo_mid_check(const uint32_t *t, int tn, const uint32_t *c, int cn, uint8_t dead) {
+  return check_vectors(k_mid, 32, t, tn, c, cn, dead);
+}
+
+/* verifier.go:114-129 */
+void sxo_fingerprint(const uint32_t *seq, int n, uint8_t fp[64]) {
+  for (int i = 0; i < 8; i++) {
+    uint8_t s[8];
+    memcpy(s, k_qc[i], 8);
+    sxo_exec_seq(s, seq, n);
+    memcpy(fp + 8 * i, s, 8);
+    fp[8 * i + SXO_P] &= PMASK;
+  }
+}
+
+/* verifier.go:591-604 */
+uint8_t sxo_flag_diff(const uint32_t *t, int tn, const uint32_t *c, int cn) {
+  uint8_t diff = 0;
+  for (int i = 0; i < 8; i++) {
+    uint8_t ts[8], cs[8];
+    memcpy(ts, k_qc[i], 8);
+    memcpy(cs, k_qc[i], 8);
+    sxo_exec_seq(ts, t, tn);
+    sxo_exec_seq(cs, c, cn);
+    for (int r = 0; r < 8; r++)
+      if (r != SXO_P && ts[r] != cs[r]) return 0;
+    diff |= (uint8_t)((ts[SXO_P] ^ cs[SXO_P]) & PMASK);
+  }
+  return diff;
+}
+
+/* verifier.go:154-259 */
+int sxo_op_reads(int op) {
+  switch (op) {
+  case SXO_TAX: case SXO_TAY: return SXO_RA;
+  case SXO_TXA: return SXO_RX;
+  case SXO_TYA: return SXO_RY;
+  case SXO_TSX: return SXO_RS;
+  case SXO_TXS: return SXO_RX;
+  case SXO_INX: case SXO_DEX: return SXO_RX;
+  case SXO_INY: case SXO_DEY: return SXO_RY;
+  case SXO_ASL_A: case SXO_LSR_A: return SXO_RA;
+  case SXO_ROL_A: case SXO_ROR_A: return SXO_RA | SXO_RP;
+  case SXO_CLC: case SXO_SEC: case SXO_CLD: case SXO_SED:
+  case SXO_CLI: case SXO_SEI: case SXO_CLV: return 0;
+  case SXO_PHA: return SXO_RA | SXO_RS0 | SXO_RS1;
+  case SXO_PLA: return SXO_RS0;
+  case SXO_PHP: return SXO_RP | SXO_RS0 | SXO_RS1;
+  case SXO_PLP: return SXO_RS0;
+  case SXO_NOP: return 0;
+  case SXO_LDA_IMM: case SXO_LDX_IMM: case SXO_LDY_IMM: return 0;
+  case SXO_ADC_IMM: case SXO_SBC_IMM: return SXO_RA | SXO_RP;
+  case SXO_AND_IMM: case SXO_ORA_IMM: case SXO_EOR_IMM: return SXO_RA;
+  case SXO_CMP_IMM: return SXO_RA;
+  case SXO_CPX_IMM: return SXO_RX;
+  case SXO_CPY_IMM: return SXO_RY;
+  case SXO_LDA_M: case SXO_LDX_M: case SXO_LDY_M: return SXO_RM;
+  case SXO_STA_M: return SXO_RA;
+  case SXO_STX_M: return SXO_RX;
+  case SXO_STY_M: return SXO_RY;
+  case SXO_ADC_M: case SXO_SBC_M: return SXO_RA | SXO_RM | SXO_RP;
+  case SXO_AND_M: case SXO_ORA_M: case SXO_EOR_M: return SXO_RA | SXO_RM;
+  case SXO_CMP_M: return SXO_RA | SXO_RM;
+  case SXO_CPX_M: return SXO_RX | SXO_RM;
+  case SXO_CPY_M: return SXO_RY | SXO_RM;
+  case SXO_INC_M: case SXO_DEC_M: return SXO_RM;
+  case SXO_ASL_M: case SXO_LSR_M: return SXO_RM;
+  case SXO_ROL_M: case SXO_ROR_M: return SXO_RM | SXO_RP;
+  case SXO_BIT_M: return SXO_RA | SXO_RM;
+  }
+  return 0;
+}
+
+/* pruner.go:76-163 */
+int sxo_op_writes(int op) {
+  switch (op) {
+  case SXO_TAX: return SXO_RX | SXO_RP;
+  case SXO_TAY: return SXO_RY | SXO_RP;
+  case SXO_TXA: case SXO_TYA: return SXO_RA | SXO_RP;
+  case SXO_TSX: return SXO_RX | SXO_RP;
+  case SXO_TXS: return SXO_RS;
+  case SXO_INX: case SXO_DEX: return SXO_RX | SXO_RP;
+  case SXO_INY: case SXO_DEY: return SXO_RY | SXO_RP;
+  case SXO_ASL_A: case SXO_LSR_A: case SXO_ROL_A: case SXO_ROR_A: return SXO_RA | SXO_RP;
+  case SXO_CLC: case SXO_SEC: case SXO_CLD: case SXO_SED:
+  case SXO_CLI: case SXO_SEI: case SXO_CLV: return SXO_RP;
+  case SXO_PHA: case SXO_PHP: return SXO_RS | SXO_RS0 | SXO_RS1;
+  case SXO_PLA: return SXO_RA | SXO_RP | SXO_RS | SXO_RS0;
+  case SXO_PLP: return SXO_RP | SXO_RS | SXO_RS0;
+  case SXO_NOP: return 0;
+  case SXO_LDA_IMM: return SXO_RA | SXO_RP;
+  case SXO_LDX_IMM: return SXO_RX | SXO_RP;
+  case SXO_LDY_IMM: return SXO_RY | SXO_RP;
+  case SXO_ADC_IMM: case SXO_SBC_IMM: case SXO_AND_IMM: case SXO_ORA_IMM:
+  case SXO_EOR_IMM: return SXO_RA | SXO_RP;
+  case SXO_CMP_IMM: case SXO_CPX_IMM: case SXO_CPY_IMM: return SXO_RP;
+  case SXO_LDA_M: return SXO_RA | SXO_RP;
+  case SXO_LDX_M: return SXO_RX | SXO_RP;
+  case SXO_LDY_M: return SXO_RY | SXO_RP;
+  case SXO_STA_M: case SXO_STX_M: case SXO_STY_M: return SXO_RM;
+  case SXO_ADC_M: case SXO_SBC_M: case SXO_AND_M: case SXO_ORA_M:
+  case SXO_EOR_M: return SXO_RA | SXO_RP;
+  case SXO_CMP_M: case SXO_CPX_M: case SXO_CPY_M: return SXO_RP;
+  case SXO_INC_M: case SXO_DEC_M: return SXO_RM | SXO_RP;
+  case SXO_ASL_M: case SXO_LSR_M: case SXO_ROL_M: case SXO_ROR_M: return SXO_RM | SXO_RP;
+  case SXO_BIT_M: return SXO_RP;
+  }
+  return 0;
+}
+
+/* verifier.go:145-151 */
+int sxo_regs_read(const uint32_t *seq, int n) {
+  int m = 0;
+  for (int i = 0; i < n; i++) m |= sxo_op_reads(SXO_OP(seq[i]));
+  return m;
+}
+
+/* verifier.go:292-310: extra registers in the order X, Y, M, S, S0, S1,
+ * as setReg offsets 1..6 (verifier.go:322-337). */
+int sxo_extra_regs(int reads, int out[6]) {
+  int k = 0;
+  if (reads & SXO_RX) out[k++] = 1;
+  if (reads & SXO_RY) out[k++] = 2;
+  if (reads & SXO_RM) out[k++] = 3;
+  if (reads & SXO_RS) out[k++] = 4;
+  if (reads & SXO_RS0) out[k++] = 5;
+  if (reads & SXO_RS1) out[k++] = 6;
+  return k;
+}
+
+/* setReg offset -> state byte index */
+static const int k_off_to_byte[7] = {SXO_A, SXO_X, SXO_Y, SXO_M, SXO_S, SXO_S0, SXO_S1};
+
+/* verifier.go:378-383 */
+static const uint8_t k_rep[32] = {
+  0x00, 0x01, 0x02, 0x0F, 0x10, 0x1F, 0x20, 0x3F,
+  0x40, 0x55, 0x7E, 0x7F, 0x80, 0x81, 0xAA, 0xBF,
+  0xC0, 0xD5, 0xE0, 0xEF, 0xF0, 0xF7, 0xFE, 0xFF,
+  0x03, 0x07, 0x11, 0x33, 0x77, 0xBB, 0xDD, 0xEE,
+};
+
+typedef union st8 {
+  uint8_t b[8];
+  uint64_t q;
+} st8;
+
+/* verifier.go:262-415 (dead == 0) and :457-588 (dead != 0).  The domain is
+ * A = 0..255, P = carry 0/1, S = 0xFD, the other bytes 0, with the extra
+ * registers read by either side swept on top (setReg, verifier.go:322-337).
+ * mode 0 follows Go: every value of up to two extra registers
+ * (exhaustiveFullSweep), 32 representative values each for three or more
+ * (exhaustiveReducedSweep).  mode 1 sweeps all 256 values of every extra
+ * register (complete).  The visiting order differs from Go's nested loops,
+ * which does not change the boolean result. */
+int sxo_exhaustive_check(const uint32_t *t, int tn, const uint32_t *c, int cn,
+                         uint8_t dead, int mode) {
+  int reads = sxo_regs_read(t, tn) | sxo_regs_read(c, cn);
+  int ex[6];
+  int k = (reads & ~(SXO_RA | SXO_RP)) ? sxo_extra_regs(reads, ex) : 0;
+  const uint8_t *vals = NULL;
+  int nvals = 256;
+  if (k >= 3 && mode == 0) {
+    vals = k_rep;
+    nvals = 32;
+  }
+  int eb[6];
+  for (int i = 0; i < k; i++) eb[i] = k_off_to_byte[ex[i]];
+  /* compare mask: every byte, P under PMask &^ dead */
+  st8 msk;
+  msk.q = ~(uint64_t)0;
+  msk.b[SXO_P] = (uint8_t)(PMASK & ~dead);
+  int odo[6] = {0, 0, 0, 0, 0, 0};
+  st8 base;
+  base.q = 0;
+  base.b[SXO_S] = 0xFD;
+  for (;;) {
+    for (int i = 0; i < k; i++) base.b[eb[i]] = vals ? vals[odo[i]] : (uint8_t)odo[i];
+    for (int carry = 0; carry <= 1; carry++) {
+      base.b[SXO_P] = (uint8_t)carry;
+      for (int a = 0; a < 256; a++) {
+        st8 ts, cs;
+        base.b[SXO_A] = (uint8_t)a;
+        ts.q = base.q;
+        cs.q = base.q;
+        for (int i = 0; i < tn; i++) exec_one(ts.b, SXO_OP(t[i]), SXO_IMM(t[i]));
+        for (int i = 0; i < cn; i++) exec_one(cs.b, SXO_OP(c[i]), SXO_IMM(c[i]));
+        if ((ts.q ^ cs.q) & msk.q) return 0;
+      }
+    }
+    int i = 0;
+    while (i < k && ++odo[i] == nvals) odo[i++] = 0;
+    if (i == k) break;
+  }
+  return 1;
+}
+
+/* ------------------------------------------------------ Pruner / enum -- */
+
+/* pruner.go:38-53 */
+static int is_dead_write(uint32_t first, uint32_t second) {
+  int w = sxo_op_writes(SXO_OP(first));
+  if (!w) return 0;
+  int r = sxo_op_reads(SXO_OP(second));
+  int w2 = sxo_op_writes(SXO_OP(second));
+  int dead = w & w2 & ~SXO_RP & ~SXO_RS & ~SXO_RS0 & ~SXO_RS1 & ~r;
+  return dead != 0;
+}
+
+/* pruner.go:57-75 */
+static int has_stack_violation(const uint32_t *seq, int n) {
+  int depth = 0;
+  for (int i = 0; i < n; i++) {
+    int op = SXO_OP(seq[i]);
+    if (op == SXO_PHA || op == SXO_PHP) {
+      if (++depth > 2) return 1;
+    } else if (op == SXO_PLA || op == SXO_PLP) {
+      if (depth <= 0) return 1;
+      depth--;
+    }
+  }
+  return 0;
+}
+
+/* pruner.go:166-181 */
+static int are_independent(uint32_t a, uint32_t b) {
+  int aR = sxo_op_reads(SXO_OP(a)), aW = sxo_op_writes(SXO_OP(a));
+  int bR = sxo_op_reads(SXO_OP(b)), bW = sxo_op_writes(SXO_OP(b));
+  return !((aW & bR) || (aR & bW) || (aW & bW));
+}
+
+/* pruner.go:6-34; instKey = op<<16 | imm is the instruction word itself. */
+int sxo_should_prune(const uint32_t *seq, int n) {
+  for (int i = 0; i < n; i++) {
+    if (SXO_OP(seq[i]) == SXO_NOP) return 1;
+    if (i + 1 < n && is_dead_write(seq[i], seq[i + 1])) return 1;
+  }
+  if (has_stack_violation(seq, n)) return 1;
+  for (int i = 0; i + 1 < n; i++) {
+    uint32_t ka = ((uint32_t)SXO_OP(seq[i]) << 16) | SXO_IMM(seq[i]);
+    uint32_t kb = ((uint32_t)SXO_OP(seq[i + 1]) << 16) | SXO_IMM(seq[i + 1]);
+    if (are_independent(seq[i], seq[i + 1]) && ka > kb) return 1;
+  }
+  return 0;
+}
+
+/* search.go:56-75 with enumerator.go:7-38 (lexicographic over Go order). */
+int64_t sxo_enumerate_pruned(int len, uint32_t *out, int64_t cap) {
+  uint32_t tab[2863];
+  int n = sxo_instruction_table(tab, 0);
+  int64_t cnt = 0;
+  uint32_t seq[3];
+  if (len == 1) {
+    for (int i = 0; i < n; i++) {
+      seq[0] = tab[i];
+      if (sxo_should_prune(seq, 1)) continue;
+      if (out && cnt < cap) out[cnt] = seq[0];
+      cnt++;
+    }
+  } else if (len == 2) {
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        seq[0] = tab[i];
+        seq[1] = tab[j];
+        if (sxo_should_prune(seq, 2)) continue;
+        if (out && cnt < cap) memcpy(out + 2 * cnt, seq, 8);
+        cnt++;
+      }
+  } else {
+    return -1;
+  }
+  return cnt;
+}
+
+/* -------------------------------------------------------- Search driver -- */
+
+typedef struct cand_info {
+  uint32_t w[2];
+  int len;
+  int bytes;
+  int cycles;
+  int prune;
+} cand_info;
+
+typedef struct worker_ctx {
+  const sxo_search_cfg *cfg;
+  const uint32_t *targets;
+  int64_t ntargets;
+  const cand_info *cands[3]; /* by length */
+  int64_t ncands[3];
+  int64_t next;              /* shared cursor, atomic */
+  pthread_mutex_t mu;
+  sxo_rule *rules;
+  int64_t nrules, caprules;
+  sxo_stats stats;
+} worker_ctx;
+
+static void push_rule(worker_ctx *w, sxo_rule *r) {
+  pthread_mutex_lock(&w->mu);
+  if (w->nrules == w->caprules) {
+    w->caprules = w->caprules ? 2 * w->caprules : 4096;
+    w->rules = (sxo_rule *)realloc(w->rules, (size_t)w->caprules * sizeof(sxo_rule));
+  }
+  w->rules[w->nrules++] = *r;
+  pthread_mutex_unlock(&w->mu);
+}
+
+/* QuickCheck against precomputed target outputs on the 8 vectors: the same
+ * predicate as verifier.go:84-93, evaluated without re-running the target
+ * for every candidate. */
+static int qc_pre(const uint8_t tout[8][8], const uint32_t *c, int cn, uint8_t dead) {
+  for (int i = 0; i < 8; i++) {
+    uint8_t cs[8];
+    memcpy(cs, k_qc[i], 8);
+    sxo_exec_seq(cs, c, cn);
+    if (!sxo_states_equal_masked(tout[i], cs, dead)) return 0;
+  }
+  return 1;
+}
+
+static void make_rule(sxo_rule *r, const uint32_t *t, int tn, const cand_info *ci,
+                      int64_t ti, int tbytes, int tcycles, uint8_t dead, int pass) {
+  memset(r, 0, sizeof(*r));
+  for (int i = 0; i < tn; i++) r->src[i] = t[i];
+  for (int i = 0; i < ci->len; i++) r->rep[i] = ci->w[i];
+  r->src_len = (uint8_t)tn;
+  r->rep_len = (uint8_t)ci->len;
+  r->dead_flags = dead;
+  r->pass = (uint8_t)pass;
+  r->bytes_saved = (int16_t)(tbytes - ci->bytes);
+  r->cycles_saved = (int16_t)(tcycles - ci->cycles);
+  r->target_index = (uint64_t)ti;
+}
+
+static void process_target(worker_ctx *w, int64_t ti, sxo_stats *st) {
+  const sxo_search_cfg *cfg = w->cfg;
+  int tn = cfg->target_len;
+  const uint32_t *t = w->targets + ti * tn;
+  int tbytes = 0, tcycles = 0;
+  for (int i = 0; i < tn; i++) {
+    tbytes += sxo_byte_size(SXO_OP(t[i]), cfg->bytes_model);
+    tcycles += sxo_cycles(SXO_OP(t[i]));
+  }
+  uint8_t tout[8][8];
+  for (int i = 0; i < 8; i++) {
+    memcpy(tout[i], k_qc[i], 8);
+    sxo_exec_seq(tout[i], t, tn);
+  }
+  sxo_rule r;
+  if (cfg->hits == 1) {
+    /* CUDA tool: one masked pass, every hit (cuda/6502_search_v2.cu:871-974) */
+    for (int len = 1; len <= cfg->max_cand_len; len++) {
+      for (int64_t j = 0; j < w->ncands[len]; j++) {
+        const cand_info *ci = &w->cands[len][j];
+        st->checked++;
+        if (ci->bytes >= tbytes || ci->prune) continue;
+        if (!qc_pre(tout, ci->w, ci->len, cfg->dead)) continue;
+        st->qc_hits++;
+        if (!sxo_mid_check(t, tn, ci->w, ci->len, cfg->dead)) continue;
+        st->mid_pass++;
+        st->exh_calls++;
+        if (!sxo_exhaustive_check(t, tn, ci->w, ci->len, cfg->dead, cfg->exh_mode)) continue;
+        st->found++;
+        make_rule(&r, t, tn, ci, ti, tbytes, tcycles, cfg->dead, 1);
+        push_rule(w, &r);
+      }
+    }
+    return;
+  }
+  /* worker.go:119-178: first hit per target, candidate lengths ascending. */
+  for (int len = 1; len <= cfg->max_cand_len; len++) {
+    int found = 0;
+    for (int64_t j = 0; j < w->ncands[len]; j++) {
+      const cand_info *ci = &w->cands[len][j];
+      st->checked++;
+      if (ci->bytes >= tbytes) continue;
+      if (ci->prune) continue;
+      if (!qc_pre(tout, ci->w, ci->len, 0)) continue;
+      st->qc_hits++;
+      if (cfg->use_mid && !sxo_mid_check(t, tn, ci->w, ci->len, 0)) continue;
+      st->exh_calls++;
+      if (!sxo_exhaustive_check(t, tn, ci->w, ci->len, 0, cfg->exh_mode)) continue;
+      st->found++;
+      make_rule(&r, t, tn, ci, ti, tbytes, tcycles, 0, 1);
+      push_rule(w, &r);
+      found = 1;
+      break;
+    }
+    if (found) break;
+  }
+  if (cfg->dead == 0) return;
+  /* worker.go:180-245: flag-relaxed pass over candidates that fail the
+   * exact QuickCheck. */
+  for (int len = 1; len <= cfg->max_cand_len; len++) {
+    int found = 0;
+    for (int64_t j = 0; j < w->ncands[len]; j++) {
+      const cand_info *ci = &w->cands[len][j];
+      st->checked++;
+      if (ci->bytes >= tbytes) continue;
+      if (ci->prune) continue;
+      if (qc_pre(tout, ci->w, ci->len, 0)) continue;
+      if (!qc_pre(tout, ci->w, ci->len, cfg->dead)) continue;
+      st->qc_hits++;
+      if (cfg->use_mid && !sxo_mid_check(t, tn, ci->w, ci->len, cfg->dead)) continue;
+      st->exh_calls++;
+      if (!sxo_exhaustive_check(t, tn, ci->w, ci->len, cfg->dead, cfg->exh_mode)) continue;
+      uint8_t fd = sxo_flag_diff(t, tn, ci->w, ci->len);
+      if (fd == 0) continue;
+      st->found++;
+      make_rule(&r, t, tn, ci, ti, tbytes, tcycles, fd, 2);
+      push_rule(w, &r);
+      found = 1;
+      break;
+    }
+    if (found) break;
+  }
+}
+
+static void *worker_main(void *arg) {
+  worker_ctx *w = (worker_ctx *)arg;
+  sxo_stats st;
+  memset(&st, 0, sizeof(st));
+  const int64_t chunk = 64;
+  for (;;) {
+    int64_t b = __atomic_fetch_add(&w->next, chunk, __ATOMIC_RELAXED);
+    if (b >= w->ntargets) break;
+    int64_t e = b + chunk < w->ntargets ? b + chunk : w->ntargets;
+    for (int64_t ti = b; ti < e; ti++) process_target(w, ti, &st);
+  }
+  pthread_mutex_lock(&w->mu);
+  w->stats.checked += st.checked;
+  w->stats.qc_hits += st.qc_hits;
+  w->stats.mid_pass += st.mid_pass;
+  w->stats.exh_calls += st.exh_calls;
+  w->stats.found += st.found;
+  pthread_mutex_unlock(&w->mu);
+  return NULL;
+}
+
+static int rule_cmp(const void *a, const void *b) {
+  const sxo_rule *x = (const sxo_rule *)a, *y = (const sxo_rule *)b;
+  if (x->target_index != y->target_index) return x->target_index < y->target_index ? -1 : 1;
+  if (x->pass != y->pass) return x->pass < y->pass ? -1 : 1;
+  if (x->rep_len != y->rep_len) return x->rep_len < y->rep_len ? -1 : 1;
+  for (int i = 0; i < 4; i++)
+    if (x->rep[i] != y->rep[i]) return x->rep[i] < y->rep[i] ? -1 : 1;
+  return 0;
+}
+
+static cand_info *build_cands(int len, int bytes_model, int64_t *count) {
+  uint32_t tab[2863];
+  int n = sxo_instruction_table(tab, 0);
+  int64_t total = len == 1 ? n : (int64_t)n * n;
+  cand_info *c = (cand_info *)calloc((size_t)total, sizeof(cand_info));
+  int64_t k = 0;
+  if (len == 1) {
+    for (int i = 0; i < n; i++, k++) {
+      c[k].w[0] = tab[i];
+      c[k].len = 1;
+    }
+  } else {
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++, k++) {
+        c[k].w[0] = tab[i];
+        c[k].w[1] = tab[j];
+        c[k].len = 2;
+      }
+  }
+  for (int64_t i = 0; i < total; i++) {
+    for (int p = 0; p < c[i].len; p++) {
+      c[i].bytes += sxo_byte_size(SXO_OP(c[i].w[p]), bytes_model);
+      c[i].cycles += sxo_cycles(SXO_OP(c[i].w[p]));
+    }
+    c[i].prune = sxo_should_prune(c[i].w, c[i].len);
+  }
+  *count = total;
+  return c;
+}
+
+int sxo_search(const sxo_search_cfg *cfg, const uint32_t *targets, int64_t ntargets,
+               sxo_rule *out, int64_t cap, int64_t *nout, sxo_stats *stats) {
+  if (!cfg || cfg->target_len < 1 || cfg->target_len > 4 || cfg->max_cand_len < 1 ||
+      cfg->max_cand_len > 2)
+    return -1;
+  worker_ctx w;
+  memset(&w, 0, sizeof(w));
+  w.cfg = cfg;
+  w.targets = targets;
+  w.ntargets = ntargets;
+  pthread_mutex_init(&w.mu, NULL);
+  for (int len = 1; len <= cfg->max_cand_len; len++)
+    w.cands[len] = build_cands(len, cfg->bytes_model, &w.ncands[len]);
+  int nt = cfg->nthreads > 0 ? cfg->nthreads : 1;
+  pthread_t *th = (pthread_t *)calloc((size_t)nt, sizeof(pthread_t));
+  for (int i = 0; i < nt; i++) pthread_create(&th[i], NULL, worker_main, &w);
+  for (int i = 0; i < nt; i++) pthread_join(th[i], NULL);
+  free(th);
+  for (int len = 1; len <= cfg->max_cand_len; len++) free((void *)w.cands[len]);
+  qsort(w.rules, (size_t)w.nrules, sizeof(sxo_rule), rule_cmp);
+  if (nout) *nout = w.nrules;
+  if (out) {
+    int64_t m = w.nrules < cap ? w.nrules : cap;
+    if (m > 0) memcpy(out, w.rules, (size_t)m * sizeof(sxo_rule));
+  }
+  if (stats) *stats = w.stats;
+  free(w.rules);
+  pthread_mutex_destroy(&w.mu);
+  return (out && w.nrules > cap) ? -3 : 0;
+}
