@@ -1,0 +1,1342 @@
+// sx_core.hip -- lib6502core: the MI355X search core of the 6502
+// superoptimizer, behind the C ABI of include/sx6502.h.
+//
+// Pipeline of one enumeration (per segment of targets):
+//   1. fingerprints  one wave per sequence, lane v = Mid vector v (0..31):
+//                    the sequence is wave-uniform, so the interpreter runs
+//                    with scalar dispatch.  Targets and candidates get the
+//                    outputs on all 32 vectors (QuickCheck vectors = 0..7).
+//   2. QuickCheck    a lane per candidate holds its 8 QC outputs in VGPRs and
+//                    walks a tile of wave-uniform targets (scalar loads); the
+//                    first vector rejects almost every pair, survivors are
+//                    collected by ballot + mbcnt into an LDS queue per wave
+//                    and flushed with one atomic per wave.
+//   3. MidCheck      a lane per survivor: byte filter, candidate pruner and a
+//                    compare of the 24 further vector outputs.
+//   4. Exhaustive    jobs = (target, candidate) pairs; a plan kernel sizes
+//                    each job's sweep(s), a scan turns sizes into chunk
+//                    offsets and a persistent kernel pulls 16K-state chunks
+//                    from a device counter (wave-uniform ops, early exit per
+//                    job).  First-hit semantics run this in rounds: each
+//                    round checks, per target and pass, the lowest-ranked
+//                    candidate not yet rejected.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sx6502.h"
+#include "sx_isa.h"
+#include "sx_plan.h"
+
+using namespace sx;
+
+// ---------------------------------------------------------------- tables --
+
+__constant__ uint8_t c_reads[OP_COUNT] = SX_READS_TABLE;
+__constant__ uint8_t c_writes[OP_COUNT] = SX_WRITES_TABLE;
+__constant__ uint8_t c_dep[OP_COUNT][8] = SX_DEP_TABLE;
+__constant__ uint8_t c_vec[32][8] = SX_MID_VECTORS;
+
+static const uint8_t h_reads[OP_COUNT] = SX_READS_TABLE;
+static const uint8_t h_writes[OP_COUNT] = SX_WRITES_TABLE;
+static const uint8_t h_dep[OP_COUNT][8] = SX_DEP_TABLE;
+static const uint8_t h_vec[32][8] = SX_MID_VECTORS;
+
+constexpr int kWave = 64;
+constexpr int kChunkLog = 14;           // states per exhaustive chunk
+constexpr uint32_t kRankNone = 0xFFFFFFFFu;
+constexpr int kQcQueue = 256;           // LDS survivor queue entries per wave
+
+// Go instruction order (pkg/search/enumerator.go:14-38): the 47 non-immediate
+// opcodes in id order, then the 11 immediate opcodes x 256.
+SX_HD uint32_t go_inst(uint32_t i) {
+  if (i < 26) return i;                       // TAX .. NOP
+  if (i < kNumNonImm) return i - 26 + LDA_M;  // LDA M .. BIT M
+  uint32_t k = i - kNumNonImm;
+  return (LDA_IMM + (k >> 8)) | ((k & 0xFFu) << 16);
+}
+
+// ------------------------------------------------------ device helpers --
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+}
+
+// ShouldPrune, pkg/search/pruner.go:6-181.
+SX_HD bool should_prune(const uint32_t *seq, int n, const uint8_t *rd, const uint8_t *wr) {
+  for (int i = 0; i < n; i++) {
+    if ((seq[i] & 0xFFFFu) == NOP) return true;
+    if (i + 1 < n) {
+      uint32_t w = wr[seq[i] & 0xFFFFu];
+      if (w) {
+        uint32_t r2 = rd[seq[i + 1] & 0xFFFFu], w2 = wr[seq[i + 1] & 0xFFFFu];
+        if (w & w2 & ~(uint32_t)(GP | GS | GS0 | GS1) & ~r2) return true;
+      }
+    }
+  }
+  int depth = 0;
+  for (int i = 0; i < n; i++) {
+    uint32_t op = seq[i] & 0xFFFFu;
+    if (op == PHA || op == PHP) {
+      if (++depth > 2) return true;
+    } else if (op == PLA || op == PLP) {
+      if (depth <= 0) return true;
+      depth--;
+    }
+  }
+  for (int i = 0; i + 1 < n; i++) {
+    uint32_t a = seq[i] & 0xFFFFu, b = seq[i + 1] & 0xFFFFu;
+    uint32_t aR = rd[a], aW = wr[a], bR = rd[b], bW = wr[b];
+    bool indep = !((aW & bR) || (aR & bW) || (aW & bW));
+    uint32_t ka = (a << 16) | (seq[i] >> 16), kb = (b << 16) | (seq[i + 1] >> 16);
+    if (indep && ka > kb) return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ void run_seq_uniform(State &st, const uint32_t *w, int n) {
+  for (int i = 0; i < n; i++) {
+    uint32_t word = uni(w[i]);
+    step(st, word & 0xFFFFu, word >> 16);
+  }
+}
+
+__device__ __forceinline__ State load_vec(uint32_t v) {
+  State s;
+  s.a = c_vec[v][0]; s.x = c_vec[v][1]; s.y = c_vec[v][2]; s.p = c_vec[v][3];
+  s.s = c_vec[v][4]; s.m = c_vec[v][5]; s.s0 = c_vec[v][6]; s.s1 = c_vec[v][7];
+  return s;
+}
+
+// ---------------------------------------------------------------- kernels --
+
+// Ordered compaction of the pruned length-2 targets (search.go:56-75):
+// pass 1 counts survivors per 256-pair block, pass 2 writes them in Go order.
+__global__ __launch_bounds__(256) void k_gen_count(uint32_t *block_counts) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  bool keep = false;
+  if (i < (uint32_t)kNumInsts * kNumInsts) {
+    uint32_t seq[2] = {go_inst(i / kNumInsts), go_inst(i % kNumInsts)};
+    keep = !should_prune(seq, 2, c_reads, c_writes);
+  }
+  __shared__ uint32_t cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  uint64_t m = __ballot(keep);
+  if (lane_id() == 0) atomicAdd(&cnt, (uint32_t)__popcll(m));
+  __syncthreads();
+  if (threadIdx.x == 0) block_counts[blockIdx.x] = cnt;
+}
+
+__global__ __launch_bounds__(256) void k_gen_write(const uint64_t *block_offsets, uint32_t *out) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  bool keep = false;
+  uint32_t seq[2] = {0, 0};
+  if (i < (uint32_t)kNumInsts * kNumInsts) {
+    seq[0] = go_inst(i / kNumInsts);
+    seq[1] = go_inst(i % kNumInsts);
+    keep = !should_prune(seq, 2, c_reads, c_writes);
+  }
+  __shared__ uint32_t wave_cnt[4];
+  uint64_t m = __ballot(keep);
+  uint32_t w = threadIdx.x >> 6;
+  if (lane_id() == 0) wave_cnt[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint64_t base = block_offsets[blockIdx.x];
+  for (uint32_t k = 0; k < w; k++) base += wave_cnt[k];
+  if (keep) {
+    uint64_t o = base + mbcnt(m);
+    out[2 * o] = seq[0];
+    out[2 * o + 1] = seq[1];
+  }
+}
+
+// Per-block exclusive scan of uint32 counts into uint64 offsets (3 kernels).
+__global__ __launch_bounds__(1024) void k_scan_block(const uint32_t *in, uint64_t n, uint64_t *out,
+                                                     uint64_t *block_sums) {
+  __shared__ uint64_t s[1024];
+  uint64_t i = (uint64_t)blockIdx.x * 1024 + threadIdx.x;
+  uint64_t v = i < n ? in[i] : 0;
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    uint64_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) out[i] = s[threadIdx.x] - v;
+  if (threadIdx.x == 1023) block_sums[blockIdx.x] = s[1023];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(uint64_t *sums, uint64_t nb, uint64_t *total) {
+  __shared__ uint64_t s[1024];
+  __shared__ uint64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint64_t base = 0; base < nb; base += 1024) {
+    uint64_t i = base + threadIdx.x;
+    uint64_t v = i < nb ? sums[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      uint64_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) sums[i] = s[threadIdx.x] - v + carry;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += s[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_add(uint64_t *out, uint64_t n, const uint64_t *sums) {
+  uint64_t i = (uint64_t)blockIdx.x * 1024 + threadIdx.x;
+  if (i < n) out[i] += sums[blockIdx.x];
+}
+
+// Outputs of n sequences on the 32 vectors: one wave per sequence, lane v
+// (< 32) runs vector v.  fp[(i * 32 + v)] = (lo, hi) with P masked by 0xCF.
+// Also writes per-sequence byte size, cycles and the pruner verdict.
+__global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uint32_t len,
+                                                      uint64_t n, int bytes_model, uint2 *fp,
+                                                      uint8_t *bytes, uint8_t *cyc, uint8_t *prune,
+                                                      uint32_t *bad_op) {
+  uint64_t seq = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (seq >= n) return;
+  uint32_t lane = lane_id();
+  uint32_t w[4];
+  bool bad = false;
+  for (uint32_t k = 0; k < len; k++) {
+    w[k] = uni(words[seq * len + k]);
+    bad |= (w[k] & 0xFFFFu) >= OP_COUNT;
+  }
+  if (bad) {
+    if (lane == 0) atomicOr(bad_op, 1u);
+    return;
+  }
+  if (lane < 32) {
+    State st = load_vec(lane);
+    run_seq_uniform(st, w, (int)len);
+    st.p &= kPMask;
+    fp[seq * 32 + lane] = make_uint2(pack_lo(st), pack_hi(st));
+  }
+  if (lane == 0) {
+    int b = 0, c = 0;
+    for (uint32_t k = 0; k < len; k++) {
+      b += byte_size(w[k] & 0xFFFFu, bytes_model);
+      c += cycles(w[k] & 0xFFFFu);
+    }
+    bytes[seq] = (uint8_t)b;
+    cyc[seq] = (uint8_t)c;
+    prune[seq] = should_prune(w, (int)len, c_reads, c_writes) ? 1 : 0;
+  }
+}
+
+// Gather the first-vector output of each target into a dense array so eight
+// consecutive targets are one 64-byte scalar load.
+__global__ void k_gather_v0(const uint2 *fp, uint64_t n, uint2 *v0) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v0[i] = fp[i * 32];
+}
+
+// QuickCheck.  Grid: x = 256-candidate group, y = target tile.
+// Survivor entry: (target, slot | full << 31), where full = the exact
+// (0xCF) compare holds; every entry satisfies the compare under pm_masked.
+__global__ __launch_bounds__(256) void k_quickcheck(const uint2 *__restrict__ tv0,
+                                                    const uint2 *__restrict__ tfp, uint32_t nt,
+                                                    uint32_t tile, const uint2 *__restrict__ cfp,
+                                                    uint32_t nc, uint32_t pm_masked,
+                                                    uint2 *__restrict__ surv, uint32_t surv_cap,
+                                                    uint32_t *__restrict__ surv_count) {
+  __shared__ uint2 queue[4][kQcQueue];
+  const uint32_t lane = lane_id();
+  const uint32_t wv = threadIdx.x >> 6;
+  const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+  const bool valid = slot < nc;
+  uint32_t clo[8], chi[8];
+#pragma unroll
+  for (int v = 0; v < 8; v++) {
+    uint2 f = valid ? cfp[(uint64_t)slot * 32 + v] : make_uint2(0, 0);
+    clo[v] = f.x;
+    chi[v] = f.y;
+  }
+  const uint32_t mlo = 0x00FFFFFFu | (pm_masked << 24);
+  const uint32_t flo = 0x00FFFFFFu | (kPMask << 24);
+  const uint32_t t0 = blockIdx.y * tile;
+  const uint32_t t1 = min(nt, t0 + tile);
+  uint32_t qn = 0;  // wave-uniform queue fill
+
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(surv_count, qn);
+    base = uni(base);
+    for (uint32_t i = lane; i < qn; i += kWave) {
+      uint32_t o = base + i;
+      if (o < surv_cap) surv[o] = queue[wv][i];
+    }
+    __builtin_amdgcn_wave_barrier();
+    qn = 0;
+  };
+
+  for (uint32_t t = t0; t < t1; t += 8) {
+    uint32_t nb = min(8u, t1 - t);
+    uint2 tv[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) tv[u] = tv0[min(t + u, nt - 1)];
+#pragma unroll
+    for (uint32_t u = 0; u < 8; u++) {
+      if (u >= nb) break;
+      uint32_t d = ((clo[0] ^ tv[u].x) & mlo) | (chi[0] ^ tv[u].y);
+      uint64_t m = __ballot(valid && d == 0);
+      if (m == 0) continue;
+      // slow path: all 8 vectors under both masks
+      const uint2 *tf = tfp + (uint64_t)(t + u) * 32;
+      bool okm = valid, okf = valid;
+#pragma unroll
+      for (int v = 0; v < 8; v++) {
+        uint2 f = tf[v];
+        uint32_t dl = clo[v] ^ f.x, dh = chi[v] ^ f.y;
+        okm &= ((dl & mlo) | dh) == 0;
+        okf &= ((dl & flo) | dh) == 0;
+      }
+      uint64_t hm = __ballot(okm);
+      if (hm == 0) continue;
+      if (okm) queue[wv][qn + mbcnt(hm)] = make_uint2(t + u, slot | (okf ? 0x80000000u : 0u));
+      qn += (uint32_t)__popcll(hm);
+      if (qn > kQcQueue - kWave) flush();
+    }
+  }
+  if (qn) flush();
+}
+
+// MidCheck and stage bookkeeping.  One lane per QuickCheck survivor.
+// hits_all = 0 (Go): pass 1 if the exact compare held, pass 2 if only the
+// masked one did (and dead != 0).  hits_all = 1 (CUDA tool): one masked pass.
+// Output entry: (target, slot | pass << 30).
+__global__ __launch_bounds__(256) void k_midcheck(const uint2 *__restrict__ surv, uint32_t ns,
+                                                  const uint2 *__restrict__ tfp,
+                                                  const uint8_t *__restrict__ tbytes,
+                                                  const uint2 *__restrict__ cfp,
+                                                  const uint8_t *__restrict__ cbytes,
+                                                  const uint8_t *__restrict__ cprune,
+                                                  uint32_t pm_masked, int hits_all, int use_mid,
+                                                  uint2 *__restrict__ out, uint32_t out_cap,
+                                                  uint32_t *__restrict__ counters) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  bool pass_ok = false;
+  uint32_t t = 0, slot = 0, pass = 0;
+  bool qc_hit = false;
+  if (i < ns) {
+    uint2 e = surv[i];
+    t = e.x;
+    slot = e.y & 0x3FFFFFFFu;
+    bool full = (e.y >> 31) != 0;
+    if (cbytes[slot] < tbytes[t] && !cprune[slot]) {
+      uint32_t pm;
+      if (hits_all) {
+        pass = 1;
+        pm = pm_masked;
+      } else if (full) {
+        pass = 1;
+        pm = kPMask;
+      } else {
+        pass = 2;
+        pm = pm_masked;
+      }
+      qc_hit = true;
+      bool ok = true;
+      if (use_mid) {
+        const uint32_t mlo = 0x00FFFFFFu | (pm << 24);
+        const uint2 *a = tfp + (uint64_t)t * 32;
+        const uint2 *b = cfp + (uint64_t)slot * 32;
+        for (int v = 8; v < 32 && ok; v++) {
+          uint2 x = a[v], y = b[v];
+          ok = (((x.x ^ y.x) & mlo) | (x.y ^ y.y)) == 0;
+        }
+      }
+      pass_ok = ok;
+    }
+  }
+  uint64_t hq = __ballot(qc_hit);
+  uint64_t m = __ballot(pass_ok);
+  uint32_t lane = lane_id();
+  if (lane == 0 && hq) atomicAdd(&counters[0], (uint32_t)__popcll(hq));
+  if (m == 0) return;
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(&counters[1], (uint32_t)__popcll(m));
+  base = uni(base);
+  if (pass_ok) {
+    uint32_t o = base + mbcnt(m);
+    if (o < out_cap) out[o] = make_uint2(t, slot | (pass << 30));
+  }
+}
+
+// First-hit rounds.  cur/rej/done are per (target, pass-1): cur = lowest rank
+// this round, rej = 1 + highest rank rejected so far, done = resolved.
+__global__ __launch_bounds__(256) void k_round_min(const uint2 *mid, uint32_t nm,
+                                                   const uint32_t *rank, uint32_t *cur,
+                                                   const uint32_t *rej, const uint8_t *done) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nm) return;
+  uint2 e = mid[i];
+  uint32_t slot = e.y & 0x3FFFFFFFu, p = (e.y >> 30) - 1;
+  uint64_t k = (uint64_t)e.x * 2 + p;
+  if (done[k]) return;
+  uint32_t r = rank[slot];
+  if (r + 1 <= rej[k]) return;
+  atomicMin(&cur[k], r);
+}
+
+struct Job {
+  uint32_t tw[4];
+  uint32_t cw[4];
+  uint8_t tl, cl, pass, pmask;
+  uint32_t target;  // index within the segment (or pair index for sx_check_pairs)
+  uint32_t slot;
+  uint8_t nsets, pad[3];
+  uint8_t sets[8];
+  uint8_t logsz[8];
+};
+
+__device__ __forceinline__ void fill_job(Job &j, const uint32_t *targets, uint32_t tl,
+                                         const uint32_t *cwords, uint32_t cl, uint32_t t,
+                                         uint32_t slot, uint32_t pass, uint32_t pmask) {
+  for (int k = 0; k < 4; k++) {
+    j.tw[k] = k < (int)tl ? targets[(uint64_t)t * tl + k] : 0;
+    j.cw[k] = 0;
+  }
+  // candidate words: slot < 2863 are single instructions, the rest pairs
+  if (slot < (uint32_t)kNumInsts) {
+    j.cw[0] = go_inst(slot);
+    j.cl = 1;
+  } else {
+    uint32_t s2 = slot - kNumInsts;
+    j.cw[0] = go_inst(s2 / kNumInsts);
+    j.cw[1] = go_inst(s2 % kNumInsts);
+    j.cl = 2;
+  }
+  (void)cwords;
+  (void)cl;
+  j.tl = (uint8_t)tl;
+  j.pass = (uint8_t)pass;
+  j.pmask = (uint8_t)pmask;
+  j.target = t;
+  j.slot = slot;
+}
+
+__global__ __launch_bounds__(256) void k_round_jobs(const uint2 *mid, uint32_t nm, const uint32_t *rank,
+                                                    const uint32_t *cur, const uint32_t *targets,
+                                                    uint32_t tl, uint32_t pm_masked, int hits_all,
+                                                    Job *jobs, uint32_t job_cap, uint32_t *njobs) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  bool take = false;
+  uint2 e = make_uint2(0, 0);
+  if (i < nm) {
+    e = mid[i];
+    if (hits_all) {
+      take = true;
+    } else {
+      uint32_t slot = e.y & 0x3FFFFFFFu, p = (e.y >> 30) - 1;
+      take = cur[(uint64_t)e.x * 2 + p] == rank[slot];
+    }
+  }
+  uint64_t m = __ballot(take);
+  if (m == 0) return;
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(njobs, (uint32_t)__popcll(m));
+  base = uni(base);
+  if (take) {
+    uint32_t slot = e.y & 0x3FFFFFFFu, pass = e.y >> 30;
+    uint32_t pm = (hits_all || pass == 2) ? pm_masked : kPMask;
+    Job j;
+    fill_job(j, targets, tl, nullptr, 0, e.x, slot, pass, pm);
+    uint32_t o = base + mbcnt(m);
+    if (o < job_cap) jobs[o] = j;
+  }
+}
+
+// Plan each job's sweep and count its chunks.
+__global__ __launch_bounds__(256) void k_exh_plan(Job *jobs, uint32_t nj, int mode,
+                                                  uint32_t *chunks, uint32_t *fail) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nj) return;
+  Job j = jobs[i];
+  Plan pl = make_plan(j.tw, j.tl, j.cw, j.cl, j.pmask, mode, c_reads, c_dep);
+  uint32_t total = 0;
+  for (int s = 0; s < pl.nsets; s++) {
+    j.sets[s] = pl.sets[s];
+    j.logsz[s] = pl.logsz[s];
+    total += pl.logsz[s] > kChunkLog ? (1u << (pl.logsz[s] - kChunkLog)) : 1u;
+  }
+  j.nsets = pl.nsets;
+  jobs[i] = j;
+  chunks[i] = total;
+  fail[i] = 0;
+}
+
+// Persistent exhaustive sweep: waves pull chunk indices from *cursor.
+__global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, uint32_t nj,
+                                                 const uint64_t *__restrict__ offs,
+                                                 uint64_t total, unsigned long long *cursor,
+                                                 uint32_t *fail) {
+  const uint32_t lane = lane_id();
+  for (;;) {
+    unsigned long long c = 0;
+    if (lane == 0) c = atomicAdd(cursor, 1ull);
+    c = (unsigned long long)uni((uint32_t)c) | ((unsigned long long)uni((uint32_t)(c >> 32)) << 32);
+    if (c >= total) break;
+    // job = last index with offs[job] <= c
+    uint32_t lo = 0, hi = nj;
+    while (hi - lo > 1) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (offs[mid] <= c) lo = mid; else hi = mid;
+    }
+    const uint32_t ji = uni(lo);
+    if (((volatile uint32_t *)fail)[ji]) continue;
+    const Job *jp = &jobs[ji];
+    uint64_t local = c - offs[ji];
+    uint32_t nsets = uni(jp->nsets);
+    uint32_t set = 0, lg = 0;
+    for (uint32_t s = 0; s < nsets; s++) {
+      uint32_t l = uni(jp->logsz[s]);
+      uint64_t nch = l > kChunkLog ? (1ull << (l - kChunkLog)) : 1ull;
+      if (local < nch) {
+        set = uni(jp->sets[s]);
+        lg = l;
+        break;
+      }
+      local -= nch;
+    }
+    // bit positions of the varying registers in the state index
+    uint32_t sh[8], wd[8], pos = 0;
+    for (int r = 0; r < 8; r++) {
+      uint32_t w = (set >> r) & 1u ? (r == RP ? 1u : 8u) : 0u;
+      sh[r] = pos;
+      wd[r] = w;
+      pos += w;
+    }
+    const uint32_t tl = uni(jp->tl), cl = uni(jp->cl);
+    uint32_t tw[4], cw[4];
+    for (int k = 0; k < 4; k++) {
+      tw[k] = uni(jp->tw[k]);
+      cw[k] = uni(jp->cw[k]);
+    }
+    const uint32_t pm = uni(jp->pmask);
+    const uint64_t start = local << kChunkLog;
+    const uint64_t size = 1ull << lg;
+    const uint32_t count = (uint32_t)(size < (1ull << kChunkLog) ? size : (1ull << kChunkLog));
+    bool bad = false;
+    for (uint32_t it = 0; it < count; it += kWave) {
+      uint32_t k = it + lane;
+      if (k < count) {
+        uint32_t idx = (uint32_t)(start + k);
+        State s;
+        s.a = wd[RA] ? (idx >> sh[RA]) & 0xFFu : 0u;
+        s.x = wd[RX] ? (idx >> sh[RX]) & 0xFFu : 0u;
+        s.y = wd[RY] ? (idx >> sh[RY]) & 0xFFu : 0u;
+        s.p = wd[RP] ? (idx >> sh[RP]) & 0x1u : 0u;
+        s.s = wd[RS] ? (idx >> sh[RS]) & 0xFFu : 0xFDu;
+        s.m = wd[RM] ? (idx >> sh[RM]) & 0xFFu : 0u;
+        s.s0 = wd[RS0] ? (idx >> sh[RS0]) & 0xFFu : 0u;
+        s.s1 = wd[RS1] ? (idx >> sh[RS1]) & 0xFFu : 0u;
+        State ts = s, cs = s;
+        for (uint32_t q = 0; q < tl; q++) step(ts, tw[q] & 0xFFFFu, tw[q] >> 16);
+        for (uint32_t q = 0; q < cl; q++) step(cs, cw[q] & 0xFFFFu, cw[q] >> 16);
+        uint32_t diff = (ts.a ^ cs.a) | (ts.x ^ cs.x) | (ts.y ^ cs.y) | ((ts.p ^ cs.p) & pm) |
+                        (ts.s ^ cs.s) | (ts.m ^ cs.m) | (ts.s0 ^ cs.s0) | (ts.s1 ^ cs.s1);
+        bad = diff != 0;
+      }
+      if (__ballot(bad)) {
+        if (lane == 0) ((volatile uint32_t *)fail)[ji] = 1u;
+        break;
+      }
+      if ((it & (32 * kWave - 1)) == 0 && it && ((volatile uint32_t *)fail)[ji]) break;
+    }
+  }
+}
+
+// Resolve a round: a passing job emits its rule and closes (target, pass); a
+// failing one records its rank as rejected.  Every job resets cur.
+__global__ __launch_bounds__(256) void k_round_resolve(const Job *jobs, uint32_t nj, const uint32_t *fail,
+                                                       const uint32_t *rank, uint32_t *cur, uint32_t *rej,
+                                                       uint8_t *done, const uint2 *tfp, const uint2 *cfp,
+                                                       const uint8_t *tbytes, const uint8_t *tcyc,
+                                                       const uint8_t *cbytes, const uint8_t *ccyc,
+                                                       int hits_all, uint32_t dead, uint64_t tbase,
+                                                       sx_rule *rules, uint32_t rule_cap,
+                                                       uint32_t *nrules) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nj) return;
+  const Job &j = jobs[i];
+  uint64_t k = (uint64_t)j.target * 2 + (j.pass - 1);
+  bool ok = !fail[i];
+  uint32_t fd = 0;
+  if (ok && !hits_all && j.pass == 2) {
+    const uint2 *a = tfp + (uint64_t)j.target * 32, *b = cfp + (uint64_t)j.slot * 32;
+    for (int v = 0; v < 8; v++) fd |= ((a[v].x ^ b[v].x) >> 24) & kPMask;
+    ok = fd != 0;  // FlagDiff == 0 is skipped (worker.go:211-213)
+  }
+  if (!hits_all) {
+    cur[k] = kRankNone;
+    if (ok) done[k] = 1;
+    else rej[k] = rank[j.slot] + 1;
+  }
+  if (!ok) return;
+  uint32_t o = atomicAdd(nrules, 1u);
+  if (o >= rule_cap) return;
+  sx_rule r;
+  for (int q = 0; q < 4; q++) {
+    r.src[q] = j.tw[q];
+    r.rep[q] = j.cw[q];
+  }
+  r.target_index = tbase + j.target;
+  r.src_len = j.tl;
+  r.rep_len = j.cl;
+  r.pass = j.pass;
+  r.dead_flags = (uint8_t)(hits_all ? dead : (j.pass == 2 ? fd : 0));
+  r.bytes_saved = (int16_t)((int)tbytes[j.target] - (int)cbytes[j.slot]);
+  r.cycles_saved = (int16_t)((int)tcyc[j.target] - (int)ccyc[j.slot]);
+  rules[o] = r;
+}
+
+// Bulk QuickCheck / MidCheck / FlagDiff for sx_check_pairs: one wave per pair.
+__global__ __launch_bounds__(256) void k_pair_vectors(const uint32_t *tw, const uint8_t *tl,
+                                                      const uint32_t *cw, const uint8_t *cl,
+                                                      uint64_t n, uint32_t pm, uint8_t *qc,
+                                                      uint8_t *mid, uint8_t *fdiff) {
+  uint64_t p = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= n) return;
+  uint32_t lane = lane_id();
+  uint32_t t[4], c[4];
+  uint32_t tn = uni(tl[p]), cn = uni(cl[p]);
+  for (int k = 0; k < 4; k++) {
+    t[k] = uni(tw[p * 4 + k]);
+    c[k] = uni(cw[p * 4 + k]);
+  }
+  bool eq = true, nonp = true;
+  uint32_t fd = 0;
+  if (lane < 32) {
+    State a = load_vec(lane), b = a;
+    run_seq_uniform(a, t, (int)tn);
+    run_seq_uniform(b, c, (int)cn);
+    nonp = ((a.a ^ b.a) | (a.x ^ b.x) | (a.y ^ b.y) | (a.s ^ b.s) | (a.m ^ b.m) | (a.s0 ^ b.s0) |
+            (a.s1 ^ b.s1)) == 0;
+    eq = nonp && ((a.p ^ b.p) & pm) == 0;
+    fd = (a.p ^ b.p) & kPMask;
+  }
+  uint64_t bad_all = __ballot(!eq);
+  uint64_t bad_qc = bad_all & 0xFFull;
+  uint64_t nonp_bad = __ballot(!nonp) & 0xFFull;
+  // FlagDiff over the 8 QuickCheck vectors (verifier.go:591-604)
+  uint32_t f = lane < 8 ? fd : 0;
+  for (int off = 32; off >= 1; off >>= 1) f |= __shfl_xor(f, off);
+  if (lane == 0) {
+    if (qc) qc[p] = bad_qc == 0;
+    if (mid) mid[p] = bad_all == 0;
+    if (fdiff) fdiff[p] = nonp_bad ? 0 : (uint8_t)f;
+  }
+}
+
+__global__ void k_exec_states(uint8_t *states, const uint32_t *words, const uint8_t *len, uint64_t n,
+                              uint32_t *bad_op) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint8_t *b = states + i * 8;
+  State s;
+  s.a = b[0]; s.x = b[1]; s.y = b[2]; s.p = b[3]; s.s = b[4]; s.m = b[5]; s.s0 = b[6]; s.s1 = b[7];
+  for (int k = 0; k < len[i]; k++) {
+    uint32_t w = words[i * 4 + k];
+    if (!step(s, w & 0xFFFFu, w >> 16)) atomicOr(bad_op, 1u);
+  }
+  b[0] = s.a; b[1] = s.x; b[2] = s.y; b[3] = s.p; b[4] = s.s; b[5] = s.m; b[6] = s.s0; b[7] = s.s1;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u8(uint8_t *p, uint64_t n, uint8_t v) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Candidate ranks: slot order is Go order, so the rank is the slot.
+__global__ void k_iota(uint32_t *p, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (uint32_t)i;
+}
+
+__global__ void k_cand_words(uint32_t *w, uint64_t n) {
+  // 2 words per slot: single instructions then Go-ordered pairs
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (i < (uint64_t)kNumInsts) {
+    w[2 * i] = go_inst((uint32_t)i);
+    w[2 * i + 1] = 0;
+  } else {
+    uint64_t s = i - kNumInsts;
+    w[2 * i] = go_inst((uint32_t)(s / kNumInsts));
+    w[2 * i + 1] = go_inst((uint32_t)(s % kNumInsts));
+  }
+}
+
+// -------------------------------------------------------------- host side --
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+};
+
+}  // namespace
+
+struct sx_handle {
+  int device = 0;
+  int num_cu = 256;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<sx_rule> rules;
+  // cached pruned length-2 target list (Go order)
+  uint32_t *d_l2 = nullptr;
+  uint64_t n_l2 = 0;
+  // buffers reused across calls
+  DevBuf bufs[40];
+};
+
+namespace {
+
+enum BufId {
+  B_TARGETS, B_TFP, B_TV0, B_TBYTES, B_TCYC, B_TPRUNE, B_CWORDS, B_CFP, B_CBYTES, B_CCYC, B_CPRUNE,
+  B_RANK, B_SURV, B_MID, B_COUNTERS, B_CUR, B_REJ, B_DONE, B_JOBS, B_CHUNKS, B_OFFS, B_SUMS,
+  B_FAIL, B_RULES, B_SCRATCH, B_GEN_COUNTS, B_GEN_OFFS, B_BADOP, B_P_TW, B_P_TL, B_P_CW, B_P_CL,
+  B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR
+};
+
+int set_err(sx_handle *h, int code, const char *fmt, ...) {
+  if (h) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    h->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK(h, call)                                                                        \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return set_err((h), SX_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),      \
+                     __FILE__, __LINE__);                                                      \
+  } while (0)
+
+template <typename T>
+int ensure(sx_handle *h, BufId id, size_t count, T **out) {
+  DevBuf &b = h->bufs[id];
+  size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+  if (b.cap < bytes) {
+    if (b.p) {
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      HIPCHK(h, hipFree(b.p));
+      b.p = nullptr;
+      b.cap = 0;
+    }
+    size_t alloc = bytes + bytes / 8;
+    if (hipMalloc(&b.p, alloc) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_err(h, SX_ENOMEM, "hipMalloc(%zu) failed", alloc);
+    }
+    b.cap = alloc;
+  }
+  *out = (T *)b.p;
+  return SX_OK;
+}
+
+#define ENSURE(id, T, n, ptr)                        \
+  do {                                               \
+    int rc_ = ensure<T>(h, (id), (size_t)(n), &(ptr)); \
+    if (rc_) return rc_;                             \
+  } while (0)
+
+inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Exclusive scan of n uint32 counts into uint64 offsets; *total on device.
+int scan_u32(sx_handle *h, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *d_total) {
+  uint64_t nb = (n + 1023) / 1024;
+  uint64_t *sums;
+  ENSURE(B_SUMS, uint64_t, nb + 1, sums);
+  if (n == 0) {
+    HIPCHK(h, hipMemsetAsync(d_total, 0, sizeof(uint64_t), h->stream));
+    return SX_OK;
+  }
+  hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(1024), 0, h->stream, in, n, out, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, h->stream, sums, nb, d_total);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(1024), 0, h->stream, out, n, sums);
+  HIPCHK(h, hipGetLastError());
+  return SX_OK;
+}
+
+int ensure_l2(sx_handle *h) {
+  if (h->d_l2) return SX_OK;
+  const uint64_t raw = (uint64_t)kNumInsts * kNumInsts;
+  const uint64_t nb = blocks_for(raw, 256);
+  uint32_t *counts;
+  uint64_t *offs, *total;
+  ENSURE(B_GEN_COUNTS, uint32_t, nb, counts);
+  ENSURE(B_GEN_OFFS, uint64_t, nb, offs);
+  ENSURE(B_TOTAL, uint64_t, 1, total);
+  hipLaunchKernelGGL(k_gen_count, dim3((unsigned)nb), dim3(256), 0, h->stream, counts);
+  int rc = scan_u32(h, counts, nb, offs, total);
+  if (rc) return rc;
+  uint64_t n = 0;
+  HIPCHK(h, hipMemcpyAsync(&n, total, sizeof(n), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMalloc(&h->d_l2, n * 2 * sizeof(uint32_t)));
+  hipLaunchKernelGGL(k_gen_write, dim3((unsigned)nb), dim3(256), 0, h->stream, offs, h->d_l2);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->n_l2 = n;
+  return SX_OK;
+}
+
+struct Timer {
+  hipEvent_t a = nullptr, b = nullptr;
+};
+
+// Run the exhaustive stage on njobs jobs already in B_JOBS.  Returns the
+// number of chunks in *chunks_out (for statistics).
+int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out) {
+  Job *jobs = (Job *)h->bufs[B_JOBS].p;
+  uint32_t *chunks;
+  uint64_t *offs, *total;
+  uint32_t *fail;
+  unsigned long long *cursor;
+  ENSURE(B_CHUNKS, uint32_t, nj + 1, chunks);
+  ENSURE(B_OFFS, uint64_t, nj + 1, offs);
+  ENSURE(B_FAIL, uint32_t, nj + 1, fail);
+  ENSURE(B_TOTAL, uint64_t, 1, total);
+  ENSURE(B_CURSOR, unsigned long long, 1, cursor);
+  hipLaunchKernelGGL(k_exh_plan, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, jobs, nj, mode,
+                     chunks, fail);
+  HIPCHK(h, hipGetLastError());
+  int rc = scan_u32(h, chunks, nj, offs, total);
+  if (rc) return rc;
+  uint64_t tot = 0;
+  HIPCHK(h, hipMemcpyAsync(&tot, total, sizeof(tot), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemsetAsync(cursor, 0, sizeof(*cursor), h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  *chunks_out = tot;
+  if (tot == 0) return SX_OK;
+  unsigned grid = (unsigned)std::min<uint64_t>((uint64_t)h->num_cu * 8, (tot + 3) / 4);
+  hipLaunchKernelGGL(k_exh_run, dim3(grid), dim3(256), 0, h->stream, jobs, nj, offs, tot, cursor,
+                     fail);
+  HIPCHK(h, hipGetLastError());
+  return SX_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- ABI --
+
+extern "C" int sx_abi_version(void) { return SX_ABI_VERSION; }
+
+extern "C" int sx_open(int device, sx_handle **out) {
+  if (!out) return SX_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) {
+    (void)hipGetLastError();
+    return SX_ENODEV;
+  }
+  sx_handle *h = new sx_handle();
+  h->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete h;
+    return SX_ENODEV;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->num_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return SX_EHIP;
+  }
+  *out = h;
+  return SX_OK;
+}
+
+extern "C" int sx_close(sx_handle *h) {
+  if (!h) return SX_EINVAL;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto &b : h->bufs)
+    if (b.p) (void)hipFree(b.p);
+  if (h->d_l2) (void)hipFree(h->d_l2);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return SX_OK;
+}
+
+extern "C" int sx_last_error(sx_handle *h, char *buf, size_t cap) {
+  if (!h || !buf || cap == 0) return SX_EINVAL;
+  snprintf(buf, cap, "%s", h->err.c_str());
+  return SX_OK;
+}
+
+extern "C" int sx_pruned_targets(sx_handle *h, uint64_t first, uint64_t n, uint32_t *out_words,
+                                 uint64_t *total) {
+  if (!h) return SX_EINVAL;
+  HIPCHK(h, hipSetDevice(h->device));
+  int rc = ensure_l2(h);
+  if (rc) return rc;
+  if (total) *total = h->n_l2;
+  if (!out_words || n == 0) return SX_OK;
+  if (first > h->n_l2 || n > h->n_l2 - first) return set_err(h, SX_EINVAL, "range out of bounds");
+  HIPCHK(h, hipMemcpy(out_words, h->d_l2 + 2 * first, n * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return SX_OK;
+}
+
+static int check_words(sx_handle *h, const uint32_t *w, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++)
+    if ((w[i] & 0xFFFFu) >= OP_COUNT) return set_err(h, SX_EOPCODE, "opcode %u out of range", w[i] & 0xFFFFu);
+  return SX_OK;
+}
+
+extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, uint64_t cap,
+                            uint64_t *n_out, sx_stats *stats) {
+  if (!h || !cfg) return SX_EINVAL;
+  if (cfg->struct_size != sizeof(sx_enum_cfg)) return set_err(h, SX_EINVAL, "sx_enum_cfg size mismatch");
+  const int tl = cfg->target_len, ml = cfg->max_cand_len;
+  if (tl < 2 || tl > 3 || ml < 1 || ml > 2 || ml >= tl)
+    return set_err(h, SX_EINVAL, "target_len %d / max_cand_len %d unsupported", tl, ml);
+  if (cfg->bytes_model < 0 || cfg->bytes_model > 1 || cfg->hits < 0 || cfg->hits > 1 ||
+      cfg->exh_mode < 0 || cfg->exh_mode > 1 || cfg->dead_flags > 0xFF)
+    return set_err(h, SX_EINVAL, "bad mode");
+  if (!cfg->targets && tl != 2) return set_err(h, SX_EINVAL, "implicit targets need target_len 2");
+  HIPCHK(h, hipSetDevice(h->device));
+  auto wall0 = std::chrono::steady_clock::now();
+  sx_stats st;
+  memset(&st, 0, sizeof(st));
+  h->rules.clear();
+
+  // ---- targets
+  const uint32_t *d_tsrc = nullptr;
+  uint64_t nt_all = 0, tbase = 0;
+  if (cfg->targets) {
+    int rc = check_words(h, cfg->targets, cfg->n_targets * tl);
+    if (rc) return rc;
+    nt_all = cfg->n_targets;
+  } else {
+    int rc = ensure_l2(h);
+    if (rc) return rc;
+    uint64_t a = cfg->first_target, b = cfg->end_target;
+    if (a == 0 && b == 0) b = h->n_l2;
+    if (a > b || b > h->n_l2) return set_err(h, SX_EINVAL, "target range out of bounds");
+    tbase = a;
+    nt_all = b - a;
+    d_tsrc = h->d_l2 + 2 * a;
+  }
+
+  // ---- candidates: slots in Go order (rank = slot)
+  const uint64_t nc = ml == 1 ? (uint64_t)kNumInsts : (uint64_t)kNumInsts + (uint64_t)kNumInsts * kNumInsts;
+  uint32_t *cwords, *rank, *badop;
+  uint2 *cfp;
+  uint8_t *cbytes, *ccyc, *cprune;
+  ENSURE(B_CWORDS, uint32_t, 2 * nc, cwords);
+  ENSURE(B_RANK, uint32_t, nc, rank);
+  ENSURE(B_CFP, uint2, nc * 32, cfp);
+  ENSURE(B_CBYTES, uint8_t, nc, cbytes);
+  ENSURE(B_CCYC, uint8_t, nc, ccyc);
+  ENSURE(B_CPRUNE, uint8_t, nc, cprune);
+  ENSURE(B_BADOP, uint32_t, 1, badop);
+  hipEvent_t e0, e1, e2, e3, e4;
+  HIPCHK(h, hipEventCreate(&e0));
+  HIPCHK(h, hipEventCreate(&e1));
+  HIPCHK(h, hipEventCreate(&e2));
+  HIPCHK(h, hipEventCreate(&e3));
+  HIPCHK(h, hipEventCreate(&e4));
+  HIPCHK(h, hipEventRecord(e0, h->stream));
+  HIPCHK(h, hipMemsetAsync(badop, 0, 4, h->stream));
+  hipLaunchKernelGGL(k_cand_words, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cwords, nc);
+  hipLaunchKernelGGL(k_iota, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, rank, nc);
+  // fingerprints of single-instruction slots, then of pairs
+  {
+    uint32_t *w1;
+    ENSURE(B_SCRATCH, uint32_t, kNumInsts, w1);
+    std::vector<uint32_t> tab(kNumInsts);
+    for (int i = 0; i < kNumInsts; i++) tab[i] = go_inst(i);
+    HIPCHK(h, hipMemcpyAsync(w1, tab.data(), kNumInsts * 4, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(kNumInsts, 4)), dim3(256), 0, h->stream, w1, 1u,
+                       (uint64_t)kNumInsts, cfg->bytes_model, cfp, cbytes, ccyc, cprune, badop);
+    if (ml == 2) {
+      // pairs: words at cwords[2*slot] for slot >= 2863 are contiguous pairs
+      hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(nc - kNumInsts, 4)), dim3(256), 0, h->stream,
+                         cwords + 2 * kNumInsts, 2u, nc - kNumInsts, cfg->bytes_model,
+                         cfp + (uint64_t)kNumInsts * 32, cbytes + kNumInsts, ccyc + kNumInsts,
+                         cprune + kNumInsts, badop);
+    }
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));  // w1 is host-staged
+  }
+
+  // ---- segments of targets
+  const uint64_t seg = cfg->targets ? std::min<uint64_t>(nt_all, 1u << 20) : (1u << 20);
+  const uint32_t tile = cfg->tile_targets ? cfg->tile_targets : (ml == 1 ? 2048u : 4096u);
+  uint32_t surv_cap = cfg->survivor_cap ? cfg->survivor_cap : (1u << 25);
+  const uint32_t pm_masked = kPMask & ~cfg->dead_flags;
+  float ms_qc = 0, ms_mid = 0, ms_exh = 0, ms_setup = 0;
+  HIPCHK(h, hipEventRecord(e1, h->stream));
+  HIPCHK(h, hipEventSynchronize(e1));
+  HIPCHK(h, hipEventElapsedTime(&ms_setup, e0, e1));
+
+  for (uint64_t s0 = 0; s0 < nt_all; s0 += seg) {
+    const uint64_t nt = std::min<uint64_t>(seg, nt_all - s0);
+    uint32_t *tw;
+    uint2 *tfp, *tv0;
+    uint8_t *tbytes, *tcyc, *tprune;
+    ENSURE(B_TARGETS, uint32_t, nt * tl, tw);
+    ENSURE(B_TFP, uint2, nt * 32, tfp);
+    ENSURE(B_TV0, uint2, nt, tv0);
+    ENSURE(B_TBYTES, uint8_t, nt, tbytes);
+    ENSURE(B_TCYC, uint8_t, nt, tcyc);
+    ENSURE(B_TPRUNE, uint8_t, nt, tprune);
+    HIPCHK(h, hipEventRecord(e1, h->stream));
+    if (cfg->targets) {
+      HIPCHK(h, hipMemcpyAsync(tw, cfg->targets + s0 * tl, nt * tl * 4, hipMemcpyHostToDevice, h->stream));
+    } else {
+      HIPCHK(h, hipMemcpyAsync(tw, d_tsrc + s0 * 2, nt * 8, hipMemcpyDeviceToDevice, h->stream));
+    }
+    hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(nt, 4)), dim3(256), 0, h->stream, tw, (uint32_t)tl,
+                       nt, cfg->bytes_model, tfp, tbytes, tcyc, tprune, badop);
+    hipLaunchKernelGGL(k_gather_v0, dim3(blocks_for(nt, 256)), dim3(256), 0, h->stream, tfp, nt, tv0);
+    HIPCHK(h, hipGetLastError());
+
+    // ---- QuickCheck, re-run with a larger buffer on overflow (no hit is dropped)
+    uint2 *surv;
+    uint32_t *counters;
+    ENSURE(B_COUNTERS, uint32_t, 8, counters);
+    uint32_t nsurv = 0;
+    for (;;) {
+      ENSURE(B_SURV, uint2, surv_cap, surv);
+      HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));
+      dim3 grid(blocks_for(nc, 256), (unsigned)((nt + tile - 1) / tile));
+      hipLaunchKernelGGL(k_quickcheck, grid, dim3(256), 0, h->stream, tv0, tfp, (uint32_t)nt, tile, cfp,
+                         (uint32_t)nc, pm_masked, surv, surv_cap, counters);
+      HIPCHK(h, hipGetLastError());
+      HIPCHK(h, hipMemcpyAsync(&nsurv, counters, 4, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      if (nsurv <= surv_cap) break;
+      st.overflow_retries++;
+      surv_cap = nsurv + nsurv / 4 + 1024;
+    }
+    HIPCHK(h, hipEventRecord(e2, h->stream));
+    st.qc_raw_hits += nsurv;
+    st.qc_pairs += nt * (uint64_t)(ml == 1 ? kNumInsts : nc);
+
+    // ---- MidCheck
+    uint2 *mid;
+    uint32_t nmid = 0;
+    for (;;) {
+      ENSURE(B_MID, uint2, std::max<uint32_t>(nsurv, 1), mid);
+      HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));
+      if (nsurv)
+        hipLaunchKernelGGL(k_midcheck, dim3(blocks_for(nsurv, 256)), dim3(256), 0, h->stream, surv, nsurv,
+                           tfp, tbytes, cfp, cbytes, cprune, pm_masked, cfg->hits, cfg->use_mid, mid,
+                           std::max<uint32_t>(nsurv, 1), counters);
+      HIPCHK(h, hipGetLastError());
+      uint32_t cnt[2];
+      HIPCHK(h, hipMemcpyAsync(cnt, counters, 8, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      st.qc_hits += cnt[0];
+      nmid = cnt[1];
+      break;
+    }
+    st.mid_pass += nmid;
+    HIPCHK(h, hipEventRecord(e3, h->stream));
+
+    // ---- ExhaustiveCheck (rounds for first-hit)
+    uint32_t *cur, *rej;
+    uint8_t *done;
+    Job *jobs;
+    sx_rule *rules;
+    ENSURE(B_CUR, uint32_t, nt * 2, cur);
+    ENSURE(B_REJ, uint32_t, nt * 2, rej);
+    ENSURE(B_DONE, uint8_t, nt * 2, done);
+    const uint32_t job_cap = std::max<uint32_t>(cfg->hits ? nmid : (uint32_t)std::min<uint64_t>(nmid, 2 * nt), 1);
+    ENSURE(B_JOBS, Job, job_cap, jobs);
+    const uint32_t rule_cap = cfg->hits ? std::max<uint32_t>(nmid, 1) : (uint32_t)(2 * nt);
+    ENSURE(B_RULES, sx_rule, rule_cap, rules);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, nt * 2, kRankNone);
+    HIPCHK(h, hipMemsetAsync(rej, 0, nt * 2 * 4, h->stream));
+    HIPCHK(h, hipMemsetAsync(done, 0, nt * 2, h->stream));
+    uint32_t *nrules_d = counters + 2, *njobs_d = counters + 3;
+    HIPCHK(h, hipMemsetAsync(nrules_d, 0, 4, h->stream));
+    for (int round = 0; nmid > 0; round++) {
+      HIPCHK(h, hipMemsetAsync(njobs_d, 0, 4, h->stream));
+      if (!cfg->hits)
+        hipLaunchKernelGGL(k_round_min, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
+                           cur, rej, done);
+      hipLaunchKernelGGL(k_round_jobs, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
+                         cur, tw, (uint32_t)tl, pm_masked, cfg->hits, jobs, job_cap, njobs_d);
+      HIPCHK(h, hipGetLastError());
+      uint32_t nj = 0;
+      HIPCHK(h, hipMemcpyAsync(&nj, njobs_d, 4, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      if (nj == 0) break;
+      if (nj > job_cap) return set_err(h, SX_ENOMEM, "job buffer overflow (%u > %u)", nj, job_cap);
+      st.exh_rounds++;
+      st.exh_pairs += nj;
+      uint64_t chunks = 0;
+      int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks);
+      if (rc) return rc;
+      st.exh_states += chunks << kChunkLog;
+      hipLaunchKernelGGL(k_round_resolve, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream,
+                         (const Job *)h->bufs[B_JOBS].p, nj, (const uint32_t *)h->bufs[B_FAIL].p, rank, cur, rej,
+                         done, tfp, cfp, tbytes, tcyc, cbytes, ccyc, cfg->hits, cfg->dead_flags, tbase + s0,
+                         rules, rule_cap, nrules_d);
+      HIPCHK(h, hipGetLastError());
+      if (cfg->hits) break;
+      // rule buffer growth check happens after the round
+      uint32_t nr = 0;
+      HIPCHK(h, hipMemcpyAsync(&nr, nrules_d, 4, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      if (nr > rule_cap) return set_err(h, SX_ENOMEM, "rule buffer overflow");
+    }
+    uint32_t nr = 0;
+    HIPCHK(h, hipMemcpyAsync(&nr, nrules_d, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipEventRecord(e4, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (nr > rule_cap) return set_err(h, SX_ENOMEM, "rule buffer overflow (%u rules)", nr);
+    size_t old = h->rules.size();
+    h->rules.resize(old + nr);
+    if (nr) HIPCHK(h, hipMemcpy(h->rules.data() + old, rules, nr * sizeof(sx_rule), hipMemcpyDeviceToHost));
+    float a = 0, b = 0, c = 0;
+    HIPCHK(h, hipEventElapsedTime(&a, e1, e2));
+    HIPCHK(h, hipEventElapsedTime(&b, e2, e3));
+    HIPCHK(h, hipEventElapsedTime(&c, e3, e4));
+    ms_qc += a;
+    ms_mid += b;
+    ms_exh += c;
+    st.targets += nt;
+  }
+  uint32_t bad = 0;
+  HIPCHK(h, hipMemcpy(&bad, badop, 4, hipMemcpyDeviceToHost));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  (void)hipEventDestroy(e3);
+  (void)hipEventDestroy(e4);
+  if (bad) return set_err(h, SX_EOPCODE, "opcode out of range in a target");
+  std::sort(h->rules.begin(), h->rules.end(), [](const sx_rule &x, const sx_rule &y) {
+    if (x.target_index != y.target_index) return x.target_index < y.target_index;
+    if (x.pass != y.pass) return x.pass < y.pass;
+    if (x.rep_len != y.rep_len) return x.rep_len < y.rep_len;
+    return memcmp(x.rep, y.rep, sizeof(x.rep)) < 0;
+  });
+  st.rules = h->rules.size();
+  st.ms_setup = ms_setup;
+  st.ms_qc = ms_qc;
+  st.ms_mid = ms_mid;
+  st.ms_exh = ms_exh;
+  st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+  if (stats) *stats = st;
+  if (n_out) *n_out = h->rules.size();
+  if (out) {
+    if (cap < h->rules.size()) return SX_ECAPACITY;
+    memcpy(out, h->rules.data(), h->rules.size() * sizeof(sx_rule));
+  }
+  return SX_OK;
+}
+
+extern "C" int sx_fetch_rules(sx_handle *h, sx_rule *out, uint64_t offset, uint64_t cap, uint64_t *n_copied) {
+  if (!h || (!out && cap)) return SX_EINVAL;
+  uint64_t n = h->rules.size();
+  uint64_t m = offset >= n ? 0 : std::min<uint64_t>(cap, n - offset);
+  if (m) memcpy(out, h->rules.data() + offset, m * sizeof(sx_rule));
+  if (n_copied) *n_copied = m;
+  return SX_OK;
+}
+
+extern "C" int sx_check_pairs(sx_handle *h, const uint32_t *t_words, const uint8_t *t_len,
+                              const uint32_t *c_words, const uint8_t *c_len, uint64_t n,
+                              uint32_t stage_mask, uint32_t dead_flags, int32_t exh_mode, uint8_t *out) {
+  if (!h || (n && (!t_words || !t_len || !c_words || !c_len || !out))) return SX_EINVAL;
+  if (dead_flags > 0xFF || exh_mode < 0 || exh_mode > 1 || (stage_mask & ~7u)) return set_err(h, SX_EINVAL, "bad argument");
+  for (uint64_t i = 0; i < n; i++) {
+    if (t_len[i] > 4 || c_len[i] > 4) return set_err(h, SX_EINVAL, "sequence longer than 4");
+    for (int k = 0; k < t_len[i]; k++)
+      if ((t_words[i * 4 + k] & 0xFFFFu) >= OP_COUNT) return set_err(h, SX_EOPCODE, "opcode out of range");
+    for (int k = 0; k < c_len[i]; k++)
+      if ((c_words[i * 4 + k] & 0xFFFFu) >= OP_COUNT) return set_err(h, SX_EOPCODE, "opcode out of range");
+  }
+  memset(out, 0, n);
+  if (n == 0) return SX_OK;
+  HIPCHK(h, hipSetDevice(h->device));
+  // zero the words beyond each length so unused slots are inert
+  std::vector<uint32_t> tw(t_words, t_words + 4 * n), cw(c_words, c_words + 4 * n);
+  for (uint64_t i = 0; i < n; i++)
+    for (int k = 0; k < 4; k++) {
+      if (k >= t_len[i]) tw[i * 4 + k] = 0;
+      if (k >= c_len[i]) cw[i * 4 + k] = 0;
+    }
+  uint32_t *dtw, *dcw;
+  uint8_t *dtl, *dcl, *o1, *o2;
+  ENSURE(B_P_TW, uint32_t, 4 * n, dtw);
+  ENSURE(B_P_CW, uint32_t, 4 * n, dcw);
+  ENSURE(B_P_TL, uint8_t, n, dtl);
+  ENSURE(B_P_CL, uint8_t, n, dcl);
+  ENSURE(B_P_OUT1, uint8_t, n, o1);
+  ENSURE(B_P_OUT2, uint8_t, n, o2);
+  HIPCHK(h, hipMemcpyAsync(dtw, tw.data(), 16 * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(dcw, cw.data(), 16 * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(dtl, t_len, n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(dcl, c_len, n, hipMemcpyHostToDevice, h->stream));
+  const uint32_t pm = kPMask & ~dead_flags;
+  std::vector<uint8_t> q(n), m(n), e(n, 0);
+  if (stage_mask & (SX_STAGE_QC | SX_STAGE_MID)) {
+    hipLaunchKernelGGL(k_pair_vectors, dim3(blocks_for(n, 4)), dim3(256), 0, h->stream, dtw, dtl, dcw, dcl, n, pm,
+                       o1, o2, (uint8_t *)nullptr);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(q.data(), o1, n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(m.data(), o2, n, hipMemcpyDeviceToHost, h->stream));
+  }
+  if (stage_mask & SX_STAGE_EXH) {
+    const uint64_t batch = 1u << 22;
+    for (uint64_t b0 = 0; b0 < n; b0 += batch) {
+      uint32_t nb = (uint32_t)std::min<uint64_t>(batch, n - b0);
+      std::vector<Job> jobs(nb);
+      for (uint32_t i = 0; i < nb; i++) {
+        Job &j = jobs[i];
+        memset(&j, 0, sizeof(j));
+        for (int k = 0; k < 4; k++) {
+          j.tw[k] = tw[(b0 + i) * 4 + k];
+          j.cw[k] = cw[(b0 + i) * 4 + k];
+        }
+        j.tl = t_len[b0 + i];
+        j.cl = c_len[b0 + i];
+        j.pass = 1;
+        j.pmask = (uint8_t)pm;
+        j.target = i;
+      }
+      Job *dj;
+      ENSURE(B_JOBS, Job, nb, dj);
+      HIPCHK(h, hipMemcpyAsync(dj, jobs.data(), nb * sizeof(Job), hipMemcpyHostToDevice, h->stream));
+      uint64_t chunks = 0;
+      int rc = run_exhaustive(h, nb, exh_mode, &chunks);
+      if (rc) return rc;
+      std::vector<uint32_t> f(nb);
+      HIPCHK(h, hipMemcpyAsync(f.data(), h->bufs[B_FAIL].p, nb * 4, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      for (uint32_t i = 0; i < nb; i++) e[b0 + i] = f[i] ? 0 : 1;
+    }
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (uint64_t i = 0; i < n; i++) {
+    uint8_t v = 0;
+    if ((stage_mask & SX_STAGE_QC) && q[i]) v |= SX_STAGE_QC;
+    if ((stage_mask & SX_STAGE_MID) && m[i]) v |= SX_STAGE_MID;
+    if ((stage_mask & SX_STAGE_EXH) && e[i]) v |= SX_STAGE_EXH;
+    out[i] = v;
+  }
+  return SX_OK;
+}
+
+extern "C" int sx_flag_diff(sx_handle *h, const uint32_t *t_words, const uint8_t *t_len, const uint32_t *c_words,
+                            const uint8_t *c_len, uint64_t n, uint8_t *out) {
+  if (!h || (n && (!t_words || !t_len || !c_words || !c_len || !out))) return SX_EINVAL;
+  if (n == 0) return SX_OK;
+  for (uint64_t i = 0; i < n; i++) {
+    if (t_len[i] > 4 || c_len[i] > 4) return set_err(h, SX_EINVAL, "sequence longer than 4");
+    for (int k = 0; k < t_len[i]; k++)
+      if ((t_words[i * 4 + k] & 0xFFFFu) >= OP_COUNT) return set_err(h, SX_EOPCODE, "opcode out of range");
+    for (int k = 0; k < c_len[i]; k++)
+      if ((c_words[i * 4 + k] & 0xFFFFu) >= OP_COUNT) return set_err(h, SX_EOPCODE, "opcode out of range");
+  }
+  HIPCHK(h, hipSetDevice(h->device));
+  uint32_t *dtw, *dcw;
+  uint8_t *dtl, *dcl, *o3;
+  ENSURE(B_P_TW, uint32_t, 4 * n, dtw);
+  ENSURE(B_P_CW, uint32_t, 4 * n, dcw);
+  ENSURE(B_P_TL, uint8_t, n, dtl);
+  ENSURE(B_P_CL, uint8_t, n, dcl);
+  ENSURE(B_P_OUT3, uint8_t, n, o3);
+  HIPCHK(h, hipMemcpyAsync(dtw, t_words, 16 * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(dcw, c_words, 16 * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(dtl, t_len, n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(dcl, c_len, n, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_pair_vectors, dim3(blocks_for(n, 4)), dim3(256), 0, h->stream, dtw, dtl, dcw, dcl, n,
+                     (uint32_t)kPMask, (uint8_t *)nullptr, (uint8_t *)nullptr, o3);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(out, o3, n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return SX_OK;
+}
+
+extern "C" int sx_exec_states(sx_handle *h, uint8_t *states, const uint32_t *words, const uint8_t *len, uint64_t n) {
+  if (!h || (n && (!states || !words || !len))) return SX_EINVAL;
+  if (n == 0) return SX_OK;
+  for (uint64_t i = 0; i < n; i++)
+    if (len[i] > 4) return set_err(h, SX_EINVAL, "sequence longer than 4");
+  HIPCHK(h, hipSetDevice(h->device));
+  uint8_t *ds, *dl;
+  uint32_t *dw, *bad;
+  ENSURE(B_STATES, uint8_t, 8 * n, ds);
+  ENSURE(B_P_TW, uint32_t, 4 * n, dw);
+  ENSURE(B_P_TL, uint8_t, n, dl);
+  ENSURE(B_BADOP, uint32_t, 1, bad);
+  HIPCHK(h, hipMemcpyAsync(ds, states, 8 * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(dw, words, 16 * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(dl, len, n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemsetAsync(bad, 0, 4, h->stream));
+  hipLaunchKernelGGL(k_exec_states, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, ds, dw, dl, n, bad);
+  HIPCHK(h, hipGetLastError());
+  uint32_t b = 0;
+  HIPCHK(h, hipMemcpyAsync(states, ds, 8 * n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(&b, bad, 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (b) return set_err(h, SX_EOPCODE, "opcode out of range");
+  return SX_OK;
+}
+
+extern "C" int sx_exh_plan(const uint32_t *t, int tn, const uint32_t *c, int cn, uint32_t dead_flags,
+                           int32_t exh_mode, uint8_t sets[8], int32_t *n_sets, uint64_t *states) {
+  if (!t || !c || tn < 0 || tn > 4 || cn < 0 || cn > 4 || !sets || !n_sets) return SX_EINVAL;
+  for (int i = 0; i < tn; i++)
+    if ((t[i] & 0xFFFFu) >= OP_COUNT) return SX_EOPCODE;
+  for (int i = 0; i < cn; i++)
+    if ((c[i] & 0xFFFFu) >= OP_COUNT) return SX_EOPCODE;
+  Plan pl = make_plan(t, tn, c, cn, kPMask & ~dead_flags, exh_mode, h_reads, h_dep);
+  uint64_t tot = 0;
+  for (int i = 0; i < 8; i++) sets[i] = i < pl.nsets ? pl.sets[i] : 0;
+  for (int i = 0; i < pl.nsets; i++) tot += 1ull << pl.logsz[i];
+  *n_sets = pl.nsets;
+  if (states) *states = tot;
+  return SX_OK;
+}
+
+extern "C" int sx_dep_table(uint8_t out[58 * 8]) {
+  if (!out) return SX_EINVAL;
+  memcpy(out, h_dep, sizeof(h_dep));
+  return SX_OK;
+}

