@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sx6502.h"
@@ -492,84 +493,84 @@ __global__ __launch_bounds__(256) void k_exh_plan(Job *jobs, uint32_t nj, int mo
   fail[i] = 0;
 }
 
-// Persistent exhaustive sweep: waves pull chunk indices from *cursor.
+// Exhaustive sweep: one wave per chunk of 2^14 states.  The chunks of all
+// jobs are numbered consecutively (offs = exclusive scan of the per-job chunk
+// counts); a launch covers chunks [chunk0, chunk0 + nchunks).  A mismatch sets
+// the job's fail flag, which the job's other waves poll to stop early.
 __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, uint32_t nj,
-                                                 const uint64_t *__restrict__ offs,
-                                                 uint64_t total, unsigned long long *cursor,
-                                                 uint32_t *fail) {
+                                                 const uint64_t *__restrict__ offs, uint64_t chunk0,
+                                                 uint64_t nchunks, uint32_t *fail) {
   const uint32_t lane = lane_id();
-  for (;;) {
-    unsigned long long c = 0;
-    if (lane == 0) c = atomicAdd(cursor, 1ull);
-    c = (unsigned long long)uni((uint32_t)c) | ((unsigned long long)uni((uint32_t)(c >> 32)) << 32);
-    if (c >= total) break;
-    // job = last index with offs[job] <= c
-    uint32_t lo = 0, hi = nj;
-    while (hi - lo > 1) {
-      uint32_t mid = (lo + hi) >> 1;
-      if (offs[mid] <= c) lo = mid; else hi = mid;
+  const uint64_t rel = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (rel >= nchunks) return;  // whole wave
+  const uint64_t c = chunk0 + rel;
+  // job = last index with offs[job] <= c
+  uint32_t lo = 0, hi = nj;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (offs[mid] <= c) lo = mid; else hi = mid;
+  }
+  const uint32_t ji = uni(lo);
+  if (((volatile uint32_t *)fail)[ji]) return;
+  const Job *jp = &jobs[ji];
+  uint64_t local = c - offs[ji];
+  const uint32_t nsets = uni(jp->nsets);
+  uint32_t set = 0, lg = 0;
+  for (uint32_t s = 0; s < nsets; s++) {
+    uint32_t l = uni(jp->logsz[s]);
+    uint64_t nch = l > kChunkLog ? (1ull << (l - kChunkLog)) : 1ull;
+    if (local < nch) {
+      set = uni(jp->sets[s]);
+      lg = l;
+      break;
     }
-    const uint32_t ji = uni(lo);
-    if (((volatile uint32_t *)fail)[ji]) continue;
-    const Job *jp = &jobs[ji];
-    uint64_t local = c - offs[ji];
-    uint32_t nsets = uni(jp->nsets);
-    uint32_t set = 0, lg = 0;
-    for (uint32_t s = 0; s < nsets; s++) {
-      uint32_t l = uni(jp->logsz[s]);
-      uint64_t nch = l > kChunkLog ? (1ull << (l - kChunkLog)) : 1ull;
-      if (local < nch) {
-        set = uni(jp->sets[s]);
-        lg = l;
-        break;
-      }
-      local -= nch;
-    }
-    // bit positions of the varying registers in the state index
-    uint32_t sh[8], wd[8], pos = 0;
-    for (int r = 0; r < 8; r++) {
-      uint32_t w = (set >> r) & 1u ? (r == RP ? 1u : 8u) : 0u;
-      sh[r] = pos;
-      wd[r] = w;
-      pos += w;
-    }
-    const uint32_t tl = uni(jp->tl), cl = uni(jp->cl);
-    uint32_t tw[4], cw[4];
-    for (int k = 0; k < 4; k++) {
-      tw[k] = uni(jp->tw[k]);
-      cw[k] = uni(jp->cw[k]);
-    }
-    const uint32_t pm = uni(jp->pmask);
-    const uint64_t start = local << kChunkLog;
-    const uint64_t size = 1ull << lg;
-    const uint32_t count = (uint32_t)(size < (1ull << kChunkLog) ? size : (1ull << kChunkLog));
+    local -= nch;
+  }
+  // bit positions of the varying registers in the 64-bit state index
+  uint32_t sh[8], wd[8], pos = 0;
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    uint32_t w = (set >> r) & 1u ? (r == RP ? 1u : 8u) : 0u;
+    sh[r] = pos;
+    wd[r] = w;
+    pos += w;
+  }
+  const uint32_t tl = uni(jp->tl), cl = uni(jp->cl);
+  uint32_t tw[4], cw[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    tw[k] = uni(jp->tw[k]);
+    cw[k] = uni(jp->cw[k]);
+  }
+  const uint32_t pm = uni(jp->pmask);
+  const uint64_t start = local << kChunkLog;
+  const uint32_t count = lg < kChunkLog ? (1u << lg) : (1u << kChunkLog);
+  for (uint32_t it = 0; it < count; it += kWave) {
+    const uint32_t k = it + lane;
     bool bad = false;
-    for (uint32_t it = 0; it < count; it += kWave) {
-      uint32_t k = it + lane;
-      if (k < count) {
-        uint32_t idx = (uint32_t)(start + k);
-        State s;
-        s.a = wd[RA] ? (idx >> sh[RA]) & 0xFFu : 0u;
-        s.x = wd[RX] ? (idx >> sh[RX]) & 0xFFu : 0u;
-        s.y = wd[RY] ? (idx >> sh[RY]) & 0xFFu : 0u;
-        s.p = wd[RP] ? (idx >> sh[RP]) & 0x1u : 0u;
-        s.s = wd[RS] ? (idx >> sh[RS]) & 0xFFu : 0xFDu;
-        s.m = wd[RM] ? (idx >> sh[RM]) & 0xFFu : 0u;
-        s.s0 = wd[RS0] ? (idx >> sh[RS0]) & 0xFFu : 0u;
-        s.s1 = wd[RS1] ? (idx >> sh[RS1]) & 0xFFu : 0u;
-        State ts = s, cs = s;
-        for (uint32_t q = 0; q < tl; q++) step(ts, tw[q] & 0xFFFFu, tw[q] >> 16);
-        for (uint32_t q = 0; q < cl; q++) step(cs, cw[q] & 0xFFFFu, cw[q] >> 16);
-        uint32_t diff = (ts.a ^ cs.a) | (ts.x ^ cs.x) | (ts.y ^ cs.y) | ((ts.p ^ cs.p) & pm) |
-                        (ts.s ^ cs.s) | (ts.m ^ cs.m) | (ts.s0 ^ cs.s0) | (ts.s1 ^ cs.s1);
-        bad = diff != 0;
-      }
-      if (__ballot(bad)) {
-        if (lane == 0) ((volatile uint32_t *)fail)[ji] = 1u;
-        break;
-      }
-      if ((it & (32 * kWave - 1)) == 0 && it && ((volatile uint32_t *)fail)[ji]) break;
+    if (k < count) {
+      const uint64_t idx = start + k;
+      State s;
+      s.a = wd[RA] ? (uint32_t)(idx >> sh[RA]) & 0xFFu : 0u;
+      s.x = wd[RX] ? (uint32_t)(idx >> sh[RX]) & 0xFFu : 0u;
+      s.y = wd[RY] ? (uint32_t)(idx >> sh[RY]) & 0xFFu : 0u;
+      s.p = wd[RP] ? (uint32_t)(idx >> sh[RP]) & 0x1u : 0u;
+      s.s = wd[RS] ? (uint32_t)(idx >> sh[RS]) & 0xFFu : 0xFDu;
+      s.m = wd[RM] ? (uint32_t)(idx >> sh[RM]) & 0xFFu : 0u;
+      s.s0 = wd[RS0] ? (uint32_t)(idx >> sh[RS0]) & 0xFFu : 0u;
+      s.s1 = wd[RS1] ? (uint32_t)(idx >> sh[RS1]) & 0xFFu : 0u;
+      State ts = s, cs = s;
+      for (uint32_t q = 0; q < tl; q++) step(ts, tw[q] & 0xFFFFu, tw[q] >> 16);
+      for (uint32_t q = 0; q < cl; q++) step(cs, cw[q] & 0xFFFFu, cw[q] >> 16);
+      uint32_t diff = (ts.a ^ cs.a) | (ts.x ^ cs.x) | (ts.y ^ cs.y) | ((ts.p ^ cs.p) & pm) |
+                      (ts.s ^ cs.s) | (ts.m ^ cs.m) | (ts.s0 ^ cs.s0) | (ts.s1 ^ cs.s1);
+      bad = diff != 0;
     }
+    if (__ballot(bad)) {
+      if (lane == 0) ((volatile uint32_t *)fail)[ji] = 1u;
+      return;
+    }
+    if ((it & (32 * kWave - 1)) == 32 * kWave - kWave && ((volatile uint32_t *)fail)[ji]) return;
   }
 }
 
@@ -713,6 +714,7 @@ struct DevBuf {
 struct sx_handle {
   int device = 0;
   int num_cu = 256;
+  bool poisoned = false;  // a device stage timed out: the stream is unusable
   hipStream_t stream = nullptr;
   std::string err;
   std::vector<sx_rule> rules;
@@ -752,13 +754,51 @@ int set_err(sx_handle *h, int code, const char *fmt, ...) {
                      __FILE__, __LINE__);                                                      \
   } while (0)
 
+// Stream synchronisation with a deadline: a stage that does not finish within
+// SX_WATCHDOG_S seconds (default 120) fails with SX_EHIP instead of blocking
+// the caller forever.  The handle must not be used after such an error.
+int sync_wd(sx_handle *h, const char *stage) {
+  static double limit = [] {
+    const char *e = getenv("SX_WATCHDOG_S");
+    return e ? atof(e) : 120.0;
+  }();
+  auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    hipError_t e = hipStreamQuery(h->stream);
+    if (e == hipSuccess) return SX_OK;
+    if (e != hipErrorNotReady)
+      return set_err(h, SX_EHIP, "%s: %s", stage, hipGetErrorString(e));
+    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (dt > limit) {
+      h->poisoned = true;
+      return set_err(h, SX_EHIP, "%s: device work did not finish within %.0f s", stage, limit);
+    }
+    std::this_thread::sleep_for(std::chrono::microseconds(dt < 0.01 ? 20 : 500));
+  }
+}
+
+// SX_TRACE=1 prints stage progress to stderr.
+void trace(sx_handle *h, const char *fmt, ...) {
+  static int on = getenv("SX_TRACE") ? atoi(getenv("SX_TRACE")) : 0;
+  if (!on) return;
+  (void)h;
+  va_list ap;
+  va_start(ap, fmt);
+  fprintf(stderr, "[sx] ");
+  vfprintf(stderr, fmt, ap);
+  fprintf(stderr, "\n");
+  fflush(stderr);
+  va_end(ap);
+}
+
 template <typename T>
 int ensure(sx_handle *h, BufId id, size_t count, T **out) {
   DevBuf &b = h->bufs[id];
   size_t bytes = std::max<size_t>(count * sizeof(T), 256);
   if (b.cap < bytes) {
     if (b.p) {
-      HIPCHK(h, hipStreamSynchronize(h->stream));
+      int rcw = sync_wd(h, "buffer resize");
+      if (rcw) return rcw;
       HIPCHK(h, hipFree(b.p));
       b.p = nullptr;
       b.cap = 0;
@@ -773,6 +813,12 @@ int ensure(sx_handle *h, BufId id, size_t count, T **out) {
   *out = (T *)b.p;
   return SX_OK;
 }
+
+#define SYNC_WD(stage)              \
+  do {                              \
+    int rcw_ = sync_wd(h, (stage)); \
+    if (rcw_) return rcw_;          \
+  } while (0)
 
 #define ENSURE(id, T, n, ptr)                        \
   do {                                               \
@@ -812,11 +858,11 @@ int ensure_l2(sx_handle *h) {
   if (rc) return rc;
   uint64_t n = 0;
   HIPCHK(h, hipMemcpyAsync(&n, total, sizeof(n), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  SYNC_WD("device sync");
   HIPCHK(h, hipMalloc(&h->d_l2, n * 2 * sizeof(uint32_t)));
   hipLaunchKernelGGL(k_gen_write, dim3((unsigned)nb), dim3(256), 0, h->stream, offs, h->d_l2);
   HIPCHK(h, hipGetLastError());
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  SYNC_WD("device sync");
   h->n_l2 = n;
   return SX_OK;
 }
@@ -825,19 +871,16 @@ struct Timer {
   hipEvent_t a = nullptr, b = nullptr;
 };
 
-// Run the exhaustive stage on njobs jobs already in B_JOBS.  Returns the
-// number of chunks in *chunks_out (for statistics).
+// Run the exhaustive stage on nj jobs already in B_JOBS; the verdicts are in
+// B_FAIL (0 = pass).  *chunks_out gets the number of chunks (statistics).
 int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out) {
   Job *jobs = (Job *)h->bufs[B_JOBS].p;
-  uint32_t *chunks;
+  uint32_t *chunks, *fail;
   uint64_t *offs, *total;
-  uint32_t *fail;
-  unsigned long long *cursor;
   ENSURE(B_CHUNKS, uint32_t, nj + 1, chunks);
   ENSURE(B_OFFS, uint64_t, nj + 1, offs);
   ENSURE(B_FAIL, uint32_t, nj + 1, fail);
   ENSURE(B_TOTAL, uint64_t, 1, total);
-  ENSURE(B_CURSOR, unsigned long long, 1, cursor);
   hipLaunchKernelGGL(k_exh_plan, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, jobs, nj, mode,
                      chunks, fail);
   HIPCHK(h, hipGetLastError());
@@ -845,14 +888,21 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out) {
   if (rc) return rc;
   uint64_t tot = 0;
   HIPCHK(h, hipMemcpyAsync(&tot, total, sizeof(tot), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipMemsetAsync(cursor, 0, sizeof(*cursor), h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  rc = sync_wd(h, "exhaustive plan");
+  if (rc) return rc;
   *chunks_out = tot;
-  if (tot == 0) return SX_OK;
-  unsigned grid = (unsigned)std::min<uint64_t>((uint64_t)h->num_cu * 8, (tot + 3) / 4);
-  hipLaunchKernelGGL(k_exh_run, dim3(grid), dim3(256), 0, h->stream, jobs, nj, offs, tot, cursor,
-                     fail);
-  HIPCHK(h, hipGetLastError());
+  trace(h, "  exhaustive: %u jobs, %llu chunks", nj, (unsigned long long)tot);
+  const uint64_t slice = 1ull << 20;  // chunks per launch (2^34 states)
+  for (uint64_t c0 = 0; c0 < tot; c0 += slice) {
+    uint64_t n = std::min<uint64_t>(slice, tot - c0);
+    hipLaunchKernelGGL(k_exh_run, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, h->stream, jobs, nj, offs, c0, n,
+                       fail);
+    HIPCHK(h, hipGetLastError());
+    if (c0 + slice < tot) {
+      rc = sync_wd(h, "exhaustive sweep");
+      if (rc) return rc;
+    }
+  }
   return SX_OK;
 }
 
@@ -926,6 +976,7 @@ static int check_words(sx_handle *h, const uint32_t *w, uint64_t n) {
 extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, uint64_t cap,
                             uint64_t *n_out, sx_stats *stats) {
   if (!h || !cfg) return SX_EINVAL;
+  if (h->poisoned) return set_err(h, SX_EHIP, "handle unusable after a device timeout");
   if (cfg->struct_size != sizeof(sx_enum_cfg)) return set_err(h, SX_EINVAL, "sx_enum_cfg size mismatch");
   const int tl = cfg->target_len, ml = cfg->max_cand_len;
   if (tl < 2 || tl > 3 || ml < 1 || ml > 2 || ml >= tl)
@@ -997,7 +1048,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
                          cprune + kNumInsts, badop);
     }
     HIPCHK(h, hipGetLastError());
-    HIPCHK(h, hipStreamSynchronize(h->stream));  // w1 is host-staged
+    SYNC_WD("device sync");  // w1 is host-staged
   }
 
   // ---- segments of targets
@@ -1007,7 +1058,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   const uint32_t pm_masked = kPMask & ~cfg->dead_flags;
   float ms_qc = 0, ms_mid = 0, ms_exh = 0, ms_setup = 0;
   HIPCHK(h, hipEventRecord(e1, h->stream));
-  HIPCHK(h, hipEventSynchronize(e1));
+  SYNC_WD("setup");
   HIPCHK(h, hipEventElapsedTime(&ms_setup, e0, e1));
 
   for (uint64_t s0 = 0; s0 < nt_all; s0 += seg) {
@@ -1045,7 +1096,9 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
                          (uint32_t)nc, pm_masked, surv, surv_cap, counters);
       HIPCHK(h, hipGetLastError());
       HIPCHK(h, hipMemcpyAsync(&nsurv, counters, 4, hipMemcpyDeviceToHost, h->stream));
-      HIPCHK(h, hipStreamSynchronize(h->stream));
+      SYNC_WD("device sync");
+      trace(h, "segment %llu: %llu targets, QuickCheck survivors %u (cap %u)", (unsigned long long)s0,
+            (unsigned long long)nt, nsurv, surv_cap);
       if (nsurv <= surv_cap) break;
       st.overflow_retries++;
       surv_cap = nsurv + nsurv / 4 + 1024;
@@ -1067,9 +1120,10 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       HIPCHK(h, hipGetLastError());
       uint32_t cnt[2];
       HIPCHK(h, hipMemcpyAsync(cnt, counters, 8, hipMemcpyDeviceToHost, h->stream));
-      HIPCHK(h, hipStreamSynchronize(h->stream));
+      SYNC_WD("device sync");
       st.qc_hits += cnt[0];
       nmid = cnt[1];
+      trace(h, "  MidCheck: %u QC hits after filters, %u pass", cnt[0], cnt[1]);
       break;
     }
     st.mid_pass += nmid;
@@ -1093,6 +1147,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     uint32_t *nrules_d = counters + 2, *njobs_d = counters + 3;
     HIPCHK(h, hipMemsetAsync(nrules_d, 0, 4, h->stream));
     for (int round = 0; nmid > 0; round++) {
+      // every round rejects or resolves one candidate per open (target, pass)
+      if ((uint64_t)round > nc + 1) return set_err(h, SX_EHIP, "first-hit rounds did not converge");
       HIPCHK(h, hipMemsetAsync(njobs_d, 0, 4, h->stream));
       if (!cfg->hits)
         hipLaunchKernelGGL(k_round_min, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
@@ -1102,9 +1158,10 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       HIPCHK(h, hipGetLastError());
       uint32_t nj = 0;
       HIPCHK(h, hipMemcpyAsync(&nj, njobs_d, 4, hipMemcpyDeviceToHost, h->stream));
-      HIPCHK(h, hipStreamSynchronize(h->stream));
+      SYNC_WD("device sync");
       if (nj == 0) break;
       if (nj > job_cap) return set_err(h, SX_ENOMEM, "job buffer overflow (%u > %u)", nj, job_cap);
+      trace(h, "  round %d: %u jobs", round, nj);
       st.exh_rounds++;
       st.exh_pairs += nj;
       uint64_t chunks = 0;
@@ -1120,13 +1177,13 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       // rule buffer growth check happens after the round
       uint32_t nr = 0;
       HIPCHK(h, hipMemcpyAsync(&nr, nrules_d, 4, hipMemcpyDeviceToHost, h->stream));
-      HIPCHK(h, hipStreamSynchronize(h->stream));
+      SYNC_WD("device sync");
       if (nr > rule_cap) return set_err(h, SX_ENOMEM, "rule buffer overflow");
     }
     uint32_t nr = 0;
     HIPCHK(h, hipMemcpyAsync(&nr, nrules_d, 4, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipEventRecord(e4, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    SYNC_WD("device sync");
     if (nr > rule_cap) return set_err(h, SX_ENOMEM, "rule buffer overflow (%u rules)", nr);
     size_t old = h->rules.size();
     h->rules.resize(old + nr);
@@ -1152,7 +1209,9 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     if (x.target_index != y.target_index) return x.target_index < y.target_index;
     if (x.pass != y.pass) return x.pass < y.pass;
     if (x.rep_len != y.rep_len) return x.rep_len < y.rep_len;
-    return memcmp(x.rep, y.rep, sizeof(x.rep)) < 0;
+    for (int q = 0; q < 4; q++)
+      if (x.rep[q] != y.rep[q]) return x.rep[q] < y.rep[q];
+    return false;
   });
   st.rules = h->rules.size();
   st.ms_setup = ms_setup;
@@ -1182,6 +1241,7 @@ extern "C" int sx_check_pairs(sx_handle *h, const uint32_t *t_words, const uint8
                               const uint32_t *c_words, const uint8_t *c_len, uint64_t n,
                               uint32_t stage_mask, uint32_t dead_flags, int32_t exh_mode, uint8_t *out) {
   if (!h || (n && (!t_words || !t_len || !c_words || !c_len || !out))) return SX_EINVAL;
+  if (h->poisoned) return set_err(h, SX_EHIP, "handle unusable after a device timeout");
   if (dead_flags > 0xFF || exh_mode < 0 || exh_mode > 1 || (stage_mask & ~7u)) return set_err(h, SX_EINVAL, "bad argument");
   for (uint64_t i = 0; i < n; i++) {
     if (t_len[i] > 4 || c_len[i] > 4) return set_err(h, SX_EINVAL, "sequence longer than 4");
@@ -1247,11 +1307,11 @@ extern "C" int sx_check_pairs(sx_handle *h, const uint32_t *t_words, const uint8
       if (rc) return rc;
       std::vector<uint32_t> f(nb);
       HIPCHK(h, hipMemcpyAsync(f.data(), h->bufs[B_FAIL].p, nb * 4, hipMemcpyDeviceToHost, h->stream));
-      HIPCHK(h, hipStreamSynchronize(h->stream));
+      SYNC_WD("device sync");
       for (uint32_t i = 0; i < nb; i++) e[b0 + i] = f[i] ? 0 : 1;
     }
   }
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  SYNC_WD("device sync");
   for (uint64_t i = 0; i < n; i++) {
     uint8_t v = 0;
     if ((stage_mask & SX_STAGE_QC) && q[i]) v |= SX_STAGE_QC;
@@ -1289,7 +1349,7 @@ extern "C" int sx_flag_diff(sx_handle *h, const uint32_t *t_words, const uint8_t
                      (uint32_t)kPMask, (uint8_t *)nullptr, (uint8_t *)nullptr, o3);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(out, o3, n, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  SYNC_WD("device sync");
   return SX_OK;
 }
 
@@ -1314,7 +1374,7 @@ extern "C" int sx_exec_states(sx_handle *h, uint8_t *states, const uint32_t *wor
   uint32_t b = 0;
   HIPCHK(h, hipMemcpyAsync(states, ds, 8 * n, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipMemcpyAsync(&b, bad, 4, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  SYNC_WD("device sync");
   if (b) return set_err(h, SX_EOPCODE, "opcode out of range");
   return SX_OK;
 }

@@ -1,0 +1,502 @@
+// 6502opt -- command-line front end over lib6502core (include/sx6502.h).
+//
+// Keeps the subcommands, flags and output schemas of the reference CLI
+// (cmd/6502opt/main.go): enumerate (:33-83), target (:88-132), verify-jsonl
+// (:136-145, :386-463) and the parsers parseDeadFlags / parseAssembly /
+// parseSingleInstruction / parseImmediate (:285-384, :465-494).  The search
+// itself runs on the GPU through the C ABI.
+//
+// Additions: --jsonl PATH writes the CUDA tool's JSONL schema
+// (cuda/6502_search_v2.cu:912-930); --bytes-model go|cuda, --hits first|all,
+// --exh projected|full, --no-mid and --device select the device semantics;
+// --first-target/--end-target select a range of the pruned target space (the
+// multi-process split of the CUDA tool, cu:775-777).
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sx6502.h"
+
+namespace {
+
+const char *kMnem[58] = {
+    "TAX", "TAY", "TXA", "TYA", "TSX", "TXS", "INX", "INY", "DEX", "DEY",
+    "ASL A", "LSR A", "ROL A", "ROR A", "CLC", "SEC", "CLD", "SED", "CLI", "SEI", "CLV",
+    "PHA", "PLA", "PHP", "PLP", "NOP",
+    "LDA #n", "LDX #n", "LDY #n", "ADC #n", "SBC #n", "AND #n", "ORA #n", "EOR #n",
+    "CMP #n", "CPX #n", "CPY #n",
+    "LDA M", "LDX M", "LDY M", "STA M", "STX M", "STY M", "ADC M", "SBC M", "AND M",
+    "ORA M", "EOR M", "CMP M", "CPX M", "CPY M", "INC M", "DEC M", "ASL M", "LSR M",
+    "ROL M", "ROR M", "BIT M"};
+
+bool has_imm(int op) { return op >= 26 && op <= 36; }
+int byte_size(int op, int model) { return model == 1 ? (op <= 25 ? 1 : 2) : (has_imm(op) ? 2 : 1); }
+int cycles(int op) {
+  if (op == 21 || op == 23) return 3;
+  if (op == 22 || op == 24) return 4;
+  if (op >= 51 && op <= 56) return 5;
+  if (op >= 37) return 3;
+  return 2;
+}
+
+std::string upper(std::string s) {
+  for (auto &c : s) c = (char)toupper((unsigned char)c);
+  return s;
+}
+std::string trim(const std::string &s) {
+  size_t a = s.find_first_not_of(" \t\r\n"), b = s.find_last_not_of(" \t\r\n");
+  return a == std::string::npos ? "" : s.substr(a, b - a + 1);
+}
+
+std::string disasm(uint32_t w) {
+  int op = w & 0xFFFF;
+  if (op >= 58) return "OP(" + std::to_string(op) + ")";
+  std::string m = kMnem[op];
+  if (!has_imm(op)) return m;
+  char hex[8];
+  snprintf(hex, sizeof(hex), "$%02X", (w >> 16) & 0xFF);
+  return m.substr(0, m.size() - 1) + hex;
+}
+
+std::string disasm_seq(const uint32_t *w, int n) {
+  std::string s;
+  for (int i = 0; i < n; i++) {
+    if (i) s += " : ";
+    s += disasm(w[i]);
+  }
+  return s;
+}
+
+// main.go:285-299
+bool parse_dead_flags(const std::string &in, uint32_t *out) {
+  std::string s = in;
+  for (auto &c : s) c = (char)tolower((unsigned char)c);
+  if (s == "none" || s.empty()) return *out = 0, true;
+  if (s == "all") return *out = 0xFF, true;
+  if (s.rfind("0x", 0) == 0) s = s.substr(2);
+  if (s.empty() || s.size() > 2) return false;
+  char *end;
+  unsigned long v = strtoul(s.c_str(), &end, 16);
+  if (*end) return false;
+  *out = (uint32_t)v;
+  return true;
+}
+
+// main.go:468-494
+bool parse_immediate(std::string s, int *out) {
+  s = trim(s);
+  if (s.empty()) return false;
+  int base = 10;
+  if (s[0] == '$') s = s.substr(1), base = 16;
+  else if (s.rfind("0X", 0) == 0 || s.rfind("0x", 0) == 0) s = s.substr(2), base = 16;
+  else if (s.back() == 'H') s.pop_back(), base = 16;
+  if (s.empty()) return false;
+  char *end;
+  unsigned long v = strtoul(s.c_str(), &end, base);
+  if (*end || v > 0xFFFF) return false;
+  *out = (int)v;
+  return true;
+}
+
+// main.go:343-384
+bool parse_single(const std::string &text_in, uint32_t *out) {
+  std::string text = trim(text_in), up = upper(text);
+  for (int op = 0; op < 58; op++) {
+    std::string m = kMnem[op];
+    if (!has_imm(op)) {
+      if (upper(m) == up) return *out = (uint32_t)op, true;
+      continue;
+    }
+    std::string pat = upper(m);
+    size_t n = pat.rfind('N');
+    std::string prefix = pat.substr(0, n), suffix = pat.substr(n + 1);
+    if (up.rfind(prefix, 0) != 0) continue;
+    if (!suffix.empty() && (up.size() < suffix.size() || up.compare(up.size() - suffix.size(), suffix.size(), suffix)))
+      continue;
+    std::string val = up.substr(prefix.size());
+    if (!suffix.empty()) val = val.substr(0, val.size() - suffix.size());
+    int v;
+    if (!parse_immediate(val, &v)) continue;
+    *out = (uint32_t)op | ((uint32_t)v << 16);
+    return true;
+  }
+  return false;
+}
+
+// main.go:302-323
+bool parse_assembly(const std::string &text, std::vector<uint32_t> *seq, std::string *err) {
+  seq->clear();
+  std::stringstream ss(text);
+  std::string part;
+  while (std::getline(ss, part, ':')) {
+    part = trim(part);
+    if (part.empty()) continue;
+    uint32_t w;
+    if (!parse_single(part, &w)) {
+      *err = "cannot parse \"" + part + "\": unknown instruction: " + part;
+      return false;
+    }
+    seq->push_back(w);
+  }
+  if (seq->empty()) {
+    *err = "no instructions parsed from \"" + text + "\"";
+    return false;
+  }
+  return true;
+}
+
+// pkg/result/output.go:77-101
+std::string dead_flag_desc(int mask) {
+  if (mask == 0) return "";
+  if (mask == 0xFF) return "all";
+  const struct { int bit; const char *n; } f[] = {{1, "C"}, {2, "Z"}, {4, "I"}, {8, "D"}, {0x40, "V"}, {0x80, "N"}};
+  std::string s;
+  for (auto &x : f)
+    if (mask & x.bit) s += (s.empty() ? "" : ",") + std::string(x.n);
+  return s;
+}
+
+std::string json_str(const std::string &s) { return "\"" + s + "\""; }
+
+// pkg/result/output.go:21-35 with table.go:38-51 ordering.
+void write_json(FILE *f, std::vector<sx_rule> rules) {
+  std::stable_sort(rules.begin(), rules.end(), [](const sx_rule &a, const sx_rule &b) {
+    if (a.bytes_saved != b.bytes_saved) return a.bytes_saved > b.bytes_saved;
+    return a.cycles_saved > b.cycles_saved;
+  });
+  if (rules.empty()) {
+    fputs("[]\n", f);
+    return;
+  }
+  fputs("[\n", f);
+  for (size_t i = 0; i < rules.size(); i++) {
+    const sx_rule &r = rules[i];
+    fputs("  {\n    \"source\": [\n", f);
+    for (int k = 0; k < r.src_len; k++)
+      fprintf(f, "      %s%s\n", json_str(disasm(r.src[k])).c_str(), k + 1 < r.src_len ? "," : "");
+    fputs("    ],\n    \"replacement\": [\n", f);
+    for (int k = 0; k < r.rep_len; k++)
+      fprintf(f, "      %s%s\n", json_str(disasm(r.rep[k])).c_str(), k + 1 < r.rep_len ? "," : "");
+    fprintf(f, "    ],\n    \"bytes_saved\": %d,\n    \"cycles_saved\": %d", r.bytes_saved, r.cycles_saved);
+    std::string d = dead_flag_desc(r.dead_flags);
+    if (!d.empty()) fprintf(f, ",\n    \"dead_flags\": \"%s\"", d.c_str());
+    fprintf(f, "\n  }%s\n", i + 1 < rules.size() ? "," : "");
+  }
+  fputs("]\n", f);
+}
+
+// cuda/6502_search_v2.cu:912-930 schema, written through one buffer.
+void write_jsonl(FILE *f, const std::vector<sx_rule> &rules, int model) {
+  std::string buf;
+  buf.reserve(rules.size() * 128);
+  char line[512];
+  for (const sx_rule &r : rules) {
+    int sb = 0, rb = 0;
+    for (int k = 0; k < r.src_len; k++) sb += byte_size(r.src[k] & 0xFFFF, model);
+    for (int k = 0; k < r.rep_len; k++) rb += byte_size(r.rep[k] & 0xFFFF, model);
+    int n = snprintf(line, sizeof(line),
+                     "{\"source_asm\":\"%s\",\"replacement_asm\":\"%s\",\"source_bytes\":%d,"
+                     "\"replacement_bytes\":%d,\"bytes_saved\":%d,\"cycles_saved\":%d",
+                     disasm_seq(r.src, r.src_len).c_str(), disasm_seq(r.rep, r.rep_len).c_str(), sb, rb,
+                     r.bytes_saved, r.cycles_saved);
+    buf.append(line, n);
+    if (r.dead_flags) {
+      n = snprintf(line, sizeof(line), ",\"dead_flags\":\"0x%02X\"", r.dead_flags);
+      buf.append(line, n);
+    }
+    buf += "}\n";
+  }
+  fwrite(buf.data(), 1, buf.size(), f);
+}
+
+struct Args {
+  std::vector<std::string> pos;
+  std::vector<std::pair<std::string, std::string>> kv;
+  bool has(const std::string &k) const {
+    for (auto &p : kv)
+      if (p.first == k) return true;
+    return false;
+  }
+  std::string get(const std::string &k, const std::string &def) const {
+    for (auto &p : kv)
+      if (p.first == k) return p.second;
+    return def;
+  }
+};
+
+Args parse_args(int argc, char **argv, int from, const std::vector<std::string> &bool_flags) {
+  Args a;
+  for (int i = from; i < argc; i++) {
+    std::string s = argv[i];
+    if (s.rfind("-", 0) == 0 && s.size() > 1) {
+      std::string k = s.substr(s.find_first_not_of('-'));
+      std::string v;
+      size_t eq = k.find('=');
+      if (eq != std::string::npos) {
+        v = k.substr(eq + 1);
+        k = k.substr(0, eq);
+      } else if (std::find(bool_flags.begin(), bool_flags.end(), k) != bool_flags.end()) {
+        v = "true";
+      } else if (i + 1 < argc) {
+        v = argv[++i];
+      }
+      if (k == "v") k = "verbose";
+      a.kv.push_back({k, v});
+    } else {
+      a.pos.push_back(s);
+    }
+  }
+  return a;
+}
+
+int die(const std::string &msg) {
+  fprintf(stderr, "Error: %s\n", msg.c_str());
+  return 1;
+}
+
+int check(sx_handle *h, int rc, const char *what) {
+  if (rc == SX_OK) return 0;
+  char buf[512] = "";
+  if (h) sx_last_error(h, buf, sizeof(buf));
+  fprintf(stderr, "Error: %s failed (%d): %s\n", what, rc, buf);
+  return 1;
+}
+
+int cmd_enumerate(int argc, char **argv) {
+  Args a = parse_args(argc, argv, 2, {"verbose", "no-mid"});
+  int max_target = std::atoi(a.get("max-target", "2").c_str());
+  uint32_t dead;
+  if (!parse_dead_flags(a.get("dead-flags", "none"), &dead))
+    return die("invalid --dead-flags value: use none, all, or hex (e.g. 0xFF)");
+  int model = a.get("bytes-model", "go") == "cuda" ? SX_BYTES_CUDA : SX_BYTES_GO;
+  int hits = a.get("hits", "first") == "all" ? SX_HITS_ALL : SX_HITS_FIRST;
+  int exh = a.get("exh", "projected") == "full" ? SX_EXH_FULL : SX_EXH_PROJECTED;
+  int device = std::atoi(a.get("device", "0").c_str());
+  if (max_target != 2) return die("this build enumerates length-2 targets (--max-target 2)");
+  printf("6502 Superoptimizer\n");
+  printf("  Max target length: %d\n", max_target);
+  printf("  Instructions: %d per position\n", 2863);
+  printf("  Device: %d (MI355X search core)\n", device);
+  if (dead) printf("  Dead flags: 0x%02X (%s)\n", dead, dead_flag_desc((int)dead).c_str());
+  printf("\n");
+  sx_handle *h = nullptr;
+  if (check(nullptr, sx_open(device, &h), "sx_open")) return 1;
+  sx_enum_cfg cfg;
+  memset(&cfg, 0, sizeof(cfg));
+  cfg.struct_size = sizeof(cfg);
+  cfg.target_len = 2;
+  cfg.max_cand_len = 1;
+  cfg.bytes_model = model;
+  cfg.hits = hits;
+  cfg.exh_mode = exh;
+  cfg.use_mid = a.has("no-mid") ? 0 : 1;
+  cfg.dead_flags = dead;
+  cfg.first_target = strtoull(a.get("first-target", "0").c_str(), nullptr, 10);
+  cfg.end_target = strtoull(a.get("end-target", "0").c_str(), nullptr, 10);
+  uint64_t n = 0;
+  sx_stats st;
+  auto t0 = std::chrono::steady_clock::now();
+  if (check(h, sx_enumerate(h, &cfg, nullptr, 0, &n, &st), "sx_enumerate")) return 1;
+  std::vector<sx_rule> rules(n);
+  uint64_t got = 0;
+  if (check(h, sx_fetch_rules(h, rules.data(), 0, n, &got), "sx_fetch_rules")) return 1;
+  double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  printf("  %llu targets | %llu QC pairs | QC hits %llu | Mid %llu | Exh jobs %llu | %.2f s (%.3g pairs/s)\n",
+         (unsigned long long)st.targets, (unsigned long long)st.qc_pairs, (unsigned long long)st.qc_hits,
+         (unsigned long long)st.mid_pass, (unsigned long long)st.exh_pairs, secs, st.qc_pairs / secs);
+  if (a.has("verbose"))
+    for (auto &r : rules)
+      printf("  FOUND%s: %s -> %s (-%d bytes, -%d cycles)\n", r.pass == 2 ? " (dead flags)" : "",
+             disasm_seq(r.src, r.src_len).c_str(), disasm_seq(r.rep, r.rep_len).c_str(), r.bytes_saved,
+             r.cycles_saved);
+  printf("\nFound %llu optimizations\n", (unsigned long long)n);
+  std::string out = a.get("output", ""), jl = a.get("jsonl", "");
+  if (!out.empty()) {
+    FILE *f = fopen(out.c_str(), "w");
+    if (!f) return die("cannot create " + out);
+    write_json(f, rules);
+    fclose(f);
+    printf("Written to %s\n", out.c_str());
+  }
+  if (!jl.empty()) {
+    FILE *f = jl == "-" ? stdout : fopen(jl.c_str(), "w");
+    if (!f) return die("cannot create " + jl);
+    write_jsonl(f, rules, model);
+    if (f != stdout) fclose(f);
+  }
+  sx_close(h);
+  return 0;
+}
+
+int cmd_target(int argc, char **argv) {
+  Args a = parse_args(argc, argv, 2, {"verbose"});
+  std::string input;
+  for (auto &p : a.pos) input += (input.empty() ? "" : " ") + p;
+  if (input.empty()) return die("requires at least 1 arg(s)");
+  uint32_t dead;
+  if (!parse_dead_flags(a.get("dead-flags", "none"), &dead)) return die("invalid --dead-flags value");
+  std::vector<uint32_t> seq;
+  std::string err;
+  if (!parse_assembly(input, &seq, &err)) return die("failed to parse: " + err);
+  int tb = 0, tc = 0;
+  for (uint32_t w : seq) tb += byte_size(w & 0xFFFF, 0), tc += cycles(w & 0xFFFF);
+  printf("Target: %s (%d bytes, %d cycles)\n", input.c_str(), tb, tc);
+  int max_cand = std::atoi(a.get("max-candidate", "4").c_str());
+  if (seq.size() < 2 || seq.size() > 3) return die("this build searches targets of length 2 or 3");
+  int ml = std::min<int>(max_cand, (int)seq.size() - 1);
+  sx_handle *h = nullptr;
+  if (check(nullptr, sx_open(std::atoi(a.get("device", "0").c_str()), &h), "sx_open")) return 1;
+  sx_enum_cfg cfg;
+  memset(&cfg, 0, sizeof(cfg));
+  cfg.struct_size = sizeof(cfg);
+  cfg.target_len = (int)seq.size();
+  cfg.max_cand_len = ml;
+  cfg.use_mid = 1;
+  cfg.dead_flags = dead;
+  cfg.targets = seq.data();
+  cfg.n_targets = 1;
+  uint64_t n = 0;
+  sx_stats st;
+  if (check(h, sx_enumerate(h, &cfg, nullptr, 0, &n, &st), "sx_enumerate")) return 1;
+  std::vector<sx_rule> rules(n);
+  uint64_t got;
+  sx_fetch_rules(h, rules.data(), 0, n, &got);
+  sx_close(h);
+  if (rules.empty()) {
+    printf("No shorter replacement found.\n");
+    return 0;
+  }
+  std::stable_sort(rules.begin(), rules.end(), [](const sx_rule &x, const sx_rule &y) {
+    if (x.bytes_saved != y.bytes_saved) return x.bytes_saved > y.bytes_saved;
+    return x.cycles_saved > y.cycles_saved;
+  });
+  const sx_rule &r = rules[0];
+  printf("Replacement: %s (-%d bytes, -%d cycles)\n", disasm_seq(r.rep, r.rep_len).c_str(), r.bytes_saved,
+         r.cycles_saved);
+  return 0;
+}
+
+// Extract a JSON string field from one flat JSONL object.
+bool json_field(const std::string &line, const std::string &key, std::string *out) {
+  std::string pat = "\"" + key + "\"";
+  size_t p = line.find(pat);
+  if (p == std::string::npos) return false;
+  p = line.find(':', p + pat.size());
+  if (p == std::string::npos) return false;
+  p = line.find('"', p);
+  if (p == std::string::npos) return false;
+  std::string v;
+  for (size_t i = p + 1; i < line.size(); i++) {
+    if (line[i] == '\\' && i + 1 < line.size()) {
+      v += line[++i];
+      continue;
+    }
+    if (line[i] == '"') {
+      *out = v;
+      return true;
+    }
+    v += line[i];
+  }
+  return false;
+}
+
+// main.go:386-463, with the checks batched into one device call.
+int cmd_verify_jsonl(int argc, char **argv) {
+  Args a = parse_args(argc, argv, 2, {"verbose"});
+  if (a.pos.size() != 1) return die("accepts 1 arg(s)");
+  uint32_t dead;
+  if (!parse_dead_flags(a.get("dead-flags", "none"), &dead)) return die("invalid --dead-flags value");
+  std::ifstream in(a.pos[0]);
+  if (!in) return die("open " + a.pos[0] + ": no such file or directory");
+  std::vector<uint32_t> tw, cw;
+  std::vector<uint8_t> tl, cl;
+  std::vector<std::string> desc;
+  std::vector<int> lineno;
+  std::string line;
+  int total = 0, skipped = 0;
+  while (std::getline(in, line)) {
+    line = trim(line);
+    if (line.empty()) continue;
+    total++;
+    std::string s, r, err;
+    if (!json_field(line, "source_asm", &s) || !json_field(line, "replacement_asm", &r)) {
+      fprintf(stderr, "  [%d] JSON parse error\n", total);
+      skipped++;
+      continue;
+    }
+    std::vector<uint32_t> src, rep;
+    if (!parse_assembly(s, &src, &err)) {
+      fprintf(stderr, "  [%d] Cannot parse source \"%s\": %s\n", total, s.c_str(), err.c_str());
+      skipped++;
+      continue;
+    }
+    if (!parse_assembly(r, &rep, &err)) {
+      fprintf(stderr, "  [%d] Cannot parse replacement \"%s\": %s\n", total, r.c_str(), err.c_str());
+      skipped++;
+      continue;
+    }
+    if (src.size() > 4 || rep.size() > 4) {
+      fprintf(stderr, "  [%d] sequence longer than 4 instructions\n", total);
+      skipped++;
+      continue;
+    }
+    for (int k = 0; k < 4; k++) {
+      tw.push_back(k < (int)src.size() ? src[k] : 0);
+      cw.push_back(k < (int)rep.size() ? rep[k] : 0);
+    }
+    tl.push_back((uint8_t)src.size());
+    cl.push_back((uint8_t)rep.size());
+    desc.push_back(s + " -> " + r);
+    lineno.push_back(total);
+  }
+  std::vector<uint8_t> ok(tl.size(), 0);
+  if (!tl.empty()) {
+    sx_handle *h = nullptr;
+    if (check(nullptr, sx_open(std::atoi(a.get("device", "0").c_str()), &h), "sx_open")) return 1;
+    if (check(h, sx_check_pairs(h, tw.data(), tl.data(), cw.data(), cl.data(), tl.size(), SX_STAGE_EXH, dead,
+                                SX_EXH_PROJECTED, ok.data()),
+              "sx_check_pairs"))
+      return 1;
+    sx_close(h);
+  }
+  int passed = 0, failed = 0;
+  for (size_t i = 0; i < tl.size(); i++) {
+    if (ok[i] & SX_STAGE_EXH) {
+      passed++;
+      if (a.has("verbose")) printf("  [%d] PASS: %s\n", lineno[i], desc[i].c_str());
+    } else {
+      failed++;
+      printf("  [%d] FAIL: %s\n", lineno[i], desc[i].c_str());
+    }
+  }
+  printf("\nVerification complete: %d total, %d passed, %d failed, %d skipped\n", total, passed, failed, skipped);
+  if (failed) {
+    fprintf(stderr, "Error: %d rules failed verification\n", failed);
+    return 1;
+  }
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc < 2) {
+    fprintf(stderr, "usage: 6502opt enumerate|target|verify-jsonl [flags]\n");
+    return 1;
+  }
+  std::string cmd = argv[1];
+  if (cmd == "enumerate") return cmd_enumerate(argc, argv);
+  if (cmd == "target") return cmd_target(argc, argv);
+  if (cmd == "verify-jsonl") return cmd_verify_jsonl(argc, argv);
+  fprintf(stderr, "Error: unknown command \"%s\" for \"6502opt\"\n", cmd.c_str());
+  return 1;
+}

@@ -27,18 +27,21 @@ import oracle as o  # noqa: E402
 import sampling  # noqa: E402
 
 
-def main():
+def main(only=None):
     l2 = o.enumerate_pruned(2)
     assert len(l2) == 7_633_612
     t, idx = sampling.c1_targets(l2)
     np.save(os.path.join(HERE, "c1_targets.npy"), t.astype(np.uint32))
     np.save(os.path.join(HERE, "c1_indices.npy"), idx.astype(np.int64))
-    counts = {}
+    cpath = os.path.join(HERE, "c1_counts.json")
+    counts = json.load(open(cpath)) if os.path.exists(cpath) else {}
     runs = [("c1_go_dead00", dict(dead=0x00)), ("c1_go_deadFF", dict(dead=0xFF)),
             ("c1_go_dead01", dict(dead=0x01)),
             ("c1_cuda_dead00", dict(dead=0x00, hits=1, bytes_model=1)),
             ("c1_cuda_deadFF", dict(dead=0xFF, hits=1, bytes_model=1))]
     for name, kw in runs:
+        if only and name not in only:
+            continue
         t0 = time.time()
         rules, st = o.search(t, **kw)
         np.save(os.path.join(HERE, name + ".npy"), rules)
@@ -46,9 +49,9 @@ def main():
                         "mid_pass": st.mid_pass, "exh_calls": st.exh_calls,
                         "seconds": round(time.time() - t0, 2)}
         print(name, counts[name], flush=True)
-    with open(os.path.join(HERE, "c1_counts.json"), "w") as f:
-        json.dump(counts, f, indent=1, sort_keys=True)
+        with open(cpath, "w") as f:
+            json.dump(counts, f, indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or None)
