@@ -72,14 +72,17 @@ def main():
         sys.exit(2)
 
     dev = core.Device(0)
-    for _ in range(args.warmup):
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
         dev.enumerate(None, dead=args.dead)
+        print(f"bench: warmup {i} {time.perf_counter() - t0:.3f} s", file=sys.stderr, flush=True)
     times, stats = [], []
-    for _ in range(args.steps):
+    for i in range(args.steps):
         t0 = time.perf_counter()
         rules, st = dev.enumerate(None, dead=args.dead)
         times.append(time.perf_counter() - t0)
         stats.append(st)
+        print(f"bench: step {i} {times[-1]:.3f} s, {st['rules']} rules", file=sys.stderr, flush=True)
     ms = 1e3 * sum(times) / len(times)
     st = stats[-1]
     qc_ms = sum(s["ms_qc"] for s in stats) / len(stats)

@@ -152,12 +152,8 @@ def test_projection_equals_full_sweep_k3(dev):
             assert bool(a) == o.exhaustive_check(t, c, dead, mode=1)
 
 
-QUARANTINE = pytest.mark.skip(reason="a run of this case went silent for 180 s on the GPU; cause not yet "
-                                      "shown, kept off the GPU (see DESIGN.md, open issues)")
-
-
 @pytest.mark.parametrize("name,kw", [
-    pytest.param("c1_go_dead00", dict(dead=0x00), marks=QUARANTINE),
+    ("c1_go_dead00", dict(dead=0x00)),
     ("c1_go_deadFF", dict(dead=0xFF)),
     ("c1_go_dead01", dict(dead=0x01)),
     ("c1_cuda_dead00", dict(dead=0x00, hits=core.HITS_ALL, bytes_model=isa.BYTES_CUDA)),
@@ -175,7 +171,7 @@ def test_c1_rule_sets(dev, name, kw):
         assert st["mid_pass"] == counts["mid_pass"]
 
 
-@pytest.mark.parametrize("dead", [pytest.param(0x00, marks=QUARANTINE), 0xFF])
+@pytest.mark.parametrize("dead", [0x00, 0xFF])
 def test_c1_full_sweep_and_no_mid_agree(dev, dead):
     """MidCheck is only a filter and the projection only a reduction: the
     rule set is the same without either."""
