@@ -41,18 +41,19 @@ using namespace sx;
 
 __constant__ uint8_t c_reads[OP_COUNT] = SX_READS_TABLE;
 __constant__ uint8_t c_writes[OP_COUNT] = SX_WRITES_TABLE;
-__constant__ uint8_t c_dep[OP_COUNT][8] = SX_DEP_TABLE;
+__constant__ DepTable c_dep = make_dep_table();
 __constant__ uint8_t c_vec[32][8] = SX_MID_VECTORS;
 
 static const uint8_t h_reads[OP_COUNT] = SX_READS_TABLE;
 static const uint8_t h_writes[OP_COUNT] = SX_WRITES_TABLE;
-static const uint8_t h_dep[OP_COUNT][8] = SX_DEP_TABLE;
+static const DepTable h_dep = make_dep_table();
 static const uint8_t h_vec[32][8] = SX_MID_VECTORS;
 
 constexpr int kWave = 64;
 constexpr int kChunkLog = 14;           // states per exhaustive chunk
 constexpr uint32_t kRankNone = 0xFFFFFFFFu;
 constexpr int kQcQueue = 256;           // LDS survivor queue entries per wave
+constexpr int kEvalShards = 1024;       // shards of the evaluated-states counter
 
 // Go instruction order (pkg/search/enumerator.go:14-38): the 47 non-immediate
 // opcodes in id order, then the 11 immediate opcodes x 256.
@@ -499,7 +500,8 @@ __global__ __launch_bounds__(256) void k_exh_plan(Job *jobs, uint32_t nj, int mo
 // the job's fail flag, which the job's other waves poll to stop early.
 __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, uint32_t nj,
                                                  const uint64_t *__restrict__ offs, uint64_t chunk0,
-                                                 uint64_t nchunks, uint32_t *fail) {
+                                                 uint64_t nchunks, uint32_t *fail,
+                                                 unsigned long long *evaluated) {
   const uint32_t lane = lane_id();
   const uint64_t rel = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (rel >= nchunks) return;  // whole wave
@@ -545,7 +547,10 @@ __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, u
   const uint32_t pm = uni(jp->pmask);
   const uint64_t start = local << kChunkLog;
   const uint32_t count = lg < kChunkLog ? (1u << lg) : (1u << kChunkLog);
+  // states this wave evaluates, added to a sharded counter when it leaves
+  uint32_t done_states = 0;
   for (uint32_t it = 0; it < count; it += kWave) {
+    done_states = it + kWave < count ? it + kWave : count;
     const uint32_t k = it + lane;
     bool bad = false;
     if (k < count) {
@@ -568,10 +573,12 @@ __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, u
     }
     if (__ballot(bad)) {
       if (lane == 0) ((volatile uint32_t *)fail)[ji] = 1u;
-      return;
+      break;
     }
-    if ((it & (32 * kWave - 1)) == 32 * kWave - kWave && ((volatile uint32_t *)fail)[ji]) return;
+    if ((it & (32 * kWave - 1)) == 32 * kWave - kWave && ((volatile uint32_t *)fail)[ji]) break;
   }
+  if (lane == 0) atomicAdd(&evaluated[(blockIdx.x * 4 + (threadIdx.x >> 6)) & (kEvalShards - 1)],
+                           (unsigned long long)done_states);
 }
 
 // Resolve a round: a passing job emits its rule and closes (target, pass); a
@@ -873,7 +880,7 @@ struct Timer {
 
 // Run the exhaustive stage on nj jobs already in B_JOBS; the verdicts are in
 // B_FAIL (0 = pass).  *chunks_out gets the number of chunks (statistics).
-int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out) {
+int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, uint64_t *states_out) {
   Job *jobs = (Job *)h->bufs[B_JOBS].p;
   uint32_t *chunks, *fail;
   uint64_t *offs, *total;
@@ -881,6 +888,9 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out) {
   ENSURE(B_OFFS, uint64_t, nj + 1, offs);
   ENSURE(B_FAIL, uint32_t, nj + 1, fail);
   ENSURE(B_TOTAL, uint64_t, 1, total);
+  unsigned long long *evald;
+  ENSURE(B_CURSOR, unsigned long long, kEvalShards, evald);
+  HIPCHK(h, hipMemsetAsync(evald, 0, kEvalShards * sizeof(unsigned long long), h->stream));
   hipLaunchKernelGGL(k_exh_plan, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, jobs, nj, mode,
                      chunks, fail);
   HIPCHK(h, hipGetLastError());
@@ -896,12 +906,22 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out) {
   for (uint64_t c0 = 0; c0 < tot; c0 += slice) {
     uint64_t n = std::min<uint64_t>(slice, tot - c0);
     hipLaunchKernelGGL(k_exh_run, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, h->stream, jobs, nj, offs, c0, n,
-                       fail);
+                       fail, evald);
     HIPCHK(h, hipGetLastError());
     if (c0 + slice < tot) {
       rc = sync_wd(h, "exhaustive sweep");
       if (rc) return rc;
     }
+  }
+  if (states_out) {
+    std::vector<unsigned long long> shards(kEvalShards);
+    HIPCHK(h, hipMemcpyAsync(shards.data(), evald, kEvalShards * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, h->stream));
+    rc = sync_wd(h, "exhaustive sweep");
+    if (rc) return rc;
+    uint64_t sum = 0;
+    for (auto v : shards) sum += v;
+    *states_out = sum;
   }
   return SX_OK;
 }
@@ -1164,10 +1184,10 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       trace(h, "  round %d: %u jobs", round, nj);
       st.exh_rounds++;
       st.exh_pairs += nj;
-      uint64_t chunks = 0;
-      int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks);
+      uint64_t chunks = 0, evaluated = 0;
+      int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, &evaluated);
       if (rc) return rc;
-      st.exh_states += chunks << kChunkLog;
+      st.exh_states += evaluated;
       hipLaunchKernelGGL(k_round_resolve, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream,
                          (const Job *)h->bufs[B_JOBS].p, nj, (const uint32_t *)h->bufs[B_FAIL].p, rank, cur, rej,
                          done, tfp, cfp, tbytes, tcyc, cbytes, ccyc, cfg->hits, cfg->dead_flags, tbase + s0,
@@ -1303,7 +1323,7 @@ extern "C" int sx_check_pairs(sx_handle *h, const uint32_t *t_words, const uint8
       ENSURE(B_JOBS, Job, nb, dj);
       HIPCHK(h, hipMemcpyAsync(dj, jobs.data(), nb * sizeof(Job), hipMemcpyHostToDevice, h->stream));
       uint64_t chunks = 0;
-      int rc = run_exhaustive(h, nb, exh_mode, &chunks);
+      int rc = run_exhaustive(h, nb, exh_mode, &chunks, nullptr);
       if (rc) return rc;
       std::vector<uint32_t> f(nb);
       HIPCHK(h, hipMemcpyAsync(f.data(), h->bufs[B_FAIL].p, nb * 4, hipMemcpyDeviceToHost, h->stream));
@@ -1395,8 +1415,8 @@ extern "C" int sx_exh_plan(const uint32_t *t, int tn, const uint32_t *c, int cn,
   return SX_OK;
 }
 
-extern "C" int sx_dep_table(uint8_t out[58 * 8]) {
+extern "C" int sx_dep_table(uint16_t out[58 * 16]) {
   if (!out) return SX_EINVAL;
-  memcpy(out, h_dep, sizeof(h_dep));
+  memcpy(out, h_dep.d, sizeof(h_dep.d));
   return SX_OK;
 }

@@ -10,6 +10,8 @@
 
 #include <stdint.h>
 
+#include <initializer_list>
+
 #ifdef __HIPCC__
 #define SX_HD __host__ __device__ __forceinline__
 #else
@@ -78,74 +80,98 @@ SX_HD int cycles(uint32_t op) {
    GA | GP, GA | GP, GP, GP, GP, GM | GP, GM | GP, GM | GP, GM | GP, GM | GP,    \
    GM | GP, GP}
 
-// Data dependencies of each opcode, per output register, over the state byte
-// positions (bit r = register r of the input).  Entry [op][r] is the set of
-// input registers that output register r is a function of; registers an op
-// does not touch depend on themselves.  Derived from pkg/cpu/exec.go:8-327
-// and checked for soundness by brute force in the tests.  0xFF marks "the
-// identity" (the register is untouched).
-#define B(r) (1u << (r))
-#define ID 0xFFu
-#define DEP_ROW(a, x, y, p, s, m, s0, s1) {a, x, y, p, s, m, s0, s1}
-#define SX_DEP_TABLE                                                                      \
-  {/*TAX*/ DEP_ROW(ID, B(RA), ID, B(RP) | B(RA), ID, ID, ID, ID),                           \
-   /*TAY*/ DEP_ROW(ID, ID, B(RA), B(RP) | B(RA), ID, ID, ID, ID),                           \
-   /*TXA*/ DEP_ROW(B(RX), ID, ID, B(RP) | B(RX), ID, ID, ID, ID),                           \
-   /*TYA*/ DEP_ROW(B(RY), ID, ID, B(RP) | B(RY), ID, ID, ID, ID),                           \
-   /*TSX*/ DEP_ROW(ID, B(RS), ID, B(RP) | B(RS), ID, ID, ID, ID),                           \
-   /*TXS*/ DEP_ROW(ID, ID, ID, ID, B(RX), ID, ID, ID),                                      \
-   /*INX*/ DEP_ROW(ID, B(RX), ID, B(RP) | B(RX), ID, ID, ID, ID),                           \
-   /*INY*/ DEP_ROW(ID, ID, B(RY), B(RP) | B(RY), ID, ID, ID, ID),                           \
-   /*DEX*/ DEP_ROW(ID, B(RX), ID, B(RP) | B(RX), ID, ID, ID, ID),                           \
-   /*DEY*/ DEP_ROW(ID, ID, B(RY), B(RP) | B(RY), ID, ID, ID, ID),                           \
-   /*ASL_A*/ DEP_ROW(B(RA), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                         \
-   /*LSR_A*/ DEP_ROW(B(RA), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                         \
-   /*ROL_A*/ DEP_ROW(B(RA) | B(RP), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                 \
-   /*ROR_A*/ DEP_ROW(B(RA) | B(RP), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                 \
-   /*CLC*/ DEP_ROW(ID, ID, ID, B(RP), ID, ID, ID, ID),                                      \
-   /*SEC*/ DEP_ROW(ID, ID, ID, B(RP), ID, ID, ID, ID),                                      \
-   /*CLD*/ DEP_ROW(ID, ID, ID, B(RP), ID, ID, ID, ID),                                      \
-   /*SED*/ DEP_ROW(ID, ID, ID, B(RP), ID, ID, ID, ID),                                      \
-   /*CLI*/ DEP_ROW(ID, ID, ID, B(RP), ID, ID, ID, ID),                                      \
-   /*SEI*/ DEP_ROW(ID, ID, ID, B(RP), ID, ID, ID, ID),                                      \
-   /*CLV*/ DEP_ROW(ID, ID, ID, B(RP), ID, ID, ID, ID),                                      \
-   /*PHA*/ DEP_ROW(ID, ID, ID, ID, B(RS), ID, B(RA), B(RS0)),                               \
-   /*PLA*/ DEP_ROW(B(RS0), ID, ID, B(RP) | B(RS0), B(RS), ID, B(RS1), 0),                   \
-   /*PHP*/ DEP_ROW(ID, ID, ID, ID, B(RS), ID, B(RP), B(RS0)),                               \
-   /*PLP*/ DEP_ROW(ID, ID, ID, B(RS0), B(RS), ID, B(RS1), 0),                               \
-   /*NOP*/ DEP_ROW(ID, ID, ID, ID, ID, ID, ID, ID),                                         \
-   /*LDA#*/ DEP_ROW(0, ID, ID, B(RP), ID, ID, ID, ID),                                      \
-   /*LDX#*/ DEP_ROW(ID, 0, ID, B(RP), ID, ID, ID, ID),                                      \
-   /*LDY#*/ DEP_ROW(ID, ID, 0, B(RP), ID, ID, ID, ID),                                      \
-   /*ADC#*/ DEP_ROW(B(RA) | B(RP), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                  \
-   /*SBC#*/ DEP_ROW(B(RA) | B(RP), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                  \
-   /*AND#*/ DEP_ROW(B(RA), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                          \
-   /*ORA#*/ DEP_ROW(B(RA), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                          \
-   /*EOR#*/ DEP_ROW(B(RA), ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                          \
-   /*CMP#*/ DEP_ROW(ID, ID, ID, B(RP) | B(RA), ID, ID, ID, ID),                              \
-   /*CPX#*/ DEP_ROW(ID, ID, ID, B(RP) | B(RX), ID, ID, ID, ID),                              \
-   /*CPY#*/ DEP_ROW(ID, ID, ID, B(RP) | B(RY), ID, ID, ID, ID),                              \
-   /*LDA M*/ DEP_ROW(B(RM), ID, ID, B(RP) | B(RM), ID, ID, ID, ID),                         \
-   /*LDX M*/ DEP_ROW(ID, B(RM), ID, B(RP) | B(RM), ID, ID, ID, ID),                         \
-   /*LDY M*/ DEP_ROW(ID, ID, B(RM), B(RP) | B(RM), ID, ID, ID, ID),                         \
-   /*STA M*/ DEP_ROW(ID, ID, ID, ID, ID, B(RA), ID, ID),                                    \
-   /*STX M*/ DEP_ROW(ID, ID, ID, ID, ID, B(RX), ID, ID),                                    \
-   /*STY M*/ DEP_ROW(ID, ID, ID, ID, ID, B(RY), ID, ID),                                    \
-   /*ADC M*/ DEP_ROW(B(RA) | B(RM) | B(RP), ID, ID, B(RP) | B(RA) | B(RM), ID, ID, ID, ID), \
-   /*SBC M*/ DEP_ROW(B(RA) | B(RM) | B(RP), ID, ID, B(RP) | B(RA) | B(RM), ID, ID, ID, ID), \
-   /*AND M*/ DEP_ROW(B(RA) | B(RM), ID, ID, B(RP) | B(RA) | B(RM), ID, ID, ID, ID),         \
-   /*ORA M*/ DEP_ROW(B(RA) | B(RM), ID, ID, B(RP) | B(RA) | B(RM), ID, ID, ID, ID),         \
-   /*EOR M*/ DEP_ROW(B(RA) | B(RM), ID, ID, B(RP) | B(RA) | B(RM), ID, ID, ID, ID),         \
-   /*CMP M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RA) | B(RM), ID, ID, ID, ID),                    \
-   /*CPX M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RX) | B(RM), ID, ID, ID, ID),                    \
-   /*CPY M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RY) | B(RM), ID, ID, ID, ID),                    \
-   /*INC M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RM), ID, B(RM), ID, ID),                         \
-   /*DEC M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RM), ID, B(RM), ID, ID),                         \
-   /*ASL M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RM), ID, B(RM), ID, ID),                         \
-   /*LSR M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RM), ID, B(RM), ID, ID),                         \
-   /*ROL M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RM), ID, B(RM) | B(RP), ID, ID),                 \
-   /*ROR M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RM), ID, B(RM) | B(RP), ID, ID),                 \
-   /*BIT M*/ DEP_ROW(ID, ID, ID, B(RP) | B(RA) | B(RM), ID, ID, ID, ID)}
+// Data dependencies per opcode at flag granularity.  Inputs and outputs are
+// numbered: state bytes A=0, X=1, Y=2, S=4, M=5, S0=6, S1=7 (position 3, the
+// flag byte, is split) and flag bits 8 + b for P bit b (C=8, Z=9, I=10, D=11,
+// B=12, U=13, V=14, N=15).  Entry [op][o] is the set of inputs that output o
+// is a function of, or kDepId when the op leaves o unchanged.  Derived from
+// pkg/cpu/exec.go:8-327; the tests check it for soundness by brute force.
+constexpr uint16_t kDepId = 0xFFFFu;
+constexpr int kDepOut = 16;
+
+struct DepTable {
+  uint16_t d[OP_COUNT][kDepOut];
+};
+
+constexpr uint16_t DB(int r) { return (uint16_t)(1u << r); }
+constexpr int FC_ = 8, FZ_ = 9, FI_ = 10, FD_ = 11, FV_ = 14, FN_ = 15;
+
+constexpr DepTable make_dep_table() {
+  DepTable t{};
+  for (int op = 0; op < OP_COUNT; op++)
+    for (int o = 0; o < kDepOut; o++) t.d[op][o] = kDepId;
+  auto nz = [&](int op, uint16_t src) {
+    t.d[op][FN_] = src;
+    t.d[op][FZ_] = src;
+  };
+  // transfers
+  t.d[TAX][RX] = DB(RA); nz(TAX, DB(RA));
+  t.d[TAY][RY] = DB(RA); nz(TAY, DB(RA));
+  t.d[TXA][RA] = DB(RX); nz(TXA, DB(RX));
+  t.d[TYA][RA] = DB(RY); nz(TYA, DB(RY));
+  t.d[TSX][RX] = DB(RS); nz(TSX, DB(RS));
+  t.d[TXS][RS] = DB(RX);
+  // inc/dec registers
+  t.d[INX][RX] = DB(RX); nz(INX, DB(RX));
+  t.d[DEX][RX] = DB(RX); nz(DEX, DB(RX));
+  t.d[INY][RY] = DB(RY); nz(INY, DB(RY));
+  t.d[DEY][RY] = DB(RY); nz(DEY, DB(RY));
+  // shifts on A: result from A (and C for rotates); C from A; N,Z from the result
+  t.d[ASL_A][RA] = DB(RA); t.d[ASL_A][FC_] = DB(RA); nz(ASL_A, DB(RA));
+  t.d[LSR_A][RA] = DB(RA); t.d[LSR_A][FC_] = DB(RA); nz(LSR_A, DB(RA));
+  t.d[ROL_A][RA] = DB(RA) | DB(FC_); t.d[ROL_A][FC_] = DB(RA); nz(ROL_A, DB(RA) | DB(FC_));
+  t.d[ROR_A][RA] = DB(RA) | DB(FC_); t.d[ROR_A][FC_] = DB(RA); nz(ROR_A, DB(RA) | DB(FC_));
+  // flag operations set a constant
+  t.d[CLC][FC_] = 0; t.d[SEC][FC_] = 0; t.d[CLD][FD_] = 0; t.d[SED][FD_] = 0;
+  t.d[CLI][FI_] = 0; t.d[SEI][FI_] = 0; t.d[CLV][FV_] = 0;
+  // stack: push moves S0 to S1 and the value to S0; pull the reverse, S1 := 0
+  t.d[PHA][RS] = DB(RS); t.d[PHA][RS0] = DB(RA); t.d[PHA][RS1] = DB(RS0);
+  t.d[PHP][RS] = DB(RS); t.d[PHP][RS1] = DB(RS0);
+  t.d[PHP][RS0] = DB(FC_) | DB(FZ_) | DB(FI_) | DB(FD_) | DB(FV_) | DB(FN_);  // B, U pushed as 1
+  t.d[PLA][RS] = DB(RS); t.d[PLA][RA] = DB(RS0); t.d[PLA][RS0] = DB(RS1); t.d[PLA][RS1] = 0;
+  nz(PLA, DB(RS0));
+  t.d[PLP][RS] = DB(RS); t.d[PLP][RS0] = DB(RS1); t.d[PLP][RS1] = 0;
+  for (int b = 8; b < 16; b++) t.d[PLP][b] = DB(RS0);
+  // immediate loads
+  t.d[LDA_IMM][RA] = 0; nz(LDA_IMM, 0);
+  t.d[LDX_IMM][RX] = 0; nz(LDX_IMM, 0);
+  t.d[LDY_IMM][RY] = 0; nz(LDY_IMM, 0);
+  // ALU with an immediate or with M
+  const uint16_t opnd_imm = 0, opnd_m = DB(RM);
+  for (int k = 0; k < 2; k++) {
+    const uint16_t opnd = k ? opnd_m : opnd_imm;
+    const int adc = k ? ADC_M : ADC_IMM, sbc = k ? SBC_M : SBC_IMM;
+    const int an = k ? AND_M : AND_IMM, ora = k ? ORA_M : ORA_IMM, eor = k ? EOR_M : EOR_IMM;
+    const int cpa = k ? CMP_M : CMP_IMM, cpx = k ? CPX_M : CPX_IMM, cpy = k ? CPY_M : CPY_IMM;
+    for (int op : {adc, sbc}) {
+      const uint16_t src = DB(RA) | opnd | DB(FC_);
+      t.d[op][RA] = src; t.d[op][FC_] = src; t.d[op][FV_] = src; nz(op, src);
+    }
+    for (int op : {an, ora, eor}) {
+      t.d[op][RA] = DB(RA) | opnd; nz(op, DB(RA) | opnd);
+    }
+    t.d[cpa][FC_] = DB(RA) | opnd; nz(cpa, DB(RA) | opnd);
+    t.d[cpx][FC_] = DB(RX) | opnd; nz(cpx, DB(RX) | opnd);
+    t.d[cpy][FC_] = DB(RY) | opnd; nz(cpy, DB(RY) | opnd);
+  }
+  // loads and stores with M
+  t.d[LDA_M][RA] = DB(RM); nz(LDA_M, DB(RM));
+  t.d[LDX_M][RX] = DB(RM); nz(LDX_M, DB(RM));
+  t.d[LDY_M][RY] = DB(RM); nz(LDY_M, DB(RM));
+  t.d[STA_M][RM] = DB(RA);
+  t.d[STX_M][RM] = DB(RX);
+  t.d[STY_M][RM] = DB(RY);
+  // read-modify-write on M
+  t.d[INC_M][RM] = DB(RM); nz(INC_M, DB(RM));
+  t.d[DEC_M][RM] = DB(RM); nz(DEC_M, DB(RM));
+  t.d[ASL_M][RM] = DB(RM); t.d[ASL_M][FC_] = DB(RM); nz(ASL_M, DB(RM));
+  t.d[LSR_M][RM] = DB(RM); t.d[LSR_M][FC_] = DB(RM); nz(LSR_M, DB(RM));
+  t.d[ROL_M][RM] = DB(RM) | DB(FC_); t.d[ROL_M][FC_] = DB(RM); nz(ROL_M, DB(RM) | DB(FC_));
+  t.d[ROR_M][RM] = DB(RM) | DB(FC_); t.d[ROR_M][FC_] = DB(RM); nz(ROR_M, DB(RM) | DB(FC_));
+  // BIT: N and V from M, Z from A & M
+  t.d[BIT_M][FN_] = DB(RM); t.d[BIT_M][FV_] = DB(RM); t.d[BIT_M][FZ_] = DB(RA) | DB(RM);
+  return t;
+}
 
 // QuickCheck and MidCheck vectors, pkg/search/verifier.go:18-67 (A X Y P S M
 // S0 S1).  The first 8 Mid vectors are the QuickCheck vectors.

@@ -34,10 +34,10 @@ SX_HD int set_log_size(uint32_t set) {
   return n;
 }
 
-// reads/writes/dep tables are passed in so that host and device can use
-// their own copies.
+// The reads table and the dependency table are passed in so that host and
+// device use their own copies.
 SX_HD Plan make_plan(const uint32_t *t, int tn, const uint32_t *c, int cn, uint32_t pmask,
-                     int mode, const uint8_t *reads_tab, const uint8_t (*dep_tab)[8]) {
+                     int mode, const uint8_t *reads_tab, const DepTable &dep) {
   uint32_t reads = 0;
   for (int i = 0; i < tn; i++) reads |= reads_tab[t[i] & 0xFFFFu];
   for (int i = 0; i < cn; i++) reads |= reads_tab[c[i] & 0xFFFFu];
@@ -57,43 +57,48 @@ SX_HD Plan make_plan(const uint32_t *t, int tn, const uint32_t *c, int cn, uint3
     pl.logsz[0] = (uint8_t)set_log_size(vary);
     return pl;
   }
-  uint32_t dt[8], dc[8];
-  for (int r = 0; r < 8; r++) dt[r] = dc[r] = 1u << r;
+  // dependency sets in the 16-input numbering of DepTable: only the carry
+  // among the flag bits varies in the domain (the others are 0)
+  const uint32_t vary16 = (vary & ~(1u << RP)) | (1u << 8);
+  uint32_t dt[kDepOut], dc[kDepOut];
+  for (int o = 0; o < kDepOut; o++) dt[o] = dc[o] = 1u << o;
   for (int side = 0; side < 2; side++) {
     const uint32_t *seq = side ? c : t;
-    int n = side ? cn : tn;
+    const int n = side ? cn : tn;
     uint32_t *d = side ? dc : dt;
     for (int i = 0; i < n; i++) {
-      const uint8_t *row = dep_tab[seq[i] & 0xFFFFu];
-      uint32_t nd[8];
-      for (int r = 0; r < 8; r++) {
-        if (row[r] == 0xFF) {
-          nd[r] = d[r];
+      const uint16_t *row = dep.d[seq[i] & 0xFFFFu];
+      uint32_t nd[kDepOut];
+      for (int o = 0; o < kDepOut; o++) {
+        if (row[o] == kDepId) {
+          nd[o] = d[o];
         } else {
           uint32_t acc = 0;
-          for (int q = 0; q < 8; q++)
-            if (row[r] & (1u << q)) acc |= d[q];
-          nd[r] = acc;
+          for (int q = 0; q < kDepOut; q++)
+            if (row[o] & (1u << q)) acc |= d[q];
+          nd[o] = acc;
         }
       }
-      for (int r = 0; r < 8; r++) d[r] = nd[r];
+      for (int o = 0; o < kDepOut; o++) d[o] = nd[o];
     }
   }
-  uint32_t cand[8];
-  int nc = 0;
-  for (int r = 0; r < 8; r++) {
-    if (r == RP && (pmask & kPMask) == 0) continue;
-    cand[nc++] = (dt[r] | dc[r]) & vary;
+  uint32_t cand[16];
+  int ncand = 0;
+  for (int o = 0; o < kDepOut; o++) {
+    if (o == RP) continue;                                     // the flag byte is split
+    if (o >= 8 && !((pmask & kPMask) & (1u << (o - 8)))) continue;  // flag not compared
+    const uint32_t v16 = (dt[o] | dc[o]) & vary16;
+    cand[ncand++] = (v16 & 0xF7u) | (((v16 >> 8) & 1u) << RP);
   }
   // keep maximal sets only (dedupe equal sets, drop strict subsets)
-  for (int i = 0; i < nc; i++) {
+  for (int i = 0; i < ncand; i++) {
     bool keep = true;
-    for (int j = 0; j < nc && keep; j++) {
+    for (int j = 0; j < ncand && keep; j++) {
       if (i == j) continue;
       bool sub = (cand[i] & ~cand[j]) == 0;
       if (sub && (cand[i] != cand[j] || j < i)) keep = false;
     }
-    if (keep) {
+    if (keep && pl.nsets < 8) {
       pl.sets[pl.nsets] = (uint8_t)cand[i];
       pl.logsz[pl.nsets] = (uint8_t)set_log_size(cand[i]);
       pl.nsets++;

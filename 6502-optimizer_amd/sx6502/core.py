@@ -83,7 +83,7 @@ def lib() -> C.CDLL:
     L.sx_exec_states.argtypes = [vp, u8p, u32p, u8p, C.c_uint64]
     L.sx_exh_plan.argtypes = [u32p, C.c_int, u32p, C.c_int, C.c_uint32, C.c_int32, u8p,
                               C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
-    L.sx_dep_table.argtypes = [u8p]
+    L.sx_dep_table.argtypes = [C.POINTER(C.c_uint16)]
     for name in EXPORTS:
         getattr(L, name).restype = C.c_int
     _lib = L
@@ -125,12 +125,14 @@ def exh_plan(t: Sequence[int], c: Sequence[int], dead: int = 0, mode: int = EXH_
 
 
 def dep_table() -> np.ndarray:
-    """(58, 8) dependency table of the projected sweep (0xFF = untouched)."""
-    out = np.zeros(58 * 8, np.uint8)
-    rc = lib().sx_dep_table(_p8(out))
+    """(58, 16) dependency table of the projected sweep: inputs/outputs A=0,
+    X=1, Y=2, S=4, M=5, S0=6, S1=7 and 8 + b for flag bit b; 0xFFFF = the
+    output is not written."""
+    out = np.zeros(58 * 16, np.uint16)
+    rc = lib().sx_dep_table(out.ctypes.data_as(C.POINTER(C.c_uint16)))
     if rc:
         raise SxError(rc, "sx_dep_table")
-    return out.reshape(58, 8)
+    return out.reshape(58, 16)
 
 
 class Device:

@@ -65,7 +65,7 @@ typedef struct sx_stats {
   uint64_t qc_hits;      /* ... and the byte filter and candidate pruner */
   uint64_t mid_pass;     /* ... and MidCheck */
   uint64_t exh_pairs;    /* ExhaustiveCheck jobs run */
-  uint64_t exh_states;   /* states evaluated by ExhaustiveCheck (upper bound with early exit) */
+  uint64_t exh_states;   /* states evaluated by ExhaustiveCheck (counted on the device) */
   uint64_t exh_rounds;   /* first-hit rounds */
   uint64_t rules;        /* rules emitted */
   uint64_t overflow_retries; /* compaction-buffer overflows recovered by re-running a tile */
@@ -157,10 +157,12 @@ int sx_exec_states(sx_handle *h, uint8_t *states, const uint32_t *words, const u
 int sx_exh_plan(const uint32_t *t, int tn, const uint32_t *c, int cn, uint32_t dead_flags,
                 int32_t exh_mode, uint8_t sets[8], int32_t *n_sets, uint64_t *states);
 
-/* The per-opcode data-dependency table behind SX_EXH_PROJECTED:
- * out[op * 8 + r] = bit set of the input state bytes that output byte r of
- * opcode op is a function of (0xFF = byte r is not written).  Host-side. */
-int sx_dep_table(uint8_t out[58 * 8]);
+/* The per-opcode data-dependency table behind SX_EXH_PROJECTED, at flag
+ * granularity: inputs and outputs are numbered A=0, X=1, Y=2, S=4, M=5, S0=6,
+ * S1=7 and 8 + b for bit b of P; out[op * 16 + o] = bit set of the inputs that
+ * output o of opcode op is a function of (0xFFFF = output o is not written).
+ * Host-side. */
+int sx_dep_table(uint16_t out[58 * 16]);
 
 #ifdef __cplusplus
 }

@@ -53,29 +53,49 @@ def test_full_plan_is_the_go_domain():
     assert n == 512
 
 
-REG_BYTE = [0, 1, 2, 3, 4, 5, 6, 7]
+BYTES = (0, 1, 2, 4, 5, 6, 7)
+
+
+def _outputs(st):
+    """Outputs in the dependency numbering: bytes, then flag bits 8..15."""
+    out = {o: st[..., o] for o in BYTES}
+    for b in range(8):
+        out[8 + b] = (st[..., 3] >> b) & 1
+    return out
 
 
 def test_dependency_table_is_sound():
-    """For every opcode, output byte r and input byte q outside dep[op][r],
-    changing q over all 256 values on 64 random base states never changes
-    output r."""
+    """For every opcode, output o and input q outside dep[op][o], changing q
+    (all 256 values of a byte, both values of a flag bit) on 64 random base
+    states never changes output o."""
     dep = core.dep_table()
     rng = np.random.default_rng(5)
     base = rng.integers(0, 256, (64, 8)).astype(np.uint8)
+    inputs = [(q, 256) for q in BYTES] + [(8 + b, 2) for b in range(8)]
     for op in range(58):
-        for q in range(8):
-            st = np.repeat(base, 256, axis=0)
-            st[:, q] = np.tile(np.arange(256, dtype=np.uint8), 64)
-            imm = rng.integers(0, 256)
-            out = o.exec_many(st, np.uint32(op | (int(imm) << 16))).reshape(64, 256, 8)
-            for r in range(8):
-                d = dep[op, r]
-                deps = (1 << r) if d == 0xFF else d
+        imm = int(rng.integers(0, 256))
+        for q, nv in inputs:
+            st = np.repeat(base, nv, axis=0)
+            vals = np.tile(np.arange(nv, dtype=np.uint8), 64)
+            if q < 8:
+                st[:, q] = vals
+            else:
+                b = q - 8
+                st[:, 3] = (st[:, 3] & ~np.uint8(1 << b)) | (vals << b)
+            res = o.exec_many(st, np.uint32(op | (imm << 16))).reshape(64, nv, 8)
+            for out_i, col in _outputs(res).items():
+                d = int(dep[op, out_i])
+                deps = (1 << out_i) if d == 0xFFFF else d
                 if deps & (1 << q):
                     continue
-                col = out[:, :, r]
-                assert (col == col[:, :1]).all(), (o.OPS[op], "out", r, "in", q)
+                assert (col == col[:, :1]).all(), (o.OPS[op], "out", out_i, "in", q)
+
+
+def test_flag_projection_shrinks_compare_chains():
+    W = o.word
+    # CMP #n : CPX #m -> CPX #m: N, Z, C come from X alone
+    sets, n = core.exh_plan([W(o.CMP_IMM, 5), W(o.CPX_IMM, 7)], [W(o.CPX_IMM, 7)], 0, core.EXH_PROJECTED)
+    assert n <= 2 * 512
 
 
 def _sweep_ok(t, c, dead, vset):
