@@ -8,7 +8,6 @@ Writes (all numpy .npy, no pickles):
   c1_go_deadFF.npy          ... 0xFF (exact pass + flag-relaxed pass)
   c1_go_dead01.npy          ... 0x01
   c1_cuda_dead00.npy        oracle rules, CUDA-tool semantics (all hits, 2-byte M ops)
-  c1_cuda_deadFF.npy        ... 0xFF
   c1_counts.json            stage counters of each run
 """
 import json
@@ -37,8 +36,7 @@ def main(only=None):
     counts = json.load(open(cpath)) if os.path.exists(cpath) else {}
     runs = [("c1_go_dead00", dict(dead=0x00)), ("c1_go_deadFF", dict(dead=0xFF)),
             ("c1_go_dead01", dict(dead=0x01)),
-            ("c1_cuda_dead00", dict(dead=0x00, hits=1, bytes_model=1)),
-            ("c1_cuda_deadFF", dict(dead=0xFF, hits=1, bytes_model=1))]
+            ("c1_cuda_dead00", dict(dead=0x00, hits=1, bytes_model=1))]
     for name, kw in runs:
         if only and name not in only:
             continue

@@ -157,7 +157,6 @@ def test_projection_equals_full_sweep_k3(dev):
     ("c1_go_deadFF", dict(dead=0xFF)),
     ("c1_go_dead01", dict(dead=0x01)),
     ("c1_cuda_dead00", dict(dead=0x00, hits=core.HITS_ALL, bytes_model=isa.BYTES_CUDA)),
-    ("c1_cuda_deadFF", dict(dead=0xFF, hits=core.HITS_ALL, bytes_model=isa.BYTES_CUDA)),
 ])
 def test_c1_rule_sets(dev, name, kw):
     targets = load("c1_targets")
