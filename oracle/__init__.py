@@ -46,6 +46,12 @@ class Stats(C.Structure):
                 ("exh_calls", C.c_uint64), ("found", C.c_uint64)]
 
 
+class Chain(C.Structure):
+    _fields_ = [("seq", C.c_uint32 * 10), ("len", C.c_int32), ("cost10", C.c_int32), ("temp", C.c_double),
+                ("step", C.c_uint64), ("accepted", C.c_uint64), ("hits", C.c_int32),
+                ("first_hit_step", C.c_int32)]
+
+
 class SearchCfg(C.Structure):
     _fields_ = [("target_len", C.c_int), ("max_cand_len", C.c_int), ("bytes_model", C.c_int),
                 ("hits", C.c_int), ("exh_mode", C.c_int), ("use_mid", C.c_int),
@@ -95,6 +101,12 @@ def lib() -> C.CDLL:
         L.sxo_search.argtypes = [C.POINTER(SearchCfg), u32p, C.c_int64, C.POINTER(Rule),
                                  C.c_int64, C.POINTER(C.c_int64), C.POINTER(Stats)]
         L.sxo_search.restype = C.c_int
+        L.sxo_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.sxo_stoke_cost10.argtypes = [u32p, C.c_int, u32p, C.c_int, C.c_uint8]
+        L.sxo_stoke_cost10.restype = C.c_int
+        L.sxo_stoke_init.argtypes = [u32p, C.c_int, C.c_double, C.c_uint8, C.POINTER(Chain)]
+        L.sxo_stoke_steps.argtypes = [u32p, C.c_int, C.c_uint8, C.c_uint32, C.c_uint64, C.c_double,
+                                      u32p, C.c_int, C.POINTER(Chain)]
         L.sxo_states_equal_masked.argtypes = [u8p, u8p, C.c_uint8]
         L.sxo_states_equal_masked.restype = C.c_int
         _lib = L
@@ -245,3 +257,35 @@ def search(targets: np.ndarray, *, max_cand_len: int = 1, dead: int = 0, bytes_m
         if rc != 0:
             raise RuntimeError(f"sxo_search failed: {rc}")
         return out[:nout.value].copy(), st
+
+
+def philox4x32_10(ctr, key):
+    a = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    out = (C.c_uint32 * 4)()
+    lib().sxo_philox4x32_10(a, k, out)
+    return list(out)
+
+
+def stoke_cost10(t, c, dead=0):
+    ta, tn = _seq(t)
+    ca, cn = _seq(c)
+    return lib().sxo_stoke_cost10(ta, tn, ca, cn, dead)
+
+
+def stoke_chain(t, *, t0=1.0, dead=0):
+    ch = Chain()
+    ta, tn = _seq(t)
+    lib().sxo_stoke_init(ta, tn, t0, dead, C.byref(ch))
+    return ch
+
+
+def stoke_steps(t, ch, nsteps, *, chain_id=0, seed=0, decay=0.9999, dead=0, weights=None):
+    ta, tn = _seq(t)
+    w = None
+    if weights is not None:
+        w = np.ascontiguousarray(weights, dtype=np.uint32)
+        assert w.shape == (2863,) and int(w.sum(dtype=np.uint64)) == 1 << 32
+        w = w.ctypes.data_as(C.POINTER(C.c_uint32))
+    lib().sxo_stoke_steps(ta, tn, dead, chain_id, seed, decay, w, nsteps, C.byref(ch))
+    return ch

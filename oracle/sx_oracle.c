@@ -10,6 +10,7 @@
  */
 #include "sx_oracle.h"
 
+#include <math.h>
 #include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -875,4 +876,211 @@ int sxo_search(const sxo_search_cfg *cfg, const uint32_t *targets, int64_t ntarg
   free(w.rules);
   pthread_mutex_destroy(&w.mu);
   return (out && w.nrules > cap) ? -3 : 0;
+}
+
+/* ------------------------------------------------------------------ STOKE -- */
+
+/* Philox4x32-10: 10 rounds of the S-box (two 32x32->64 multiplies) with the
+ * Weyl key schedule; multipliers and key increments from the SC'11 paper. */
+void sxo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n1 = (uint32_t)p1;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    uint32_t n3 = (uint32_t)p0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static int seq_bytes_go(const uint32_t *s, int n) {
+  int b = 0;
+  for (int i = 0; i < n; i++) b += sxo_byte_size(SXO_OP(s[i]), 0);
+  return b;
+}
+
+/* Hamming distance of the output states over the 32 Mid vectors, P compared
+ * under 0xCF &^ dead (pkg/stoke/cost.go:32-81 counts mismatching vectors on
+ * the 8 QuickCheck vectors instead; this build's cost is defined by bits). */
+static int hamming32(const uint8_t tout[32][8], const uint32_t *c, int cn, uint8_t dead) {
+  int h = 0;
+  uint8_t pm = (uint8_t)(PMASK & ~dead);
+  for (int v = 0; v < 32; v++) {
+    uint8_t s[8];
+    memcpy(s, k_mid[v], 8);
+    sxo_exec_seq(s, c, cn);
+    for (int r = 0; r < 8; r++) {
+      uint8_t d = (uint8_t)(s[r] ^ tout[v][r]);
+      if (r == SXO_P) d &= pm;
+      h += __builtin_popcount(d);
+    }
+  }
+  return h;
+}
+
+static void target_outs(const uint32_t *t, int tn, uint8_t out[32][8]) {
+  for (int v = 0; v < 32; v++) {
+    memcpy(out[v], k_mid[v], 8);
+    sxo_exec_seq(out[v], t, tn);
+  }
+}
+
+int sxo_stoke_cost10(const uint32_t *t, int tn, const uint32_t *c, int cn, uint8_t dead) {
+  uint8_t tout[32][8];
+  target_outs(t, tn, tout);
+  return 10 * hamming32(tout, c, cn, dead) + seq_bytes_go(c, cn);
+}
+
+void sxo_stoke_init(const uint32_t *t, int tn, double t0, uint8_t dead, sxo_chain *ch) {
+  memset(ch, 0, sizeof(*ch));
+  for (int i = 0; i < tn; i++) ch->seq[i] = t[i];
+  ch->len = tn;
+  ch->cost10 = sxo_stoke_cost10(t, tn, t, tn, dead);
+  ch->temp = t0;
+  ch->first_hit_step = -1;
+}
+
+/* Move-type thresholds on the top byte of the first random word: replace
+ * 102/256, swap 52/256, delete 51/256, insert 25/256, immediate 26/256 (the
+ * 40/20/20/10/10 split of pkg/stoke/mutator.go:30-44 in 1/256 units). */
+#define MV_REPLACE_END 102u
+#define MV_SWAP_END 154u
+#define MV_DELETE_END 205u
+#define MV_INSERT_END 230u
+#define W_REP (102.0 / 256.0)
+#define W_DEL (51.0 / 256.0)
+#define W_INS (25.0 / 256.0)
+#define W_IMM (26.0 / 256.0)
+
+static inline uint32_t mulhi32(uint32_t a, uint32_t n) { return (uint32_t)(((uint64_t)a * n) >> 32); }
+
+/* Instruction index with cumulative weight containing u (weights sum to 2^32). */
+static int pick_inst(const uint32_t *w, uint32_t u) {
+  uint64_t acc = 0;
+  for (int i = 0; i < 2863; i++) {
+    acc += w[i];
+    if ((uint64_t)u < acc) return i;
+  }
+  return 2862;
+}
+
+static void uniform_weights(uint32_t w[2863]) {
+  uint64_t total = (uint64_t)1 << 32;
+  uint32_t base = (uint32_t)(total / 2863), extra = (uint32_t)(total % 2863);
+  for (int i = 0; i < 2863; i++) w[i] = base + (i < (int)extra ? 1u : 0u);
+}
+
+void sxo_stoke_steps(const uint32_t *t, int tn, uint8_t dead, uint32_t chain_id, uint64_t seed,
+                     double decay, const uint32_t *weights, int nsteps, sxo_chain *ch) {
+  uint32_t tab[2863], uw[2863];
+  sxo_instruction_table(tab, 0);
+  if (!weights) {
+    uniform_weights(uw);
+    weights = uw;
+  }
+  /* Go-order index of an instruction word */
+  int32_t idx_of[58][256];
+  for (int i = 0; i < 2863; i++) idx_of[SXO_OP(tab[i])][SXO_IMM(tab[i]) & 0xFF] = i;
+#define QW(word) ((double)weights[idx_of[SXO_OP(word)][sxo_has_imm(SXO_OP(word)) ? (SXO_IMM(word) & 0xFF) : 0]] / 4294967296.0)
+  uint8_t tout[32][8];
+  target_outs(t, tn, tout);
+  const int tbytes = seq_bytes_go(t, tn);
+  const int max_len = tn + 2 > 10 ? tn + 2 : 10;
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int it = 0; it < nsteps; it++) {
+    uint32_t ctr[4] = {(uint32_t)ch->step, (uint32_t)(ch->step >> 32), chain_id, 0}, r[4], a[4];
+    sxo_philox4x32_10(ctr, key, r);
+    ctr[3] = 1;
+    sxo_philox4x32_10(ctr, key, a);
+    uint32_t x[10];
+    int L = ch->len, nl = L;
+    memcpy(x, ch->seq, sizeof(x));
+    uint32_t m = r[0] >> 24;
+    double ratio = 1.0;
+    int changed = 0;
+    if (m < MV_REPLACE_END) {
+      int pos = (int)mulhi32(r[1], (uint32_t)L);
+      uint32_t v = tab[pick_inst(weights, r[2])];
+      if (v != x[pos]) {
+        double same = 0.0;
+        int nimm = 0;
+        for (int i = 0; i < L; i++) nimm += sxo_has_imm(SXO_OP(x[i]));
+        if (SXO_OP(v) == SXO_OP(x[pos]) && sxo_has_imm(SXO_OP(v))) same = W_IMM / (nimm * 256.0);
+        ratio = (W_REP * QW(x[pos]) / L + same) / (W_REP * QW(v) / L + same);
+        x[pos] = v;
+        changed = 1;
+      }
+    } else if (m < MV_SWAP_END) {
+      if (L >= 2) {
+        int pos = (int)mulhi32(r[1], (uint32_t)(L - 1));
+        if (x[pos] != x[pos + 1]) {
+          uint32_t tmp = x[pos];
+          x[pos] = x[pos + 1];
+          x[pos + 1] = tmp;
+          changed = 1;
+        }
+      }
+    } else if (m < MV_DELETE_END) {
+      if (L > 1) {
+        int pos = (int)mulhi32(r[1], (uint32_t)L);
+        ratio = W_INS * QW(x[pos]) / W_DEL;
+        for (int i = pos; i + 1 < L; i++) x[i] = x[i + 1];
+        nl = L - 1;
+        changed = 1;
+      }
+    } else if (m < MV_INSERT_END) {
+      if (L < max_len) {
+        int pos = (int)mulhi32(r[1], (uint32_t)(L + 1));
+        uint32_t v = tab[pick_inst(weights, r[2])];
+        ratio = W_DEL / (W_INS * QW(v));
+        for (int i = L; i > pos; i--) x[i] = x[i - 1];
+        x[pos] = v;
+        nl = L + 1;
+        changed = 1;
+      }
+    } else {
+      int nimm = 0;
+      for (int i = 0; i < L; i++) nimm += sxo_has_imm(SXO_OP(x[i]));
+      if (nimm > 0) {
+        int k = (int)mulhi32(r[1], (uint32_t)nimm), pos = 0;
+        for (int i = 0; i < L; i++)
+          if (sxo_has_imm(SXO_OP(x[i])) && k-- == 0) {
+            pos = i;
+            break;
+          }
+        uint32_t v = SXO_WORD(SXO_OP(x[pos]), r[3] >> 24);
+        if (v != x[pos]) {
+          double same = W_IMM / (nimm * 256.0);
+          ratio = (W_REP * QW(x[pos]) / L + same) / (W_REP * QW(v) / L + same);
+          x[pos] = v;
+          changed = 1;
+        }
+      }
+    }
+    if (changed) {
+      int h = hamming32(tout, x, nl, dead);
+      int c10 = 10 * h + seq_bytes_go(x, nl);
+      double la = -((double)(c10 - ch->cost10) / 10.0) / ch->temp + log(ratio);
+      double u = ((double)(a[0] >> 5) * 67108864.0 + (double)(a[1] >> 6) + 0.5) / 9007199254740992.0;
+      if (la >= 0.0 || u < exp(la)) {
+        memcpy(ch->seq, x, sizeof(x));
+        ch->len = nl;
+        ch->cost10 = c10;
+        ch->accepted++;
+        if (h == 0 && seq_bytes_go(x, nl) < tbytes) {
+          ch->hits++;
+          if (ch->first_hit_step < 0) ch->first_hit_step = (int32_t)ch->step;
+        }
+      }
+    }
+    ch->temp *= decay;
+    ch->step++;
+  }
+#undef QW
 }

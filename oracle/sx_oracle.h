@@ -120,6 +120,35 @@ typedef struct sxo_search_cfg {
 int sxo_search(const sxo_search_cfg *cfg, const uint32_t *targets, int64_t ntargets,
                sxo_rule *out, int64_t cap, int64_t *nout, sxo_stats *stats);
 
+/* STOKE ------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel Random Numbers: As
+ * Easy as 1, 2, 3", SC'11). */
+void sxo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Chain state of the STOKE search as defined for this build (the reference's
+ * move set and verify trigger, pkg/stoke/mutator.go and search.go, with the
+ * exact Metropolis-Hastings ratio, Philox randomness and the Hamming cost). */
+typedef struct sxo_chain {
+  uint32_t seq[10];
+  int32_t len;
+  int32_t cost10;     /* 10 x Hamming distance over the 32 Mid vectors + bytes */
+  double temp;
+  uint64_t step;
+  uint64_t accepted;
+  int32_t hits;       /* steps whose accepted state had Hamming 0 and fewer bytes */
+  int32_t first_hit_step;
+} sxo_chain;
+
+/* cost10 of a candidate against a target (Go byte sizes). */
+int sxo_stoke_cost10(const uint32_t *t, int tn, const uint32_t *c, int cn, uint8_t dead);
+/* Initialise a chain at the target. */
+void sxo_stoke_init(const uint32_t *t, int tn, double t0, uint8_t dead, sxo_chain *ch);
+/* Run nsteps steps.  weights: NULL = uniform over the 2,863 instructions in
+ * Go order, else 2,863 uint32 weights summing to 2^32 (the proposal
+ * distribution of an instruction).  max_len = max(tn + 2, 10). */
+void sxo_stoke_steps(const uint32_t *t, int tn, uint8_t dead, uint32_t chain_id, uint64_t seed,
+                     double decay, const uint32_t *weights, int nsteps, sxo_chain *ch);
+
 #ifdef __cplusplus
 }
 #endif
