@@ -407,9 +407,11 @@ __global__ __launch_bounds__(256) void k_round_min(const uint2 *mid, uint32_t nm
   atomicMin(&cur[k], r);
 }
 
+constexpr int kJobWords = 10;  // longest sequence a job holds (STOKE candidates)
+
 struct Job {
-  uint32_t tw[4];
-  uint32_t cw[4];
+  uint32_t tw[kJobWords];
+  uint32_t cw[kJobWords];
   uint8_t tl, cl, pass, pmask;
   uint32_t target;  // index within the segment (or pair index for sx_check_pairs)
   uint32_t slot;
@@ -421,7 +423,7 @@ struct Job {
 __device__ __forceinline__ void fill_job(Job &j, const uint32_t *targets, uint32_t tl,
                                          const uint32_t *cwords, uint32_t cl, uint32_t t,
                                          uint32_t slot, uint32_t pass, uint32_t pmask) {
-  for (int k = 0; k < 4; k++) {
+  for (int k = 0; k < kJobWords; k++) {
     j.tw[k] = k < (int)tl ? targets[(uint64_t)t * tl + k] : 0;
     j.cw[k] = 0;
   }
@@ -538,9 +540,9 @@ __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, u
     pos += w;
   }
   const uint32_t tl = uni(jp->tl), cl = uni(jp->cl);
-  uint32_t tw[4], cw[4];
+  uint32_t tw[kJobWords], cw[kJobWords];
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
+  for (int k = 0; k < kJobWords; k++) {
     tw[k] = uni(jp->tw[k]);
     cw[k] = uni(jp->cw[k]);
   }
@@ -1309,9 +1311,9 @@ extern "C" int sx_check_pairs(sx_handle *h, const uint32_t *t_words, const uint8
       for (uint32_t i = 0; i < nb; i++) {
         Job &j = jobs[i];
         memset(&j, 0, sizeof(j));
-        for (int k = 0; k < 4; k++) {
-          j.tw[k] = tw[(b0 + i) * 4 + k];
-          j.cw[k] = cw[(b0 + i) * 4 + k];
+        for (int k = 0; k < kJobWords; k++) {
+          j.tw[k] = k < 4 ? tw[(b0 + i) * 4 + k] : 0u;
+          j.cw[k] = k < 4 ? cw[(b0 + i) * 4 + k] : 0u;
         }
         j.tl = t_len[b0 + i];
         j.cl = c_len[b0 + i];
@@ -1418,5 +1420,48 @@ extern "C" int sx_exh_plan(const uint32_t *t, int tn, const uint32_t *c, int cn,
 extern "C" int sx_dep_table(uint16_t out[58 * 16]) {
   if (!out) return SX_EINVAL;
   memcpy(out, h_dep.d, sizeof(h_dep.d));
+  return SX_OK;
+}
+
+// ----------------------------------------------------- internal (C++) entry --
+
+hipStream_t sx_internal_stream(sx_handle *h) { return h->stream; }
+
+int sx_internal_error(sx_handle *h, int code, const char *msg) { return set_err(h, code, "%s", msg); }
+
+// Complete (projected) ExhaustiveCheck of n pairs of sequences of up to 10
+// words, stride 10; ok[i] = 1 when pair i passes.  Used by the STOKE driver.
+int sx_verify_seqs(sx_handle *h, const uint32_t *t_words, const uint8_t *t_len, const uint32_t *c_words,
+                   const uint8_t *c_len, uint64_t n, uint32_t dead_flags, uint8_t *ok) {
+  if (h->poisoned) return set_err(h, SX_EHIP, "handle unusable after a device timeout");
+  const uint32_t pm = kPMask & ~dead_flags;
+  const uint64_t batch = 1u << 20;
+  for (uint64_t b0 = 0; b0 < n; b0 += batch) {
+    uint32_t nb = (uint32_t)std::min<uint64_t>(batch, n - b0);
+    std::vector<Job> jobs(nb);
+    for (uint32_t i = 0; i < nb; i++) {
+      Job &j = jobs[i];
+      memset(&j, 0, sizeof(j));
+      for (int k = 0; k < kJobWords; k++) {
+        j.tw[k] = k < t_len[b0 + i] ? t_words[(b0 + i) * kJobWords + k] : 0u;
+        j.cw[k] = k < c_len[b0 + i] ? c_words[(b0 + i) * kJobWords + k] : 0u;
+      }
+      j.tl = t_len[b0 + i];
+      j.cl = c_len[b0 + i];
+      j.pass = 1;
+      j.pmask = (uint8_t)pm;
+      j.target = i;
+    }
+    Job *dj;
+    ENSURE(B_JOBS, Job, nb, dj);
+    HIPCHK(h, hipMemcpyAsync(dj, jobs.data(), nb * sizeof(Job), hipMemcpyHostToDevice, h->stream));
+    uint64_t chunks = 0;
+    int rc = run_exhaustive(h, nb, SX_EXH_PROJECTED, &chunks, nullptr);
+    if (rc) return rc;
+    std::vector<uint32_t> f(nb);
+    HIPCHK(h, hipMemcpyAsync(f.data(), h->bufs[B_FAIL].p, nb * 4, hipMemcpyDeviceToHost, h->stream));
+    SYNC_WD("verify");
+    for (uint32_t i = 0; i < nb; i++) ok[b0 + i] = f[i] ? 0 : 1;
+  }
   return SX_OK;
 }

@@ -23,13 +23,20 @@ EXH_PROJECTED, EXH_FULL = 0, 1
 # Every symbol include/sx6502.h declares.
 EXPORTS = ("sx_abi_version", "sx_open", "sx_close", "sx_last_error", "sx_enumerate", "sx_fetch_rules",
            "sx_pruned_targets", "sx_check_pairs", "sx_flag_diff", "sx_exec_states", "sx_exh_plan",
-           "sx_dep_table")
+           "sx_dep_table", "sx_stoke_run")
+
+class RuleC(C.Structure):
+    _fields_ = [("src", C.c_uint32 * 4), ("rep", C.c_uint32 * 4), ("target_index", C.c_uint64),
+                ("src_len", C.c_uint8), ("rep_len", C.c_uint8), ("dead_flags", C.c_uint8), ("pass_", C.c_uint8),
+                ("bytes_saved", C.c_int16), ("cycles_saved", C.c_int16)]
+
 
 RULE_DTYPE = np.dtype([
     ("src", "<u4", (4,)), ("rep", "<u4", (4,)), ("target_index", "<u8"),
     ("src_len", "u1"), ("rep_len", "u1"), ("dead_flags", "u1"), ("pass_", "u1"),
     ("bytes_saved", "<i2"), ("cycles_saved", "<i2"),
 ])
+assert RULE_DTYPE.itemsize == C.sizeof(RuleC) == 48
 
 
 class EnumCfg(C.Structure):
@@ -48,6 +55,31 @@ class Stats(C.Structure):
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class StokeCfg(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("chains_per_target", C.c_uint32), ("steps", C.c_uint64),
+                ("steps_per_launch", C.c_uint32), ("stop_when_verified", C.c_int32), ("t0", C.c_double),
+                ("decay", C.c_double), ("seed", C.c_uint64), ("dead_flags", C.c_uint32), ("pad", C.c_uint32),
+                ("weights", C.POINTER(C.c_uint32))]
+
+
+class StokeResult(C.Structure):
+    _fields_ = [("rep", C.c_uint32 * 10), ("rep_len", C.c_uint8), ("verified", C.c_uint8), ("pad", C.c_uint8 * 2),
+                ("bytes_saved", C.c_int16), ("cycles_saved", C.c_int16), ("chain", C.c_uint32),
+                ("step", C.c_uint64), ("seconds", C.c_double)]
+
+
+class StokeChain(C.Structure):
+    _fields_ = [("seq", C.c_uint32 * 10), ("len", C.c_int32), ("cost10", C.c_int32), ("step", C.c_uint64),
+                ("accepted", C.c_uint64)]
+
+
+# numpy views with the C layout (alignment included), checked against ctypes
+STOKE_RESULT_DTYPE = np.dtype(StokeResult)
+STOKE_CHAIN_DTYPE = np.dtype(StokeChain)
+assert STOKE_RESULT_DTYPE.itemsize == C.sizeof(StokeResult) == 72
+assert STOKE_CHAIN_DTYPE.itemsize == C.sizeof(StokeChain) == 64
 
 
 class SxError(RuntimeError):
@@ -84,6 +116,7 @@ def lib() -> C.CDLL:
     L.sx_exh_plan.argtypes = [u32p, C.c_int, u32p, C.c_int, C.c_uint32, C.c_int32, u8p,
                               C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
     L.sx_dep_table.argtypes = [C.POINTER(C.c_uint16)]
+    L.sx_stoke_run.argtypes = [vp, C.POINTER(StokeCfg), u32p, u8p, C.c_uint64, vp, vp]
     for name in EXPORTS:
         getattr(L, name).restype = C.c_int
     _lib = L
@@ -234,3 +267,34 @@ class Device:
         w, ln = pack_seqs(seqs) if not isinstance(seqs, tuple) else seqs
         self._check(lib().sx_exec_states(self._h, _p8(st), _p32(w), _p8(ln), len(ln)), "sx_exec_states")
         return st
+
+    # -- STOKE -----------------------------------------------------------
+    def stoke(self, targets, *, chains: int = 16, steps: int = 100000, steps_per_launch: int = 0,
+              t0: float = 1.0, decay: float = 0.9999, seed: int = 0, dead: int = 0, weights=None,
+              stop_when_verified: bool = True, return_chains: bool = False):
+        """STOKE over target sequences (lists of up to 8 words); returns the
+        per-target results (STOKE_RESULT_DTYPE) and, on request, the final
+        chain states."""
+        n = len(targets)
+        tw = np.zeros((n, 8), np.uint32)
+        tl = np.zeros(n, np.uint8)
+        for i, t in enumerate(targets):
+            tw[i, :len(t)] = t
+            tl[i] = len(t)
+        cfg = StokeCfg()
+        cfg.struct_size = C.sizeof(StokeCfg)
+        cfg.chains_per_target, cfg.steps, cfg.steps_per_launch = chains, steps, steps_per_launch
+        cfg.stop_when_verified = 1 if stop_when_verified else 0
+        cfg.t0, cfg.decay, cfg.seed, cfg.dead_flags = t0, decay, seed, dead
+        keep = None
+        if weights is not None:
+            keep = np.ascontiguousarray(weights, dtype=np.uint32)
+            assert keep.shape == (n, 2863)
+            cfg.weights = _p32(keep)
+        out = np.zeros(n, STOKE_RESULT_DTYPE)
+        ch = np.zeros(n * chains, STOKE_CHAIN_DTYPE) if return_chains else None
+        rc = lib().sx_stoke_run(self._h, C.byref(cfg), _p32(tw), _p8(tl), n, out.ctypes.data_as(C.c_void_p),
+                                ch.ctypes.data_as(C.c_void_p) if ch is not None else None)
+        self._check(rc, "sx_stoke_run")
+        del keep
+        return (out, ch) if return_chains else out

@@ -157,6 +157,52 @@ int sx_exec_states(sx_handle *h, uint8_t *states, const uint32_t *words, const u
 int sx_exh_plan(const uint32_t *t, int tn, const uint32_t *c, int cn, uint32_t dead_flags,
                 int32_t exh_mode, uint8_t sets[8], int32_t *n_sets, uint64_t *states);
 
+/* STOKE search (pkg/stoke): chains_per_target Metropolis-Hastings chains per
+ * target, each starting at its target.  Cost = 10 x Hamming distance of the
+ * output state over the 32 Mid vectors + byte size (Go sizes), in tenths;
+ * moves and weights of pkg/stoke/mutator.go:30-106 (replace 102, swap 52,
+ * delete 51, insert 25, immediate 26, out of 256); instructions drawn from
+ * integer weights summing to 2^32 (uniform when weights == NULL, else one
+ * table of 2,863 weights per target, Go instruction order); acceptance with
+ * the exact proposal ratio at temperature t0 * decay^step; randomness from
+ * Philox4x32-10 with counter (step, chain, stream) and key seed.  A chain whose
+ * accepted state matches the target on all 32 vectors with fewer bytes posts
+ * it; posted candidates are verified with the complete ExhaustiveCheck (the
+ * verify trigger of pkg/stoke/search.go:101-112).  Replaces stoke.Run
+ * (pkg/stoke/search.go:32-176). */
+typedef struct sx_stoke_cfg {
+  uint32_t struct_size;      /* sizeof(sx_stoke_cfg) */
+  uint32_t chains_per_target;
+  uint64_t steps;            /* steps per chain */
+  uint32_t steps_per_launch; /* 0 = 4096 */
+  int32_t stop_when_verified;/* stop a target's chains at its first verified rewrite */
+  double t0, decay;
+  uint64_t seed;
+  uint32_t dead_flags;
+  uint32_t pad;
+  const uint32_t *weights;   /* NULL or n x 2,863 uint32, each row summing to 2^32 */
+} sx_stoke_cfg;
+
+typedef struct sx_stoke_result {
+  uint32_t rep[10];
+  uint8_t rep_len, verified, pad[2];
+  int16_t bytes_saved, cycles_saved;
+  uint32_t chain;            /* chain that found it */
+  uint64_t step;             /* its step index */
+  double seconds;            /* wall time from the start of the search to its verification */
+} sx_stoke_result;
+
+typedef struct sx_stoke_chain {
+  uint32_t seq[10];
+  int32_t len, cost10;
+  uint64_t step, accepted;
+} sx_stoke_chain;
+
+/* targets: n x 8 words (lengths in lens, 1..8); out: n results; chains_out:
+ * optional n * chains_per_target final chain states. */
+int sx_stoke_run(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_t *targets, const uint8_t *lens,
+                 uint64_t n, sx_stoke_result *out, sx_stoke_chain *chains_out);
+
 /* The per-opcode data-dependency table behind SX_EXH_PROJECTED, at flag
  * granularity: inputs and outputs are numbered A=0, X=1, Y=2, S=4, M=5, S0=6,
  * S1=7 and 8 + b for bit b of P; out[op * 16 + o] = bit set of the inputs that

@@ -9,6 +9,9 @@ numpy's generator versions.
 * C3: 65,536 length-3 targets drawn by rejection (seed 1): three independent
   uniform picks from the 2,863 instructions, kept when ShouldPrune is false
   (pkg/search/pruner.go:6-34) and not already drawn.
+* C4/C5: 256 STOKE targets (seed 2): length uniform in 4..8, instructions
+  uniform over the 2,863, rejected when a pull (PLA/PLP) meets an empty stack,
+  i.e. the sequence reads a stack slot it never wrote.
 """
 from __future__ import annotations
 
@@ -64,3 +67,26 @@ def c3_targets(table: np.ndarray, should_prune, k: int = 65536, seed: int = 1) -
         seen.add(tri)
         out.append(seq)
     return np.array(out, dtype=np.uint32)
+
+
+def c4_targets(table: np.ndarray, k: int = 256, seed: int = 2):
+    rng = SplitMix64(seed)
+    pla, plp, pha, php = 22, 24, 21, 23
+    out = []
+    n = len(table)
+    while len(out) < k:
+        length = 4 + rng.below(5)
+        seq = [int(table[rng.below(n)]) for _ in range(length)]
+        depth, ok = 0, True
+        for w in seq:
+            op = w & 0xFFFF
+            if op in (pha, php):
+                depth += 1
+            elif op in (pla, plp):
+                if depth == 0:
+                    ok = False
+                    break
+                depth -= 1
+        if ok:
+            out.append(seq)
+    return out

@@ -142,3 +142,31 @@ def test_projected_plan_decides_like_the_complete_sweep():
             assert got == want, (isa.disasm_seq(t), isa.disasm_seq(cand), dead, sets)
             checked += 1
     assert checked > 500
+
+
+def test_ctypes_mirrors_match_the_c_header(tmp_path):
+    """sizeof/offsetof of every ABI struct, from a C program compiled against
+    include/sx6502.h, equal the ctypes/numpy mirrors used by the package."""
+    import ctypes as C
+    import subprocess
+    structs = {"sx_rule": core.RuleC, "sx_enum_cfg": core.EnumCfg, "sx_stats": core.Stats,
+               "sx_stoke_cfg": core.StokeCfg, "sx_stoke_result": core.StokeResult,
+               "sx_stoke_chain": core.StokeChain}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sx6502.h"', 'int main(void) {']
+    for name, py in structs.items():
+        lines.append(f'  printf("{name} size %zu\\n", sizeof({name}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{name}.{f} %zu\\n", offsetof({name}, {f.rstrip("_")}));')
+    lines.append("  return 0; }")
+    src = tmp_path / "sizes.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-I", os.path.dirname(HDR), str(src), "-o", str(exe)], check=True)
+    got = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                               check=True).stdout.splitlines())
+    for name, py in structs.items():
+        assert int(got[f"{name} size"]) == C.sizeof(py), name
+        for f, _ in py._fields_:
+            assert int(got[f"{name}.{f}"]) == getattr(py, f).offset, (name, f)
+    assert core.STOKE_RESULT_DTYPE.itemsize == C.sizeof(core.StokeResult)
+    assert core.RULE_DTYPE.itemsize == C.sizeof(core.RuleC)
