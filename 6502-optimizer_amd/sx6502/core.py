@@ -23,7 +23,12 @@ EXH_PROJECTED, EXH_FULL = 0, 1
 # Every symbol include/sx6502.h declares.
 EXPORTS = ("sx_abi_version", "sx_open", "sx_close", "sx_last_error", "sx_enumerate", "sx_fetch_rules",
            "sx_pruned_targets", "sx_check_pairs", "sx_flag_diff", "sx_exec_states", "sx_exh_plan",
-           "sx_dep_table", "sx_stoke_run")
+           "sx_dep_table", "sx_stoke_run", "sx_mlp_param_count", "sx_mlp_create", "sx_mlp_destroy",
+           "sx_mlp_get_params", "sx_mlp_set_params", "sx_mlp_loss_grad", "sx_mlp_train", "sx_mlp_predict",
+           "sx_proposal_weights", "sx_mfma_gemm_test")
+MLP_CLASSES = 2863
+MLP_CTX = 8
+PAD_TOKEN = 2863
 
 class RuleC(C.Structure):
     _fields_ = [("src", C.c_uint32 * 4), ("rep", C.c_uint32 * 4), ("target_index", C.c_uint64),
@@ -117,8 +122,19 @@ def lib() -> C.CDLL:
                               C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
     L.sx_dep_table.argtypes = [C.POINTER(C.c_uint16)]
     L.sx_stoke_run.argtypes = [vp, C.POINTER(StokeCfg), u32p, u8p, C.c_uint64, vp, vp]
+    u16p, f32p = C.POINTER(C.c_uint16), C.POINTER(C.c_float)
+    L.sx_mlp_create.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
+    L.sx_mlp_destroy.argtypes = [vp]
+    L.sx_mlp_get_params.argtypes = [vp, f32p]
+    L.sx_mlp_set_params.argtypes = [vp, f32p]
+    L.sx_mlp_loss_grad.argtypes = [vp, u16p, u16p, C.c_uint32, f32p, f32p]
+    L.sx_mlp_train.argtypes = [vp, u16p, u16p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, f32p]
+    L.sx_mlp_predict.argtypes = [vp, u16p, C.c_uint32, f32p]
+    L.sx_proposal_weights.argtypes = [f32p, C.c_uint64, C.c_double, u32p]
+    L.sx_mfma_gemm_test.argtypes = [vp, u16p, u16p, C.c_int, C.c_int, C.c_int, f32p]
     for name in EXPORTS:
         getattr(L, name).restype = C.c_int
+    L.sx_mlp_param_count.restype = C.c_uint64
     _lib = L
     return L
 
@@ -129,6 +145,26 @@ def _p32(a: np.ndarray):
 
 def _p8(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def _p16(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint16))
+
+
+def _pf(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def proposal_weights(probs: np.ndarray, eps: float = 0.1) -> np.ndarray:
+    """sx_proposal_weights: n x 2,863 probabilities -> uint32 tables summing
+    to 2^32 with a uniform share eps (host-side, no device needed)."""
+    p = np.ascontiguousarray(probs, dtype=np.float32)
+    assert p.ndim == 2 and p.shape[1] == MLP_CLASSES
+    out = np.zeros(p.shape, np.uint32)
+    rc = lib().sx_proposal_weights(_pf(p), len(p), float(eps), _p32(out))
+    if rc != SX_OK:
+        raise SxError(rc, "sx_proposal_weights: bad probabilities or eps")
+    return out
 
 
 def pack_seqs(seqs: Sequence[Sequence[int]]):
@@ -298,3 +334,80 @@ class Device:
         self._check(rc, "sx_stoke_run")
         del keep
         return (out, ch) if return_chains else out
+
+    # -- learned proposal network ----------------------------------------
+    def mlp(self, seed: int = 0) -> "Mlp":
+        return Mlp(self, seed)
+
+    def mfma_gemm_test(self, A: np.ndarray, B: np.ndarray) -> np.ndarray:
+        """C = A B through the MLP tile kernel; A, B given as uint16 bf16 bits."""
+        a = np.ascontiguousarray(A, dtype=np.uint16)
+        b = np.ascontiguousarray(B, dtype=np.uint16)
+        M, K = a.shape
+        K2, N = b.shape
+        assert K == K2
+        c = np.zeros((M, N), np.float32)
+        self._check(lib().sx_mfma_gemm_test(self._h, _p16(a), _p16(b), M, N, K, _pf(c)), "sx_mfma_gemm_test")
+        return c
+
+
+class Mlp:
+    """The learned STOKE proposal network on one device (sx_mlp_* in
+    include/sx6502.h).  Contexts are n x 8 uint16 tokens (Go instruction
+    index, 2863 = padding), labels n uint16 instruction indices."""
+
+    def __init__(self, dev: Device, seed: int = 0):
+        self.dev = dev
+        self._m = C.c_void_p()
+        dev._check(lib().sx_mlp_create(dev._h, seed, C.byref(self._m)), "sx_mlp_create")
+        self.n_params = int(lib().sx_mlp_param_count())
+
+    def close(self):
+        if self._m:
+            lib().sx_mlp_destroy(self._m)
+            self._m = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _ctx(ctx) -> np.ndarray:
+        c = np.ascontiguousarray(ctx, dtype=np.uint16)
+        assert c.ndim == 2 and c.shape[1] == MLP_CTX
+        return c
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, np.float32)
+        self.dev._check(lib().sx_mlp_get_params(self._m, _pf(out)), "sx_mlp_get_params")
+        return out
+
+    def set_params(self, p: np.ndarray):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        assert a.shape == (self.n_params,)
+        self.dev._check(lib().sx_mlp_set_params(self._m, _pf(a)), "sx_mlp_set_params")
+
+    def loss_grad(self, ctx, labels):
+        c = self._ctx(ctx)
+        y = np.ascontiguousarray(labels, dtype=np.uint16)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        self.dev._check(lib().sx_mlp_loss_grad(self._m, _p16(c), _p16(y), len(c), _pf(loss), _pf(g)),
+                        "sx_mlp_loss_grad")
+        return float(loss[0]), g
+
+    def train(self, ctx, labels, steps: int, batch: int = 256, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+        c = self._ctx(ctx)
+        y = np.ascontiguousarray(labels, dtype=np.uint16)
+        losses = np.zeros(max(steps, 1), np.float32)
+        self.dev._check(lib().sx_mlp_train(self._m, _p16(c), _p16(y), len(c), steps, batch, lr, seed,
+                                           _pf(losses)), "sx_mlp_train")
+        return losses[:steps]
+
+    def predict(self, ctx) -> np.ndarray:
+        c = self._ctx(ctx)
+        out = np.zeros((len(c), MLP_CLASSES), np.float32)
+        self.dev._check(lib().sx_mlp_predict(self._m, _p16(c), len(c), _pf(out)), "sx_mlp_predict")
+        return out

@@ -210,6 +210,50 @@ int sx_stoke_run(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_t *targets,
  * Host-side. */
 int sx_dep_table(uint16_t out[58 * 16]);
 
+/* ---- learned STOKE proposal (no counterpart in the reference: the
+ * reference's mutator draws instructions uniformly, pkg/stoke/mutator.go:108-121).
+ *
+ * A two-layer MLP maps a target sequence (context: 8 tokens, token = the Go
+ * instruction index 0..2862 of search.AllInstructions order, 2863 = padding)
+ * to a distribution over the 2,863 instructions:
+ *   X = [E[t0] .. E[t7]] (8 x 64 = 512), H = relu(X W1 + b1) (256),
+ *   p = softmax(H W2 + b2) (2,863).
+ * Parameters (fp32 master copy, row-major, in this order): E[2864][64],
+ * W1[512][256], b1[256], W2[256][2863], b2[2863].  Products run on MFMA with
+ * bf16 operands (E, W1, W2, H and the back-propagated dZ, dH rounded to
+ * bf16, round-to-nearest-even) and fp32 accumulation; biases, softmax and
+ * Adam in fp32. Training minimises the mean cross entropy of the label
+ * (the Go index of one instruction of a known rewrite of the context). */
+typedef struct sx_mlp sx_mlp;
+
+uint64_t sx_mlp_param_count(void);
+/* initial weights from seed (xorshift64): E ~ U(-0.1, 0.1), W1 ~ U(+-1/sqrt(512)),
+ * W2 ~ U(+-1/sqrt(256)), zero biases; Adam state cleared. */
+int sx_mlp_create(sx_handle *h, uint64_t seed, sx_mlp **out);
+int sx_mlp_destroy(sx_mlp *m);
+int sx_mlp_get_params(sx_mlp *m, float *out);       /* sx_mlp_param_count() floats */
+int sx_mlp_set_params(sx_mlp *m, const float *in);  /* also clears the Adam state */
+/* Mean loss over the n examples and its gradient (n x 8 ctx, n labels). */
+int sx_mlp_loss_grad(sx_mlp *m, const uint16_t *ctx, const uint16_t *labels, uint32_t n, float *loss,
+                     float *grad);
+/* `steps` Adam steps (beta 0.9/0.999, eps 1e-8, bias-corrected) on batches of
+ * `batch` (<= 4096) examples drawn uniformly with replacement from the n
+ * examples (Philox4x32-10, counter = (Adam step, row), key = seed).  The step
+ * is one captured hipGraph replayed `steps` times.  losses: optional, one
+ * batch loss per step. */
+int sx_mlp_train(sx_mlp *m, const uint16_t *ctx, const uint16_t *labels, uint32_t n, uint32_t steps,
+                 uint32_t batch, float lr, uint64_t seed, float *losses);
+/* probs: n x 2,863 softmax outputs. */
+int sx_mlp_predict(sx_mlp *m, const uint16_t *ctx, uint32_t n, float *probs);
+/* Proposal tables for sx_stoke_cfg.weights: row i = (1 - eps) * probs[i] +
+ * eps / 2863 as uint32 weights summing to 2^32, every weight >= 1 (the
+ * uniform fallback keeps every instruction reachable, so the chain stays
+ * irreducible and every proposal ratio finite).  Host-side. */
+int sx_proposal_weights(const float *probs, uint64_t n, double eps, uint32_t *out);
+/* C = A B through the MLP's MFMA tile kernel (A: M x K, B: K x N bf16
+ * row-major, C: M x N fp32); a check of the kernel's operand layout. */
+int sx_mfma_gemm_test(sx_handle *h, const uint16_t *A, const uint16_t *B, int M, int N, int K, float *C);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,7 +18,7 @@ HDR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 
 
 
 def test_every_declared_symbol_is_exported():
-    decl = set(re.findall(r"^int (sx_\w+)\(", open(HDR).read(), re.M))
+    decl = set(re.findall(r"^(?:int|uint64_t) (sx_\w+)\(", open(HDR).read(), re.M))
     assert decl == set(core.EXPORTS)
     L = core.lib()
     for name in decl:
