@@ -4,7 +4,10 @@
 //   tokens (8 per example: Go-order instruction index, 2863 = padding)
 //   -> embeddings E[2864][64] (bf16), concatenated to X[B][512]
 //   -> H = relu(X W1 + b1), W1[512][256]
-//   -> Z = H W2 + b2, W2[256][2863]  -> softmax over the 2,863 instructions.
+//   -> Z = H W2 + b2, W2[256][2871]  -> softmax over 2,871 outputs: the 2,863
+//      instructions, then 8 pointers ("the instruction at context position j").
+// A target's proposal folds each pointer's mass onto the instruction at that
+// position (sx_mlp_proposal).
 //
 // Weights are used in bf16 with fp32 master copies for Adam.  All products
 // run on MFMA (v_mfma_f32_16x16x16bf16_1k: bf16 inputs, fp32 accumulation)
@@ -27,19 +30,20 @@ int sx_internal_error(sx_handle *h, int code, const char *msg);
 namespace {
 
 constexpr int kTok = 2864;   // 2,863 instructions + padding
-constexpr int kCls = 2863;
+constexpr int kInsts = 2863;  // instructions
 constexpr int kCtx = 8;
 constexpr int kEmb = 64;
 constexpr int kIn = kCtx * kEmb;  // 512
 constexpr int kHid = 256;
-constexpr int kClsPad = 2880;     // classes padded to a multiple of 64
+constexpr int kOut = kInsts + 8;  // outputs: 2,863 instructions, then 8 context positions
+constexpr int kOutPad = 2880;     // outputs padded to a multiple of 64
 
 constexpr size_t kOffE = 0;
 constexpr size_t kOffW1 = kOffE + (size_t)kTok * kEmb;
 constexpr size_t kOffB1 = kOffW1 + (size_t)kIn * kHid;
 constexpr size_t kOffW2 = kOffB1 + kHid;
-constexpr size_t kOffB2 = kOffW2 + (size_t)kHid * kCls;
-constexpr size_t kParams = kOffB2 + kCls;
+constexpr size_t kOffB2 = kOffW2 + (size_t)kHid * kOut;
+constexpr size_t kParams = kOffB2 + kOut;
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -155,7 +159,7 @@ __global__ void k_softmax_ce(const float *Z, long ldz, const uint16_t *lab, int 
   __shared__ float red[256];
   const float *z = Z + (long)b * ldz;
   float mx = -INFINITY;
-  for (int c = threadIdx.x; c < kCls; c += blockDim.x) mx = fmaxf(mx, z[c]);
+  for (int c = threadIdx.x; c < kOut; c += blockDim.x) mx = fmaxf(mx, z[c]);
   red[threadIdx.x] = mx;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -165,7 +169,7 @@ __global__ void k_softmax_ce(const float *Z, long ldz, const uint16_t *lab, int 
   mx = red[0];
   __syncthreads();
   float sum = 0.f;
-  for (int c = threadIdx.x; c < kCls; c += blockDim.x) sum += expf(z[c] - mx);
+  for (int c = threadIdx.x; c < kOut; c += blockDim.x) sum += expf(z[c] - mx);
   red[threadIdx.x] = sum;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -175,13 +179,13 @@ __global__ void k_softmax_ce(const float *Z, long ldz, const uint16_t *lab, int 
   sum = red[0];
   const float inv = 1.f / sum;
   const int y = lab ? lab[b] : -1;
-  for (int c = threadIdx.x; c < kClsPad; c += blockDim.x) {
-    float p = c < kCls ? expf(z[c] - mx) * inv : 0.f;
-    if (probs && c < kCls) probs[(long)b * kCls + c] = p;
+  for (int c = threadIdx.x; c < kOutPad; c += blockDim.x) {
+    float p = c < kOut ? expf(z[c] - mx) * inv : 0.f;
+    if (probs && c < kOut) probs[(long)b * kOut + c] = p;
     if (dZ) {
-      float g = c < kCls ? (p - (c == y ? 1.f : 0.f)) / (float)B : 0.f;
-      dZ[(long)b * kClsPad + c] = g;
-      dZb[(long)b * kClsPad + c] = f2bf(g);
+      float g = c < kOut ? (p - (c == y ? 1.f : 0.f)) / (float)B : 0.f;
+      dZ[(long)b * kOutPad + c] = g;
+      dZb[(long)b * kOutPad + c] = f2bf(g);
     }
   }
   if (threadIdx.x == 0 && loss_rows && y >= 0) loss_rows[b] = -(z[y] - mx - logf(sum));
@@ -291,9 +295,9 @@ int alloc_work(sx_mlp *m, int B) {
   if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
   bool ok = hipMalloc(&m->X, (size_t)B * kIn * 2) == hipSuccess && hipMalloc(&m->tok, (size_t)B * kCtx * 2) == hipSuccess &&
             hipMalloc(&m->lab, (size_t)B * 2) == hipSuccess && hipMalloc(&m->Hb, (size_t)B * kHid * 2) == hipSuccess &&
-            hipMalloc(&m->dZb, (size_t)B * kClsPad * 2) == hipSuccess &&
+            hipMalloc(&m->dZb, (size_t)B * kOutPad * 2) == hipSuccess &&
             hipMalloc(&m->dHb, (size_t)B * kHid * 2) == hipSuccess && hipMalloc(&m->H, (size_t)B * kHid * 4) == hipSuccess &&
-            hipMalloc(&m->Z, (size_t)B * kClsPad * 4) == hipSuccess && hipMalloc(&m->dZ, (size_t)B * kClsPad * 4) == hipSuccess &&
+            hipMalloc(&m->Z, (size_t)B * kOutPad * 4) == hipSuccess && hipMalloc(&m->dZ, (size_t)B * kOutPad * 4) == hipSuccess &&
             hipMalloc(&m->dH, (size_t)B * kHid * 4) == hipSuccess && hipMalloc(&m->dX, (size_t)B * kIn * 4) == hipSuccess &&
             hipMalloc(&m->lossr, (size_t)B * 4) == hipSuccess;
   if (!ok) return sx_internal_error(m->h, SX_ENOMEM, "mlp work buffers");
@@ -308,21 +312,21 @@ void enqueue_fwd_bwd(sx_mlp *m, int B, bool backward, float *probs, hipStream_t 
   hipLaunchKernelGGL(k_gemm, dim3(nb(kHid, 64), nb(B, 64)), dim3(256), 0, s, m->X, (long)kIn, 1L, W1b, (long)kHid, 1L,
                      m->H, (long)kHid, m->Hb, B, kHid, kIn, m->p + kOffB1, 1, 0);
   // Z = H W2 + b2
-  hipLaunchKernelGGL(k_gemm, dim3(nb(kCls, 64), nb(B, 64)), dim3(256), 0, s, m->Hb, (long)kHid, 1L, W2b, (long)kCls,
-                     1L, m->Z, (long)kClsPad, (uint16_t *)nullptr, B, kCls, kHid, m->p + kOffB2, 0, 0);
-  hipLaunchKernelGGL(k_softmax_ce, dim3(B), dim3(256), 0, s, m->Z, (long)kClsPad, backward ? m->lab : nullptr, B,
+  hipLaunchKernelGGL(k_gemm, dim3(nb(kOut, 64), nb(B, 64)), dim3(256), 0, s, m->Hb, (long)kHid, 1L, W2b, (long)kOut,
+                     1L, m->Z, (long)kOutPad, (uint16_t *)nullptr, B, kOut, kHid, m->p + kOffB2, 0, 0);
+  hipLaunchKernelGGL(k_softmax_ce, dim3(B), dim3(256), 0, s, m->Z, (long)kOutPad, backward ? m->lab : nullptr, B,
                      backward ? m->dZ : nullptr, backward ? m->dZb : nullptr, backward ? m->lossr : nullptr, probs);
   if (!backward) return;
   hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(64), 0, s, m->lossr, B, m->loss);
   hipLaunchKernelGGL(k_fill_f32, dim3(nb(kParams, 256)), dim3(256), 0, s, m->g, kParams, 0.f);
   // dW2 = H^T dZ   (A = Hb viewed transposed: A(m=k_hid, k=b) = Hb[b*kHid + m])
-  hipLaunchKernelGGL(k_gemm, dim3(nb(kCls, 64), nb(kHid, 64)), dim3(256), 0, s, m->Hb, 1L, (long)kHid, m->dZb,
-                     (long)kClsPad, 1L, m->g + kOffW2, (long)kCls, (uint16_t *)nullptr, kHid, kCls, B,
+  hipLaunchKernelGGL(k_gemm, dim3(nb(kOut, 64), nb(kHid, 64)), dim3(256), 0, s, m->Hb, 1L, (long)kHid, m->dZb,
+                     (long)kOutPad, 1L, m->g + kOffW2, (long)kOut, (uint16_t *)nullptr, kHid, kOut, B,
                      (const float *)nullptr, 0, 0);
-  hipLaunchKernelGGL(k_colsum, dim3(nb(kCls, 256)), dim3(256), 0, s, m->dZ, (long)kClsPad, B, kCls, m->g + kOffB2);
-  // dH = dZ W2^T  (B(k=c, n=j) = W2[j*kCls + c])
-  hipLaunchKernelGGL(k_gemm, dim3(nb(kHid, 64), nb(B, 64)), dim3(256), 0, s, m->dZb, (long)kClsPad, 1L, W2b, 1L,
-                     (long)kCls, m->dH, (long)kHid, (uint16_t *)nullptr, B, kHid, kCls, (const float *)nullptr, 0, 0);
+  hipLaunchKernelGGL(k_colsum, dim3(nb(kOut, 256)), dim3(256), 0, s, m->dZ, (long)kOutPad, B, kOut, m->g + kOffB2);
+  // dH = dZ W2^T  (B(k=c, n=j) = W2[j*kOut + c])
+  hipLaunchKernelGGL(k_gemm, dim3(nb(kHid, 64), nb(B, 64)), dim3(256), 0, s, m->dZb, (long)kOutPad, 1L, W2b, 1L,
+                     (long)kOut, m->dH, (long)kHid, (uint16_t *)nullptr, B, kHid, kOut, (const float *)nullptr, 0, 0);
   hipLaunchKernelGGL(k_relu_back, dim3(nb((size_t)B * kHid, 256)), dim3(256), 0, s, m->dH, m->H, m->dHb, B * kHid);
   hipLaunchKernelGGL(k_colsum, dim3(nb(kHid, 256)), dim3(256), 0, s, m->dH, (long)kHid, B, kHid, m->g + kOffB1);
   // dW1 = X^T dH
@@ -353,7 +357,7 @@ int check_tokens(sx_mlp *m, const uint16_t *ctx, const uint16_t *labels, size_t 
     if (ctx[i] >= kTok) return sx_internal_error(m->h, SX_EINVAL, "context token out of range");
   if (labels)
     for (size_t i = 0; i < n; i++)
-      if (labels[i] >= kCls) return sx_internal_error(m->h, SX_EINVAL, "label out of range");
+      if (labels[i] >= kOut) return sx_internal_error(m->h, SX_EINVAL, "label out of range");
   return SX_OK;
 }
 
@@ -489,7 +493,7 @@ extern "C" int sx_mlp_predict(sx_mlp *m, const uint16_t *ctx, uint32_t n, float 
   if ((rc = alloc_work(m, (int)std::min<uint32_t>(n, 1024))) ) return rc;
   hipStream_t s = sx_internal_stream(m->h);
   float *pd = nullptr;
-  if (hipMalloc(&pd, (size_t)std::min<uint32_t>(n, 1024) * kCls * 4) != hipSuccess)
+  if (hipMalloc(&pd, (size_t)std::min<uint32_t>(n, 1024) * kOut * 4) != hipSuccess)
     return sx_internal_error(m->h, SX_ENOMEM, "predict buffer");
   if ((rc = upload_set(m, ctx, nullptr, n))) {
     (void)hipFree(pd);
@@ -501,7 +505,7 @@ extern "C" int sx_mlp_predict(sx_mlp *m, const uint16_t *ctx, uint32_t n, float 
                        m->step, 0u, 0u, m->pb + kOffE, m->X, m->tok, m->lab, B);
     enqueue_fwd_bwd(m, B, false, pd, s);
     if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(probs + (size_t)b0 * kCls, pd, (size_t)B * kCls * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(probs + (size_t)b0 * kOut, pd, (size_t)B * kOut * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
       (void)hipFree(pd);
       return sx_internal_error(m->h, SX_EHIP, "predict");
@@ -515,20 +519,20 @@ extern "C" int sx_proposal_weights(const float *probs, uint64_t n, double eps, u
   if (!probs || !out || !(eps > 0.0) || eps > 1.0) return SX_EINVAL;
   // floor(w * (2^32 - 2863)) + 1 per instruction sums to at most 2^32; the
   // remainder goes to the largest weight.
-  const double scale = 4294967296.0 - (double)kCls;
+  const double scale = 4294967296.0 - (double)kInsts;
   for (uint64_t r = 0; r < n; r++) {
-    const float *p = probs + r * kCls;
-    uint32_t *o = out + r * kCls;
+    const float *p = probs + r * kInsts;
+    uint32_t *o = out + r * kInsts;
     double tot = 0.0;
-    for (int i = 0; i < kCls; i++) {
+    for (int i = 0; i < kInsts; i++) {
       if (!(p[i] >= 0.f)) return SX_EINVAL;
       tot += p[i];
     }
     if (!(tot > 0.0)) return SX_EINVAL;
     uint64_t sum = 0;
     int best = 0;
-    for (int i = 0; i < kCls; i++) {
-      double w = (1.0 - eps) * (double)p[i] / tot + eps / (double)kCls;
+    for (int i = 0; i < kInsts; i++) {
+      double w = (1.0 - eps) * (double)p[i] / tot + eps / (double)kInsts;
       uint64_t u = (uint64_t)std::floor(w * scale) + 1;
       o[i] = (uint32_t)u;
       sum += u;
@@ -538,6 +542,23 @@ extern "C" int sx_proposal_weights(const float *probs, uint64_t n, double eps, u
     o[best] += (uint32_t)(4294967296ull - sum);
   }
   return SX_OK;
+}
+
+extern "C" int sx_mlp_proposal(sx_mlp *m, const uint16_t *ctx, uint32_t n, double eps, uint32_t *out) {
+  if (!m || !ctx || n == 0 || !out) return SX_EINVAL;
+  std::vector<float> probs((size_t)n * kOut), q((size_t)n * kInsts);
+  int rc = sx_mlp_predict(m, ctx, n, probs.data());
+  if (rc) return rc;
+  for (uint32_t r = 0; r < n; r++) {
+    float *qr = &q[(size_t)r * kInsts];
+    const float *pr = &probs[(size_t)r * kOut];
+    for (int i = 0; i < kInsts; i++) qr[i] = pr[i];
+    for (int j = 0; j < kCtx; j++) {
+      const uint16_t t = ctx[(size_t)r * kCtx + j];
+      if (t < kInsts) qr[t] += pr[kInsts + j];  // padding positions carry no instruction
+    }
+  }
+  return sx_proposal_weights(q.data(), n, eps, out);
 }
 
 // C = A B on the device for the MFMA layout test: A[M][K], B[K][N] bf16

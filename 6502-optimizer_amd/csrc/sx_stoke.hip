@@ -9,7 +9,9 @@
 // 102/52/51/25/26 out of 256, instruction draws from integer weights that sum
 // to 2^32 (uniform or learned), and Metropolis-Hastings acceptance with the
 // exact proposal ratio.  A chain whose accepted state matches the target on
-// all 32 vectors with fewer bytes posts it for verification; the host checks
+// all 32 vectors with fewer bytes posts it for verification (up to kPosts
+// distinct candidates per chain and launch, so one candidate that matches on
+// the vectors but fails the sweep does not hide the next); the host checks
 // posted candidates with the complete ExhaustiveCheck between launches.
 #include <hip/hip_runtime.h>
 
@@ -33,6 +35,7 @@ namespace {
 
 constexpr int kMaxSeq = 10;
 constexpr int kWave = 64;
+constexpr int kPosts = 4;  // post slots per chain
 
 __constant__ uint8_t s_vec[32][8] = SX_MID_VECTORS;
 
@@ -43,7 +46,7 @@ struct ChainState {
   double temp;
   uint64_t step;
   uint64_t accepted;
-  uint32_t posted;   // a candidate waits in the post slot
+  uint32_t posted;   // filled post slots (0..kPosts)
   uint32_t pad;
 };
 
@@ -93,6 +96,22 @@ __device__ __forceinline__ uint32_t go_inst_d(uint32_t i) {
 
 __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t n) { return __umulhi(a, n); }
 
+// The instruction whose cumulative weight interval holds r: the smallest i
+// with r < cum[i] (cum inclusive, cum[2862] = 2^32).  Two wave-wide steps
+// instead of a 12-step dependent binary search: lane k tests the last entry
+// of segment k (64 segments of 45), then lane j tests entry j of the first
+// segment that passed.  Called with all lanes active.
+__device__ __forceinline__ uint32_t draw_inst(const uint64_t *__restrict__ cum, uint32_t r, uint32_t lane) {
+  constexpr uint32_t kSeg = 45;  // 64 x 45 = 2,880 >= 2,863
+  const uint32_t e1 = min(lane * kSeg + kSeg - 1, (uint32_t)kNumInsts - 1);
+  const uint64_t b1 = __ballot((uint64_t)r < cum[e1]);
+  const uint32_t k0 = (uint32_t)__builtin_ctzll(b1);  // nonzero: the last entry is 2^32 > r
+  const uint32_t idx = k0 * kSeg + lane;
+  const bool in_seg = lane < kSeg && idx < (uint32_t)kNumInsts;
+  const uint64_t b2 = __ballot(in_seg && (uint64_t)r < cum[in_seg ? idx : 0]);
+  return uni(k0 * kSeg + (uint32_t)__builtin_ctzll(b2));
+}
+
 constexpr double W_REP = 102.0 / 256.0, W_DEL = 51.0 / 256.0, W_INS = 25.0 / 256.0, W_IMM = 26.0 / 256.0;
 
 // chains: n; targets: words [nt][8], lens, per-chain target index = c / cpt.
@@ -108,6 +127,7 @@ __global__ __launch_bounds__(256) void k_stoke(ChainState *chains, uint32_t n, u
                                                Post *posts) {
   __shared__ uint32_t xs[4][kMaxSeq + 1];
   __shared__ uint32_t ys[4][kMaxSeq + 1];
+  __shared__ uint32_t ps[4][kMaxSeq + 1];  // last posted candidate (words, then length)
   const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
   const uint32_t c = blockIdx.x * 4 + wv;
   if (c >= n) return;
@@ -141,6 +161,7 @@ __global__ __launch_bounds__(256) void k_stoke(ChainState *chains, uint32_t n, u
   uint64_t st = cs->step;
   uint64_t acc = cs->accepted;
   uint32_t posted = uni(cs->posted);
+  if (lane <= kMaxSeq) ps[wv][lane] = 0xFFFFFFFFu;  // no post yet in this launch
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
 
@@ -168,13 +189,7 @@ __global__ __launch_bounds__(256) void k_stoke(ChainState *chains, uint32_t n, u
     double ratio = 1.0;
     if (m < 102u) {
       pos = (int)mulhi32(r[1], (uint32_t)L);
-      // instruction with cumulative weight containing r[2]
-      uint32_t lo = 0, hi = kNumInsts - 1;
-      while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if ((uint64_t)r[2] < cumt[mid]) hi = mid; else lo = mid + 1;
-      }
-      vnew = go_inst_d(uni(lo));
+      vnew = go_inst_d(draw_inst(cumt, r[2], lane));
       uint32_t old = x[0];
 #pragma unroll
       for (int k = 1; k < kMaxSeq; k++) if (k == pos) old = x[k];
@@ -205,12 +220,7 @@ __global__ __launch_bounds__(256) void k_stoke(ChainState *chains, uint32_t n, u
     } else if (m < 230u) {
       if (L < max_len) {
         pos = (int)mulhi32(r[1], (uint32_t)(L + 1));
-        uint32_t lo = 0, hi = kNumInsts - 1;
-        while (lo < hi) {
-          uint32_t mid = (lo + hi) >> 1;
-          if ((uint64_t)r[2] < cumt[mid]) hi = mid; else lo = mid + 1;
-        }
-        vnew = go_inst_d(uni(lo));
+        vnew = go_inst_d(draw_inst(cumt, r[2], lane));
         ratio = W_DEL / (W_INS * (wt[go_index(vnew)] * (1.0 / 4294967296.0)));
         kind = 4;
         nl = L + 1;
@@ -275,14 +285,25 @@ __global__ __launch_bounds__(256) void k_stoke(ChainState *chains, uint32_t n, u
         L = nl;
         cost10 = c10;
         acc++;
-        if (h == 0 && bytes < tbytes && !posted) {
-          Post *pp = &posts[c];
-          if (lane < kMaxSeq) pp->seq[lane] = ys[wv][lane];
-          if (lane == 0) {
-            pp->len = nl;
-            pp->step = st;
+        if (h == 0 && bytes < tbytes && posted < (uint32_t)kPosts) {
+          // skip a repeat of the last posted candidate
+          const bool same_lane = lane < kMaxSeq ? (lane >= (uint32_t)nl || ps[wv][lane] == ys[wv][lane]) : true;
+          const bool repeat = __all(same_lane) && ps[wv][kMaxSeq] == (uint32_t)nl;
+          if (!repeat) {
+            Post *pp = &posts[(uint64_t)c * kPosts + posted];
+            if (lane < kMaxSeq) {
+              pp->seq[lane] = ys[wv][lane];
+              ps[wv][lane] = ys[wv][lane];
+            }
+            if (lane == 0) {
+              pp->len = nl;
+              pp->step = st;
+              ps[wv][kMaxSeq] = (uint32_t)nl;
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            posted++;
           }
-          posted = 1;
         }
       }
     }
@@ -367,7 +388,7 @@ extern "C" int sx_stoke_run(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_
   std::vector<uint32_t> w;
   std::vector<uint64_t> cumh;
   std::vector<ChainState> hch(nch);
-  std::vector<Post> hpost(nch);
+  std::vector<Post> hpost((size_t)nch * kPosts);
   std::vector<uint32_t> stoph(n, 0);
   auto t_start = std::chrono::steady_clock::now();
   uint64_t done_steps = 0;
@@ -390,7 +411,7 @@ extern "C" int sx_stoke_run(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_
     for (int i = 0; i < kNumInsts; i++) cumh[t * kNumInsts + i] = (s += w[t * kNumInsts + i]);
   }
   HCHK(hipMalloc(&d_ch, nch * sizeof(ChainState)));
-  HCHK(hipMalloc(&d_post, nch * sizeof(Post)));
+  HCHK(hipMalloc(&d_post, (size_t)nch * kPosts * sizeof(Post)));
   HCHK(hipMalloc(&d_tw, n * 8 * sizeof(uint32_t)));
   HCHK(hipMalloc(&d_tl, n));
   HCHK(hipMalloc(&d_w, ntab * kNumInsts * sizeof(uint32_t)));
@@ -426,19 +447,23 @@ extern "C" int sx_stoke_run(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_
       for (uint32_t c = 0; c < nch; c++)
         if (hch[c].posted) cand.push_back(c);
       if (!cand.empty()) {
-        HCHK(hipMemcpyAsync(hpost.data(), d_post, nch * sizeof(Post), hipMemcpyDeviceToHost, stream));
+        HCHK(hipMemcpyAsync(hpost.data(), d_post, (size_t)nch * kPosts * sizeof(Post), hipMemcpyDeviceToHost,
+                            stream));
         HCHK(hipStreamSynchronize(stream));
         std::vector<uint32_t> vt, vc;
         std::vector<uint8_t> vtl, vcl;
-        std::vector<uint32_t> ids;
+        std::vector<uint64_t> ids;  // post slot index c * kPosts + k
         for (uint32_t c : cand) {
           uint32_t ti = c / cfg->chains_per_target;
           if (stoph[ti]) continue;
-          ids.push_back(c);
-          for (int k = 0; k < kMaxSeq; k++) vt.push_back(k < lens[ti] ? targets[ti * 8 + k] : 0u);
-          for (int k = 0; k < kMaxSeq; k++) vc.push_back(k < hpost[c].len ? hpost[c].seq[k] : 0u);
-          vtl.push_back(lens[ti]);
-          vcl.push_back((uint8_t)hpost[c].len);
+          for (uint32_t k = 0; k < hch[c].posted && k < (uint32_t)kPosts; k++) {
+            const Post &pp = hpost[(size_t)c * kPosts + k];
+            ids.push_back((uint64_t)c * kPosts + k);
+            for (int q = 0; q < kMaxSeq; q++) vt.push_back(q < lens[ti] ? targets[ti * 8 + q] : 0u);
+            for (int q = 0; q < kMaxSeq; q++) vc.push_back(q < pp.len ? pp.seq[q] : 0u);
+            vtl.push_back(lens[ti]);
+            vcl.push_back((uint8_t)pp.len);
+          }
         }
         if (!ids.empty()) {
           std::vector<uint8_t> res(ids.size());
@@ -446,22 +471,31 @@ extern "C" int sx_stoke_run(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_
                               res.data());
           if (rc) goto done;
           double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+          // per target, the verified post with the smallest step (then chain) wins
+          std::vector<int64_t> win(n, -1);
           for (size_t i = 0; i < ids.size(); i++) {
-            uint32_t c = ids[i], ti = c / cfg->chains_per_target;
-            if (!res[i] || out[ti].verified) continue;
+            if (!res[i]) continue;
+            const uint32_t c = (uint32_t)(ids[i] / kPosts), ti = c / cfg->chains_per_target;
+            if (out[ti].verified) continue;
+            const int64_t b = win[ti];
+            if (b < 0 || hpost[ids[i]].step < hpost[(size_t)b].step) win[ti] = (int64_t)ids[i];
+          }
+          for (uint64_t ti = 0; ti < n; ti++) {
+            if (win[ti] < 0) continue;
+            const Post &pp = hpost[(size_t)win[ti]];
             sx_stoke_result &o = out[ti];
             o.verified = 1;
-            o.rep_len = (uint8_t)hpost[c].len;
-            for (int k = 0; k < kMaxSeq; k++) o.rep[k] = k < hpost[c].len ? hpost[c].seq[k] : 0u;
+            o.rep_len = (uint8_t)pp.len;
+            for (int k = 0; k < kMaxSeq; k++) o.rep[k] = k < pp.len ? pp.seq[k] : 0u;
             int tb = 0, tc = 0, cb = 0, cc = 0;
             for (int k = 0; k < lens[ti]; k++)
               tb += byte_size(targets[ti * 8 + k] & 0xFFFFu, 0), tc += cycles(targets[ti * 8 + k] & 0xFFFFu);
-            for (int k = 0; k < hpost[c].len; k++)
-              cb += byte_size(hpost[c].seq[k] & 0xFFFFu, 0), cc += cycles(hpost[c].seq[k] & 0xFFFFu);
+            for (int k = 0; k < pp.len; k++)
+              cb += byte_size(pp.seq[k] & 0xFFFFu, 0), cc += cycles(pp.seq[k] & 0xFFFFu);
             o.bytes_saved = (int16_t)(tb - cb);
             o.cycles_saved = (int16_t)(tc - cc);
-            o.chain = c;
-            o.step = hpost[c].step;
+            o.chain = (uint32_t)(win[ti] / kPosts);
+            o.step = pp.step;
             o.seconds = secs;
             if (cfg->stop_when_verified) {
               stoph[ti] = 1;

@@ -25,8 +25,9 @@ EXPORTS = ("sx_abi_version", "sx_open", "sx_close", "sx_last_error", "sx_enumera
            "sx_pruned_targets", "sx_check_pairs", "sx_flag_diff", "sx_exec_states", "sx_exh_plan",
            "sx_dep_table", "sx_stoke_run", "sx_mlp_param_count", "sx_mlp_create", "sx_mlp_destroy",
            "sx_mlp_get_params", "sx_mlp_set_params", "sx_mlp_loss_grad", "sx_mlp_train", "sx_mlp_predict",
-           "sx_proposal_weights", "sx_mfma_gemm_test")
-MLP_CLASSES = 2863
+           "sx_proposal_weights", "sx_mlp_proposal", "sx_mfma_gemm_test")
+MLP_CLASSES = 2863   # instructions
+MLP_OUTPUTS = 2871   # instructions, then 8 context-position pointers
 MLP_CTX = 8
 PAD_TOKEN = 2863
 
@@ -131,6 +132,7 @@ def lib() -> C.CDLL:
     L.sx_mlp_train.argtypes = [vp, u16p, u16p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, f32p]
     L.sx_mlp_predict.argtypes = [vp, u16p, C.c_uint32, f32p]
     L.sx_proposal_weights.argtypes = [f32p, C.c_uint64, C.c_double, u32p]
+    L.sx_mlp_proposal.argtypes = [vp, u16p, C.c_uint32, C.c_double, u32p]
     L.sx_mfma_gemm_test.argtypes = [vp, u16p, u16p, C.c_int, C.c_int, C.c_int, f32p]
     for name in EXPORTS:
         getattr(L, name).restype = C.c_int
@@ -407,7 +409,16 @@ class Mlp:
         return losses[:steps]
 
     def predict(self, ctx) -> np.ndarray:
+        """n x 2,871 softmax outputs (instructions, then context pointers)."""
         c = self._ctx(ctx)
-        out = np.zeros((len(c), MLP_CLASSES), np.float32)
+        out = np.zeros((len(c), MLP_OUTPUTS), np.float32)
         self.dev._check(lib().sx_mlp_predict(self._m, _p16(c), len(c), _pf(out)), "sx_mlp_predict")
+        return out
+
+    def proposal(self, ctx, eps: float = 0.1) -> np.ndarray:
+        """sx_mlp_proposal: per-context uint32 instruction tables (rows sum
+        to 2^32) with the pointer outputs folded onto their instructions."""
+        c = self._ctx(ctx)
+        out = np.zeros((len(c), MLP_CLASSES), np.uint32)
+        self.dev._check(lib().sx_mlp_proposal(self._m, _p16(c), len(c), float(eps), _p32(out)), "sx_mlp_proposal")
         return out

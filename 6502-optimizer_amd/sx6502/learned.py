@@ -11,10 +11,15 @@ exact ratio of these tables, so the chain's stationary distribution is the
 same as under uniform proposals -- only how fast it gets there changes.
 
 Training examples are (context, label): the context is the rewritten
-sequence as eight tokens (Go instruction index, 2863 = padding), the label
-one instruction of its verified replacement or, with ``labels="both"``, also
-one of the instructions the replacement drops (the deletions STOKE must
-accept).  Sources are the enumerator's rules (length 2 and 3 targets) and
+sequence as eight tokens (Go instruction index, 2863 = padding); the label is
+an output of the network -- an instruction index, or 2863 + j, the pointer
+to context position j.  An instruction of the sequence is labelled by its
+pointer, so the network learns which position to act on rather than having to
+reproduce an immediate.  ``labels="dropped"`` labels the instructions the
+verified replacement drops (the deletions a chain must accept, whose
+acceptance ratio is W_INS * q(v) / W_DEL), ``"rewrite"`` the replacement's
+instructions (what insert and replace moves must propose), ``"both"`` either.
+Sources are the enumerator's rules (length 2 and 3 targets) and
 the rewrites STOKE chains verify on sequences drawn like the evaluation
 targets but from other seeds.
 """
@@ -86,18 +91,29 @@ def stoke_pairs(targets: Sequence[Sequence[int]], results: np.ndarray) -> List[P
 
 
 def examples(pairs: Iterable[Pair], labels: str = "both", repeat: int = 1) -> Tuple[np.ndarray, np.ndarray]:
-    """Training examples of rewrite pairs: one per replacement instruction
-    and, for labels="both", one per instruction the replacement drops
-    (multiset difference source - replacement)."""
-    if labels not in ("both", "rewrite"):
+    """Training examples of rewrite pairs: one per instruction the
+    replacement drops (multiset difference source - replacement) and/or one
+    per replacement instruction, as `labels` selects."""
+    if labels not in ("both", "rewrite", "dropped"):
         raise ValueError(labels)
     ctx_rows: List[List[int]] = []
     lab: List[int] = []
     for src, rep in pairs:
-        ys = [INDEX[w] for w in rep]
-        if labels == "both":
+        used = [False] * len(src)
+
+        def label_of(w: int) -> int:
+            for j, x in enumerate(src):
+                if x == w and not used[j]:
+                    used[j] = True
+                    return core.MLP_CLASSES + j
+            return INDEX[w]
+
+        ys: List[int] = []
+        if labels != "rewrite":
             dropped = Counter(src) - Counter(rep)
-            ys += [INDEX[w] for w in dropped.elements()]
+            ys += [label_of(w) for w in dropped.elements()]
+        if labels != "dropped":
+            ys += [label_of(w) for w in rep]
         for y in ys:
             for _ in range(repeat):
                 ctx_rows.append(src)
@@ -107,20 +123,25 @@ def examples(pairs: Iterable[Pair], labels: str = "both", repeat: int = 1) -> Tu
     return tokens(ctx_rows), np.array(lab, dtype=np.uint16)
 
 
-def harvest_enumerator(dev: core.Device, l3_targets: Optional[np.ndarray] = None, dead: int = 0) -> List[Pair]:
-    """Rules of the full length-2 search and of the given length-3 targets."""
+def harvest_enumerator(dev: core.Device, l3_targets: Optional[np.ndarray] = None, dead: int = 0,
+                       l3_dead: int = 0x01) -> List[Pair]:
+    """Rules of the full length-2 search (dead flags `dead`) and of the given
+    length-3 targets (dead flags `l3_dead`)."""
     pairs = rule_pairs(dev.enumerate(None, target_len=2, max_cand_len=1, dead=dead)[0])
     if l3_targets is not None and len(l3_targets):
         pairs += rule_pairs(dev.enumerate(np.asarray(l3_targets, np.uint32), target_len=3, max_cand_len=2,
-                                          dead=dead)[0])
+                                          dead=l3_dead)[0])
     return pairs
 
 
-def harvest_stoke(dev: core.Device, targets: Sequence[Sequence[int]], *, chains: int = 16, steps: int = 200000,
-                  seed: int = 1, weights: Optional[np.ndarray] = None, dead: int = 0) -> List[Pair]:
-    """Verified rewrites STOKE chains find on the given targets."""
-    res = dev.stoke(targets, chains=chains, steps=steps, seed=seed, dead=dead, weights=weights,
-                    stop_when_verified=True)
+def harvest_stoke(dev: core.Device, targets: Sequence[Sequence[int]], *, chains: int = 4, steps: int = 20000,
+                  t0: float = 0.01, decay: float = 1.0, seed: int = 1, weights: Optional[np.ndarray] = None,
+                  dead: int = 0) -> List[Pair]:
+    """Verified rewrites STOKE chains find on the given targets (by default
+    cold chains: at T = 0.01 a chain started on its target keeps every
+    output and accepts the deletions that shorten it)."""
+    res = dev.stoke(targets, chains=chains, steps=steps, t0=t0, decay=decay, seed=seed, dead=dead,
+                    weights=weights, stop_when_verified=True)
     return stoke_pairs(targets, res)
 
 
@@ -136,4 +157,4 @@ def train_proposal(dev: core.Device, ctx: np.ndarray, labels: np.ndarray, *, ste
 
 def proposal_tables(m: core.Mlp, targets: Sequence[Sequence[int]], eps: float = 0.1) -> np.ndarray:
     """Per-target uint32 instruction tables (rows sum to 2^32) for sx_stoke_run."""
-    return core.proposal_weights(m.predict(tokens(targets)), eps)
+    return m.proposal(tokens(targets), eps)

@@ -57,6 +57,35 @@ def cpu_baseline(sample_targets: int, dead: int):
                       f"{dt * N_TARGETS / sample_targets:.0f} s"}
 
 
+def c3_section(dev):
+    """Length-3 batched configuration: 65,536 sampled length-3 targets (the
+    committed fixture) x all 8,199,632 length-1 and length-2 candidates,
+    --dead-flags 0x01, Go semantics; one warm-up and one timed run."""
+    t = np.load(os.path.join(ROOT, "tests", "golden", "c3_targets.npy"))
+    dev.enumerate(t, target_len=3, max_cand_len=2, dead=0x01)
+    t0 = time.perf_counter()
+    rules, st = dev.enumerate(t, target_len=3, max_cand_len=2, dead=0x01)
+    dt = time.perf_counter() - t0
+    n_c = 2_863 + 2_863 * 2_863
+    print(f"bench: C3 {dt:.3f} s, {len(rules)} rules", file=sys.stderr, flush=True)
+    return {"workload": "65,536 length-3 targets x 8,199,632 length-1/2 candidates, dead 0x01, Go semantics",
+            "wall_s": dt, "rules": int(len(rules)), "qc_pairs": len(t) * n_c,
+            "qc_pairs_per_s": len(t) * n_c / dt,
+            "stage_ms": {"qc": st["ms_qc"], "mid": st["ms_mid"], "exh": st["ms_exh"]}}
+
+
+def calib_copy(mib: int):
+    """A plain device copy of known size, for scaling HBM counters
+    (FETCH_SIZE / WRITE_SIZE) read under a profiler."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"bench: calibration copy {mib} MiB", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,12 +94,17 @@ def main():
     ap.add_argument("--dead", type=lambda s: int(s, 0), default=0x00)
     ap.add_argument("--cpu-sample", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the enumeration would shard by target range, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
+    if args.calib_copy:
+        calib_copy(args.calib_copy)
     dev = core.Device(0)
     for i in range(args.warmup):
         t0 = time.perf_counter()
@@ -113,6 +147,8 @@ def main():
         "qc_stage_pairs_per_s": PAIRS / (qc_ms * 1e-3),
         "roofline": roof,
     }
+    if not args.no_c3:
+        out["c3"] = c3_section(dev)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.dead)
     print(json.dumps(out))

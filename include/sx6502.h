@@ -215,15 +215,17 @@ int sx_dep_table(uint16_t out[58 * 16]);
  *
  * A two-layer MLP maps a target sequence (context: 8 tokens, token = the Go
  * instruction index 0..2862 of search.AllInstructions order, 2863 = padding)
- * to a distribution over the 2,863 instructions:
+ * to a distribution over 2,871 outputs: the 2,863 instructions, then 8
+ * pointers, output 2863 + j meaning "the instruction at context position j":
  *   X = [E[t0] .. E[t7]] (8 x 64 = 512), H = relu(X W1 + b1) (256),
- *   p = softmax(H W2 + b2) (2,863).
+ *   p = softmax(H W2 + b2) (2,871).
  * Parameters (fp32 master copy, row-major, in this order): E[2864][64],
- * W1[512][256], b1[256], W2[256][2863], b2[2863].  Products run on MFMA with
- * bf16 operands (E, W1, W2, H and the back-propagated dZ, dH rounded to
- * bf16, round-to-nearest-even) and fp32 accumulation; biases, softmax and
- * Adam in fp32. Training minimises the mean cross entropy of the label
- * (the Go index of one instruction of a known rewrite of the context). */
+ * W1[512][256], b1[256], W2[256][2871], b2[2871] (1,052,471 floats).
+ * Products run on MFMA with bf16 operands (E, W1, W2, H and the
+ * back-propagated dZ, dH rounded to bf16, round-to-nearest-even) and fp32
+ * accumulation; biases, softmax and Adam in fp32.  Training minimises the
+ * mean cross entropy of the label (an output index: an instruction of a
+ * known rewrite of the context, or the pointer to one of its positions). */
 typedef struct sx_mlp sx_mlp;
 
 uint64_t sx_mlp_param_count(void);
@@ -243,8 +245,12 @@ int sx_mlp_loss_grad(sx_mlp *m, const uint16_t *ctx, const uint16_t *labels, uin
  * batch loss per step. */
 int sx_mlp_train(sx_mlp *m, const uint16_t *ctx, const uint16_t *labels, uint32_t n, uint32_t steps,
                  uint32_t batch, float lr, uint64_t seed, float *losses);
-/* probs: n x 2,863 softmax outputs. */
+/* probs: n x 2,871 softmax outputs. */
 int sx_mlp_predict(sx_mlp *m, const uint16_t *ctx, uint32_t n, float *probs);
+/* Proposal tables of n targets for sx_stoke_cfg.weights: predict, add each
+ * pointer's probability to the instruction at its position (in position
+ * order, fp32; padding positions add nothing), then sx_proposal_weights. */
+int sx_mlp_proposal(sx_mlp *m, const uint16_t *ctx, uint32_t n, double eps, uint32_t *out);
 /* Proposal tables for sx_stoke_cfg.weights: row i = (1 - eps) * probs[i] +
  * eps / 2863 as uint32 weights summing to 2^32, every weight >= 1 (the
  * uniform fallback keeps every instruction reachable, so the chain stays

@@ -9,7 +9,9 @@ is the definition in include/sx6502.h:
 
     X = [E[t0] .. E[t7]]            8 tokens x 64-d embeddings = 512
     H = relu(X W1 + b1)             256
-    Z = H W2 + b2                   2,863 logits
+    Z = H W2 + b2                   2,871 logits: 2,863 instructions, then
+                                    8 pointers (output 2863 + j = "the
+                                    instruction at context position j")
     loss = mean_b -log softmax(Z_b)[y_b]
 
 with bf16 operands for the products (E, W1, W2, H and the back-propagated
@@ -24,11 +26,12 @@ import numpy as np
 N_TOK = 2864
 N_CLS = 2863
 CTX = 8
+N_OUT = N_CLS + CTX
 EMB = 64
 IN = CTX * EMB
 HID = 256
 
-SHAPES = [("E", (N_TOK, EMB)), ("W1", (IN, HID)), ("b1", (HID,)), ("W2", (HID, N_CLS)), ("b2", (N_CLS,))]
+SHAPES = [("E", (N_TOK, EMB)), ("W1", (IN, HID)), ("b1", (HID,)), ("W2", (HID, N_OUT)), ("b2", (N_OUT,))]
 N_PARAMS = sum(int(np.prod(s)) for _, s in SHAPES)
 
 
@@ -53,7 +56,7 @@ def pack(d: dict) -> np.ndarray:
 
 
 def forward(p: np.ndarray, ctx: np.ndarray, bf16_ops: bool = True, dtype=np.float32):
-    """Logits Z (n x 2863) plus the intermediates the backward pass needs."""
+    """Logits Z (n x 2871) plus the intermediates the backward pass needs."""
     w = unpack(np.asarray(p, dtype=dtype))
     rnd = (lambda a: bf16(a.astype(np.float32)).astype(dtype)) if bf16_ops else (lambda a: a)
     E, W1, W2 = rnd(w["E"]), rnd(w["W1"]), rnd(w["W2"])
@@ -109,7 +112,7 @@ def init_params(seed: int) -> np.ndarray:
     """The initial weights of sx_mlp_create: xorshift64 from seed * golden + 1."""
     mask = (1 << 64) - 1
     st = (seed * 0x9E3779B97F4A7C15 + 1) & mask
-    n_e, n_w1, n_w2 = N_TOK * EMB, IN * HID, HID * N_CLS
+    n_e, n_w1, n_w2 = N_TOK * EMB, IN * HID, HID * N_OUT
     draws = np.empty(n_e + n_w1 + n_w2, dtype=np.float64)
     for i in range(len(draws)):
         st ^= (st << 13) & mask
@@ -122,7 +125,7 @@ def init_params(seed: int) -> np.ndarray:
         "W1": r[n_e:n_e + n_w1] / np.float32(np.sqrt(np.float32(IN))),
         "b1": np.zeros(HID, np.float32),
         "W2": r[n_e + n_w1:] / np.float32(np.sqrt(np.float32(HID))),
-        "b2": np.zeros(N_CLS, np.float32),
+        "b2": np.zeros(N_OUT, np.float32),
     }
     return pack(w).astype(np.float32)
 
@@ -135,6 +138,20 @@ def batch_indices(step: int, batch: int, n: int, seed: int, philox) -> np.ndarra
         r = philox((step, b, 0x5354, 0), (seed & 0xFFFFFFFF, seed >> 32))
         out[b] = (int(r[0]) * n) >> 32
     return out
+
+
+def fold_pointers(probs: np.ndarray, ctx: np.ndarray) -> np.ndarray:
+    """Instruction distribution of sx_mlp_proposal: the 2,863 instruction
+    outputs plus, in position order and in fp32, each pointer's output on
+    the instruction at its context position (padding adds nothing)."""
+    probs = np.asarray(probs, dtype=np.float32)
+    q = probs[:, :N_CLS].copy()
+    for r in range(len(q)):
+        for j in range(CTX):
+            t = int(ctx[r][j])
+            if t < N_CLS:
+                q[r, t] = np.float32(q[r, t] + probs[r, N_CLS + j])
+    return q
 
 
 def proposal_weights(probs: np.ndarray, eps: float) -> np.ndarray:

@@ -48,7 +48,7 @@ def random_set(seed, n):
     rng = np.random.default_rng(seed)
     ctx = rng.integers(0, mlp.N_TOK, size=(n, mlp.CTX)).astype(np.uint16)
     ctx[: n // 3, 4:] = mlp.N_TOK - 1  # padded contexts
-    labels = rng.integers(0, mlp.N_CLS, size=n).astype(np.uint16)
+    labels = rng.integers(0, mlp.N_OUT, size=n).astype(np.uint16)
     p = rng.normal(0, 0.05, mlp.N_PARAMS).astype(np.float32)
     return p, ctx, labels
 
@@ -78,7 +78,7 @@ def test_loss_grad_matches_oracle(dev, n):
     assert max(errs.values()) < 2e-3, errs
     # and close to the exact float64 gradient
     _, g64 = mlp.loss_grad(p.astype(np.float64), ctx, labels, bf16_ops=False, dtype=np.float64)
-    assert max(block_rel_err(g, g64).values()) < 3e-2
+    assert max(block_rel_err(g, g64).values()) < 6e-2  # bf16 operands: ~2^-8 relative per rounding
     m.close()
 
 
@@ -121,6 +121,18 @@ def test_predict_matches_oracle(dev):
     m.close()
 
 
+def test_proposal_folds_pointers_like_the_oracle(dev):
+    p, ctx, _ = random_set(11, 300)
+    m = dev.mlp(seed=0)
+    m.set_params(p)
+    probs = m.predict(ctx)
+    for eps in (0.1, 0.02):
+        w = m.proposal(ctx, eps)
+        assert np.array_equal(w, mlp.proposal_weights(mlp.fold_pointers(probs, ctx), eps))
+        assert (w.sum(axis=1, dtype=np.uint64) == 2 ** 32).all()
+    m.close()
+
+
 def test_training_learns_a_copy_task(dev):
     """Label = the context's first token: the loss must fall far below the
     uniform log(2863) and the argmax must recover the token."""
@@ -145,5 +157,5 @@ def test_bad_tokens_are_rejected(dev):
     with pytest.raises(core.SxError):
         m.predict(ctx)
     with pytest.raises(core.SxError):
-        m.loss_grad(np.zeros((2, 8), np.uint16), np.array([0, 2863], np.uint16))
+        m.loss_grad(np.zeros((2, 8), np.uint16), np.array([0, 2871], np.uint16))
     m.close()

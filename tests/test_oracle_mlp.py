@@ -13,12 +13,12 @@ def small_problem(seed=0, n=6):
     p = rng.normal(0, 0.2, mlp.N_PARAMS)
     ctx = rng.integers(0, mlp.N_TOK, size=(n, mlp.CTX))
     ctx[0, 5:] = mlp.N_TOK - 1  # padding
-    labels = rng.integers(0, mlp.N_CLS, size=n)
+    labels = rng.integers(0, mlp.N_OUT, size=n)  # instructions and pointers
     return p, ctx, labels
 
 
 def test_param_layout():
-    assert mlp.N_PARAMS == 2864 * 64 + 512 * 256 + 256 + 256 * 2863 + 2863 == 1050415
+    assert mlp.N_PARAMS == 2864 * 64 + 512 * 256 + 256 + 256 * 2871 + 2871 == 1052471
     p = np.arange(mlp.N_PARAMS, dtype=np.float64)
     assert np.array_equal(mlp.pack(mlp.unpack(p)), p)
 
@@ -78,7 +78,7 @@ def test_bf16_gradient_close_to_exact():
     _, g64 = mlp.loss_grad(p, ctx, labels, bf16_ops=False, dtype=np.float64)
     _, g16 = mlp.loss_grad(p.astype(np.float32), ctx, labels, bf16_ops=True, dtype=np.float32)
     rel = np.linalg.norm(g16 - g64) / np.linalg.norm(g64)
-    assert rel < 3e-2, rel
+    assert rel < 6e-2, rel  # bf16 operands: ~2^-8 relative per rounding
 
 
 def test_adam_matches_torch():
@@ -122,3 +122,19 @@ def test_proposal_weights_library_matches_restatement():
         core.proposal_weights(-probs, 0.1)
     with pytest.raises(core.SxError):
         core.proposal_weights(probs, 0.0)
+
+
+def test_fold_pointers():
+    probs = np.zeros((2, mlp.N_OUT), np.float32)
+    probs[0, 5] = 0.25
+    probs[0, mlp.N_CLS + 0] = 0.5   # position 0 holds instruction 7
+    probs[0, mlp.N_CLS + 2] = 0.25  # position 2 is padding: dropped
+    probs[1, mlp.N_CLS + 1] = 1.0   # position 1 holds instruction 5
+    ctx = np.full((2, 8), mlp.N_TOK - 1)
+    ctx[0, 0], ctx[0, 1] = 7, 5
+    ctx[1, 0], ctx[1, 1] = 7, 5
+    q = mlp.fold_pointers(probs, ctx)
+    assert q.shape == (2, mlp.N_CLS)
+    assert q[0, 7] == 0.5 and q[0, 5] == 0.25 and q[0].sum() == 0.75
+    assert q[1, 5] == 1.0 and q[1].sum() == 1.0
+
