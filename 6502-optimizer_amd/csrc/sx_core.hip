@@ -55,14 +55,6 @@ constexpr uint32_t kRankNone = 0xFFFFFFFFu;
 constexpr int kQcQueue = 256;           // LDS survivor queue entries per wave
 constexpr int kEvalShards = 1024;       // shards of the evaluated-states counter
 
-// Go instruction order (pkg/search/enumerator.go:14-38): the 47 non-immediate
-// opcodes in id order, then the 11 immediate opcodes x 256.
-SX_HD uint32_t go_inst(uint32_t i) {
-  if (i < 26) return i;                       // TAX .. NOP
-  if (i < kNumNonImm) return i - 26 + LDA_M;  // LDA M .. BIT M
-  uint32_t k = i - kNumNonImm;
-  return (LDA_IMM + (k >> 8)) | ((k & 0xFFu) << 16);
-}
 
 // ------------------------------------------------------ device helpers --
 
@@ -74,38 +66,6 @@ __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (kWave - 1)
 
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-}
-
-// ShouldPrune, pkg/search/pruner.go:6-181.
-SX_HD bool should_prune(const uint32_t *seq, int n, const uint8_t *rd, const uint8_t *wr) {
-  for (int i = 0; i < n; i++) {
-    if ((seq[i] & 0xFFFFu) == NOP) return true;
-    if (i + 1 < n) {
-      uint32_t w = wr[seq[i] & 0xFFFFu];
-      if (w) {
-        uint32_t r2 = rd[seq[i + 1] & 0xFFFFu], w2 = wr[seq[i + 1] & 0xFFFFu];
-        if (w & w2 & ~(uint32_t)(GP | GS | GS0 | GS1) & ~r2) return true;
-      }
-    }
-  }
-  int depth = 0;
-  for (int i = 0; i < n; i++) {
-    uint32_t op = seq[i] & 0xFFFFu;
-    if (op == PHA || op == PHP) {
-      if (++depth > 2) return true;
-    } else if (op == PLA || op == PLP) {
-      if (depth <= 0) return true;
-      depth--;
-    }
-  }
-  for (int i = 0; i + 1 < n; i++) {
-    uint32_t a = seq[i] & 0xFFFFu, b = seq[i + 1] & 0xFFFFu;
-    uint32_t aR = rd[a], aW = wr[a], bR = rd[b], bW = wr[b];
-    bool indep = !((aW & bR) || (aR & bW) || (aW & bW));
-    uint32_t ka = (a << 16) | (seq[i] >> 16), kb = (b << 16) | (seq[i + 1] >> 16);
-    if (indep && ka > kb) return true;
-  }
-  return false;
 }
 
 __device__ __forceinline__ void run_seq_uniform(State &st, const uint32_t *w, int n) {
@@ -627,6 +587,85 @@ __global__ __launch_bounds__(256) void k_round_resolve(const Job *jobs, uint32_t
   rules[o] = r;
 }
 
+// First-hit tail in one batch.  After a few rounds most (target, pass)
+// entries are resolved; for the rest, every Mid survivor not yet tested
+// (rank >= rej) becomes a job at once, and the smallest passing Go rank is
+// the first hit (the rounds would reach the same candidate one rank at a
+// time).
+__global__ __launch_bounds__(256) void k_bulk_jobs(const uint2 *mid, uint32_t nm, const uint32_t *rank,
+                                                   const uint32_t *rej, const uint8_t *done,
+                                                   const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
+                                                   Job *jobs, uint32_t job_cap, uint32_t *njobs) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  bool take = false;
+  uint2 e = make_uint2(0, 0);
+  if (i < nm) {
+    e = mid[i];
+    uint32_t slot = e.y & 0x3FFFFFFFu, p = (e.y >> 30) - 1;
+    uint64_t k = (uint64_t)e.x * 2 + p;
+    take = !done[k] && rank[slot] >= rej[k];
+  }
+  uint64_t m = __ballot(take);
+  if (m == 0) return;
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(njobs, (uint32_t)__popcll(m));
+  base = uni(base);
+  if (take) {
+    uint32_t slot = e.y & 0x3FFFFFFFu, pass = e.y >> 30;
+    Job j;
+    fill_job(j, targets, tl, nullptr, 0, e.x, slot, pass, pass == 2 ? pm_masked : kPMask);
+    uint32_t o = base + mbcnt(m);
+    if (o < job_cap) jobs[o] = j;
+  }
+}
+
+// FlagDiff over the 8 QuickCheck vectors from the fingerprints: 0 when a
+// non-P byte differs is already excluded by the masked QuickCheck.
+__device__ __forceinline__ uint32_t fp_flag_diff(const uint2 *tfp, const uint2 *cfp, uint32_t t, uint32_t slot) {
+  const uint2 *a = tfp + (uint64_t)t * 32, *b = cfp + (uint64_t)slot * 32;
+  uint32_t fd = 0;
+  for (int v = 0; v < 8; v++) fd |= ((a[v].x ^ b[v].x) >> 24) & kPMask;
+  return fd;
+}
+
+__global__ __launch_bounds__(256) void k_bulk_min(const Job *jobs, uint32_t nj, const uint32_t *fail,
+                                                  const uint32_t *rank, const uint2 *tfp, const uint2 *cfp,
+                                                  uint32_t *best) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nj || fail[i]) return;
+  const Job &j = jobs[i];
+  if (j.pass == 2 && fp_flag_diff(tfp, cfp, j.target, j.slot) == 0) return;  // worker.go:211-213
+  atomicMin(&best[(uint64_t)j.target * 2 + (j.pass - 1)], rank[j.slot]);
+}
+
+// Rules of the (target, pass) entries the tail resolved; rank == slot.
+__global__ __launch_bounds__(256) void k_bulk_emit(const uint32_t *best, uint32_t nt, const uint32_t *targets,
+                                                   uint32_t tl, const uint2 *tfp, const uint2 *cfp,
+                                                   const uint8_t *tbytes, const uint8_t *tcyc,
+                                                   const uint8_t *cbytes, const uint8_t *ccyc, uint64_t tbase,
+                                                   sx_rule *rules, uint32_t rule_cap, uint32_t *nrules) {
+  uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= 2 * nt || best[k] == kRankNone) return;
+  const uint32_t t = k >> 1, pass = (k & 1) + 1, slot = best[k];
+  Job j;
+  fill_job(j, targets, tl, nullptr, 0, t, slot, pass, 0);
+  uint32_t o = atomicAdd(nrules, 1u);
+  if (o >= rule_cap) return;
+  sx_rule r;
+  for (int q = 0; q < 4; q++) {
+    r.src[q] = j.tw[q];
+    r.rep[q] = j.cw[q];
+  }
+  r.target_index = tbase + t;
+  r.src_len = j.tl;
+  r.rep_len = j.cl;
+  r.pass = (uint8_t)pass;
+  r.dead_flags = (uint8_t)(pass == 2 ? fp_flag_diff(tfp, cfp, t, slot) : 0);
+  r.bytes_saved = (int16_t)((int)tbytes[t] - (int)cbytes[slot]);
+  r.cycles_saved = (int16_t)((int)tcyc[t] - (int)ccyc[slot]);
+  rules[o] = r;
+}
+
 // Bulk QuickCheck / MidCheck / FlagDiff for sx_check_pairs: one wave per pair.
 __global__ __launch_bounds__(256) void k_pair_vectors(const uint32_t *tw, const uint8_t *tl,
                                                       const uint32_t *cw, const uint8_t *cl,
@@ -724,6 +763,11 @@ struct sx_handle {
   int device = 0;
   int num_cu = 256;
   bool poisoned = false;  // a device stage timed out: the stream is unusable
+  int bulk_after = 2;     // first-hit rounds before the tail runs as one batch (SX_BULK_AFTER)
+  // kernel-only timing of the QuickCheck and exhaustive-sweep launches
+  hipEvent_t kev[2] = {nullptr, nullptr};
+  double k_qc_ms = 0, k_exh_ms = 0;
+  uint64_t k_qc_launches = 0, k_exh_launches = 0;
   hipStream_t stream = nullptr;
   std::string err;
   std::vector<sx_rule> rules;
@@ -876,9 +920,6 @@ int ensure_l2(sx_handle *h) {
   return SX_OK;
 }
 
-struct Timer {
-  hipEvent_t a = nullptr, b = nullptr;
-};
 
 // Run the exhaustive stage on nj jobs already in B_JOBS; the verdicts are in
 // B_FAIL (0 = pass).  *chunks_out gets the number of chunks (statistics).
@@ -907,13 +948,17 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, ui
   const uint64_t slice = 1ull << 20;  // chunks per launch (2^34 states)
   for (uint64_t c0 = 0; c0 < tot; c0 += slice) {
     uint64_t n = std::min<uint64_t>(slice, tot - c0);
+    HIPCHK(h, hipEventRecord(h->kev[0], h->stream));
     hipLaunchKernelGGL(k_exh_run, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, h->stream, jobs, nj, offs, c0, n,
                        fail, evald);
     HIPCHK(h, hipGetLastError());
-    if (c0 + slice < tot) {
-      rc = sync_wd(h, "exhaustive sweep");
-      if (rc) return rc;
-    }
+    HIPCHK(h, hipEventRecord(h->kev[1], h->stream));
+    rc = sync_wd(h, "exhaustive sweep");
+    if (rc) return rc;
+    float ms = 0;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->kev[0], h->kev[1]));
+    h->k_exh_ms += ms;
+    h->k_exh_launches++;
   }
   if (states_out) {
     std::vector<unsigned long long> shards(kEvalShards);
@@ -950,7 +995,9 @@ extern "C" int sx_open(int device, sx_handle **out) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->num_cu = prop.multiProcessorCount;
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (const char *e = getenv("SX_BULK_AFTER")) h->bulk_after = std::max(0, atoi(e));
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&h->kev[0]) != hipSuccess || hipEventCreate(&h->kev[1]) != hipSuccess) {
     delete h;
     return SX_EHIP;
   }
@@ -965,6 +1012,8 @@ extern "C" int sx_close(sx_handle *h) {
   for (auto &b : h->bufs)
     if (b.p) (void)hipFree(b.p);
   if (h->d_l2) (void)hipFree(h->d_l2);
+  for (hipEvent_t e : h->kev)
+    if (e) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return SX_OK;
@@ -1009,6 +1058,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   if (!cfg->targets && tl != 2) return set_err(h, SX_EINVAL, "implicit targets need target_len 2");
   HIPCHK(h, hipSetDevice(h->device));
   auto wall0 = std::chrono::steady_clock::now();
+  const double qc_ms0 = h->k_qc_ms, exh_ms0 = h->k_exh_ms;
+  const uint64_t qc_l0 = h->k_qc_launches, exh_l0 = h->k_exh_launches;
   sx_stats st;
   memset(&st, 0, sizeof(st));
   h->rules.clear();
@@ -1114,11 +1165,19 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       ENSURE(B_SURV, uint2, surv_cap, surv);
       HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));
       dim3 grid(blocks_for(nc, 256), (unsigned)((nt + tile - 1) / tile));
+      HIPCHK(h, hipEventRecord(h->kev[0], h->stream));
       hipLaunchKernelGGL(k_quickcheck, grid, dim3(256), 0, h->stream, tv0, tfp, (uint32_t)nt, tile, cfp,
                          (uint32_t)nc, pm_masked, surv, surv_cap, counters);
       HIPCHK(h, hipGetLastError());
+      HIPCHK(h, hipEventRecord(h->kev[1], h->stream));
       HIPCHK(h, hipMemcpyAsync(&nsurv, counters, 4, hipMemcpyDeviceToHost, h->stream));
       SYNC_WD("device sync");
+      {
+        float ms = 0;
+        HIPCHK(h, hipEventElapsedTime(&ms, h->kev[0], h->kev[1]));
+        h->k_qc_ms += ms;
+        h->k_qc_launches++;
+      }
       trace(h, "segment %llu: %llu targets, QuickCheck survivors %u (cap %u)", (unsigned long long)s0,
             (unsigned long long)nt, nsurv, surv_cap);
       if (nsurv <= surv_cap) break;
@@ -1159,7 +1218,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     ENSURE(B_CUR, uint32_t, nt * 2, cur);
     ENSURE(B_REJ, uint32_t, nt * 2, rej);
     ENSURE(B_DONE, uint8_t, nt * 2, done);
-    const uint32_t job_cap = std::max<uint32_t>(cfg->hits ? nmid : (uint32_t)std::min<uint64_t>(nmid, 2 * nt), 1);
+    const uint32_t job_cap = std::max<uint32_t>(nmid, 1);  // the first-hit tail may queue every survivor
     ENSURE(B_JOBS, Job, job_cap, jobs);
     const uint32_t rule_cap = cfg->hits ? std::max<uint32_t>(nmid, 1) : (uint32_t)(2 * nt);
     ENSURE(B_RULES, sx_rule, rule_cap, rules);
@@ -1168,9 +1227,37 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     HIPCHK(h, hipMemsetAsync(done, 0, nt * 2, h->stream));
     uint32_t *nrules_d = counters + 2, *njobs_d = counters + 3;
     HIPCHK(h, hipMemsetAsync(nrules_d, 0, 4, h->stream));
+    const int bulk_after = h->bulk_after;  // rounds before the first-hit tail runs in one batch
     for (int round = 0; nmid > 0; round++) {
       // every round rejects or resolves one candidate per open (target, pass)
       if ((uint64_t)round > nc + 1) return set_err(h, SX_EHIP, "first-hit rounds did not converge");
+      if (!cfg->hits && round >= bulk_after) {
+        HIPCHK(h, hipMemsetAsync(njobs_d, 0, 4, h->stream));
+        hipLaunchKernelGGL(k_bulk_jobs, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
+                           rej, done, tw, (uint32_t)tl, pm_masked, jobs, job_cap, njobs_d);
+        HIPCHK(h, hipGetLastError());
+        uint32_t nj = 0;
+        HIPCHK(h, hipMemcpyAsync(&nj, njobs_d, 4, hipMemcpyDeviceToHost, h->stream));
+        SYNC_WD("device sync");
+        if (nj == 0) break;
+        if (nj > job_cap) return set_err(h, SX_ENOMEM, "job buffer overflow (%u > %u)", nj, job_cap);
+        trace(h, "  first-hit tail: %u jobs", nj);
+        st.exh_rounds++;
+        st.exh_pairs += nj;
+        uint64_t chunks = 0, evaluated = 0;
+        int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, &evaluated);
+        if (rc) return rc;
+        st.exh_states += evaluated;
+        // cur holds kRankNone for every entry after a resolve: reuse it for the minimum
+        hipLaunchKernelGGL(k_bulk_min, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream,
+                           (const Job *)h->bufs[B_JOBS].p, nj, (const uint32_t *)h->bufs[B_FAIL].p, rank, tfp,
+                           cfp, cur);
+        hipLaunchKernelGGL(k_bulk_emit, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur,
+                           (uint32_t)nt, tw, (uint32_t)tl, tfp, cfp, tbytes, tcyc, cbytes, ccyc, tbase + s0, rules,
+                           rule_cap, nrules_d);
+        HIPCHK(h, hipGetLastError());
+        break;
+      }
       HIPCHK(h, hipMemsetAsync(njobs_d, 0, 4, h->stream));
       if (!cfg->hits)
         hipLaunchKernelGGL(k_round_min, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
@@ -1237,6 +1324,10 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   });
   st.rules = h->rules.size();
   st.ms_setup = ms_setup;
+  st.ms_qc_kernel = h->k_qc_ms - qc_ms0;
+  st.ms_exh_kernel = h->k_exh_ms - exh_ms0;
+  st.qc_launches = h->k_qc_launches - qc_l0;
+  st.exh_launches = h->k_exh_launches - exh_l0;
   st.ms_qc = ms_qc;
   st.ms_mid = ms_mid;
   st.ms_exh = ms_exh;

@@ -45,6 +45,15 @@ enum GoReg : uint32_t { GA = 1, GP = 2, GX = 4, GY = 8, GS = 16, GM = 32, GS0 = 
 
 SX_HD bool has_imm(uint32_t op) { return op >= LDA_IMM && op <= CPY_IMM; }
 
+// Go instruction order (pkg/search/enumerator.go:14-38): the 47 non-immediate
+// opcodes in id order, then the 11 immediate opcodes x 256.  Index -> word.
+SX_HD uint32_t go_inst(uint32_t i) {
+  if (i < 26) return i;                       // TAX .. NOP
+  if (i < kNumNonImm) return i - 26 + LDA_M;  // LDA M .. BIT M
+  uint32_t k = i - kNumNonImm;
+  return (LDA_IMM + (k >> 8)) | ((k & 0xFFu) << 16);
+}
+
 // Byte sizes: model 0 = Go (pkg/inst/catalog.go:50-56, the 21 M ops count one
 // byte), model 1 = the CUDA tool (cuda/6502_search_v2.cu:437-440).
 SX_HD int byte_size(uint32_t op, int model) {
@@ -79,6 +88,38 @@ SX_HD int cycles(uint32_t op) {
    GP, GP, GA | GP, GX | GP, GY | GP, GM, GM, GM, GA | GP, GA | GP, GA | GP,     \
    GA | GP, GA | GP, GP, GP, GP, GM | GP, GM | GP, GM | GP, GM | GP, GM | GP,    \
    GM | GP, GP}
+
+// ShouldPrune, pkg/search/pruner.go:6-181 (rd / wr: the two tables above).
+SX_HD bool should_prune(const uint32_t *seq, int n, const uint8_t *rd, const uint8_t *wr) {
+  for (int i = 0; i < n; i++) {
+    if ((seq[i] & 0xFFFFu) == NOP) return true;
+    if (i + 1 < n) {
+      uint32_t w = wr[seq[i] & 0xFFFFu];
+      if (w) {
+        uint32_t r2 = rd[seq[i + 1] & 0xFFFFu], w2 = wr[seq[i + 1] & 0xFFFFu];
+        if (w & w2 & ~(uint32_t)(GP | GS | GS0 | GS1) & ~r2) return true;
+      }
+    }
+  }
+  int depth = 0;
+  for (int i = 0; i < n; i++) {
+    uint32_t op = seq[i] & 0xFFFFu;
+    if (op == PHA || op == PHP) {
+      if (++depth > 2) return true;
+    } else if (op == PLA || op == PLP) {
+      if (depth <= 0) return true;
+      depth--;
+    }
+  }
+  for (int i = 0; i + 1 < n; i++) {
+    uint32_t a = seq[i] & 0xFFFFu, b = seq[i + 1] & 0xFFFFu;
+    uint32_t aR = rd[a], aW = wr[a], bR = rd[b], bW = wr[b];
+    bool indep = !((aW & bR) || (aR & bW) || (aW & bW));
+    uint32_t ka = (a << 16) | (seq[i] >> 16), kb = (b << 16) | (seq[i + 1] >> 16);
+    if (indep && ka > kb) return true;
+  }
+  return false;
+}
 
 // Data dependencies per opcode at flag granularity.  Inputs and outputs are
 // numbered: state bytes A=0, X=1, Y=2, S=4, M=5, S0=6, S1=7 (position 3, the

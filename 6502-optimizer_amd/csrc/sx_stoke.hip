@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <set>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -96,21 +97,41 @@ __device__ __forceinline__ uint32_t go_inst_d(uint32_t i) {
 
 __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t n) { return __umulhi(a, n); }
 
-// The instruction whose cumulative weight interval holds r: the smallest i
-// with r < cum[i] (cum inclusive, cum[2862] = 2^32).  Two wave-wide steps
-// instead of a 12-step dependent binary search: lane k tests the last entry
-// of segment k (64 segments of 45), then lane j tests entry j of the first
-// segment that passed.  Called with all lanes active.
-__device__ __forceinline__ uint32_t draw_inst(const uint64_t *__restrict__ cum, uint32_t r, uint32_t lane) {
-  constexpr uint32_t kSeg = 45;  // 64 x 45 = 2,880 >= 2,863
-  const uint32_t e1 = min(lane * kSeg + kSeg - 1, (uint32_t)kNumInsts - 1);
-  const uint64_t b1 = __ballot((uint64_t)r < cum[e1]);
-  const uint32_t k0 = (uint32_t)__builtin_ctzll(b1);  // nonzero: the last entry is 2^32 > r
-  const uint32_t idx = k0 * kSeg + lane;
-  const bool in_seg = lane < kSeg && idx < (uint32_t)kNumInsts;
-  const uint64_t b2 = __ballot(in_seg && (uint64_t)r < cum[in_seg ? idx : 0]);
-  return uni(k0 * kSeg + (uint32_t)__builtin_ctzll(b2));
-}
+// Instruction tables.  The uniform table is 2^32 / 2863 per instruction, one
+// more for the first 2^32 mod 2863, so both its draw and its weights are
+// closed forms and need no memory.  A per-target table is searched in two
+// wave-wide steps instead of a 12-step dependent binary search: lane k keeps
+// the last cumulative weight of segment k (64 segments of 45) in a register
+// for the whole launch, so a ballot finds the segment, and one coalesced load
+// of that segment and a second ballot find the entry.
+constexpr uint32_t kSeg = 45;  // 64 x 45 = 2,880 >= 2,863
+constexpr uint32_t kUniBase = (uint32_t)(4294967296ull / kNumInsts);
+constexpr uint32_t kUniExtra = (uint32_t)(4294967296ull % kNumInsts);
+
+struct Table {
+  const uint64_t *cum;  // inclusive cumulative weights (cum[2862] = 2^32), null = uniform
+  const uint32_t *w;
+  uint64_t pivot;       // this lane's segment end
+
+  // the smallest i with r < cum[i]; called with all lanes active
+  __device__ __forceinline__ uint32_t draw(uint32_t r, uint32_t lane) const {
+    if (!cum) {
+      const uint64_t hi = (uint64_t)kUniExtra * (kUniBase + 1);  // draws below hi land on a heavier entry
+      return (uint64_t)r < hi ? r / (kUniBase + 1) : kUniExtra + (uint32_t)(((uint64_t)r - hi) / kUniBase);
+    }
+    const uint64_t b1 = __ballot((uint64_t)r < pivot);
+    const uint32_t k0 = (uint32_t)__builtin_ctzll(b1);  // nonzero: the last entry is 2^32 > r
+    const uint32_t idx = k0 * kSeg + lane;
+    const bool in_seg = lane < kSeg && idx < (uint32_t)kNumInsts;
+    const uint64_t b2 = __ballot(in_seg && (uint64_t)r < cum[in_seg ? idx : 0]);
+    return uni(k0 * kSeg + (uint32_t)__builtin_ctzll(b2));
+  }
+  // q(i) = weight / 2^32
+  __device__ __forceinline__ double q(uint32_t i) const {
+    const uint32_t wi = cum ? w[i] : kUniBase + (i < kUniExtra ? 1u : 0u);
+    return wi * (1.0 / 4294967296.0);
+  }
+};
 
 constexpr double W_REP = 102.0 / 256.0, W_DEL = 51.0 / 256.0, W_INS = 25.0 / 256.0, W_IMM = 26.0 / 256.0;
 
@@ -151,8 +172,10 @@ __global__ __launch_bounds__(256) void k_stoke(ChainState *chains, uint32_t n, u
   for (uint32_t k = 0; k < tl; k++) step(to, tw[k] & 0xFFFFu, tw[k] >> 16);
   const uint32_t tlo = pack_lo(to), thi = pack_hi(to);
   const uint32_t plo = 0x00FFFFFFu | (pmask << 24);
-  const uint64_t *cumt = cum + (per_target ? (uint64_t)ti * kNumInsts : 0);
-  const uint32_t *wt = weights + (per_target ? (uint64_t)ti * kNumInsts : 0);
+  Table tab;
+  tab.cum = per_target ? cum + (uint64_t)ti * kNumInsts : nullptr;
+  tab.w = per_target ? weights + (uint64_t)ti * kNumInsts : nullptr;
+  tab.pivot = per_target ? tab.cum[min(lane * kSeg + kSeg - 1, (uint32_t)kNumInsts - 1)] : 0;
 
   if (lane < kMaxSeq) xs[wv][lane] = cs->seq[lane];
   int L = (int)uni((uint32_t)cs->len);
@@ -189,13 +212,13 @@ __global__ __launch_bounds__(256) void k_stoke(ChainState *chains, uint32_t n, u
     double ratio = 1.0;
     if (m < 102u) {
       pos = (int)mulhi32(r[1], (uint32_t)L);
-      vnew = go_inst_d(draw_inst(cumt, r[2], lane));
+      vnew = go_inst_d(tab.draw(r[2], lane));
       uint32_t old = x[0];
 #pragma unroll
       for (int k = 1; k < kMaxSeq; k++) if (k == pos) old = x[k];
       if (vnew != old) {
         double same = ((vnew & 0xFFFFu) == (old & 0xFFFFu) && has_imm(old & 0xFFFFu)) ? W_IMM / (nimm * 256.0) : 0.0;
-        double qo = wt[go_index(old)] * (1.0 / 4294967296.0), qn = wt[go_index(vnew)] * (1.0 / 4294967296.0);
+        double qo = tab.q(go_index(old)), qn = tab.q(go_index(vnew));
         ratio = (W_REP * qo / L + same) / (W_REP * qn / L + same);
         kind = 1;
       }
@@ -213,15 +236,15 @@ __global__ __launch_bounds__(256) void k_stoke(ChainState *chains, uint32_t n, u
         uint32_t old = x[0];
 #pragma unroll
         for (int k = 1; k < kMaxSeq; k++) if (k == pos) old = x[k];
-        ratio = W_INS * (wt[go_index(old)] * (1.0 / 4294967296.0)) / W_DEL;
+        ratio = W_INS * tab.q(go_index(old)) / W_DEL;
         kind = 3;
         nl = L - 1;
       }
     } else if (m < 230u) {
       if (L < max_len) {
         pos = (int)mulhi32(r[1], (uint32_t)(L + 1));
-        vnew = go_inst_d(draw_inst(cumt, r[2], lane));
-        ratio = W_DEL / (W_INS * (wt[go_index(vnew)] * (1.0 / 4294967296.0)));
+        vnew = go_inst_d(tab.draw(r[2], lane));
+        ratio = W_DEL / (W_INS * tab.q(go_index(vnew)));
         kind = 4;
         nl = L + 1;
       }
@@ -242,7 +265,7 @@ __global__ __launch_bounds__(256) void k_stoke(ChainState *chains, uint32_t n, u
         vnew = (old & 0xFFFFu) | ((r[3] >> 24) << 16);
         if (vnew != old) {
           double same = W_IMM / (nimm * 256.0);
-          double qo = wt[go_index(old)] * (1.0 / 4294967296.0), qn = wt[go_index(vnew)] * (1.0 / 4294967296.0);
+          double qo = tab.q(go_index(old)), qn = tab.q(go_index(vnew));
           ratio = (W_REP * qo / L + same) / (W_REP * qn / L + same);
           kind = 1;
         }
@@ -453,11 +476,15 @@ extern "C" int sx_stoke_run(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_
         std::vector<uint32_t> vt, vc;
         std::vector<uint8_t> vtl, vcl;
         std::vector<uint64_t> ids;  // post slot index c * kPosts + k
+        std::set<std::vector<uint32_t>> seen;  // (target, candidate) pairs already queued this launch
         for (uint32_t c : cand) {
           uint32_t ti = c / cfg->chains_per_target;
-          if (stoph[ti]) continue;
+          if (stoph[ti] || out[ti].verified) continue;  // only a target's first verified rewrite is reported
           for (uint32_t k = 0; k < hch[c].posted && k < (uint32_t)kPosts; k++) {
             const Post &pp = hpost[(size_t)c * kPosts + k];
+            std::vector<uint32_t> key(pp.seq, pp.seq + pp.len);
+            key.push_back(ti);
+            if (!seen.insert(key).second) continue;
             ids.push_back((uint64_t)c * kPosts + k);
             for (int q = 0; q < kMaxSeq; q++) vt.push_back(q < lens[ti] ? targets[ti * 8 + q] : 0u);
             for (int q = 0; q < kMaxSeq; q++) vc.push_back(q < pp.len ? pp.seq[q] : 0u);

@@ -351,37 +351,44 @@ int cmd_target(int argc, char **argv) {
   for (uint32_t w : seq) tb += byte_size(w & 0xFFFF, 0), tc += cycles(w & 0xFFFF);
   printf("Target: %s (%d bytes, %d cycles)\n", input.c_str(), tb, tc);
   int max_cand = std::atoi(a.get("max-candidate", "4").c_str());
-  if (seq.size() < 2 || seq.size() > 3) return die("this build searches targets of length 2 or 3");
-  int ml = std::min<int>(max_cand, (int)seq.size() - 1);
+  if (seq.empty() || seq.size() > 8) return die("target must have 1..8 instructions");
+  if (max_cand < 1 || max_cand > 4) return die("--max-candidate must be 1..4");
   sx_handle *h = nullptr;
   if (check(nullptr, sx_open(std::atoi(a.get("device", "0").c_str()), &h), "sx_open")) return 1;
-  sx_enum_cfg cfg;
-  memset(&cfg, 0, sizeof(cfg));
-  cfg.struct_size = sizeof(cfg);
-  cfg.target_len = (int)seq.size();
-  cfg.max_cand_len = ml;
-  cfg.use_mid = 1;
-  cfg.dead_flags = dead;
-  cfg.targets = seq.data();
-  cfg.n_targets = 1;
-  uint64_t n = 0;
-  sx_stats st;
-  if (check(h, sx_enumerate(h, &cfg, nullptr, 0, &n, &st), "sx_enumerate")) return 1;
-  std::vector<sx_rule> rules(n);
-  uint64_t got;
-  sx_fetch_rules(h, rules.data(), 0, n, &got);
+  sx_target_result res[2];
+  uint64_t scanned = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  if (check(h, sx_target_search(h, seq.data(), (int)seq.size(), max_cand, dead, 0, res, &scanned),
+            "sx_target_search"))
+    return 1;
+  double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   sx_close(h);
-  if (rules.empty()) {
+  if (a.has("verbose")) {
+    for (const sx_target_result &r : res)
+      if (r.found) {
+        if (r.pass == 2)
+          printf("  FOUND (dead flags 0x%02X): %s -> %s (-%d bytes, -%d cycles)\n", r.dead_flags, input.c_str(),
+                 disasm_seq(r.rep, r.rep_len).c_str(), r.bytes_saved, r.cycles_saved);
+        else
+          printf("  FOUND: %s -> %s (-%d bytes, -%d cycles)\n", input.c_str(), disasm_seq(r.rep, r.rep_len).c_str(),
+                 r.bytes_saved, r.cycles_saved);
+      }
+    fprintf(stderr, "  %llu candidates in %.3f s\n", (unsigned long long)scanned, secs);
+  }
+  // Table.Rules() order (pkg/result/table.go:40-50): bytes saved, then cycles saved, descending
+  const sx_target_result *best = nullptr;
+  for (const sx_target_result &r : res) {
+    if (!r.found) continue;
+    if (!best || r.bytes_saved > best->bytes_saved ||
+        (r.bytes_saved == best->bytes_saved && r.cycles_saved > best->cycles_saved))
+      best = &r;
+  }
+  if (!best) {
     printf("No shorter replacement found.\n");
     return 0;
   }
-  std::stable_sort(rules.begin(), rules.end(), [](const sx_rule &x, const sx_rule &y) {
-    if (x.bytes_saved != y.bytes_saved) return x.bytes_saved > y.bytes_saved;
-    return x.cycles_saved > y.cycles_saved;
-  });
-  const sx_rule &r = rules[0];
-  printf("Replacement: %s (-%d bytes, -%d cycles)\n", disasm_seq(r.rep, r.rep_len).c_str(), r.bytes_saved,
-         r.cycles_saved);
+  printf("Replacement: %s (-%d bytes, -%d cycles)\n", disasm_seq(best->rep, best->rep_len).c_str(),
+         best->bytes_saved, best->cycles_saved);
   return 0;
 }
 

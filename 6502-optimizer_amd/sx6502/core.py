@@ -25,7 +25,7 @@ EXPORTS = ("sx_abi_version", "sx_open", "sx_close", "sx_last_error", "sx_enumera
            "sx_pruned_targets", "sx_check_pairs", "sx_flag_diff", "sx_exec_states", "sx_exh_plan",
            "sx_dep_table", "sx_stoke_run", "sx_mlp_param_count", "sx_mlp_create", "sx_mlp_destroy",
            "sx_mlp_get_params", "sx_mlp_set_params", "sx_mlp_loss_grad", "sx_mlp_train", "sx_mlp_predict",
-           "sx_proposal_weights", "sx_mlp_proposal", "sx_mfma_gemm_test")
+           "sx_proposal_weights", "sx_mlp_proposal", "sx_mfma_gemm_test", "sx_target_search")
 MLP_CLASSES = 2863   # instructions
 MLP_OUTPUTS = 2871   # instructions, then 8 context-position pointers
 MLP_CTX = 8
@@ -57,7 +57,9 @@ class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("targets", "qc_pairs", "qc_raw_hits", "qc_hits", "mid_pass",
                                           "exh_pairs", "exh_states", "exh_rounds", "rules",
                                           "overflow_retries")] + \
-               [(n, C.c_double) for n in ("ms_setup", "ms_qc", "ms_mid", "ms_exh", "ms_total")]
+               [(n, C.c_double) for n in ("ms_setup", "ms_qc", "ms_mid", "ms_exh", "ms_total",
+                                          "ms_qc_kernel", "ms_exh_kernel")] + \
+               [(n, C.c_uint64) for n in ("qc_launches", "exh_launches")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -86,6 +88,14 @@ STOKE_RESULT_DTYPE = np.dtype(StokeResult)
 STOKE_CHAIN_DTYPE = np.dtype(StokeChain)
 assert STOKE_RESULT_DTYPE.itemsize == C.sizeof(StokeResult) == 72
 assert STOKE_CHAIN_DTYPE.itemsize == C.sizeof(StokeChain) == 64
+
+
+class TargetResult(C.Structure):
+    _fields_ = [("rep", C.c_uint32 * 4), ("rep_len", C.c_uint8), ("found", C.c_uint8), ("pass_", C.c_uint8),
+                ("dead_flags", C.c_uint8), ("bytes_saved", C.c_int16), ("cycles_saved", C.c_int16)]
+
+
+assert C.sizeof(TargetResult) == 24
 
 
 class SxError(RuntimeError):
@@ -134,6 +144,8 @@ def lib() -> C.CDLL:
     L.sx_proposal_weights.argtypes = [f32p, C.c_uint64, C.c_double, u32p]
     L.sx_mlp_proposal.argtypes = [vp, u16p, C.c_uint32, C.c_double, u32p]
     L.sx_mfma_gemm_test.argtypes = [vp, u16p, u16p, C.c_int, C.c_int, C.c_int, f32p]
+    L.sx_target_search.argtypes = [vp, u32p, C.c_int, C.c_int, C.c_uint32, C.c_int, C.POINTER(TargetResult),
+                                   C.POINTER(C.c_uint64)]
     for name in EXPORTS:
         getattr(L, name).restype = C.c_int
     L.sx_mlp_param_count.restype = C.c_uint64
@@ -336,6 +348,23 @@ class Device:
         self._check(rc, "sx_stoke_run")
         del keep
         return (out, ch) if return_chains else out
+
+    # -- target command --------------------------------------------------
+    def target_search(self, target: Sequence[int], max_cand_len: int = 4, dead: int = 0, bytes_model: int = 0):
+        """sx_target_search: ([exact-pass result, masked-pass result], candidates
+        scanned); a result is None where its pass found nothing."""
+        t = np.ascontiguousarray(list(target), dtype=np.uint32)
+        out = (TargetResult * 2)()
+        scanned = C.c_uint64(0)
+        self._check(lib().sx_target_search(self._h, _p32(t), len(t), max_cand_len, dead, bytes_model, out,
+                                           C.byref(scanned)), "sx_target_search")
+        res = []
+        for r in out:
+            res.append(None if not r.found else {
+                "rep": [int(r.rep[k]) for k in range(r.rep_len)], "pass": int(r.pass_),
+                "dead_flags": int(r.dead_flags), "bytes_saved": int(r.bytes_saved),
+                "cycles_saved": int(r.cycles_saved)})
+        return res, int(scanned.value)
 
     # -- learned proposal network ----------------------------------------
     def mlp(self, seed: int = 0) -> "Mlp":

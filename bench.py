@@ -1,4 +1,5 @@
-"""Benchmark: the full length-2 enumeration on one MI355X.
+"""Benchmark: the full length-2 enumeration on one MI355X, plus the batched
+length-3 configuration and STOKE with uniform and learned proposals.
 
 Workload (one step): all 7,633,612 pruned length-2 targets x the 2,863
 length-1 candidates (21,855,031,156 QuickCheck pairs), QuickCheck ->
@@ -12,6 +13,9 @@ pipeline (pairs / step wall time).  The roofline object describes the
 dominant kernel, timed with HIP events on the library's own stream over the
 timed steps; the CPU baseline is the C restatement of the Go enumerate
 (oracle/) on a bounded sample of the same targets, on the host's cores.
+`c3` times 65,536 length-3 targets against all 8,199,632 length-1/2
+candidates; `stoke` gives the median and p90 time to the first verified
+rewrite over 256 targets for uniform and for learned proposals.
 """
 import argparse
 import json
@@ -74,6 +78,74 @@ def c3_section(dev):
             "stage_ms": {"qc": st["ms_qc"], "mid": st["ms_mid"], "exh": st["ms_exh"]}}
 
 
+def _stoke_stats(res, extra_s: float = 0.0):
+    v = res["verified"].astype(bool)
+    secs = np.where(v, res["seconds"] + extra_s, np.inf)
+    steps = np.where(v, res["step"].astype(float), np.inf)
+    med = float(np.median(secs))
+    p90 = float(np.quantile(secs, 0.9, method="inverted_cdf"))
+    return {"verified": int(v.sum()), "median_s": med if np.isfinite(med) else None,
+            "p90_s": p90 if np.isfinite(p90) else None,
+            "median_step": float(np.median(steps)) if np.isfinite(np.median(steps)) else None,
+            "bytes_saved": int(res["bytes_saved"][v].sum())}
+
+
+def stoke_section(dev, steps: int):
+    """STOKE with uniform proposals against STOKE with the learned proposal on
+    the 256 committed length-4..8 targets (16 chains each, `steps` steps per
+    chain, T0 = 1 and T x 0.9999 per step, Philox seed 2).  The learned side
+    first harvests rewrites (the enumerator's length-2 rules and length-3
+    rules of the 65,536 committed targets; cold-chain STOKE on 65,536 other
+    random targets), trains the network for 20,000 Adam steps on the device,
+    and computes the 256 proposal tables; the time of that last step is added
+    to every learned target's time."""
+    from sx6502 import learned
+    gold = os.path.join(ROOT, "tests", "golden")
+    t = np.load(os.path.join(gold, "c4_targets.npy"))
+    ln = np.load(os.path.join(gold, "c4_lens.npy"))
+    targets = [t[i, :ln[i]].tolist() for i in range(len(t))]
+    kw = dict(chains=16, steps=steps, t0=1.0, decay=0.9999, seed=2, stop_when_verified=True)
+    t0 = time.perf_counter()
+    ru = dev.stoke(targets, **kw)
+    wall_u = time.perf_counter() - t0
+    print(f"bench: STOKE uniform {wall_u:.2f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    pairs_e = learned.harvest_enumerator(dev, np.load(os.path.join(gold, "c3_targets.npy")), dead=0, l3_dead=0x01)
+    train_t = learned.random_targets(65536, seed=1000)
+    pairs_s = []
+    for sd in (11, 12, 14, 15):
+        pairs_s += learned.harvest_stoke(dev, train_t, chains=4, steps=4096, seed=sd)
+    harvest_s = time.perf_counter() - t0
+    ce, le = learned.examples(pairs_e, "dropped")
+    cs, ls = learned.examples(pairs_s, "dropped", repeat=4)
+    t0 = time.perf_counter()
+    m, losses = learned.train_proposal(dev, np.concatenate([ce, cs]), np.concatenate([le, ls]), steps=20000,
+                                       batch=256, lr=1e-3, seed=3)
+    train_s = time.perf_counter() - t0
+    print(f"bench: harvest {harvest_s:.1f} s, training {train_s:.1f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    w = learned.proposal_tables(m, targets, 0.1)
+    tables_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rl = dev.stoke(targets, weights=w, **kw)
+    wall_l = time.perf_counter() - t0
+    m.close()
+    su, sl = _stoke_stats(ru), _stoke_stats(rl, tables_s)
+    return {
+        "workload": "256 length-4..8 targets (seed 2) x 16 chains, <= %d steps per chain, T0 1.0 x0.9999/step, "
+                    "cost = Hamming over 32 Mid vectors + 0.1 bytes, exact-ratio MH, rewrites verified by "
+                    "complete ExhaustiveCheck" % steps,
+        "metric": "time to first verified rewrite per target (median, p90 over 256; null = fewer verified)",
+        "uniform": dict(su, wall_s=wall_u),
+        "learned": dict(sl, wall_s=wall_l, tables_s=tables_s),
+        "speedup_median": (su["median_s"] / sl["median_s"]) if su["median_s"] and sl["median_s"] else None,
+        "training": {"harvest_s": harvest_s, "train_s": train_s, "adam_steps": 20000, "batch": 256,
+                     "examples": int(len(le) + len(ls)), "enumerator_rewrites": len(pairs_e),
+                     "stoke_rewrites": len(pairs_s), "loss_first100": float(losses[:100].mean()),
+                     "loss_last500": float(losses[-500:].mean())},
+    }
+
+
 def calib_copy(mib: int):
     """A plain device copy of known size, for scaling HBM counters
     (FETCH_SIZE / WRITE_SIZE) read under a profiler."""
@@ -95,6 +167,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-stoke", action="store_true")
+    ap.add_argument("--stoke-steps", type=int, default=1_000_000)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -149,6 +223,8 @@ def main():
     }
     if not args.no_c3:
         out["c3"] = c3_section(dev)
+    if not args.no_stoke:
+        out["stoke"] = stoke_section(dev, args.stoke_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.dead)
     print(json.dumps(out))

@@ -70,6 +70,8 @@ typedef struct sx_stats {
   uint64_t rules;        /* rules emitted */
   uint64_t overflow_retries; /* compaction-buffer overflows recovered by re-running a tile */
   double ms_setup, ms_qc, ms_mid, ms_exh, ms_total; /* device times (HIP events) and wall */
+  double ms_qc_kernel, ms_exh_kernel; /* HIP events around the QuickCheck / exhaustive-sweep launches only */
+  uint64_t qc_launches, exh_launches;
 } sx_stats;
 
 /* Enumeration semantics. */
@@ -140,6 +142,25 @@ int sx_pruned_targets(sx_handle *h, uint64_t first, uint64_t n, uint32_t *out_wo
 int sx_check_pairs(sx_handle *h, const uint32_t *t_words, const uint8_t *t_len,
                    const uint32_t *c_words, const uint8_t *c_len, uint64_t n,
                    uint32_t stage_mask, uint32_t dead_flags, int32_t exh_mode, uint8_t *out);
+
+/* Shortest replacement of one target of 1..8 instructions, candidates of
+ * length 1..max_cand_len (<= 4) enumerated on the device in Go order.
+ * Replaces search.SearchSingle / SearchSingleMasked (pkg/search/search.go:
+ * 76-106) behind `6502opt target` (cmd/6502opt/main.go:88-132), i.e. the
+ * per-target loops of worker.go:119-245: out[0] is the exact pass (first
+ * candidate in Go order, shortest length first, that passes the byte filter,
+ * the pruner, QuickCheck and the complete ExhaustiveCheck), out[1] the
+ * masked pass (dead_flags != 0: first candidate failing the exact QuickCheck,
+ * passing the masked QuickCheck and ExhaustiveCheck, FlagDiff != 0; its
+ * dead_flags field is that FlagDiff).  found = 0 where a pass has no rule.
+ * scanned: candidates enumerated (worker.go:126 counts the same). */
+typedef struct sx_target_result {
+  uint32_t rep[4];
+  uint8_t rep_len, found, pass, dead_flags;
+  int16_t bytes_saved, cycles_saved;
+} sx_target_result;
+int sx_target_search(sx_handle *h, const uint32_t *target, int target_len, int max_cand_len, uint32_t dead_flags,
+                     int bytes_model, sx_target_result out[2], uint64_t *scanned);
 
 /* FlagDiff for n pairs (pkg/search/verifier.go:591-604), same layout. */
 int sx_flag_diff(sx_handle *h, const uint32_t *t_words, const uint8_t *t_len,

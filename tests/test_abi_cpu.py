@@ -151,7 +151,7 @@ def test_ctypes_mirrors_match_the_c_header(tmp_path):
     import subprocess
     structs = {"sx_rule": core.RuleC, "sx_enum_cfg": core.EnumCfg, "sx_stats": core.Stats,
                "sx_stoke_cfg": core.StokeCfg, "sx_stoke_result": core.StokeResult,
-               "sx_stoke_chain": core.StokeChain}
+               "sx_stoke_chain": core.StokeChain, "sx_target_result": core.TargetResult}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sx6502.h"', 'int main(void) {']
     for name, py in structs.items():
         lines.append(f'  printf("{name} size %zu\\n", sizeof({name}));')
