@@ -220,16 +220,36 @@ __global__ void k_gather_v0(const uint2 *fp, uint64_t n, uint2 *v0) {
 // QuickCheck.  Grid: x = 256-candidate group, y = target tile.
 // Survivor entry: (target, slot | full << 31), where full = the exact
 // (0xCF) compare holds; every entry satisfies the compare under pm_masked.
-__global__ __launch_bounds__(256) void k_quickcheck(const uint2 *__restrict__ tv0,
-                                                    const uint2 *__restrict__ tfp, uint32_t nt,
-                                                    uint32_t tile, const uint2 *__restrict__ cfp,
-                                                    uint32_t nc, uint32_t pm_masked,
-                                                    uint2 *__restrict__ surv, uint32_t surv_cap,
-                                                    uint32_t *__restrict__ surv_count) {
-  __shared__ uint2 queue[4][kQcQueue];
+// One QuickCheck work item: 256 candidate slots (lane = candidate, its 8 QC
+// outputs in registers) against the targets [t0, t1).  Survivors go to the
+// calling wave's LDS queue (fill qn, wave-uniform) and are flushed with one
+// atomic per batch.
+struct QcQueue {
+  uint2 *q;       // this wave's LDS queue
+  uint32_t qn;
+  uint2 *surv;
+  uint32_t cap;
+  uint32_t *count;
+
+  __device__ __forceinline__ void flush(uint32_t lane) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count, qn);
+    base = uni(base);
+    for (uint32_t i = lane; i < qn; i += kWave) {
+      uint32_t o = base + i;
+      if (o < cap) surv[o] = q[i];
+    }
+    __builtin_amdgcn_wave_barrier();
+    qn = 0;
+  }
+};
+
+__device__ __forceinline__ void qc_item(const uint2 *__restrict__ tv0, const uint2 *__restrict__ tfp, uint32_t nt,
+                                        uint32_t t0, uint32_t t1, const uint2 *__restrict__ cfp, uint32_t nc,
+                                        uint32_t slot, uint32_t pm_masked, QcQueue &Q) {
   const uint32_t lane = lane_id();
-  const uint32_t wv = threadIdx.x >> 6;
-  const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
   const bool valid = slot < nc;
   uint32_t clo[8], chi[8];
 #pragma unroll
@@ -240,24 +260,6 @@ __global__ __launch_bounds__(256) void k_quickcheck(const uint2 *__restrict__ tv
   }
   const uint32_t mlo = 0x00FFFFFFu | (pm_masked << 24);
   const uint32_t flo = 0x00FFFFFFu | (kPMask << 24);
-  const uint32_t t0 = blockIdx.y * tile;
-  const uint32_t t1 = min(nt, t0 + tile);
-  uint32_t qn = 0;  // wave-uniform queue fill
-
-  auto flush = [&]() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(surv_count, qn);
-    base = uni(base);
-    for (uint32_t i = lane; i < qn; i += kWave) {
-      uint32_t o = base + i;
-      if (o < surv_cap) surv[o] = queue[wv][i];
-    }
-    __builtin_amdgcn_wave_barrier();
-    qn = 0;
-  };
-
   for (uint32_t t = t0; t < t1; t += 8) {
     uint32_t nb = min(8u, t1 - t);
     uint2 tv[8];
@@ -281,12 +283,61 @@ __global__ __launch_bounds__(256) void k_quickcheck(const uint2 *__restrict__ tv
       }
       uint64_t hm = __ballot(okm);
       if (hm == 0) continue;
-      if (okm) queue[wv][qn + mbcnt(hm)] = make_uint2(t + u, slot | (okf ? 0x80000000u : 0u));
-      qn += (uint32_t)__popcll(hm);
-      if (qn > kQcQueue - kWave) flush();
+      if (okm) Q.q[Q.qn + mbcnt(hm)] = make_uint2(t + u, slot | (okf ? 0x80000000u : 0u));
+      Q.qn += (uint32_t)__popcll(hm);
+      if (Q.qn > kQcQueue - kWave) Q.flush(lane);
     }
   }
-  if (qn) flush();
+}
+
+// Grid form: block (x, y) = candidate slots [256 x, 256 x + 256) against
+// target tile y.
+__global__ __launch_bounds__(256) void k_quickcheck(const uint2 *__restrict__ tv0,
+                                                    const uint2 *__restrict__ tfp, uint32_t nt,
+                                                    uint32_t tile, const uint2 *__restrict__ cfp,
+                                                    uint32_t nc, uint32_t pm_masked,
+                                                    uint2 *__restrict__ surv, uint32_t surv_cap,
+                                                    uint32_t *__restrict__ surv_count) {
+  __shared__ uint2 queue[4][kQcQueue];
+  QcQueue Q{queue[threadIdx.x >> 6], 0, surv, surv_cap, surv_count};
+  const uint32_t t0 = blockIdx.y * tile;
+  qc_item(tv0, tfp, nt, t0, min(nt, t0 + tile), cfp, nc, blockIdx.x * 256 + threadIdx.x, pm_masked, Q);
+  if (Q.qn) Q.flush(lane_id());
+}
+
+// Persistent form: a fixed grid pulls work items (candidate block, target
+// tile) from 8 device queues, one per XCD (workgroups are dispatched to the
+// XCDs round-robin, so block b runs on XCD b mod 8).  Queue q owns the target
+// tiles q, q + 8, ... so a tile's target data is read through one XCD's L2;
+// a block drains its own queue first, then the others.  Every block leaves
+// once all 8 queues are drained.
+__global__ __launch_bounds__(256) void k_qc_persistent(const uint2 *__restrict__ tv0,
+                                                       const uint2 *__restrict__ tfp, uint32_t nt, uint32_t tile,
+                                                       const uint2 *__restrict__ cfp, uint32_t nc,
+                                                       uint32_t pm_masked, uint2 *__restrict__ surv,
+                                                       uint32_t surv_cap, uint32_t *__restrict__ surv_count,
+                                                       uint32_t *__restrict__ heads, uint32_t ncb,
+                                                       uint32_t ntiles) {
+  __shared__ uint2 queue[4][kQcQueue];
+  __shared__ uint32_t item_s;
+  QcQueue Q{queue[threadIdx.x >> 6], 0, surv, surv_cap, surv_count};
+  const uint32_t home = blockIdx.x & 7u;
+  for (uint32_t k = 0; k < 8; k++) {
+    const uint32_t q = (home + k) & 7u;
+    const uint32_t tiles_q = q < ntiles ? (ntiles - q + 7) / 8 : 0;
+    const uint32_t nitems = tiles_q * ncb;
+    for (;;) {
+      __syncthreads();
+      if (threadIdx.x == 0) item_s = nitems ? atomicAdd(&heads[q], 1u) : 0xFFFFFFFFu;
+      __syncthreads();
+      const uint32_t item = item_s;
+      if (item >= nitems) break;
+      const uint32_t tt = q + 8 * (item / ncb), cb = item % ncb;
+      const uint32_t t0 = tt * tile;
+      qc_item(tv0, tfp, nt, t0, min(nt, t0 + tile), cfp, nc, cb * 256 + threadIdx.x, pm_masked, Q);
+    }
+  }
+  if (Q.qn) Q.flush(lane_id());
 }
 
 // MidCheck and stage bookkeeping.  One lane per QuickCheck survivor.
@@ -406,12 +457,16 @@ __device__ __forceinline__ void fill_job(Job &j, const uint32_t *targets, uint32
   j.slot = slot;
 }
 
+// Jobs of one round: the current candidate of every open (target, pass).
+// In first-hit mode it also counts the survivors not yet tested (*nrest),
+// which decides when the tail runs as one batch.
 __global__ __launch_bounds__(256) void k_round_jobs(const uint2 *mid, uint32_t nm, const uint32_t *rank,
-                                                    const uint32_t *cur, const uint32_t *targets,
-                                                    uint32_t tl, uint32_t pm_masked, int hits_all,
-                                                    Job *jobs, uint32_t job_cap, uint32_t *njobs) {
+                                                    const uint32_t *cur, const uint32_t *rej, const uint8_t *done,
+                                                    const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
+                                                    int hits_all, Job *jobs, uint32_t job_cap, uint32_t *njobs,
+                                                    uint32_t *nrest) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  bool take = false;
+  bool take = false, rest = false;
   uint2 e = make_uint2(0, 0);
   if (i < nm) {
     e = mid[i];
@@ -419,8 +474,14 @@ __global__ __launch_bounds__(256) void k_round_jobs(const uint2 *mid, uint32_t n
       take = true;
     } else {
       uint32_t slot = e.y & 0x3FFFFFFFu, p = (e.y >> 30) - 1;
-      take = cur[(uint64_t)e.x * 2 + p] == rank[slot];
+      uint64_t k = (uint64_t)e.x * 2 + p;
+      take = cur[k] == rank[slot];
+      rest = !done[k] && rank[slot] >= rej[k];
     }
+  }
+  if (!hits_all) {
+    uint64_t r = __ballot(rest);
+    if (r && lane_id() == 0) atomicAdd(nrest, (uint32_t)__popcll(r));
   }
   uint64_t m = __ballot(take);
   if (m == 0) return;
@@ -552,7 +613,7 @@ __global__ __launch_bounds__(256) void k_round_resolve(const Job *jobs, uint32_t
                                                        const uint8_t *cbytes, const uint8_t *ccyc,
                                                        int hits_all, uint32_t dead, uint64_t tbase,
                                                        sx_rule *rules, uint32_t rule_cap,
-                                                       uint32_t *nrules) {
+                                                       uint32_t *nrules, uint32_t *rule_flag) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nj) return;
   const Job &j = jobs[i];
@@ -570,8 +631,16 @@ __global__ __launch_bounds__(256) void k_round_resolve(const Job *jobs, uint32_t
     else rej[k] = rank[j.slot] + 1;
   }
   if (!ok) return;
-  uint32_t o = atomicAdd(nrules, 1u);
-  if (o >= rule_cap) return;
+  // first hit: one rule per (target, pass), written to its own slot k (the
+  // slots are compacted in order afterwards); all hits: appended
+  uint32_t o;
+  if (hits_all) {
+    o = atomicAdd(nrules, 1u);
+    if (o >= rule_cap) return;
+  } else {
+    o = (uint32_t)k;
+    rule_flag[k] = 1;
+  }
   sx_rule r;
   for (int q = 0; q < 4; q++) {
     r.src[q] = j.tw[q];
@@ -643,14 +712,14 @@ __global__ __launch_bounds__(256) void k_bulk_emit(const uint32_t *best, uint32_
                                                    uint32_t tl, const uint2 *tfp, const uint2 *cfp,
                                                    const uint8_t *tbytes, const uint8_t *tcyc,
                                                    const uint8_t *cbytes, const uint8_t *ccyc, uint64_t tbase,
-                                                   sx_rule *rules, uint32_t rule_cap, uint32_t *nrules) {
+                                                   sx_rule *rules, uint32_t *rule_flag) {
   uint32_t k = blockIdx.x * 256 + threadIdx.x;
   if (k >= 2 * nt || best[k] == kRankNone) return;
   const uint32_t t = k >> 1, pass = (k & 1) + 1, slot = best[k];
   Job j;
   fill_job(j, targets, tl, nullptr, 0, t, slot, pass, 0);
-  uint32_t o = atomicAdd(nrules, 1u);
-  if (o >= rule_cap) return;
+  const uint32_t o = k;
+  rule_flag[k] = 1;
   sx_rule r;
   for (int q = 0; q < 4; q++) {
     r.src[q] = j.tw[q];
@@ -664,6 +733,14 @@ __global__ __launch_bounds__(256) void k_bulk_emit(const uint32_t *best, uint32_
   r.bytes_saved = (int16_t)((int)tbytes[t] - (int)cbytes[slot]);
   r.cycles_saved = (int16_t)((int)tcyc[t] - (int)ccyc[slot]);
   rules[o] = r;
+}
+
+// Ordered compaction of the first-hit rule slots (flag[k] = 1 where slot k
+// holds a rule; offs = exclusive scan of the flags).
+__global__ __launch_bounds__(256) void k_compact_rules(const sx_rule *slots, const uint32_t *flag,
+                                                       const uint64_t *offs, uint64_t n, sx_rule *out) {
+  uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k < n && flag[k]) out[offs[k]] = slots[k];
 }
 
 // Bulk QuickCheck / MidCheck / FlagDiff for sx_check_pairs: one wave per pair.
@@ -763,7 +840,8 @@ struct sx_handle {
   int device = 0;
   int num_cu = 256;
   bool poisoned = false;  // a device stage timed out: the stream is unusable
-  int bulk_after = 2;     // first-hit rounds before the tail runs as one batch (SX_BULK_AFTER)
+  int bulk_after = 1;     // first-hit rounds before the one-batch tail (SX_BULK_AFTER); -1 = by survivor count
+  uint32_t surv_cap_hint = 0;  // QuickCheck survivor buffer size that sufficed before
   // kernel-only timing of the QuickCheck and exhaustive-sweep launches
   hipEvent_t kev[2] = {nullptr, nullptr};
   double k_qc_ms = 0, k_exh_ms = 0;
@@ -771,11 +849,17 @@ struct sx_handle {
   hipStream_t stream = nullptr;
   std::string err;
   std::vector<sx_rule> rules;
+  // QuickCheck launch configurations chosen by the autotuner, per (target_len, max_cand_len)
+  struct QcChoice {
+    int tl, ml, big;  // big: segments of >= 65,536 targets
+    uint32_t tile, blocks_per_cu;  // blocks_per_cu 0 = grid form
+  };
+  std::vector<QcChoice> qc_tuned;
   // cached pruned length-2 target list (Go order)
   uint32_t *d_l2 = nullptr;
   uint64_t n_l2 = 0;
   // buffers reused across calls
-  DevBuf bufs[40];
+  DevBuf bufs[48];
 };
 
 namespace {
@@ -784,7 +868,8 @@ enum BufId {
   B_TARGETS, B_TFP, B_TV0, B_TBYTES, B_TCYC, B_TPRUNE, B_CWORDS, B_CFP, B_CBYTES, B_CCYC, B_CPRUNE,
   B_RANK, B_SURV, B_MID, B_COUNTERS, B_CUR, B_REJ, B_DONE, B_JOBS, B_CHUNKS, B_OFFS, B_SUMS,
   B_FAIL, B_RULES, B_SCRATCH, B_GEN_COUNTS, B_GEN_OFFS, B_BADOP, B_P_TW, B_P_TL, B_P_CW, B_P_CL,
-  B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR
+  B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR, B_HEADS, B_RULEFLAG, B_RULEOFFS, B_RULES2,
+  B_RULETOT
 };
 
 int set_err(sx_handle *h, int code, const char *fmt, ...) {
@@ -921,6 +1006,73 @@ int ensure_l2(sx_handle *h) {
 }
 
 
+// One QuickCheck pass over targets [0, nt) x candidate slots [0, nc) with a
+// given launch configuration (blocks_per_cu 0: grid form with `tile` targets
+// per block row; else persistent form, num_cu x blocks_per_cu blocks).
+int launch_qc(sx_handle *h, uint32_t tile, uint32_t blocks_per_cu, const uint2 *tv0, const uint2 *tfp, uint64_t nt,
+              const uint2 *cfp, uint64_t nc, uint32_t pm_masked, uint2 *surv, uint32_t surv_cap,
+              uint32_t *counters) {
+  if (blocks_per_cu == 0) {
+    dim3 grid(blocks_for(nc, 256), (unsigned)((nt + tile - 1) / tile));
+    hipLaunchKernelGGL(k_quickcheck, grid, dim3(256), 0, h->stream, tv0, tfp, (uint32_t)nt, tile, cfp,
+                       (uint32_t)nc, pm_masked, surv, surv_cap, counters);
+  } else {
+    uint32_t *heads;
+    ENSURE(B_HEADS, uint32_t, 8, heads);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, h->stream, heads, (uint64_t)8, 0u);
+    const uint32_t ncb = blocks_for(nc, 256), ntiles = (uint32_t)((nt + tile - 1) / tile);
+    hipLaunchKernelGGL(k_qc_persistent, dim3((unsigned)(h->num_cu * blocks_per_cu)), dim3(256), 0, h->stream, tv0,
+                       tfp, (uint32_t)nt, tile, cfp, (uint32_t)nc, pm_masked, surv, surv_cap, counters, heads, ncb,
+                       ntiles);
+  }
+  HIPCHK(h, hipGetLastError());
+  return SX_OK;
+}
+
+// The QuickCheck autotuner: on the first segment of a (target_len,
+// max_cand_len) shape, time every candidate configuration on a slice of the
+// real work (about 4e9 pairs) and keep the fastest for the handle.  The
+// survivors of these runs are discarded; the real pass runs afterwards.
+int tune_qc(sx_handle *h, int tl, int ml, const uint2 *tv0, const uint2 *tfp, uint64_t nt, const uint2 *cfp,
+            uint64_t nc, uint32_t pm_masked, uint2 *surv, uint32_t surv_cap, uint32_t *counters,
+            sx_handle::QcChoice *out) {
+  const int big = nt >= 65536;
+  for (const auto &c : h->qc_tuned)
+    if (c.tl == tl && c.ml == ml && c.big == big) {
+      *out = c;
+      return SX_OK;
+    }
+  const uint64_t nc_t = std::min<uint64_t>(nc, 1u << 18);
+  const uint64_t nt_t = std::min<uint64_t>(nt, std::max<uint64_t>(16384, 4000000000ull / nc_t));
+  struct Cand {
+    uint32_t tile, bpc;
+  };
+  const Cand cands[] = {{1024, 0}, {2048, 0}, {4096, 0}, {8192, 0}, {2048, 2}, {2048, 4},
+                        {2048, 8}, {4096, 2}, {4096, 4}, {4096, 8}, {8192, 4}};
+  float best = 1e30f;
+  sx_handle::QcChoice choice{tl, ml, big, 2048, 0};
+  for (int rep = 0; rep < 2; rep++)  // the first sweep warms up code and caches
+    for (const Cand &c : cands) {
+      HIPCHK(h, hipEventRecord(h->kev[0], h->stream));
+      int rc = launch_qc(h, c.tile, c.bpc, tv0, tfp, nt_t, cfp, nc_t, pm_masked, surv, surv_cap, counters);
+      if (rc) return rc;
+      HIPCHK(h, hipEventRecord(h->kev[1], h->stream));
+      rc = sync_wd(h, "QuickCheck autotune");
+      if (rc) return rc;
+      float ms = 0;
+      HIPCHK(h, hipEventElapsedTime(&ms, h->kev[0], h->kev[1]));
+      trace(h, "  autotune QuickCheck tile %u blocks/CU %u: %.3f ms", c.tile, c.bpc, ms);
+      if (rep == 1 && ms < best) {
+        best = ms;
+        choice.tile = c.tile;
+        choice.blocks_per_cu = c.bpc;
+      }
+    }
+  h->qc_tuned.push_back(choice);
+  *out = choice;
+  return SX_OK;
+}
+
 // Run the exhaustive stage on nj jobs already in B_JOBS; the verdicts are in
 // B_FAIL (0 = pass).  *chunks_out gets the number of chunks (statistics).
 int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, uint64_t *states_out) {
@@ -995,7 +1147,7 @@ extern "C" int sx_open(int device, sx_handle **out) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->num_cu = prop.multiProcessorCount;
-  if (const char *e = getenv("SX_BULK_AFTER")) h->bulk_after = std::max(0, atoi(e));
+  if (const char *e = getenv("SX_BULK_AFTER")) h->bulk_after = std::max(-1, atoi(e));
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->kev[0]) != hipSuccess || hipEventCreate(&h->kev[1]) != hipSuccess) {
     delete h;
@@ -1126,8 +1278,17 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
 
   // ---- segments of targets
   const uint64_t seg = cfg->targets ? std::min<uint64_t>(nt_all, 1u << 20) : (1u << 20);
-  const uint32_t tile = cfg->tile_targets ? cfg->tile_targets : (ml == 1 ? 2048u : 4096u);
-  uint32_t surv_cap = cfg->survivor_cap ? cfg->survivor_cap : (1u << 25);
+  // QuickCheck launch shape: the caller's tile (grid form), else the autotuner's choice
+  sx_handle::QcChoice qcc{tl, ml, 0, cfg->tile_targets, 0};
+  bool qc_tuned = cfg->tile_targets != 0;
+  // survivor buffer: the caller's size, else what sufficed before on this
+  // handle, else about one entry per 4,096 pairs (QuickCheck passes ~1e-4 of
+  // the pairs at --dead-flags 0 and a few times more when flags are dead)
+  uint32_t surv_cap = cfg->survivor_cap;
+  if (!surv_cap) {
+    const uint64_t est = std::min<uint64_t>(seg, nt_all) * nc / 4096;
+    surv_cap = (uint32_t)std::max<uint64_t>({1u << 25, std::min<uint64_t>(est, 1u << 28), h->surv_cap_hint});
+  }
   const uint32_t pm_masked = kPMask & ~cfg->dead_flags;
   float ms_qc = 0, ms_mid = 0, ms_exh = 0, ms_setup = 0;
   HIPCHK(h, hipEventRecord(e1, h->stream));
@@ -1163,12 +1324,18 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     uint32_t nsurv = 0;
     for (;;) {
       ENSURE(B_SURV, uint2, surv_cap, surv);
+      if (!qc_tuned) {
+        int rc = tune_qc(h, tl, ml, tv0, tfp, nt, cfp, nc, pm_masked, surv, surv_cap, counters, &qcc);
+        if (rc) return rc;
+        qc_tuned = true;
+      }
       HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));
-      dim3 grid(blocks_for(nc, 256), (unsigned)((nt + tile - 1) / tile));
       HIPCHK(h, hipEventRecord(h->kev[0], h->stream));
-      hipLaunchKernelGGL(k_quickcheck, grid, dim3(256), 0, h->stream, tv0, tfp, (uint32_t)nt, tile, cfp,
-                         (uint32_t)nc, pm_masked, surv, surv_cap, counters);
-      HIPCHK(h, hipGetLastError());
+      {
+        int rc = launch_qc(h, qcc.tile, qcc.blocks_per_cu, tv0, tfp, nt, cfp, nc, pm_masked, surv, surv_cap,
+                           counters);
+        if (rc) return rc;
+      }
       HIPCHK(h, hipEventRecord(h->kev[1], h->stream));
       HIPCHK(h, hipMemcpyAsync(&nsurv, counters, 4, hipMemcpyDeviceToHost, h->stream));
       SYNC_WD("device sync");
@@ -1183,6 +1350,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       if (nsurv <= surv_cap) break;
       st.overflow_retries++;
       surv_cap = nsurv + nsurv / 4 + 1024;
+      if (!cfg->survivor_cap) h->surv_cap_hint = std::max(h->surv_cap_hint, surv_cap);
     }
     HIPCHK(h, hipEventRecord(e2, h->stream));
     st.qc_raw_hits += nsurv;
@@ -1222,16 +1390,31 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     ENSURE(B_JOBS, Job, job_cap, jobs);
     const uint32_t rule_cap = cfg->hits ? std::max<uint32_t>(nmid, 1) : (uint32_t)(2 * nt);
     ENSURE(B_RULES, sx_rule, rule_cap, rules);
+    uint32_t *rule_flag;  // first hit: slot k = target * 2 + pass - 1 holds a rule
+    ENSURE(B_RULEFLAG, uint32_t, nt * 2, rule_flag);
+    if (!cfg->hits)
+      hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, rule_flag, nt * 2, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, nt * 2, kRankNone);
     HIPCHK(h, hipMemsetAsync(rej, 0, nt * 2 * 4, h->stream));
     HIPCHK(h, hipMemsetAsync(done, 0, nt * 2, h->stream));
     uint32_t *nrules_d = counters + 2, *njobs_d = counters + 3;
     HIPCHK(h, hipMemsetAsync(nrules_d, 0, 4, h->stream));
-    const int bulk_after = h->bulk_after;  // rounds before the first-hit tail runs in one batch
+    // The first-hit tail runs as one batch once the untested survivors of the
+    // open (target, pass) entries number at most kBulkJobs, or at most
+    // kBulkRatio x the entries themselves (one batch then costs at most that
+    // many rounds' sweeps); or after a fixed number of rounds when
+    // SX_BULK_AFTER is set.  Rounds stay in use while a batch would sweep far
+    // more pairs than the first hits need (at --dead-flags 0xFF the first
+    // round resolves most targets out of ~470M survivors).
+    constexpr uint32_t kBulkRatio = 4;
+    constexpr uint32_t kBulkJobs = 1u << 22;
+    uint32_t *nrest_d = counters + 4;
+    bool go_bulk = false;
     for (int round = 0; nmid > 0; round++) {
       // every round rejects or resolves one candidate per open (target, pass)
       if ((uint64_t)round > nc + 1) return set_err(h, SX_EHIP, "first-hit rounds did not converge");
-      if (!cfg->hits && round >= bulk_after) {
+      if (!cfg->hits && h->bulk_after >= 0 && round >= h->bulk_after) go_bulk = true;
+      if (!cfg->hits && go_bulk) {
         HIPCHK(h, hipMemsetAsync(njobs_d, 0, 4, h->stream));
         hipLaunchKernelGGL(k_bulk_jobs, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
                            rej, done, tw, (uint32_t)tl, pm_masked, jobs, job_cap, njobs_d);
@@ -1254,21 +1437,32 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
                            cfp, cur);
         hipLaunchKernelGGL(k_bulk_emit, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur,
                            (uint32_t)nt, tw, (uint32_t)tl, tfp, cfp, tbytes, tcyc, cbytes, ccyc, tbase + s0, rules,
-                           rule_cap, nrules_d);
+                           rule_flag);
         HIPCHK(h, hipGetLastError());
         break;
       }
-      HIPCHK(h, hipMemsetAsync(njobs_d, 0, 4, h->stream));
+      HIPCHK(h, hipMemsetAsync(njobs_d, 0, 8, h->stream));  // njobs and nrest
       if (!cfg->hits)
         hipLaunchKernelGGL(k_round_min, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
                            cur, rej, done);
       hipLaunchKernelGGL(k_round_jobs, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
-                         cur, tw, (uint32_t)tl, pm_masked, cfg->hits, jobs, job_cap, njobs_d);
+                         cur, rej, done, tw, (uint32_t)tl, pm_masked, cfg->hits, jobs, job_cap, njobs_d, nrest_d);
       HIPCHK(h, hipGetLastError());
-      uint32_t nj = 0;
-      HIPCHK(h, hipMemcpyAsync(&nj, njobs_d, 4, hipMemcpyDeviceToHost, h->stream));
+      uint32_t cnt2[2] = {0, 0};
+      HIPCHK(h, hipMemcpyAsync(cnt2, njobs_d, 8, hipMemcpyDeviceToHost, h->stream));
       SYNC_WD("device sync");
+      uint32_t nj = cnt2[0];
       if (nj == 0) break;
+      if (!cfg->hits && h->bulk_after < 0 &&
+          ((uint64_t)cnt2[1] <= (uint64_t)kBulkRatio * nj || cnt2[1] <= kBulkJobs)) {
+        // drop this round's jobs: the batch covers them; cur back to "none" for the minimum
+        hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, nt * 2,
+                           kRankNone);
+        HIPCHK(h, hipGetLastError());
+        go_bulk = true;
+        round--;
+        continue;
+      }
       if (nj > job_cap) return set_err(h, SX_ENOMEM, "job buffer overflow (%u > %u)", nj, job_cap);
       trace(h, "  round %d: %u jobs", round, nj);
       st.exh_rounds++;
@@ -1280,23 +1474,38 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       hipLaunchKernelGGL(k_round_resolve, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream,
                          (const Job *)h->bufs[B_JOBS].p, nj, (const uint32_t *)h->bufs[B_FAIL].p, rank, cur, rej,
                          done, tfp, cfp, tbytes, tcyc, cbytes, ccyc, cfg->hits, cfg->dead_flags, tbase + s0,
-                         rules, rule_cap, nrules_d);
+                         rules, rule_cap, nrules_d, rule_flag);
       HIPCHK(h, hipGetLastError());
       if (cfg->hits) break;
-      // rule buffer growth check happens after the round
-      uint32_t nr = 0;
-      HIPCHK(h, hipMemcpyAsync(&nr, nrules_d, 4, hipMemcpyDeviceToHost, h->stream));
-      SYNC_WD("device sync");
-      if (nr > rule_cap) return set_err(h, SX_ENOMEM, "rule buffer overflow");
     }
     uint32_t nr = 0;
-    HIPCHK(h, hipMemcpyAsync(&nr, nrules_d, 4, hipMemcpyDeviceToHost, h->stream));
+    const sx_rule *rules_out = rules;
+    if (cfg->hits) {
+      HIPCHK(h, hipMemcpyAsync(&nr, nrules_d, 4, hipMemcpyDeviceToHost, h->stream));
+    } else {
+      // the filled slots in (target, pass) order
+      uint64_t *roffs, *rtot;
+      sx_rule *packed;
+      ENSURE(B_RULEOFFS, uint64_t, nt * 2 + 1, roffs);
+      ENSURE(B_RULETOT, uint64_t, 1, rtot);
+      int rc = scan_u32(h, rule_flag, nt * 2, roffs, rtot);
+      if (rc) return rc;
+      uint64_t tot = 0;
+      HIPCHK(h, hipMemcpyAsync(&tot, rtot, 8, hipMemcpyDeviceToHost, h->stream));
+      SYNC_WD("device sync");
+      nr = (uint32_t)tot;
+      ENSURE(B_RULES2, sx_rule, std::max<uint32_t>(nr, 1), packed);
+      hipLaunchKernelGGL(k_compact_rules, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, rules, rule_flag,
+                         roffs, nt * 2, packed);
+      HIPCHK(h, hipGetLastError());
+      rules_out = packed;
+    }
     HIPCHK(h, hipEventRecord(e4, h->stream));
     SYNC_WD("device sync");
     if (nr > rule_cap) return set_err(h, SX_ENOMEM, "rule buffer overflow (%u rules)", nr);
     size_t old = h->rules.size();
     h->rules.resize(old + nr);
-    if (nr) HIPCHK(h, hipMemcpy(h->rules.data() + old, rules, nr * sizeof(sx_rule), hipMemcpyDeviceToHost));
+    if (nr) HIPCHK(h, hipMemcpy(h->rules.data() + old, rules_out, nr * sizeof(sx_rule), hipMemcpyDeviceToHost));
     float a = 0, b = 0, c = 0;
     HIPCHK(h, hipEventElapsedTime(&a, e1, e2));
     HIPCHK(h, hipEventElapsedTime(&b, e2, e3));
@@ -1314,7 +1523,9 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   (void)hipEventDestroy(e3);
   (void)hipEventDestroy(e4);
   if (bad) return set_err(h, SX_EOPCODE, "opcode out of range in a target");
-  std::sort(h->rules.begin(), h->rules.end(), [](const sx_rule &x, const sx_rule &y) {
+  // first hit: segments in order, slots compacted in (target, pass) order, so
+  // the rules are already sorted; all hits: sort on the host
+  if (cfg->hits) std::sort(h->rules.begin(), h->rules.end(), [](const sx_rule &x, const sx_rule &y) {
     if (x.target_index != y.target_index) return x.target_index < y.target_index;
     if (x.pass != y.pass) return x.pass < y.pass;
     if (x.rep_len != y.rep_len) return x.rep_len < y.rep_len;
@@ -1324,6 +1535,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   });
   st.rules = h->rules.size();
   st.ms_setup = ms_setup;
+  st.qc_tile = qcc.tile;
+  st.qc_blocks_per_cu = qcc.blocks_per_cu;
   st.ms_qc_kernel = h->k_qc_ms - qc_ms0;
   st.ms_exh_kernel = h->k_exh_ms - exh_ms0;
   st.qc_launches = h->k_qc_launches - qc_l0;

@@ -2,7 +2,7 @@
 //
 // Keeps the subcommands, flags and output schemas of the reference CLI
 // (cmd/6502opt/main.go): enumerate (:33-83), target (:88-132), verify-jsonl
-// (:136-145, :386-463) and the parsers parseDeadFlags / parseAssembly /
+// (:136-145, :386-463), stoke (:196-275) and the parsers parseDeadFlags / parseAssembly /
 // parseSingleInstruction / parseImmediate (:285-384, :465-494).  The search
 // itself runs on the GPU through the C ABI.
 //
@@ -10,7 +10,8 @@
 // (cuda/6502_search_v2.cu:912-930); --bytes-model go|cuda, --hits first|all,
 // --exh projected|full, --no-mid and --device select the device semantics;
 // --first-target/--end-target select a range of the pruned target space (the
-// multi-process split of the CUDA tool, cu:775-777).
+// multi-process split of the CUDA tool, cu:775-777); stoke takes --seed (the
+// chains are reproducible) and reports the first verified rewrite.
 #include <algorithm>
 #include <chrono>
 #include <cstdint>
@@ -392,6 +393,75 @@ int cmd_target(int argc, char **argv) {
   return 0;
 }
 
+// stoke (main.go:196-275): chains on the device, every rewrite verified by
+// the complete ExhaustiveCheck before it is reported.
+int cmd_stoke(int argc, char **argv) {
+  Args a = parse_args(argc, argv, 2, {"verbose"});
+  std::string target = a.get("target", "");
+  if (target.empty()) return die("--target is required");
+  std::vector<uint32_t> seq;
+  std::string err;
+  if (!parse_assembly(target, &seq, &err)) return die("failed to parse target: " + err);
+  if (seq.size() > 8) return die("target must have at most 8 instructions");
+  uint32_t dead;
+  if (!parse_dead_flags(a.get("dead-flags", "none"), &dead)) return die("invalid --dead-flags value");
+  sx_stoke_cfg cfg;
+  memset(&cfg, 0, sizeof(cfg));
+  cfg.struct_size = sizeof(cfg);
+  cfg.chains_per_target = (uint32_t)std::atoi(a.get("chains", "16").c_str());
+  cfg.steps = strtoull(a.get("iterations", "10000000").c_str(), nullptr, 10);
+  cfg.decay = std::atof(a.get("decay", "0.9999").c_str());
+  cfg.t0 = 1.0;
+  cfg.seed = strtoull(a.get("seed", "0").c_str(), nullptr, 10);
+  cfg.dead_flags = dead;
+  cfg.stop_when_verified = 1;
+  uint32_t tw[8] = {0};
+  for (size_t k = 0; k < seq.size(); k++) tw[k] = seq[k];
+  uint8_t tl = (uint8_t)seq.size();
+  sx_handle *h = nullptr;
+  if (check(nullptr, sx_open(std::atoi(a.get("device", "0").c_str()), &h), "sx_open")) return 1;
+  sx_stoke_result res;
+  auto t0 = std::chrono::steady_clock::now();
+  if (check(h, sx_stoke_run(h, &cfg, tw, &tl, 1, &res, nullptr), "sx_stoke_run")) return 1;
+  double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  sx_close(h);
+  std::vector<sx_rule> rules;
+  if (res.verified) {
+    sx_rule r;
+    memset(&r, 0, sizeof(r));
+    for (size_t k = 0; k < seq.size() && k < 4; k++) r.src[k] = seq[k];
+    for (int k = 0; k < res.rep_len && k < 4; k++) r.rep[k] = res.rep[k];
+    r.src_len = (uint8_t)std::min<size_t>(seq.size(), 4);
+    r.rep_len = (uint8_t)std::min<int>(res.rep_len, 4);
+    r.dead_flags = (uint8_t)dead;
+    r.bytes_saved = res.bytes_saved;
+    r.cycles_saved = res.cycles_saved;
+    rules.push_back(r);
+  }
+  if (a.has("verbose"))
+    fprintf(stderr, "  %u chains, %.3f s%s\n", cfg.chains_per_target, secs,
+            res.verified ? "" : ", nothing verified");
+  printf("\n%zu unique optimizations found\n", rules.size());
+  if (res.verified) {
+    printf("  1. %s", disasm_seq(res.rep, res.rep_len).c_str());
+    if (dead)
+      printf(" (-%d bytes, -%d cycles, dead flags: %s)\n", res.bytes_saved, res.cycles_saved,
+             dead_flag_desc((int)dead).c_str());
+    else
+      printf(" (-%d bytes, -%d cycles)\n", res.bytes_saved, res.cycles_saved);
+  }
+  std::string outp = a.get("output", "");
+  if (!outp.empty() && !rules.empty()) {
+    if (seq.size() > 4 || res.rep_len > 4) return die("--output holds sequences of up to 4 instructions");
+    FILE *f = fopen(outp.c_str(), "w");
+    if (!f) return die("cannot create " + outp);
+    write_json(f, rules);
+    fclose(f);
+    printf("Written to %s\n", outp.c_str());
+  }
+  return 0;
+}
+
 // Extract a JSON string field from one flat JSONL object.
 bool json_field(const std::string &line, const std::string &key, std::string *out) {
   std::string pat = "\"" + key + "\"";
@@ -497,13 +567,14 @@ int cmd_verify_jsonl(int argc, char **argv) {
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    fprintf(stderr, "usage: 6502opt enumerate|target|verify-jsonl [flags]\n");
+    fprintf(stderr, "usage: 6502opt enumerate|target|verify-jsonl|stoke [flags]\n");
     return 1;
   }
   std::string cmd = argv[1];
   if (cmd == "enumerate") return cmd_enumerate(argc, argv);
   if (cmd == "target") return cmd_target(argc, argv);
   if (cmd == "verify-jsonl") return cmd_verify_jsonl(argc, argv);
+  if (cmd == "stoke") return cmd_stoke(argc, argv);
   fprintf(stderr, "Error: unknown command \"%s\" for \"6502opt\"\n", cmd.c_str());
   return 1;
 }

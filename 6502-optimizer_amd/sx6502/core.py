@@ -59,7 +59,8 @@ class Stats(C.Structure):
                                           "overflow_retries")] + \
                [(n, C.c_double) for n in ("ms_setup", "ms_qc", "ms_mid", "ms_exh", "ms_total",
                                           "ms_qc_kernel", "ms_exh_kernel")] + \
-               [(n, C.c_uint64) for n in ("qc_launches", "exh_launches")]
+               [(n, C.c_uint64) for n in ("qc_launches", "exh_launches")] + \
+               [(n, C.c_uint32) for n in ("qc_tile", "qc_blocks_per_cu")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

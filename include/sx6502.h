@@ -72,6 +72,8 @@ typedef struct sx_stats {
   double ms_setup, ms_qc, ms_mid, ms_exh, ms_total; /* device times (HIP events) and wall */
   double ms_qc_kernel, ms_exh_kernel; /* HIP events around the QuickCheck / exhaustive-sweep launches only */
   uint64_t qc_launches, exh_launches;
+  uint32_t qc_tile;          /* QuickCheck targets per work item (autotuned unless tile_targets is set) */
+  uint32_t qc_blocks_per_cu; /* 0 = grid launch; else persistent launch with this many blocks per CU */
 } sx_stats;
 
 /* Enumeration semantics. */
@@ -96,7 +98,7 @@ typedef struct sx_enum_cfg {
   uint64_t n_targets;
   uint64_t first_target;  /* with targets == NULL: half-open range of the */
   uint64_t end_target;    /* pruned target index space (0, 0 = everything) */
-  uint32_t tile_targets;  /* QuickCheck targets per tile (0 = autotuned default) */
+  uint32_t tile_targets;  /* QuickCheck targets per tile, grid launch (0 = autotuned shape) */
   uint32_t survivor_cap;  /* compaction buffer entries per tile (0 = default) */
 } sx_enum_cfg;
 

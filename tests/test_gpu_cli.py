@@ -1,0 +1,64 @@
+"""The 6502opt command line (host/6502opt.cpp) end to end on the device:
+the reference CLI's subcommands, flags and output formats
+(cmd/6502opt/main.go)."""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "6502-optimizer_amd", "6502opt")
+
+
+def run(*args, check=True):
+    p = subprocess.run([CLI, *args], capture_output=True, text=True, timeout=120)
+    if check:
+        assert p.returncode == 0, p.stderr + p.stdout
+    return p
+
+
+def test_target_readme_example():
+    out = run("target", "TXA : TAX").stdout
+    assert "Target: TXA : TAX (2 bytes, 4 cycles)" in out
+    assert "Replacement: TXA (-1 bytes, -2 cycles)" in out
+
+
+def test_target_with_dead_carry():
+    out = run("target", "CLC : LDA #$05", "--dead-flags", "0x01", "--max-candidate", "2").stdout
+    assert "Replacement: LDA #$05 (-1 bytes, -2 cycles)" in out
+    out = run("target", "CLC : LDA #$05", "--max-candidate", "2").stdout
+    assert "No shorter replacement found." in out
+
+
+def test_target_length3_candidate():
+    out = run("target", "LDA #$01 : LDX #$02 : LDA #$03 : LDY #$04", "--max-candidate", "3").stdout
+    assert "Replacement: LDA #$03 : LDX #$02 : LDY #$04 (-2 bytes, -2 cycles)" in out
+
+
+def test_stoke_finds_and_writes(tmp_path):
+    js = tmp_path / "stoke.json"
+    out = run("stoke", "--target", "LDA #$01 : LDA #$02", "--chains", "4", "--iterations", "20000",
+              "--seed", "1", "--output", str(js)).stdout
+    assert "1 unique optimizations found" in out
+    assert "LDA #$02 (-2 bytes, -2 cycles)" in out
+    assert '"LDA #$02"' in js.read_text()
+
+
+def test_enumerate_slice_then_verify_jsonl(tmp_path):
+    jl = tmp_path / "rules.jsonl"
+    js = tmp_path / "rules.json"
+    out = run("enumerate", "--max-target", "2", "--first-target", "0", "--end-target", "20000",
+              "--jsonl", str(jl), "--output", str(js)).stdout
+    lines = jl.read_text().splitlines()
+    assert lines and all(line.startswith("{") for line in lines)
+    assert js.read_text().lstrip().startswith("[")
+    out = run("verify-jsonl", str(jl)).stdout
+    assert f"Verification complete: {len(lines)} total, {len(lines)} passed, 0 failed, 0 skipped" in out
+
+
+def test_bad_input_is_an_error():
+    p = run("target", "FOO #$01", check=False)
+    assert p.returncode != 0 and "failed to parse" in (p.stderr + p.stdout)
+    p = run("stoke", check=False)
+    assert p.returncode != 0 and "--target is required" in (p.stderr + p.stdout)
