@@ -125,6 +125,58 @@ __global__ __launch_bounds__(256) void k_gen_write(const uint64_t *block_offsets
   }
 }
 
+// Pruned length-3 targets of a range of raw Go-order triple indices
+// [r0, r0 + n) (index = i0 * 2863^2 + i1 * 2863 + i2): the same two-pass
+// ordered compaction, also recording each survivor's offset in the range.
+__device__ __forceinline__ void triple_words(uint64_t idx, uint32_t seq[3]) {
+  const uint64_t sq = (uint64_t)kNumInsts * kNumInsts;
+  seq[0] = go_inst((uint32_t)(idx / sq));
+  seq[1] = go_inst((uint32_t)((idx / kNumInsts) % kNumInsts));
+  seq[2] = go_inst((uint32_t)(idx % kNumInsts));
+}
+
+__global__ __launch_bounds__(256) void k_gen3_count(uint64_t r0, uint32_t n, uint32_t *block_counts) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  bool keep = false;
+  if (i < n) {
+    uint32_t seq[3];
+    triple_words(r0 + i, seq);
+    keep = !should_prune(seq, 3, c_reads, c_writes);
+  }
+  __shared__ uint32_t cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  uint64_t m = __ballot(keep);
+  if (lane_id() == 0) atomicAdd(&cnt, (uint32_t)__popcll(m));
+  __syncthreads();
+  if (threadIdx.x == 0) block_counts[blockIdx.x] = cnt;
+}
+
+__global__ __launch_bounds__(256) void k_gen3_write(uint64_t r0, uint32_t n, const uint64_t *block_offsets,
+                                                    uint32_t *out, uint32_t *out_off) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  bool keep = false;
+  uint32_t seq[3] = {0, 0, 0};
+  if (i < n) {
+    triple_words(r0 + i, seq);
+    keep = !should_prune(seq, 3, c_reads, c_writes);
+  }
+  __shared__ uint32_t wave_cnt[4];
+  uint64_t m = __ballot(keep);
+  uint32_t w = threadIdx.x >> 6;
+  if (lane_id() == 0) wave_cnt[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint64_t base = block_offsets[blockIdx.x];
+  for (uint32_t k = 0; k < w; k++) base += wave_cnt[k];
+  if (keep) {
+    uint64_t o = base + mbcnt(m);
+    out[3 * o] = seq[0];
+    out[3 * o + 1] = seq[1];
+    out[3 * o + 2] = seq[2];
+    out_off[o] = i;
+  }
+}
+
 // Per-block exclusive scan of uint32 counts into uint64 offsets (3 kernels).
 __global__ __launch_bounds__(1024) void k_scan_block(const uint32_t *in, uint64_t n, uint64_t *out,
                                                      uint64_t *block_sums) {
@@ -869,7 +921,7 @@ enum BufId {
   B_RANK, B_SURV, B_MID, B_COUNTERS, B_CUR, B_REJ, B_DONE, B_JOBS, B_CHUNKS, B_OFFS, B_SUMS,
   B_FAIL, B_RULES, B_SCRATCH, B_GEN_COUNTS, B_GEN_OFFS, B_BADOP, B_P_TW, B_P_TL, B_P_CW, B_P_CL,
   B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR, B_HEADS, B_RULEFLAG, B_RULEOFFS, B_RULES2,
-  B_RULETOT
+  B_RULETOT, B_TRAW
 };
 
 int set_err(sx_handle *h, int code, const char *fmt, ...) {
@@ -1207,7 +1259,10 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   if (cfg->bytes_model < 0 || cfg->bytes_model > 1 || cfg->hits < 0 || cfg->hits > 1 ||
       cfg->exh_mode < 0 || cfg->exh_mode > 1 || cfg->dead_flags > 0xFF)
     return set_err(h, SX_EINVAL, "bad mode");
-  if (!cfg->targets && tl != 2) return set_err(h, SX_EINVAL, "implicit targets need target_len 2");
+  // implicit targets: length 2 = the cached pruned list, indexed by position;
+  // length 3 = ranges of raw Go-order triple indices, pruned on the device
+  const bool implicit3 = !cfg->targets && tl == 3;
+  const uint64_t raw3 = (uint64_t)kNumInsts * kNumInsts * kNumInsts;
   HIPCHK(h, hipSetDevice(h->device));
   auto wall0 = std::chrono::steady_clock::now();
   const double qc_ms0 = h->k_qc_ms, exh_ms0 = h->k_exh_ms;
@@ -1223,6 +1278,12 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     int rc = check_words(h, cfg->targets, cfg->n_targets * tl);
     if (rc) return rc;
     nt_all = cfg->n_targets;
+  } else if (implicit3) {
+    uint64_t a = cfg->first_target, b = cfg->end_target;
+    if (a == 0 && b == 0) b = raw3;
+    if (a > b || b > raw3) return set_err(h, SX_EINVAL, "target range out of bounds");
+    tbase = a;
+    nt_all = b - a;  // raw indices; each segment keeps its unpruned triples
   } else {
     int rc = ensure_l2(h);
     if (rc) return rc;
@@ -1296,7 +1357,29 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   HIPCHK(h, hipEventElapsedTime(&ms_setup, e0, e1));
 
   for (uint64_t s0 = 0; s0 < nt_all; s0 += seg) {
-    const uint64_t nt = std::min<uint64_t>(seg, nt_all - s0);
+    uint64_t nt = std::min<uint64_t>(seg, nt_all - s0);
+    uint32_t *traw = nullptr;  // implicit length 3: offset of each target in the raw range
+    if (implicit3) {
+      uint32_t *tw3, *counts;
+      uint64_t *offs, *total;
+      const uint64_t nb = blocks_for(nt, 256);
+      ENSURE(B_TARGETS, uint32_t, nt * 3, tw3);
+      ENSURE(B_TRAW, uint32_t, nt, traw);
+      ENSURE(B_GEN_COUNTS, uint32_t, nb, counts);
+      ENSURE(B_GEN_OFFS, uint64_t, nb, offs);
+      ENSURE(B_TOTAL, uint64_t, 1, total);
+      hipLaunchKernelGGL(k_gen3_count, dim3((unsigned)nb), dim3(256), 0, h->stream, tbase + s0, (uint32_t)nt, counts);
+      int rc = scan_u32(h, counts, nb, offs, total);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_gen3_write, dim3((unsigned)nb), dim3(256), 0, h->stream, tbase + s0, (uint32_t)nt, offs,
+                         tw3, traw);
+      HIPCHK(h, hipGetLastError());
+      uint64_t kept = 0;
+      HIPCHK(h, hipMemcpyAsync(&kept, total, 8, hipMemcpyDeviceToHost, h->stream));
+      SYNC_WD("device sync");
+      nt = kept;
+      if (nt == 0) continue;
+    }
     uint32_t *tw;
     uint2 *tfp, *tv0;
     uint8_t *tbytes, *tcyc, *tprune;
@@ -1309,6 +1392,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     HIPCHK(h, hipEventRecord(e1, h->stream));
     if (cfg->targets) {
       HIPCHK(h, hipMemcpyAsync(tw, cfg->targets + s0 * tl, nt * tl * 4, hipMemcpyHostToDevice, h->stream));
+    } else if (implicit3) {
+      // already generated in place
     } else {
       HIPCHK(h, hipMemcpyAsync(tw, d_tsrc + s0 * 2, nt * 8, hipMemcpyDeviceToDevice, h->stream));
     }
@@ -1506,6 +1591,15 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     size_t old = h->rules.size();
     h->rules.resize(old + nr);
     if (nr) HIPCHK(h, hipMemcpy(h->rules.data() + old, rules_out, nr * sizeof(sx_rule), hipMemcpyDeviceToHost));
+    if (implicit3 && nr) {
+      // rules name their target by its raw Go-order triple index
+      std::vector<uint32_t> off(nt);
+      HIPCHK(h, hipMemcpy(off.data(), traw, nt * 4, hipMemcpyDeviceToHost));
+      for (size_t i = old; i < old + nr; i++) {
+        sx_rule &r = h->rules[i];
+        r.target_index = tbase + s0 + off[r.target_index - (tbase + s0)];
+      }
+    }
     float a = 0, b = 0, c = 0;
     HIPCHK(h, hipEventElapsedTime(&a, e1, e2));
     HIPCHK(h, hipEventElapsedTime(&b, e2, e3));

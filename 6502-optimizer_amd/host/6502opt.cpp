@@ -10,8 +10,11 @@
 // (cuda/6502_search_v2.cu:912-930); --bytes-model go|cuda, --hits first|all,
 // --exh projected|full, --no-mid and --device select the device semantics;
 // --first-target/--end-target select a range of the pruned target space (the
-// multi-process split of the CUDA tool, cu:775-777); stoke takes --seed (the
-// chains are reproducible) and reports the first verified rewrite.
+// multi-process split of the CUDA tool, cu:775-777); --max-target 3 walks raw
+// Go-order triple index ranges; --checkpoint FILE [--resume] [--chunk N]
+// appends JSONL chunk by chunk and records the next target index; stoke
+// takes --seed (the chains are reproducible) and reports the first verified
+// rewrite.
 #include <algorithm>
 #include <chrono>
 #include <cstdint>
@@ -272,8 +275,50 @@ int check(sx_handle *h, int rc, const char *what) {
   return 1;
 }
 
+// Checkpoint of a chunked enumeration (the role of pkg/result/checkpoint.go:
+// 9-37, as JSON text): the next unprocessed target index, the range end, the
+// rules written so far.
+struct Checkpoint {
+  uint64_t next = 0, end = 0, rules = 0;
+  int max_target = 2;
+  uint32_t dead = 0;
+};
+
+bool read_checkpoint(const std::string &path, Checkpoint *c) {
+  std::ifstream in(path);
+  if (!in) return false;
+  std::stringstream ss;
+  ss << in.rdbuf();
+  std::string t = ss.str();
+  auto num = [&](const char *key, uint64_t *v) {
+    size_t p = t.find(std::string("\"") + key + "\"");
+    if (p == std::string::npos) return false;
+    p = t.find(':', p);
+    if (p == std::string::npos) return false;
+    *v = strtoull(t.c_str() + p + 1, nullptr, 10);
+    return true;
+  };
+  uint64_t mt = 0, dd = 0;
+  if (!num("next", &c->next) || !num("end", &c->end) || !num("rules", &c->rules) || !num("max_target", &mt) ||
+      !num("dead_flags", &dd))
+    return false;
+  c->max_target = (int)mt;
+  c->dead = (uint32_t)dd;
+  return true;
+}
+
+bool write_checkpoint(const std::string &path, const Checkpoint &c) {
+  std::string tmp = path + ".tmp";
+  FILE *f = fopen(tmp.c_str(), "w");
+  if (!f) return false;
+  fprintf(f, "{\"next\": %llu, \"end\": %llu, \"rules\": %llu, \"max_target\": %d, \"dead_flags\": %u}\n",
+          (unsigned long long)c.next, (unsigned long long)c.end, (unsigned long long)c.rules, c.max_target, c.dead);
+  fclose(f);
+  return rename(tmp.c_str(), path.c_str()) == 0;  // the checkpoint is replaced atomically
+}
+
 int cmd_enumerate(int argc, char **argv) {
-  Args a = parse_args(argc, argv, 2, {"verbose", "no-mid"});
+  Args a = parse_args(argc, argv, 2, {"verbose", "no-mid", "resume"});
   int max_target = std::atoi(a.get("max-target", "2").c_str());
   uint32_t dead;
   if (!parse_dead_flags(a.get("dead-flags", "none"), &dead))
@@ -282,7 +327,11 @@ int cmd_enumerate(int argc, char **argv) {
   int hits = a.get("hits", "first") == "all" ? SX_HITS_ALL : SX_HITS_FIRST;
   int exh = a.get("exh", "projected") == "full" ? SX_EXH_FULL : SX_EXH_PROJECTED;
   int device = std::atoi(a.get("device", "0").c_str());
-  if (max_target != 2) return die("this build enumerates length-2 targets (--max-target 2)");
+  if (max_target != 2 && max_target != 3) return die("this build enumerates targets of length 2 or 3 (--max-target)");
+  std::string out = a.get("output", ""), jl = a.get("jsonl", ""), ckpt = a.get("checkpoint", "");
+  const bool resume = a.has("resume");
+  if (!ckpt.empty() && (jl.empty() || jl == "-" || !out.empty()))
+    return die("--checkpoint appends to a --jsonl file (and takes no --output)");
   printf("6502 Superoptimizer\n");
   printf("  Max target length: %d\n", max_target);
   printf("  Instructions: %d per position\n", 2863);
@@ -294,46 +343,97 @@ int cmd_enumerate(int argc, char **argv) {
   sx_enum_cfg cfg;
   memset(&cfg, 0, sizeof(cfg));
   cfg.struct_size = sizeof(cfg);
-  cfg.target_len = 2;
-  cfg.max_cand_len = 1;
+  cfg.target_len = max_target;
+  cfg.max_cand_len = max_target - 1;  // enumerator.go: candidates shorter than the target
   cfg.bytes_model = model;
   cfg.hits = hits;
   cfg.exh_mode = exh;
   cfg.use_mid = a.has("no-mid") ? 0 : 1;
   cfg.dead_flags = dead;
-  cfg.first_target = strtoull(a.get("first-target", "0").c_str(), nullptr, 10);
-  cfg.end_target = strtoull(a.get("end-target", "0").c_str(), nullptr, 10);
-  uint64_t n = 0;
-  sx_stats st;
+  // the target index space: positions in the pruned length-2 list, or raw
+  // Go-order triple indices for length 3
+  uint64_t total = 0;
+  if (max_target == 2) {
+    if (check(h, sx_pruned_targets(h, 0, 0, nullptr, &total), "sx_pruned_targets")) return 1;
+  } else {
+    total = 2863ull * 2863ull * 2863ull;
+  }
+  uint64_t first = strtoull(a.get("first-target", "0").c_str(), nullptr, 10);
+  uint64_t end = strtoull(a.get("end-target", "0").c_str(), nullptr, 10);
+  if (end == 0) end = total;
+  if (first > end || end > total) return die("target range out of bounds");
+  Checkpoint ck;
+  ck.next = first;
+  ck.end = end;
+  ck.max_target = max_target;
+  ck.dead = dead;
+  if (resume) {
+    if (ckpt.empty()) return die("--resume needs --checkpoint");
+    if (!read_checkpoint(ckpt, &ck)) return die("cannot read checkpoint " + ckpt);
+    if (ck.max_target != max_target || ck.dead != dead) return die("checkpoint is for another configuration");
+    printf("  Resuming at target %llu of [.., %llu), %llu rules so far\n", (unsigned long long)ck.next,
+           (unsigned long long)ck.end, (unsigned long long)ck.rules);
+  }
+  // chunks: everything at once unless checkpointing
+  const uint64_t chunk = ckpt.empty() ? (ck.end - ck.next)
+                                      : strtoull(a.get("chunk", max_target == 2 ? "1048576" : "16777216").c_str(),
+                                                 nullptr, 10);
+  FILE *jf = nullptr;
+  if (!jl.empty()) {
+    jf = jl == "-" ? stdout : fopen(jl.c_str(), resume ? "a" : "w");
+    if (!jf) return die("cannot create " + jl);
+  }
+  std::vector<sx_rule> all;
+  sx_stats tot;
+  memset(&tot, 0, sizeof(tot));
   auto t0 = std::chrono::steady_clock::now();
-  if (check(h, sx_enumerate(h, &cfg, nullptr, 0, &n, &st), "sx_enumerate")) return 1;
-  std::vector<sx_rule> rules(n);
-  uint64_t got = 0;
-  if (check(h, sx_fetch_rules(h, rules.data(), 0, n, &got), "sx_fetch_rules")) return 1;
+  while (ck.next < ck.end) {
+    cfg.first_target = ck.next;
+    cfg.end_target = std::min<uint64_t>(ck.end, ck.next + std::max<uint64_t>(chunk, 1));
+    uint64_t n = 0;
+    sx_stats st;
+    if (check(h, sx_enumerate(h, &cfg, nullptr, 0, &n, &st), "sx_enumerate")) return 1;
+    std::vector<sx_rule> rules(n);
+    uint64_t got = 0;
+    if (check(h, sx_fetch_rules(h, rules.data(), 0, n, &got), "sx_fetch_rules")) return 1;
+    tot.targets += st.targets;
+    tot.qc_pairs += st.qc_pairs;
+    tot.qc_hits += st.qc_hits;
+    tot.mid_pass += st.mid_pass;
+    tot.exh_pairs += st.exh_pairs;
+    if (jf) {
+      write_jsonl(jf, rules, model);
+      fflush(jf);
+    }
+    ck.rules += n;
+    ck.next = cfg.end_target;
+    if (!ckpt.empty()) {
+      if (!write_checkpoint(ckpt, ck)) return die("cannot write checkpoint " + ckpt);
+      double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      fprintf(stderr, "  [%.0fs] %llu/%llu (%.2f%%) | %llu rules\n", secs, (unsigned long long)ck.next,
+              (unsigned long long)ck.end, 100.0 * (double)ck.next / (double)std::max<uint64_t>(ck.end, 1),
+              (unsigned long long)ck.rules);
+    }
+    if (out.size() || a.has("verbose")) all.insert(all.end(), rules.begin(), rules.end());
+  }
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   printf("  %llu targets | %llu QC pairs | QC hits %llu | Mid %llu | Exh jobs %llu | %.2f s (%.3g pairs/s)\n",
-         (unsigned long long)st.targets, (unsigned long long)st.qc_pairs, (unsigned long long)st.qc_hits,
-         (unsigned long long)st.mid_pass, (unsigned long long)st.exh_pairs, secs, st.qc_pairs / secs);
+         (unsigned long long)tot.targets, (unsigned long long)tot.qc_pairs, (unsigned long long)tot.qc_hits,
+         (unsigned long long)tot.mid_pass, (unsigned long long)tot.exh_pairs, secs, tot.qc_pairs / secs);
   if (a.has("verbose"))
-    for (auto &r : rules)
+    for (auto &r : all)
       printf("  FOUND%s: %s -> %s (-%d bytes, -%d cycles)\n", r.pass == 2 ? " (dead flags)" : "",
              disasm_seq(r.src, r.src_len).c_str(), disasm_seq(r.rep, r.rep_len).c_str(), r.bytes_saved,
              r.cycles_saved);
-  printf("\nFound %llu optimizations\n", (unsigned long long)n);
-  std::string out = a.get("output", ""), jl = a.get("jsonl", "");
+  printf("\nFound %llu optimizations\n", (unsigned long long)ck.rules);
   if (!out.empty()) {
     FILE *f = fopen(out.c_str(), "w");
     if (!f) return die("cannot create " + out);
-    write_json(f, rules);
+    write_json(f, all);
     fclose(f);
     printf("Written to %s\n", out.c_str());
   }
-  if (!jl.empty()) {
-    FILE *f = jl == "-" ? stdout : fopen(jl.c_str(), "w");
-    if (!f) return die("cannot create " + jl);
-    write_jsonl(f, rules, model);
-    if (f != stdout) fclose(f);
-  }
+  if (jf && jf != stdout) fclose(jf);
   sx_close(h);
   return 0;
 }

@@ -40,6 +40,10 @@ PAIRS = N_TARGETS * N_CANDS
 OPS_QC_PAIR = 128
 OPS_EXH_STATE = 36
 PEAK_INT32 = 256 * 64 * 2.4e9  # 256 CUs x 64 lanes x 2.4 GHz = 39.3 T int32 ops/s
+# The reference's one published number: the CUDA tool's full length-2 search,
+# 21,855,031,156 pairs in 1,041 s on an RTX 4060 Ti (README.md:127-132).
+PUBLISHED_S = 1041.0
+PUBLISHED_PAIRS_PER_S = PAIRS / PUBLISHED_S
 PEAK_HBM = 8.0e12
 
 
@@ -72,6 +76,23 @@ def cpu_baseline(sample_targets: int, dead: int):
                       f"dead 0x{dead:02X}, C restatement of the Go enumerate (first hit, Go order), "
                       f"{dt:.2f} s, {len(rules)} rules; full-run extrapolation "
                       f"{dt * N_TARGETS / sample_targets:.0f} s"}
+
+
+def c2_cuda_section(dev):
+    """The published workload itself: the CUDA tool's semantics (every hit,
+    M ops 2 bytes, one masked pass) over the same 21,855,031,156 pairs at
+    --dead-flags 0x00; one warm-up and one timed run."""
+    kw = dict(dead=0, hits=core.HITS_ALL, bytes_model=1)
+    dev.enumerate(None, **kw)
+    t0 = time.perf_counter()
+    rules, st = dev.enumerate(None, **kw)
+    dt = time.perf_counter() - t0
+    print(f"bench: C2 CUDA semantics {dt:.3f} s, {len(rules)} rules", file=sys.stderr, flush=True)
+    return {"workload": "full length-2, CUDA-tool semantics (all hits, M ops 2 bytes), dead 0x00",
+            "wall_s": dt, "rules": int(len(rules)), "qc_hits": int(st["qc_hits"]), "mid_pass": int(st["mid_pass"]),
+            "pairs_per_s": PAIRS / dt, "speedup_vs_published": PUBLISHED_S / dt,
+            "published": {"wall_s": PUBLISHED_S, "rules": 1078897, "qc_hits": 20654396, "mid_pass": 5373297,
+                          "hardware": "RTX 4060 Ti (README.md:127-136)"}}
 
 
 def c3_section(dev):
@@ -180,6 +201,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-cuda", action="store_true", help="skip the CUDA-tool-semantics run")
     ap.add_argument("--no-stoke", action="store_true")
     ap.add_argument("--stoke-steps", type=int, default=1_000_000)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -234,7 +256,10 @@ def main():
     out = {
         "metric": "QC pairs/s", "value": PAIRS / (ms * 1e-3), "unit": "pairs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic (exhaustive target space)",
+        "scaling": "weak", "vs_baseline": PAIRS / (ms * 1e-3) / PUBLISHED_PAIRS_PER_S,
+        "vs_baseline_basis": "published 2.10e7 pairs/s: the CUDA tool's all-hits run of the same pairs "
+                             "(README.md:127-132); c2_cuda times that exact workload here",
+        "dtype": "u8", "data": "synthetic (exhaustive target space)",
         "config": {"workload": "full length-2 enumeration: 7,633,612 pruned targets x 2,863 candidates, "
                                "QC->Mid->complete Exhaustive, Go semantics (first hit)",
                    "dead_flags": f"0x{args.dead:02X}", "targets": N_TARGETS, "candidates": N_CANDS},
@@ -248,6 +273,8 @@ def main():
                       "form": "persistent, per-XCD work queues" if st["qc_blocks_per_cu"] else "grid"},
         "roofline": roof,
     }
+    if not args.no_cuda:
+        out["c2_cuda"] = c2_cuda_section(dev)
     if not args.no_c3:
         out["c3"] = c3_section(dev)
     if not args.no_stoke:
