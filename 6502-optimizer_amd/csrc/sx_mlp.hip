@@ -56,46 +56,72 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 // C[M][N] (fp32, ldc) = sum_k A(m,k) B(k,n) with A(m,k) = A[m*sam + k*sak] and
-// B(k,n) = B[k*sbk + n*sbn] (bf16).  Epilogue: + bias[n] (if bias), relu (if
-// relu), and an optional bf16 copy Cb.  64x64 tile per 256-thread block, each
-// wave a 32x32 quarter as 2x2 MFMA 16x16 blocks, K in steps of 16 staged in LDS.
+// B(k,n) = B[k*sbk + n*sbn] (bf16).  64x64 tile per 256-thread block, each
+// wave a 32x32 quarter as 2x2 MFMA 16x16 blocks; K in steps of 32 staged in
+// LDS (two MFMA k-steps of 16 per stage).  gridDim.z > 1 splits K: block z
+// takes K range [z * kchunk, ...) and adds its partial product to C with
+// atomics (C zeroed by the caller; no epilogue).  Otherwise the epilogue
+// applies + bias[n] (if bias), relu (if relu), + C (if accumulate) and
+// writes an optional bf16 copy Cb.
 __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, long sam, long sak,
                                               const uint16_t *__restrict__ B, long sbk, long sbn, float *C,
                                               long ldc, uint16_t *Cb, int M, int N, int K,
                                               const float *__restrict__ bias, int relu, int accumulate) {
-  __shared__ uint16_t As[64][16 + 4];
-  __shared__ uint16_t Bs[16][64 + 4];
+  __shared__ uint16_t As[64][32 + 4];
+  __shared__ uint16_t Bs[32][64 + 4];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+  const int kchunk = (((K + gridDim.z - 1) / gridDim.z) + 31) & ~31;
+  const int kbeg = blockIdx.z * kchunk, kend = min(K, kbeg + kchunk);
   v4f acc[2][2];
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    // stage A: 64 x 16 = 1024 elements, 4 per thread
-    for (int e = tid; e < 64 * 16; e += 256) {
-      int r = e >> 4, k = e & 15;
-      int gm = m0 + r, gk = k0 + k;
-      As[r][k] = (gm < M && gk < K) ? A[(long)gm * sam + (long)gk * sak] : (uint16_t)0;
+  for (int k0 = kbeg; k0 < kend; k0 += 32) {
+    // stage A: 64 x 32, B: 32 x 64; consecutive threads walk the operand's
+    // contiguous dimension so the loads coalesce
+    if (sak == 1) {
+      for (int e = tid; e < 64 * 32; e += 256) {
+        const int r = e >> 5, k = e & 31, gm = m0 + r, gk = k0 + k;
+        As[r][k] = (gm < M && gk < kend) ? A[(long)gm * sam + gk] : (uint16_t)0;
+      }
+    } else {
+      for (int e = tid; e < 64 * 32; e += 256) {
+        const int r = e & 63, k = e >> 6, gm = m0 + r, gk = k0 + k;
+        As[r][k] = (gm < M && gk < kend) ? A[(long)gm * sam + (long)gk * sak] : (uint16_t)0;
+      }
     }
-    for (int e = tid; e < 16 * 64; e += 256) {
-      int k = e >> 6, c = e & 63;
-      int gk = k0 + k, gn = n0 + c;
-      Bs[k][c] = (gk < K && gn < N) ? B[(long)gk * sbk + (long)gn * sbn] : (uint16_t)0;
-    }
-    __syncthreads();
-    const int kq = (lane >> 4) * 4, l16 = lane & 15;
-    for (int i = 0; i < 2; i++) {
-      v4s a;
-      for (int q = 0; q < 4; q++) a[q] = (short)As[wm + i * 16 + l16][kq + q];
-      for (int j = 0; j < 2; j++) {
-        v4s b;
-        for (int q = 0; q < 4; q++) b[q] = (short)Bs[kq + q][wn + j * 16 + l16];
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i][j], 0, 0, 0);
+    if (sbn == 1) {
+      for (int e = tid; e < 32 * 64; e += 256) {
+        const int k = e >> 6, c = e & 63, gk = k0 + k, gn = n0 + c;
+        Bs[k][c] = (gk < kend && gn < N) ? B[(long)gk * sbk + gn] : (uint16_t)0;
+      }
+    } else {
+      for (int e = tid; e < 32 * 64; e += 256) {
+        const int k = e & 31, c = e >> 5, gk = k0 + k, gn = n0 + c;
+        Bs[k][c] = (gk < kend && gn < N) ? B[(long)gk * sbk + (long)gn * sbn] : (uint16_t)0;
       }
     }
     __syncthreads();
+    const int l16 = lane & 15;
+#pragma unroll
+    for (int kk = 0; kk < 32; kk += 16) {
+      const int kq = kk + (lane >> 4) * 4;
+      v4s a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+        for (int q = 0; q < 4; q++) a[i][q] = (short)As[wm + i * 16 + l16][kq + q];
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+        for (int q = 0; q < 4; q++) b[j][q] = (short)Bs[kq + q][wn + j * 16 + l16];
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
   }
+  const bool split = gridDim.z > 1;
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++)
       for (int q = 0; q < 4; q++) {
@@ -103,6 +129,10 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, lo
         int c = n0 + wn + j * 16 + (lane & 15);
         if (r < M && c < N) {
           float v = acc[i][j][q];
+          if (split) {
+            atomicAdd(&C[(long)r * ldc + c], v);
+            continue;
+          }
           if (accumulate) v += C[(long)r * ldc + c];
           if (bias) v += bias[c];
           if (relu) v = v > 0.f ? v : 0.f;
@@ -191,13 +221,18 @@ __global__ void k_softmax_ce(const float *Z, long ldz, const uint16_t *lab, int 
   if (threadIdx.x == 0 && loss_rows && y >= 0) loss_rows[b] = -(z[y] - mx - logf(sum));
 }
 
-// column sums: out[n] (+)= sum_b in[b][n]
-__global__ void k_colsum(const float *in, long ld, int B, int N, float *out) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
-  float s = 0.f;
-  for (int b = 0; b < B; b++) s += in[(long)b * ld + n];
-  out[n] = s;
+// column sums: out[n] = sum_b in[b][n]; a block takes 64 columns, its 4
+// waves a quarter of the rows each, reduced through LDS.
+__global__ __launch_bounds__(256) void k_colsum(const float *in, long ld, int B, int N, float *out) {
+  __shared__ float part[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int n = blockIdx.x * 64 + lane;
+  float acc = 0.f;
+  if (n < N)
+    for (int b = wv; b < B; b += 4) acc += in[(long)b * ld + n];
+  part[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0 && n < N) out[n] = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
 }
 
 // dH *= relu'(H), and a bf16 copy
@@ -319,18 +354,24 @@ void enqueue_fwd_bwd(sx_mlp *m, int B, bool backward, float *probs, hipStream_t 
   if (!backward) return;
   hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(64), 0, s, m->lossr, B, m->loss);
   hipLaunchKernelGGL(k_fill_f32, dim3(nb(kParams, 256)), dim3(256), 0, s, m->g, kParams, 0.f);
+  // products over the batch (K = B) and over the 2,871 outputs are split in K
+  // and accumulated with atomics into zeroed outputs: the output tiles alone
+  // are too few to fill the chip
+  const unsigned split_b = (unsigned)std::max(1, B / 256);
   // dW2 = H^T dZ   (A = Hb viewed transposed: A(m=k_hid, k=b) = Hb[b*kHid + m])
-  hipLaunchKernelGGL(k_gemm, dim3(nb(kOut, 64), nb(kHid, 64)), dim3(256), 0, s, m->Hb, 1L, (long)kHid, m->dZb,
+  hipLaunchKernelGGL(k_gemm, dim3(nb(kOut, 64), nb(kHid, 64), split_b), dim3(256), 0, s, m->Hb, 1L, (long)kHid, m->dZb,
                      (long)kOutPad, 1L, m->g + kOffW2, (long)kOut, (uint16_t *)nullptr, kHid, kOut, B,
                      (const float *)nullptr, 0, 0);
-  hipLaunchKernelGGL(k_colsum, dim3(nb(kOut, 256)), dim3(256), 0, s, m->dZ, (long)kOutPad, B, kOut, m->g + kOffB2);
+  hipLaunchKernelGGL(k_colsum, dim3(nb(kOut, 64)), dim3(256), 0, s, m->dZ, (long)kOutPad, B, kOut, m->g + kOffB2);
   // dH = dZ W2^T  (B(k=c, n=j) = W2[j*kOut + c])
-  hipLaunchKernelGGL(k_gemm, dim3(nb(kHid, 64), nb(B, 64)), dim3(256), 0, s, m->dZb, (long)kOutPad, 1L, W2b, 1L,
+  hipLaunchKernelGGL(k_fill_f32, dim3(nb((size_t)B * kHid, 256)), dim3(256), 0, s, m->dH, (size_t)B * kHid, 0.f);
+  hipLaunchKernelGGL(k_gemm, dim3(nb(kHid, 64), nb(B, 64), 8), dim3(256), 0, s, m->dZb, (long)kOutPad, 1L, W2b, 1L,
                      (long)kOut, m->dH, (long)kHid, (uint16_t *)nullptr, B, kHid, kOut, (const float *)nullptr, 0, 0);
   hipLaunchKernelGGL(k_relu_back, dim3(nb((size_t)B * kHid, 256)), dim3(256), 0, s, m->dH, m->H, m->dHb, B * kHid);
-  hipLaunchKernelGGL(k_colsum, dim3(nb(kHid, 256)), dim3(256), 0, s, m->dH, (long)kHid, B, kHid, m->g + kOffB1);
+  hipLaunchKernelGGL(k_colsum, dim3(nb(kHid, 64)), dim3(256), 0, s, m->dH, (long)kHid, B, kHid, m->g + kOffB1);
   // dW1 = X^T dH
-  hipLaunchKernelGGL(k_gemm, dim3(nb(kHid, 64), nb(kIn, 64)), dim3(256), 0, s, m->X, 1L, (long)kIn, m->dHb, (long)kHid,
+  hipLaunchKernelGGL(k_gemm, dim3(nb(kHid, 64), nb(kIn, 64), (unsigned)std::max(1, B / 128)), dim3(256), 0, s, m->X, 1L,
+                     (long)kIn, m->dHb, (long)kHid,
                      1L, m->g + kOffW1, (long)kHid, (uint16_t *)nullptr, kIn, kHid, B, (const float *)nullptr, 0, 0);
   // dX = dH W1^T, scattered into dE
   hipLaunchKernelGGL(k_gemm, dim3(nb(kIn, 64), nb(B, 64)), dim3(256), 0, s, m->dHb, (long)kHid, 1L, W1b, 1L,

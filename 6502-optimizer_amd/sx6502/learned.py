@@ -146,7 +146,7 @@ def harvest_stoke(dev: core.Device, targets: Sequence[Sequence[int]], *, chains:
 
 
 def train_proposal(dev: core.Device, ctx: np.ndarray, labels: np.ndarray, *, steps: int = 20000,
-                   batch: int = 256, lr: float = 1e-3, seed: int = 0) -> Tuple[core.Mlp, np.ndarray]:
+                   batch: int = 1024, lr: float = 2e-3, seed: int = 0) -> Tuple[core.Mlp, np.ndarray]:
     """Train a fresh network for `steps` Adam steps on the examples."""
     if len(labels) == 0:
         raise ValueError("no training examples")
@@ -155,6 +155,6 @@ def train_proposal(dev: core.Device, ctx: np.ndarray, labels: np.ndarray, *, ste
     return m, losses
 
 
-def proposal_tables(m: core.Mlp, targets: Sequence[Sequence[int]], eps: float = 0.1) -> np.ndarray:
+def proposal_tables(m: core.Mlp, targets: Sequence[Sequence[int]], eps: float = 0.05) -> np.ndarray:
     """Per-target uint32 instruction tables (rows sum to 2^32) for sx_stoke_run."""
     return m.proposal(tokens(targets), eps)

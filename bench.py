@@ -124,15 +124,16 @@ def _stoke_stats(res, extra_s: float = 0.0):
             "bytes_saved": int(res["bytes_saved"][v].sum())}
 
 
-def stoke_section(dev, steps: int):
+def stoke_section(dev, steps: int, train_steps: int = 20000, harvest_targets: int = 131072):
     """STOKE with uniform proposals against STOKE with the learned proposal on
     the 256 committed length-4..8 targets (16 chains each, `steps` steps per
     chain, T0 = 1 and T x 0.9999 per step, Philox seed 2).  The learned side
     first harvests rewrites (the enumerator's length-2 rules and length-3
-    rules of the 65,536 committed targets; cold-chain STOKE on 65,536 other
-    random targets), trains the network for 20,000 Adam steps on the device,
-    and computes the 256 proposal tables; the time of that last step is added
-    to every learned target's time."""
+    rules of the 65,536 committed targets; six cold-chain STOKE runs on
+    131,072 other random targets), trains the network for 20,000 Adam steps
+    (batch 1,024) on the device, and computes the 256 proposal tables (5%
+    uniform share); the time of that last step is added to every learned
+    target's time."""
     from sx6502 import learned
     gold = os.path.join(ROOT, "tests", "golden")
     t = np.load(os.path.join(gold, "c4_targets.npy"))
@@ -143,22 +144,25 @@ def stoke_section(dev, steps: int):
     ru = dev.stoke(targets, **kw)
     wall_u = time.perf_counter() - t0
     print(f"bench: STOKE uniform {wall_u:.2f} s", file=sys.stderr, flush=True)
+    print("bench: harvesting rewrites", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     pairs_e = learned.harvest_enumerator(dev, np.load(os.path.join(gold, "c3_targets.npy")), dead=0, l3_dead=0x01)
-    train_t = learned.random_targets(65536, seed=1000)
+    train_t = learned.random_targets(harvest_targets, seed=1000)
     pairs_s = []
-    for sd in (11, 12, 14, 15):
+    for sd in (11, 12, 13, 14, 15, 16):
         pairs_s += learned.harvest_stoke(dev, train_t, chains=4, steps=4096, seed=sd)
+        print(f"bench: harvest seed {sd}: {len(pairs_s)} STOKE rewrites", file=sys.stderr, flush=True)
     harvest_s = time.perf_counter() - t0
     ce, le = learned.examples(pairs_e, "dropped")
-    cs, ls = learned.examples(pairs_s, "dropped", repeat=4)
+    cs, ls = learned.examples(pairs_s, "dropped", repeat=2)
     t0 = time.perf_counter()
-    m, losses = learned.train_proposal(dev, np.concatenate([ce, cs]), np.concatenate([le, ls]), steps=20000,
-                                       batch=256, lr=1e-3, seed=3)
+    print(f"bench: training {train_steps} Adam steps", file=sys.stderr, flush=True)
+    m, losses = learned.train_proposal(dev, np.concatenate([ce, cs]), np.concatenate([le, ls]), steps=train_steps,
+                                       batch=1024, lr=2e-3, seed=3)
     train_s = time.perf_counter() - t0
     print(f"bench: harvest {harvest_s:.1f} s, training {train_s:.1f} s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    w = learned.proposal_tables(m, targets, 0.1)
+    w = learned.proposal_tables(m, targets, 0.05)
     tables_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     rl = dev.stoke(targets, weights=w, **kw)
@@ -173,7 +177,7 @@ def stoke_section(dev, steps: int):
         "uniform": dict(su, wall_s=wall_u),
         "learned": dict(sl, wall_s=wall_l, tables_s=tables_s),
         "speedup_median": (su["median_s"] / sl["median_s"]) if su["median_s"] and sl["median_s"] else None,
-        "training": {"harvest_s": harvest_s, "train_s": train_s, "adam_steps": 20000, "batch": 256,
+        "training": {"harvest_s": harvest_s, "train_s": train_s, "adam_steps": train_steps, "batch": 1024,
                      "examples": int(len(le) + len(ls)), "enumerator_rewrites": len(pairs_e),
                      "stoke_rewrites": len(pairs_s), "loss_first100": float(losses[:100].mean()),
                      "loss_last500": float(losses[-500:].mean())},
@@ -204,6 +208,8 @@ def main():
     ap.add_argument("--no-cuda", action="store_true", help="skip the CUDA-tool-semantics run")
     ap.add_argument("--no-stoke", action="store_true")
     ap.add_argument("--stoke-steps", type=int, default=1_000_000)
+    ap.add_argument("--train-steps", type=int, default=20000, help="Adam steps of the proposal network")
+    ap.add_argument("--harvest-targets", type=int, default=131072)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -278,7 +284,7 @@ def main():
     if not args.no_c3:
         out["c3"] = c3_section(dev)
     if not args.no_stoke:
-        out["stoke"] = stoke_section(dev, args.stoke_steps)
+        out["stoke"] = stoke_section(dev, args.stoke_steps, args.train_steps, args.harvest_targets)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.dead)
     print(json.dumps(out))
