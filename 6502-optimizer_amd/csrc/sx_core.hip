@@ -227,38 +227,58 @@ __global__ __launch_bounds__(1024) void k_scan_add(uint64_t *out, uint64_t n, co
 // Outputs of n sequences on the 32 vectors: one wave per sequence, lane v
 // (< 32) runs vector v.  fp[(i * 32 + v)] = (lo, hi) with P masked by 0xCF.
 // Also writes per-sequence byte size, cycles and the pruner verdict.
+// Outputs of sequences on the 32 Mid vectors.  A wave takes kFpPerWave
+// sequences in turn (lane v < 32 runs vector v with the sequence's
+// wave-uniform opcodes), which spreads the per-wave setup over several
+// sequences; lane k < kFpPerWave computes bytes, cycles and the pruner flag of
+// the wave's k-th sequence.
+constexpr uint32_t kFpPerWave = 8;
 __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uint32_t len,
                                                       uint64_t n, int bytes_model, uint2 *fp,
                                                       uint8_t *bytes, uint8_t *cyc, uint8_t *prune,
                                                       uint32_t *bad_op) {
-  uint64_t seq = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (seq >= n) return;
-  uint32_t lane = lane_id();
-  uint32_t w[4];
-  bool bad = false;
-  for (uint32_t k = 0; k < len; k++) {
-    w[k] = uni(words[seq * len + k]);
-    bad |= (w[k] & 0xFFFFu) >= OP_COUNT;
-  }
-  if (bad) {
-    if (lane == 0) atomicOr(bad_op, 1u);
-    return;
-  }
-  if (lane < 32) {
-    State st = load_vec(lane);
-    run_seq_uniform(st, w, (int)len);
-    st.p &= kPMask;
-    fp[seq * 32 + lane] = make_uint2(pack_lo(st), pack_hi(st));
-  }
-  if (lane == 0) {
-    int b = 0, c = 0;
+  const uint64_t first = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kFpPerWave;
+  if (first >= n) return;
+  const uint32_t lane = lane_id();
+  const uint64_t last = first + kFpPerWave < n ? first + kFpPerWave : n;
+  bool bad_any = false;
+  for (uint64_t seq = first; seq < last; seq++) {
+    uint32_t w[4];
+    bool bad = false;
     for (uint32_t k = 0; k < len; k++) {
-      b += byte_size(w[k] & 0xFFFFu, bytes_model);
-      c += cycles(w[k] & 0xFFFFu);
+      w[k] = uni(words[seq * len + k]);
+      bad |= (w[k] & 0xFFFFu) >= OP_COUNT;
     }
-    bytes[seq] = (uint8_t)b;
-    cyc[seq] = (uint8_t)c;
-    prune[seq] = should_prune(w, (int)len, c_reads, c_writes) ? 1 : 0;
+    if (bad) {
+      bad_any = true;
+      continue;
+    }
+    if (lane < 32) {
+      State st = load_vec(lane);
+      run_seq_uniform(st, w, (int)len);
+      st.p &= kPMask;
+      fp[seq * 32 + lane] = make_uint2(pack_lo(st), pack_hi(st));
+    }
+  }
+  if (bad_any && lane == 0) atomicOr(bad_op, 1u);
+  const uint64_t seq = first + lane;
+  if (lane < kFpPerWave && seq < last) {
+    uint32_t w[4] = {0, 0, 0, 0};
+    bool bad = false;
+    for (uint32_t k = 0; k < len; k++) {
+      w[k] = words[seq * len + k];
+      bad |= (w[k] & 0xFFFFu) >= OP_COUNT;
+    }
+    if (!bad) {
+      int b = 0, c = 0;
+      for (uint32_t k = 0; k < len; k++) {
+        b += byte_size(w[k] & 0xFFFFu, bytes_model);
+        c += cycles(w[k] & 0xFFFFu);
+      }
+      bytes[seq] = (uint8_t)b;
+      cyc[seq] = (uint8_t)c;
+      prune[seq] = should_prune(w, (int)len, c_reads, c_writes) ? 1 : 0;
+    }
   }
 }
 
@@ -429,12 +449,32 @@ __global__ __launch_bounds__(256) void k_midcheck(const uint2 *__restrict__ surv
       qc_hit = true;
       bool ok = true;
       if (use_mid) {
+        // vectors 8..31 as 16-byte loads issued together (two vectors per
+        // load: (lo, hi, lo, hi)), in two stages so most failures stop after
+        // the first 8
         const uint32_t mlo = 0x00FFFFFFu | (pm << 24);
-        const uint2 *a = tfp + (uint64_t)t * 32;
-        const uint2 *b = cfp + (uint64_t)slot * 32;
-        for (int v = 8; v < 32 && ok; v++) {
-          uint2 x = a[v], y = b[v];
-          ok = (((x.x ^ y.x) & mlo) | (x.y ^ y.y)) == 0;
+        const uint4 *a = reinterpret_cast<const uint4 *>(tfp + (uint64_t)t * 32 + 8);
+        const uint4 *b = reinterpret_cast<const uint4 *>(cfp + (uint64_t)slot * 32 + 8);
+        auto same = [&](const uint4 &x, const uint4 &y) {
+          return ((((x.x ^ y.x) & mlo) | (x.y ^ y.y)) | (((x.z ^ y.z) & mlo) | (x.w ^ y.w))) == 0;
+        };
+        uint4 x[4], y[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          x[k] = a[k];
+          y[k] = b[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) ok = ok && same(x[k], y[k]);
+        if (ok) {
+          uint4 x2[8], y2[8];
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            x2[k] = a[4 + k];
+            y2[k] = b[4 + k];
+          }
+#pragma unroll
+          for (int k = 0; k < 8; k++) ok = ok && same(x2[k], y2[k]);
         }
       }
       pass_ok = ok;
@@ -558,12 +598,17 @@ __global__ __launch_bounds__(256) void k_exh_plan(Job *jobs, uint32_t nj, int mo
   Job j = jobs[i];
   Plan pl = make_plan(j.tw, j.tl, j.cw, j.cl, j.pmask, mode, c_reads, c_dep);
   uint32_t total = 0;
+  uint64_t states = 0;
   for (int s = 0; s < pl.nsets; s++) {
     j.sets[s] = pl.sets[s];
     j.logsz[s] = pl.logsz[s];
     total += pl.logsz[s] > kChunkLog ? (1u << (pl.logsz[s] - kChunkLog)) : 1u;
+    states += 1ull << pl.logsz[s];
   }
   j.nsets = pl.nsets;
+  // a job whose sub-sweeps together fit one chunk is swept by one wave
+  j.pad[0] = (pl.nsets > 1 && states <= (1ull << kChunkLog)) ? 1 : 0;
+  if (j.pad[0]) total = 1;
   jobs[i] = j;
   chunks[i] = total;
   fail[i] = 0;
@@ -598,26 +643,20 @@ __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, u
   const Job *jp = &jobs[ji];
   uint64_t local = c - offs[ji];
   const uint32_t nsets = uni(jp->nsets);
-  uint32_t set = 0, lg = 0;
-  for (uint32_t s = 0; s < nsets; s++) {
-    uint32_t l = uni(jp->logsz[s]);
-    uint64_t nch = l > kChunkLog ? (1ull << (l - kChunkLog)) : 1ull;
-    if (local < nch) {
-      set = uni(jp->sets[s]);
-      lg = l;
-      break;
+  const bool packed = uni(jp->pad[0]) != 0;  // one wave sweeps every sub-sweep of the job
+  uint32_t s_first = 0;
+  if (!packed) {
+    for (uint32_t s = 0; s < nsets; s++) {
+      uint32_t l = uni(jp->logsz[s]);
+      uint64_t nch = l > kChunkLog ? (1ull << (l - kChunkLog)) : 1ull;
+      if (local < nch) {
+        s_first = s;
+        break;
+      }
+      local -= nch;
     }
-    local -= nch;
   }
-  // bit positions of the varying registers in the 64-bit state index
-  uint32_t sh[8], wd[8], pos = 0;
-#pragma unroll
-  for (int r = 0; r < 8; r++) {
-    uint32_t w = (set >> r) & 1u ? (r == RP ? 1u : 8u) : 0u;
-    sh[r] = pos;
-    wd[r] = w;
-    pos += w;
-  }
+  const uint32_t s_last = packed ? nsets : s_first + 1;
   const uint32_t tl = uni(jp->tl), cl = uni(jp->cl);
   uint32_t tw[kJobWords], cw[kJobWords];
 #pragma unroll
@@ -626,37 +665,55 @@ __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, u
     cw[k] = uni(jp->cw[k]);
   }
   const uint32_t pm = uni(jp->pmask);
-  const uint64_t start = local << kChunkLog;
-  const uint32_t count = lg < kChunkLog ? (1u << lg) : (1u << kChunkLog);
   // states this wave evaluates, added to a sharded counter when it leaves
   uint32_t done_states = 0;
-  for (uint32_t it = 0; it < count; it += kWave) {
-    done_states = it + kWave < count ? it + kWave : count;
-    const uint32_t k = it + lane;
-    bool bad = false;
-    if (k < count) {
-      const uint64_t idx = start + k;
-      State s;
-      s.a = wd[RA] ? (uint32_t)(idx >> sh[RA]) & 0xFFu : 0u;
-      s.x = wd[RX] ? (uint32_t)(idx >> sh[RX]) & 0xFFu : 0u;
-      s.y = wd[RY] ? (uint32_t)(idx >> sh[RY]) & 0xFFu : 0u;
-      s.p = wd[RP] ? (uint32_t)(idx >> sh[RP]) & 0x1u : 0u;
-      s.s = wd[RS] ? (uint32_t)(idx >> sh[RS]) & 0xFFu : 0xFDu;
-      s.m = wd[RM] ? (uint32_t)(idx >> sh[RM]) & 0xFFu : 0u;
-      s.s0 = wd[RS0] ? (uint32_t)(idx >> sh[RS0]) & 0xFFu : 0u;
-      s.s1 = wd[RS1] ? (uint32_t)(idx >> sh[RS1]) & 0xFFu : 0u;
-      State ts = s, cs = s;
-      for (uint32_t q = 0; q < tl; q++) step(ts, tw[q] & 0xFFFFu, tw[q] >> 16);
-      for (uint32_t q = 0; q < cl; q++) step(cs, cw[q] & 0xFFFFu, cw[q] >> 16);
-      uint32_t diff = (ts.a ^ cs.a) | (ts.x ^ cs.x) | (ts.y ^ cs.y) | ((ts.p ^ cs.p) & pm) |
-                      (ts.s ^ cs.s) | (ts.m ^ cs.m) | (ts.s0 ^ cs.s0) | (ts.s1 ^ cs.s1);
-      bad = diff != 0;
+  bool failed = false;
+  for (uint32_t si = s_first; si < s_last && !failed; si++) {
+    const uint32_t set = uni(jp->sets[si]), lg = uni(jp->logsz[si]);
+    // bit positions of the varying registers in the 64-bit state index
+    uint32_t sh[8], wd[8], pos = 0;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      uint32_t w = (set >> r) & 1u ? (r == RP ? 1u : 8u) : 0u;
+      sh[r] = pos;
+      wd[r] = w;
+      pos += w;
     }
-    if (__ballot(bad)) {
-      if (lane == 0) ((volatile uint32_t *)fail)[ji] = 1u;
-      break;
+    const uint64_t start = packed ? 0 : local << kChunkLog;
+    const uint32_t count = lg < kChunkLog ? (1u << lg) : (1u << kChunkLog);
+    const uint32_t done_before = done_states;
+    for (uint32_t it = 0; it < count; it += kWave) {
+      done_states = done_before + (it + kWave < count ? it + kWave : count);
+      const uint32_t k = it + lane;
+      bool bad = false;
+      if (k < count) {
+        const uint64_t idx = start + k;
+        State s;
+        s.a = wd[RA] ? (uint32_t)(idx >> sh[RA]) & 0xFFu : 0u;
+        s.x = wd[RX] ? (uint32_t)(idx >> sh[RX]) & 0xFFu : 0u;
+        s.y = wd[RY] ? (uint32_t)(idx >> sh[RY]) & 0xFFu : 0u;
+        s.p = wd[RP] ? (uint32_t)(idx >> sh[RP]) & 0x1u : 0u;
+        s.s = wd[RS] ? (uint32_t)(idx >> sh[RS]) & 0xFFu : 0xFDu;
+        s.m = wd[RM] ? (uint32_t)(idx >> sh[RM]) & 0xFFu : 0u;
+        s.s0 = wd[RS0] ? (uint32_t)(idx >> sh[RS0]) & 0xFFu : 0u;
+        s.s1 = wd[RS1] ? (uint32_t)(idx >> sh[RS1]) & 0xFFu : 0u;
+        State ts = s, cs = s;
+        for (uint32_t q = 0; q < tl; q++) step(ts, tw[q] & 0xFFFFu, tw[q] >> 16);
+        for (uint32_t q = 0; q < cl; q++) step(cs, cw[q] & 0xFFFFu, cw[q] >> 16);
+        uint32_t diff = (ts.a ^ cs.a) | (ts.x ^ cs.x) | (ts.y ^ cs.y) | ((ts.p ^ cs.p) & pm) |
+                        (ts.s ^ cs.s) | (ts.m ^ cs.m) | (ts.s0 ^ cs.s0) | (ts.s1 ^ cs.s1);
+        bad = diff != 0;
+      }
+      if (__ballot(bad)) {
+        if (lane == 0) ((volatile uint32_t *)fail)[ji] = 1u;
+        failed = true;
+        break;
+      }
+      if ((it & (32 * kWave - 1)) == 32 * kWave - kWave && ((volatile uint32_t *)fail)[ji]) {
+        failed = true;
+        break;
+      }
     }
-    if ((it & (32 * kWave - 1)) == 32 * kWave - kWave && ((volatile uint32_t *)fail)[ji]) break;
   }
   if (lane == 0) atomicAdd(&evaluated[(blockIdx.x * 4 + (threadIdx.x >> 6)) & (kEvalShards - 1)],
                            (unsigned long long)done_states);
@@ -1330,11 +1387,11 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     std::vector<uint32_t> tab(kNumInsts);
     for (int i = 0; i < kNumInsts; i++) tab[i] = go_inst(i);
     HIPCHK(h, hipMemcpyAsync(w1, tab.data(), kNumInsts * 4, hipMemcpyHostToDevice, h->stream));
-    hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(kNumInsts, 4)), dim3(256), 0, h->stream, w1, 1u,
+    hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(kNumInsts, 4 * kFpPerWave)), dim3(256), 0, h->stream, w1, 1u,
                        (uint64_t)kNumInsts, cfg->bytes_model, cfp, cbytes, ccyc, cprune, badop);
     if (ml == 2) {
       // pairs: words at cwords[2*slot] for slot >= 2863 are contiguous pairs
-      hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(nc - kNumInsts, 4)), dim3(256), 0, h->stream,
+      hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(nc - kNumInsts, 4 * kFpPerWave)), dim3(256), 0, h->stream,
                          cwords + 2 * kNumInsts, 2u, nc - kNumInsts, cfg->bytes_model,
                          cfp + (uint64_t)kNumInsts * 32, cbytes + kNumInsts, ccyc + kNumInsts,
                          cprune + kNumInsts, badop);
@@ -1403,7 +1460,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     } else {
       HIPCHK(h, hipMemcpyAsync(tw, d_tsrc + s0 * 2, nt * 8, hipMemcpyDeviceToDevice, h->stream));
     }
-    hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(nt, 4)), dim3(256), 0, h->stream, tw, (uint32_t)tl,
+    hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(nt, 4 * kFpPerWave)), dim3(256), 0, h->stream, tw, (uint32_t)tl,
                        nt, cfg->bytes_model, tfp, tbytes, tcyc, tprune, badop);
     hipLaunchKernelGGL(k_gather_v0, dim3(blocks_for(nt, 256)), dim3(256), 0, h->stream, tfp, nt, tv0);
     HIPCHK(h, hipGetLastError());
