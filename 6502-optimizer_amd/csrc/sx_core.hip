@@ -618,18 +618,31 @@ __global__ __launch_bounds__(256) void k_exh_plan(Job *jobs, uint32_t nj, int mo
 // jobs are numbered consecutively (offs = exclusive scan of the per-job chunk
 // counts); a launch covers chunks [chunk0, chunk0 + nchunks).  A mismatch sets
 // the job's fail flag, which the job's other waves poll to stop early.
+// Chunk -> job map: thread j writes j into the map entries of its chunks.
+__global__ void k_chunk_map(const uint64_t *offs, const uint32_t *chunks, uint32_t nj, uint32_t *map) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nj) return;
+  const uint64_t o = offs[j];
+  for (uint32_t k = 0; k < chunks[j]; k++) map[o + k] = j;
+}
+
 __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, uint32_t nj,
-                                                 const uint64_t *__restrict__ offs, uint64_t chunk0,
+                                                 const uint64_t *__restrict__ offs,
+                                                 const uint32_t *__restrict__ chunk_job, uint64_t chunk0,
                                                  uint64_t nchunks, uint32_t *fail,
                                                  unsigned long long *evaluated) {
   const uint32_t lane = lane_id();
   const uint64_t rel = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (rel >= nchunks) return;  // whole wave
   const uint64_t c = chunk0 + rel;
-  // job = last index with offs[job] <= c, by a 64-ary search: each step the
-  // 64 lanes test 64 evenly spaced offsets and the range shrinks 64-fold
-  // (4 dependent loads for 5M jobs instead of 23)
+  // job = last index with offs[job] <= c: one load from the chunk map when
+  // there is one, else a 64-ary search (each step the 64 lanes test 64 evenly
+  // spaced offsets and the range shrinks 64-fold)
   uint32_t lo = 0, hi = nj;
+  if (chunk_job) {
+    lo = uni(chunk_job[c]);
+    hi = lo + 1;
+  }
   while (hi - lo > 1) {
     const uint32_t step = (hi - lo + kWave - 1) / kWave;
     const uint32_t p = lo + lane * step;
@@ -850,6 +863,13 @@ __global__ __launch_bounds__(256) void k_bulk_emit(const uint32_t *best, uint32_
   rules[o] = r;
 }
 
+// Length-3 ranges: a rule's target_index (base + position in the segment)
+// becomes base + the target's offset in the raw triple index range.
+__global__ void k_fix_raw_index(sx_rule *rules, uint32_t n, const uint32_t *raw_off, uint64_t base) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) rules[i].target_index = base + raw_off[rules[i].target_index - base];
+}
+
 // Ordered compaction of the first-hit rule slots (flag[k] = 1 where slot k
 // holds a rule; offs = exclusive scan of the flags).
 __global__ __launch_bounds__(256) void k_compact_rules(const sx_rule *slots, const uint32_t *flag,
@@ -963,7 +983,11 @@ struct sx_handle {
   uint64_t k_qc_launches = 0, k_exh_launches = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  std::vector<sx_rule> rules;
+  std::vector<sx_rule> rules;  // all-hits mode: sorted on the host
+  // first-hit mode: the rules stay on the device, in order, until fetched
+  sx_rule *d_rules = nullptr;
+  uint64_t d_rules_cap = 0, d_rules_n = 0;
+  bool rules_on_device = false;
   // QuickCheck launch configurations chosen by the autotuner, per (target_len, max_cand_len)
   struct QcChoice {
     int tl, ml, big;  // big: segments of >= 65,536 targets
@@ -984,7 +1008,7 @@ enum BufId {
   B_RANK, B_SURV, B_MID, B_COUNTERS, B_CUR, B_REJ, B_DONE, B_JOBS, B_CHUNKS, B_OFFS, B_SUMS,
   B_FAIL, B_RULES, B_SCRATCH, B_GEN_COUNTS, B_GEN_OFFS, B_BADOP, B_P_TW, B_P_TL, B_P_CW, B_P_CL,
   B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR, B_HEADS, B_RULEFLAG, B_RULEOFFS, B_RULES2,
-  B_RULETOT, B_TRAW
+  B_RULETOT, B_TRAW, B_CHUNKMAP
 };
 
 int set_err(sx_handle *h, int code, const char *fmt, ...) {
@@ -1212,12 +1236,19 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, ui
   if (rc) return rc;
   *chunks_out = tot;
   trace(h, "  exhaustive: %u jobs, %llu chunks", nj, (unsigned long long)tot);
+  // a direct chunk -> job map while it stays small (huge sweeps use the search)
+  uint32_t *chunk_job = nullptr;
+  if (tot <= (1ull << 26)) {
+    ENSURE(B_CHUNKMAP, uint32_t, tot + 1, chunk_job);
+    hipLaunchKernelGGL(k_chunk_map, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, offs, chunks, nj, chunk_job);
+    HIPCHK(h, hipGetLastError());
+  }
   const uint64_t slice = 1ull << 20;  // chunks per launch (2^34 states)
   for (uint64_t c0 = 0; c0 < tot; c0 += slice) {
     uint64_t n = std::min<uint64_t>(slice, tot - c0);
     HIPCHK(h, hipEventRecord(h->kev[0], h->stream));
-    hipLaunchKernelGGL(k_exh_run, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, h->stream, jobs, nj, offs, c0, n,
-                       fail, evald);
+    hipLaunchKernelGGL(k_exh_run, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, h->stream, jobs, nj, offs,
+                       chunk_job, c0, n, fail, evald);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipEventRecord(h->kev[1], h->stream));
     rc = sync_wd(h, "exhaustive sweep");
@@ -1279,6 +1310,7 @@ extern "C" int sx_close(sx_handle *h) {
   for (auto &b : h->bufs)
     if (b.p) (void)hipFree(b.p);
   if (h->d_l2) (void)hipFree(h->d_l2);
+  if (h->d_rules) (void)hipFree(h->d_rules);
   for (hipEvent_t e : h->kev)
     if (e) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1333,6 +1365,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   sx_stats st;
   memset(&st, 0, sizeof(st));
   h->rules.clear();
+  h->d_rules_n = 0;
+  h->rules_on_device = !cfg->hits;
 
   // ---- targets
   const uint32_t *d_tsrc = nullptr;
@@ -1651,17 +1685,34 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     HIPCHK(h, hipEventRecord(e4, h->stream));
     SYNC_WD("device sync");
     if (nr > rule_cap) return set_err(h, SX_ENOMEM, "rule buffer overflow (%u rules)", nr);
-    size_t old = h->rules.size();
-    h->rules.resize(old + nr);
-    if (nr) HIPCHK(h, hipMemcpy(h->rules.data() + old, rules_out, nr * sizeof(sx_rule), hipMemcpyDeviceToHost));
     if (implicit3 && nr) {
       // rules name their target by its raw Go-order triple index
-      std::vector<uint32_t> off(nt);
-      HIPCHK(h, hipMemcpy(off.data(), traw, nt * 4, hipMemcpyDeviceToHost));
-      for (size_t i = old; i < old + nr; i++) {
-        sx_rule &r = h->rules[i];
-        r.target_index = tbase + s0 + off[r.target_index - (tbase + s0)];
+      hipLaunchKernelGGL(k_fix_raw_index, dim3(blocks_for(nr, 256)), dim3(256), 0, h->stream,
+                         const_cast<sx_rule *>(rules_out), nr, traw, tbase + s0);
+      HIPCHK(h, hipGetLastError());
+    }
+    if (h->rules_on_device) {
+      // append to the device-resident list, doubling its capacity as needed
+      if (h->d_rules_n + nr > h->d_rules_cap) {
+        uint64_t cap2 = std::max<uint64_t>(std::max<uint64_t>(h->d_rules_cap * 2, h->d_rules_n + nr), 1u << 20);
+        sx_rule *grown = nullptr;
+        if (hipMalloc(&grown, cap2 * sizeof(sx_rule)) != hipSuccess) return set_err(h, SX_ENOMEM, "rule list");
+        if (h->d_rules_n)
+          HIPCHK(h, hipMemcpyAsync(grown, h->d_rules, h->d_rules_n * sizeof(sx_rule), hipMemcpyDeviceToDevice,
+                                   h->stream));
+        SYNC_WD("device sync");
+        if (h->d_rules) HIPCHK(h, hipFree(h->d_rules));
+        h->d_rules = grown;
+        h->d_rules_cap = cap2;
       }
+      if (nr)
+        HIPCHK(h, hipMemcpyAsync(h->d_rules + h->d_rules_n, rules_out, (size_t)nr * sizeof(sx_rule),
+                                 hipMemcpyDeviceToDevice, h->stream));
+      h->d_rules_n += nr;
+    } else {
+      size_t old = h->rules.size();
+      h->rules.resize(old + nr);
+      if (nr) HIPCHK(h, hipMemcpy(h->rules.data() + old, rules_out, nr * sizeof(sx_rule), hipMemcpyDeviceToHost));
     }
     float a = 0, b = 0, c = 0;
     HIPCHK(h, hipEventElapsedTime(&a, e1, e2));
@@ -1690,7 +1741,9 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       if (x.rep[q] != y.rep[q]) return x.rep[q] < y.rep[q];
     return false;
   });
-  st.rules = h->rules.size();
+  SYNC_WD("device sync");
+  const uint64_t n_rules = h->rules_on_device ? h->d_rules_n : h->rules.size();
+  st.rules = n_rules;
   st.ms_setup = ms_setup;
   st.qc_tile = qcc.tile;
   st.qc_blocks_per_cu = qcc.blocks_per_cu;
@@ -1703,19 +1756,25 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   st.ms_exh = ms_exh;
   st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
   if (stats) *stats = st;
-  if (n_out) *n_out = h->rules.size();
+  if (n_out) *n_out = n_rules;
   if (out) {
-    if (cap < h->rules.size()) return SX_ECAPACITY;
-    memcpy(out, h->rules.data(), h->rules.size() * sizeof(sx_rule));
+    if (cap < n_rules) return SX_ECAPACITY;
+    uint64_t got = 0;
+    return sx_fetch_rules(h, out, 0, n_rules, &got);
   }
   return SX_OK;
 }
 
 extern "C" int sx_fetch_rules(sx_handle *h, sx_rule *out, uint64_t offset, uint64_t cap, uint64_t *n_copied) {
   if (!h || (!out && cap)) return SX_EINVAL;
-  uint64_t n = h->rules.size();
+  const uint64_t n = h->rules_on_device ? h->d_rules_n : h->rules.size();
   uint64_t m = offset >= n ? 0 : std::min<uint64_t>(cap, n - offset);
-  if (m) memcpy(out, h->rules.data() + offset, m * sizeof(sx_rule));
+  if (m) {
+    if (h->rules_on_device)
+      HIPCHK(h, hipMemcpy(out, h->d_rules + offset, m * sizeof(sx_rule), hipMemcpyDeviceToHost));
+    else
+      memcpy(out, h->rules.data() + offset, m * sizeof(sx_rule));
+  }
   if (n_copied) *n_copied = m;
   return SX_OK;
 }

@@ -95,6 +95,22 @@ def c2_cuda_section(dev):
                           "hardware": "RTX 4060 Ti (README.md:127-136)"}}
 
 
+def target_section(dev):
+    """The `target` command on the device: a 4-instruction sequence with no
+    shorter equivalent (A, X, Y and carry all set), so every candidate of
+    length 1..3 (23,475,549,279) is enumerated and checked."""
+    from sx6502 import isa
+    t = isa.parse_assembly("LDA #$01 : LDX #$02 : LDY #$03 : SEC")
+    dev.target_search(t, max_cand_len=2)  # warm-up
+    t0 = time.perf_counter()
+    res, scanned = dev.target_search(t, max_cand_len=3)
+    dt = time.perf_counter() - t0
+    print(f"bench: target search {dt:.3f} s, {scanned} candidates", file=sys.stderr, flush=True)
+    return {"workload": "target LDA #$01 : LDX #$02 : LDY #$03 : SEC, candidates of length 1..3",
+            "wall_s": dt, "candidates": scanned, "candidates_per_s": scanned / dt,
+            "found": [r is not None for r in res]}
+
+
 def c3_section(dev):
     """Length-3 batched configuration: 65,536 sampled length-3 targets (the
     committed fixture) x all 8,199,632 length-1 and length-2 candidates,
@@ -283,6 +299,7 @@ def main():
         out["c2_cuda"] = c2_cuda_section(dev)
     if not args.no_c3:
         out["c3"] = c3_section(dev)
+        out["target"] = target_section(dev)
     if not args.no_stoke:
         out["stoke"] = stoke_section(dev, args.stoke_steps, args.train_steps, args.harvest_targets)
     if not args.no_cpu:
