@@ -64,6 +64,10 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) {
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (kWave - 1); }
 
+// Lanes whose predicate holds, as the compare's lane mask (no round trip
+// through a VGPR).
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
 }
@@ -332,16 +336,21 @@ __device__ __forceinline__ void qc_item(const uint2 *__restrict__ tv0, const uin
   }
   const uint32_t mlo = 0x00FFFFFFu | (pm_masked << 24);
   const uint32_t flo = 0x00FFFFFFu | (kPMask << 24);
+  const uint64_t vmask = ballot(valid);
   for (uint32_t t = t0; t < t1; t += 8) {
     uint32_t nb = min(8u, t1 - t);
-    uint2 tv[8];
-#pragma unroll
-    for (int u = 0; u < 8; u++) tv[u] = tv0[min(t + u, nt - 1)];
+    // one 64-byte scalar load of 8 consecutive targets (the plane has 8
+    // entries of slack past nt, so no clamping is needed)
+    struct alignas(64) Tv8 {
+      uint2 v[8];
+    };
+    const Tv8 blk = *reinterpret_cast<const Tv8 *>(tv0 + t);  // t is a multiple of 8
+    const uint2 *tv = blk.v;
 #pragma unroll
     for (uint32_t u = 0; u < 8; u++) {
       if (u >= nb) break;
       uint32_t d = ((clo[0] ^ tv[u].x) & mlo) | (chi[0] ^ tv[u].y);
-      uint64_t m = __ballot(valid && d == 0);
+      uint64_t m = ballot(d == 0) & vmask;
       if (m == 0) continue;
       // slow path: all 8 vectors under both masks
       const uint2 *tf = tfp + (uint64_t)(t + u) * 32;
@@ -412,20 +421,47 @@ __global__ __launch_bounds__(256) void k_qc_persistent(const uint2 *__restrict__
   if (Q.qn) Q.flush(lane_id());
 }
 
+// Compaction kernels run 1,024-thread blocks and reserve their output with
+// one atomic per block: atomics on one address serialise (about 9e7 per
+// second here), so one per wave would cost more than the work.
+constexpr int kBlk = 1024, kBlkWaves = kBlk / kWave;
+
+// Per-wave counts -> this wave's exclusive prefix within the block and the
+// block total.  Every thread of the block calls it; it synchronises twice.
+__device__ __forceinline__ uint32_t block_prefix(uint32_t cnt, uint32_t *s, uint32_t *total) {
+  const uint32_t w = threadIdx.x >> 6;
+  __syncthreads();  // s is free
+  if (lane_id() == 0) s[w] = cnt;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kBlkWaves; k++) {
+    const uint32_t c = s[k];
+    pre += (uint32_t)k < w ? c : 0u;
+    tot += c;
+  }
+  *total = tot;
+  return pre;
+}
+
 // MidCheck and stage bookkeeping.  One lane per QuickCheck survivor.
 // hits_all = 0 (Go): pass 1 if the exact compare held, pass 2 if only the
 // masked one did (and dead != 0).  hits_all = 1 (CUDA tool): one masked pass.
-// Output entry: (target, slot | pass << 30).
-__global__ __launch_bounds__(256) void k_midcheck(const uint2 *__restrict__ surv, uint32_t ns,
-                                                  const uint2 *__restrict__ tfp,
-                                                  const uint8_t *__restrict__ tbytes,
-                                                  const uint2 *__restrict__ cfp,
-                                                  const uint8_t *__restrict__ cbytes,
-                                                  const uint8_t *__restrict__ cprune,
-                                                  uint32_t pm_masked, int hits_all, int use_mid,
-                                                  uint2 *__restrict__ out, uint32_t out_cap,
-                                                  uint32_t *__restrict__ counters) {
-  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+// Output entry: (target, slot | pass << 30).  counters[0] += QuickCheck hits
+// after the byte/prune filters and counters[1] += passes, both with one
+// 64-bit atomic per block (counters[1] is also the output cursor).
+__global__ __launch_bounds__(kBlk) void k_midcheck(const uint2 *__restrict__ surv, uint32_t ns,
+                                                   const uint2 *__restrict__ tfp,
+                                                   const uint8_t *__restrict__ tbytes,
+                                                   const uint2 *__restrict__ cfp,
+                                                   const uint8_t *__restrict__ cbytes,
+                                                   const uint8_t *__restrict__ cprune,
+                                                   uint32_t pm_masked, int hits_all, int use_mid,
+                                                   uint2 *__restrict__ out, uint32_t out_cap,
+                                                   uint32_t *__restrict__ counters) {
+  __shared__ uint32_t s_hits[kBlkWaves], s_pass[kBlkWaves];
+  __shared__ unsigned long long s_base;
+  const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
   bool pass_ok = false;
   uint32_t t = 0, slot = 0, pass = 0;
   bool qc_hit = false;
@@ -480,16 +516,15 @@ __global__ __launch_bounds__(256) void k_midcheck(const uint2 *__restrict__ surv
       pass_ok = ok;
     }
   }
-  uint64_t hq = __ballot(qc_hit);
-  uint64_t m = __ballot(pass_ok);
-  uint32_t lane = lane_id();
-  if (lane == 0 && hq) atomicAdd(&counters[0], (uint32_t)__popcll(hq));
-  if (m == 0) return;
-  uint32_t base = 0;
-  if (lane == 0) base = atomicAdd(&counters[1], (uint32_t)__popcll(m));
-  base = uni(base);
+  const uint64_t hq = __ballot(qc_hit), m = __ballot(pass_ok);
+  uint32_t th = 0, tp = 0;
+  block_prefix((uint32_t)__popcll(hq), s_hits, &th);
+  const uint32_t pre = block_prefix((uint32_t)__popcll(m), s_pass, &tp);
+  if (threadIdx.x == 0 && (th | tp))
+    s_base = atomicAdd(reinterpret_cast<unsigned long long *>(counters), ((unsigned long long)tp << 32) | th);
+  __syncthreads();
   if (pass_ok) {
-    uint32_t o = base + mbcnt(m);
+    const uint32_t o = (uint32_t)(s_base >> 32) + pre + mbcnt(m);
     if (o < out_cap) out[o] = make_uint2(t, slot | (pass << 30));
   }
 }
@@ -552,12 +587,14 @@ __device__ __forceinline__ void fill_job(Job &j, const uint32_t *targets, uint32
 // Jobs of one round: the current candidate of every open (target, pass).
 // In first-hit mode it also counts the survivors not yet tested (*nrest),
 // which decides when the tail runs as one batch.
-__global__ __launch_bounds__(256) void k_round_jobs(const uint2 *mid, uint32_t nm, const uint32_t *rank,
-                                                    const uint32_t *cur, const uint32_t *rej, const uint8_t *done,
-                                                    const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
-                                                    int hits_all, Job *jobs, uint32_t job_cap, uint32_t *njobs,
-                                                    uint32_t *nrest) {
-  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(kBlk) void k_round_jobs(const uint2 *mid, uint32_t nm, const uint32_t *rank,
+                                                     const uint32_t *cur, const uint32_t *rej, const uint8_t *done,
+                                                     const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
+                                                     int hits_all, Job *jobs, uint32_t job_cap, uint32_t *njobs,
+                                                     uint32_t *nrest) {
+  __shared__ uint32_t s_rest[kBlkWaves], s_take[kBlkWaves];
+  __shared__ uint32_t s_base;
+  const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
   bool take = false, rest = false;
   uint2 e = make_uint2(0, 0);
   if (i < nm) {
@@ -571,21 +608,21 @@ __global__ __launch_bounds__(256) void k_round_jobs(const uint2 *mid, uint32_t n
       rest = !done[k] && rank[slot] >= rej[k];
     }
   }
-  if (!hits_all) {
-    uint64_t r = __ballot(rest);
-    if (r && lane_id() == 0) atomicAdd(nrest, (uint32_t)__popcll(r));
+  const uint64_t r = __ballot(rest), m = __ballot(take);
+  uint32_t tr = 0, tt = 0;
+  block_prefix((uint32_t)__popcll(r), s_rest, &tr);
+  const uint32_t pre = block_prefix((uint32_t)__popcll(m), s_take, &tt);
+  if (threadIdx.x == 0) {
+    if (tr) atomicAdd(nrest, tr);
+    if (tt) s_base = atomicAdd(njobs, tt);
   }
-  uint64_t m = __ballot(take);
-  if (m == 0) return;
-  uint32_t base = 0;
-  if (lane_id() == 0) base = atomicAdd(njobs, (uint32_t)__popcll(m));
-  base = uni(base);
+  __syncthreads();
   if (take) {
     uint32_t slot = e.y & 0x3FFFFFFFu, pass = e.y >> 30;
     uint32_t pm = (hits_all || pass == 2) ? pm_masked : kPMask;
     Job j;
     fill_job(j, targets, tl, nullptr, 0, e.x, slot, pass, pm);
-    uint32_t o = base + mbcnt(m);
+    uint32_t o = s_base + pre + mbcnt(m);
     if (o < job_cap) jobs[o] = j;
   }
 }
@@ -618,14 +655,102 @@ __global__ __launch_bounds__(256) void k_exh_plan(Job *jobs, uint32_t nj, int mo
 // jobs are numbered consecutively (offs = exclusive scan of the per-job chunk
 // counts); a launch covers chunks [chunk0, chunk0 + nchunks).  A mismatch sets
 // the job's fail flag, which the job's other waves poll to stop early.
-// Chunk -> job map: thread j writes j into the map entries of its chunks.
+// Chunk -> job map: thread j writes j into the map entries of its chunks,
+// with bit 31 set when the job is a single chunk (its wave then needs neither
+// the job's chunk offset nor its fail flag).
+constexpr uint32_t kMapSingle = 0x80000000u;
 __global__ void k_chunk_map(const uint64_t *offs, const uint32_t *chunks, uint32_t nj, uint32_t *map) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nj) return;
   const uint64_t o = offs[j];
-  for (uint32_t k = 0; k < chunks[j]; k++) map[o + k] = j;
+  const uint32_t n = chunks[j];
+  for (uint32_t k = 0; k < n; k++) map[o + k] = j | (n == 1 ? kMapSingle : 0u);
 }
 
+// A job as 28 dwords, one per lane of a wave (lane l holds dword l): words
+// by readlane instead of dependent scalar loads.
+constexpr int kJobDwords = (int)(sizeof(Job) / 4);
+static_assert(sizeof(Job) == 112 && kJobDwords <= kWave, "job layout");
+static_assert(offsetof(Job, tw) == 0 && offsetof(Job, cw) == 40 && offsetof(Job, tl) == 80 &&
+                  offsetof(Job, nsets) == 92 && offsetof(Job, sets) == 96 && offsetof(Job, logsz) == 104,
+              "job layout");
+__device__ __forceinline__ uint32_t job_byte(uint32_t jv, uint32_t dword0, uint32_t i) {
+  return (__builtin_amdgcn_readlane(jv, dword0 + (i >> 2)) >> (8 * (i & 3u))) & 0xFFu;
+}
+
+// Sub-sweep fields: varying register r occupies wd[r] bits (8, 1 for the
+// carry, 0 when fixed) at bit sh[r] of the state index.
+struct ExhFields {
+  uint32_t sh[8], wd[8];
+};
+
+// One pass of a sub-sweep: lane L evaluates states it + j * 64 + L (j < K)
+// below count, each opcode of both sequences dispatched once for the K
+// states.  jv is the job spread over the lanes.  Returns true (and raises
+// the job's fail flag) on a mismatch.  Wide: the index has more than 32 bits.
+template <int K, bool Wide>
+__device__ __forceinline__ bool exh_pass(uint32_t jv, uint32_t tl, uint32_t cl, uint32_t pm, const ExhFields &f,
+                                         uint64_t start, uint32_t it, uint32_t count, uint32_t lane, uint32_t ji,
+                                         uint32_t *fail) {
+  State ts[K], cs[K];
+  bool act[K];
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    const uint32_t k = it + j * kWave + lane;
+    act[j] = k < count;
+    const uint32_t kk = act[j] ? k : 0u;
+    State s;
+    if (Wide) {
+      const uint64_t idx = start + kk;
+      s.a = f.wd[RA] ? (uint32_t)(idx >> f.sh[RA]) & 0xFFu : 0u;
+      s.x = f.wd[RX] ? (uint32_t)(idx >> f.sh[RX]) & 0xFFu : 0u;
+      s.y = f.wd[RY] ? (uint32_t)(idx >> f.sh[RY]) & 0xFFu : 0u;
+      s.p = f.wd[RP] ? (uint32_t)(idx >> f.sh[RP]) & 0x1u : 0u;
+      s.s = f.wd[RS] ? (uint32_t)(idx >> f.sh[RS]) & 0xFFu : 0xFDu;
+      s.m = f.wd[RM] ? (uint32_t)(idx >> f.sh[RM]) & 0xFFu : 0u;
+      s.s0 = f.wd[RS0] ? (uint32_t)(idx >> f.sh[RS0]) & 0xFFu : 0u;
+      s.s1 = f.wd[RS1] ? (uint32_t)(idx >> f.sh[RS1]) & 0xFFu : 0u;
+    } else {
+      // start < 2^32 is a multiple of the chunk size and kk < the chunk
+      // size, so the index is start | kk; a zero width extracts 0
+      const uint32_t idx = (uint32_t)start | kk;
+      s.a = __builtin_amdgcn_ubfe(idx, f.sh[RA], f.wd[RA]);
+      s.x = __builtin_amdgcn_ubfe(idx, f.sh[RX], f.wd[RX]);
+      s.y = __builtin_amdgcn_ubfe(idx, f.sh[RY], f.wd[RY]);
+      s.p = __builtin_amdgcn_ubfe(idx, f.sh[RP], f.wd[RP]);
+      s.s = f.wd[RS] ? __builtin_amdgcn_ubfe(idx, f.sh[RS], 8u) : 0xFDu;
+      s.m = __builtin_amdgcn_ubfe(idx, f.sh[RM], f.wd[RM]);
+      s.s0 = __builtin_amdgcn_ubfe(idx, f.sh[RS0], f.wd[RS0]);
+      s.s1 = __builtin_amdgcn_ubfe(idx, f.sh[RS1], f.wd[RS1]);
+    }
+    ts[j] = s;
+    cs[j] = s;
+  }
+  for (uint32_t q = 0; q < tl; q++) {
+    const uint32_t w = __builtin_amdgcn_readlane(jv, q);
+    step_many<K>(ts, w & 0xFFFFu, w >> 16);
+  }
+  for (uint32_t q = 0; q < cl; q++) {
+    const uint32_t w = __builtin_amdgcn_readlane(jv, kJobWords + q);
+    step_many<K>(cs, w & 0xFFFFu, w >> 16);
+  }
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    const uint32_t diff = (ts[j].a ^ cs[j].a) | (ts[j].x ^ cs[j].x) | (ts[j].y ^ cs[j].y) |
+                          ((ts[j].p ^ cs[j].p) & pm) | (ts[j].s ^ cs[j].s) | (ts[j].m ^ cs[j].m) |
+                          (ts[j].s0 ^ cs[j].s0) | (ts[j].s1 ^ cs[j].s1);
+    bad |= act[j] && diff != 0;
+  }
+  if (__ballot(bad)) {
+    if (lane == 0) ((volatile uint32_t *)fail)[ji] = 1u;
+    return true;
+  }
+  return false;
+}
+
+// Exhaustive sweep kernel; K = states per lane in the passes after the first.
+template <int K>
 __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, uint32_t nj,
                                                  const uint64_t *__restrict__ offs,
                                                  const uint32_t *__restrict__ chunk_job, uint64_t chunk0,
@@ -639,8 +764,11 @@ __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, u
   // there is one, else a 64-ary search (each step the 64 lanes test 64 evenly
   // spaced offsets and the range shrinks 64-fold)
   uint32_t lo = 0, hi = nj;
+  bool single = false;
   if (chunk_job) {
-    lo = uni(chunk_job[c]);
+    const uint32_t e = uni(chunk_job[c]);
+    lo = e & ~kMapSingle;
+    single = (e & kMapSingle) != 0;
     hi = lo + 1;
   }
   while (hi - lo > 1) {
@@ -652,16 +780,22 @@ __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, u
     hi = min(hi, lo + step);
   }
   const uint32_t ji = uni(lo);
-  if (((volatile uint32_t *)fail)[ji]) return;
-  const Job *jp = &jobs[ji];
-  uint64_t local = c - offs[ji];
-  const uint32_t nsets = uni(jp->nsets);
-  const bool packed = uni(jp->pad[0]) != 0;  // one wave sweeps every sub-sweep of the job
+  const uint32_t jv = lane < (uint32_t)kJobDwords ? reinterpret_cast<const uint32_t *>(jobs + ji)[lane] : 0u;
+  uint64_t local = 0;
+  if (!single) {
+    if (((volatile uint32_t *)fail)[ji]) return;
+    local = c - offs[ji];
+  }
+  const uint32_t hdr = __builtin_amdgcn_readlane(jv, 20);  // tl, cl, pass, pmask
+  const uint32_t tl = hdr & 0xFFu, cl = (hdr >> 8) & 0xFFu, pm = hdr >> 24;
+  const uint32_t h23 = __builtin_amdgcn_readlane(jv, 23);  // nsets, pad[0] = packed
+  const uint32_t nsets = h23 & 0xFFu;
+  const bool packed = ((h23 >> 8) & 0xFFu) != 0;  // one wave sweeps every sub-sweep of the job
   uint32_t s_first = 0;
   if (!packed) {
     for (uint32_t s = 0; s < nsets; s++) {
-      uint32_t l = uni(jp->logsz[s]);
-      uint64_t nch = l > kChunkLog ? (1ull << (l - kChunkLog)) : 1ull;
+      const uint32_t l = job_byte(jv, 26, s);
+      const uint64_t nch = l > kChunkLog ? (1ull << (l - kChunkLog)) : 1ull;
       if (local < nch) {
         s_first = s;
         break;
@@ -670,61 +804,43 @@ __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, u
     }
   }
   const uint32_t s_last = packed ? nsets : s_first + 1;
-  const uint32_t tl = uni(jp->tl), cl = uni(jp->cl);
-  uint32_t tw[kJobWords], cw[kJobWords];
-#pragma unroll
-  for (int k = 0; k < kJobWords; k++) {
-    tw[k] = uni(jp->tw[k]);
-    cw[k] = uni(jp->cw[k]);
-  }
-  const uint32_t pm = uni(jp->pmask);
   // states this wave evaluates, added to a sharded counter when it leaves
   uint32_t done_states = 0;
   bool failed = false;
   for (uint32_t si = s_first; si < s_last && !failed; si++) {
-    const uint32_t set = uni(jp->sets[si]), lg = uni(jp->logsz[si]);
-    // bit positions of the varying registers in the 64-bit state index
-    uint32_t sh[8], wd[8], pos = 0;
+    const uint32_t set = job_byte(jv, 24, si), lg = job_byte(jv, 26, si);
+    // bit positions of the varying registers in the state index
+    ExhFields f;
+    uint32_t pos = 0;
 #pragma unroll
     for (int r = 0; r < 8; r++) {
-      uint32_t w = (set >> r) & 1u ? (r == RP ? 1u : 8u) : 0u;
-      sh[r] = pos;
-      wd[r] = w;
+      const uint32_t w = (set >> r) & 1u ? (r == RP ? 1u : 8u) : 0u;
+      f.sh[r] = pos;
+      f.wd[r] = w;
       pos += w;
     }
     const uint64_t start = packed ? 0 : local << kChunkLog;
     const uint32_t count = lg < kChunkLog ? (1u << lg) : (1u << kChunkLog);
     const uint32_t done_before = done_states;
-    for (uint32_t it = 0; it < count; it += kWave) {
-      done_states = done_before + (it + kWave < count ? it + kWave : count);
-      const uint32_t k = it + lane;
-      bool bad = false;
-      if (k < count) {
-        const uint64_t idx = start + k;
-        State s;
-        s.a = wd[RA] ? (uint32_t)(idx >> sh[RA]) & 0xFFu : 0u;
-        s.x = wd[RX] ? (uint32_t)(idx >> sh[RX]) & 0xFFu : 0u;
-        s.y = wd[RY] ? (uint32_t)(idx >> sh[RY]) & 0xFFu : 0u;
-        s.p = wd[RP] ? (uint32_t)(idx >> sh[RP]) & 0x1u : 0u;
-        s.s = wd[RS] ? (uint32_t)(idx >> sh[RS]) & 0xFFu : 0xFDu;
-        s.m = wd[RM] ? (uint32_t)(idx >> sh[RM]) & 0xFFu : 0u;
-        s.s0 = wd[RS0] ? (uint32_t)(idx >> sh[RS0]) & 0xFFu : 0u;
-        s.s1 = wd[RS1] ? (uint32_t)(idx >> sh[RS1]) & 0xFFu : 0u;
-        State ts = s, cs = s;
-        for (uint32_t q = 0; q < tl; q++) step(ts, tw[q] & 0xFFFFu, tw[q] >> 16);
-        for (uint32_t q = 0; q < cl; q++) step(cs, cw[q] & 0xFFFFu, cw[q] >> 16);
-        uint32_t diff = (ts.a ^ cs.a) | (ts.x ^ cs.x) | (ts.y ^ cs.y) | ((ts.p ^ cs.p) & pm) |
-                        (ts.s ^ cs.s) | (ts.m ^ cs.m) | (ts.s0 ^ cs.s0) | (ts.s1 ^ cs.s1);
-        bad = diff != 0;
+    if (lg > 32) {
+      // index wider than 32 bits: one state per lane and pass
+      for (uint32_t it = 0; it < count && !failed; it += kWave) {
+        done_states = done_before + min(it + kWave, count);
+        failed = exh_pass<1, true>(jv, tl, cl, pm, f, start, it, count, lane, ji, fail);
+        if (!failed && (it & (32 * kWave - 1)) == 31 * kWave && ((volatile uint32_t *)fail)[ji]) failed = true;
       }
-      if (__ballot(bad)) {
-        if (lane == 0) ((volatile uint32_t *)fail)[ji] = 1u;
-        failed = true;
-        break;
-      }
-      if ((it & (32 * kWave - 1)) == 32 * kWave - kWave && ((volatile uint32_t *)fail)[ji]) {
-        failed = true;
-        break;
+    } else {
+      // the first pass covers 64 states (most failing pairs stop there), the
+      // later ones K x 64 with one opcode dispatch per K states
+      done_states = done_before + min((uint32_t)kWave, count);
+      failed = exh_pass<1, false>(jv, tl, cl, pm, f, start, 0, count, lane, ji, fail);
+      for (uint32_t it = kWave; it < count && !failed; it += K * kWave) {
+        done_states = done_before + min(it + K * kWave, count);
+        failed = exh_pass<K, false>(jv, tl, cl, pm, f, start, it, count, lane, ji, fail);
+        // other chunks of this job may have failed: look every 2,048 states
+        if (!failed && !single && ((it - kWave) & (32 * kWave - 1)) == 32 * kWave - K * kWave &&
+            ((volatile uint32_t *)fail)[ji])
+          failed = true;
       }
     }
   }
@@ -789,11 +905,13 @@ __global__ __launch_bounds__(256) void k_round_resolve(const Job *jobs, uint32_t
 // (rank >= rej) becomes a job at once, and the smallest passing Go rank is
 // the first hit (the rounds would reach the same candidate one rank at a
 // time).
-__global__ __launch_bounds__(256) void k_bulk_jobs(const uint2 *mid, uint32_t nm, const uint32_t *rank,
-                                                   const uint32_t *rej, const uint8_t *done,
-                                                   const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
-                                                   Job *jobs, uint32_t job_cap, uint32_t *njobs) {
-  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(kBlk) void k_bulk_jobs(const uint2 *mid, uint32_t nm, const uint32_t *rank,
+                                                    const uint32_t *rej, const uint8_t *done,
+                                                    const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
+                                                    Job *jobs, uint32_t job_cap, uint32_t *njobs) {
+  __shared__ uint32_t s_take[kBlkWaves];
+  __shared__ uint32_t s_base;
+  const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
   bool take = false;
   uint2 e = make_uint2(0, 0);
   if (i < nm) {
@@ -802,16 +920,16 @@ __global__ __launch_bounds__(256) void k_bulk_jobs(const uint2 *mid, uint32_t nm
     uint64_t k = (uint64_t)e.x * 2 + p;
     take = !done[k] && rank[slot] >= rej[k];
   }
-  uint64_t m = __ballot(take);
-  if (m == 0) return;
-  uint32_t base = 0;
-  if (lane_id() == 0) base = atomicAdd(njobs, (uint32_t)__popcll(m));
-  base = uni(base);
+  const uint64_t m = __ballot(take);
+  uint32_t tt = 0;
+  const uint32_t pre = block_prefix((uint32_t)__popcll(m), s_take, &tt);
+  if (threadIdx.x == 0 && tt) s_base = atomicAdd(njobs, tt);
+  __syncthreads();
   if (take) {
     uint32_t slot = e.y & 0x3FFFFFFFu, pass = e.y >> 30;
     Job j;
     fill_job(j, targets, tl, nullptr, 0, e.x, slot, pass, pass == 2 ? pm_masked : kPMask);
-    uint32_t o = base + mbcnt(m);
+    uint32_t o = s_base + pre + mbcnt(m);
     if (o < job_cap) jobs[o] = j;
   }
 }
@@ -976,6 +1094,7 @@ struct sx_handle {
   int num_cu = 256;
   bool poisoned = false;  // a device stage timed out: the stream is unusable
   int bulk_after = 1;     // first-hit rounds before the one-batch tail (SX_BULK_AFTER); -1 = by survivor count
+  int exh_k = 2;          // states per lane and opcode dispatch in the exhaustive sweep (SX_EXH_K: 1, 2 or 4)
   uint32_t surv_cap_hint = 0;  // QuickCheck survivor buffer size that sufficed before
   // kernel-only timing of the QuickCheck and exhaustive-sweep launches
   hipEvent_t kev[2] = {nullptr, nullptr};
@@ -1247,8 +1366,13 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, ui
   for (uint64_t c0 = 0; c0 < tot; c0 += slice) {
     uint64_t n = std::min<uint64_t>(slice, tot - c0);
     HIPCHK(h, hipEventRecord(h->kev[0], h->stream));
-    hipLaunchKernelGGL(k_exh_run, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, h->stream, jobs, nj, offs,
-                       chunk_job, c0, n, fail, evald);
+    const dim3 grid((unsigned)((n + 3) / 4));
+    if (h->exh_k == 1)
+      hipLaunchKernelGGL(k_exh_run<1>, grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n, fail, evald);
+    else if (h->exh_k == 4)
+      hipLaunchKernelGGL(k_exh_run<4>, grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n, fail, evald);
+    else
+      hipLaunchKernelGGL(k_exh_run<2>, grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n, fail, evald);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipEventRecord(h->kev[1], h->stream));
     rc = sync_wd(h, "exhaustive sweep");
@@ -1294,6 +1418,10 @@ extern "C" int sx_open(int device, sx_handle **out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->num_cu = prop.multiProcessorCount;
   if (const char *e = getenv("SX_BULK_AFTER")) h->bulk_after = std::max(-1, atoi(e));
+  if (const char *e = getenv("SX_EXH_K")) {
+    const int k = atoi(e);
+    h->exh_k = (k == 1 || k == 4) ? k : 2;
+  }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->kev[0]) != hipSuccess || hipEventCreate(&h->kev[1]) != hipSuccess) {
     delete h;
@@ -1482,7 +1610,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     uint8_t *tbytes, *tcyc, *tprune;
     ENSURE(B_TARGETS, uint32_t, nt * tl, tw);
     ENSURE(B_TFP, uint2, nt * 32, tfp);
-    ENSURE(B_TV0, uint2, nt, tv0);
+    ENSURE(B_TV0, uint2, nt + 8, tv0);  // slack for the 8-target loads of QuickCheck
     ENSURE(B_TBYTES, uint8_t, nt, tbytes);
     ENSURE(B_TCYC, uint8_t, nt, tcyc);
     ENSURE(B_TPRUNE, uint8_t, nt, tprune);
@@ -1545,7 +1673,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       ENSURE(B_MID, uint2, std::max<uint32_t>(nsurv, 1), mid);
       HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));
       if (nsurv)
-        hipLaunchKernelGGL(k_midcheck, dim3(blocks_for(nsurv, 256)), dim3(256), 0, h->stream, surv, nsurv,
+        hipLaunchKernelGGL(k_midcheck, dim3(blocks_for(nsurv, kBlk)), dim3(kBlk), 0, h->stream, surv, nsurv,
                            tfp, tbytes, cfp, cbytes, cprune, pm_masked, cfg->hits, cfg->use_mid, mid,
                            std::max<uint32_t>(nsurv, 1), counters);
       HIPCHK(h, hipGetLastError());
@@ -1598,7 +1726,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       if (!cfg->hits && h->bulk_after >= 0 && round >= h->bulk_after) go_bulk = true;
       if (!cfg->hits && go_bulk) {
         HIPCHK(h, hipMemsetAsync(njobs_d, 0, 4, h->stream));
-        hipLaunchKernelGGL(k_bulk_jobs, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
+        hipLaunchKernelGGL(k_bulk_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, nmid, rank,
                            rej, done, tw, (uint32_t)tl, pm_masked, jobs, job_cap, njobs_d);
         HIPCHK(h, hipGetLastError());
         uint32_t nj = 0;
@@ -1627,7 +1755,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       if (!cfg->hits)
         hipLaunchKernelGGL(k_round_min, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
                            cur, rej, done);
-      hipLaunchKernelGGL(k_round_jobs, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
+      hipLaunchKernelGGL(k_round_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, nmid, rank,
                          cur, rej, done, tw, (uint32_t)tl, pm_masked, cfg->hits, jobs, job_cap, njobs_d, nrest_d);
       HIPCHK(h, hipGetLastError());
       uint32_t cnt2[2] = {0, 0};

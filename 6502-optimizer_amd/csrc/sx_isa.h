@@ -283,74 +283,88 @@ SX_HD uint32_t ror(State &st, uint32_t v) {
   return r;
 }
 
-// One instruction step, pkg/cpu/exec.go:8-189.  Returns false on an unknown
-// opcode (where Go panics); the state is then left unchanged.
-SX_HD bool step(State &st, uint32_t op, uint32_t imm) {
+// One instruction step over K states, pkg/cpu/exec.go:8-189: the opcode is
+// dispatched once and its body applied to every state.  Returns false on an
+// unknown opcode (where Go panics); the states are then left unchanged.
+#define SX_EACH(...)                       \
+  {                                        \
+    for (int j_ = 0; j_ < K; j_++) {       \
+      State &s = st[j_];                   \
+      __VA_ARGS__;                         \
+    }                                      \
+  }                                        \
+  break;
+template <int K>
+SX_HD bool step_many(State (&st)[K], uint32_t op, uint32_t imm) {
   imm &= 0xFFu;
   switch (op) {
-  case TAX: st.x = st.a; set_nz(st, st.x); break;
-  case TAY: st.y = st.a; set_nz(st, st.y); break;
-  case TXA: st.a = st.x; set_nz(st, st.a); break;
-  case TYA: st.a = st.y; set_nz(st, st.a); break;
-  case TSX: st.x = st.s; set_nz(st, st.x); break;
-  case TXS: st.s = st.x; break;
-  case INX: st.x = (st.x + 1) & 0xFFu; set_nz(st, st.x); break;
-  case INY: st.y = (st.y + 1) & 0xFFu; set_nz(st, st.y); break;
-  case DEX: st.x = (st.x - 1) & 0xFFu; set_nz(st, st.x); break;
-  case DEY: st.y = (st.y - 1) & 0xFFu; set_nz(st, st.y); break;
-  case ASL_A: st.a = asl(st, st.a); break;
-  case LSR_A: st.a = lsr(st, st.a); break;
-  case ROL_A: st.a = rol(st, st.a); break;
-  case ROR_A: st.a = ror(st, st.a); break;
-  case CLC: st.p &= ~0x01u; break;
-  case SEC: st.p |= 0x01u; break;
-  case CLD: st.p &= ~0x08u; break;
-  case SED: st.p |= 0x08u; break;
-  case CLI: st.p &= ~0x04u; break;
-  case SEI: st.p |= 0x04u; break;
-  case CLV: st.p &= ~0x40u; break;
-  case PHA: st.s1 = st.s0; st.s0 = st.a; st.s = (st.s - 1) & 0xFFu; break;
-  case PLA: st.a = st.s0; st.s0 = st.s1; st.s1 = 0; st.s = (st.s + 1) & 0xFFu; set_nz(st, st.a); break;
-  case PHP: st.s1 = st.s0; st.s0 = st.p | 0x30u; st.s = (st.s - 1) & 0xFFu; break;
-  case PLP: st.p = st.s0; st.s0 = st.s1; st.s1 = 0; st.s = (st.s + 1) & 0xFFu; break;
-  case NOP: break;
-  case LDA_IMM: st.a = imm; set_nz(st, st.a); break;
-  case LDX_IMM: st.x = imm; set_nz(st, st.x); break;
-  case LDY_IMM: st.y = imm; set_nz(st, st.y); break;
-  case ADC_IMM: adc(st, imm); break;
-  case SBC_IMM: sbc(st, imm); break;
-  case AND_IMM: st.a &= imm; set_nz(st, st.a); break;
-  case ORA_IMM: st.a |= imm; set_nz(st, st.a); break;
-  case EOR_IMM: st.a ^= imm; set_nz(st, st.a); break;
-  case CMP_IMM: cmp(st, st.a, imm); break;
-  case CPX_IMM: cmp(st, st.x, imm); break;
-  case CPY_IMM: cmp(st, st.y, imm); break;
-  case LDA_M: st.a = st.m; set_nz(st, st.a); break;
-  case LDX_M: st.x = st.m; set_nz(st, st.x); break;
-  case LDY_M: st.y = st.m; set_nz(st, st.y); break;
-  case STA_M: st.m = st.a; break;
-  case STX_M: st.m = st.x; break;
-  case STY_M: st.m = st.y; break;
-  case ADC_M: adc(st, st.m); break;
-  case SBC_M: sbc(st, st.m); break;
-  case AND_M: st.a &= st.m; set_nz(st, st.a); break;
-  case ORA_M: st.a |= st.m; set_nz(st, st.a); break;
-  case EOR_M: st.a ^= st.m; set_nz(st, st.a); break;
-  case CMP_M: cmp(st, st.a, st.m); break;
-  case CPX_M: cmp(st, st.x, st.m); break;
-  case CPY_M: cmp(st, st.y, st.m); break;
-  case INC_M: st.m = (st.m + 1) & 0xFFu; set_nz(st, st.m); break;
-  case DEC_M: st.m = (st.m - 1) & 0xFFu; set_nz(st, st.m); break;
-  case ASL_M: st.m = asl(st, st.m); break;
-  case LSR_M: st.m = lsr(st, st.m); break;
-  case ROL_M: st.m = rol(st, st.m); break;
-  case ROR_M: st.m = ror(st, st.m); break;
-  case BIT_M:
-    st.p = (st.p & ~0xC2u) | (st.m & 0xC0u) | ((st.a & st.m) == 0 ? 0x02u : 0u);
-    break;
+  case TAX: SX_EACH(s.x = s.a; set_nz(s, s.x);)
+  case TAY: SX_EACH(s.y = s.a; set_nz(s, s.y);)
+  case TXA: SX_EACH(s.a = s.x; set_nz(s, s.a);)
+  case TYA: SX_EACH(s.a = s.y; set_nz(s, s.a);)
+  case TSX: SX_EACH(s.x = s.s; set_nz(s, s.x);)
+  case TXS: SX_EACH(s.s = s.x;)
+  case INX: SX_EACH(s.x = (s.x + 1) & 0xFFu; set_nz(s, s.x);)
+  case INY: SX_EACH(s.y = (s.y + 1) & 0xFFu; set_nz(s, s.y);)
+  case DEX: SX_EACH(s.x = (s.x - 1) & 0xFFu; set_nz(s, s.x);)
+  case DEY: SX_EACH(s.y = (s.y - 1) & 0xFFu; set_nz(s, s.y);)
+  case ASL_A: SX_EACH(s.a = asl(s, s.a);)
+  case LSR_A: SX_EACH(s.a = lsr(s, s.a);)
+  case ROL_A: SX_EACH(s.a = rol(s, s.a);)
+  case ROR_A: SX_EACH(s.a = ror(s, s.a);)
+  case CLC: SX_EACH(s.p &= ~0x01u;)
+  case SEC: SX_EACH(s.p |= 0x01u;)
+  case CLD: SX_EACH(s.p &= ~0x08u;)
+  case SED: SX_EACH(s.p |= 0x08u;)
+  case CLI: SX_EACH(s.p &= ~0x04u;)
+  case SEI: SX_EACH(s.p |= 0x04u;)
+  case CLV: SX_EACH(s.p &= ~0x40u;)
+  case PHA: SX_EACH(s.s1 = s.s0; s.s0 = s.a; s.s = (s.s - 1) & 0xFFu;)
+  case PLA: SX_EACH(s.a = s.s0; s.s0 = s.s1; s.s1 = 0; s.s = (s.s + 1) & 0xFFu; set_nz(s, s.a);)
+  case PHP: SX_EACH(s.s1 = s.s0; s.s0 = s.p | 0x30u; s.s = (s.s - 1) & 0xFFu;)
+  case PLP: SX_EACH(s.p = s.s0; s.s0 = s.s1; s.s1 = 0; s.s = (s.s + 1) & 0xFFu;)
+  case NOP: SX_EACH()
+  case LDA_IMM: SX_EACH(s.a = imm; set_nz(s, s.a);)
+  case LDX_IMM: SX_EACH(s.x = imm; set_nz(s, s.x);)
+  case LDY_IMM: SX_EACH(s.y = imm; set_nz(s, s.y);)
+  case ADC_IMM: SX_EACH(adc(s, imm);)
+  case SBC_IMM: SX_EACH(sbc(s, imm);)
+  case AND_IMM: SX_EACH(s.a &= imm; set_nz(s, s.a);)
+  case ORA_IMM: SX_EACH(s.a |= imm; set_nz(s, s.a);)
+  case EOR_IMM: SX_EACH(s.a ^= imm; set_nz(s, s.a);)
+  case CMP_IMM: SX_EACH(cmp(s, s.a, imm);)
+  case CPX_IMM: SX_EACH(cmp(s, s.x, imm);)
+  case CPY_IMM: SX_EACH(cmp(s, s.y, imm);)
+  case LDA_M: SX_EACH(s.a = s.m; set_nz(s, s.a);)
+  case LDX_M: SX_EACH(s.x = s.m; set_nz(s, s.x);)
+  case LDY_M: SX_EACH(s.y = s.m; set_nz(s, s.y);)
+  case STA_M: SX_EACH(s.m = s.a;)
+  case STX_M: SX_EACH(s.m = s.x;)
+  case STY_M: SX_EACH(s.m = s.y;)
+  case ADC_M: SX_EACH(adc(s, s.m);)
+  case SBC_M: SX_EACH(sbc(s, s.m);)
+  case AND_M: SX_EACH(s.a &= s.m; set_nz(s, s.a);)
+  case ORA_M: SX_EACH(s.a |= s.m; set_nz(s, s.a);)
+  case EOR_M: SX_EACH(s.a ^= s.m; set_nz(s, s.a);)
+  case CMP_M: SX_EACH(cmp(s, s.a, s.m);)
+  case CPX_M: SX_EACH(cmp(s, s.x, s.m);)
+  case CPY_M: SX_EACH(cmp(s, s.y, s.m);)
+  case INC_M: SX_EACH(s.m = (s.m + 1) & 0xFFu; set_nz(s, s.m);)
+  case DEC_M: SX_EACH(s.m = (s.m - 1) & 0xFFu; set_nz(s, s.m);)
+  case ASL_M: SX_EACH(s.m = asl(s, s.m);)
+  case LSR_M: SX_EACH(s.m = lsr(s, s.m);)
+  case ROL_M: SX_EACH(s.m = rol(s, s.m);)
+  case ROR_M: SX_EACH(s.m = ror(s, s.m);)
+  case BIT_M: SX_EACH(s.p = (s.p & ~0xC2u) | (s.m & 0xC0u) | ((s.a & s.m) == 0 ? 0x02u : 0u);)
   default: return false;
   }
   return true;
+}
+#undef SX_EACH
+
+// One instruction step on one state.
+SX_HD bool step(State &st, uint32_t op, uint32_t imm) {
+  return step_many<1>(*reinterpret_cast<State(*)[1]>(&st), op, imm);
 }
 
 // Pack / unpack to two little-endian words: lo = A|X<<8|Y<<16|P<<24,
