@@ -352,16 +352,18 @@ __device__ __forceinline__ void qc_item(const uint2 *__restrict__ tv0, const uin
       uint32_t d = ((clo[0] ^ tv[u].x) & mlo) | (chi[0] ^ tv[u].y);
       uint64_t m = ballot(d == 0) & vmask;
       if (m == 0) continue;
-      // slow path: all 8 vectors under both masks
+      // slow path: all 8 vectors under both masks (the differences ORed
+      // together first: one 3-input VALU op per dword)
       const uint2 *tf = tfp + (uint64_t)(t + u) * 32;
-      bool okm = valid, okf = valid;
+      uint32_t acc_lo = 0, acc_hi = 0;
 #pragma unroll
       for (int v = 0; v < 8; v++) {
-        uint2 f = tf[v];
-        uint32_t dl = clo[v] ^ f.x, dh = chi[v] ^ f.y;
-        okm &= ((dl & mlo) | dh) == 0;
-        okf &= ((dl & flo) | dh) == 0;
+        const uint2 f = tf[v];
+        acc_lo |= clo[v] ^ f.x;
+        acc_hi |= chi[v] ^ f.y;
       }
+      const bool okm = valid && ((acc_lo & mlo) | acc_hi) == 0;
+      const bool okf = valid && ((acc_lo & flo) | acc_hi) == 0;
       uint64_t hm = __ballot(okm);
       if (hm == 0) continue;
       if (okm) Q.q[Q.qn + mbcnt(hm)] = make_uint2(t + u, slot | (okf ? 0x80000000u : 0u));
@@ -1100,6 +1102,14 @@ struct sx_handle {
   hipEvent_t kev[2] = {nullptr, nullptr};
   double k_qc_ms = 0, k_exh_ms = 0;
   uint64_t k_qc_launches = 0, k_exh_launches = 0;
+  // event pairs around QuickCheck and sweep launches, read at the next host
+  // synchronisation instead of synchronising per launch
+  struct Timed {
+    hipEvent_t a, b;
+    int kind;  // 0 QuickCheck, 1 exhaustive sweep
+  };
+  std::vector<Timed> timed;
+  size_t timed_used = 0;
   hipStream_t stream = nullptr;
   std::string err;
   std::vector<sx_rule> rules;  // all-hits mode: sorted on the host
@@ -1107,6 +1117,12 @@ struct sx_handle {
   sx_rule *d_rules = nullptr;
   uint64_t d_rules_cap = 0, d_rules_n = 0;
   bool rules_on_device = false;
+  // ... and a pinned host copy of them, filled segment by segment on
+  // copy_stream while the later segments compute
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t seg_ev = nullptr;
+  sx_rule *h_rules = nullptr;
+  uint64_t h_rules_cap = 0;
   // QuickCheck launch configurations chosen by the autotuner, per (target_len, max_cand_len)
   struct QcChoice {
     int tl, ml, big;  // big: segments of >= 65,536 targets
@@ -1153,6 +1169,46 @@ int set_err(sx_handle *h, int code, const char *fmt, ...) {
 // Stream synchronisation with a deadline: a stage that does not finish within
 // SX_WATCHDOG_S seconds (default 120) fails with SX_EHIP instead of blocking
 // the caller forever.  The handle must not be used after such an error.
+// Kernel timing: record the start of a timed launch (returns the pool slot)
+// and its end; the durations are added up by harvest_timed once the stream
+// has drained.
+int timed_begin(sx_handle *h, int kind, size_t *slot) {
+  if (h->timed_used == h->timed.size()) {
+    sx_handle::Timed t{nullptr, nullptr, 0};
+    HIPCHK(h, hipEventCreate(&t.a));
+    if (hipEventCreate(&t.b) != hipSuccess) {
+      (void)hipEventDestroy(t.a);
+      return set_err(h, SX_EHIP, "hipEventCreate failed");
+    }
+    h->timed.push_back(t);
+  }
+  *slot = h->timed_used++;
+  h->timed[*slot].kind = kind;
+  HIPCHK(h, hipEventRecord(h->timed[*slot].a, h->stream));
+  return SX_OK;
+}
+int timed_end(sx_handle *h, size_t slot) {
+  HIPCHK(h, hipEventRecord(h->timed[slot].b, h->stream));
+  return SX_OK;
+}
+void harvest_timed(sx_handle *h) {
+  for (size_t i = 0; i < h->timed_used; i++) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, h->timed[i].a, h->timed[i].b) != hipSuccess) {
+      (void)hipGetLastError();
+      continue;
+    }
+    if (h->timed[i].kind == 0) {
+      h->k_qc_ms += ms;
+      h->k_qc_launches++;
+    } else {
+      h->k_exh_ms += ms;
+      h->k_exh_launches++;
+    }
+  }
+  h->timed_used = 0;
+}
+
 int sync_wd(sx_handle *h, const char *stage) {
   static double limit = [] {
     const char *e = getenv("SX_WATCHDOG_S");
@@ -1161,7 +1217,10 @@ int sync_wd(sx_handle *h, const char *stage) {
   auto t0 = std::chrono::steady_clock::now();
   for (;;) {
     hipError_t e = hipStreamQuery(h->stream);
-    if (e == hipSuccess) return SX_OK;
+    if (e == hipSuccess) {
+      harvest_timed(h);
+      return SX_OK;
+    }
     if (e != hipErrorNotReady)
       return set_err(h, SX_EHIP, "%s: %s", stage, hipGetErrorString(e));
     double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1223,6 +1282,32 @@ int ensure(sx_handle *h, BufId id, size_t count, T **out) {
   } while (0)
 
 inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Host copy of a large block with up to 16 threads: the destination is
+// usually fresh memory, whose page faults then spread over the threads too.
+void parallel_memcpy(void *dst, const void *src, size_t bytes) {
+  const size_t kPiece = 4u << 20;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const unsigned nt = (unsigned)std::min<size_t>(std::min(16u, hw), bytes / kPiece);
+  if (nt < 2) {
+    memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t per = ((bytes + nt - 1) / nt + 4095) & ~(size_t)4095;
+  auto piece = [=](unsigned i) {
+    const size_t o = i * per;
+    if (o < bytes) memcpy(static_cast<char *>(dst) + o, static_cast<const char *>(src) + o, std::min(per, bytes - o));
+  };
+  std::vector<std::thread> pool;
+  unsigned i = 1;
+  try {
+    for (; i < nt; i++) pool.emplace_back(piece, i);
+  } catch (...) {  // no thread to be had: the caller copies the rest
+    for (; i < nt; i++) piece(i);
+  }
+  piece(0);
+  for (auto &t : pool) t.join();
+}
 
 // Exclusive scan of n uint32 counts into uint64 offsets; *total on device.
 int scan_u32(sx_handle *h, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *d_total) {
@@ -1333,7 +1418,7 @@ int tune_qc(sx_handle *h, int tl, int ml, const uint2 *tv0, const uint2 *tfp, ui
 
 // Run the exhaustive stage on nj jobs already in B_JOBS; the verdicts are in
 // B_FAIL (0 = pass).  *chunks_out gets the number of chunks (statistics).
-int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, uint64_t *states_out) {
+int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out) {
   Job *jobs = (Job *)h->bufs[B_JOBS].p;
   uint32_t *chunks, *fail;
   uint64_t *offs, *total;
@@ -1341,9 +1426,8 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, ui
   ENSURE(B_OFFS, uint64_t, nj + 1, offs);
   ENSURE(B_FAIL, uint32_t, nj + 1, fail);
   ENSURE(B_TOTAL, uint64_t, 1, total);
-  unsigned long long *evald;
+  unsigned long long *evald;  // evaluated states, added up across calls
   ENSURE(B_CURSOR, unsigned long long, kEvalShards, evald);
-  HIPCHK(h, hipMemsetAsync(evald, 0, kEvalShards * sizeof(unsigned long long), h->stream));
   hipLaunchKernelGGL(k_exh_plan, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, jobs, nj, mode,
                      chunks, fail);
   HIPCHK(h, hipGetLastError());
@@ -1365,7 +1449,9 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, ui
   const uint64_t slice = 1ull << 20;  // chunks per launch (2^34 states)
   for (uint64_t c0 = 0; c0 < tot; c0 += slice) {
     uint64_t n = std::min<uint64_t>(slice, tot - c0);
-    HIPCHK(h, hipEventRecord(h->kev[0], h->stream));
+    size_t ts = 0;
+    rc = timed_begin(h, 1, &ts);
+    if (rc) return rc;
     const dim3 grid((unsigned)((n + 3) / 4));
     if (h->exh_k == 1)
       hipLaunchKernelGGL(k_exh_run<1>, grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n, fail, evald);
@@ -1374,24 +1460,29 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, ui
     else
       hipLaunchKernelGGL(k_exh_run<2>, grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n, fail, evald);
     HIPCHK(h, hipGetLastError());
-    HIPCHK(h, hipEventRecord(h->kev[1], h->stream));
-    rc = sync_wd(h, "exhaustive sweep");
+    rc = timed_end(h, ts);
     if (rc) return rc;
-    float ms = 0;
-    HIPCHK(h, hipEventElapsedTime(&ms, h->kev[0], h->kev[1]));
-    h->k_exh_ms += ms;
-    h->k_exh_launches++;
   }
-  if (states_out) {
-    std::vector<unsigned long long> shards(kEvalShards);
-    HIPCHK(h, hipMemcpyAsync(shards.data(), evald, kEvalShards * sizeof(unsigned long long),
-                             hipMemcpyDeviceToHost, h->stream));
-    rc = sync_wd(h, "exhaustive sweep");
-    if (rc) return rc;
-    uint64_t sum = 0;
-    for (auto v : shards) sum += v;
-    *states_out = sum;
-  }
+  return SX_OK;
+}
+
+// The evaluated-state counter (sharded): cleared, and summed on the host.
+int clear_evaluated(sx_handle *h) {
+  unsigned long long *evald;
+  ENSURE(B_CURSOR, unsigned long long, kEvalShards, evald);
+  HIPCHK(h, hipMemsetAsync(evald, 0, kEvalShards * sizeof(unsigned long long), h->stream));
+  return SX_OK;
+}
+int read_evaluated(sx_handle *h, uint64_t *sum) {
+  unsigned long long *evald;
+  ENSURE(B_CURSOR, unsigned long long, kEvalShards, evald);
+  std::vector<unsigned long long> shards(kEvalShards);
+  HIPCHK(h, hipMemcpyAsync(shards.data(), evald, kEvalShards * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           h->stream));
+  int rc = sync_wd(h, "exhaustive sweep");
+  if (rc) return rc;
+  *sum = 0;
+  for (auto v : shards) *sum += v;
   return SX_OK;
 }
 
@@ -1423,8 +1514,11 @@ extern "C" int sx_open(int device, sx_handle **out) {
     h->exh_k = (k == 1 || k == 4) ? k : 2;
   }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&h->kev[0]) != hipSuccess || hipEventCreate(&h->kev[1]) != hipSuccess) {
-    delete h;
+      hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&h->kev[0]) != hipSuccess || hipEventCreate(&h->kev[1]) != hipSuccess ||
+      hipEventCreateWithFlags(&h->seg_ev, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    sx_close(h);
     return SX_EHIP;
   }
   *out = h;
@@ -1435,12 +1529,20 @@ extern "C" int sx_close(sx_handle *h) {
   if (!h) return SX_EINVAL;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->copy_stream) (void)hipStreamSynchronize(h->copy_stream);
   for (auto &b : h->bufs)
     if (b.p) (void)hipFree(b.p);
   if (h->d_l2) (void)hipFree(h->d_l2);
   if (h->d_rules) (void)hipFree(h->d_rules);
+  if (h->h_rules) (void)hipHostFree(h->h_rules);
   for (hipEvent_t e : h->kev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &t : h->timed) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  if (h->seg_ev) (void)hipEventDestroy(h->seg_ev);
+  if (h->copy_stream) (void)hipStreamDestroy(h->copy_stream);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return SX_OK;
@@ -1493,8 +1595,13 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   sx_stats st;
   memset(&st, 0, sizeof(st));
   h->rules.clear();
+  HIPCHK(h, hipStreamSynchronize(h->copy_stream));  // the last call's rule copies read d_rules
   h->d_rules_n = 0;
   h->rules_on_device = !cfg->hits;
+  {
+    int rc = clear_evaluated(h);
+    if (rc) return rc;
+  }
 
   // ---- targets
   const uint32_t *d_tsrc = nullptr;
@@ -1640,21 +1747,18 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
         qc_tuned = true;
       }
       HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));
-      HIPCHK(h, hipEventRecord(h->kev[0], h->stream));
       {
-        int rc = launch_qc(h, qcc.tile, qcc.blocks_per_cu, tv0, tfp, nt, cfp, nc, pm_masked, surv, surv_cap,
-                           counters);
+        size_t ts = 0;
+        int rc = timed_begin(h, 0, &ts);
+        if (rc) return rc;
+        rc = launch_qc(h, qcc.tile, qcc.blocks_per_cu, tv0, tfp, nt, cfp, nc, pm_masked, surv, surv_cap,
+                       counters);
+        if (rc) return rc;
+        rc = timed_end(h, ts);
         if (rc) return rc;
       }
-      HIPCHK(h, hipEventRecord(h->kev[1], h->stream));
       HIPCHK(h, hipMemcpyAsync(&nsurv, counters, 4, hipMemcpyDeviceToHost, h->stream));
       SYNC_WD("device sync");
-      {
-        float ms = 0;
-        HIPCHK(h, hipEventElapsedTime(&ms, h->kev[0], h->kev[1]));
-        h->k_qc_ms += ms;
-        h->k_qc_launches++;
-      }
       trace(h, "segment %llu: %llu targets, QuickCheck survivors %u (cap %u)", (unsigned long long)s0,
             (unsigned long long)nt, nsurv, surv_cap);
       if (nsurv <= surv_cap) break;
@@ -1737,10 +1841,9 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
         trace(h, "  first-hit tail: %u jobs", nj);
         st.exh_rounds++;
         st.exh_pairs += nj;
-        uint64_t chunks = 0, evaluated = 0;
-        int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, &evaluated);
+        uint64_t chunks = 0;
+        int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks);
         if (rc) return rc;
-        st.exh_states += evaluated;
         // cur holds kRankNone for every entry after a resolve: reuse it for the minimum
         hipLaunchKernelGGL(k_bulk_min, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream,
                            (const Job *)h->bufs[B_JOBS].p, nj, (const uint32_t *)h->bufs[B_FAIL].p, rank, tfp,
@@ -1777,10 +1880,9 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       trace(h, "  round %d: %u jobs", round, nj);
       st.exh_rounds++;
       st.exh_pairs += nj;
-      uint64_t chunks = 0, evaluated = 0;
-      int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, &evaluated);
+      uint64_t chunks = 0;
+      int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks);
       if (rc) return rc;
-      st.exh_states += evaluated;
       hipLaunchKernelGGL(k_round_resolve, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream,
                          (const Job *)h->bufs[B_JOBS].p, nj, (const uint32_t *)h->bufs[B_FAIL].p, rank, cur, rej,
                          done, tfp, cfp, tbytes, tcyc, cbytes, ccyc, cfg->hits, cfg->dead_flags, tbase + s0,
@@ -1829,13 +1931,35 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
           HIPCHK(h, hipMemcpyAsync(grown, h->d_rules, h->d_rules_n * sizeof(sx_rule), hipMemcpyDeviceToDevice,
                                    h->stream));
         SYNC_WD("device sync");
+        HIPCHK(h, hipStreamSynchronize(h->copy_stream));  // copies in flight read the old list
         if (h->d_rules) HIPCHK(h, hipFree(h->d_rules));
         h->d_rules = grown;
         h->d_rules_cap = cap2;
       }
-      if (nr)
+      if (h->d_rules_n + nr > h->h_rules_cap) {
+        // grow the pinned host copy (kept across calls, so this happens once
+        // per size class)
+        HIPCHK(h, hipStreamSynchronize(h->copy_stream));
+        uint64_t cap2 = std::max<uint64_t>(std::max<uint64_t>(h->h_rules_cap * 2, h->d_rules_n + nr), 1u << 20);
+        sx_rule *grown = nullptr;
+        if (hipHostMalloc(&grown, cap2 * sizeof(sx_rule), hipHostMallocDefault) != hipSuccess) {
+          (void)hipGetLastError();
+          return set_err(h, SX_ENOMEM, "pinned rule list");
+        }
+        if (h->d_rules_n) memcpy(grown, h->h_rules, h->d_rules_n * sizeof(sx_rule));
+        if (h->h_rules) HIPCHK(h, hipHostFree(h->h_rules));
+        h->h_rules = grown;
+        h->h_rules_cap = cap2;
+      }
+      if (nr) {
         HIPCHK(h, hipMemcpyAsync(h->d_rules + h->d_rules_n, rules_out, (size_t)nr * sizeof(sx_rule),
                                  hipMemcpyDeviceToDevice, h->stream));
+        // copy the segment's rules to the host while the next segments run
+        HIPCHK(h, hipEventRecord(h->seg_ev, h->stream));
+        HIPCHK(h, hipStreamWaitEvent(h->copy_stream, h->seg_ev, 0));
+        HIPCHK(h, hipMemcpyAsync(h->h_rules + h->d_rules_n, h->d_rules + h->d_rules_n,
+                                 (size_t)nr * sizeof(sx_rule), hipMemcpyDeviceToHost, h->copy_stream));
+      }
       h->d_rules_n += nr;
     } else {
       size_t old = h->rules.size();
@@ -1853,6 +1977,10 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   }
   uint32_t bad = 0;
   HIPCHK(h, hipMemcpy(&bad, badop, 4, hipMemcpyDeviceToHost));
+  {
+    int rc = read_evaluated(h, &st.exh_states);
+    if (rc) return rc;
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipEventDestroy(e2);
@@ -1898,10 +2026,14 @@ extern "C" int sx_fetch_rules(sx_handle *h, sx_rule *out, uint64_t offset, uint6
   const uint64_t n = h->rules_on_device ? h->d_rules_n : h->rules.size();
   uint64_t m = offset >= n ? 0 : std::min<uint64_t>(cap, n - offset);
   if (m) {
-    if (h->rules_on_device)
-      HIPCHK(h, hipMemcpy(out, h->d_rules + offset, m * sizeof(sx_rule), hipMemcpyDeviceToHost));
-    else
+    if (h->rules_on_device) {
+      // the pinned copy is complete once the copy stream drains
+      HIPCHK(h, hipSetDevice(h->device));
+      HIPCHK(h, hipStreamSynchronize(h->copy_stream));
+      parallel_memcpy(out, h->h_rules + offset, m * sizeof(sx_rule));
+    } else {
       memcpy(out, h->rules.data() + offset, m * sizeof(sx_rule));
+    }
   }
   if (n_copied) *n_copied = m;
   return SX_OK;
@@ -1973,7 +2105,7 @@ extern "C" int sx_check_pairs(sx_handle *h, const uint32_t *t_words, const uint8
       ENSURE(B_JOBS, Job, nb, dj);
       HIPCHK(h, hipMemcpyAsync(dj, jobs.data(), nb * sizeof(Job), hipMemcpyHostToDevice, h->stream));
       uint64_t chunks = 0;
-      int rc = run_exhaustive(h, nb, exh_mode, &chunks, nullptr);
+      int rc = run_exhaustive(h, nb, exh_mode, &chunks);
       if (rc) return rc;
       std::vector<uint32_t> f(nb);
       HIPCHK(h, hipMemcpyAsync(f.data(), h->bufs[B_FAIL].p, nb * 4, hipMemcpyDeviceToHost, h->stream));
@@ -2104,7 +2236,7 @@ int sx_verify_seqs(sx_handle *h, const uint32_t *t_words, const uint8_t *t_len, 
     ENSURE(B_JOBS, Job, nb, dj);
     HIPCHK(h, hipMemcpyAsync(dj, jobs.data(), nb * sizeof(Job), hipMemcpyHostToDevice, h->stream));
     uint64_t chunks = 0;
-    int rc = run_exhaustive(h, nb, SX_EXH_PROJECTED, &chunks, nullptr);
+    int rc = run_exhaustive(h, nb, SX_EXH_PROJECTED, &chunks);
     if (rc) return rc;
     std::vector<uint32_t> f(nb);
     HIPCHK(h, hipMemcpyAsync(f.data(), h->bufs[B_FAIL].p, nb * 4, hipMemcpyDeviceToHost, h->stream));
