@@ -228,45 +228,49 @@ __global__ __launch_bounds__(1024) void k_scan_add(uint64_t *out, uint64_t n, co
   if (i < n) out[i] += sums[blockIdx.x];
 }
 
-// Outputs of n sequences on the 32 vectors: one wave per sequence, lane v
-// (< 32) runs vector v.  fp[(i * 32 + v)] = (lo, hi) with P masked by 0xCF.
-// Also writes per-sequence byte size, cycles and the pruner verdict.
 // Outputs of sequences on the 32 Mid vectors.  A wave takes kFpPerWave
 // sequences in turn (lane v < 32 runs vector v with the sequence's
-// wave-uniform opcodes), which spreads the per-wave setup over several
-// sequences; lane k < kFpPerWave computes bytes, cycles and the pruner flag of
-// the wave's k-th sequence.
+// wave-uniform opcodes); their words are fetched at once, one per lane, and
+// read back with readlane, so the sequences do not wait on dependent scalar
+// loads.  fp[i * 32 + v] = (lo, hi) with P masked by 0xCF; v0[i] (when given)
+// = the vector-0 output again, densely, with lo ANDed with v0_lo_mask (the
+// QuickCheck compare mask), so that eight consecutive targets are one 64-byte
+// scalar load in QuickCheck and need no masking there.  Lane k < kFpPerWave computes bytes,
+// cycles and the pruner flag of the wave's k-th sequence.
 constexpr uint32_t kFpPerWave = 8;
+static_assert(kFpPerWave * 4 <= kWave, "sequence words must fit one per lane");
 __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uint32_t len,
-                                                      uint64_t n, int bytes_model, uint2 *fp,
-                                                      uint8_t *bytes, uint8_t *cyc, uint8_t *prune,
+                                                      uint64_t n, int bytes_model, uint2 *fp, uint2 *v0,
+                                                      uint32_t v0_lo_mask, uint8_t *bytes, uint8_t *cyc, uint8_t *prune,
                                                       uint32_t *bad_op) {
   const uint64_t first = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kFpPerWave;
   if (first >= n) return;
   const uint32_t lane = lane_id();
-  const uint64_t last = first + kFpPerWave < n ? first + kFpPerWave : n;
+  const uint32_t cnt = (uint32_t)(first + kFpPerWave < n ? kFpPerWave : n - first);
+  const uint32_t wv = lane < cnt * len ? words[first * len + lane] : 0u;  // len <= 4
   bool bad_any = false;
-  for (uint64_t seq = first; seq < last; seq++) {
-    uint32_t w[4];
+  for (uint32_t q = 0; q < cnt; q++) {
     bool bad = false;
-    for (uint32_t k = 0; k < len; k++) {
-      w[k] = uni(words[seq * len + k]);
-      bad |= (w[k] & 0xFFFFu) >= OP_COUNT;
-    }
+    for (uint32_t k = 0; k < len; k++) bad |= (__builtin_amdgcn_readlane(wv, q * len + k) & 0xFFFFu) >= OP_COUNT;
     if (bad) {
       bad_any = true;
       continue;
     }
     if (lane < 32) {
       State st = load_vec(lane);
-      run_seq_uniform(st, w, (int)len);
+      for (uint32_t k = 0; k < len; k++) {
+        const uint32_t w = __builtin_amdgcn_readlane(wv, q * len + k);
+        step(st, w & 0xFFFFu, w >> 16);
+      }
       st.p &= kPMask;
-      fp[seq * 32 + lane] = make_uint2(pack_lo(st), pack_hi(st));
+      const uint2 o = make_uint2(pack_lo(st), pack_hi(st));
+      fp[(first + q) * 32 + lane] = o;
+      if (v0 && lane == 0) v0[first + q] = make_uint2(o.x & v0_lo_mask, o.y);
     }
   }
   if (bad_any && lane == 0) atomicOr(bad_op, 1u);
   const uint64_t seq = first + lane;
-  if (lane < kFpPerWave && seq < last) {
+  if (lane < cnt) {
     uint32_t w[4] = {0, 0, 0, 0};
     bool bad = false;
     for (uint32_t k = 0; k < len; k++) {
@@ -284,13 +288,6 @@ __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uin
       prune[seq] = should_prune(w, (int)len, c_reads, c_writes) ? 1 : 0;
     }
   }
-}
-
-// Gather the first-vector output of each target into a dense array so eight
-// consecutive targets are one 64-byte scalar load.
-__global__ void k_gather_v0(const uint2 *fp, uint64_t n, uint2 *v0) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) v0[i] = fp[i * 32];
 }
 
 // QuickCheck.  Grid: x = 256-candidate group, y = target tile.
@@ -336,6 +333,7 @@ __device__ __forceinline__ void qc_item(const uint2 *__restrict__ tv0, const uin
   }
   const uint32_t mlo = 0x00FFFFFFu | (pm_masked << 24);
   const uint32_t flo = 0x00FFFFFFu | (kPMask << 24);
+  const uint32_t clo0 = clo[0] & mlo;  // the vector-0 plane's lo words carry the same mask
   const uint64_t vmask = ballot(valid);
   for (uint32_t t = t0; t < t1; t += 8) {
     uint32_t nb = min(8u, t1 - t);
@@ -346,12 +344,10 @@ __device__ __forceinline__ void qc_item(const uint2 *__restrict__ tv0, const uin
     };
     const Tv8 blk = *reinterpret_cast<const Tv8 *>(tv0 + t);  // t is a multiple of 8
     const uint2 *tv = blk.v;
-#pragma unroll
-    for (uint32_t u = 0; u < 8; u++) {
-      if (u >= nb) break;
-      uint32_t d = ((clo[0] ^ tv[u].x) & mlo) | (chi[0] ^ tv[u].y);
-      uint64_t m = ballot(d == 0) & vmask;
-      if (m == 0) continue;
+    // target t + u against the wave's candidates, d = the vector-0 difference
+    auto check = [&](uint32_t u, uint32_t d) {
+      const uint64_t m = ballot(d == 0) & vmask;
+      if (m == 0) return;
       // slow path: all 8 vectors under both masks (the differences ORed
       // together first: one 3-input VALU op per dword)
       const uint2 *tf = tfp + (uint64_t)(t + u) * 32;
@@ -364,11 +360,31 @@ __device__ __forceinline__ void qc_item(const uint2 *__restrict__ tv0, const uin
       }
       const bool okm = valid && ((acc_lo & mlo) | acc_hi) == 0;
       const bool okf = valid && ((acc_lo & flo) | acc_hi) == 0;
-      uint64_t hm = __ballot(okm);
-      if (hm == 0) continue;
+      const uint64_t hm = __ballot(okm);
+      if (hm == 0) return;
       if (okm) Q.q[Q.qn + mbcnt(hm)] = make_uint2(t + u, slot | (okf ? 0x80000000u : 0u));
       Q.qn += (uint32_t)__popcll(hm);
       if (Q.qn > kQcQueue - kWave) Q.flush(lane);
+    };
+    if (nb == 8) {
+      // groups of 4 targets: one ballot and branch per group when no lane
+      // matches any of the 4 on vector 0 (the common case)
+#pragma unroll
+      for (uint32_t g = 0; g < 8; g += 4) {
+        uint32_t d[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) d[j] = (clo0 ^ tv[g + j].x) | (chi[0] ^ tv[g + j].y);
+        const uint32_t dmin = min(min(d[0], d[1]), min(d[2], d[3]));
+        if ((ballot(dmin == 0) & vmask) == 0) continue;
+#pragma unroll
+        for (int j = 0; j < 4; j++) check(g + j, d[j]);
+      }
+    } else {
+#pragma unroll
+      for (uint32_t u = 0; u < 8; u++) {
+        if (u >= nb) break;
+        check(u, (clo0 ^ tv[u].x) | (chi[0] ^ tv[u].y));
+      }
     }
   }
 }
@@ -586,6 +602,24 @@ __device__ __forceinline__ void fill_job(Job &j, const uint32_t *targets, uint32
   j.slot = slot;
 }
 
+// The sweep plan of a job: its sub-sweeps, and the number of 2^14-state
+// chunks returned.  pad[0] marks a job whose sub-sweeps together fit one
+// chunk: one wave sweeps them all.
+__device__ __forceinline__ uint32_t plan_job(Job &j, int mode) {
+  Plan pl = make_plan(j.tw, j.tl, j.cw, j.cl, j.pmask, mode, c_reads, c_dep);
+  uint32_t total = 0;
+  uint64_t states = 0;
+  for (int s = 0; s < pl.nsets; s++) {
+    j.sets[s] = pl.sets[s];
+    j.logsz[s] = pl.logsz[s];
+    total += pl.logsz[s] > kChunkLog ? (1u << (pl.logsz[s] - kChunkLog)) : 1u;
+    states += 1ull << pl.logsz[s];
+  }
+  j.nsets = pl.nsets;
+  j.pad[0] = (pl.nsets > 1 && states <= (1ull << kChunkLog)) ? 1 : 0;
+  return j.pad[0] ? 1u : total;
+}
+
 // Jobs of one round: the current candidate of every open (target, pass).
 // In first-hit mode it also counts the survivors not yet tested (*nrest),
 // which decides when the tail runs as one batch.
@@ -593,7 +627,8 @@ __global__ __launch_bounds__(kBlk) void k_round_jobs(const uint2 *mid, uint32_t 
                                                      const uint32_t *cur, const uint32_t *rej, const uint8_t *done,
                                                      const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
                                                      int hits_all, Job *jobs, uint32_t job_cap, uint32_t *njobs,
-                                                     uint32_t *nrest) {
+                                                     uint32_t *nrest, int plan_mode, uint32_t *chunks,
+                                                     uint32_t *fail) {
   __shared__ uint32_t s_rest[kBlkWaves], s_take[kBlkWaves];
   __shared__ uint32_t s_base;
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
@@ -625,7 +660,11 @@ __global__ __launch_bounds__(kBlk) void k_round_jobs(const uint2 *mid, uint32_t 
     Job j;
     fill_job(j, targets, tl, nullptr, 0, e.x, slot, pass, pm);
     uint32_t o = s_base + pre + mbcnt(m);
-    if (o < job_cap) jobs[o] = j;
+    if (o < job_cap) {
+      chunks[o] = plan_job(j, plan_mode);  // planned here: no separate plan pass
+      fail[o] = 0;
+      jobs[o] = j;
+    }
   }
 }
 
@@ -635,21 +674,8 @@ __global__ __launch_bounds__(256) void k_exh_plan(Job *jobs, uint32_t nj, int mo
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nj) return;
   Job j = jobs[i];
-  Plan pl = make_plan(j.tw, j.tl, j.cw, j.cl, j.pmask, mode, c_reads, c_dep);
-  uint32_t total = 0;
-  uint64_t states = 0;
-  for (int s = 0; s < pl.nsets; s++) {
-    j.sets[s] = pl.sets[s];
-    j.logsz[s] = pl.logsz[s];
-    total += pl.logsz[s] > kChunkLog ? (1u << (pl.logsz[s] - kChunkLog)) : 1u;
-    states += 1ull << pl.logsz[s];
-  }
-  j.nsets = pl.nsets;
-  // a job whose sub-sweeps together fit one chunk is swept by one wave
-  j.pad[0] = (pl.nsets > 1 && states <= (1ull << kChunkLog)) ? 1 : 0;
-  if (j.pad[0]) total = 1;
+  chunks[i] = plan_job(j, mode);
   jobs[i] = j;
-  chunks[i] = total;
   fail[i] = 0;
 }
 
@@ -751,41 +777,14 @@ __device__ __forceinline__ bool exh_pass(uint32_t jv, uint32_t tl, uint32_t cl, 
   return false;
 }
 
-// Exhaustive sweep kernel; K = states per lane in the passes after the first.
+// One chunk of job ji (jv = the job spread over the lanes; single = the job
+// is this one chunk, c its global chunk number).  Returns the states swept.
 template <int K>
-__global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, uint32_t nj,
-                                                 const uint64_t *__restrict__ offs,
-                                                 const uint32_t *__restrict__ chunk_job, uint64_t chunk0,
-                                                 uint64_t nchunks, uint32_t *fail,
-                                                 unsigned long long *evaluated) {
-  const uint32_t lane = lane_id();
-  const uint64_t rel = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (rel >= nchunks) return;  // whole wave
-  const uint64_t c = chunk0 + rel;
-  // job = last index with offs[job] <= c: one load from the chunk map when
-  // there is one, else a 64-ary search (each step the 64 lanes test 64 evenly
-  // spaced offsets and the range shrinks 64-fold)
-  uint32_t lo = 0, hi = nj;
-  bool single = false;
-  if (chunk_job) {
-    const uint32_t e = uni(chunk_job[c]);
-    lo = e & ~kMapSingle;
-    single = (e & kMapSingle) != 0;
-    hi = lo + 1;
-  }
-  while (hi - lo > 1) {
-    const uint32_t step = (hi - lo + kWave - 1) / kWave;
-    const uint32_t p = lo + lane * step;
-    const uint64_t below = __ballot(p < hi && offs[p < hi ? p : lo] <= c);  // lane 0 always holds
-    const uint32_t k = 63u - (uint32_t)__builtin_clzll(below);
-    lo = uni(lo + k * step);
-    hi = min(hi, lo + step);
-  }
-  const uint32_t ji = uni(lo);
-  const uint32_t jv = lane < (uint32_t)kJobDwords ? reinterpret_cast<const uint32_t *>(jobs + ji)[lane] : 0u;
+__device__ __forceinline__ uint32_t exh_chunk(uint32_t ji, uint32_t jv, bool single, uint64_t c,
+                                              const uint64_t *__restrict__ offs, uint32_t *fail, uint32_t lane) {
   uint64_t local = 0;
   if (!single) {
-    if (((volatile uint32_t *)fail)[ji]) return;
+    if (((volatile uint32_t *)fail)[ji]) return 0;
     local = c - offs[ji];
   }
   const uint32_t hdr = __builtin_amdgcn_readlane(jv, 20);  // tl, cl, pass, pmask
@@ -806,7 +805,6 @@ __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, u
     }
   }
   const uint32_t s_last = packed ? nsets : s_first + 1;
-  // states this wave evaluates, added to a sharded counter when it leaves
   uint32_t done_states = 0;
   bool failed = false;
   for (uint32_t si = s_first; si < s_last && !failed; si++) {
@@ -846,8 +844,51 @@ __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, u
       }
     }
   }
-  if (lane == 0) atomicAdd(&evaluated[(blockIdx.x * 4 + (threadIdx.x >> 6)) & (kEvalShards - 1)],
-                           (unsigned long long)done_states);
+  return done_states;
+}
+
+__device__ __forceinline__ uint32_t job_dword(const Job *jobs, uint32_t ji, uint32_t lane) {
+  return lane < (uint32_t)kJobDwords ? reinterpret_cast<const uint32_t *>(jobs + ji)[lane] : 0u;
+}
+
+// Exhaustive sweep: one wave per chunk of 2^14 states (launching a wave per
+// chunk lets the hardware balance the uneven chunks; persistent waves with
+// prefetch and two chunks per wave both measured slower).  The chunks of all
+// jobs are numbered consecutively (offs = exclusive scan of the per-job chunk
+// counts); a launch covers chunks [chunk0, chunk0 + nchunks).  The job is one
+// map load away, or found by a 64-ary search over the chunk offsets when
+// there is no map (each step the 64 lanes test 64 evenly spaced offsets and
+// the range shrinks 64-fold).  K = states per lane in the passes after the
+// first.
+template <int K>
+__global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, uint32_t nj,
+                                                 const uint64_t *__restrict__ offs,
+                                                 const uint32_t *__restrict__ chunk_job, uint64_t chunk0,
+                                                 uint64_t nchunks, uint32_t *fail,
+                                                 unsigned long long *evaluated) {
+  const uint32_t lane = lane_id();
+  const uint64_t w = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= nchunks) return;  // whole wave
+  const uint64_t c = chunk0 + w;
+  uint32_t lo = 0, hi = nj;
+  bool single = false;
+  if (chunk_job) {
+    const uint32_t e = uni(chunk_job[c]);
+    lo = e & ~kMapSingle;
+    single = (e & kMapSingle) != 0;
+    hi = lo + 1;
+  }
+  while (hi - lo > 1) {
+    const uint32_t step = (hi - lo + kWave - 1) / kWave;
+    const uint32_t p = lo + lane * step;
+    const uint64_t below = __ballot(p < hi && offs[p < hi ? p : lo] <= c);  // lane 0 always holds
+    const uint32_t k = 63u - (uint32_t)__builtin_clzll(below);
+    lo = uni(lo + k * step);
+    hi = min(hi, lo + step);
+  }
+  const uint32_t ji = uni(lo);
+  const uint32_t done = exh_chunk<K>(ji, job_dword(jobs, ji, lane), single, c, offs, fail, lane);
+  if (lane == 0 && done) atomicAdd(&evaluated[w & (kEvalShards - 1)], (unsigned long long)done);
 }
 
 // Resolve a round: a passing job emits its rule and closes (target, pass); a
@@ -910,7 +951,8 @@ __global__ __launch_bounds__(256) void k_round_resolve(const Job *jobs, uint32_t
 __global__ __launch_bounds__(kBlk) void k_bulk_jobs(const uint2 *mid, uint32_t nm, const uint32_t *rank,
                                                     const uint32_t *rej, const uint8_t *done,
                                                     const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
-                                                    Job *jobs, uint32_t job_cap, uint32_t *njobs) {
+                                                    Job *jobs, uint32_t job_cap, uint32_t *njobs, int plan_mode,
+                                                    uint32_t *chunks, uint32_t *fail) {
   __shared__ uint32_t s_take[kBlkWaves];
   __shared__ uint32_t s_base;
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
@@ -932,7 +974,11 @@ __global__ __launch_bounds__(kBlk) void k_bulk_jobs(const uint2 *mid, uint32_t n
     Job j;
     fill_job(j, targets, tl, nullptr, 0, e.x, slot, pass, pass == 2 ? pm_masked : kPMask);
     uint32_t o = s_base + pre + mbcnt(m);
-    if (o < job_cap) jobs[o] = j;
+    if (o < job_cap) {
+      chunks[o] = plan_job(j, plan_mode);
+      fail[o] = 0;
+      jobs[o] = j;
+    }
   }
 }
 
@@ -1418,7 +1464,9 @@ int tune_qc(sx_handle *h, int tl, int ml, const uint2 *tv0, const uint2 *tfp, ui
 
 // Run the exhaustive stage on nj jobs already in B_JOBS; the verdicts are in
 // B_FAIL (0 = pass).  *chunks_out gets the number of chunks (statistics).
-int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out) {
+// planned: the jobs' plans, chunk counts and fail flags are already written
+// (by the kernel that made the jobs).
+int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, bool planned = false) {
   Job *jobs = (Job *)h->bufs[B_JOBS].p;
   uint32_t *chunks, *fail;
   uint64_t *offs, *total;
@@ -1428,8 +1476,9 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out) {
   ENSURE(B_TOTAL, uint64_t, 1, total);
   unsigned long long *evald;  // evaluated states, added up across calls
   ENSURE(B_CURSOR, unsigned long long, kEvalShards, evald);
-  hipLaunchKernelGGL(k_exh_plan, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, jobs, nj, mode,
-                     chunks, fail);
+  if (!planned)
+    hipLaunchKernelGGL(k_exh_plan, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, jobs, nj, mode,
+                       chunks, fail);
   HIPCHK(h, hipGetLastError());
   int rc = scan_u32(h, chunks, nj, offs, total);
   if (rc) return rc;
@@ -1657,12 +1706,12 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     for (int i = 0; i < kNumInsts; i++) tab[i] = go_inst(i);
     HIPCHK(h, hipMemcpyAsync(w1, tab.data(), kNumInsts * 4, hipMemcpyHostToDevice, h->stream));
     hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(kNumInsts, 4 * kFpPerWave)), dim3(256), 0, h->stream, w1, 1u,
-                       (uint64_t)kNumInsts, cfg->bytes_model, cfp, cbytes, ccyc, cprune, badop);
+                       (uint64_t)kNumInsts, cfg->bytes_model, cfp, nullptr, ~0u, cbytes, ccyc, cprune, badop);
     if (ml == 2) {
       // pairs: words at cwords[2*slot] for slot >= 2863 are contiguous pairs
       hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(nc - kNumInsts, 4 * kFpPerWave)), dim3(256), 0, h->stream,
                          cwords + 2 * kNumInsts, 2u, nc - kNumInsts, cfg->bytes_model,
-                         cfp + (uint64_t)kNumInsts * 32, cbytes + kNumInsts, ccyc + kNumInsts,
+                         cfp + (uint64_t)kNumInsts * 32, nullptr, ~0u, cbytes + kNumInsts, ccyc + kNumInsts,
                          cprune + kNumInsts, badop);
     }
     HIPCHK(h, hipGetLastError());
@@ -1730,8 +1779,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       HIPCHK(h, hipMemcpyAsync(tw, d_tsrc + s0 * 2, nt * 8, hipMemcpyDeviceToDevice, h->stream));
     }
     hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(nt, 4 * kFpPerWave)), dim3(256), 0, h->stream, tw, (uint32_t)tl,
-                       nt, cfg->bytes_model, tfp, tbytes, tcyc, tprune, badop);
-    hipLaunchKernelGGL(k_gather_v0, dim3(blocks_for(nt, 256)), dim3(256), 0, h->stream, tfp, nt, tv0);
+                       nt, cfg->bytes_model, tfp, tv0, 0x00FFFFFFu | (pm_masked << 24), tbytes, tcyc, tprune, badop);
     HIPCHK(h, hipGetLastError());
 
     // ---- QuickCheck, re-run with a larger buffer on overflow (no hit is dropped)
@@ -1802,6 +1850,9 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     ENSURE(B_DONE, uint8_t, nt * 2, done);
     const uint32_t job_cap = std::max<uint32_t>(nmid, 1);  // the first-hit tail may queue every survivor
     ENSURE(B_JOBS, Job, job_cap, jobs);
+    uint32_t *jchunks, *jfail;  // written by the job kernels, read by the sweep
+    ENSURE(B_CHUNKS, uint32_t, job_cap + 1, jchunks);
+    ENSURE(B_FAIL, uint32_t, job_cap + 1, jfail);
     const uint32_t rule_cap = cfg->hits ? std::max<uint32_t>(nmid, 1) : (uint32_t)(2 * nt);
     ENSURE(B_RULES, sx_rule, rule_cap, rules);
     uint32_t *rule_flag;  // first hit: slot k = target * 2 + pass - 1 holds a rule
@@ -1831,7 +1882,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       if (!cfg->hits && go_bulk) {
         HIPCHK(h, hipMemsetAsync(njobs_d, 0, 4, h->stream));
         hipLaunchKernelGGL(k_bulk_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, nmid, rank,
-                           rej, done, tw, (uint32_t)tl, pm_masked, jobs, job_cap, njobs_d);
+                           rej, done, tw, (uint32_t)tl, pm_masked, jobs, job_cap, njobs_d, cfg->exh_mode, jchunks,
+                           jfail);
         HIPCHK(h, hipGetLastError());
         uint32_t nj = 0;
         HIPCHK(h, hipMemcpyAsync(&nj, njobs_d, 4, hipMemcpyDeviceToHost, h->stream));
@@ -1842,7 +1894,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
         st.exh_rounds++;
         st.exh_pairs += nj;
         uint64_t chunks = 0;
-        int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks);
+        int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, true);
         if (rc) return rc;
         // cur holds kRankNone for every entry after a resolve: reuse it for the minimum
         hipLaunchKernelGGL(k_bulk_min, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream,
@@ -1859,7 +1911,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
         hipLaunchKernelGGL(k_round_min, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
                            cur, rej, done);
       hipLaunchKernelGGL(k_round_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, nmid, rank,
-                         cur, rej, done, tw, (uint32_t)tl, pm_masked, cfg->hits, jobs, job_cap, njobs_d, nrest_d);
+                         cur, rej, done, tw, (uint32_t)tl, pm_masked, cfg->hits, jobs, job_cap, njobs_d, nrest_d,
+                         cfg->exh_mode, jchunks, jfail);
       HIPCHK(h, hipGetLastError());
       uint32_t cnt2[2] = {0, 0};
       HIPCHK(h, hipMemcpyAsync(cnt2, njobs_d, 8, hipMemcpyDeviceToHost, h->stream));
@@ -1881,7 +1934,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       st.exh_rounds++;
       st.exh_pairs += nj;
       uint64_t chunks = 0;
-      int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks);
+      int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, true);
       if (rc) return rc;
       hipLaunchKernelGGL(k_round_resolve, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream,
                          (const Job *)h->bufs[B_JOBS].p, nj, (const uint32_t *)h->bufs[B_FAIL].p, rank, cur, rej,
