@@ -34,6 +34,7 @@
 #include "../../include/sx6502.h"
 #include "sx_isa.h"
 #include "sx_plan.h"
+#include "sx_pk.h"
 
 using namespace sx;
 
@@ -777,9 +778,67 @@ __device__ __forceinline__ bool exh_pass(uint32_t jv, uint32_t tl, uint32_t cl, 
   return false;
 }
 
+// Packed pass: lane L evaluates states it + 128 j + L (low halves) and
+// it + 128 j + 64 + L (high halves), j < KP, below count, with the packed
+// interpreter (two states per register).  it is a multiple of 128, so bit 6
+// of a low state's index is 0 and the high state's index is that index | 64:
+// each of its fields is the low state's field plus hiadd[r] (the field
+// holding index bit 6 gets that bit).  start < 2^32 is a multiple of the
+// chunk size.
+template <int KP>
+__device__ __forceinline__ bool exh_pass_pk(uint32_t jv, uint32_t tl, uint32_t cl, uint32_t pm, const ExhFields &f,
+                                            const uint32_t (&hiadd)[8], uint32_t start, uint32_t it,
+                                            uint32_t count, uint32_t lane, uint32_t ji, uint32_t *fail) {
+  PState ts[KP], cs[KP];
+  uint32_t actm[KP];
+  auto pack = [&](uint32_t idx, int r) {
+    // v * 0x10001 + add as one 24-bit multiply-add (v < 256)
+    return __builtin_bit_cast(pk16, __umul24(__builtin_amdgcn_ubfe(idx, f.sh[r], f.wd[r]), 0x10001u) + hiadd[r]);
+  };
+#pragma unroll
+  for (int j = 0; j < KP; j++) {
+    const uint32_t k0 = it + j * 2 * kWave + lane;
+    const bool a0 = k0 < count, a1 = k0 + kWave < count;
+    actm[j] = (a0 ? 0x0000FFFFu : 0u) | (a1 ? 0xFFFF0000u : 0u);
+    const uint32_t idx = start | (a0 ? k0 : 0u);
+    PState s;
+    s.a = pack(idx, RA);
+    s.x = pack(idx, RX);
+    s.y = pack(idx, RY);
+    s.p = pack(idx, RP);
+    s.s = f.wd[RS] ? pack(idx, RS) : pk_splat(0xFD);
+    s.m = pack(idx, RM);
+    s.s0 = pack(idx, RS0);
+    s.s1 = pack(idx, RS1);
+    ts[j] = s;
+    cs[j] = s;
+  }
+  for (uint32_t q = 0; q < tl; q++) {
+    const uint32_t w = __builtin_amdgcn_readlane(jv, q);
+    step_pk<KP>(ts, w & 0xFFFFu, w >> 16);
+  }
+  for (uint32_t q = 0; q < cl; q++) {
+    const uint32_t w = __builtin_amdgcn_readlane(jv, kJobWords + q);
+    step_pk<KP>(cs, w & 0xFFFFu, w >> 16);
+  }
+  const pk16 pm2 = pk_splat(pm);
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < KP; j++) {
+    const pk16 d = (ts[j].a ^ cs[j].a) | (ts[j].x ^ cs[j].x) | (ts[j].y ^ cs[j].y) | ((ts[j].p ^ cs[j].p) & pm2) |
+                   (ts[j].s ^ cs[j].s) | (ts[j].m ^ cs[j].m) | (ts[j].s0 ^ cs[j].s0) | (ts[j].s1 ^ cs[j].s1);
+    bad |= (__builtin_bit_cast(uint32_t, d) & actm[j]) != 0;
+  }
+  if (__ballot(bad)) {
+    if (lane == 0) ((volatile uint32_t *)fail)[ji] = 1u;
+    return true;
+  }
+  return false;
+}
+
 // One chunk of job ji (jv = the job spread over the lanes; single = the job
 // is this one chunk, c its global chunk number).  Returns the states swept.
-template <int K>
+template <int K, bool Pk>
 __device__ __forceinline__ uint32_t exh_chunk(uint32_t ji, uint32_t jv, bool single, uint64_t c,
                                               const uint64_t *__restrict__ offs, uint32_t *fail, uint32_t lane) {
   uint64_t local = 0;
@@ -829,6 +888,22 @@ __device__ __forceinline__ uint32_t exh_chunk(uint32_t ji, uint32_t jv, bool sin
         failed = exh_pass<1, true>(jv, tl, cl, pm, f, start, it, count, lane, ji, fail);
         if (!failed && (it & (32 * kWave - 1)) == 31 * kWave && ((volatile uint32_t *)fail)[ji]) failed = true;
       }
+    } else if (Pk) {
+      // packed: the first pass covers 128 states, the later ones K x 128
+      uint32_t hiadd[8];
+#pragma unroll
+      for (int r = 0; r < 8; r++)
+        hiadd[r] = (f.wd[r] && f.sh[r] <= 6 && 6 < f.sh[r] + f.wd[r]) ? (1u << (6 - f.sh[r])) << 16 : 0u;
+      const uint32_t st32 = (uint32_t)start;
+      done_states = done_before + min(2u * kWave, count);
+      failed = exh_pass_pk<1>(jv, tl, cl, pm, f, hiadd, st32, 0, count, lane, ji, fail);
+      for (uint32_t it = 2 * kWave; it < count && !failed; it += K * 2 * kWave) {
+        done_states = done_before + min(it + K * 2 * kWave, count);
+        failed = exh_pass_pk<K>(jv, tl, cl, pm, f, hiadd, st32, it, count, lane, ji, fail);
+        if (!failed && !single && ((it - 2 * kWave) & (32 * kWave - 1)) == 32 * kWave - K * 2 * kWave &&
+            ((volatile uint32_t *)fail)[ji])
+          failed = true;
+      }
     } else {
       // the first pass covers 64 states (most failing pairs stop there), the
       // later ones K x 64 with one opcode dispatch per K states
@@ -860,7 +935,7 @@ __device__ __forceinline__ uint32_t job_dword(const Job *jobs, uint32_t ji, uint
 // there is no map (each step the 64 lanes test 64 evenly spaced offsets and
 // the range shrinks 64-fold).  K = states per lane in the passes after the
 // first.
-template <int K>
+template <int K, bool Pk>
 __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, uint32_t nj,
                                                  const uint64_t *__restrict__ offs,
                                                  const uint32_t *__restrict__ chunk_job, uint64_t chunk0,
@@ -887,7 +962,7 @@ __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, u
     hi = min(hi, lo + step);
   }
   const uint32_t ji = uni(lo);
-  const uint32_t done = exh_chunk<K>(ji, job_dword(jobs, ji, lane), single, c, offs, fail, lane);
+  const uint32_t done = exh_chunk<K, Pk>(ji, job_dword(jobs, ji, lane), single, c, offs, fail, lane);
   if (lane == 0 && done) atomicAdd(&evaluated[w & (kEvalShards - 1)], (unsigned long long)done);
 }
 
@@ -1142,7 +1217,8 @@ struct sx_handle {
   int num_cu = 256;
   bool poisoned = false;  // a device stage timed out: the stream is unusable
   int bulk_after = 1;     // first-hit rounds before the one-batch tail (SX_BULK_AFTER); -1 = by survivor count
-  int exh_k = 2;          // states per lane and opcode dispatch in the exhaustive sweep (SX_EXH_K: 1, 2 or 4)
+  int exh_k = 1;          // states (pairs when packed) per lane and opcode dispatch in the sweep (SX_EXH_K: 1 or 2)
+  int exh_pk = 1;         // packed two-states-per-register interpreter in the sweep (SX_EXH_PK=0: one state)
   uint32_t surv_cap_hint = 0;  // QuickCheck survivor buffer size that sufficed before
   // kernel-only timing of the QuickCheck and exhaustive-sweep launches
   hipEvent_t kev[2] = {nullptr, nullptr};
@@ -1502,12 +1578,21 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, bo
     rc = timed_begin(h, 1, &ts);
     if (rc) return rc;
     const dim3 grid((unsigned)((n + 3) / 4));
-    if (h->exh_k == 1)
-      hipLaunchKernelGGL(k_exh_run<1>, grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n, fail, evald);
-    else if (h->exh_k == 4)
-      hipLaunchKernelGGL(k_exh_run<4>, grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n, fail, evald);
-    else
-      hipLaunchKernelGGL(k_exh_run<2>, grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n, fail, evald);
+    if (h->exh_pk) {
+      if (h->exh_k == 1)
+        hipLaunchKernelGGL((k_exh_run<1, true>), grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n,
+                           fail, evald);
+      else
+        hipLaunchKernelGGL((k_exh_run<2, true>), grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n,
+                           fail, evald);
+    } else {
+      if (h->exh_k == 1)
+        hipLaunchKernelGGL((k_exh_run<1, false>), grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n,
+                           fail, evald);
+      else
+        hipLaunchKernelGGL((k_exh_run<2, false>), grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n,
+                           fail, evald);
+    }
     HIPCHK(h, hipGetLastError());
     rc = timed_end(h, ts);
     if (rc) return rc;
@@ -1558,10 +1643,8 @@ extern "C" int sx_open(int device, sx_handle **out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->num_cu = prop.multiProcessorCount;
   if (const char *e = getenv("SX_BULK_AFTER")) h->bulk_after = std::max(-1, atoi(e));
-  if (const char *e = getenv("SX_EXH_K")) {
-    const int k = atoi(e);
-    h->exh_k = (k == 1 || k == 4) ? k : 2;
-  }
+  if (const char *e = getenv("SX_EXH_K")) h->exh_k = atoi(e) == 1 ? 1 : 2;
+  if (const char *e = getenv("SX_EXH_PK")) h->exh_pk = atoi(e) != 0;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->kev[0]) != hipSuccess || hipEventCreate(&h->kev[1]) != hipSuccess ||

@@ -10,8 +10,9 @@ call and stays resident in HBM, so timed steps start with their inputs there.
 
 Prints one JSON line.  `value` is QuickCheck pairs per second of the whole
 pipeline (pairs / step wall time).  The roofline object describes the
-dominant kernel, timed with HIP events on the library's own stream over the
-timed steps; the CPU baseline is the C restatement of the Go enumerate
+exhaustive sweep (the kernel the frozen cost model prices as executed; the
+QuickCheck kernel's model figure is reported beside it), timed with HIP
+events on the library's own stream over the timed steps; the CPU baseline is the C restatement of the Go enumerate
 (oracle/) on a bounded sample of the same targets, on the host's cores.
 `c3` times 65,536 length-3 targets against all 8,199,632 length-1/2
 candidates; `stoke` gives the median and p90 time to the first verified
@@ -268,13 +269,24 @@ def main():
     for k in kernels.values():
         k["units_per_launch"] = k["units"] / max(k["launches"], 1)
         k["achieved_Tops"] = k["units"] * k["ops_per_unit"] / max(k["kernel_ms"], 1e-9) / 1e9
-    name = "k_exh_run" if k_exh >= k_qc else "k_quickcheck"
+    for k in kernels.values():
+        k["model_frac"] = k["achieved_Tops"] * 1e12 / PEAK_INT32
+    # The roofline is the exhaustive sweep's: the cost model prices every
+    # state it evaluates exactly as executed (T and C run, one compare).  The
+    # model's QuickCheck pair (8 vector evaluations) is not what k_quickcheck
+    # executes (precomputed outputs, rejection on vector 0), so its model
+    # figure exceeds the peak; it stays under "kernels" with that caveat.
+    name = "k_exh_run"
     dom = kernels[name]
     achieved = dom["achieved_Tops"] * 1e12
     roof = {"kernel": name, "ops_per_unit": dom["ops_per_unit"], "units_per_launch": dom["units_per_launch"],
             "avg_launch_ms": dom["avg_launch_ms"], "bound": "valu-int32", "achieved": achieved / 1e12,
             "peak": PEAK_INT32 / 1e12, "unit": "Tops/s", "frac": achieved / PEAK_INT32,
-            "traffic": hbm_traffic(name)}
+            "traffic": hbm_traffic(name),
+            "kernel_time_share": k_exh / max(k_exh + k_qc, 1e-9),
+            "note": "k_quickcheck has the larger kernel time (kernels); the 128-op QC-pair model charges 8 vector "
+                    "evaluations per pair that the kernel does not execute, so its model_frac exceeds 1 and it "
+                    "is not used as the roofline"}
     out = {
         "metric": "QC pairs/s", "value": PAIRS / (ms * 1e-3), "unit": "pairs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
