@@ -229,20 +229,29 @@ __global__ __launch_bounds__(1024) void k_scan_add(uint64_t *out, uint64_t n, co
   if (i < n) out[i] += sums[blockIdx.x];
 }
 
+// QuickCheck filter hash: a 32-bit mix of the outputs on QuickCheck vectors
+// 0-3, lo word ANDed with the compare mask.  Equal outputs give equal
+// hashes, so comparing hashes first drops no pair that passes; a collision
+// only costs the exact 8-vector compare that every hash match gets.
+__device__ __forceinline__ uint32_t qc_mix(uint32_t lo_m, uint32_t hi, uint32_t v) {
+  const uint32_t h = (lo_m * 0x9E3779B1u) ^ ((hi + v * 0x632BE5ABu) * 0x85EBCA77u);
+  return h ^ (h >> 16);
+}
+
 // Outputs of sequences on the 32 Mid vectors.  A wave takes kFpPerWave
 // sequences in turn (lane v < 32 runs vector v with the sequence's
 // wave-uniform opcodes); their words are fetched at once, one per lane, and
 // read back with readlane, so the sequences do not wait on dependent scalar
-// loads.  fp[i * 32 + v] = (lo, hi) with P masked by 0xCF; v0[i] (when given)
-// = the vector-0 output again, densely, with lo ANDed with v0_lo_mask (the
-// QuickCheck compare mask), so that eight consecutive targets are one 64-byte
-// scalar load in QuickCheck and need no masking there.  Lane k < kFpPerWave computes bytes,
+// loads.  fp[i * 32 + v] = (lo, hi) with P masked by 0xCF; th[i] (when
+// given) = the QuickCheck filter hash under th_lo_mask (the QuickCheck compare
+// mask), densely, so that eight consecutive targets are one 32-byte scalar
+// load in QuickCheck.  Lane k < kFpPerWave computes bytes,
 // cycles and the pruner flag of the wave's k-th sequence.
 constexpr uint32_t kFpPerWave = 8;
 static_assert(kFpPerWave * 4 <= kWave, "sequence words must fit one per lane");
 __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uint32_t len,
-                                                      uint64_t n, int bytes_model, uint2 *fp, uint2 *v0,
-                                                      uint32_t v0_lo_mask, uint8_t *bytes, uint8_t *cyc, uint8_t *prune,
+                                                      uint64_t n, int bytes_model, uint2 *fp, uint32_t *th,
+                                                      uint32_t th_lo_mask, uint8_t *bytes, uint8_t *cyc, uint8_t *prune,
                                                       uint32_t *bad_op) {
   const uint64_t first = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kFpPerWave;
   if (first >= n) return;
@@ -257,6 +266,7 @@ __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uin
       bad_any = true;
       continue;
     }
+    uint32_t hv = 0;
     if (lane < 32) {
       State st = load_vec(lane);
       for (uint32_t k = 0; k < len; k++) {
@@ -266,7 +276,12 @@ __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uin
       st.p &= kPMask;
       const uint2 o = make_uint2(pack_lo(st), pack_hi(st));
       fp[(first + q) * 32 + lane] = o;
-      if (v0 && lane == 0) v0[first + q] = make_uint2(o.x & v0_lo_mask, o.y);
+      if (lane < 4) hv = qc_mix(o.x & th_lo_mask, o.y, lane);
+    }
+    if (th) {
+      const uint32_t h = __builtin_amdgcn_readlane(hv, 0) ^ __builtin_amdgcn_readlane(hv, 1) ^
+                         __builtin_amdgcn_readlane(hv, 2) ^ __builtin_amdgcn_readlane(hv, 3);
+      if (lane == 0) th[first + q] = h;
     }
   }
   if (bad_any && lane == 0) atomicOr(bad_op, 1u);
@@ -320,7 +335,7 @@ struct QcQueue {
   }
 };
 
-__device__ __forceinline__ void qc_item(const uint2 *__restrict__ tv0, const uint2 *__restrict__ tfp, uint32_t nt,
+__device__ __forceinline__ void qc_item(const uint32_t *__restrict__ th, const uint2 *__restrict__ tfp, uint32_t nt,
                                         uint32_t t0, uint32_t t1, const uint2 *__restrict__ cfp, uint32_t nc,
                                         uint32_t slot, uint32_t pm_masked, QcQueue &Q) {
   const uint32_t lane = lane_id();
@@ -334,23 +349,24 @@ __device__ __forceinline__ void qc_item(const uint2 *__restrict__ tv0, const uin
   }
   const uint32_t mlo = 0x00FFFFFFu | (pm_masked << 24);
   const uint32_t flo = 0x00FFFFFFu | (kPMask << 24);
-  const uint32_t clo0 = clo[0] & mlo;  // the vector-0 plane's lo words carry the same mask
+  // the candidate's filter hash, as the targets' plane holds it
+  const uint32_t hc = qc_mix(clo[0] & mlo, chi[0], 0) ^ qc_mix(clo[1] & mlo, chi[1], 1) ^
+                      qc_mix(clo[2] & mlo, chi[2], 2) ^ qc_mix(clo[3] & mlo, chi[3], 3);
   const uint64_t vmask = ballot(valid);
   for (uint32_t t = t0; t < t1; t += 8) {
     uint32_t nb = min(8u, t1 - t);
-    // one 64-byte scalar load of 8 consecutive targets (the plane has 8
-    // entries of slack past nt, so no clamping is needed)
-    struct alignas(64) Tv8 {
-      uint2 v[8];
+    // one 32-byte scalar load of 8 consecutive target hashes (the plane has
+    // 8 entries of slack past nt, so no clamping is needed)
+    struct alignas(32) Th8 {
+      uint32_t h[8];
     };
-    const Tv8 blk = *reinterpret_cast<const Tv8 *>(tv0 + t);  // t is a multiple of 8
-    const uint2 *tv = blk.v;
-    // target t + u against the wave's candidates, d = the vector-0 difference
-    auto check = [&](uint32_t u, uint32_t d) {
-      const uint64_t m = ballot(d == 0) & vmask;
+    const Th8 blk = *reinterpret_cast<const Th8 *>(th + t);  // t is a multiple of 8
+    // target t + u against the wave's candidates: hash filter, then all 8
+    // vectors exactly under both masks (the differences ORed together first:
+    // one 3-input VALU op per dword)
+    auto check = [&](uint32_t u) {
+      const uint64_t m = ballot(hc == blk.h[u]) & vmask;
       if (m == 0) return;
-      // slow path: all 8 vectors under both masks (the differences ORed
-      // together first: one 3-input VALU op per dword)
       const uint2 *tf = tfp + (uint64_t)(t + u) * 32;
       uint32_t acc_lo = 0, acc_hi = 0;
 #pragma unroll
@@ -368,23 +384,19 @@ __device__ __forceinline__ void qc_item(const uint2 *__restrict__ tv0, const uin
       if (Q.qn > kQcQueue - kWave) Q.flush(lane);
     };
     if (nb == 8) {
-      // groups of 4 targets: one ballot and branch per group when no lane
-      // matches any of the 4 on vector 0 (the common case)
+      // one ballot per compare, ORed: one branch per 8 targets when no lane
+      // matches any of them (the common case)
+      uint64_t m = 0;
 #pragma unroll
-      for (uint32_t g = 0; g < 8; g += 4) {
-        uint32_t d[4];
+      for (int j = 0; j < 8; j++) m |= ballot(hc == blk.h[j]);
+      if ((m & vmask) == 0) continue;
 #pragma unroll
-        for (int j = 0; j < 4; j++) d[j] = (clo0 ^ tv[g + j].x) | (chi[0] ^ tv[g + j].y);
-        const uint32_t dmin = min(min(d[0], d[1]), min(d[2], d[3]));
-        if ((ballot(dmin == 0) & vmask) == 0) continue;
-#pragma unroll
-        for (int j = 0; j < 4; j++) check(g + j, d[j]);
-      }
+      for (uint32_t u = 0; u < 8; u++) check(u);
     } else {
 #pragma unroll
       for (uint32_t u = 0; u < 8; u++) {
         if (u >= nb) break;
-        check(u, (clo0 ^ tv[u].x) | (chi[0] ^ tv[u].y));
+        check(u);
       }
     }
   }
@@ -392,7 +404,7 @@ __device__ __forceinline__ void qc_item(const uint2 *__restrict__ tv0, const uin
 
 // Grid form: block (x, y) = candidate slots [256 x, 256 x + 256) against
 // target tile y.
-__global__ __launch_bounds__(256) void k_quickcheck(const uint2 *__restrict__ tv0,
+__global__ __launch_bounds__(256) void k_quickcheck(const uint32_t *__restrict__ th,
                                                     const uint2 *__restrict__ tfp, uint32_t nt,
                                                     uint32_t tile, const uint2 *__restrict__ cfp,
                                                     uint32_t nc, uint32_t pm_masked,
@@ -401,7 +413,7 @@ __global__ __launch_bounds__(256) void k_quickcheck(const uint2 *__restrict__ tv
   __shared__ uint2 queue[4][kQcQueue];
   QcQueue Q{queue[threadIdx.x >> 6], 0, surv, surv_cap, surv_count};
   const uint32_t t0 = blockIdx.y * tile;
-  qc_item(tv0, tfp, nt, t0, min(nt, t0 + tile), cfp, nc, blockIdx.x * 256 + threadIdx.x, pm_masked, Q);
+  qc_item(th, tfp, nt, t0, min(nt, t0 + tile), cfp, nc, blockIdx.x * 256 + threadIdx.x, pm_masked, Q);
   if (Q.qn) Q.flush(lane_id());
 }
 
@@ -411,7 +423,7 @@ __global__ __launch_bounds__(256) void k_quickcheck(const uint2 *__restrict__ tv
 // tiles q, q + 8, ... so a tile's target data is read through one XCD's L2;
 // a block drains its own queue first, then the others.  Every block leaves
 // once all 8 queues are drained.
-__global__ __launch_bounds__(256) void k_qc_persistent(const uint2 *__restrict__ tv0,
+__global__ __launch_bounds__(256) void k_qc_persistent(const uint32_t *__restrict__ th,
                                                        const uint2 *__restrict__ tfp, uint32_t nt, uint32_t tile,
                                                        const uint2 *__restrict__ cfp, uint32_t nc,
                                                        uint32_t pm_masked, uint2 *__restrict__ surv,
@@ -434,7 +446,7 @@ __global__ __launch_bounds__(256) void k_qc_persistent(const uint2 *__restrict__
       if (item >= nitems) break;
       const uint32_t tt = q + 8 * (item / ncb), cb = item % ncb;
       const uint32_t t0 = tt * tile;
-      qc_item(tv0, tfp, nt, t0, min(nt, t0 + tile), cfp, nc, cb * 256 + threadIdx.x, pm_masked, Q);
+      qc_item(th, tfp, nt, t0, min(nt, t0 + tile), cfp, nc, cb * 256 + threadIdx.x, pm_masked, Q);
     }
   }
   if (Q.qn) Q.flush(lane_id());
@@ -1219,6 +1231,7 @@ struct sx_handle {
   int bulk_after = 1;     // first-hit rounds before the one-batch tail (SX_BULK_AFTER); -1 = by survivor count
   int exh_k = 1;          // states (pairs when packed) per lane and opcode dispatch in the sweep (SX_EXH_K: 1 or 2)
   int exh_pk = 1;         // packed two-states-per-register interpreter in the sweep (SX_EXH_PK=0: one state)
+  int seg_log = 20;       // targets per segment = 2^seg_log (SX_SEG_LOG, 16..24)
   uint32_t surv_cap_hint = 0;  // QuickCheck survivor buffer size that sufficed before
   // kernel-only timing of the QuickCheck and exhaustive-sweep launches
   hipEvent_t kev[2] = {nullptr, nullptr};
@@ -1474,7 +1487,7 @@ int ensure_l2(sx_handle *h) {
 // One QuickCheck pass over targets [0, nt) x candidate slots [0, nc) with a
 // given launch configuration (blocks_per_cu 0: grid form with `tile` targets
 // per block row; else persistent form, num_cu x blocks_per_cu blocks).
-int launch_qc(sx_handle *h, uint32_t tile, uint32_t blocks_per_cu, const uint2 *tv0, const uint2 *tfp, uint64_t nt,
+int launch_qc(sx_handle *h, uint32_t tile, uint32_t blocks_per_cu, const uint32_t *tv0, const uint2 *tfp, uint64_t nt,
               const uint2 *cfp, uint64_t nc, uint32_t pm_masked, uint2 *surv, uint32_t surv_cap,
               uint32_t *counters) {
   if (blocks_per_cu == 0) {
@@ -1498,7 +1511,7 @@ int launch_qc(sx_handle *h, uint32_t tile, uint32_t blocks_per_cu, const uint2 *
 // max_cand_len) shape, time every candidate configuration on a slice of the
 // real work (about 4e9 pairs) and keep the fastest for the handle.  The
 // survivors of these runs are discarded; the real pass runs afterwards.
-int tune_qc(sx_handle *h, int tl, int ml, const uint2 *tv0, const uint2 *tfp, uint64_t nt, const uint2 *cfp,
+int tune_qc(sx_handle *h, int tl, int ml, const uint32_t *tv0, const uint2 *tfp, uint64_t nt, const uint2 *cfp,
             uint64_t nc, uint32_t pm_masked, uint2 *surv, uint32_t surv_cap, uint32_t *counters,
             sx_handle::QcChoice *out) {
   const int big = nt >= 65536;
@@ -1645,6 +1658,7 @@ extern "C" int sx_open(int device, sx_handle **out) {
   if (const char *e = getenv("SX_BULK_AFTER")) h->bulk_after = std::max(-1, atoi(e));
   if (const char *e = getenv("SX_EXH_K")) h->exh_k = atoi(e) == 1 ? 1 : 2;
   if (const char *e = getenv("SX_EXH_PK")) h->exh_pk = atoi(e) != 0;
+  if (const char *e = getenv("SX_SEG_LOG")) h->seg_log = std::min(24, std::max(16, atoi(e)));
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->kev[0]) != hipSuccess || hipEventCreate(&h->kev[1]) != hipSuccess ||
@@ -1802,7 +1816,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   }
 
   // ---- segments of targets
-  const uint64_t seg = cfg->targets ? std::min<uint64_t>(nt_all, 1u << 20) : (1u << 20);
+  const uint64_t seg_max = 1ull << h->seg_log;
+  const uint64_t seg = cfg->targets ? std::min<uint64_t>(nt_all, seg_max) : seg_max;
   // QuickCheck launch shape: the caller's tile (grid form), else the autotuner's choice
   sx_handle::QcChoice qcc{tl, ml, 0, cfg->tile_targets, 0};
   bool qc_tuned = cfg->tile_targets != 0;
@@ -1845,11 +1860,12 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       if (nt == 0) continue;
     }
     uint32_t *tw;
-    uint2 *tfp, *tv0;
+    uint2 *tfp;
+    uint32_t *tv0;  // the targets' QuickCheck filter hashes
     uint8_t *tbytes, *tcyc, *tprune;
     ENSURE(B_TARGETS, uint32_t, nt * tl, tw);
     ENSURE(B_TFP, uint2, nt * 32, tfp);
-    ENSURE(B_TV0, uint2, nt + 8, tv0);  // slack for the 8-target loads of QuickCheck
+    ENSURE(B_TV0, uint32_t, nt + 8, tv0);  // slack for the 8-target loads of QuickCheck
     ENSURE(B_TBYTES, uint8_t, nt, tbytes);
     ENSURE(B_TCYC, uint8_t, nt, tcyc);
     ENSURE(B_TPRUNE, uint8_t, nt, tprune);
