@@ -1231,7 +1231,7 @@ struct sx_handle {
   int bulk_after = 1;     // first-hit rounds before the one-batch tail (SX_BULK_AFTER); -1 = by survivor count
   int exh_k = 1;          // states (pairs when packed) per lane and opcode dispatch in the sweep (SX_EXH_K: 1 or 2)
   int exh_pk = 1;         // packed two-states-per-register interpreter in the sweep (SX_EXH_PK=0: one state)
-  int seg_log = 20;       // targets per segment = 2^seg_log (SX_SEG_LOG, 16..24)
+  int seg_log = 22;       // targets per segment = 2^seg_log (SX_SEG_LOG, 16..24)
   uint32_t surv_cap_hint = 0;  // QuickCheck survivor buffer size that sufficed before
   // kernel-only timing of the QuickCheck and exhaustive-sweep launches
   hipEvent_t kev[2] = {nullptr, nullptr};
