@@ -1,25 +1,34 @@
 // sx_core.hip -- lib6502core: the MI355X search core of the 6502
 // superoptimizer, behind the C ABI of include/sx6502.h.
 //
-// Pipeline of one enumeration (per segment of targets):
-//   1. fingerprints  one wave per sequence, lane v = Mid vector v (0..31):
-//                    the sequence is wave-uniform, so the interpreter runs
-//                    with scalar dispatch.  Targets and candidates get the
-//                    outputs on all 32 vectors (QuickCheck vectors = 0..7).
-//   2. QuickCheck    a lane per candidate holds its 8 QC outputs in VGPRs and
-//                    walks a tile of wave-uniform targets (scalar loads); the
-//                    first vector rejects almost every pair, survivors are
-//                    collected by ballot + mbcnt into an LDS queue per wave
-//                    and flushed with one atomic per wave.
+// Pipeline of one enumeration (per segment of up to 2^22 targets):
+//   1. fingerprints  8 sequences per wave in turn, lane v = Mid vector v
+//                    (0..31): the sequence is wave-uniform, so the
+//                    interpreter runs with scalar dispatch.  Targets and
+//                    candidates get the outputs on all 32 vectors
+//                    (QuickCheck vectors = 0..7); targets also get a 32-bit
+//                    filter hash of their masked outputs on vectors 0..3.
+//   2. QuickCheck    a lane per candidate holds its 8 QC outputs and its hash
+//                    in VGPRs and walks a tile of wave-uniform targets
+//                    (scalar loads of 8 hashes at a time); a hash match gets
+//                    the exact 8-vector compare; survivors are collected by
+//                    ballot + mbcnt into an LDS queue per wave and flushed
+//                    with one atomic per batch.
 //   3. MidCheck      a lane per survivor: byte filter, candidate pruner and a
-//                    compare of the 24 further vector outputs.
-//   4. Exhaustive    jobs = (target, candidate) pairs; a plan kernel sizes
-//                    each job's sweep(s), a scan turns sizes into chunk
-//                    offsets and a persistent kernel pulls 16K-state chunks
-//                    from a device counter (wave-uniform ops, early exit per
-//                    job).  First-hit semantics run this in rounds: each
-//                    round checks, per target and pass, the lowest-ranked
-//                    candidate not yet rejected.
+//                    compare of the 24 further vector outputs; output
+//                    reserved once per 1,024-thread block.
+//   4. Exhaustive    jobs = (target, candidate) pairs, planned where they are
+//                    made (sub-sweeps, 16K-state chunk counts); a scan turns
+//                    counts into chunk offsets and a chunk -> job map, and a
+//                    wave per chunk sweeps it with the packed interpreter
+//                    (two states per register), early exit per job.
+//                    First-hit semantics run one round (the lowest-ranked
+//                    untested candidate per target and pass) and then the
+//                    rest as one batch whose minimal passing rank per target
+//                    and pass is the rule.
+//   5. Rules         compacted in (target, pass) order on the device and
+//                    copied to a pinned host buffer per segment on a second
+//                    stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -229,29 +238,45 @@ __global__ __launch_bounds__(1024) void k_scan_add(uint64_t *out, uint64_t n, co
   if (i < n) out[i] += sums[blockIdx.x];
 }
 
-// QuickCheck filter hash: a 32-bit mix of the outputs on QuickCheck vectors
-// 0-3, lo word ANDed with the compare mask.  Equal outputs give equal
-// hashes, so comparing hashes first drops no pair that passes; a collision
-// only costs the exact 8-vector compare that every hash match gets.
-__device__ __forceinline__ uint32_t qc_mix(uint32_t lo_m, uint32_t hi, uint32_t v) {
-  const uint32_t h = (lo_m * 0x9E3779B1u) ^ ((hi + v * 0x632BE5ABu) * 0x85EBCA77u);
+// Filter hashes.  qc_mix(lo, hi, v) hashes one vector's output (64 bits and
+// the vector index) to 32 bits with two rounds of the murmur3 finalizer,
+// which spreads every input bit over the whole word; sequences are hashed by
+// XOR over vectors.  (A single multiply leaves a top-bit difference in the
+// top bit only, and two such differences cancel in the XOR: that version
+// let through 13% more MidCheck pairs on the reference's sample.)  Equal
+// outputs give equal hashes, so comparing hashes drops no pair whose
+// outputs agree; a collision only costs the exact check that follows.
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
   return h ^ (h >> 16);
+}
+__device__ __forceinline__ uint32_t qc_mix(uint32_t lo_m, uint32_t hi, uint32_t v) {
+  return fmix32(lo_m + fmix32(hi ^ (v * 0x9E3779B1u)));
 }
 
 // Outputs of sequences on the 32 Mid vectors.  A wave takes kFpPerWave
 // sequences in turn (lane v < 32 runs vector v with the sequence's
 // wave-uniform opcodes); their words are fetched at once, one per lane, and
 // read back with readlane, so the sequences do not wait on dependent scalar
-// loads.  fp[i * 32 + v] = (lo, hi) with P masked by 0xCF; th[i] (when
-// given) = the QuickCheck filter hash under th_lo_mask (the QuickCheck compare
-// mask), densely, so that eight consecutive targets are one 32-byte scalar
-// load in QuickCheck.  Lane k < kFpPerWave computes bytes,
+// loads.  fp[i * 8 + v] = (lo, hi) on QuickCheck vector v < 8, with P masked
+// by 0xCF; mh[i] = (hash of the outputs on Mid vectors 8..31 with P masked by
+// 0xCF, the same with the lo words ANDed with lo_mask): MidCheck compares
+// these instead of 24 vectors (equal outputs give equal hashes, so the
+// filter drops no pair the exact check would keep); th[i] (when given) = the
+// QuickCheck filter hash under lo_mask, densely, so that eight consecutive
+// targets are one 32-byte scalar load in QuickCheck.  lo_mask is the
+// QuickCheck compare mask, 0x00FFFFFF | pm_masked << 24.  Lane k < kFpPerWave computes bytes,
 // cycles and the pruner flag of the wave's k-th sequence.
 constexpr uint32_t kFpPerWave = 8;
+constexpr uint32_t kFpVecs = 8;  // vectors kept per sequence: the QuickCheck ones
 static_assert(kFpPerWave * 4 <= kWave, "sequence words must fit one per lane");
 __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uint32_t len,
-                                                      uint64_t n, int bytes_model, uint2 *fp, uint32_t *th,
-                                                      uint32_t th_lo_mask, uint8_t *bytes, uint8_t *cyc, uint8_t *prune,
+                                                      uint64_t n, int bytes_model, uint2 *fp, uint2 *mh,
+                                                      uint32_t *th, uint32_t lo_mask, uint8_t *bytes, uint8_t *cyc,
+                                                      uint8_t *prune,
                                                       uint32_t *bad_op) {
   const uint64_t first = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kFpPerWave;
   if (first >= n) return;
@@ -259,6 +284,7 @@ __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uin
   const uint32_t cnt = (uint32_t)(first + kFpPerWave < n ? kFpPerWave : n - first);
   const uint32_t wv = lane < cnt * len ? words[first * len + lane] : 0u;  // len <= 4
   bool bad_any = false;
+  const bool two_masks = (lo_mask >> 24) != kPMask;  // else the masked hash is the exact one
   for (uint32_t q = 0; q < cnt; q++) {
     bool bad = false;
     for (uint32_t k = 0; k < len; k++) bad |= (__builtin_amdgcn_readlane(wv, q * len + k) & 0xFFFFu) >= OP_COUNT;
@@ -266,7 +292,7 @@ __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uin
       bad_any = true;
       continue;
     }
-    uint32_t hv = 0;
+    uint32_t hv = 0, hx = 0, hm = 0;
     if (lane < 32) {
       State st = load_vec(lane);
       for (uint32_t k = 0; k < len; k++) {
@@ -275,9 +301,24 @@ __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uin
       }
       st.p &= kPMask;
       const uint2 o = make_uint2(pack_lo(st), pack_hi(st));
-      fp[(first + q) * 32 + lane] = o;
-      if (lane < 4) hv = qc_mix(o.x & th_lo_mask, o.y, lane);
+      if (lane < kFpVecs) {
+        fp[(first + q) * kFpVecs + lane] = o;
+        if (lane < 4) hv = qc_mix(o.x & lo_mask, o.y, lane);
+      } else {
+        hx = qc_mix(o.x, o.y, lane);
+        if (two_masks) hm = qc_mix(o.x & lo_mask, o.y, lane);
+      }
     }
+    // XOR of the Mid hashes over lanes 8..31 (the others hold 0)
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) hx ^= __shfl_xor(hx, d, 32);
+    if (two_masks) {
+#pragma unroll
+      for (int d = 1; d < 32; d <<= 1) hm ^= __shfl_xor(hm, d, 32);
+    } else {
+      hm = hx;
+    }
+    if (lane == 0) mh[first + q] = make_uint2(hx, hm);
     if (th) {
       const uint32_t h = __builtin_amdgcn_readlane(hv, 0) ^ __builtin_amdgcn_readlane(hv, 1) ^
                          __builtin_amdgcn_readlane(hv, 2) ^ __builtin_amdgcn_readlane(hv, 3);
@@ -343,7 +384,7 @@ __device__ __forceinline__ void qc_item(const uint32_t *__restrict__ th, const u
   uint32_t clo[8], chi[8];
 #pragma unroll
   for (int v = 0; v < 8; v++) {
-    uint2 f = valid ? cfp[(uint64_t)slot * 32 + v] : make_uint2(0, 0);
+    uint2 f = valid ? cfp[(uint64_t)slot * kFpVecs + v] : make_uint2(0, 0);
     clo[v] = f.x;
     chi[v] = f.y;
   }
@@ -367,7 +408,7 @@ __device__ __forceinline__ void qc_item(const uint32_t *__restrict__ th, const u
     auto check = [&](uint32_t u) {
       const uint64_t m = ballot(hc == blk.h[u]) & vmask;
       if (m == 0) return;
-      const uint2 *tf = tfp + (uint64_t)(t + u) * 32;
+      const uint2 *tf = tfp + (uint64_t)(t + u) * kFpVecs;
       uint32_t acc_lo = 0, acc_hi = 0;
 #pragma unroll
       for (int v = 0; v < 8; v++) {
@@ -478,16 +519,18 @@ __device__ __forceinline__ uint32_t block_prefix(uint32_t cnt, uint32_t *s, uint
 // MidCheck and stage bookkeeping.  One lane per QuickCheck survivor.
 // hits_all = 0 (Go): pass 1 if the exact compare held, pass 2 if only the
 // masked one did (and dead != 0).  hits_all = 1 (CUDA tool): one masked pass.
-// Output entry: (target, slot | pass << 30).  counters[0] += QuickCheck hits
-// after the byte/prune filters and counters[1] += passes, both with one
-// 64-bit atomic per block (counters[1] is also the output cursor).
+// The 24 Mid vectors are compared through their hashes (tmh/cmh: exact and
+// masked P), which keeps every pair whose outputs agree.  Output entry:
+// (target, slot | pass << 30).  counters[0] += QuickCheck hits after the
+// byte/prune filters and counters[1] += passes, both with one 64-bit atomic
+// per block (counters[1] is also the output cursor).
 __global__ __launch_bounds__(kBlk) void k_midcheck(const uint2 *__restrict__ surv, uint32_t ns,
-                                                   const uint2 *__restrict__ tfp,
+                                                   const uint2 *__restrict__ tmh,
                                                    const uint8_t *__restrict__ tbytes,
-                                                   const uint2 *__restrict__ cfp,
+                                                   const uint2 *__restrict__ cmh,
                                                    const uint8_t *__restrict__ cbytes,
                                                    const uint8_t *__restrict__ cprune,
-                                                   uint32_t pm_masked, int hits_all, int use_mid,
+                                                   int hits_all, int use_mid,
                                                    uint2 *__restrict__ out, uint32_t out_cap,
                                                    uint32_t *__restrict__ counters) {
   __shared__ uint32_t s_hits[kBlkWaves], s_pass[kBlkWaves];
@@ -497,52 +540,18 @@ __global__ __launch_bounds__(kBlk) void k_midcheck(const uint2 *__restrict__ sur
   uint32_t t = 0, slot = 0, pass = 0;
   bool qc_hit = false;
   if (i < ns) {
-    uint2 e = surv[i];
+    const uint2 e = surv[i];
     t = e.x;
     slot = e.y & 0x3FFFFFFFu;
-    bool full = (e.y >> 31) != 0;
+    const bool full = (e.y >> 31) != 0;
     if (cbytes[slot] < tbytes[t] && !cprune[slot]) {
-      uint32_t pm;
-      if (hits_all) {
-        pass = 1;
-        pm = pm_masked;
-      } else if (full) {
-        pass = 1;
-        pm = kPMask;
-      } else {
-        pass = 2;
-        pm = pm_masked;
-      }
+      const bool exact = !hits_all && full;  // pass 1 compares P under 0xCF
+      pass = (hits_all || full) ? 1 : 2;
       qc_hit = true;
       bool ok = true;
       if (use_mid) {
-        // vectors 8..31 as 16-byte loads issued together (two vectors per
-        // load: (lo, hi, lo, hi)), in two stages so most failures stop after
-        // the first 8
-        const uint32_t mlo = 0x00FFFFFFu | (pm << 24);
-        const uint4 *a = reinterpret_cast<const uint4 *>(tfp + (uint64_t)t * 32 + 8);
-        const uint4 *b = reinterpret_cast<const uint4 *>(cfp + (uint64_t)slot * 32 + 8);
-        auto same = [&](const uint4 &x, const uint4 &y) {
-          return ((((x.x ^ y.x) & mlo) | (x.y ^ y.y)) | (((x.z ^ y.z) & mlo) | (x.w ^ y.w))) == 0;
-        };
-        uint4 x[4], y[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          x[k] = a[k];
-          y[k] = b[k];
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) ok = ok && same(x[k], y[k]);
-        if (ok) {
-          uint4 x2[8], y2[8];
-#pragma unroll
-          for (int k = 0; k < 8; k++) {
-            x2[k] = a[4 + k];
-            y2[k] = b[4 + k];
-          }
-#pragma unroll
-          for (int k = 0; k < 8; k++) ok = ok && same(x2[k], y2[k]);
-        }
+        const uint2 a = tmh[t], b = cmh[slot];
+        ok = exact ? a.x == b.x : a.y == b.y;
       }
       pass_ok = ok;
     }
@@ -995,7 +1004,7 @@ __global__ __launch_bounds__(256) void k_round_resolve(const Job *jobs, uint32_t
   bool ok = !fail[i];
   uint32_t fd = 0;
   if (ok && !hits_all && j.pass == 2) {
-    const uint2 *a = tfp + (uint64_t)j.target * 32, *b = cfp + (uint64_t)j.slot * 32;
+    const uint2 *a = tfp + (uint64_t)j.target * kFpVecs, *b = cfp + (uint64_t)j.slot * kFpVecs;
     for (int v = 0; v < 8; v++) fd |= ((a[v].x ^ b[v].x) >> 24) & kPMask;
     ok = fd != 0;  // FlagDiff == 0 is skipped (worker.go:211-213)
   }
@@ -1072,7 +1081,7 @@ __global__ __launch_bounds__(kBlk) void k_bulk_jobs(const uint2 *mid, uint32_t n
 // FlagDiff over the 8 QuickCheck vectors from the fingerprints: 0 when a
 // non-P byte differs is already excluded by the masked QuickCheck.
 __device__ __forceinline__ uint32_t fp_flag_diff(const uint2 *tfp, const uint2 *cfp, uint32_t t, uint32_t slot) {
-  const uint2 *a = tfp + (uint64_t)t * 32, *b = cfp + (uint64_t)slot * 32;
+  const uint2 *a = tfp + (uint64_t)t * kFpVecs, *b = cfp + (uint64_t)slot * kFpVecs;
   uint32_t fd = 0;
   for (int v = 0; v < 8; v++) fd |= ((a[v].x ^ b[v].x) >> 24) & kPMask;
   return fd;
@@ -1268,7 +1277,7 @@ struct sx_handle {
   uint32_t *d_l2 = nullptr;
   uint64_t n_l2 = 0;
   // buffers reused across calls
-  DevBuf bufs[48];
+  DevBuf bufs[64];
 };
 
 namespace {
@@ -1278,8 +1287,9 @@ enum BufId {
   B_RANK, B_SURV, B_MID, B_COUNTERS, B_CUR, B_REJ, B_DONE, B_JOBS, B_CHUNKS, B_OFFS, B_SUMS,
   B_FAIL, B_RULES, B_SCRATCH, B_GEN_COUNTS, B_GEN_OFFS, B_BADOP, B_P_TW, B_P_TL, B_P_CW, B_P_CL,
   B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR, B_HEADS, B_RULEFLAG, B_RULEOFFS, B_RULES2,
-  B_RULETOT, B_TRAW, B_CHUNKMAP
+  B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_COUNT
 };
+static_assert(B_COUNT <= 64, "sx_handle::bufs holds every buffer id");
 
 int set_err(sx_handle *h, int code, const char *fmt, ...) {
   if (h) {
@@ -1776,11 +1786,14 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   // ---- candidates: slots in Go order (rank = slot)
   const uint64_t nc = ml == 1 ? (uint64_t)kNumInsts : (uint64_t)kNumInsts + (uint64_t)kNumInsts * kNumInsts;
   uint32_t *cwords, *rank, *badop;
-  uint2 *cfp;
+  uint2 *cfp, *cmh;
   uint8_t *cbytes, *ccyc, *cprune;
+  const uint32_t pm_masked = kPMask & ~cfg->dead_flags;
+  const uint32_t lo_mask = 0x00FFFFFFu | (pm_masked << 24);
   ENSURE(B_CWORDS, uint32_t, 2 * nc, cwords);
   ENSURE(B_RANK, uint32_t, nc, rank);
-  ENSURE(B_CFP, uint2, nc * 32, cfp);
+  ENSURE(B_CFP, uint2, nc * kFpVecs, cfp);
+  ENSURE(B_CMH, uint2, nc, cmh);
   ENSURE(B_CBYTES, uint8_t, nc, cbytes);
   ENSURE(B_CCYC, uint8_t, nc, ccyc);
   ENSURE(B_CPRUNE, uint8_t, nc, cprune);
@@ -1803,13 +1816,14 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     for (int i = 0; i < kNumInsts; i++) tab[i] = go_inst(i);
     HIPCHK(h, hipMemcpyAsync(w1, tab.data(), kNumInsts * 4, hipMemcpyHostToDevice, h->stream));
     hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(kNumInsts, 4 * kFpPerWave)), dim3(256), 0, h->stream, w1, 1u,
-                       (uint64_t)kNumInsts, cfg->bytes_model, cfp, nullptr, ~0u, cbytes, ccyc, cprune, badop);
+                       (uint64_t)kNumInsts, cfg->bytes_model, cfp, cmh, nullptr, lo_mask, cbytes, ccyc, cprune,
+                       badop);
     if (ml == 2) {
       // pairs: words at cwords[2*slot] for slot >= 2863 are contiguous pairs
       hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(nc - kNumInsts, 4 * kFpPerWave)), dim3(256), 0, h->stream,
                          cwords + 2 * kNumInsts, 2u, nc - kNumInsts, cfg->bytes_model,
-                         cfp + (uint64_t)kNumInsts * 32, nullptr, ~0u, cbytes + kNumInsts, ccyc + kNumInsts,
-                         cprune + kNumInsts, badop);
+                         cfp + (uint64_t)kNumInsts * kFpVecs, cmh + kNumInsts, nullptr, lo_mask,
+                         cbytes + kNumInsts, ccyc + kNumInsts, cprune + kNumInsts, badop);
     }
     HIPCHK(h, hipGetLastError());
     SYNC_WD("device sync");  // w1 is host-staged
@@ -1829,7 +1843,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     const uint64_t est = std::min<uint64_t>(seg, nt_all) * nc / 4096;
     surv_cap = (uint32_t)std::max<uint64_t>({1u << 25, std::min<uint64_t>(est, 1u << 28), h->surv_cap_hint});
   }
-  const uint32_t pm_masked = kPMask & ~cfg->dead_flags;
+  // pm_masked (the masked compare's P mask) is defined with the candidates
   float ms_qc = 0, ms_mid = 0, ms_exh = 0, ms_setup = 0;
   HIPCHK(h, hipEventRecord(e1, h->stream));
   SYNC_WD("setup");
@@ -1860,11 +1874,12 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       if (nt == 0) continue;
     }
     uint32_t *tw;
-    uint2 *tfp;
+    uint2 *tfp, *tmh;
     uint32_t *tv0;  // the targets' QuickCheck filter hashes
     uint8_t *tbytes, *tcyc, *tprune;
     ENSURE(B_TARGETS, uint32_t, nt * tl, tw);
-    ENSURE(B_TFP, uint2, nt * 32, tfp);
+    ENSURE(B_TFP, uint2, nt * kFpVecs, tfp);
+    ENSURE(B_TMH, uint2, nt, tmh);
     ENSURE(B_TV0, uint32_t, nt + 8, tv0);  // slack for the 8-target loads of QuickCheck
     ENSURE(B_TBYTES, uint8_t, nt, tbytes);
     ENSURE(B_TCYC, uint8_t, nt, tcyc);
@@ -1878,7 +1893,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       HIPCHK(h, hipMemcpyAsync(tw, d_tsrc + s0 * 2, nt * 8, hipMemcpyDeviceToDevice, h->stream));
     }
     hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(nt, 4 * kFpPerWave)), dim3(256), 0, h->stream, tw, (uint32_t)tl,
-                       nt, cfg->bytes_model, tfp, tv0, 0x00FFFFFFu | (pm_masked << 24), tbytes, tcyc, tprune, badop);
+                       nt, cfg->bytes_model, tfp, tmh, tv0, lo_mask, tbytes, tcyc, tprune, badop);
     HIPCHK(h, hipGetLastError());
 
     // ---- QuickCheck, re-run with a larger buffer on overflow (no hit is dropped)
@@ -1925,7 +1940,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));
       if (nsurv)
         hipLaunchKernelGGL(k_midcheck, dim3(blocks_for(nsurv, kBlk)), dim3(kBlk), 0, h->stream, surv, nsurv,
-                           tfp, tbytes, cfp, cbytes, cprune, pm_masked, cfg->hits, cfg->use_mid, mid,
+                           tmh, tbytes, cmh, cbytes, cprune, cfg->hits, cfg->use_mid, mid,
                            std::max<uint32_t>(nsurv, 1), counters);
       HIPCHK(h, hipGetLastError());
       uint32_t cnt[2];

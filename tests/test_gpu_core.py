@@ -170,6 +170,21 @@ def test_c1_rule_sets(dev, name, kw):
         assert st["mid_pass"] == counts["mid_pass"]
 
 
+def test_full_length2_cuda_semantics_published_counts(dev):
+    """The full length-2 run in the CUDA tool's semantics (all hits, M ops 2
+    bytes, dead 0x00) reproduces the published QuickCheck and MidCheck
+    counts exactly (README.md:133-134: 20,654,396 and 5,373,297).  Its rule
+    count is 1,075,805 against the published 1,078,897: the reference checks
+    the 4,902 survivors with >= 3 extra registers by a 32-value reduced sweep
+    that admits 3,092 pairs the complete sweep rejects (counted with the
+    oracle on the CPU)."""
+    rules, st = dev.enumerate(None, dead=0x00, hits=core.HITS_ALL, bytes_model=isa.BYTES_CUDA)
+    assert st["qc_pairs"] == 7633612 * 2863
+    assert st["qc_hits"] == 20654396
+    assert st["mid_pass"] == 5373297
+    assert st["rules"] == len(rules) == 1075805
+
+
 @pytest.mark.parametrize("dead", [0x00, 0xFF])
 def test_c1_full_sweep_and_no_mid_agree(dev, dead):
     """MidCheck is only a filter and the projection only a reduction: the
