@@ -301,13 +301,14 @@ __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uin
       }
       st.p &= kPMask;
       const uint2 o = make_uint2(pack_lo(st), pack_hi(st));
-      if (lane < kFpVecs) {
-        fp[(first + q) * kFpVecs + lane] = o;
-        if (lane < 4) hv = qc_mix(o.x & lo_mask, o.y, lane);
-      } else {
-        hx = qc_mix(o.x, o.y, lane);
-        if (two_masks) hm = qc_mix(o.x & lo_mask, o.y, lane);
-      }
+      if (lane < kFpVecs) fp[(first + q) * kFpVecs + lane] = o;
+      // one mix per lane: QuickCheck vectors 0..3 under the compare mask,
+      // Mid vectors 8..31 exact (the masked Mid hash is a second mix, only
+      // when the masks differ)
+      const uint32_t h = qc_mix(lane < 4 ? o.x & lo_mask : o.x, o.y, lane);
+      hv = lane < 4 ? h : 0u;
+      hx = lane >= kFpVecs ? h : 0u;
+      if (two_masks && lane >= kFpVecs) hm = qc_mix(o.x & lo_mask, o.y, lane);
     }
     // XOR of the Mid hashes over lanes 8..31 (the others hold 0)
 #pragma unroll
@@ -631,11 +632,14 @@ __device__ __forceinline__ uint32_t plan_job(Job &j, int mode) {
   Plan pl = make_plan(j.tw, j.tl, j.cw, j.cl, j.pmask, mode, c_reads, c_dep);
   uint32_t total = 0;
   uint64_t states = 0;
-  for (int s = 0; s < pl.nsets; s++) {
+#pragma unroll
+  for (int s = 0; s < 8; s++) {  // unused entries are 0
     j.sets[s] = pl.sets[s];
     j.logsz[s] = pl.logsz[s];
-    total += pl.logsz[s] > kChunkLog ? (1u << (pl.logsz[s] - kChunkLog)) : 1u;
-    states += 1ull << pl.logsz[s];
+    if (s < pl.nsets) {
+      total += pl.logsz[s] > kChunkLog ? (1u << (pl.logsz[s] - kChunkLog)) : 1u;
+      states += 1ull << pl.logsz[s];
+    }
   }
   j.nsets = pl.nsets;
   j.pad[0] = (pl.nsets > 1 && states <= (1ull << kChunkLog)) ? 1 : 0;
