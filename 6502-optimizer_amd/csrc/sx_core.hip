@@ -71,6 +71,9 @@ constexpr int kEvalShards = 1024;       // shards of the evaluated-states counte
 __device__ __forceinline__ uint32_t uni(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+  return (uint64_t)uni((uint32_t)v) | ((uint64_t)uni((uint32_t)(v >> 32)) << 32);
+}
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (kWave - 1); }
 
@@ -278,7 +281,7 @@ __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uin
                                                       uint32_t *th, uint32_t lo_mask, uint8_t *bytes, uint8_t *cyc,
                                                       uint8_t *prune,
                                                       uint32_t *bad_op) {
-  const uint64_t first = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kFpPerWave;
+  const uint64_t first = ((uint64_t)blockIdx.x * 4 + uni(threadIdx.x >> 6)) * kFpPerWave;  // wave-uniform
   if (first >= n) return;
   const uint32_t lane = lane_id();
   const uint32_t cnt = (uint32_t)(first + kFpPerWave < n ? kFpPerWave : n - first);
@@ -1149,7 +1152,7 @@ __global__ __launch_bounds__(256) void k_pair_vectors(const uint32_t *tw, const 
                                                       const uint32_t *cw, const uint8_t *cl,
                                                       uint64_t n, uint32_t pm, uint8_t *qc,
                                                       uint8_t *mid, uint8_t *fdiff) {
-  uint64_t p = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  uint64_t p = (uint64_t)blockIdx.x * 4 + uni(threadIdx.x >> 6);  // wave-uniform
   if (p >= n) return;
   uint32_t lane = lane_id();
   uint32_t t[4], c[4];

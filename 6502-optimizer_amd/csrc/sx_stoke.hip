@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256) void k_stoke(ChainState *chains, uint32_t n, u
   __shared__ uint32_t xs[4][kMaxSeq + 1];
   __shared__ uint32_t ys[4][kMaxSeq + 1];
   __shared__ uint32_t ps[4][kMaxSeq + 1];  // last posted candidate (words, then length)
-  const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & (kWave - 1), wv = uni(threadIdx.x >> 6);  // wave-uniform
   const uint32_t c = blockIdx.x * 4 + wv;
   if (c >= n) return;
   const uint32_t ti = uni(c / cpt);

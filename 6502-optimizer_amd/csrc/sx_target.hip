@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void k_target_qc(uint32_t kp, uint64_t p0, uin
                                                    uint32_t pm_masked, uint64_t *__restrict__ surv,
                                                    uint32_t cap, uint32_t *__restrict__ count) {
   __shared__ uint64_t queue[4][kQueue];
-  const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & (kWave - 1), wv = uni(threadIdx.x >> 6);  // wave-uniform
   const uint32_t g = lane >> 3, v = lane & 7;
   const uint64_t pi = p0 + ((uint64_t)blockIdx.x * 4 + wv) * 8 + g;
   bool active = pi < p0 + np;
