@@ -1656,6 +1656,51 @@ int read_evaluated(sx_handle *h, uint64_t *sum) {
 
 extern "C" int sx_abi_version(void) { return SX_ABI_VERSION; }
 
+namespace {
+// 16 independent chains per lane, 32 v_xor_b32 per iteration (2 per chain):
+// the int32 VALU rate with nothing else in the way but the loop branch.
+__global__ __launch_bounds__(256) void k_int32_rate(uint32_t *out, uint32_t iters, uint32_t k) {
+  uint32_t a[16];
+#pragma unroll
+  for (int j = 0; j < 16; j++) a[j] = threadIdx.x * 16 + j;
+  for (uint32_t i = 0; i < iters; i++) {
+#pragma unroll
+    for (int j = 0; j < 16; j++) a[j] ^= k + j;
+#pragma unroll
+    for (int j = 0; j < 16; j++) a[j] ^= a[(j + 1) & 15];
+  }
+  uint32_t s = 0;
+#pragma unroll
+  for (int j = 0; j < 16; j++) s ^= a[j];
+  out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+}  // namespace
+
+extern "C" int sx_measure_int32_rate(sx_handle *h, double *ops_per_s) {
+  if (!h || !ops_per_s) return SX_EINVAL;
+  if (h->poisoned) return set_err(h, SX_EHIP, "handle unusable after a device timeout");
+  HIPCHK(h, hipSetDevice(h->device));
+  const unsigned blocks = (unsigned)h->num_cu * 32;
+  const uint32_t iters = 5000;
+  uint32_t *out;
+  ENSURE(B_SCRATCH, uint32_t, (size_t)blocks * 256, out);
+  double best = 0;
+  for (int rep = 0; rep < 4; rep++) {  // the first run warms up
+    HIPCHK(h, hipEventRecord(h->kev[0], h->stream));
+    hipLaunchKernelGGL(k_int32_rate, dim3(blocks), dim3(256), 0, h->stream, out, iters, 7u);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipEventRecord(h->kev[1], h->stream));
+    int rc = sync_wd(h, "int32 rate");
+    if (rc) return rc;
+    float ms = 0;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->kev[0], h->kev[1]));
+    const double r = (double)blocks * 256 * iters * 32 / (ms * 1e-3);
+    if (rep > 0 && r > best) best = r;
+  }
+  *ops_per_s = best;
+  return SX_OK;
+}
+
 extern "C" int sx_open(int device, sx_handle **out) {
   if (!out) return SX_EINVAL;
   *out = nullptr;

@@ -25,7 +25,8 @@ EXPORTS = ("sx_abi_version", "sx_open", "sx_close", "sx_last_error", "sx_enumera
            "sx_pruned_targets", "sx_check_pairs", "sx_flag_diff", "sx_exec_states", "sx_exh_plan",
            "sx_dep_table", "sx_stoke_run", "sx_mlp_param_count", "sx_mlp_create", "sx_mlp_destroy",
            "sx_mlp_get_params", "sx_mlp_set_params", "sx_mlp_loss_grad", "sx_mlp_train", "sx_mlp_predict",
-           "sx_proposal_weights", "sx_mlp_proposal", "sx_mfma_gemm_test", "sx_target_search")
+           "sx_proposal_weights", "sx_mlp_proposal", "sx_mfma_gemm_test", "sx_target_search",
+           "sx_measure_int32_rate")
 MLP_CLASSES = 2863   # instructions
 MLP_OUTPUTS = 2871   # instructions, then 8 context-position pointers
 MLP_CTX = 8
@@ -145,6 +146,7 @@ def lib() -> C.CDLL:
     L.sx_proposal_weights.argtypes = [f32p, C.c_uint64, C.c_double, u32p]
     L.sx_mlp_proposal.argtypes = [vp, u16p, C.c_uint32, C.c_double, u32p]
     L.sx_mfma_gemm_test.argtypes = [vp, u16p, u16p, C.c_int, C.c_int, C.c_int, f32p]
+    L.sx_measure_int32_rate.argtypes = [vp, C.POINTER(C.c_double)]
     L.sx_target_search.argtypes = [vp, u32p, C.c_int, C.c_int, C.c_uint32, C.c_int, C.POINTER(TargetResult),
                                    C.POINTER(C.c_uint64)]
     for name in EXPORTS:
@@ -381,6 +383,12 @@ class Device:
         c = np.zeros((M, N), np.float32)
         self._check(lib().sx_mfma_gemm_test(self._h, _p16(a), _p16(b), M, N, K, _pf(c)), "sx_mfma_gemm_test")
         return c
+
+    def measure_int32_rate(self) -> float:
+        """The device's int32 VALU rate, lane-operations per second."""
+        r = C.c_double(0)
+        self._check(lib().sx_measure_int32_rate(self._h, C.byref(r)), "sx_measure_int32_rate")
+        return r.value
 
 
 class Mlp:

@@ -282,6 +282,11 @@ int sx_proposal_weights(const float *probs, uint64_t n, double eps, uint32_t *ou
 /* C = A B through the MLP's MFMA tile kernel (A: M x K, B: K x N bf16
  * row-major, C: M x N fp32); a check of the kernel's operand layout. */
 int sx_mfma_gemm_test(sx_handle *h, const uint16_t *A, const uint16_t *B, int M, int N, int K, float *C);
+/* Measurement helper (no counterpart in the reference): the device's int32
+ * VALU rate in lane-operations per second, from a kernel of independent
+ * v_xor_b32 chains on every CU (best of three runs, about 30 ms).  The bench
+ * reports it beside the roofline peak. */
+int sx_measure_int32_rate(sx_handle *h, double *ops_per_s);
 
 #ifdef __cplusplus
 }

@@ -185,6 +185,13 @@ def test_full_length2_cuda_semantics_published_counts(dev):
     assert st["rules"] == len(rules) == 1075805
 
 
+def test_int32_rate_measurement(dev):
+    """The roofline's peak check: the int32 VALU rate the device sustains on
+    independent xor chains (the bench reports it beside the 78.6 T peak)."""
+    r = dev.measure_int32_rate()
+    assert 2e13 < r < 1.2e14, r
+
+
 @pytest.mark.parametrize("dead", [0x00, 0xFF])
 def test_c1_full_sweep_and_no_mid_agree(dev, dead):
     """MidCheck is only a filter and the projection only a reduction: the
