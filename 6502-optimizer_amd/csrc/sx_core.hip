@@ -62,6 +62,7 @@ static const uint8_t h_vec[32][8] = SX_MID_VECTORS;
 constexpr int kWave = 64;
 constexpr int kChunkLog = 14;           // states per exhaustive chunk
 constexpr uint32_t kRankNone = 0xFFFFFFFFu;
+constexpr uint32_t kQcBlk = 8;          // targets per QuickCheck hash block (one scalar load)
 constexpr int kQcQueue = 256;           // LDS survivor queue entries per wave
 constexpr int kEvalShards = 1024;       // shards of the evaluated-states counter
 
@@ -398,14 +399,14 @@ __device__ __forceinline__ void qc_item(const uint32_t *__restrict__ th, const u
   const uint32_t hc = qc_mix(clo[0] & mlo, chi[0], 0) ^ qc_mix(clo[1] & mlo, chi[1], 1) ^
                       qc_mix(clo[2] & mlo, chi[2], 2) ^ qc_mix(clo[3] & mlo, chi[3], 3);
   const uint64_t vmask = ballot(valid);
-  for (uint32_t t = t0; t < t1; t += 8) {
-    uint32_t nb = min(8u, t1 - t);
-    // one 32-byte scalar load of 8 consecutive target hashes (the plane has
-    // 8 entries of slack past nt, so no clamping is needed)
-    struct alignas(32) Th8 {
-      uint32_t h[8];
+  for (uint32_t t = t0; t < t1; t += kQcBlk) {
+    uint32_t nb = min(kQcBlk, t1 - t);
+    // one 32-byte scalar load of 8 consecutive target hashes (the plane
+    // has kQcBlk entries of slack past nt, so no clamping is needed)
+    struct alignas(32) ThBlk {
+      uint32_t h[kQcBlk];
     };
-    const Th8 blk = *reinterpret_cast<const Th8 *>(th + t);  // t is a multiple of 8
+    const ThBlk blk = *reinterpret_cast<const ThBlk *>(th + t);  // t is a multiple of kQcBlk
     // target t + u against the wave's candidates: hash filter, then all 8
     // vectors exactly under both masks (the differences ORed together first:
     // one 3-input VALU op per dword)
@@ -428,18 +429,20 @@ __device__ __forceinline__ void qc_item(const uint32_t *__restrict__ th, const u
       Q.qn += (uint32_t)__popcll(hm);
       if (Q.qn > kQcQueue - kWave) Q.flush(lane);
     };
-    if (nb == 8) {
-      // one ballot per compare, ORed: one branch per 8 targets when no lane
-      // matches any of them (the common case)
-      uint64_t m = 0;
+    if (nb == kQcBlk) {
+      // the minimum of the differences is 0 iff some target's hash matches:
+      // v_xor and a v_min3 tree on the vector unit, one ballot and one branch
+      // per block (the scalar unit, shared by the CU's four SIMDs, was the
+      // bound when each compare was its own scalar mask)
+      uint32_t dmin = hc ^ blk.h[0];
 #pragma unroll
-      for (int j = 0; j < 8; j++) m |= ballot(hc == blk.h[j]);
-      if ((m & vmask) == 0) continue;
+      for (uint32_t j = 1; j < kQcBlk; j++) dmin = min(dmin, hc ^ blk.h[j]);
+      if ((ballot(dmin == 0) & vmask) == 0) continue;
 #pragma unroll
-      for (uint32_t u = 0; u < 8; u++) check(u);
+      for (uint32_t u = 0; u < kQcBlk; u++) check(u);
     } else {
 #pragma unroll
-      for (uint32_t u = 0; u < 8; u++) {
+      for (uint32_t u = 0; u < kQcBlk; u++) {
         if (u >= nb) break;
         check(u);
       }
@@ -1932,7 +1935,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     ENSURE(B_TARGETS, uint32_t, nt * tl, tw);
     ENSURE(B_TFP, uint2, nt * kFpVecs, tfp);
     ENSURE(B_TMH, uint2, nt, tmh);
-    ENSURE(B_TV0, uint32_t, nt + 8, tv0);  // slack for the 8-target loads of QuickCheck
+    ENSURE(B_TV0, uint32_t, nt + kQcBlk, tv0);  // slack for the block loads of QuickCheck
     ENSURE(B_TBYTES, uint8_t, nt, tbytes);
     ENSURE(B_TCYC, uint8_t, nt, tcyc);
     ENSURE(B_TPRUNE, uint8_t, nt, tprune);
