@@ -726,6 +726,22 @@ __global__ void k_chunk_map(const uint64_t *offs, const uint32_t *chunks, uint32
   const uint32_t n = chunks[j];
   for (uint32_t k = 0; k < n; k++) map[o + k] = j | (n == 1 ? kMapSingle : 0u);
 }
+// The same map written per chunk: a binary search over the non-decreasing
+// offsets finds the job j with offs[j] <= c < offs[j + 1] (offs[nj] is tot;
+// jobs without chunks never satisfy it).  Used for sweeps where a few jobs
+// hold most chunks and one thread per job would write thousands of entries.
+__global__ void k_chunk_map_search(const uint64_t *offs, const uint32_t *chunks, uint32_t nj, uint64_t tot,
+                                   uint32_t *map) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < tot; c += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = nj;  // offs[lo] <= c < offs[hi] (offs[nj] = tot)
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (offs[mid] <= c) lo = mid;
+      else hi = mid;
+    }
+    map[c] = lo | (chunks[lo] == 1 ? kMapSingle : 0u);
+  }
+}
 
 // A job as 28 dwords, one per lane of a wave (lane l holds dword l): words
 // by readlane instead of dependent scalar loads.
@@ -1601,7 +1617,11 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, bo
   uint32_t *chunk_job = nullptr;
   if (tot <= (1ull << 26)) {
     ENSURE(B_CHUNKMAP, uint32_t, tot + 1, chunk_job);
-    hipLaunchKernelGGL(k_chunk_map, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, offs, chunks, nj, chunk_job);
+    if (tot <= 2ull * nj)
+      hipLaunchKernelGGL(k_chunk_map, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, offs, chunks, nj, chunk_job);
+    else
+      hipLaunchKernelGGL(k_chunk_map_search, dim3((unsigned)std::min<uint64_t>(blocks_for(tot, 256), 1u << 16)),
+                         dim3(256), 0, h->stream, offs, chunks, nj, tot, chunk_job);
     HIPCHK(h, hipGetLastError());
   }
   const uint64_t slice = 1ull << 20;  // chunks per launch (2^34 states)
