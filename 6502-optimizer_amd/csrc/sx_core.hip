@@ -261,22 +261,41 @@ __device__ __forceinline__ uint32_t qc_mix(uint32_t lo_m, uint32_t hi, uint32_t 
   return fmix32(lo_m + fmix32(hi ^ (v * 0x9E3779B1u)));
 }
 
+// XOR of x over each 16-lane row, in every lane of the row: DPP moves
+// (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror, row_mirror) at full VALU
+// rate, no LDS round trip.  quad_only stops after the two quad steps.
+template <bool QuadOnly = false>
+__device__ __forceinline__ uint32_t row_xor(uint32_t x) {
+  x ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
+  x ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);
+  if (QuadOnly) return x;
+  x ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x141, 0xF, 0xF, false);
+  x ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x140, 0xF, 0xF, false);
+  return x;
+}
+
 // Outputs of sequences on the 32 Mid vectors.  A wave takes kFpPerWave
-// sequences in turn (lane v < 32 runs vector v with the sequence's
-// wave-uniform opcodes); their words are fetched at once, one per lane, and
-// read back with readlane, so the sequences do not wait on dependent scalar
-// loads.  fp[i * 8 + v] = (lo, hi) on QuickCheck vector v < 8, with P masked
-// by 0xCF; mh[i] = (hash of the outputs on Mid vectors 8..31 with P masked by
-// 0xCF, the same with the lo words ANDed with lo_mask): MidCheck compares
-// these instead of 24 vectors (equal outputs give equal hashes, so the
-// filter drops no pair the exact check would keep); th[i] (when given) = the
-// QuickCheck filter hash under lo_mask, densely, so that eight consecutive
-// targets are one 32-byte scalar load in QuickCheck.  lo_mask is the
-// QuickCheck compare mask, 0x00FFFFFF | pm_masked << 24.  Lane k < kFpPerWave computes bytes,
-// cycles and the pruner flag of the wave's k-th sequence.
+// sequences, two per pass: lanes 0..31 run sequence q on vector (lane & 31)
+// and lanes 32..63 sequence q + 1.  The opcodes are wave-uniform (one scalar
+// dispatch per instruction); the immediates are per lane, so the two halves
+// share a pass whenever the sequences differ only in immediates (neighbours
+// in the enumeration order nearly always do), and otherwise take a pass each
+// with the other half idle.  The words are fetched at once, one per lane, and
+// read back with readlane.  fp[i * 8 + v] = (lo, hi) on QuickCheck vector
+// v < 8, with P masked by 0xCF; mh[i] = (hash of the outputs on Mid vectors
+// 8..31 with P masked by 0xCF, the same with the lo words ANDed with
+// lo_mask): MidCheck compares these instead of 24 vectors (equal outputs give
+// equal hashes, so the filter drops no pair the exact check would keep);
+// th[i] (when given) = the QuickCheck filter hash under lo_mask, densely, so
+// that eight consecutive targets are one 32-byte scalar load in QuickCheck.
+// lo_mask is the QuickCheck compare mask, 0x00FFFFFF | pm_masked << 24.  The
+// hashes are XOR-reduced per 16-lane row by DPP, gathered into lane q of the
+// wave and stored once per wave.  Lane k < kFpPerWave computes bytes, cycles
+// and (when prune is given) the pruner flag of the wave's k-th sequence.
 constexpr uint32_t kFpPerWave = 8;
 constexpr uint32_t kFpVecs = 8;  // vectors kept per sequence: the QuickCheck ones
 static_assert(kFpPerWave * 4 <= kWave, "sequence words must fit one per lane");
+static_assert(kFpPerWave % 2 == 0 && kFpPerWave <= kWave, "sequences go two per pass");
 __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uint32_t len,
                                                       uint64_t n, int bytes_model, uint2 *fp, uint2 *mh,
                                                       uint32_t *th, uint32_t lo_mask, uint8_t *bytes, uint8_t *cyc,
@@ -284,52 +303,81 @@ __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uin
                                                       uint32_t *bad_op) {
   const uint64_t first = ((uint64_t)blockIdx.x * 4 + uni(threadIdx.x >> 6)) * kFpPerWave;  // wave-uniform
   if (first >= n) return;
-  const uint32_t lane = lane_id();
+  const uint32_t lane = lane_id(), v = lane & 31, half = lane >> 5;
   const uint32_t cnt = (uint32_t)(first + kFpPerWave < n ? kFpPerWave : n - first);
   const uint32_t wv = lane < cnt * len ? words[first * len + lane] : 0u;  // len <= 4
-  bool bad_any = false;
+  const State base = load_vec(v);
   const bool two_masks = (lo_mask >> 24) != kPMask;  // else the masked hash is the exact one
-  for (uint32_t q = 0; q < cnt; q++) {
-    bool bad = false;
-    for (uint32_t k = 0; k < len; k++) bad |= (__builtin_amdgcn_readlane(wv, q * len + k) & 0xFFFFu) >= OP_COUNT;
-    if (bad) {
-      bad_any = true;
-      continue;
+  uint32_t bad_seqs = 0;                              // bit q: sequence q has an unknown opcode
+  uint32_t g_hx = 0, g_hm = 0, g_th = 0;              // lane q: the hashes of sequence q
+  for (uint32_t q = 0; q < cnt; q += 2) {
+    const bool pair = q + 1 < cnt;
+    bool bad_a = false, bad_b = false, same = pair;
+    for (uint32_t k = 0; k < len; k++) {
+      const uint32_t oa = __builtin_amdgcn_readlane(wv, q * len + k) & 0xFFFFu;
+      const uint32_t ob = pair ? __builtin_amdgcn_readlane(wv, (q + 1) * len + k) & 0xFFFFu : oa;
+      bad_a |= oa >= OP_COUNT;
+      bad_b |= ob >= OP_COUNT;
+      same &= oa == ob;
     }
-    uint32_t hv = 0, hx = 0, hm = 0;
-    if (lane < 32) {
-      State st = load_vec(lane);
-      for (uint32_t k = 0; k < len; k++) {
-        const uint32_t w = __builtin_amdgcn_readlane(wv, q * len + k);
-        step(st, w & 0xFFFFu, w >> 16);
+    bad_b &= pair;
+    bad_seqs |= (bad_a ? 1u << q : 0u) | (bad_b ? 2u << q : 0u);
+    // pass 0 runs a (and b when same), pass 1 runs b alone
+    const uint32_t npass = pair && !same ? 2u : 1u;
+    for (uint32_t ps = 0; ps < npass; ps++) {
+      const uint32_t s_lo = same ? q : q + ps;  // sequence of half 0 (and of half 1 when same)
+      const uint32_t sq = same ? q + half : q + ps;
+      const bool run = (same ? (half == 0 || pair) : half == ps) && !((bad_seqs >> sq) & 1u);
+      uint32_t hv = 0, hx = 0, hm = 0;
+      if (run) {
+        State st = base;
+        for (uint32_t k = 0; k < len; k++) {
+          const uint32_t w = __builtin_amdgcn_readlane(wv, s_lo * len + k);
+          const uint32_t wb = same && pair ? __builtin_amdgcn_readlane(wv, (s_lo + 1) * len + k) : w;
+          step(st, w & 0xFFFFu, (half ? wb : w) >> 16);
+        }
+        st.p &= kPMask;
+        const uint2 o = make_uint2(pack_lo(st), pack_hi(st));
+        if (v < kFpVecs) fp[(first + sq) * kFpVecs + v] = o;
+        // one mix per lane: QuickCheck vectors 0..3 under the compare mask,
+        // Mid vectors 8..31 exact (the masked Mid hash is a second mix, only
+        // when the masks differ)
+        const uint32_t h = qc_mix(v < 4 ? o.x & lo_mask : o.x, o.y, v);
+        hv = v < 4 ? h : 0u;
+        hx = v >= kFpVecs ? h : 0u;
+        if (two_masks && v >= kFpVecs) hm = qc_mix(o.x & lo_mask, o.y, v);
       }
-      st.p &= kPMask;
-      const uint2 o = make_uint2(pack_lo(st), pack_hi(st));
-      if (lane < kFpVecs) fp[(first + q) * kFpVecs + lane] = o;
-      // one mix per lane: QuickCheck vectors 0..3 under the compare mask,
-      // Mid vectors 8..31 exact (the masked Mid hash is a second mix, only
-      // when the masks differ)
-      const uint32_t h = qc_mix(lane < 4 ? o.x & lo_mask : o.x, o.y, lane);
-      hv = lane < 4 ? h : 0u;
-      hx = lane >= kFpVecs ? h : 0u;
-      if (two_masks && lane >= kFpVecs) hm = qc_mix(o.x & lo_mask, o.y, lane);
-    }
-    // XOR of the Mid hashes over lanes 8..31 (the others hold 0)
-#pragma unroll
-    for (int d = 1; d < 32; d <<= 1) hx ^= __shfl_xor(hx, d, 32);
-    if (two_masks) {
-#pragma unroll
-      for (int d = 1; d < 32; d <<= 1) hm ^= __shfl_xor(hm, d, 32);
-    } else {
-      hm = hx;
-    }
-    if (lane == 0) mh[first + q] = make_uint2(hx, hm);
-    if (th) {
-      const uint32_t h = __builtin_amdgcn_readlane(hv, 0) ^ __builtin_amdgcn_readlane(hv, 1) ^
-                         __builtin_amdgcn_readlane(hv, 2) ^ __builtin_amdgcn_readlane(hv, 3);
-      if (lane == 0) th[first + q] = h;
+      // full wave again: reduce per row, rows 0+1 are half 0, rows 2+3 half 1
+      hv = row_xor<true>(hv);
+      hx = row_xor(hx);
+      const uint32_t tha = __builtin_amdgcn_readlane(hv, 0), thb = __builtin_amdgcn_readlane(hv, 32);
+      const uint32_t hxa = __builtin_amdgcn_readlane(hx, 0) ^ __builtin_amdgcn_readlane(hx, 16);
+      const uint32_t hxb = __builtin_amdgcn_readlane(hx, 32) ^ __builtin_amdgcn_readlane(hx, 48);
+      uint32_t hma = hxa, hmb = hxb;
+      if (two_masks) {
+        hm = row_xor(hm);
+        hma = __builtin_amdgcn_readlane(hm, 0) ^ __builtin_amdgcn_readlane(hm, 16);
+        hmb = __builtin_amdgcn_readlane(hm, 32) ^ __builtin_amdgcn_readlane(hm, 48);
+      }
+      if (same || ps == 0) {  // half 0 ran sequence q
+        const bool mine = lane == q;
+        g_th = mine ? tha : g_th;
+        g_hx = mine ? hxa : g_hx;
+        g_hm = mine ? hma : g_hm;
+      }
+      if (pair && (same || ps == 1)) {  // half 1 ran sequence q + 1
+        const bool mine = lane == q + 1;
+        g_th = mine ? thb : g_th;
+        g_hx = mine ? hxb : g_hx;
+        g_hm = mine ? hmb : g_hm;
+      }
     }
   }
+  if (lane < cnt && !((bad_seqs >> lane) & 1u)) {
+    mh[first + lane] = make_uint2(g_hx, g_hm);
+    if (th) th[first + lane] = g_th;
+  }
+  const bool bad_any = bad_seqs != 0;
   if (bad_any && lane == 0) atomicOr(bad_op, 1u);
   const uint64_t seq = first + lane;
   if (lane < cnt) {
@@ -347,7 +395,7 @@ __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uin
       }
       bytes[seq] = (uint8_t)b;
       cyc[seq] = (uint8_t)c;
-      prune[seq] = should_prune(w, (int)len, c_reads, c_writes) ? 1 : 0;
+      if (prune) prune[seq] = should_prune(w, (int)len, c_reads, c_writes) ? 1 : 0;
     }
   }
 }
@@ -1234,6 +1282,12 @@ __global__ void k_iota(uint32_t *p, uint64_t n) {
   if (i < n) p[i] = (uint32_t)i;
 }
 
+// The 2,863 instruction words in Go order, one word each.
+__global__ void k_inst_words(uint32_t *w) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (uint32_t)kNumInsts) w[i] = go_inst(i);
+}
+
 __global__ void k_cand_words(uint32_t *w, uint64_t n) {
   // 2 words per slot: single instructions then Go-ordered pairs
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1309,7 +1363,7 @@ struct sx_handle {
 namespace {
 
 enum BufId {
-  B_TARGETS, B_TFP, B_TV0, B_TBYTES, B_TCYC, B_TPRUNE, B_CWORDS, B_CFP, B_CBYTES, B_CCYC, B_CPRUNE,
+  B_TARGETS, B_TFP, B_TV0, B_TBYTES, B_TCYC, B_CWORDS, B_CFP, B_CBYTES, B_CCYC, B_CPRUNE,
   B_RANK, B_SURV, B_MID, B_COUNTERS, B_CUR, B_REJ, B_DONE, B_JOBS, B_CHUNKS, B_OFFS, B_SUMS,
   B_FAIL, B_RULES, B_SCRATCH, B_GEN_COUNTS, B_GEN_OFFS, B_BADOP, B_P_TW, B_P_TL, B_P_CW, B_P_CL,
   B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR, B_HEADS, B_RULEFLAG, B_RULEOFFS, B_RULES2,
@@ -1399,7 +1453,11 @@ int sync_wd(sx_handle *h, const char *stage) {
       h->poisoned = true;
       return set_err(h, SX_EHIP, "%s: device work did not finish within %.0f s", stage, limit);
     }
-    std::this_thread::sleep_for(std::chrono::microseconds(dt < 0.01 ? 20 : 500));
+    // Most waits here are a kernel or two (tens to hundreds of microseconds):
+    // spin for the first 2 ms (a timed sleep oversleeps by the timer slack,
+    // about 50 us, on every wait), then poll at 50 us and later at 0.5 ms.
+    if (dt < 0.002) std::this_thread::yield();
+    else std::this_thread::sleep_for(std::chrono::microseconds(dt < 0.01 ? 50 : 500));
   }
 }
 
@@ -1873,12 +1931,13 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   ENSURE(B_CCYC, uint8_t, nc, ccyc);
   ENSURE(B_CPRUNE, uint8_t, nc, cprune);
   ENSURE(B_BADOP, uint32_t, 1, badop);
-  hipEvent_t e0, e1, e2, e3, e4;
+  hipEvent_t e0, e1, e2, e3, e4, es;
   HIPCHK(h, hipEventCreate(&e0));
   HIPCHK(h, hipEventCreate(&e1));
   HIPCHK(h, hipEventCreate(&e2));
   HIPCHK(h, hipEventCreate(&e3));
   HIPCHK(h, hipEventCreate(&e4));
+  HIPCHK(h, hipEventCreate(&es));
   HIPCHK(h, hipEventRecord(e0, h->stream));
   HIPCHK(h, hipMemsetAsync(badop, 0, 4, h->stream));
   hipLaunchKernelGGL(k_cand_words, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cwords, nc);
@@ -1887,9 +1946,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   {
     uint32_t *w1;
     ENSURE(B_SCRATCH, uint32_t, kNumInsts, w1);
-    std::vector<uint32_t> tab(kNumInsts);
-    for (int i = 0; i < kNumInsts; i++) tab[i] = go_inst(i);
-    HIPCHK(h, hipMemcpyAsync(w1, tab.data(), kNumInsts * 4, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_inst_words, dim3(blocks_for(kNumInsts, 256)), dim3(256), 0, h->stream, w1);
     hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(kNumInsts, 4 * kFpPerWave)), dim3(256), 0, h->stream, w1, 1u,
                        (uint64_t)kNumInsts, cfg->bytes_model, cfp, cmh, nullptr, lo_mask, cbytes, ccyc, cprune,
                        badop);
@@ -1901,7 +1958,6 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
                          cbytes + kNumInsts, ccyc + kNumInsts, cprune + kNumInsts, badop);
     }
     HIPCHK(h, hipGetLastError());
-    SYNC_WD("device sync");  // w1 is host-staged
   }
 
   // ---- segments of targets
@@ -1920,9 +1976,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   }
   // pm_masked (the masked compare's P mask) is defined with the candidates
   float ms_qc = 0, ms_mid = 0, ms_exh = 0, ms_setup = 0;
-  HIPCHK(h, hipEventRecord(e1, h->stream));
-  SYNC_WD("setup");
-  HIPCHK(h, hipEventElapsedTime(&ms_setup, e0, e1));
+  HIPCHK(h, hipEventRecord(es, h->stream));  // end of the setup; read after the last sync
 
   for (uint64_t s0 = 0; s0 < nt_all; s0 += seg) {
     uint64_t nt = std::min<uint64_t>(seg, nt_all - s0);
@@ -1951,14 +2005,13 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     uint32_t *tw;
     uint2 *tfp, *tmh;
     uint32_t *tv0;  // the targets' QuickCheck filter hashes
-    uint8_t *tbytes, *tcyc, *tprune;
+    uint8_t *tbytes, *tcyc;
     ENSURE(B_TARGETS, uint32_t, nt * tl, tw);
     ENSURE(B_TFP, uint2, nt * kFpVecs, tfp);
     ENSURE(B_TMH, uint2, nt, tmh);
     ENSURE(B_TV0, uint32_t, nt + kQcBlk, tv0);  // slack for the block loads of QuickCheck
     ENSURE(B_TBYTES, uint8_t, nt, tbytes);
     ENSURE(B_TCYC, uint8_t, nt, tcyc);
-    ENSURE(B_TPRUNE, uint8_t, nt, tprune);
     HIPCHK(h, hipEventRecord(e1, h->stream));
     if (cfg->targets) {
       HIPCHK(h, hipMemcpyAsync(tw, cfg->targets + s0 * tl, nt * tl * 4, hipMemcpyHostToDevice, h->stream));
@@ -1968,7 +2021,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       HIPCHK(h, hipMemcpyAsync(tw, d_tsrc + s0 * 2, nt * 8, hipMemcpyDeviceToDevice, h->stream));
     }
     hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(nt, 4 * kFpPerWave)), dim3(256), 0, h->stream, tw, (uint32_t)tl,
-                       nt, cfg->bytes_model, tfp, tmh, tv0, lo_mask, tbytes, tcyc, tprune, badop);
+                       nt, cfg->bytes_model, tfp, tmh, tv0, lo_mask, tbytes, tcyc, nullptr, badop);
     HIPCHK(h, hipGetLastError());
 
     // ---- QuickCheck, re-run with a larger buffer on overflow (no hit is dropped)
@@ -2223,11 +2276,14 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     int rc = read_evaluated(h, &st.exh_states);
     if (rc) return rc;
   }
+  HIPCHK(h, hipEventSynchronize(es));  // long complete unless there were no targets
+  HIPCHK(h, hipEventElapsedTime(&ms_setup, e0, es));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipEventDestroy(e2);
   (void)hipEventDestroy(e3);
   (void)hipEventDestroy(e4);
+  (void)hipEventDestroy(es);
   if (bad) return set_err(h, SX_EOPCODE, "opcode out of range in a target");
   // first hit: segments in order, slots compacted in (target, pass) order, so
   // the rules are already sorted; all hits: sort on the host
