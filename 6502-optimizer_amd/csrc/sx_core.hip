@@ -280,22 +280,27 @@ __device__ __forceinline__ uint32_t row_xor(uint32_t x) {
 // dispatch per instruction); the immediates are per lane, so the two halves
 // share a pass whenever the sequences differ only in immediates (neighbours
 // in the enumeration order nearly always do), and otherwise take a pass each
-// with the other half idle.  The words are fetched at once, one per lane, and
-// read back with readlane.  fp[i * 8 + v] = (lo, hi) on QuickCheck vector
-// v < 8, with P masked by 0xCF; mh[i] = (hash of the outputs on Mid vectors
-// 8..31 with P masked by 0xCF, the same with the lo words ANDed with
-// lo_mask): MidCheck compares these instead of 24 vectors (equal outputs give
-// equal hashes, so the filter drops no pair the exact check would keep);
-// th[i] (when given) = the QuickCheck filter hash under lo_mask, densely, so
-// that eight consecutive targets are one 32-byte scalar load in QuickCheck.
+// with the other half idle.  When every sequence of the wave has the same
+// words but the last (runs of targets with one first instruction), that
+// prefix runs once per wave and the passes run only the last instruction.
+// The words are fetched at once, one per lane; the per-sequence checks (an
+// unknown opcode, same opcodes as the next sequence, same prefix) are lane
+// work and ballots, since the scalar unit, shared by the CU's four SIMDs, is
+// this kernel's bound.  fp[i * 8 + v] = (lo, hi) on QuickCheck vector v < 8,
+// with P masked by 0xCF; mh[i] = (hash of the outputs on Mid vectors 8..31
+// with P masked by 0xCF, the same with the lo words ANDed with lo_mask):
+// MidCheck compares these instead of 24 vectors (equal outputs give equal
+// hashes, so the filter drops no pair the exact check would keep); th[i]
+// (when given) = the QuickCheck filter hash under lo_mask, densely, so that
+// eight consecutive targets are one 32-byte scalar load in QuickCheck.
 // lo_mask is the QuickCheck compare mask, 0x00FFFFFF | pm_masked << 24.  The
 // hashes are XOR-reduced per 16-lane row by DPP, gathered into lane q of the
 // wave and stored once per wave.  Lane k < kFpPerWave computes bytes, cycles
 // and (when prune is given) the pruner flag of the wave's k-th sequence.
-constexpr uint32_t kFpPerWave = 8;
+constexpr uint32_t kFpPerWave = 16;
 constexpr uint32_t kFpVecs = 8;  // vectors kept per sequence: the QuickCheck ones
-static_assert(kFpPerWave * 4 <= kWave, "sequence words must fit one per lane");
-static_assert(kFpPerWave % 2 == 0 && kFpPerWave <= kWave, "sequences go two per pass");
+static_assert(kFpPerWave * 4 <= kWave, "sequence words (len <= 4) must fit one per lane");
+static_assert(kFpPerWave % 2 == 0 && kFpPerWave <= 32, "sequences go two per pass; masks are 32-bit");
 __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uint32_t len,
                                                       uint64_t n, int bytes_model, uint2 *fp, uint2 *mh,
                                                       uint32_t *th, uint32_t lo_mask, uint8_t *bytes, uint8_t *cyc,
@@ -306,34 +311,51 @@ __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uin
   const uint32_t lane = lane_id(), v = lane & 31, half = lane >> 5;
   const uint32_t cnt = (uint32_t)(first + kFpPerWave < n ? kFpPerWave : n - first);
   const uint32_t wv = lane < cnt * len ? words[first * len + lane] : 0u;  // len <= 4
-  const State base = load_vec(v);
+  // lane q < cnt: flags of sequence q (the other lanes read sequence 0 and
+  // are masked out of the ballots)
+  const uint32_t cmask = cnt >= 32 ? 0xFFFFFFFFu : (1u << cnt) - 1u;
+  uint32_t bad_seqs, same_next;
+  bool share;
+  {
+    const uint32_t q = lane < cnt ? lane : 0u, qn = q + 1 < cnt ? q + 1 : q;
+    bool l_bad = false, l_same = q + 1 < cnt, l_pre = true;
+    for (uint32_t k = 0; k < len; k++) {
+      const uint32_t w = (uint32_t)__shfl((int)wv, (int)(q * len + k));
+      const uint32_t wn = (uint32_t)__shfl((int)wv, (int)(qn * len + k));
+      l_bad |= (w & 0xFFFFu) >= OP_COUNT;
+      l_same &= (w & 0xFFFFu) == (wn & 0xFFFFu);
+      if (k + 1 < len) l_pre &= w == __builtin_amdgcn_readlane(wv, k);
+    }
+    bad_seqs = (uint32_t)ballot(l_bad) & cmask;
+    same_next = (uint32_t)ballot(l_same) & cmask;
+    share = len > 1 && ((uint32_t)ballot(l_pre) & cmask) == cmask;
+  }
   const bool two_masks = (lo_mask >> 24) != kPMask;  // else the masked hash is the exact one
-  uint32_t bad_seqs = 0;                              // bit q: sequence q has an unknown opcode
-  uint32_t g_hx = 0, g_hm = 0, g_th = 0;              // lane q: the hashes of sequence q
+  State base = load_vec(v);
+  uint32_t k0 = 0;  // first word the passes run
+  if (share) {
+    for (uint32_t k = 0; k + 1 < len; k++) {
+      const uint32_t w = __builtin_amdgcn_readlane(wv, k);
+      step(base, w & 0xFFFFu, w >> 16);  // an unknown opcode marks every sequence bad
+    }
+    k0 = len - 1;
+  }
+  uint32_t g_hx = 0, g_hm = 0, g_th = 0;  // lane q: the hashes of sequence q
   for (uint32_t q = 0; q < cnt; q += 2) {
     const bool pair = q + 1 < cnt;
-    bool bad_a = false, bad_b = false, same = pair;
-    for (uint32_t k = 0; k < len; k++) {
-      const uint32_t oa = __builtin_amdgcn_readlane(wv, q * len + k) & 0xFFFFu;
-      const uint32_t ob = pair ? __builtin_amdgcn_readlane(wv, (q + 1) * len + k) & 0xFFFFu : oa;
-      bad_a |= oa >= OP_COUNT;
-      bad_b |= ob >= OP_COUNT;
-      same &= oa == ob;
-    }
-    bad_b &= pair;
-    bad_seqs |= (bad_a ? 1u << q : 0u) | (bad_b ? 2u << q : 0u);
-    // pass 0 runs a (and b when same), pass 1 runs b alone
+    const bool same = (same_next >> q) & 1u;  // implies pair
+    // pass 0 runs q (and q + 1 when same), pass 1 runs q + 1 alone
     const uint32_t npass = pair && !same ? 2u : 1u;
     for (uint32_t ps = 0; ps < npass; ps++) {
       const uint32_t s_lo = same ? q : q + ps;  // sequence of half 0 (and of half 1 when same)
       const uint32_t sq = same ? q + half : q + ps;
-      const bool run = (same ? (half == 0 || pair) : half == ps) && !((bad_seqs >> sq) & 1u);
+      const bool run = (same ? true : (pair ? half == ps : half == 0)) && !((bad_seqs >> sq) & 1u);
       uint32_t hv = 0, hx = 0, hm = 0;
       if (run) {
         State st = base;
-        for (uint32_t k = 0; k < len; k++) {
+        for (uint32_t k = k0; k < len; k++) {
           const uint32_t w = __builtin_amdgcn_readlane(wv, s_lo * len + k);
-          const uint32_t wb = same && pair ? __builtin_amdgcn_readlane(wv, (s_lo + 1) * len + k) : w;
+          const uint32_t wb = same ? __builtin_amdgcn_readlane(wv, (s_lo + 1) * len + k) : w;
           step(st, w & 0xFFFFu, (half ? wb : w) >> 16);
         }
         st.p &= kPMask;
