@@ -724,12 +724,15 @@ __device__ __forceinline__ uint32_t plan_job(Job &j, int mode) {
 
 // Jobs of one round: the current candidate of every open (target, pass).
 // In first-hit mode it also counts the survivors not yet tested (*nrest),
-// which decides when the tail runs as one batch.
+// which decides when the tail runs as one batch.  plan: plan the jobs here
+// (all hits, where every lane takes a job); in first-hit mode one survivor in
+// several takes one, so the planner would run on mostly idle waves and the
+// jobs are planned afterwards, densely, by k_exh_plan.
 __global__ __launch_bounds__(kBlk) void k_round_jobs(const uint2 *mid, uint32_t nm, const uint32_t *rank,
                                                      const uint32_t *cur, const uint32_t *rej, const uint8_t *done,
                                                      const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
                                                      int hits_all, Job *jobs, uint32_t job_cap, uint32_t *njobs,
-                                                     uint32_t *nrest, int plan_mode, uint32_t *chunks,
+                                                     uint32_t *nrest, int plan, int plan_mode, uint32_t *chunks,
                                                      uint32_t *fail) {
   __shared__ uint32_t s_rest[kBlkWaves], s_take[kBlkWaves];
   __shared__ uint32_t s_base;
@@ -763,8 +766,10 @@ __global__ __launch_bounds__(kBlk) void k_round_jobs(const uint2 *mid, uint32_t 
     fill_job(j, targets, tl, nullptr, 0, e.x, slot, pass, pm);
     uint32_t o = s_base + pre + mbcnt(m);
     if (o < job_cap) {
-      chunks[o] = plan_job(j, plan_mode);  // planned here: no separate plan pass
-      fail[o] = 0;
+      if (plan) {
+        chunks[o] = plan_job(j, plan_mode);
+        fail[o] = 0;
+      }
       jobs[o] = j;
     }
   }
@@ -2176,7 +2181,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
                            cur, rej, done);
       hipLaunchKernelGGL(k_round_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, nmid, rank,
                          cur, rej, done, tw, (uint32_t)tl, pm_masked, cfg->hits, jobs, job_cap, njobs_d, nrest_d,
-                         cfg->exh_mode, jchunks, jfail);
+                         cfg->hits ? 1 : 0, cfg->exh_mode, jchunks, jfail);
       HIPCHK(h, hipGetLastError());
       uint32_t cnt2[2] = {0, 0};
       HIPCHK(h, hipMemcpyAsync(cnt2, njobs_d, 8, hipMemcpyDeviceToHost, h->stream));
@@ -2198,7 +2203,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       st.exh_rounds++;
       st.exh_pairs += nj;
       uint64_t chunks = 0;
-      int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, true);
+      int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, cfg->hits != 0);  // first hit: planned there
       if (rc) return rc;
       hipLaunchKernelGGL(k_round_resolve, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream,
                          (const Job *)h->bufs[B_JOBS].p, nj, (const uint32_t *)h->bufs[B_FAIL].p, rank, cur, rej,
