@@ -722,6 +722,66 @@ __device__ __forceinline__ uint32_t plan_job(Job &j, int mode) {
   return j.pad[0] ? 1u : total;
 }
 
+// Plans of length-2 targets against single-instruction candidates, by
+// opcode.  The planner reads only the opcodes (an immediate does not change
+// which state bytes an instruction reads or writes), so an enumeration of
+// length-2 targets tabulates the 58^3 opcode triples under both compare masks
+// (390,224 plans, 9.4 MB) and its job kernels look plans up instead of
+// running the planner per job.  Entry i = ((t0 * 58 + t1) * 58 + c) * 2 + m,
+// m = 0 for the 0xCF mask and 1 for the relaxed one.
+struct PlanEntry {
+  uint32_t sets_lo, sets_hi, log_lo, log_hi, chunks, meta;  // meta = nsets | single-chunk << 8
+};
+constexpr uint32_t kPlanTab = (uint32_t)OP_COUNT * OP_COUNT * OP_COUNT * 2;
+__global__ void k_plan_table(int mode, uint32_t pm_masked, PlanEntry *tab) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kPlanTab) return;
+  const uint32_t trip = i >> 1, c = trip % OP_COUNT, t1 = (trip / OP_COUNT) % OP_COUNT,
+                 t0 = trip / (OP_COUNT * OP_COUNT);
+  Job j;
+  for (int k = 0; k < kJobWords; k++) {
+    j.tw[k] = 0;
+    j.cw[k] = 0;
+  }
+  j.tw[0] = t0;  // immediate 0: the plan does not depend on it
+  j.tw[1] = t1;
+  j.cw[0] = c;
+  j.tl = 2;
+  j.cl = 1;
+  j.pmask = (uint8_t)((i & 1) ? pm_masked : kPMask);
+  PlanEntry e;
+  e.chunks = plan_job(j, mode);
+  e.sets_lo = j.sets[0] | j.sets[1] << 8 | j.sets[2] << 16 | (uint32_t)j.sets[3] << 24;
+  e.sets_hi = j.sets[4] | j.sets[5] << 8 | j.sets[6] << 16 | (uint32_t)j.sets[7] << 24;
+  e.log_lo = j.logsz[0] | j.logsz[1] << 8 | j.logsz[2] << 16 | (uint32_t)j.logsz[3] << 24;
+  e.log_hi = j.logsz[4] | j.logsz[5] << 8 | j.logsz[6] << 16 | (uint32_t)j.logsz[7] << 24;
+  e.meta = j.nsets | (uint32_t)j.pad[0] << 8;
+  tab[i] = e;
+}
+// plan_job through the table when it covers the job (tab given, a length-2
+// target, a single-instruction candidate, opcodes in range, one of the two
+// masks); otherwise the planner.
+__device__ __forceinline__ uint32_t plan_job_tab(Job &j, int mode, const PlanEntry *tab, uint32_t pm_masked) {
+  if (tab && j.tl == 2 && j.cl == 1) {
+    const uint32_t t0 = j.tw[0] & 0xFFFFu, t1 = j.tw[1] & 0xFFFFu, c = j.cw[0] & 0xFFFFu;
+    const bool m0 = j.pmask == kPMask, m1 = j.pmask == pm_masked;
+    if (t0 < (uint32_t)OP_COUNT && t1 < (uint32_t)OP_COUNT && c < (uint32_t)OP_COUNT && (m0 || m1)) {
+      const PlanEntry e = tab[((t0 * OP_COUNT + t1) * OP_COUNT + c) * 2 + (m0 ? 0u : 1u)];
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        j.sets[s] = (uint8_t)(e.sets_lo >> (8 * s));
+        j.sets[s + 4] = (uint8_t)(e.sets_hi >> (8 * s));
+        j.logsz[s] = (uint8_t)(e.log_lo >> (8 * s));
+        j.logsz[s + 4] = (uint8_t)(e.log_hi >> (8 * s));
+      }
+      j.nsets = (uint8_t)e.meta;
+      j.pad[0] = (uint8_t)(e.meta >> 8);
+      return e.chunks;
+    }
+  }
+  return plan_job(j, mode);
+}
+
 // Jobs of one round: the current candidate of every open (target, pass).
 // In first-hit mode it also counts the survivors not yet tested (*nrest),
 // which decides when the tail runs as one batch.  plan: plan the jobs here
@@ -732,8 +792,8 @@ __global__ __launch_bounds__(kBlk) void k_round_jobs(const uint2 *mid, uint32_t 
                                                      const uint32_t *cur, const uint32_t *rej, const uint8_t *done,
                                                      const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
                                                      int hits_all, Job *jobs, uint32_t job_cap, uint32_t *njobs,
-                                                     uint32_t *nrest, int plan, int plan_mode, uint32_t *chunks,
-                                                     uint32_t *fail) {
+                                                     uint32_t *nrest, int plan, int plan_mode,
+                                                     const PlanEntry *ptab, uint32_t *chunks, uint32_t *fail) {
   __shared__ uint32_t s_rest[kBlkWaves], s_take[kBlkWaves];
   __shared__ uint32_t s_base;
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
@@ -767,7 +827,7 @@ __global__ __launch_bounds__(kBlk) void k_round_jobs(const uint2 *mid, uint32_t 
     uint32_t o = s_base + pre + mbcnt(m);
     if (o < job_cap) {
       if (plan) {
-        chunks[o] = plan_job(j, plan_mode);
+        chunks[o] = plan_job_tab(j, plan_mode, ptab, pm_masked);
         fail[o] = 0;
       }
       jobs[o] = j;
@@ -776,12 +836,12 @@ __global__ __launch_bounds__(kBlk) void k_round_jobs(const uint2 *mid, uint32_t 
 }
 
 // Plan each job's sweep and count its chunks.
-__global__ __launch_bounds__(256) void k_exh_plan(Job *jobs, uint32_t nj, int mode,
-                                                  uint32_t *chunks, uint32_t *fail) {
+__global__ __launch_bounds__(256) void k_exh_plan(Job *jobs, uint32_t nj, int mode, const PlanEntry *ptab,
+                                                  uint32_t pm_masked, uint32_t *chunks, uint32_t *fail) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nj) return;
   Job j = jobs[i];
-  chunks[i] = plan_job(j, mode);
+  chunks[i] = plan_job_tab(j, mode, ptab, pm_masked);
   jobs[i] = j;
   fail[i] = 0;
 }
@@ -1149,7 +1209,7 @@ __global__ __launch_bounds__(kBlk) void k_bulk_jobs(const uint2 *mid, uint32_t n
                                                     const uint32_t *rej, const uint8_t *done,
                                                     const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
                                                     Job *jobs, uint32_t job_cap, uint32_t *njobs, int plan_mode,
-                                                    uint32_t *chunks, uint32_t *fail) {
+                                                    const PlanEntry *ptab, uint32_t *chunks, uint32_t *fail) {
   __shared__ uint32_t s_take[kBlkWaves];
   __shared__ uint32_t s_base;
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
@@ -1172,7 +1232,7 @@ __global__ __launch_bounds__(kBlk) void k_bulk_jobs(const uint2 *mid, uint32_t n
     fill_job(j, targets, tl, nullptr, 0, e.x, slot, pass, pass == 2 ? pm_masked : kPMask);
     uint32_t o = s_base + pre + mbcnt(m);
     if (o < job_cap) {
-      chunks[o] = plan_job(j, plan_mode);
+      chunks[o] = plan_job_tab(j, plan_mode, ptab, pm_masked);
       fail[o] = 0;
       jobs[o] = j;
     }
@@ -1394,7 +1454,7 @@ enum BufId {
   B_RANK, B_SURV, B_MID, B_COUNTERS, B_CUR, B_REJ, B_DONE, B_JOBS, B_CHUNKS, B_OFFS, B_SUMS,
   B_FAIL, B_RULES, B_SCRATCH, B_GEN_COUNTS, B_GEN_OFFS, B_BADOP, B_P_TW, B_P_TL, B_P_CW, B_P_CL,
   B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR, B_HEADS, B_RULEFLAG, B_RULEOFFS, B_RULES2,
-  B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_COUNT
+  B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_PLANTAB, B_COUNT
 };
 static_assert(B_COUNT <= 64, "sx_handle::bufs holds every buffer id");
 
@@ -1676,7 +1736,9 @@ int tune_qc(sx_handle *h, int tl, int ml, const uint32_t *tv0, const uint2 *tfp,
 // B_FAIL (0 = pass).  *chunks_out gets the number of chunks (statistics).
 // planned: the jobs' plans, chunk counts and fail flags are already written
 // (by the kernel that made the jobs).
-int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, bool planned = false) {
+// ptab / pm_masked: the plan table of the call (or null) and its relaxed mask.
+int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, bool planned = false,
+                   const PlanEntry *ptab = nullptr, uint32_t pm_masked = 0) {
   Job *jobs = (Job *)h->bufs[B_JOBS].p;
   uint32_t *chunks, *fail;
   uint64_t *offs, *total;
@@ -1687,7 +1749,7 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, bo
   unsigned long long *evald;  // evaluated states, added up across calls
   ENSURE(B_CURSOR, unsigned long long, kEvalShards, evald);
   if (!planned)
-    hipLaunchKernelGGL(k_exh_plan, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, jobs, nj, mode,
+    hipLaunchKernelGGL(k_exh_plan, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, jobs, nj, mode, ptab, pm_masked,
                        chunks, fail);
   HIPCHK(h, hipGetLastError());
   int rc = scan_u32(h, chunks, nj, offs, total);
@@ -1986,6 +2048,14 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     }
     HIPCHK(h, hipGetLastError());
   }
+  // plans by opcode for length-2 targets (whose candidates are single instructions)
+  PlanEntry *ptab = nullptr;
+  if (tl == 2 && ml == 1) {
+    ENSURE(B_PLANTAB, PlanEntry, kPlanTab, ptab);
+    hipLaunchKernelGGL(k_plan_table, dim3(blocks_for(kPlanTab, 256)), dim3(256), 0, h->stream, cfg->exh_mode,
+                       pm_masked, ptab);
+    HIPCHK(h, hipGetLastError());
+  }
 
   // ---- segments of targets
   const uint64_t seg_max = 1ull << h->seg_log;
@@ -2151,8 +2221,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       if (!cfg->hits && go_bulk) {
         HIPCHK(h, hipMemsetAsync(njobs_d, 0, 4, h->stream));
         hipLaunchKernelGGL(k_bulk_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, nmid, rank,
-                           rej, done, tw, (uint32_t)tl, pm_masked, jobs, job_cap, njobs_d, cfg->exh_mode, jchunks,
-                           jfail);
+                           rej, done, tw, (uint32_t)tl, pm_masked, jobs, job_cap, njobs_d, cfg->exh_mode, ptab,
+                           jchunks, jfail);
         HIPCHK(h, hipGetLastError());
         uint32_t nj = 0;
         HIPCHK(h, hipMemcpyAsync(&nj, njobs_d, 4, hipMemcpyDeviceToHost, h->stream));
@@ -2181,7 +2251,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
                            cur, rej, done);
       hipLaunchKernelGGL(k_round_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, nmid, rank,
                          cur, rej, done, tw, (uint32_t)tl, pm_masked, cfg->hits, jobs, job_cap, njobs_d, nrest_d,
-                         cfg->hits ? 1 : 0, cfg->exh_mode, jchunks, jfail);
+                         cfg->hits ? 1 : 0, cfg->exh_mode, ptab, jchunks, jfail);
       HIPCHK(h, hipGetLastError());
       uint32_t cnt2[2] = {0, 0};
       HIPCHK(h, hipMemcpyAsync(cnt2, njobs_d, 8, hipMemcpyDeviceToHost, h->stream));
@@ -2203,7 +2273,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       st.exh_rounds++;
       st.exh_pairs += nj;
       uint64_t chunks = 0;
-      int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, cfg->hits != 0);  // first hit: planned there
+      int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, cfg->hits != 0, ptab, pm_masked);  // first hit: planned there
       if (rc) return rc;
       hipLaunchKernelGGL(k_round_resolve, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream,
                          (const Job *)h->bufs[B_JOBS].p, nj, (const uint32_t *)h->bufs[B_FAIL].p, rank, cur, rej,
