@@ -449,7 +449,30 @@ struct QcQueue {
     __builtin_amdgcn_wave_barrier();
     qn = 0;
   }
+  // End of a block: the four waves' queues in one reservation (one atomic per
+  // block instead of one per wave; atomics on one address serialise, and
+  // nearly every wave ends with survivors queued when hits are dense).  Every
+  // thread of the block calls it.
+  __device__ __forceinline__ void flush_block(uint32_t lane, uint32_t *s_cnt, uint32_t *s_base) {
+    const uint32_t w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) s_cnt[w] = qn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+      *s_base = tot ? atomicAdd(count, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t base = *s_base;
+    for (uint32_t k = 0; k < w; k++) base += s_cnt[k];
+    for (uint32_t i = lane; i < qn; i += kWave) {
+      const uint32_t o = base + i;
+      if (o < cap) surv[o] = q[i];
+    }
+    qn = 0;
+  }
 };
+static_assert(kWave * 4 == 256, "QuickCheck blocks are four waves");
 
 __device__ __forceinline__ void qc_item(const uint32_t *__restrict__ th, const uint2 *__restrict__ tfp, uint32_t nt,
                                         uint32_t t0, uint32_t t1, const uint2 *__restrict__ cfp, uint32_t nc,
@@ -529,10 +552,11 @@ __global__ __launch_bounds__(256) void k_quickcheck(const uint32_t *__restrict__
                                                     uint2 *__restrict__ surv, uint32_t surv_cap,
                                                     uint32_t *__restrict__ surv_count) {
   __shared__ uint2 queue[4][kQcQueue];
+  __shared__ uint32_t s_cnt[4], s_base;
   QcQueue Q{queue[threadIdx.x >> 6], 0, surv, surv_cap, surv_count};
   const uint32_t t0 = blockIdx.y * tile;
   qc_item(th, tfp, nt, t0, min(nt, t0 + tile), cfp, nc, blockIdx.x * 256 + threadIdx.x, pm_masked, Q);
-  if (Q.qn) Q.flush(lane_id());
+  Q.flush_block(lane_id(), s_cnt, &s_base);
 }
 
 // Persistent form: a fixed grid pulls work items (candidate block, target
@@ -549,7 +573,7 @@ __global__ __launch_bounds__(256) void k_qc_persistent(const uint32_t *__restric
                                                        uint32_t *__restrict__ heads, uint32_t ncb,
                                                        uint32_t ntiles) {
   __shared__ uint2 queue[4][kQcQueue];
-  __shared__ uint32_t item_s;
+  __shared__ uint32_t item_s, s_cnt[4], s_base;
   QcQueue Q{queue[threadIdx.x >> 6], 0, surv, surv_cap, surv_count};
   const uint32_t home = blockIdx.x & 7u;
   for (uint32_t k = 0; k < 8; k++) {
@@ -567,7 +591,7 @@ __global__ __launch_bounds__(256) void k_qc_persistent(const uint32_t *__restric
       qc_item(th, tfp, nt, t0, min(nt, t0 + tile), cfp, nc, cb * 256 + threadIdx.x, pm_masked, Q);
     }
   }
-  if (Q.qn) Q.flush(lane_id());
+  Q.flush_block(lane_id(), s_cnt, &s_base);
 }
 
 // Compaction kernels run 1,024-thread blocks and reserve their output with
