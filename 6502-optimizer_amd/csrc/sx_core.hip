@@ -1325,6 +1325,26 @@ __global__ __launch_bounds__(256) void k_compact_rules(const sx_rule *slots, con
   if (k < n && flag[k]) out[offs[k]] = slots[k];
 }
 
+// Rules of length-2 targets against single-instruction candidates packed
+// for the trip to the host: 6 words instead of the 12 of an sx_rule (the
+// unused src[2..3] and rep[1..3] are 0 and the target index fits 32 bits):
+// src[0], src[1], rep[0], target index, src_len | rep_len << 8 | dead << 16 |
+// pass << 24, bytes_saved | cycles_saved << 16.  The copy over the host link
+// is the one part of an enumeration that its own compute cannot hide.
+constexpr uint32_t kPackedWords = 6;
+static_assert(sizeof(sx_rule) == 48, "sx_rule is 12 words");
+__global__ __launch_bounds__(256) void k_pack_rules(const sx_rule *r, uint32_t n, uint2 *w) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const sx_rule x = r[i];
+  uint2 *o = w + (uint64_t)i * 3;
+  o[0] = make_uint2(x.src[0], x.src[1]);
+  o[1] = make_uint2(x.rep[0], (uint32_t)x.target_index);
+  o[2] = make_uint2((uint32_t)x.src_len | (uint32_t)x.rep_len << 8 | (uint32_t)x.dead_flags << 16 |
+                        (uint32_t)x.pass << 24,
+                    (uint32_t)(uint16_t)x.bytes_saved | (uint32_t)(uint16_t)x.cycles_saved << 16);
+}
+
 // Bulk QuickCheck / MidCheck / FlagDiff for sx_check_pairs: one wave per pair.
 __global__ __launch_bounds__(256) void k_pair_vectors(const uint32_t *tw, const uint8_t *tl,
                                                       const uint32_t *cw, const uint8_t *cl,
@@ -1448,16 +1468,19 @@ struct sx_handle {
   hipStream_t stream = nullptr;
   std::string err;
   std::vector<sx_rule> rules;  // all-hits mode: sorted on the host
-  // first-hit mode: the rules stay on the device, in order, until fetched
-  sx_rule *d_rules = nullptr;
-  uint64_t d_rules_cap = 0, d_rules_n = 0;
+  // first-hit mode: the rules stay on the device, in order, until fetched,
+  // rec_words 32-bit words per rule: 12 (an sx_rule) or 6 (the packed form of
+  // k_pack_rules, which sx_fetch_rules expands)
+  uint32_t *d_rules = nullptr;
+  uint64_t d_rules_capw = 0, d_rules_n = 0;
+  uint32_t rec_words = 12;
   bool rules_on_device = false;
   // ... and a pinned host copy of them, filled segment by segment on
   // copy_stream while the later segments compute
   hipStream_t copy_stream = nullptr;
   hipEvent_t seg_ev = nullptr;
-  sx_rule *h_rules = nullptr;
-  uint64_t h_rules_cap = 0;
+  uint32_t *h_rules = nullptr;
+  uint64_t h_rules_capw = 0;
   // QuickCheck launch configurations chosen by the autotuner, per (target_len, max_cand_len)
   struct QcChoice {
     int tl, ml, big;  // big: segments of >= 65,536 targets
@@ -1643,6 +1666,47 @@ void parallel_memcpy(void *dst, const void *src, size_t bytes) {
   try {
     for (; i < nt; i++) pool.emplace_back(piece, i);
   } catch (...) {  // no thread to be had: the caller copies the rest
+    for (; i < nt; i++) piece(i);
+  }
+  piece(0);
+  for (auto &t : pool) t.join();
+}
+
+// The packed rules of k_pack_rules back into sx_rule records, in pieces on
+// up to 16 threads like parallel_memcpy.
+void parallel_unpack_rules(sx_rule *out, const uint32_t *w, uint64_t n) {
+  auto unpack = [=](uint64_t a, uint64_t b) {
+    for (uint64_t i = a; i < b; i++) {
+      const uint32_t *p = w + i * kPackedWords;
+      sx_rule r;
+      memset(&r, 0, sizeof(r));
+      r.src[0] = p[0];
+      r.src[1] = p[1];
+      r.rep[0] = p[2];
+      r.target_index = p[3];
+      r.src_len = (uint8_t)p[4];
+      r.rep_len = (uint8_t)(p[4] >> 8);
+      r.dead_flags = (uint8_t)(p[4] >> 16);
+      r.pass = (uint8_t)(p[4] >> 24);
+      r.bytes_saved = (int16_t)(uint16_t)p[5];
+      r.cycles_saved = (int16_t)(uint16_t)(p[5] >> 16);
+      out[i] = r;
+    }
+  };
+  const uint64_t kPiece = 1u << 16;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const unsigned nt = (unsigned)std::min<uint64_t>(std::min(16u, hw), n / kPiece);
+  if (nt < 2) {
+    unpack(0, n);
+    return;
+  }
+  const uint64_t per = (n + nt - 1) / nt;
+  auto piece = [=](unsigned i) { unpack(std::min<uint64_t>(n, i * per), std::min<uint64_t>(n, (i + 1) * per)); };
+  std::vector<std::thread> pool;
+  unsigned i = 1;
+  try {
+    for (; i < nt; i++) pool.emplace_back(piece, i);
+  } catch (...) {  // no thread to be had: the caller unpacks the rest
     for (; i < nt; i++) piece(i);
   }
   piece(0);
@@ -2000,6 +2064,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   HIPCHK(h, hipStreamSynchronize(h->copy_stream));  // the last call's rule copies read d_rules
   h->d_rules_n = 0;
   h->rules_on_device = !cfg->hits;
+  h->rec_words = 12;
   {
     int rc = clear_evaluated(h);
     if (rc) return rc;
@@ -2028,6 +2093,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     nt_all = b - a;
     d_tsrc = h->d_l2 + 2 * a;
   }
+  if (h->rules_on_device && tl == 2 && ml == 1 && tbase + nt_all <= 0xFFFFFFFFull) h->rec_words = kPackedWords;
 
   // ---- candidates: slots in Go order (rank = slot)
   const uint64_t nc = ml == 1 ? (uint64_t)kNumInsts : (uint64_t)kNumInsts + (uint64_t)kNumInsts * kNumInsts;
@@ -2338,43 +2404,50 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       HIPCHK(h, hipGetLastError());
     }
     if (h->rules_on_device) {
-      // append to the device-resident list, doubling its capacity as needed
-      if (h->d_rules_n + nr > h->d_rules_cap) {
-        uint64_t cap2 = std::max<uint64_t>(std::max<uint64_t>(h->d_rules_cap * 2, h->d_rules_n + nr), 1u << 20);
-        sx_rule *grown = nullptr;
-        if (hipMalloc(&grown, cap2 * sizeof(sx_rule)) != hipSuccess) return set_err(h, SX_ENOMEM, "rule list");
-        if (h->d_rules_n)
-          HIPCHK(h, hipMemcpyAsync(grown, h->d_rules, h->d_rules_n * sizeof(sx_rule), hipMemcpyDeviceToDevice,
-                                   h->stream));
+      // append to the device-resident list (in 32-bit words), doubling its
+      // capacity as needed
+      const uint64_t rw = h->rec_words, need = (h->d_rules_n + nr) * rw, used = h->d_rules_n * rw;
+      if (need > h->d_rules_capw) {
+        uint64_t cap2 = std::max<uint64_t>(std::max<uint64_t>(h->d_rules_capw * 2, need), 12u << 20);
+        uint32_t *grown = nullptr;
+        if (hipMalloc(&grown, cap2 * 4) != hipSuccess) return set_err(h, SX_ENOMEM, "rule list");
+        if (used)
+          HIPCHK(h, hipMemcpyAsync(grown, h->d_rules, used * 4, hipMemcpyDeviceToDevice, h->stream));
         SYNC_WD("device sync");
         HIPCHK(h, hipStreamSynchronize(h->copy_stream));  // copies in flight read the old list
         if (h->d_rules) HIPCHK(h, hipFree(h->d_rules));
         h->d_rules = grown;
-        h->d_rules_cap = cap2;
+        h->d_rules_capw = cap2;
       }
-      if (h->d_rules_n + nr > h->h_rules_cap) {
+      if (need > h->h_rules_capw) {
         // grow the pinned host copy (kept across calls, so this happens once
         // per size class)
         HIPCHK(h, hipStreamSynchronize(h->copy_stream));
-        uint64_t cap2 = std::max<uint64_t>(std::max<uint64_t>(h->h_rules_cap * 2, h->d_rules_n + nr), 1u << 20);
-        sx_rule *grown = nullptr;
-        if (hipHostMalloc(&grown, cap2 * sizeof(sx_rule), hipHostMallocDefault) != hipSuccess) {
+        uint64_t cap2 = std::max<uint64_t>(std::max<uint64_t>(h->h_rules_capw * 2, need), 12u << 20);
+        uint32_t *grown = nullptr;
+        if (hipHostMalloc(&grown, cap2 * 4, hipHostMallocDefault) != hipSuccess) {
           (void)hipGetLastError();
           return set_err(h, SX_ENOMEM, "pinned rule list");
         }
-        if (h->d_rules_n) memcpy(grown, h->h_rules, h->d_rules_n * sizeof(sx_rule));
+        if (used) memcpy(grown, h->h_rules, used * 4);
         if (h->h_rules) HIPCHK(h, hipHostFree(h->h_rules));
         h->h_rules = grown;
-        h->h_rules_cap = cap2;
+        h->h_rules_capw = cap2;
       }
       if (nr) {
-        HIPCHK(h, hipMemcpyAsync(h->d_rules + h->d_rules_n, rules_out, (size_t)nr * sizeof(sx_rule),
-                                 hipMemcpyDeviceToDevice, h->stream));
+        if (rw == kPackedWords) {
+          hipLaunchKernelGGL(k_pack_rules, dim3(blocks_for(nr, 256)), dim3(256), 0, h->stream, rules_out, nr,
+                             reinterpret_cast<uint2 *>(h->d_rules + used));
+          HIPCHK(h, hipGetLastError());
+        } else {
+          HIPCHK(h, hipMemcpyAsync(h->d_rules + used, rules_out, (size_t)nr * sizeof(sx_rule),
+                                   hipMemcpyDeviceToDevice, h->stream));
+        }
         // copy the segment's rules to the host while the next segments run
         HIPCHK(h, hipEventRecord(h->seg_ev, h->stream));
         HIPCHK(h, hipStreamWaitEvent(h->copy_stream, h->seg_ev, 0));
-        HIPCHK(h, hipMemcpyAsync(h->h_rules + h->d_rules_n, h->d_rules + h->d_rules_n,
-                                 (size_t)nr * sizeof(sx_rule), hipMemcpyDeviceToHost, h->copy_stream));
+        HIPCHK(h, hipMemcpyAsync(h->h_rules + used, h->d_rules + used, (size_t)nr * rw * 4, hipMemcpyDeviceToHost,
+                                 h->copy_stream));
       }
       h->d_rules_n += nr;
     } else {
@@ -2449,7 +2522,10 @@ extern "C" int sx_fetch_rules(sx_handle *h, sx_rule *out, uint64_t offset, uint6
       // the pinned copy is complete once the copy stream drains
       HIPCHK(h, hipSetDevice(h->device));
       HIPCHK(h, hipStreamSynchronize(h->copy_stream));
-      parallel_memcpy(out, h->h_rules + offset, m * sizeof(sx_rule));
+      if (h->rec_words == kPackedWords)
+        parallel_unpack_rules(out, h->h_rules + offset * kPackedWords, m);
+      else
+        parallel_memcpy(out, h->h_rules + offset * 12, m * sizeof(sx_rule));
     } else {
       memcpy(out, h->rules.data() + offset, m * sizeof(sx_rule));
     }
