@@ -670,6 +670,17 @@ __global__ __launch_bounds__(kBlk) void k_midcheck(const uint2 *__restrict__ sur
   }
 }
 
+// A segment's first-hit state, cleared in one launch: cur = none, rej = 0,
+// done = 0 and, when given, the rule slots' flags = 0.
+__global__ void k_seg_clear(uint32_t *cur, uint32_t *rej, uint8_t *done, uint32_t *rule_flag, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  cur[i] = kRankNone;
+  rej[i] = 0;
+  done[i] = 0;
+  if (rule_flag) rule_flag[i] = 0;
+}
+
 // First-hit rounds.  cur/rej/done are per (target, pass-1): cur = lowest rank
 // this round, rej = 1 + highest rank rejected so far, done = resolved.
 __global__ __launch_bounds__(256) void k_round_min(const uint2 *mid, uint32_t nm,
@@ -2250,13 +2261,23 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     // ---- MidCheck
     uint2 *mid;
     uint32_t nmid = 0;
+    // the rounds' per-(target, pass) state, cleared while the host waits for
+    // the MidCheck count
+    uint32_t *cur, *rej, *rule_flag;  // rule_flag (first hit): slot k = target * 2 + pass - 1 holds a rule
+    uint8_t *done;
+    ENSURE(B_CUR, uint32_t, nt * 2, cur);
+    ENSURE(B_REJ, uint32_t, nt * 2, rej);
+    ENSURE(B_DONE, uint8_t, nt * 2, done);
+    ENSURE(B_RULEFLAG, uint32_t, nt * 2, rule_flag);
     for (;;) {
       ENSURE(B_MID, uint2, std::max<uint32_t>(nsurv, 1), mid);
-      HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));
+      HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));  // counters[2..7] stay 0 for the rounds
       if (nsurv)
         hipLaunchKernelGGL(k_midcheck, dim3(blocks_for(nsurv, kBlk)), dim3(kBlk), 0, h->stream, surv, nsurv,
                            tmh, tbytes, cmh, cbytes, cprune, cfg->hits, cfg->use_mid, mid,
                            std::max<uint32_t>(nsurv, 1), counters);
+      hipLaunchKernelGGL(k_seg_clear, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, rej, done,
+                         cfg->hits ? nullptr : rule_flag, nt * 2);
       HIPCHK(h, hipGetLastError());
       uint32_t cnt[2];
       HIPCHK(h, hipMemcpyAsync(cnt, counters, 8, hipMemcpyDeviceToHost, h->stream));
@@ -2270,13 +2291,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     HIPCHK(h, hipEventRecord(e3, h->stream));
 
     // ---- ExhaustiveCheck (rounds for first-hit)
-    uint32_t *cur, *rej;
-    uint8_t *done;
     Job *jobs;
     sx_rule *rules;
-    ENSURE(B_CUR, uint32_t, nt * 2, cur);
-    ENSURE(B_REJ, uint32_t, nt * 2, rej);
-    ENSURE(B_DONE, uint8_t, nt * 2, done);
     const uint32_t job_cap = std::max<uint32_t>(nmid, 1);  // the first-hit tail may queue every survivor
     ENSURE(B_JOBS, Job, job_cap, jobs);
     uint32_t *jchunks, *jfail;  // written by the job kernels, read by the sweep
@@ -2284,15 +2300,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     ENSURE(B_FAIL, uint32_t, job_cap + 1, jfail);
     const uint32_t rule_cap = cfg->hits ? std::max<uint32_t>(nmid, 1) : (uint32_t)(2 * nt);
     ENSURE(B_RULES, sx_rule, rule_cap, rules);
-    uint32_t *rule_flag;  // first hit: slot k = target * 2 + pass - 1 holds a rule
-    ENSURE(B_RULEFLAG, uint32_t, nt * 2, rule_flag);
-    if (!cfg->hits)
-      hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, rule_flag, nt * 2, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, nt * 2, kRankNone);
-    HIPCHK(h, hipMemsetAsync(rej, 0, nt * 2 * 4, h->stream));
-    HIPCHK(h, hipMemsetAsync(done, 0, nt * 2, h->stream));
-    uint32_t *nrules_d = counters + 2, *njobs_d = counters + 3;
-    HIPCHK(h, hipMemsetAsync(nrules_d, 0, 4, h->stream));
+    uint32_t *nrules_d = counters + 2, *njobs_d = counters + 3;  // zeroed before MidCheck
     // The first-hit tail runs as one batch once the untested survivors of the
     // open (target, pass) entries number at most kBulkJobs, or at most
     // kBulkRatio x the entries themselves (one batch then costs at most that
