@@ -492,53 +492,66 @@ __device__ __forceinline__ void qc_item(const uint32_t *__restrict__ th, const u
   const uint32_t hc = qc_mix(clo[0] & mlo, chi[0], 0) ^ qc_mix(clo[1] & mlo, chi[1], 1) ^
                       qc_mix(clo[2] & mlo, chi[2], 2) ^ qc_mix(clo[3] & mlo, chi[3], 3);
   const uint64_t vmask = ballot(valid);
-  for (uint32_t t = t0; t < t1; t += kQcBlk) {
-    uint32_t nb = min(kQcBlk, t1 - t);
-    // one 32-byte scalar load of 8 consecutive target hashes (the plane
-    // has kQcBlk entries of slack past nt, so no clamping is needed)
-    struct alignas(32) ThBlk {
-      uint32_t h[kQcBlk];
-    };
-    const ThBlk blk = *reinterpret_cast<const ThBlk *>(th + t);  // t is a multiple of kQcBlk
-    // target t + u against the wave's candidates: hash filter, then all 8
-    // vectors exactly under both masks (the differences ORed together first:
-    // one 3-input VALU op per dword)
-    auto check = [&](uint32_t u) {
-      const uint64_t m = ballot(hc == blk.h[u]) & vmask;
-      if (m == 0) return;
-      const uint2 *tf = tfp + (uint64_t)(t + u) * kFpVecs;
-      uint32_t acc_lo = 0, acc_hi = 0;
+  // 8 consecutive target hashes, one 32-byte scalar load (the plane has
+  // kQcBlk entries of slack past nt, so no clamping is needed)
+  struct alignas(32) ThBlk {
+    uint32_t h[kQcBlk];
+  };
+  // target tb + u against the wave's candidates: hash filter, then all 8
+  // vectors exactly under both masks (the differences ORed together first:
+  // one 3-input VALU op per dword)
+  auto check = [&](const ThBlk &blk, uint32_t tb, uint32_t u) {
+    const uint64_t m = ballot(hc == blk.h[u]) & vmask;
+    if (m == 0) return;
+    const uint2 *tf = tfp + (uint64_t)(tb + u) * kFpVecs;
+    uint32_t acc_lo = 0, acc_hi = 0;
 #pragma unroll
-      for (int v = 0; v < 8; v++) {
-        const uint2 f = tf[v];
-        acc_lo |= clo[v] ^ f.x;
-        acc_hi |= chi[v] ^ f.y;
-      }
-      const bool okm = valid && ((acc_lo & mlo) | acc_hi) == 0;
-      const bool okf = valid && ((acc_lo & flo) | acc_hi) == 0;
-      const uint64_t hm = __ballot(okm);
-      if (hm == 0) return;
-      if (okm) Q.q[Q.qn + mbcnt(hm)] = make_uint2(t + u, slot | (okf ? 0x80000000u : 0u));
-      Q.qn += (uint32_t)__popcll(hm);
-      if (Q.qn > kQcQueue - kWave) Q.flush(lane);
-    };
-    if (nb == kQcBlk) {
-      // the minimum of the differences is 0 iff some target's hash matches:
-      // v_xor and a v_min3 tree on the vector unit, one ballot and one branch
-      // per block (the scalar unit, shared by the CU's four SIMDs, was the
-      // bound when each compare was its own scalar mask)
-      uint32_t dmin = hc ^ blk.h[0];
+    for (int v = 0; v < 8; v++) {
+      const uint2 f = tf[v];
+      acc_lo |= clo[v] ^ f.x;
+      acc_hi |= chi[v] ^ f.y;
+    }
+    const bool okm = valid && ((acc_lo & mlo) | acc_hi) == 0;
+    const bool okf = valid && ((acc_lo & flo) | acc_hi) == 0;
+    const uint64_t hm = __ballot(okm);
+    if (hm == 0) return;
+    if (okm) Q.q[Q.qn + mbcnt(hm)] = make_uint2(tb + u, slot | (okf ? 0x80000000u : 0u));
+    Q.qn += (uint32_t)__popcll(hm);
+    if (Q.qn > kQcQueue - kWave) Q.flush(lane);
+  };
+  // the minimum of the differences is 0 iff some target's hash matches: v_xor
+  // and a v_min3 tree on the vector unit, one ballot and one branch per 16
+  // targets (the scalar unit, shared by the CU's four SIMDs, was the bound
+  // when each compare was its own scalar mask); the two blocks' loads are in
+  // flight together
+  auto dmin_of = [&](const ThBlk &blk) {
+    uint32_t d = hc ^ blk.h[0];
 #pragma unroll
-      for (uint32_t j = 1; j < kQcBlk; j++) dmin = min(dmin, hc ^ blk.h[j]);
-      if ((ballot(dmin == 0) & vmask) == 0) continue;
+    for (uint32_t j = 1; j < kQcBlk; j++) d = min(d, hc ^ blk.h[j]);
+    return d;
+  };
+  uint32_t t = t0;  // a multiple of kQcBlk
+  for (; t + 2 * kQcBlk <= t1; t += 2 * kQcBlk) {
+    const ThBlk b0 = *reinterpret_cast<const ThBlk *>(th + t);
+    const ThBlk b1 = *reinterpret_cast<const ThBlk *>(th + t + kQcBlk);
+    const uint32_t d0 = dmin_of(b0), d1 = dmin_of(b1);
+    if ((ballot(min(d0, d1) == 0) & vmask) == 0) continue;
+    if (ballot(d0 == 0) & vmask) {
 #pragma unroll
-      for (uint32_t u = 0; u < kQcBlk; u++) check(u);
-    } else {
+      for (uint32_t u = 0; u < kQcBlk; u++) check(b0, t, u);
+    }
+    if (ballot(d1 == 0) & vmask) {
 #pragma unroll
-      for (uint32_t u = 0; u < kQcBlk; u++) {
-        if (u >= nb) break;
-        check(u);
-      }
+      for (uint32_t u = 0; u < kQcBlk; u++) check(b1, t + kQcBlk, u);
+    }
+  }
+  for (; t < t1; t += kQcBlk) {
+    const uint32_t nb = min(kQcBlk, t1 - t);
+    const ThBlk blk = *reinterpret_cast<const ThBlk *>(th + t);
+#pragma unroll
+    for (uint32_t u = 0; u < kQcBlk; u++) {
+      if (u >= nb) break;
+      check(blk, t, u);
     }
   }
 }
