@@ -520,9 +520,9 @@ __device__ __forceinline__ void qc_item(const uint32_t *__restrict__ th, const u
     if (Q.qn > kQcQueue - kWave) Q.flush(lane);
   };
   // the minimum of the differences is 0 iff some target's hash matches: v_xor
-  // and a v_min3 tree on the vector unit, one ballot and one branch per 16
+  // and a v_min3 tree on the vector unit, one ballot and one branch per 32
   // targets (the scalar unit, shared by the CU's four SIMDs, was the bound
-  // when each compare was its own scalar mask); the two blocks' loads are in
+  // when each compare was its own scalar mask); four blocks' loads are in
   // flight together
   auto dmin_of = [&](const ThBlk &blk) {
     uint32_t d = hc ^ blk.h[0];
@@ -531,18 +531,20 @@ __device__ __forceinline__ void qc_item(const uint32_t *__restrict__ th, const u
     return d;
   };
   uint32_t t = t0;  // a multiple of kQcBlk
-  for (; t + 2 * kQcBlk <= t1; t += 2 * kQcBlk) {
-    const ThBlk b0 = *reinterpret_cast<const ThBlk *>(th + t);
-    const ThBlk b1 = *reinterpret_cast<const ThBlk *>(th + t + kQcBlk);
-    const uint32_t d0 = dmin_of(b0), d1 = dmin_of(b1);
-    if ((ballot(min(d0, d1) == 0) & vmask) == 0) continue;
-    if (ballot(d0 == 0) & vmask) {
+  for (; t + 4 * kQcBlk <= t1; t += 4 * kQcBlk) {
+    ThBlk b[4];
 #pragma unroll
-      for (uint32_t u = 0; u < kQcBlk; u++) check(b0, t, u);
-    }
-    if (ballot(d1 == 0) & vmask) {
+    for (int k = 0; k < 4; k++) b[k] = *reinterpret_cast<const ThBlk *>(th + t + k * kQcBlk);
+    uint32_t d[4];
 #pragma unroll
-      for (uint32_t u = 0; u < kQcBlk; u++) check(b1, t + kQcBlk, u);
+    for (int k = 0; k < 4; k++) d[k] = dmin_of(b[k]);
+    if ((ballot(min(min(d[0], d[1]), min(d[2], d[3])) == 0) & vmask) == 0) continue;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (ballot(d[k] == 0) & vmask) {
+#pragma unroll
+        for (uint32_t u = 0; u < kQcBlk; u++) check(b[k], t + k * kQcBlk, u);
+      }
     }
   }
   for (; t < t1; t += kQcBlk) {
@@ -2175,7 +2177,9 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   const uint64_t seg_max = 1ull << h->seg_log;
   const uint64_t seg = cfg->targets ? std::min<uint64_t>(nt_all, seg_max) : seg_max;
   // QuickCheck launch shape: the caller's tile (grid form), else the autotuner's choice
-  sx_handle::QcChoice qcc{tl, ml, 0, cfg->tile_targets, 0};
+  // (a caller's tile is rounded up to whole 32-target scan steps: tiles start
+  // on hash-block boundaries; the tile only distributes the work)
+  sx_handle::QcChoice qcc{tl, ml, 0, (cfg->tile_targets + 4 * kQcBlk - 1) / (4 * kQcBlk) * (4 * kQcBlk), 0};
   bool qc_tuned = cfg->tile_targets != 0;
   // survivor buffer: the caller's size, else what sufficed before on this
   // handle, else about one entry per 4,096 pairs (QuickCheck passes ~1e-4 of

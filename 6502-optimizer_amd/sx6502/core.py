@@ -279,12 +279,12 @@ class Device:
         st = Stats()
         rc = lib().sx_enumerate(self._h, C.byref(cfg), None, 0, C.byref(n), C.byref(st))
         self._check(rc, "sx_enumerate")
-        out = np.zeros(n.value, RULE_DTYPE)
+        out = np.empty(n.value, RULE_DTYPE)  # sx_fetch_rules writes every byte of every record it returns
         got = C.c_uint64(0)
         rc = lib().sx_fetch_rules(self._h, out.ctypes.data_as(C.c_void_p), 0, n.value, C.byref(got))
         self._check(rc, "sx_fetch_rules")
         del keep
-        return out, st.as_dict()
+        return out[: got.value], st.as_dict()
 
     def pruned_targets(self, first: int = 0, n: Optional[int] = None) -> np.ndarray:
         total = C.c_uint64(0)

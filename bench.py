@@ -254,7 +254,8 @@ def main():
         rules, st = dev.enumerate(None, dead=args.dead)
         times.append(time.perf_counter() - t0)
         stats.append(st)
-        print(f"bench: step {i} {times[-1]:.3f} s, {st['rules']} rules", file=sys.stderr, flush=True)
+        print(f"bench: step {i} {times[-1]:.4f} s ({st['ms_total']:.2f} ms in sx_enumerate), {st['rules']} rules",
+              file=sys.stderr, flush=True)
     ms = 1e3 * sum(times) / len(times)
     st = stats[-1]
     qc_ms = sum(s["ms_qc"] for s in stats) / len(stats)
@@ -308,6 +309,9 @@ def main():
                    "dead_flags": f"0x{args.dead:02X}", "targets": N_TARGETS, "candidates": N_CANDS},
         "l2_wall_s": ms * 1e-3, "rules": int(st["rules"]),
         "stage_ms": {"setup": st["ms_setup"], "qc": qc_ms, "mid": mid_ms, "exh": exh_ms},
+        # wall time inside sx_enumerate; the rest of ms_per_step is the rule
+        # array's allocation and sx_fetch_rules (unpacking into it)
+        "ms_in_enumerate": sum(s["ms_total"] for s in stats) / len(stats),
         "counters": {k: int(st[k]) for k in ("qc_raw_hits", "qc_hits", "mid_pass", "exh_pairs", "exh_states",
                                              "exh_rounds")},
         "qc_stage_pairs_per_s": PAIRS / (qc_ms * 1e-3),
