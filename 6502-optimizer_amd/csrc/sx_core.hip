@@ -63,7 +63,7 @@ constexpr int kWave = 64;
 constexpr int kChunkLog = 14;           // states per exhaustive chunk
 constexpr uint32_t kRankNone = 0xFFFFFFFFu;
 constexpr uint32_t kQcBlk = 8;          // targets per QuickCheck hash block (one scalar load)
-constexpr int kQcQueue = 256;           // LDS survivor queue entries per wave
+constexpr int kQcQueue = 512;           // LDS survivor queue entries per wave
 constexpr int kEvalShards = 1024;       // shards of the evaluated-states counter
 
 
