@@ -64,6 +64,15 @@ constexpr int kChunkLog = 14;           // states per exhaustive chunk
 constexpr uint32_t kRankNone = 0xFFFFFFFFu;
 constexpr uint32_t kQcBlk = 8;          // targets per QuickCheck hash block (one scalar load)
 constexpr int kQcQueue = 512;           // LDS survivor queue entries per wave
+// QuickCheck survivors go to 8 regions of the survivor buffer, each with its
+// own counter (block b uses region b mod 8): a queue flush reserves its
+// output with an atomic, and atomics on one address serialise, which bound
+// the kernel where hits are dense.  MidCheck reads the regions as one list.
+constexpr uint32_t kQcRegions = 8;
+struct SurvRegions {
+  uint32_t pre[kQcRegions + 1];  // region r holds list entries [pre[r], pre[r + 1])
+  uint32_t cap;                  // entries per region
+};
 constexpr int kEvalShards = 1024;       // shards of the evaluated-states counter
 
 
@@ -564,11 +573,12 @@ __global__ __launch_bounds__(256) void k_quickcheck(const uint32_t *__restrict__
                                                     const uint2 *__restrict__ tfp, uint32_t nt,
                                                     uint32_t tile, const uint2 *__restrict__ cfp,
                                                     uint32_t nc, uint32_t pm_masked,
-                                                    uint2 *__restrict__ surv, uint32_t surv_cap,
+                                                    uint2 *__restrict__ surv, uint32_t region_cap,
                                                     uint32_t *__restrict__ surv_count) {
   __shared__ uint2 queue[4][kQcQueue];
   __shared__ uint32_t s_cnt[4], s_base;
-  QcQueue Q{queue[threadIdx.x >> 6], 0, surv, surv_cap, surv_count};
+  const uint32_t reg = (blockIdx.x + blockIdx.y * gridDim.x) & (kQcRegions - 1);
+  QcQueue Q{queue[threadIdx.x >> 6], 0, surv + (uint64_t)reg * region_cap, region_cap, surv_count + reg};
   const uint32_t t0 = blockIdx.y * tile;
   qc_item(th, tfp, nt, t0, min(nt, t0 + tile), cfp, nc, blockIdx.x * 256 + threadIdx.x, pm_masked, Q);
   Q.flush_block(lane_id(), s_cnt, &s_base);
@@ -584,12 +594,13 @@ __global__ __launch_bounds__(256) void k_qc_persistent(const uint32_t *__restric
                                                        const uint2 *__restrict__ tfp, uint32_t nt, uint32_t tile,
                                                        const uint2 *__restrict__ cfp, uint32_t nc,
                                                        uint32_t pm_masked, uint2 *__restrict__ surv,
-                                                       uint32_t surv_cap, uint32_t *__restrict__ surv_count,
+                                                       uint32_t region_cap, uint32_t *__restrict__ surv_count,
                                                        uint32_t *__restrict__ heads, uint32_t ncb,
                                                        uint32_t ntiles) {
   __shared__ uint2 queue[4][kQcQueue];
   __shared__ uint32_t item_s, s_cnt[4], s_base;
-  QcQueue Q{queue[threadIdx.x >> 6], 0, surv, surv_cap, surv_count};
+  const uint32_t reg = blockIdx.x & (kQcRegions - 1);
+  QcQueue Q{queue[threadIdx.x >> 6], 0, surv + (uint64_t)reg * region_cap, region_cap, surv_count + reg};
   const uint32_t home = blockIdx.x & 7u;
   for (uint32_t k = 0; k < 8; k++) {
     const uint32_t q = (home + k) & 7u;
@@ -640,7 +651,7 @@ __device__ __forceinline__ uint32_t block_prefix(uint32_t cnt, uint32_t *s, uint
 // (target, slot | pass << 30).  counters[0] += QuickCheck hits after the
 // byte/prune filters and counters[1] += passes, both with one 64-bit atomic
 // per block (counters[1] is also the output cursor).
-__global__ __launch_bounds__(kBlk) void k_midcheck(const uint2 *__restrict__ surv, uint32_t ns,
+__global__ __launch_bounds__(kBlk) void k_midcheck(const uint2 *__restrict__ surv, const SurvRegions R, uint32_t ns,
                                                    const uint2 *__restrict__ tmh,
                                                    const uint8_t *__restrict__ tbytes,
                                                    const uint2 *__restrict__ cmh,
@@ -656,7 +667,15 @@ __global__ __launch_bounds__(kBlk) void k_midcheck(const uint2 *__restrict__ sur
   uint32_t t = 0, slot = 0, pass = 0;
   bool qc_hit = false;
   if (i < ns) {
-    const uint2 e = surv[i];
+    // list entry i -> its region and place there
+    uint32_t base = 0, r = 0;
+#pragma unroll
+    for (uint32_t k = 1; k < kQcRegions; k++)
+      if (i >= R.pre[k]) {
+        base = R.pre[k];
+        r = k;
+      }
+    const uint2 e = surv[(uint64_t)r * R.cap + (i - base)];
     t = e.x;
     slot = e.y & 0x3FFFFFFFu;
     const bool full = (e.y >> 31) != 0;
@@ -1788,15 +1807,15 @@ int launch_qc(sx_handle *h, uint32_t tile, uint32_t blocks_per_cu, const uint32_
   if (blocks_per_cu == 0) {
     dim3 grid(blocks_for(nc, 256), (unsigned)((nt + tile - 1) / tile));
     hipLaunchKernelGGL(k_quickcheck, grid, dim3(256), 0, h->stream, tv0, tfp, (uint32_t)nt, tile, cfp,
-                       (uint32_t)nc, pm_masked, surv, surv_cap, counters);
+                       (uint32_t)nc, pm_masked, surv, surv_cap / kQcRegions, counters);
   } else {
     uint32_t *heads;
     ENSURE(B_HEADS, uint32_t, 8, heads);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, h->stream, heads, (uint64_t)8, 0u);
     const uint32_t ncb = blocks_for(nc, 256), ntiles = (uint32_t)((nt + tile - 1) / tile);
     hipLaunchKernelGGL(k_qc_persistent, dim3((unsigned)(h->num_cu * blocks_per_cu)), dim3(256), 0, h->stream, tv0,
-                       tfp, (uint32_t)nt, tile, cfp, (uint32_t)nc, pm_masked, surv, surv_cap, counters, heads, ncb,
-                       ntiles);
+                       tfp, (uint32_t)nt, tile, cfp, (uint32_t)nc, pm_masked, surv, surv_cap / kQcRegions, counters,
+                       heads, ncb, ntiles);
   }
   HIPCHK(h, hipGetLastError());
   return SX_OK;
@@ -2241,34 +2260,52 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
 
     // ---- QuickCheck, re-run with a larger buffer on overflow (no hit is dropped)
     uint2 *surv;
-    uint32_t *counters;
-    ENSURE(B_COUNTERS, uint32_t, 8, counters);
+    uint32_t *counters;  // [0, 8): MidCheck and the rounds; [8, 16): QuickCheck's region counts
+    ENSURE(B_COUNTERS, uint32_t, 8 + kQcRegions, counters);
+    uint32_t *qc_counts = counters + 8;
     uint32_t nsurv = 0;
+    SurvRegions regions;
     for (;;) {
       ENSURE(B_SURV, uint2, surv_cap, surv);
       if (!qc_tuned) {
-        int rc = tune_qc(h, tl, ml, tv0, tfp, nt, cfp, nc, pm_masked, surv, surv_cap, counters, &qcc);
+        int rc = tune_qc(h, tl, ml, tv0, tfp, nt, cfp, nc, pm_masked, surv, surv_cap, qc_counts, &qcc);
         if (rc) return rc;
         qc_tuned = true;
       }
-      HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));
+      HIPCHK(h, hipMemsetAsync(qc_counts, 0, kQcRegions * 4, h->stream));
       {
         size_t ts = 0;
         int rc = timed_begin(h, 0, &ts);
         if (rc) return rc;
         rc = launch_qc(h, qcc.tile, qcc.blocks_per_cu, tv0, tfp, nt, cfp, nc, pm_masked, surv, surv_cap,
-                       counters);
+                       qc_counts);
         if (rc) return rc;
         rc = timed_end(h, ts);
         if (rc) return rc;
       }
-      HIPCHK(h, hipMemcpyAsync(&nsurv, counters, 4, hipMemcpyDeviceToHost, h->stream));
+      uint32_t cnt8[kQcRegions];
+      HIPCHK(h, hipMemcpyAsync(cnt8, qc_counts, sizeof(cnt8), hipMemcpyDeviceToHost, h->stream));
       SYNC_WD("device sync");
-      trace(h, "segment %llu: %llu targets, QuickCheck survivors %u (cap %u)", (unsigned long long)s0,
-            (unsigned long long)nt, nsurv, surv_cap);
-      if (nsurv <= surv_cap) break;
+      const uint32_t region_cap = surv_cap / kQcRegions;
+      uint64_t total = 0;
+      uint32_t most = 0;
+      regions.cap = region_cap;
+      for (uint32_t r = 0; r < kQcRegions; r++) {
+        regions.pre[r] = (uint32_t)std::min<uint64_t>(total, 0xFFFFFFFFu);
+        total += cnt8[r];
+        most = std::max(most, cnt8[r]);
+      }
+      regions.pre[kQcRegions] = (uint32_t)std::min<uint64_t>(total, 0xFFFFFFFFu);
+      trace(h, "segment %llu: %llu targets, QuickCheck survivors %llu (largest region %u of %u)",
+            (unsigned long long)s0, (unsigned long long)nt, (unsigned long long)total, most, region_cap);
+      if (most <= region_cap) {
+        nsurv = (uint32_t)total;
+        break;
+      }
       st.overflow_retries++;
-      surv_cap = nsurv + nsurv / 4 + 1024;
+      if ((uint64_t)most * 5 / 4 + 1024 > 0xFFFFFFFFull / kQcRegions)
+        return set_err(h, SX_ENOMEM, "QuickCheck survivor buffer (%u in one region)", most);
+      surv_cap = kQcRegions * (most + most / 4 + 1024);
       if (!cfg->survivor_cap) h->surv_cap_hint = std::max(h->surv_cap_hint, surv_cap);
     }
     HIPCHK(h, hipEventRecord(e2, h->stream));
@@ -2290,8 +2327,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       ENSURE(B_MID, uint2, std::max<uint32_t>(nsurv, 1), mid);
       HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));  // counters[2..7] stay 0 for the rounds
       if (nsurv)
-        hipLaunchKernelGGL(k_midcheck, dim3(blocks_for(nsurv, kBlk)), dim3(kBlk), 0, h->stream, surv, nsurv,
-                           tmh, tbytes, cmh, cbytes, cprune, cfg->hits, cfg->use_mid, mid,
+        hipLaunchKernelGGL(k_midcheck, dim3(blocks_for(nsurv, kBlk)), dim3(kBlk), 0, h->stream, surv, regions,
+                           nsurv, tmh, tbytes, cmh, cbytes, cprune, cfg->hits, cfg->use_mid, mid,
                            std::max<uint32_t>(nsurv, 1), counters);
       hipLaunchKernelGGL(k_seg_clear, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, rej, done,
                          cfg->hits ? nullptr : rule_flag, nt * 2);
