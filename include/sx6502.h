@@ -98,7 +98,8 @@ typedef struct sx_enum_cfg {
   uint64_t n_targets;
   uint64_t first_target;  /* with targets == NULL: half-open range of the */
   uint64_t end_target;    /* pruned target index space (0, 0 = everything) */
-  uint32_t tile_targets;  /* QuickCheck targets per tile, grid launch (0 = autotuned shape) */
+  uint32_t tile_targets;  /* QuickCheck targets per tile, grid launch, rounded up to a multiple of 32
+                             (0 = autotuned shape) */
   uint32_t survivor_cap;  /* QuickCheck survivor buffer entries per segment, split over 8 regions
                              (0 = default); a full region re-runs the segment with a larger buffer */
 } sx_enum_cfg;
