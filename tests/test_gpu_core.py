@@ -212,6 +212,16 @@ def test_survivor_overflow_is_recovered(dev):
     same_rules(got, gold)
 
 
+def test_odd_caller_tile_is_rounded(dev):
+    # a tile that is not a whole number of QuickCheck scan steps is rounded
+    # up; the rules do not depend on the tile
+    targets = load("c1_targets")
+    gold = load("c1_go_deadFF")
+    got, st = dev.enumerate(targets, dead=0xFF, tile=100)
+    assert st["qc_tile"] == 128
+    same_rules(got, gold)
+
+
 def test_pruned_target_list(dev):
     got = dev.pruned_targets()
     assert got.shape == (7_633_612, 2)
