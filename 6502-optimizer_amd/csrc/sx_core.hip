@@ -1496,6 +1496,10 @@ struct sx_handle {
   int bulk_after = 1;     // first-hit rounds before the one-batch tail (SX_BULK_AFTER); -1 = by survivor count
   int exh_k = 1;          // states (pairs when packed) per lane and opcode dispatch in the sweep (SX_EXH_K: 1 or 2)
   int exh_pk = 1;         // packed two-states-per-register interpreter in the sweep (SX_EXH_PK=0: one state)
+  // the plan table in B_PLANTAB holds the plans of (sweep mode, relaxed mask)
+  // = (ptab_mode, ptab_pm); -1: none yet
+  int ptab_mode = -1;
+  uint32_t ptab_pm = 0;
   int seg_log = 22;       // targets per segment = 2^seg_log (SX_SEG_LOG, 16..24)
   uint32_t surv_cap_hint = 0;  // QuickCheck survivor buffer size that sufficed before
   // kernel-only timing of the QuickCheck and exhaustive-sweep launches
@@ -2184,12 +2188,18 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     HIPCHK(h, hipGetLastError());
   }
   // plans by opcode for length-2 targets (whose candidates are single instructions)
+  // (a function of the ISA, the sweep mode and the mask only, so the handle
+  // keeps it for the next call with the same two)
   PlanEntry *ptab = nullptr;
   if (tl == 2 && ml == 1) {
-    ENSURE(B_PLANTAB, PlanEntry, kPlanTab, ptab);
-    hipLaunchKernelGGL(k_plan_table, dim3(blocks_for(kPlanTab, 256)), dim3(256), 0, h->stream, cfg->exh_mode,
-                       pm_masked, ptab);
-    HIPCHK(h, hipGetLastError());
+    ENSURE(B_PLANTAB, PlanEntry, kPlanTab, ptab);  // fixed size: allocated once
+    if (h->ptab_mode != cfg->exh_mode || h->ptab_pm != pm_masked) {
+      hipLaunchKernelGGL(k_plan_table, dim3(blocks_for(kPlanTab, 256)), dim3(256), 0, h->stream, cfg->exh_mode,
+                         pm_masked, ptab);
+      HIPCHK(h, hipGetLastError());
+      h->ptab_mode = cfg->exh_mode;
+      h->ptab_pm = pm_masked;
+    }
   }
 
   // ---- segments of targets
