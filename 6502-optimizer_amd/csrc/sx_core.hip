@@ -2525,7 +2525,14 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     } else {
       size_t old = h->rules.size();
       h->rules.resize(old + nr);
-      if (nr) HIPCHK(h, hipMemcpy(h->rules.data() + old, rules_out, nr * sizeof(sx_rule), hipMemcpyDeviceToHost));
+      if (nr) {
+        // on the library's stream: k_fix_raw_index above may still be
+        // rewriting these rules (a plain hipMemcpy on the null stream would
+        // not wait for this non-blocking stream)
+        HIPCHK(h, hipMemcpyAsync(h->rules.data() + old, rules_out, nr * sizeof(sx_rule), hipMemcpyDeviceToHost,
+                                 h->stream));
+        SYNC_WD("rule copy");
+      }
     }
     float a = 0, b = 0, c = 0;
     HIPCHK(h, hipEventElapsedTime(&a, e1, e2));
@@ -2537,9 +2544,9 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
     st.targets += nt;
   }
   uint32_t bad = 0;
-  HIPCHK(h, hipMemcpy(&bad, badop, 4, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpyAsync(&bad, badop, 4, hipMemcpyDeviceToHost, h->stream));
   {
-    int rc = read_evaluated(h, &st.exh_states);
+    int rc = read_evaluated(h, &st.exh_states);  // synchronises the stream
     if (rc) return rc;
   }
   HIPCHK(h, hipEventSynchronize(es));  // long complete unless there were no targets
