@@ -36,6 +36,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1428,6 +1429,39 @@ __global__ __launch_bounds__(256) void k_pair_vectors(const uint32_t *tw, const 
   }
 }
 
+// Exhaustive jobs of sx_check_pairs' pairs built on the device (4 words per
+// sequence; words beyond a sequence's length are zeroed).
+__global__ void k_pair_jobs(const uint32_t *tw, const uint8_t *tl, const uint32_t *cw, const uint8_t *cl, uint32_t n,
+                            uint32_t pm, Job *jobs) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Job j{};
+  const uint32_t a = tl[i], b = cl[i];
+  for (uint32_t k = 0; k < 4; k++) {
+    j.tw[k] = k < a ? tw[(uint64_t)i * 4 + k] : 0u;
+    j.cw[k] = k < b ? cw[(uint64_t)i * 4 + k] : 0u;
+  }
+  j.tl = (uint8_t)a;
+  j.cl = (uint8_t)b;
+  j.pass = 1;
+  j.pmask = (uint8_t)pm;
+  j.target = i;
+  jobs[i] = j;
+}
+
+// sx_check_pairs' per-pair result bits from the stage outputs (qc, mid: 1 =
+// holds; fail: 0 = the exhaustive check passed); null = stage not run.
+__global__ void k_pair_bits(const uint8_t *qc, const uint8_t *mid, const uint32_t *fail, uint32_t n,
+                            uint32_t stage_mask, uint8_t *out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t v = 0;
+  if ((stage_mask & SX_STAGE_QC) && qc && qc[i]) v |= SX_STAGE_QC;
+  if ((stage_mask & SX_STAGE_MID) && mid && mid[i]) v |= SX_STAGE_MID;
+  if ((stage_mask & SX_STAGE_EXH) && fail && !fail[i]) v |= SX_STAGE_EXH;
+  out[i] = (uint8_t)v;
+}
+
 __global__ void k_exec_states(uint8_t *states, const uint32_t *words, const uint8_t *len, uint64_t n,
                               uint32_t *bad_op) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1541,6 +1575,10 @@ struct sx_handle {
   uint64_t n_l2 = 0;
   // buffers reused across calls
   DevBuf bufs[64];
+  // device address of the no-alloc scratch array on this handle's device
+  // (null when the lookup failed; scratch_rc says why)
+  uint8_t *scratch = nullptr;
+  hipError_t scratch_rc = hipSuccess;
 };
 
 namespace {
@@ -1869,6 +1907,31 @@ int tune_qc(sx_handle *h, int tl, int ml, const uint32_t *tv0, const uint2 *tfp,
   return SX_OK;
 }
 
+// One launch of the sweep over chunks [c0, c0 + n) (a wave per chunk) in the
+// handle's interpreter variant.  chunk_job may be null (the waves then find
+// their job by searching the chunk offsets).
+int launch_exh_run(sx_handle *h, const Job *jobs, uint32_t nj, const uint64_t *offs, const uint32_t *chunk_job,
+                   uint64_t c0, uint64_t n, uint32_t *fail, unsigned long long *evald) {
+  const dim3 grid((unsigned)((n + 3) / 4));
+  if (h->exh_pk) {
+    if (h->exh_k == 1)
+      hipLaunchKernelGGL((k_exh_run<1, true>), grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n,
+                         fail, evald);
+    else
+      hipLaunchKernelGGL((k_exh_run<2, true>), grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n,
+                         fail, evald);
+  } else {
+    if (h->exh_k == 1)
+      hipLaunchKernelGGL((k_exh_run<1, false>), grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n,
+                         fail, evald);
+    else
+      hipLaunchKernelGGL((k_exh_run<2, false>), grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n,
+                         fail, evald);
+  }
+  HIPCHK(h, hipGetLastError());
+  return SX_OK;
+}
+
 // Run the exhaustive stage on nj jobs already in B_JOBS; the verdicts are in
 // B_FAIL (0 = pass).  *chunks_out gets the number of chunks (statistics).
 // planned: the jobs' plans, chunk counts and fail flags are already written
@@ -1914,23 +1977,8 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, bo
     size_t ts = 0;
     rc = timed_begin(h, 1, &ts);
     if (rc) return rc;
-    const dim3 grid((unsigned)((n + 3) / 4));
-    if (h->exh_pk) {
-      if (h->exh_k == 1)
-        hipLaunchKernelGGL((k_exh_run<1, true>), grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n,
-                           fail, evald);
-      else
-        hipLaunchKernelGGL((k_exh_run<2, true>), grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n,
-                           fail, evald);
-    } else {
-      if (h->exh_k == 1)
-        hipLaunchKernelGGL((k_exh_run<1, false>), grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n,
-                           fail, evald);
-      else
-        hipLaunchKernelGGL((k_exh_run<2, false>), grid, dim3(256), 0, h->stream, jobs, nj, offs, chunk_job, c0, n,
-                           fail, evald);
-    }
-    HIPCHK(h, hipGetLastError());
+    rc = launch_exh_run(h, jobs, nj, offs, chunk_job, c0, n, fail, evald);
+    if (rc) return rc;
     rc = timed_end(h, ts);
     if (rc) return rc;
   }
@@ -2008,6 +2056,67 @@ extern "C" int sx_measure_int32_rate(sx_handle *h, double *ops_per_s) {
   return SX_OK;
 }
 
+// ---- sx_check_pairs_noalloc: the pair check without device allocations.
+// Its buffers live in `scratch`, a file-scope device array of fixed size
+// that every GPU the library runs on holds until the process ends.  Host
+// code never passes `scratch` itself to a kernel or a copy (in host code the
+// name is a host-side stand-in, not a device address): sx_open looks up the
+// array's address on its device with hipGetSymbolAddress, once per device,
+// and the entry point uses that pointer.  All calls on one GPU share the
+// array, so calls that overlap on one GPU race.
+constexpr uint64_t kNaMaxPairs = SX_NOALLOC_MAX_PAIRS;  // pairs per call
+constexpr size_t na_align(size_t x) { return (x + 255) & ~(size_t)255; }
+struct NaLayout {  // byte offsets into the scratch array
+  size_t tw, cw, tl, cl, qc, mid, out, jobs, chunks, offs, fail, sums, total, evald, bytes;
+};
+constexpr NaLayout na_layout() {
+  const size_t n = kNaMaxPairs;
+  NaLayout L{};
+  size_t o = 0;
+  L.tw = o;     o += na_align(16 * n);
+  L.cw = o;     o += na_align(16 * n);
+  L.tl = o;     o += na_align(n);
+  L.cl = o;     o += na_align(n);
+  L.qc = o;     o += na_align(n);
+  L.mid = o;    o += na_align(n);
+  L.out = o;    o += na_align(n);
+  L.jobs = o;   o += na_align(sizeof(Job) * n);
+  L.chunks = o; o += na_align(4 * (n + 1));
+  L.offs = o;   o += na_align(8 * (n + 1));
+  L.fail = o;   o += na_align(4 * (n + 1));
+  L.sums = o;   o += na_align(8 * (n / 1024 + 2));
+  L.total = o;  o += na_align(8);
+  L.evald = o;  o += na_align(8 * kEvalShards);
+  L.bytes = o;
+  return L;
+}
+constexpr NaLayout kNa = na_layout();
+static_assert(kNa.bytes <= (12u << 20), "the no-alloc scratch stays at 12 MiB");
+static __device__ __align__(256) uint8_t scratch[kNa.bytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];      // per device: the device address of `scratch`
+hipError_t g_scratch_rc[kMaxDevices];  // the lookup's status (valid once g_scratch_ptr or g_scratch_tried is set)
+bool g_scratch_tried[kMaxDevices];
+
+// With `device` current: look up the address of `scratch` on it (once per
+// device; later handles on the device reuse the result).
+hipError_t scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_tried[device]) {
+    void *p = nullptr;
+    g_scratch_rc[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (g_scratch_rc[device] != hipSuccess) (void)hipGetLastError();
+    else g_scratch_ptr[device] = p;
+    g_scratch_tried[device] = true;
+  }
+  return g_scratch_rc[device];
+}
+}  // namespace
+
 extern "C" int sx_open(int device, sx_handle **out) {
   if (!out) return SX_EINVAL;
   *out = nullptr;
@@ -2024,6 +2133,9 @@ extern "C" int sx_open(int device, sx_handle **out) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->num_cu = prop.multiProcessorCount;
+  h->err.reserve(512);  // set_err's messages then fit without a heap allocation
+  h->scratch_rc = scratch_init(device);
+  h->scratch = h->scratch_rc == hipSuccess ? static_cast<uint8_t *>(g_scratch_ptr[device]) : nullptr;
   if (const char *e = getenv("SX_BULK_AFTER")) h->bulk_after = std::max(-1, atoi(e));
   if (const char *e = getenv("SX_EXH_K")) h->exh_k = atoi(e) == 1 ? 1 : 2;
   if (const char *e = getenv("SX_EXH_PK")) h->exh_pk = atoi(e) != 0;
@@ -2695,6 +2807,83 @@ extern "C" int sx_check_pairs(sx_handle *h, const uint32_t *t_words, const uint8
     if ((stage_mask & SX_STAGE_EXH) && e[i]) v |= SX_STAGE_EXH;
     out[i] = v;
   }
+  return SX_OK;
+}
+
+// sx_check_pairs with every device buffer in the scratch array: no hipMalloc,
+// hipFree or other allocation (host or device), for callers that cannot
+// allocate.  Same arguments and result bits; n <= kNaMaxPairs.
+extern "C" int sx_check_pairs_noalloc(sx_handle *h, const uint32_t *t_words, const uint8_t *t_len,
+                                      const uint32_t *c_words, const uint8_t *c_len, uint64_t n,
+                                      uint32_t stage_mask, uint32_t dead_flags, int32_t exh_mode, uint8_t *out) {
+  if (!h || (n && (!t_words || !t_len || !c_words || !c_len || !out))) return SX_EINVAL;
+  if (h->poisoned) return set_err(h, SX_EHIP, "handle unusable after a device timeout");
+  if (dead_flags > 0xFF || exh_mode < 0 || exh_mode > 1 || (stage_mask & ~7u)) return set_err(h, SX_EINVAL, "bad argument");
+  if (n > kNaMaxPairs)
+    return set_err(h, SX_EINVAL, "sx_check_pairs_noalloc: %llu pairs exceed its maximum of %llu per call",
+                   (unsigned long long)n, (unsigned long long)kNaMaxPairs);
+  if (!h->scratch)
+    return set_err(h, SX_EHIP, "scratch array address unavailable: %s", hipGetErrorString(h->scratch_rc));
+  for (uint64_t i = 0; i < n; i++) {
+    if (t_len[i] > 4 || c_len[i] > 4) return set_err(h, SX_EINVAL, "sequence longer than 4");
+    for (int k = 0; k < t_len[i]; k++)
+      if ((t_words[i * 4 + k] & 0xFFFFu) >= OP_COUNT) return set_err(h, SX_EOPCODE, "opcode out of range");
+    for (int k = 0; k < c_len[i]; k++)
+      if ((c_words[i * 4 + k] & 0xFFFFu) >= OP_COUNT) return set_err(h, SX_EOPCODE, "opcode out of range");
+  }
+  if (n == 0) return SX_OK;
+  HIPCHK(h, hipSetDevice(h->device));
+  uint8_t *const S = h->scratch;  // never the array's own name (see above)
+  uint32_t *dtw = reinterpret_cast<uint32_t *>(S + kNa.tw), *dcw = reinterpret_cast<uint32_t *>(S + kNa.cw);
+  uint8_t *dtl = S + kNa.tl, *dcl = S + kNa.cl, *dqc = S + kNa.qc, *dmid = S + kNa.mid, *dout = S + kNa.out;
+  Job *jobs = reinterpret_cast<Job *>(S + kNa.jobs);
+  uint32_t *chunks = reinterpret_cast<uint32_t *>(S + kNa.chunks), *fail = reinterpret_cast<uint32_t *>(S + kNa.fail);
+  uint64_t *offs = reinterpret_cast<uint64_t *>(S + kNa.offs), *sums = reinterpret_cast<uint64_t *>(S + kNa.sums);
+  uint64_t *total = reinterpret_cast<uint64_t *>(S + kNa.total);
+  unsigned long long *evald = reinterpret_cast<unsigned long long *>(S + kNa.evald);
+  const uint32_t nn = (uint32_t)n, pm = kPMask & ~dead_flags;
+  HIPCHK(h, hipMemcpyAsync(dtw, t_words, 16 * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(dcw, c_words, 16 * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(dtl, t_len, n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(dcl, c_len, n, hipMemcpyHostToDevice, h->stream));
+  const bool vec = (stage_mask & (SX_STAGE_QC | SX_STAGE_MID)) != 0, exh = (stage_mask & SX_STAGE_EXH) != 0;
+  if (vec) {
+    // k_pair_vectors runs each sequence for its own length only
+    hipLaunchKernelGGL(k_pair_vectors, dim3(blocks_for(n, 4)), dim3(256), 0, h->stream, dtw, dtl, dcw, dcl, n, pm,
+                       dqc, dmid, (uint8_t *)nullptr);
+    HIPCHK(h, hipGetLastError());
+  }
+  if (exh) {
+    // the array keeps the last call's contents: the evaluated-state shards
+    // start at zero (a fill kernel); the plan kernel writes every chunk count
+    // and clears every fail flag
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(2 * kEvalShards, 256)), dim3(256), 0, h->stream,
+                       reinterpret_cast<uint32_t *>(evald), (uint64_t)(2 * kEvalShards), 0u);
+    hipLaunchKernelGGL(k_pair_jobs, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, dtw, dtl, dcw, dcl, nn, pm,
+                       jobs);
+    hipLaunchKernelGGL(k_exh_plan, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, jobs, nn, (int)exh_mode,
+                       (const PlanEntry *)nullptr, pm, chunks, fail);
+    const uint64_t nb = (n + 1023) / 1024;
+    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(1024), 0, h->stream, chunks, n, offs, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, h->stream, sums, nb, total);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(1024), 0, h->stream, offs, n, sums);
+    HIPCHK(h, hipGetLastError());
+    uint64_t tot = 0;
+    HIPCHK(h, hipMemcpyAsync(&tot, total, sizeof(tot), hipMemcpyDeviceToHost, h->stream));
+    SYNC_WD("exhaustive plan");
+    // no chunk -> job map (its size depends on the sweeps): each wave finds
+    // its job by searching the chunk offsets
+    const uint64_t slice = 1ull << 20;
+    for (uint64_t c0 = 0; c0 < tot; c0 += slice) {
+      int rc = launch_exh_run(h, jobs, nn, offs, nullptr, c0, std::min<uint64_t>(slice, tot - c0), fail, evald);
+      if (rc) return rc;
+    }
+  }
+  hipLaunchKernelGGL(k_pair_bits, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, vec ? dqc : nullptr,
+                     vec ? dmid : nullptr, exh ? fail : nullptr, nn, stage_mask, dout);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(out, dout, n, hipMemcpyDeviceToHost, h->stream));
+  SYNC_WD("device sync");
   return SX_OK;
 }
 

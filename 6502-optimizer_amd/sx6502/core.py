@@ -26,7 +26,8 @@ EXPORTS = ("sx_abi_version", "sx_open", "sx_close", "sx_last_error", "sx_enumera
            "sx_dep_table", "sx_stoke_run", "sx_mlp_param_count", "sx_mlp_create", "sx_mlp_destroy",
            "sx_mlp_get_params", "sx_mlp_set_params", "sx_mlp_loss_grad", "sx_mlp_train", "sx_mlp_predict",
            "sx_proposal_weights", "sx_mlp_proposal", "sx_mfma_gemm_test", "sx_target_search",
-           "sx_measure_int32_rate")
+           "sx_measure_int32_rate", "sx_check_pairs_noalloc")
+NOALLOC_MAX_PAIRS = 65536
 MLP_CLASSES = 2863   # instructions
 MLP_OUTPUTS = 2871   # instructions, then 8 context-position pointers
 MLP_CTX = 8
@@ -129,6 +130,7 @@ def lib() -> C.CDLL:
     L.sx_pruned_targets.argtypes = [vp, C.c_uint64, C.c_uint64, u32p, C.POINTER(C.c_uint64)]
     L.sx_check_pairs.argtypes = [vp, u32p, u8p, u32p, u8p, C.c_uint64, C.c_uint32, C.c_uint32,
                                  C.c_int32, u8p]
+    L.sx_check_pairs_noalloc.argtypes = L.sx_check_pairs.argtypes
     L.sx_flag_diff.argtypes = [vp, u32p, u8p, u32p, u8p, C.c_uint64, u8p]
     L.sx_exec_states.argtypes = [vp, u8p, u32p, u8p, C.c_uint64]
     L.sx_exh_plan.argtypes = [u32p, C.c_int, u32p, C.c_int, C.c_uint32, C.c_int32, u8p,
@@ -305,6 +307,21 @@ class Device:
         rc = lib().sx_check_pairs(self._h, _p32(tw), _p8(tl), _p32(cw), _p8(cl), n, stages, dead, exh_mode,
                                   _p8(out))
         self._check(rc, "sx_check_pairs")
+        return out
+
+    def check_pairs_noalloc(self, targets, cands, stages: int = STAGE_QC | STAGE_MID | STAGE_EXH, dead: int = 0,
+                            exh_mode: int = EXH_PROJECTED) -> np.ndarray:
+        """check_pairs through sx_check_pairs_noalloc (no allocation in the
+        library; at most NOALLOC_MAX_PAIRS pairs)."""
+        tw, tl = pack_seqs(targets) if not isinstance(targets, tuple) else targets
+        cw, cl = pack_seqs(cands) if not isinstance(cands, tuple) else cands
+        tw, cw = np.ascontiguousarray(tw, np.uint32), np.ascontiguousarray(cw, np.uint32)
+        tl, cl = np.ascontiguousarray(tl, np.uint8), np.ascontiguousarray(cl, np.uint8)
+        n = len(tl)
+        out = np.zeros(n, np.uint8)
+        rc = lib().sx_check_pairs_noalloc(self._h, _p32(tw), _p8(tl), _p32(cw), _p8(cl), n, stages, dead,
+                                          exh_mode, _p8(out))
+        self._check(rc, "sx_check_pairs_noalloc")
         return out
 
     def flag_diff(self, targets, cands) -> np.ndarray:

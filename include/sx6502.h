@@ -147,6 +147,20 @@ int sx_check_pairs(sx_handle *h, const uint32_t *t_words, const uint8_t *t_len,
                    const uint32_t *c_words, const uint8_t *c_len, uint64_t n,
                    uint32_t stage_mask, uint32_t dead_flags, int32_t exh_mode, uint8_t *out);
 
+/* sx_check_pairs without any allocation.  No counterpart in the reference:
+ * this variant is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It calls
+ * no hipMalloc, hipFree or other allocating routine; its device buffers live
+ * in one fixed array of about 10.4 MiB that each GPU the library runs on
+ * holds until the process ends.  Same arguments and result bits as
+ * sx_check_pairs; at most SX_NOALLOC_MAX_PAIRS pairs per call (more give
+ * SX_EINVAL).  Every call on one GPU shares that GPU's array, so calls that
+ * overlap on one GPU (from any handle or thread) race. */
+#define SX_NOALLOC_MAX_PAIRS 65536u
+int sx_check_pairs_noalloc(sx_handle *h, const uint32_t *t_words, const uint8_t *t_len,
+                           const uint32_t *c_words, const uint8_t *c_len, uint64_t n,
+                           uint32_t stage_mask, uint32_t dead_flags, int32_t exh_mode, uint8_t *out);
+
 /* Shortest replacement of one target of 1..8 instructions, candidates of
  * length 1..max_cand_len (<= 4) enumerated on the device in Go order.
  * Replaces search.SearchSingle / SearchSingleMasked (pkg/search/search.go:
