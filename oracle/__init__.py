@@ -55,7 +55,7 @@ class Chain(C.Structure):
 class SearchCfg(C.Structure):
     _fields_ = [("target_len", C.c_int), ("max_cand_len", C.c_int), ("bytes_model", C.c_int),
                 ("hits", C.c_int), ("exh_mode", C.c_int), ("use_mid", C.c_int),
-                ("nthreads", C.c_int), ("dead", C.c_uint8)]
+                ("nthreads", C.c_int), ("dead", C.c_uint8), ("include_empty", C.c_int)]
 
 
 RULE_DTYPE = np.dtype([
@@ -85,6 +85,15 @@ def lib() -> C.CDLL:
             getattr(L, fn).restype = C.c_int
         L.sxo_exhaustive_check.argtypes = [u32p, C.c_int, u32p, C.c_int, C.c_uint8, C.c_int]
         L.sxo_exhaustive_check.restype = C.c_int
+        L.sxo_exhaustive_cex.argtypes = [u32p, C.c_int, u32p, C.c_int, C.c_uint8, C.c_int, u8p]
+        L.sxo_exhaustive_cex.restype = C.c_int
+        L.sxo_exhaustive_brute.argtypes = [u32p, C.c_int, u32p, C.c_int, C.c_uint8, C.c_int, u8p]
+        L.sxo_exhaustive_brute.restype = C.c_int
+        L.sxo_exec_block.argtypes = [u8p, C.c_int, C.c_uint32]
+        L.sxo_dep_units.argtypes = [C.c_int, C.POINTER(C.c_uint16)]
+        L.sxo_dep_units.restype = C.c_int
+        L.sxo_cuda_fallback_pairs.argtypes = [C.c_int, u32p, C.c_int64, C.c_int]
+        L.sxo_cuda_fallback_pairs.restype = C.c_int64
         L.sxo_flag_diff.argtypes = [u32p, C.c_int, u32p, C.c_int]
         L.sxo_flag_diff.restype = C.c_uint8
         L.sxo_fingerprint.argtypes = [u32p, C.c_int, u8p]
@@ -160,11 +169,60 @@ def mid_check(t, c, dead: int = 0) -> bool:
     return bool(lib().sxo_mid_check(ta, tn, ca, cn, dead))
 
 
+EXH_GO, EXH_BRUTE, EXH_COMPLETE = 0, 1, 2
+
+
 def exhaustive_check(t, c, dead: int = 0, mode: int = 0) -> bool:
-    """mode 0 = Go (reduced sweep for >= 3 extra registers), 1 = complete."""
+    """mode 0 = Go (reduced sweep for >= 3 extra registers), 1 = complete by
+    brute force over the whole domain, 2 = complete by projection."""
     ta, tn = _seq(t)
     ca, cn = _seq(c)
     return bool(lib().sxo_exhaustive_check(ta, tn, ca, cn, dead, mode))
+
+
+def exhaustive_cex(t, c, dead: int = 0, mode: int = 0):
+    """(passes, counterexample state or None) of sxo_exhaustive_cex."""
+    ta, tn = _seq(t)
+    ca, cn = _seq(c)
+    cex = np.zeros(8, np.uint8)
+    ok = bool(lib().sxo_exhaustive_cex(ta, tn, ca, cn, dead, mode, cex.ctypes.data_as(C.POINTER(C.c_uint8))))
+    return ok, (None if ok else cex)
+
+
+def exhaustive_brute(t, c, dead: int = 0, nthreads: int = 8):
+    """(passes, counterexample or None): the complete brute-force sweep over
+    the whole domain on nthreads threads (block execution)."""
+    ta, tn = _seq(t)
+    ca, cn = _seq(c)
+    cex = np.zeros(8, np.uint8)
+    ok = bool(lib().sxo_exhaustive_brute(ta, tn, ca, cn, dead, nthreads, cex.ctypes.data_as(C.POINTER(C.c_uint8))))
+    return ok, (None if ok else cex)
+
+
+def exec_block(states: np.ndarray, w: int) -> np.ndarray:
+    """One instruction word on n states through the block executor."""
+    st = np.ascontiguousarray(states, dtype=np.uint8).copy()
+    lib().sxo_exec_block(st.ctypes.data_as(C.POINTER(C.c_uint8)), len(st), w)
+    return st
+
+
+def dep_units(op: int) -> np.ndarray:
+    """The oracle's dependency table row of opcode op (16 unit masks)."""
+    d = np.zeros(16, np.uint16)
+    if lib().sxo_dep_units(op, d.ctypes.data_as(C.POINTER(C.c_uint16))) != 0:
+        raise ValueError(op)
+    return d
+
+
+def cuda_fallback_pairs(min_extra: int = 3, nthreads: int = 8) -> np.ndarray:
+    """(n, 3) words (t0, t1, c): the full length-2 run's QuickCheck + MidCheck
+    survivors (CUDA-tool semantics, dead flags 0) reading >= min_extra extra
+    registers."""
+    n = lib().sxo_cuda_fallback_pairs(min_extra, None, 0, nthreads)
+    out = np.zeros((n, 3), np.uint32)
+    m = lib().sxo_cuda_fallback_pairs(min_extra, out.ctypes.data_as(C.POINTER(C.c_uint32)), n, nthreads)
+    assert m == n
+    return out
 
 
 def flag_diff(t, c) -> int:
@@ -234,15 +292,18 @@ def enumerate_pruned(length: int) -> np.ndarray:
 
 
 def search(targets: np.ndarray, *, max_cand_len: int = 1, dead: int = 0, bytes_model: int = 0,
-           hits: int = 0, exh_mode: int = 0, use_mid: int = 0, nthreads: int | None = None):
+           hits: int = 0, exh_mode: int = 0, use_mid: int = 0, nthreads: int | None = None,
+           include_empty: bool = False):
     """Run the restated worker pool over ``targets`` (shape (n, len) uint32).
 
     Returns (rules as a RULE_DTYPE array sorted by target index then pass, Stats).
+    With include_empty the empty-candidate rules (rep_len 0, a stream of
+    their own) come first within their (target, pass).
     """
     targets = np.ascontiguousarray(targets, dtype=np.uint32)
     n, tl = targets.shape
     cfg = SearchCfg(tl, max_cand_len, bytes_model, hits, exh_mode, use_mid,
-                    nthreads or os.cpu_count() or 1, dead)
+                    nthreads or os.cpu_count() or 1, dead, 1 if include_empty else 0)
     st = Stats()
     nout = C.c_int64(0)
     tp = targets.ctypes.data_as(C.POINTER(C.c_uint32))

@@ -515,54 +515,343 @@ typedef union st8 {
   uint64_t q;
 } st8;
 
-/* verifier.go:262-415 (dead == 0) and :457-588 (dead != 0).  The domain is
- * A = 0..255, P = carry 0/1, S = 0xFD, the other bytes 0, with the extra
- * registers read by either side swept on top (setReg, verifier.go:322-337).
- * mode 0 follows Go: every value of up to two extra registers
- * (exhaustiveFullSweep), 32 representative values each for three or more
- * (exhaustiveReducedSweep).  mode 1 sweeps all 256 values of every extra
- * register (complete).  The visiting order differs from Go's nested loops,
- * which does not change the boolean result. */
-int sxo_exhaustive_check(const uint32_t *t, int tn, const uint32_t *c, int cn,
-                         uint8_t dead, int mode) {
-  int reads = sxo_regs_read(t, tn) | sxo_regs_read(c, cn);
-  int ex[6];
-  int k = (reads & ~(SXO_RA | SXO_RP)) ? sxo_extra_regs(reads, ex) : 0;
-  const uint8_t *vals = NULL;
-  int nvals = 256;
-  if (k >= 3 && mode == 0) {
-    vals = k_rep;
-    nvals = 32;
-  }
-  int eb[6];
-  for (int i = 0; i < k; i++) eb[i] = k_off_to_byte[ex[i]];
-  /* compare mask: every byte, P under PMask &^ dead */
+/* Masked compare of two states: every byte, P under PMask &^ dead
+ * (verifier.go:418-425). */
+static inline int st_equal(const st8 *a, const st8 *b, uint8_t dead) {
   st8 msk;
   msk.q = ~(uint64_t)0;
   msk.b[SXO_P] = (uint8_t)(PMASK & ~dead);
+  return ((a->q ^ b->q) & msk.q) == 0;
+}
+
+/* Run both sequences on one state; 1 when the outputs agree. */
+static inline int pair_agrees(const st8 *in, const uint32_t *t, int tn, const uint32_t *c, int cn,
+                              uint8_t dead) {
+  st8 ts = *in, cs = *in;
+  for (int i = 0; i < tn; i++) exec_one(ts.b, SXO_OP(t[i]), SXO_IMM(t[i]));
+  for (int i = 0; i < cn; i++) exec_one(cs.b, SXO_OP(c[i]), SXO_IMM(c[i]));
+  return st_equal(&ts, &cs, dead);
+}
+
+/* The domain of verifier.go:262-415: A = 0..255, P = carry 0/1 (other flags
+ * 0), S = 0xFD unless read, X, Y, M, S0, S1 = 0 unless read; the registers
+ * either side reads (opReads) are swept on top.  Returns the state bytes it
+ * varies as a bit set (bit r = state byte r; P stands for the carry). */
+static int domain_bytes(const uint32_t *t, int tn, const uint32_t *c, int cn) {
+  int reads = sxo_regs_read(t, tn) | sxo_regs_read(c, cn);
+  int v = (1 << SXO_A) | (1 << SXO_P);
+  if (reads & SXO_RX) v |= 1 << SXO_X;
+  if (reads & SXO_RY) v |= 1 << SXO_Y;
+  if (reads & SXO_RM) v |= 1 << SXO_M;
+  if (reads & SXO_RS) v |= 1 << SXO_S;
+  if (reads & SXO_RS0) v |= 1 << SXO_S0;
+  if (reads & SXO_RS1) v |= 1 << SXO_S1;
+  return v;
+}
+
+/* Sweep the bytes of `set` (bit r = state byte r; P = carry 0/1) over their
+ * domain values with every other byte at its domain default; 1 when T and C
+ * agree everywhere, else 0 with the first disagreeing state in cex. */
+static int sweep_set(const uint32_t *t, int tn, const uint32_t *c, int cn, uint8_t dead, int set,
+                     uint8_t cex[8]) {
+  int regs[8], k = 0;
+  for (int r = 0; r < 8; r++)
+    if (set & (1 << r)) regs[k++] = r;
+  int odo[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  st8 s;
+  s.q = 0;
+  s.b[SXO_S] = 0xFD;
+  for (;;) {
+    for (int i = 0; i < k; i++) s.b[regs[i]] = (uint8_t)odo[i];
+    if (!pair_agrees(&s, t, tn, c, cn, dead)) {
+      if (cex) memcpy(cex, s.b, 8);
+      return 0;
+    }
+    int i = 0;
+    while (i < k && ++odo[i] == (regs[i] == SXO_P ? 2 : 256)) odo[i++] = 0;
+    if (i == k) return 1;
+  }
+}
+
+/* exec_one over a block of states with one opcode dispatch per block: each
+ * case calls the (inlined) executor with a constant opcode, so its switch
+ * folds away and the loop runs that opcode's code only.  Same semantics as
+ * exec_one by construction. */
+#define BLOCK_CASE(OPC) \
+  case OPC: for (int i = 0; i < n; i++) exec_one(s[i].b, OPC, imm); break;
+static void exec_block(st8 *s, int n, int op, uint16_t imm) {
+  switch (op) {
+  BLOCK_CASE(SXO_TAX) BLOCK_CASE(SXO_TAY) BLOCK_CASE(SXO_TXA) BLOCK_CASE(SXO_TYA) BLOCK_CASE(SXO_TSX)
+  BLOCK_CASE(SXO_TXS) BLOCK_CASE(SXO_INX) BLOCK_CASE(SXO_INY) BLOCK_CASE(SXO_DEX) BLOCK_CASE(SXO_DEY)
+  BLOCK_CASE(SXO_ASL_A) BLOCK_CASE(SXO_LSR_A) BLOCK_CASE(SXO_ROL_A) BLOCK_CASE(SXO_ROR_A)
+  BLOCK_CASE(SXO_CLC) BLOCK_CASE(SXO_SEC) BLOCK_CASE(SXO_CLD) BLOCK_CASE(SXO_SED) BLOCK_CASE(SXO_CLI)
+  BLOCK_CASE(SXO_SEI) BLOCK_CASE(SXO_CLV) BLOCK_CASE(SXO_PHA) BLOCK_CASE(SXO_PLA) BLOCK_CASE(SXO_PHP)
+  BLOCK_CASE(SXO_PLP) BLOCK_CASE(SXO_NOP) BLOCK_CASE(SXO_LDA_IMM) BLOCK_CASE(SXO_LDX_IMM)
+  BLOCK_CASE(SXO_LDY_IMM) BLOCK_CASE(SXO_ADC_IMM) BLOCK_CASE(SXO_SBC_IMM) BLOCK_CASE(SXO_AND_IMM)
+  BLOCK_CASE(SXO_ORA_IMM) BLOCK_CASE(SXO_EOR_IMM) BLOCK_CASE(SXO_CMP_IMM) BLOCK_CASE(SXO_CPX_IMM)
+  BLOCK_CASE(SXO_CPY_IMM) BLOCK_CASE(SXO_LDA_M) BLOCK_CASE(SXO_LDX_M) BLOCK_CASE(SXO_LDY_M)
+  BLOCK_CASE(SXO_STA_M) BLOCK_CASE(SXO_STX_M) BLOCK_CASE(SXO_STY_M) BLOCK_CASE(SXO_ADC_M)
+  BLOCK_CASE(SXO_SBC_M) BLOCK_CASE(SXO_AND_M) BLOCK_CASE(SXO_ORA_M) BLOCK_CASE(SXO_EOR_M)
+  BLOCK_CASE(SXO_CMP_M) BLOCK_CASE(SXO_CPX_M) BLOCK_CASE(SXO_CPY_M) BLOCK_CASE(SXO_INC_M)
+  BLOCK_CASE(SXO_DEC_M) BLOCK_CASE(SXO_ASL_M) BLOCK_CASE(SXO_LSR_M) BLOCK_CASE(SXO_ROL_M)
+  BLOCK_CASE(SXO_ROR_M) BLOCK_CASE(SXO_BIT_M)
+  default: break;
+  }
+}
+#undef BLOCK_CASE
+
+void sxo_exec_block(uint8_t *states, int n, uint32_t w) {
+  exec_block((st8 *)states, n, SXO_OP(w), SXO_IMM(w));
+}
+
+/* Complete brute force over the whole domain (mode 1) with block execution:
+ * the 512 states of A x carry are one block, the extra registers an odometer
+ * split over nthreads threads.  1 = pass; else 0 and, in cex, a state where
+ * the outputs differ (the first one in the odometer order of its thread). */
+typedef struct brute_ctx {
+  const uint32_t *t, *c;
+  int tn, cn;
+  uint8_t dead;
+  int regs[6], k;
+  uint64_t nouter, next;
+  int failed;
+  uint8_t cex[8];
+  pthread_mutex_t mu;
+} brute_ctx;
+
+static void *brute_worker(void *arg) {
+  brute_ctx *w = (brute_ctx *)arg;
+  st8 ts[512], cs[512], base[512];
+  for (int i = 0; i < 512; i++) {
+    base[i].q = 0;
+    base[i].b[SXO_A] = (uint8_t)(i & 255);
+    base[i].b[SXO_P] = (uint8_t)(i >> 8);
+    base[i].b[SXO_S] = 0xFD;
+  }
+  st8 msk;
+  msk.q = ~(uint64_t)0;
+  msk.b[SXO_P] = (uint8_t)(PMASK & ~w->dead);
+  const uint64_t chunk = 256;
+  for (;;) {
+    if (__atomic_load_n(&w->failed, __ATOMIC_RELAXED)) break;
+    uint64_t b = __atomic_fetch_add(&w->next, chunk, __ATOMIC_RELAXED);
+    if (b >= w->nouter) break;
+    uint64_t e = b + chunk < w->nouter ? b + chunk : w->nouter;
+    for (uint64_t o = b; o < e; o++) {
+      for (int i = 0; i < w->k; i++) {
+        const uint8_t v = (uint8_t)(o >> (8 * i));
+        for (int j = 0; j < 512; j++) base[j].b[w->regs[i]] = v;
+      }
+      memcpy(ts, base, sizeof(ts));
+      memcpy(cs, base, sizeof(cs));
+      for (int q = 0; q < w->tn; q++) exec_block(ts, 512, SXO_OP(w->t[q]), SXO_IMM(w->t[q]));
+      for (int q = 0; q < w->cn; q++) exec_block(cs, 512, SXO_OP(w->c[q]), SXO_IMM(w->c[q]));
+      uint64_t bad = 0;
+      for (int j = 0; j < 512; j++) bad |= (ts[j].q ^ cs[j].q) & msk.q;
+      if (bad) {
+        for (int j = 0; j < 512; j++)
+          if ((ts[j].q ^ cs[j].q) & msk.q) {
+            pthread_mutex_lock(&w->mu);
+            if (!w->failed) memcpy(w->cex, base[j].b, 8);
+            w->failed = 1;
+            pthread_mutex_unlock(&w->mu);
+            return NULL;
+          }
+      }
+    }
+  }
+  return NULL;
+}
+
+int sxo_exhaustive_brute(const uint32_t *t, int tn, const uint32_t *c, int cn, uint8_t dead, int nthreads,
+                         uint8_t cex[8]) {
+  brute_ctx w;
+  memset(&w, 0, sizeof(w));
+  w.t = t;
+  w.c = c;
+  w.tn = tn;
+  w.cn = cn;
+  w.dead = dead;
+  const int dom = domain_bytes(t, tn, c, cn);
+  for (int r = 0; r < 8; r++)
+    if (r != SXO_A && r != SXO_P && (dom & (1 << r))) w.regs[w.k++] = r;
+  w.nouter = 1ull << (8 * w.k);
+  pthread_mutex_init(&w.mu, NULL);
+  int nth = nthreads > 0 ? nthreads : 1;
+  pthread_t th[64];
+  if (nth > 64) nth = 64;
+  for (int i = 0; i < nth; i++) pthread_create(&th[i], NULL, brute_worker, &w);
+  for (int i = 0; i < nth; i++) pthread_join(th[i], NULL);
+  pthread_mutex_destroy(&w.mu);
+  if (w.failed && cex) memcpy(cex, w.cex, 8);
+  return !w.failed;
+}
+
+/* Dependency units: state byte r (A=0, X=1, Y=2, S=4, M=5, S0=6, S1=7) and
+ * bit b of P as unit 8 + b.  sxo_dep_units(op, d): d[u] = the input units
+ * that output unit u of opcode op is a function of (an over-approximation),
+ * d[u] = 1 << u where op leaves u unchanged; d[3] (the P byte) = 0.  Written
+ * from the executor above (exec.go:8-327), one line per opcode family. */
+#define UA (1u << SXO_A)
+#define UX (1u << SXO_X)
+#define UY (1u << SXO_Y)
+#define US (1u << SXO_S)
+#define UM (1u << SXO_M)
+#define US0 (1u << SXO_S0)
+#define US1 (1u << SXO_S1)
+#define UP(b) (1u << (8 + (b)))
+#define UC UP(0)
+#define UALLP 0xFF00u
+int sxo_dep_units(int op, uint16_t d[16]) {
+  if (op < 0 || op >= SXO_OPCOUNT) return -1;
+  for (int u = 0; u < 16; u++) d[u] = (uint16_t)(u == 3 ? 0 : 1u << u);
+  uint32_t nz = 0xFFFFFFFFu; /* N and Z come from this set (none when unchanged) */
+#define SET(u, m) d[u] = (uint16_t)(m)
+  switch (op) {
+  case SXO_TAX: SET(SXO_X, UA); nz = UA; break;
+  case SXO_TAY: SET(SXO_Y, UA); nz = UA; break;
+  case SXO_TXA: SET(SXO_A, UX); nz = UX; break;
+  case SXO_TYA: SET(SXO_A, UY); nz = UY; break;
+  case SXO_TSX: SET(SXO_X, US); nz = US; break;
+  case SXO_TXS: SET(SXO_S, UX); break;
+  case SXO_INX: case SXO_DEX: SET(SXO_X, UX); nz = UX; break;
+  case SXO_INY: case SXO_DEY: SET(SXO_Y, UY); nz = UY; break;
+  case SXO_ASL_A: case SXO_LSR_A: SET(SXO_A, UA); SET(8, UA); nz = UA; break;
+  case SXO_ROL_A: case SXO_ROR_A: SET(SXO_A, UA | UC); SET(8, UA); nz = UA | UC; break;
+  case SXO_CLC: case SXO_SEC: SET(8, 0); break;
+  case SXO_CLI: case SXO_SEI: SET(8 + 2, 0); break;
+  case SXO_CLD: case SXO_SED: SET(8 + 3, 0); break;
+  case SXO_CLV: SET(8 + 6, 0); break;
+  case SXO_PHA: SET(SXO_S1, US0); SET(SXO_S0, UA); break;
+  case SXO_PHP: SET(SXO_S1, US0); SET(SXO_S0, UALLP); break;
+  case SXO_PLA: SET(SXO_A, US0); SET(SXO_S0, US1); SET(SXO_S1, 0); nz = US0; break;
+  case SXO_PLP:
+    for (int b = 0; b < 8; b++) SET(8 + b, US0);
+    SET(SXO_S0, US1); SET(SXO_S1, 0);
+    break;
+  case SXO_NOP: break;
+  case SXO_LDA_IMM: SET(SXO_A, 0); nz = 0; break;
+  case SXO_LDX_IMM: SET(SXO_X, 0); nz = 0; break;
+  case SXO_LDY_IMM: SET(SXO_Y, 0); nz = 0; break;
+  case SXO_ADC_IMM: case SXO_SBC_IMM:
+    SET(SXO_A, UA | UC); SET(8, UA | UC); SET(8 + 6, UA | UC); nz = UA | UC; break;
+  case SXO_AND_IMM: case SXO_ORA_IMM: case SXO_EOR_IMM: SET(SXO_A, UA); nz = UA; break;
+  case SXO_CMP_IMM: SET(8, UA); nz = UA; break;
+  case SXO_CPX_IMM: SET(8, UX); nz = UX; break;
+  case SXO_CPY_IMM: SET(8, UY); nz = UY; break;
+  case SXO_LDA_M: SET(SXO_A, UM); nz = UM; break;
+  case SXO_LDX_M: SET(SXO_X, UM); nz = UM; break;
+  case SXO_LDY_M: SET(SXO_Y, UM); nz = UM; break;
+  case SXO_STA_M: SET(SXO_M, UA); break;
+  case SXO_STX_M: SET(SXO_M, UX); break;
+  case SXO_STY_M: SET(SXO_M, UY); break;
+  case SXO_ADC_M: case SXO_SBC_M:
+    SET(SXO_A, UA | UM | UC); SET(8, UA | UM | UC); SET(8 + 6, UA | UM | UC); nz = UA | UM | UC; break;
+  case SXO_AND_M: case SXO_ORA_M: case SXO_EOR_M: SET(SXO_A, UA | UM); nz = UA | UM; break;
+  case SXO_CMP_M: SET(8, UA | UM); nz = UA | UM; break;
+  case SXO_CPX_M: SET(8, UX | UM); nz = UX | UM; break;
+  case SXO_CPY_M: SET(8, UY | UM); nz = UY | UM; break;
+  case SXO_INC_M: case SXO_DEC_M: SET(SXO_M, UM); nz = UM; break;
+  case SXO_ASL_M: case SXO_LSR_M: SET(SXO_M, UM); SET(8, UM); nz = UM; break;
+  case SXO_ROL_M: case SXO_ROR_M: SET(SXO_M, UM | UC); SET(8, UM); nz = UM | UC; break;
+  case SXO_BIT_M: SET(8 + 1, UA | UM); SET(8 + 6, UM); SET(8 + 7, UM); break;
+  }
+  if (nz != 0xFFFFFFFFu) {
+    SET(8 + 1, nz);
+    SET(8 + 7, nz);
+  }
+#undef SET
+  return 0;
+}
+
+/* Output units of a sequence as functions of its input units: compose the
+ * per-opcode table along the sequence (out[u] = input units u depends on). */
+static void seq_deps(const uint32_t *seq, int n, uint32_t out[16]) {
+  for (int u = 0; u < 16; u++) out[u] = u == 3 ? 0 : 1u << u;
+  for (int i = 0; i < n; i++) {
+    uint16_t d[16];
+    sxo_dep_units(SXO_OP(seq[i]), d);
+    uint32_t nd[16];
+    for (int u = 0; u < 16; u++) {
+      nd[u] = 0;
+      for (int q = 0; q < 16; q++)
+        if (d[u] & (1u << q)) nd[u] |= out[q];
+    }
+    memcpy(out, nd, sizeof(nd));
+  }
+}
+
+/* verifier.go:262-415 (dead == 0) and :457-588 (dead != 0).  Returns 1 when
+ * the pair passes, else 0 with a state of the sweep on which the outputs
+ * differ (cex may be NULL).
+ *   mode 0: Go: every value of up to two extra registers
+ *           (exhaustiveFullSweep), 32 representative values each for three or
+ *           more (exhaustiveReducedSweep);
+ *   mode 1: complete, by brute force over the whole domain (2^(9+8k) states);
+ *   mode 2: complete, by projection: every compared output unit (a state byte
+ *           or a flag bit of P under PMask &^ dead) of T and of C is a
+ *           function of the inputs sxo_dep_units composes for it, so T and C
+ *           agree on it over the whole domain iff they agree on the sweep of
+ *           the domain bytes among those inputs (the other bytes at their
+ *           defaults).  Each such sweep compares the whole masked state, so a
+ *           disagreement there is a real counterexample in the domain.
+ * The visiting order differs from Go's nested loops, which does not change
+ * the boolean result. */
+int sxo_exhaustive_cex(const uint32_t *t, int tn, const uint32_t *c, int cn, uint8_t dead, int mode,
+                       uint8_t cex[8]) {
+  const int dom = domain_bytes(t, tn, c, cn);
+  if (mode == 1) return sweep_set(t, tn, c, cn, dead, dom, cex);
+  if (mode == 2) {
+    uint32_t dt[16], dc[16];
+    seq_deps(t, tn, dt);
+    seq_deps(c, cn, dc);
+    /* domain bytes as input units: the carry is unit 8 */
+    const uint32_t dom_units = (uint32_t)(dom & ~(1 << SXO_P)) | UC;
+    int done[16], nd = 0;
+    for (int u = 0; u < 16; u++) {
+      if (u == 3 || (u >= 8 && !((PMASK & ~dead) & (1 << (u - 8))))) continue; /* not compared */
+      const uint32_t in = (dt[u] | dc[u]) & dom_units;
+      const int set = (int)(in & 0xF7u) | ((in & UC) ? 1 << SXO_P : 0);
+      int seen = 0;
+      for (int i = 0; i < nd; i++) seen |= done[i] == set;
+      if (seen) continue;
+      done[nd++] = set;
+      if (!sweep_set(t, tn, c, cn, dead, set, cex)) return 0;
+    }
+    return 1;
+  }
+  /* mode 0 */
+  int reads = sxo_regs_read(t, tn) | sxo_regs_read(c, cn);
+  int ex[6];
+  int k = (reads & ~(SXO_RA | SXO_RP)) ? sxo_extra_regs(reads, ex) : 0;
+  if (k < 3) return sweep_set(t, tn, c, cn, dead, dom, cex);
+  int eb[6];
+  for (int i = 0; i < k; i++) eb[i] = k_off_to_byte[ex[i]];
   int odo[6] = {0, 0, 0, 0, 0, 0};
   st8 base;
   base.q = 0;
   base.b[SXO_S] = 0xFD;
   for (;;) {
-    for (int i = 0; i < k; i++) base.b[eb[i]] = vals ? vals[odo[i]] : (uint8_t)odo[i];
+    for (int i = 0; i < k; i++) base.b[eb[i]] = k_rep[odo[i]];
     for (int carry = 0; carry <= 1; carry++) {
       base.b[SXO_P] = (uint8_t)carry;
       for (int a = 0; a < 256; a++) {
-        st8 ts, cs;
         base.b[SXO_A] = (uint8_t)a;
-        ts.q = base.q;
-        cs.q = base.q;
-        for (int i = 0; i < tn; i++) exec_one(ts.b, SXO_OP(t[i]), SXO_IMM(t[i]));
-        for (int i = 0; i < cn; i++) exec_one(cs.b, SXO_OP(c[i]), SXO_IMM(c[i]));
-        if ((ts.q ^ cs.q) & msk.q) return 0;
+        if (!pair_agrees(&base, t, tn, c, cn, dead)) {
+          if (cex) memcpy(cex, base.b, 8);
+          return 0;
+        }
       }
     }
     int i = 0;
-    while (i < k && ++odo[i] == nvals) odo[i++] = 0;
+    while (i < k && ++odo[i] == 32) odo[i++] = 0;
     if (i == k) break;
   }
   return 1;
+}
+
+int sxo_exhaustive_check(const uint32_t *t, int tn, const uint32_t *c, int cn, uint8_t dead, int mode) {
+  return sxo_exhaustive_cex(t, tn, c, cn, dead, mode, NULL);
 }
 
 /* ------------------------------------------------------ Pruner / enum -- */
@@ -717,6 +1006,30 @@ static void process_target(worker_ctx *w, int64_t ti, sxo_stats *st) {
     sxo_exec_seq(tout[i], t, tn);
   }
   sxo_rule r;
+  if (cfg->include_empty) {
+    /* The empty sequence as a candidate, in a stream of its own: Go never
+     * tries it (worker.go:123 starts at length 1), so it does not take part
+     * in the first-hit order of the other candidates.  It has 0 bytes and
+     * is never pruned; pass 1 is the exact check, pass 2 (dead flags) the
+     * flag-relaxed one with FlagDiff != 0; in the CUDA tool's semantics one
+     * masked pass. */
+    static const cand_info empty = {{0, 0}, 0, 0, 0, 0};
+    const uint8_t d1 = cfg->hits == 1 ? cfg->dead : 0;
+    st->checked++;
+    if (qc_pre(tout, empty.w, 0, d1) && (!(cfg->use_mid || cfg->hits == 1) || sxo_mid_check(t, tn, empty.w, 0, d1)) &&
+        sxo_exhaustive_check(t, tn, empty.w, 0, d1, cfg->exh_mode)) {
+      make_rule(&r, t, tn, &empty, ti, tbytes, tcycles, d1, 1);
+      push_rule(w, &r);
+    } else if (cfg->hits == 0 && cfg->dead && !qc_pre(tout, empty.w, 0, 0) && qc_pre(tout, empty.w, 0, cfg->dead) &&
+               (!cfg->use_mid || sxo_mid_check(t, tn, empty.w, 0, cfg->dead)) &&
+               sxo_exhaustive_check(t, tn, empty.w, 0, cfg->dead, cfg->exh_mode)) {
+      const uint8_t fd = sxo_flag_diff(t, tn, empty.w, 0);
+      if (fd) {
+        make_rule(&r, t, tn, &empty, ti, tbytes, tcycles, fd, 2);
+        push_rule(w, &r);
+      }
+    }
+  }
   if (cfg->hits == 1) {
     /* CUDA tool: one masked pass, every hit (cuda/6502_search_v2.cu:871-974) */
     for (int len = 1; len <= cfg->max_cand_len; len++) {
@@ -876,6 +1189,121 @@ int sxo_search(const sxo_search_cfg *cfg, const uint32_t *targets, int64_t ntarg
   free(w.rules);
   pthread_mutex_destroy(&w.mu);
   return (out && w.nrules > cap) ? -3 : 0;
+}
+
+/* ------------------------------------------- CUDA-tool fallback pairs -- */
+
+typedef struct fb_ctx {
+  const uint32_t *targets; /* pruned length-2 targets, Go order */
+  int64_t ntargets;
+  const uint32_t *cands;   /* 2,863 single instructions, Go order */
+  uint8_t cout[2863][32][8];
+  int creads[2863], cbytes[2863], cprune[2863];
+  int min_extra;
+  int64_t next;
+  pthread_mutex_t mu;
+  uint32_t *pairs; /* (target index, candidate index) */
+  int64_t npairs, cap;
+} fb_ctx;
+
+static int extra_count(int reads) {
+  return __builtin_popcount((unsigned)reads & (SXO_RX | SXO_RY | SXO_RS | SXO_RM | SXO_RS0 | SXO_RS1));
+}
+
+static void *fb_worker(void *arg) {
+  fb_ctx *w = (fb_ctx *)arg;
+  const int64_t chunk = 4096;
+  uint32_t *loc = NULL;
+  int64_t nloc = 0, caploc = 0;
+  for (;;) {
+    int64_t b = __atomic_fetch_add(&w->next, chunk, __ATOMIC_RELAXED);
+    if (b >= w->ntargets) break;
+    int64_t e = b + chunk < w->ntargets ? b + chunk : w->ntargets;
+    for (int64_t ti = b; ti < e; ti++) {
+      const uint32_t *t = w->targets + 2 * ti;
+      const int treads = sxo_regs_read(t, 2);
+      const int tbytes = sxo_byte_size(SXO_OP(t[0]), 1) + sxo_byte_size(SXO_OP(t[1]), 1);
+      uint8_t tout[32][8];
+      int have_out = 0;
+      for (int j = 0; j < 2863; j++) {
+        if (extra_count(treads | w->creads[j]) < w->min_extra) continue;
+        if (w->cbytes[j] >= tbytes || w->cprune[j]) continue; /* cu:877-880 */
+        if (!have_out) {
+          for (int v = 0; v < 32; v++) {
+            memcpy(tout[v], k_mid[v], 8);
+            sxo_exec_seq(tout[v], t, 2);
+          }
+          have_out = 1;
+        }
+        int ok = 1;
+        for (int v = 0; v < 32 && ok; v++) ok = sxo_states_equal_masked(tout[v], w->cout[j][v], 0);
+        if (!ok) continue;
+        if (nloc + 2 > caploc) {
+          caploc = caploc ? 2 * caploc : 4096;
+          loc = (uint32_t *)realloc(loc, (size_t)caploc * sizeof(uint32_t));
+        }
+        loc[nloc++] = (uint32_t)ti;
+        loc[nloc++] = (uint32_t)j;
+      }
+    }
+  }
+  pthread_mutex_lock(&w->mu);
+  if (w->npairs + nloc > w->cap) {
+    w->cap = (w->npairs + nloc) * 2;
+    w->pairs = (uint32_t *)realloc(w->pairs, (size_t)w->cap * sizeof(uint32_t));
+  }
+  if (nloc) memcpy(w->pairs + w->npairs, loc, (size_t)nloc * sizeof(uint32_t));
+  w->npairs += nloc;
+  pthread_mutex_unlock(&w->mu);
+  free(loc);
+  return NULL;
+}
+
+static int pair_cmp(const void *a, const void *b) {
+  const uint32_t *x = (const uint32_t *)a, *y = (const uint32_t *)b;
+  if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
+  return x[1] < y[1] ? -1 : (x[1] > y[1]);
+}
+
+int64_t sxo_cuda_fallback_pairs(int min_extra, uint32_t *out, int64_t cap, int nthreads) {
+  int64_t nt = sxo_enumerate_pruned(2, NULL, 0);
+  uint32_t *targets = (uint32_t *)malloc((size_t)nt * 2 * sizeof(uint32_t));
+  sxo_enumerate_pruned(2, targets, nt);
+  uint32_t cands[2863];
+  sxo_instruction_table(cands, 0);
+  fb_ctx *w = (fb_ctx *)calloc(1, sizeof(fb_ctx));
+  w->targets = targets;
+  w->ntargets = nt;
+  w->cands = cands;
+  w->min_extra = min_extra;
+  for (int j = 0; j < 2863; j++) {
+    w->creads[j] = sxo_op_reads(SXO_OP(cands[j]));
+    w->cbytes[j] = sxo_byte_size(SXO_OP(cands[j]), 1);
+    w->cprune[j] = sxo_should_prune(&cands[j], 1);
+    for (int v = 0; v < 32; v++) {
+      memcpy(w->cout[j][v], k_mid[v], 8);
+      sxo_exec_seq(w->cout[j][v], &cands[j], 1);
+    }
+  }
+  pthread_mutex_init(&w->mu, NULL);
+  int nth = nthreads > 0 ? nthreads : 1;
+  pthread_t *th = (pthread_t *)calloc((size_t)nth, sizeof(pthread_t));
+  for (int i = 0; i < nth; i++) pthread_create(&th[i], NULL, fb_worker, w);
+  for (int i = 0; i < nth; i++) pthread_join(th[i], NULL);
+  free(th);
+  const int64_t n = w->npairs / 2;
+  qsort(w->pairs, (size_t)n, 2 * sizeof(uint32_t), pair_cmp);
+  if (out)
+    for (int64_t i = 0; i < n && i < cap; i++) {
+      out[3 * i] = targets[2 * (int64_t)w->pairs[2 * i]];
+      out[3 * i + 1] = targets[2 * (int64_t)w->pairs[2 * i] + 1];
+      out[3 * i + 2] = cands[w->pairs[2 * i + 1]];
+    }
+  pthread_mutex_destroy(&w->mu);
+  free(w->pairs);
+  free(w);
+  free(targets);
+  return n;
 }
 
 /* ------------------------------------------------------------------ STOKE -- */

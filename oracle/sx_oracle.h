@@ -87,9 +87,24 @@ const uint8_t *sxo_mid_vectors(void);  /* 32 x 8 bytes */
 int sxo_states_equal_masked(const uint8_t *a, const uint8_t *b, uint8_t dead);
 int sxo_quick_check(const uint32_t *t, int tn, const uint32_t *c, int cn, uint8_t dead);
 int sxo_mid_check(const uint32_t *t, int tn, const uint32_t *c, int cn, uint8_t dead);
-/* mode 0 = Go (32-value reduced sweep for >= 3 extra registers), 1 = complete. */
+/* mode 0 = Go (32-value reduced sweep for >= 3 extra registers), 1 =
+ * complete by brute force over the whole domain, 2 = complete by projection
+ * over the dependencies of sxo_dep_units.  sxo_exhaustive_cex also writes a
+ * disagreeing domain state to cex (when not NULL) if the pair fails. */
 int sxo_exhaustive_check(const uint32_t *t, int tn, const uint32_t *c, int cn,
                          uint8_t dead, int mode);
+int sxo_exhaustive_cex(const uint32_t *t, int tn, const uint32_t *c, int cn,
+                       uint8_t dead, int mode, uint8_t cex[8]);
+/* Mode 1 (complete brute force) on nthreads threads, executing 512 states
+ * per opcode dispatch; cex as above.  sxo_exec_block runs one word on n
+ * states (8 bytes each). */
+int sxo_exhaustive_brute(const uint32_t *t, int tn, const uint32_t *c, int cn, uint8_t dead, int nthreads,
+                         uint8_t cex[8]);
+void sxo_exec_block(uint8_t *states, int n, uint32_t w);
+/* d[u] = input units output unit u of opcode op is a function of (units: state
+ * byte r = A 0, X 1, Y 2, S 4, M 5, S0 6, S1 7; flag bit b of P = 8 + b; an
+ * unchanged unit depends on itself).  Returns -1 for an unknown opcode. */
+int sxo_dep_units(int op, uint16_t d[16]);
 uint8_t sxo_flag_diff(const uint32_t *t, int tn, const uint32_t *c, int cn);
 void sxo_fingerprint(const uint32_t *seq, int n, uint8_t fp[64]);
 int sxo_op_reads(int op);
@@ -111,14 +126,24 @@ typedef struct sxo_search_cfg {
   int max_cand_len;   /* 1 or 2 */
   int bytes_model;    /* 0 Go, 1 CUDA */
   int hits;           /* 0 first, 1 all */
-  int exh_mode;       /* 0 Go reduced, 1 complete */
+  int exh_mode;       /* as sxo_exhaustive_check: 0 Go reduced, 1 complete (brute force), 2 complete (projection) */
   int use_mid;        /* first-hit mode: 1 = run MidCheck as a pre-filter */
   int nthreads;
   uint8_t dead;
+  int include_empty;  /* 1: also check the empty sequence per target (rules with rep_len 0, a separate stream) */
 } sxo_search_cfg;
 
 int sxo_search(const sxo_search_cfg *cfg, const uint32_t *targets, int64_t ntargets,
                sxo_rule *out, int64_t cap, int64_t *nout, sxo_stats *stats);
+
+/* The pairs of the CUDA tool's full length-2 run (all pruned length-2
+ * targets x the 2,863 single instructions, CUDA byte sizes, --dead-flags 0,
+ * cuda/6502_search_v2.cu:848-977) that pass its byte filter, candidate
+ * pruner, QuickCheck and MidCheck and read at least min_extra extra
+ * registers (X, Y, M, S, S0, S1) between them -- with min_extra 3, the pairs
+ * the tool sends to its CPU reduced sweep (cu:650,945-951).  out: (t0, t1, c)
+ * words per pair in Go order of (target, candidate); returns the count. */
+int64_t sxo_cuda_fallback_pairs(int min_extra, uint32_t *out, int64_t cap, int nthreads);
 
 /* STOKE ------------------------------------------------------------------ */
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel Random Numbers: As
