@@ -1356,6 +1356,81 @@ __global__ __launch_bounds__(256) void k_bulk_emit(const uint32_t *best, uint32_
   rules[o] = r;
 }
 
+// The empty candidate (the empty sequence: its outputs are its inputs, 0
+// bytes, never pruned) against every target of a segment.  efp / emh are its
+// fingerprint and Mid hashes (k_fingerprints of a length-0 sequence).  Go
+// semantics: pass 1 when the exact QuickCheck (and MidCheck) hold, else pass
+// 2 when dead flags are given and the masked ones hold; all hits (the CUDA
+// tool): one masked pass.  Each such target gets an exhaustive job, planned
+// here; slots are reserved once per wave.
+__global__ __launch_bounds__(256) void k_empty_jobs(const uint2 *__restrict__ tfp, const uint2 *__restrict__ tmh,
+                                                    const uint2 *__restrict__ efp, const uint2 *__restrict__ emh,
+                                                    uint32_t nt, const uint32_t *__restrict__ targets, uint32_t tl,
+                                                    uint32_t pm_masked, int hits_all, int dead_any, int use_mid,
+                                                    int plan_mode, Job *jobs, uint32_t *njobs, uint32_t *chunks,
+                                                    uint32_t *fail) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  uint32_t pass = 0;
+  if (t < nt) {
+    uint32_t acc_lo = 0, acc_hi = 0;
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+      const uint2 a = tfp[(uint64_t)t * kFpVecs + v], b = efp[v];
+      acc_lo |= a.x ^ b.x;
+      acc_hi |= a.y ^ b.y;
+    }
+    const uint32_t mlo = 0x00FFFFFFu | (pm_masked << 24), flo = 0x00FFFFFFu | (kPMask << 24);
+    const bool exact = ((acc_lo & flo) | acc_hi) == 0, masked = ((acc_lo & mlo) | acc_hi) == 0;
+    const uint2 th = tmh[t], eh = emh[0];
+    if (hits_all) pass = (masked && (!use_mid || th.y == eh.y)) ? 1u : 0u;
+    else if (exact) pass = (!use_mid || th.x == eh.x) ? 1u : 0u;
+    else if (dead_any && masked) pass = (!use_mid || th.y == eh.y) ? 2u : 0u;
+  }
+  const uint64_t m = __ballot(pass != 0);
+  uint32_t base = 0;
+  if (lane_id() == 0 && m) base = atomicAdd(njobs, (uint32_t)__popcll(m));
+  base = uni(base);
+  if (!pass) return;
+  const uint32_t o = base + mbcnt(m);
+  Job j{};
+  for (uint32_t k = 0; k < tl; k++) j.tw[k] = targets[(uint64_t)t * tl + k];
+  j.tl = (uint8_t)tl;
+  j.cl = 0;
+  j.pass = (uint8_t)pass;
+  j.pmask = (uint8_t)((hits_all || pass == 2) ? pm_masked : kPMask);
+  j.target = t;
+  chunks[o] = plan_job(j, plan_mode);
+  fail[o] = 0;
+  jobs[o] = j;
+}
+
+// Rules of the empty-candidate jobs that passed; traw (length-3 ranges)
+// maps a target's segment position to its raw triple offset.
+__global__ __launch_bounds__(256) void k_empty_emit(const Job *jobs, uint32_t nj, const uint32_t *fail,
+                                                    const uint2 *tfp, const uint2 *efp, const uint8_t *tbytes,
+                                                    const uint8_t *tcyc, int hits_all, uint32_t dead, uint64_t tbase,
+                                                    const uint32_t *traw, sx_rule *rules, uint32_t *nrules) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nj || fail[i]) return;
+  const Job &j = jobs[i];
+  uint32_t fd = 0;
+  if (!hits_all && j.pass == 2) {
+    for (int v = 0; v < 8; v++) fd |= ((tfp[(uint64_t)j.target * kFpVecs + v].x ^ efp[v].x) >> 24) & kPMask;
+    if (fd == 0) return;  // FlagDiff == 0 is skipped (worker.go:211-213)
+  }
+  sx_rule r;
+  memset(&r, 0, sizeof(r));
+  for (int q = 0; q < 4; q++) r.src[q] = j.tw[q];
+  r.target_index = tbase + (traw ? traw[j.target] : j.target);
+  r.src_len = j.tl;
+  r.rep_len = 0;
+  r.pass = j.pass;
+  r.dead_flags = (uint8_t)(hits_all ? dead : fd);
+  r.bytes_saved = (int16_t)tbytes[j.target];
+  r.cycles_saved = (int16_t)tcyc[j.target];
+  rules[atomicAdd(nrules, 1u)] = r;
+}
+
 // Length-3 ranges: a rule's target_index (base + position in the segment)
 // becomes base + the target's offset in the raw triple index range.
 __global__ void k_fix_raw_index(sx_rule *rules, uint32_t n, const uint32_t *raw_off, uint64_t base) {
@@ -1551,6 +1626,7 @@ struct sx_handle {
   hipStream_t stream = nullptr;
   std::string err;
   std::vector<sx_rule> rules;  // all-hits mode: sorted on the host
+  std::vector<sx_rule> empty_rules;  // the empty-candidate stream (include_empty), sorted on the host
   // first-hit mode: the rules stay on the device, in order, until fetched,
   // rec_words 32-bit words per rule: 12 (an sx_rule) or 6 (the packed form of
   // k_pack_rules, which sx_fetch_rules expands)
@@ -1588,7 +1664,7 @@ enum BufId {
   B_RANK, B_SURV, B_MID, B_COUNTERS, B_CUR, B_REJ, B_DONE, B_JOBS, B_CHUNKS, B_OFFS, B_SUMS,
   B_FAIL, B_RULES, B_SCRATCH, B_GEN_COUNTS, B_GEN_OFFS, B_BADOP, B_P_TW, B_P_TL, B_P_CW, B_P_CL,
   B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR, B_HEADS, B_RULEFLAG, B_RULEOFFS, B_RULES2,
-  B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_PLANTAB, B_COUNT
+  B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_PLANTAB, B_EFP, B_ERULES, B_COUNT
 };
 static_assert(B_COUNT <= 64, "sx_handle::bufs holds every buffer id");
 
@@ -2200,13 +2276,21 @@ static int check_words(sx_handle *h, const uint32_t *w, uint64_t n) {
   return SX_OK;
 }
 
-extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, uint64_t cap,
+extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *out, uint64_t cap,
                             uint64_t *n_out, sx_stats *stats) {
-  if (!h || !cfg) return SX_EINVAL;
+  if (!h || !cfg_in) return SX_EINVAL;
   if (h->poisoned) return set_err(h, SX_EHIP, "handle unusable after a device timeout");
-  if (cfg->struct_size != sizeof(sx_enum_cfg)) return set_err(h, SX_EINVAL, "sx_enum_cfg size mismatch");
+  if (cfg_in->struct_size != sizeof(sx_enum_cfg) && cfg_in->struct_size != SX_ENUM_CFG_SIZE_V1)
+    return set_err(h, SX_EINVAL, "sx_enum_cfg size mismatch");
+  // a caller of the first layout leaves the later fields at 0
+  sx_enum_cfg cfg_local;
+  memset(&cfg_local, 0, sizeof(cfg_local));
+  memcpy(&cfg_local, cfg_in, cfg_in->struct_size);
+  const sx_enum_cfg *cfg = &cfg_local;
   const int tl = cfg->target_len, ml = cfg->max_cand_len;
-  if (tl < 2 || tl > 3 || ml < 1 || ml > 2 || ml >= tl)
+  // search.go:26-50: --max-target L runs every target length 2..L with
+  // candidates of length 1..L-1, so (2, 1), (2, 2) and (3, 2) (and (3, 1))
+  if (tl < 2 || tl > 3 || ml < 1 || ml > 2)
     return set_err(h, SX_EINVAL, "target_len %d / max_cand_len %d unsupported", tl, ml);
   if (cfg->bytes_model < 0 || cfg->bytes_model > 1 || cfg->hits < 0 || cfg->hits > 1 ||
       cfg->exh_mode < 0 || cfg->exh_mode > 1 || cfg->dead_flags > 0xFF)
@@ -2222,6 +2306,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
   sx_stats st;
   memset(&st, 0, sizeof(st));
   h->rules.clear();
+  h->empty_rules.clear();
   HIPCHK(h, hipStreamSynchronize(h->copy_stream));  // the last call's rule copies read d_rules
   h->d_rules_n = 0;
   h->rules_on_device = !cfg->hits;
@@ -2297,6 +2382,17 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
                          cfp + (uint64_t)kNumInsts * kFpVecs, cmh + kNumInsts, nullptr, lo_mask,
                          cbytes + kNumInsts, ccyc + kNumInsts, cprune + kNumInsts, badop);
     }
+    HIPCHK(h, hipGetLastError());
+  }
+  // the empty candidate's fingerprint: a length-0 sequence's outputs are the
+  // vectors themselves (no word is read)
+  uint2 *efp = nullptr;
+  if (cfg->include_empty) {
+    uint8_t *eb;
+    ENSURE(B_EFP, uint8_t, 256, eb);
+    efp = reinterpret_cast<uint2 *>(eb);  // [0, 8): fingerprint, [8]: Mid hashes, then bytes and cycles
+    hipLaunchKernelGGL(k_fingerprints, dim3(1), dim3(256), 0, h->stream, cwords, 0u, (uint64_t)1, cfg->bytes_model,
+                       efp, efp + 8, (uint32_t *)nullptr, lo_mask, eb + 80, eb + 81, (uint8_t *)nullptr, badop);
     HIPCHK(h, hipGetLastError());
   }
   // plans by opcode for length-2 targets (whose candidates are single instructions)
@@ -2646,6 +2742,45 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
         SYNC_WD("rule copy");
       }
     }
+    if (cfg->include_empty) {
+      // the empty candidate: one slot per target, in a stream of its own
+      st.qc_pairs += nt;
+      Job *ej;
+      uint32_t *ech, *efl, *ecnt = counters + 5;  // [5] jobs, [6] rules
+      sx_rule *er;
+      ENSURE(B_JOBS, Job, nt, ej);
+      ENSURE(B_CHUNKS, uint32_t, nt + 1, ech);
+      ENSURE(B_FAIL, uint32_t, nt + 1, efl);
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, h->stream, ecnt, (uint64_t)2, 0u);
+      hipLaunchKernelGGL(k_empty_jobs, dim3(blocks_for(nt, 256)), dim3(256), 0, h->stream, tfp, tmh, efp, efp + 8,
+                         (uint32_t)nt, tw, (uint32_t)tl, pm_masked, cfg->hits, cfg->dead_flags ? 1 : 0, cfg->use_mid,
+                         cfg->exh_mode, ej, ecnt, ech, efl);
+      HIPCHK(h, hipGetLastError());
+      uint32_t nje = 0;
+      HIPCHK(h, hipMemcpyAsync(&nje, ecnt, 4, hipMemcpyDeviceToHost, h->stream));
+      SYNC_WD("device sync");
+      if (nje) {
+        st.exh_pairs += nje;
+        uint64_t chunks = 0;
+        int rc = run_exhaustive(h, nje, cfg->exh_mode, &chunks, true);
+        if (rc) return rc;
+        ENSURE(B_ERULES, sx_rule, nje, er);
+        hipLaunchKernelGGL(k_empty_emit, dim3(blocks_for(nje, 256)), dim3(256), 0, h->stream,
+                           (const Job *)h->bufs[B_JOBS].p, nje, (const uint32_t *)h->bufs[B_FAIL].p, tfp, efp,
+                           tbytes, tcyc, cfg->hits, cfg->dead_flags, tbase + s0, traw, er, ecnt + 1);
+        HIPCHK(h, hipGetLastError());
+        uint32_t ner = 0;
+        HIPCHK(h, hipMemcpyAsync(&ner, ecnt + 1, 4, hipMemcpyDeviceToHost, h->stream));
+        SYNC_WD("device sync");
+        const size_t old = h->empty_rules.size();
+        h->empty_rules.resize(old + ner);
+        if (ner) {
+          HIPCHK(h, hipMemcpyAsync(h->empty_rules.data() + old, er, (size_t)ner * sizeof(sx_rule),
+                                   hipMemcpyDeviceToHost, h->stream));
+          SYNC_WD("empty-candidate rules");
+        }
+      }
+    }
     float a = 0, b = 0, c = 0;
     HIPCHK(h, hipEventElapsedTime(&a, e1, e2));
     HIPCHK(h, hipEventElapsedTime(&b, e2, e3));
@@ -2680,6 +2815,10 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, 
       if (x.rep[q] != y.rep[q]) return x.rep[q] < y.rep[q];
     return false;
   });
+  std::sort(h->empty_rules.begin(), h->empty_rules.end(), [](const sx_rule &x, const sx_rule &y) {
+    return x.target_index != y.target_index ? x.target_index < y.target_index : x.pass < y.pass;
+  });
+  st.empty_rules = h->empty_rules.size();
   SYNC_WD("device sync");
   const uint64_t n_rules = h->rules_on_device ? h->d_rules_n : h->rules.size();
   st.rules = n_rules;
@@ -2721,6 +2860,16 @@ extern "C" int sx_fetch_rules(sx_handle *h, sx_rule *out, uint64_t offset, uint6
       memcpy(out, h->rules.data() + offset, m * sizeof(sx_rule));
     }
   }
+  if (n_copied) *n_copied = m;
+  return SX_OK;
+}
+
+extern "C" int sx_fetch_empty_rules(sx_handle *h, sx_rule *out, uint64_t offset, uint64_t cap,
+                                    uint64_t *n_copied) {
+  if (!h || (!out && cap)) return SX_EINVAL;
+  const uint64_t n = h->empty_rules.size();
+  const uint64_t m = offset >= n ? 0 : std::min<uint64_t>(cap, n - offset);
+  if (m) memcpy(out, h->empty_rules.data() + offset, m * sizeof(sx_rule));
   if (n_copied) *n_copied = m;
   return SX_OK;
 }

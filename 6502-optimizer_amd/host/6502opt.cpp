@@ -9,10 +9,12 @@
 // Additions: --jsonl PATH writes the CUDA tool's JSONL schema
 // (cuda/6502_search_v2.cu:912-930); --bytes-model go|cuda, --hits first|all,
 // --exh projected|full, --no-mid and --device select the device semantics;
-// --first-target/--end-target select a range of the pruned target space (the
-// multi-process split of the CUDA tool, cu:775-777); --max-target 3 walks raw
-// Go-order triple index ranges; --checkpoint FILE [--resume] [--chunk N]
-// appends JSONL chunk by chunk and records the next target index; stoke
+// --first-target/--end-target select a range of the target index space (the
+// multi-process split of the CUDA tool, cu:775-777); --max-target 3 runs the
+// length-2 targets and then the length-3 ones (raw Go-order triple indices),
+// both against candidates of length 1 and 2 (search.go:26-50);
+// --checkpoint FILE [--resume] [--chunk N] appends JSONL chunk by chunk and
+// records the next target index and the committed JSONL size; stoke
 // takes --seed (the chains are reproducible) and reports the first verified
 // rewrite.
 #include <algorithm>
@@ -27,6 +29,8 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+#include <unistd.h>
 
 #include "sx6502.h"
 
@@ -276,12 +280,20 @@ int check(sx_handle *h, int rc, const char *what) {
 }
 
 // Checkpoint of a chunked enumeration (the role of pkg/result/checkpoint.go:
-// 9-37, as JSON text): the next unprocessed target index, the range end, the
-// rules written so far.
+// 9-37, as JSON text): the next unprocessed index of the enumeration's
+// target space, the range end, the rules and the JSONL bytes written and
+// committed so far, and the whole configuration.  Per chunk the JSONL is
+// written and flushed first, then the checkpoint is replaced atomically; a
+// resume truncates the JSONL to the committed offset, so a crash between the
+// two writes (or in the middle of either) neither duplicates nor loses rules.
 struct Checkpoint {
-  uint64_t next = 0, end = 0, rules = 0;
-  int max_target = 2;
+  uint64_t next = 0, end = 0, rules = 0, offset = 0;
+  int max_target = 2, bytes_model = 0, hits = 0, exh = 0, use_mid = 1;
   uint32_t dead = 0;
+  bool same_config(const Checkpoint &o) const {
+    return max_target == o.max_target && dead == o.dead && bytes_model == o.bytes_model && hits == o.hits &&
+           exh == o.exh && use_mid == o.use_mid && end == o.end;
+  }
 };
 
 bool read_checkpoint(const std::string &path, Checkpoint *c) {
@@ -298,12 +310,17 @@ bool read_checkpoint(const std::string &path, Checkpoint *c) {
     *v = strtoull(t.c_str() + p + 1, nullptr, 10);
     return true;
   };
-  uint64_t mt = 0, dd = 0;
-  if (!num("next", &c->next) || !num("end", &c->end) || !num("rules", &c->rules) || !num("max_target", &mt) ||
-      !num("dead_flags", &dd))
+  uint64_t mt = 0, dd = 0, bm = 0, hi = 0, ex = 0, um = 0;
+  if (!num("next", &c->next) || !num("end", &c->end) || !num("rules", &c->rules) || !num("offset", &c->offset) ||
+      !num("max_target", &mt) || !num("dead_flags", &dd) || !num("bytes_model", &bm) || !num("hits", &hi) ||
+      !num("exh", &ex) || !num("use_mid", &um))
     return false;
   c->max_target = (int)mt;
   c->dead = (uint32_t)dd;
+  c->bytes_model = (int)bm;
+  c->hits = (int)hi;
+  c->exh = (int)ex;
+  c->use_mid = (int)um;
   return true;
 }
 
@@ -311,11 +328,27 @@ bool write_checkpoint(const std::string &path, const Checkpoint &c) {
   std::string tmp = path + ".tmp";
   FILE *f = fopen(tmp.c_str(), "w");
   if (!f) return false;
-  fprintf(f, "{\"next\": %llu, \"end\": %llu, \"rules\": %llu, \"max_target\": %d, \"dead_flags\": %u}\n",
-          (unsigned long long)c.next, (unsigned long long)c.end, (unsigned long long)c.rules, c.max_target, c.dead);
+  fprintf(f,
+          "{\"next\": %llu, \"end\": %llu, \"rules\": %llu, \"offset\": %llu, \"max_target\": %d, "
+          "\"dead_flags\": %u, \"bytes_model\": %d, \"hits\": %d, \"exh\": %d, \"use_mid\": %d}\n",
+          (unsigned long long)c.next, (unsigned long long)c.end, (unsigned long long)c.rules,
+          (unsigned long long)c.offset, c.max_target, c.dead, c.bytes_model, c.hits, c.exh, c.use_mid);
+  if (fflush(f) != 0 || fsync(fileno(f)) != 0) {
+    fclose(f);
+    return false;
+  }
   fclose(f);
   return rename(tmp.c_str(), path.c_str()) == 0;  // the checkpoint is replaced atomically
 }
+
+// search.go:26-50: --max-target L enumerates every target length 2..L, each
+// against candidates of length 1..L-1.  The lengths' target spaces are laid
+// end to end as one index space: the pruned length-2 targets (positions in Go
+// order), then for L = 3 the raw Go-order triple indices.
+struct Phase {
+  int target_len;
+  uint64_t base, size;  // global indices [base, base + size)
+};
 
 int cmd_enumerate(int argc, char **argv) {
   Args a = parse_args(argc, argv, 2, {"verbose", "no-mid", "resume"});
@@ -343,21 +376,17 @@ int cmd_enumerate(int argc, char **argv) {
   sx_enum_cfg cfg;
   memset(&cfg, 0, sizeof(cfg));
   cfg.struct_size = sizeof(cfg);
-  cfg.target_len = max_target;
-  cfg.max_cand_len = max_target - 1;  // enumerator.go: candidates shorter than the target
+  cfg.max_cand_len = max_target - 1;  // enumerator.go: candidates shorter than the longest target
   cfg.bytes_model = model;
   cfg.hits = hits;
   cfg.exh_mode = exh;
   cfg.use_mid = a.has("no-mid") ? 0 : 1;
   cfg.dead_flags = dead;
-  // the target index space: positions in the pruned length-2 list, or raw
-  // Go-order triple indices for length 3
-  uint64_t total = 0;
-  if (max_target == 2) {
-    if (check(h, sx_pruned_targets(h, 0, 0, nullptr, &total), "sx_pruned_targets")) return 1;
-  } else {
-    total = 2863ull * 2863ull * 2863ull;
-  }
+  uint64_t n2 = 0;
+  if (check(h, sx_pruned_targets(h, 0, 0, nullptr, &n2), "sx_pruned_targets")) return 1;
+  std::vector<Phase> phases{{2, 0, n2}};
+  if (max_target == 3) phases.push_back({3, n2, 2863ull * 2863ull * 2863ull});
+  const uint64_t total = phases.back().base + phases.back().size;
   uint64_t first = strtoull(a.get("first-target", "0").c_str(), nullptr, 10);
   uint64_t end = strtoull(a.get("end-target", "0").c_str(), nullptr, 10);
   if (end == 0) end = total;
@@ -367,29 +396,48 @@ int cmd_enumerate(int argc, char **argv) {
   ck.end = end;
   ck.max_target = max_target;
   ck.dead = dead;
+  ck.bytes_model = model;
+  ck.hits = hits;
+  ck.exh = exh;
+  ck.use_mid = cfg.use_mid;
+  FILE *jf = nullptr;
   if (resume) {
     if (ckpt.empty()) return die("--resume needs --checkpoint");
-    if (!read_checkpoint(ckpt, &ck)) return die("cannot read checkpoint " + ckpt);
-    if (ck.max_target != max_target || ck.dead != dead) return die("checkpoint is for another configuration");
+    Checkpoint got;
+    if (!read_checkpoint(ckpt, &got)) return die("cannot read checkpoint " + ckpt);
+    if (!got.same_config(ck)) return die("checkpoint is for another configuration");
+    ck = got;
+    // drop whatever follows the last committed chunk, then append
+    jf = fopen(jl.c_str(), "r+");
+    if (!jf || ftruncate(fileno(jf), (off_t)ck.offset) != 0 || fseek(jf, 0, SEEK_END) != 0)
+      return die("cannot reopen " + jl + " at its committed size");
     printf("  Resuming at target %llu of [.., %llu), %llu rules so far\n", (unsigned long long)ck.next,
            (unsigned long long)ck.end, (unsigned long long)ck.rules);
+  } else if (!jl.empty()) {
+    jf = jl == "-" ? stdout : fopen(jl.c_str(), "w");
+    if (!jf) return die("cannot create " + jl);
+    if (!ckpt.empty() && !write_checkpoint(ckpt, ck)) return die("cannot write checkpoint " + ckpt);
   }
-  // chunks: everything at once unless checkpointing
-  const uint64_t chunk = ckpt.empty() ? (ck.end - ck.next)
+  // chunks: one per phase unless checkpointing
+  const uint64_t chunk = ckpt.empty() ? total
                                       : strtoull(a.get("chunk", max_target == 2 ? "1048576" : "16777216").c_str(),
                                                  nullptr, 10);
-  FILE *jf = nullptr;
-  if (!jl.empty()) {
-    jf = jl == "-" ? stdout : fopen(jl.c_str(), resume ? "a" : "w");
-    if (!jf) return die("cannot create " + jl);
-  }
+  // test hook: exit after this many chunks' JSONL writes, before their checkpoint
+  const char *crash_env = getenv("SX_TEST_EXIT_BEFORE_CHECKPOINT");
+  const long crash_after = crash_env ? atol(crash_env) : 0;
+  long chunks_done = 0;
   std::vector<sx_rule> all;
   sx_stats tot;
   memset(&tot, 0, sizeof(tot));
   auto t0 = std::chrono::steady_clock::now();
   while (ck.next < ck.end) {
-    cfg.first_target = ck.next;
-    cfg.end_target = std::min<uint64_t>(ck.end, ck.next + std::max<uint64_t>(chunk, 1));
+    const Phase *ph = &phases[0];
+    for (const Phase &p : phases)
+      if (ck.next >= p.base && ck.next < p.base + p.size) ph = &p;
+    const uint64_t stop = std::min<uint64_t>({ck.end, ph->base + ph->size, ck.next + std::max<uint64_t>(chunk, 1)});
+    cfg.target_len = ph->target_len;
+    cfg.first_target = ck.next - ph->base;
+    cfg.end_target = stop - ph->base;
     uint64_t n = 0;
     sx_stats st;
     if (check(h, sx_enumerate(h, &cfg, nullptr, 0, &n, &st), "sx_enumerate")) return 1;
@@ -403,11 +451,14 @@ int cmd_enumerate(int argc, char **argv) {
     tot.exh_pairs += st.exh_pairs;
     if (jf) {
       write_jsonl(jf, rules, model);
-      fflush(jf);
+      if (fflush(jf) != 0) return die("cannot write " + jl);
     }
     ck.rules += n;
-    ck.next = cfg.end_target;
+    ck.next = stop;
     if (!ckpt.empty()) {
+      if (fsync(fileno(jf)) != 0) return die("cannot sync " + jl);
+      ck.offset = (uint64_t)ftello(jf);
+      if (crash_after > 0 && ++chunks_done >= crash_after) exit(3);
       if (!write_checkpoint(ckpt, ck)) return die("cannot write checkpoint " + ckpt);
       double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       fprintf(stderr, "  [%.0fs] %llu/%llu (%.2f%%) | %llu rules\n", secs, (unsigned long long)ck.next,

@@ -26,7 +26,7 @@ EXPORTS = ("sx_abi_version", "sx_open", "sx_close", "sx_last_error", "sx_enumera
            "sx_dep_table", "sx_stoke_run", "sx_mlp_param_count", "sx_mlp_create", "sx_mlp_destroy",
            "sx_mlp_get_params", "sx_mlp_set_params", "sx_mlp_loss_grad", "sx_mlp_train", "sx_mlp_predict",
            "sx_proposal_weights", "sx_mlp_proposal", "sx_mfma_gemm_test", "sx_target_search",
-           "sx_measure_int32_rate", "sx_check_pairs_noalloc")
+           "sx_measure_int32_rate", "sx_check_pairs_noalloc", "sx_fetch_empty_rules")
 NOALLOC_MAX_PAIRS = 65536
 MLP_CLASSES = 2863   # instructions
 MLP_OUTPUTS = 2871   # instructions, then 8 context-position pointers
@@ -52,7 +52,8 @@ class EnumCfg(C.Structure):
                 ("bytes_model", C.c_int32), ("hits", C.c_int32), ("exh_mode", C.c_int32),
                 ("use_mid", C.c_int32), ("dead_flags", C.c_uint32), ("targets", C.POINTER(C.c_uint32)),
                 ("n_targets", C.c_uint64), ("first_target", C.c_uint64), ("end_target", C.c_uint64),
-                ("tile_targets", C.c_uint32), ("survivor_cap", C.c_uint32)]
+                ("tile_targets", C.c_uint32), ("survivor_cap", C.c_uint32), ("include_empty", C.c_int32),
+                ("reserved", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -62,7 +63,7 @@ class Stats(C.Structure):
                [(n, C.c_double) for n in ("ms_setup", "ms_qc", "ms_mid", "ms_exh", "ms_total",
                                           "ms_qc_kernel", "ms_exh_kernel")] + \
                [(n, C.c_uint64) for n in ("qc_launches", "exh_launches")] + \
-               [(n, C.c_uint32) for n in ("qc_tile", "qc_blocks_per_cu")]
+               [(n, C.c_uint32) for n in ("qc_tile", "qc_blocks_per_cu")] + [("empty_rules", C.c_uint64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -127,6 +128,7 @@ def lib() -> C.CDLL:
     L.sx_enumerate.argtypes = [vp, C.POINTER(EnumCfg), vp, C.c_uint64, C.POINTER(C.c_uint64),
                                C.POINTER(Stats)]
     L.sx_fetch_rules.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]
+    L.sx_fetch_empty_rules.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]
     L.sx_pruned_targets.argtypes = [vp, C.c_uint64, C.c_uint64, u32p, C.POINTER(C.c_uint64)]
     L.sx_check_pairs.argtypes = [vp, u32p, u8p, u32p, u8p, C.c_uint64, C.c_uint32, C.c_uint32,
                                  C.c_int32, u8p]
@@ -258,7 +260,10 @@ class Device:
     # -- enumeration ----------------------------------------------------
     def enumerate(self, targets: Optional[np.ndarray] = None, *, target_len: int = 2, max_cand_len: int = 1,
                   dead: int = 0, bytes_model: int = 0, hits: int = HITS_FIRST, exh_mode: int = EXH_PROJECTED,
-                  use_mid: bool = True, first: int = 0, end: int = 0, tile: int = 0, survivor_cap: int = 0):
+                  use_mid: bool = True, first: int = 0, end: int = 0, tile: int = 0, survivor_cap: int = 0,
+                  include_empty: bool = False):
+        """sx_enumerate: (rules, stats).  With include_empty the empty
+        candidate's stream is read with fetch_empty_rules()."""
         cfg = EnumCfg()
         cfg.struct_size = C.sizeof(EnumCfg)
         cfg.target_len = target_len
@@ -277,6 +282,7 @@ class Device:
             cfg.n_targets = keep.shape[0]
         cfg.first_target, cfg.end_target = first, end
         cfg.tile_targets, cfg.survivor_cap = tile, survivor_cap
+        cfg.include_empty = 1 if include_empty else 0
         n = C.c_uint64(0)
         st = Stats()
         rc = lib().sx_enumerate(self._h, C.byref(cfg), None, 0, C.byref(n), C.byref(st))
@@ -287,6 +293,15 @@ class Device:
         self._check(rc, "sx_fetch_rules")
         del keep
         return out[: got.value], st.as_dict()
+
+    def fetch_empty_rules(self, n: int) -> np.ndarray:
+        """The empty-candidate stream of the last enumeration (its size is
+        stats["empty_rules"])."""
+        out = np.empty(n, RULE_DTYPE)
+        got = C.c_uint64(0)
+        rc = lib().sx_fetch_empty_rules(self._h, out.ctypes.data_as(C.c_void_p), 0, n, C.byref(got))
+        self._check(rc, "sx_fetch_empty_rules")
+        return out[: got.value]
 
     def pruned_targets(self, first: int = 0, n: Optional[int] = None) -> np.ndarray:
         total = C.c_uint64(0)

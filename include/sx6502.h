@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SX_ABI_VERSION 1
+#define SX_ABI_VERSION 2
 
 /* Status codes. */
 #define SX_OK 0
@@ -74,6 +74,7 @@ typedef struct sx_stats {
   uint64_t qc_launches, exh_launches;
   uint32_t qc_tile;          /* QuickCheck targets per work item (autotuned unless tile_targets is set) */
   uint32_t qc_blocks_per_cu; /* 0 = grid launch; else persistent launch with this many blocks per CU */
+  uint64_t empty_rules;      /* rules of the empty-candidate stream (include_empty) */
 } sx_stats;
 
 /* Enumeration semantics. */
@@ -87,7 +88,7 @@ typedef struct sx_stats {
 typedef struct sx_enum_cfg {
   uint32_t struct_size;   /* sizeof(sx_enum_cfg) */
   int32_t target_len;     /* 2 or 3 */
-  int32_t max_cand_len;   /* 1, or 2 when target_len == 3 */
+  int32_t max_cand_len;   /* 1 or 2 (search.go:26-28: --max-target L gives L - 1 for every target length) */
   int32_t bytes_model;    /* SX_BYTES_* */
   int32_t hits;           /* SX_HITS_* */
   int32_t exh_mode;       /* SX_EXH_* */
@@ -102,7 +103,15 @@ typedef struct sx_enum_cfg {
                              (0 = autotuned shape) */
   uint32_t survivor_cap;  /* QuickCheck survivor buffer entries per segment, split over 8 regions
                              (0 = default); a full region re-runs the segment with a larger buffer */
+  int32_t include_empty;  /* 1: also check every target against the empty sequence (0 bytes, never
+                             pruned).  Go never tries it (worker.go:123 starts at length 1), so its
+                             rules (rep_len 0) form a stream of their own, fetched with
+                             sx_fetch_empty_rules, and do not change the other rules; QuickCheck
+                             pairs then count the empty slot too */
+  uint32_t reserved;      /* 0 */
 } sx_enum_cfg;
+/* The size of sx_enum_cfg before include_empty; a caller passing it gets include_empty = 0. */
+#define SX_ENUM_CFG_SIZE_V1 72u
 
 /* Open a handle on HIP device `device`.  Replaces the process start of the
  * CUDA tool (cuda/6502_search_v2.cu:763-797) and the WorkerPool construction
@@ -126,6 +135,12 @@ int sx_abi_version(void);
 int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg, sx_rule *out, uint64_t cap,
                  uint64_t *n_out, sx_stats *stats);
 int sx_fetch_rules(sx_handle *h, sx_rule *out, uint64_t offset, uint64_t cap, uint64_t *n_copied);
+/* The empty-candidate stream of the last enumeration (include_empty), sorted
+ * by (target_index, pass): per target at most one rule, pass 1 when the
+ * target equals the empty sequence under the exact checks, pass 2 when only
+ * the flag-relaxed ones hold (dead_flags = FlagDiff); in the CUDA-tool
+ * semantics one masked pass.  Same calling pattern as sx_fetch_rules. */
+int sx_fetch_empty_rules(sx_handle *h, sx_rule *out, uint64_t offset, uint64_t cap, uint64_t *n_copied);
 
 /* Number of pruned length-2 targets in Go order (7,633,612) and the words of
  * the range [first, first + n) of them (search.go:56-75 collectTasks). */

@@ -23,7 +23,7 @@ def test_every_declared_symbol_is_exported():
     L = core.lib()
     for name in decl:
         assert hasattr(L, name), name
-    assert L.sx_abi_version() == 1
+    assert L.sx_abi_version() == 2
 
 
 def test_open_without_device_fails_cleanly():
@@ -168,5 +168,6 @@ def test_ctypes_mirrors_match_the_c_header(tmp_path):
         assert int(got[f"{name} size"]) == C.sizeof(py), name
         for f, _ in py._fields_:
             assert int(got[f"{name}.{f}"]) == getattr(py, f).offset, (name, f)
+    assert int(got["sx_enum_cfg.include_empty"]) == 72  # SX_ENUM_CFG_SIZE_V1: the layout before it
     assert core.STOKE_RESULT_DTYPE.itemsize == C.sizeof(core.StokeResult)
     assert core.RULE_DTYPE.itemsize == C.sizeof(core.RuleC)

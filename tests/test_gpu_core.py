@@ -170,6 +170,30 @@ def test_c1_rule_sets(dev, name, kw):
         assert st["mid_pass"] == counts["mid_pass"]
 
 
+@pytest.mark.parametrize("name,kw", [
+    ("c1_go_dead00", dict(dead=0x00)),
+    ("c1_go_deadFF", dict(dead=0xFF)),
+    ("c1_go_dead01", dict(dead=0x01)),
+    ("c1_cuda_dead00", dict(dead=0x00, hits=core.HITS_ALL, bytes_model=isa.BYTES_CUDA)),
+])
+def test_c1_with_the_empty_candidate(dev, name, kw):
+    """C1 as configured: 2,863 candidates plus the empty sequence, 11,730,944
+    QuickCheck pairs.  The empty candidate's rules come in their own stream
+    (the oracle's, c1_empty.npz) and leave the other rules unchanged."""
+    targets = load("c1_targets")
+    got, st = dev.enumerate(targets, include_empty=True, **kw)
+    same_rules(got, load(name))
+    assert st["qc_pairs"] == 4096 * 2864 == 11_730_944
+    gold = np.load(os.path.join(GOLD, "c1_empty.npz"), allow_pickle=False)[name]
+    emp = dev.fetch_empty_rules(st["empty_rules"])
+    same_rules(emp, gold)
+    assert (emp["rep_len"] == 0).all()
+    assert emp["bytes_saved"].tolist() == [sum(isa.byte_size(int(w) & 0xFFFF, kw.get("bytes_model", 0))
+                                               for w in r["src"][: r["src_len"]]) for r in emp]
+    if name == "c1_go_deadFF":
+        assert len(emp) > 100  # flag-only targets such as CLC : CLV
+
+
 def test_full_length2_cuda_semantics_published_counts(dev):
     """The full length-2 run in the CUDA tool's semantics (all hits, M ops 2
     bytes, dead 0x00) reproduces the published QuickCheck and MidCheck
@@ -245,5 +269,5 @@ def test_error_paths(dev):
         dev.enumerate(np.array([[o.word(60), 0]], np.uint32))
     assert e.value.code == core.SX_EOPCODE
     with pytest.raises(core.SxError) as e:
-        dev.enumerate(np.zeros((1, 2), np.uint32), max_cand_len=2)
+        dev.enumerate(np.zeros((1, 2), np.uint32), max_cand_len=3)
     assert e.value.code == core.SX_EINVAL
