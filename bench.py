@@ -67,22 +67,81 @@ def hbm_traffic(kernel: str):
         return None
 
 
-def cpu_baseline(sample_targets: int, dead: int):
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(samples):
+    """The C restatement of the Go enumerate (oracle/, first hit in Go order,
+    Go's reduced sweep) on every core this process may use, on a bounded
+    sample of targets per dead-flag mask: {dead: n_targets}."""
     import oracle as o
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
     import sampling
     l2 = o.enumerate_pruned(2)
-    idx = sampling.sample_indices(len(l2), sample_targets, 1234)
-    t = l2[idx]
-    cores = min(16, os.cpu_count() or 1)
-    t0 = time.perf_counter()
-    rules, st = o.search(t, dead=dead, nthreads=cores)
-    dt = time.perf_counter() - t0
-    return {"value": sample_targets * N_CANDS / dt, "unit": "pairs/s", "cores": cores, "kind": "port",
-            "sample": f"{sample_targets} pruned length-2 targets (seed 1234) x 2,863 candidates, "
-                      f"dead 0x{dead:02X}, C restatement of the Go enumerate (first hit, Go order), "
-                      f"{dt:.2f} s, {len(rules)} rules; full-run extrapolation "
-                      f"{dt * N_TARGETS / sample_targets:.0f} s"}
+    cores = len(os.sched_getaffinity(0))
+    runs = {}
+    for dead, n in samples.items():
+        idx = sampling.sample_indices(len(l2), n, 1234)
+        t0 = time.perf_counter()
+        rules, st = o.search(l2[idx], dead=dead, nthreads=cores)
+        dt = time.perf_counter() - t0
+        runs[f"0x{dead:02X}"] = {"pairs_per_s": n * N_CANDS / dt, "seconds": dt, "targets": n,
+                                 "rules": int(len(rules)), "checks_per_s": st.checked / dt,
+                                 "full_run_s": dt * N_TARGETS / n}
+        print(f"bench: CPU baseline dead 0x{dead:02X}: {n} targets in {dt:.2f} s", file=sys.stderr, flush=True)
+    first = runs[f"0x{min(samples):02X}"]
+    return {"value": first["pairs_per_s"], "unit": "pairs/s", "cores": cores, "kind": "port",
+            "nproc": os.cpu_count(), "cpu_model": _cpu_model(), "runs": runs,
+            "sample": "pruned length-2 targets drawn with SplitMix64 seed 1234 x 2,863 candidates, C restatement "
+                      "of the Go enumerate (first hit in Go order, Go's reduced sweep), on every core this "
+                      "process may use; value = the dead 0x%02X run; checks counted like worker.go:126; "
+                      "full_run_s extrapolates to all 7,633,612 targets" % min(samples)}
+
+
+def cli_wall(dead: int) -> dict:
+    """`6502opt enumerate --max-target 2 --jsonl FILE` from process start to
+    exit (the JSONL written and closed), at one dead-flag mask."""
+    import subprocess
+    import tempfile
+    cli = os.path.join(ROOT, "6502-optimizer_amd", "6502opt")
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "rules.jsonl")
+        t0 = time.perf_counter()
+        p = subprocess.run([cli, "enumerate", "--max-target", "2", "--dead-flags", f"0x{dead:02X}", "--jsonl", path],
+                           capture_output=True, text=True, timeout=600)
+        dt = time.perf_counter() - t0
+        if p.returncode != 0:
+            raise RuntimeError(p.stderr)
+        size = os.path.getsize(path)
+        with open(path, "rb") as f:
+            lines = sum(1 for _ in f)
+    print(f"bench: CLI enumerate dead 0x{dead:02X}: {dt:.3f} s, {lines} lines", file=sys.stderr, flush=True)
+    return {"wall_s": dt, "jsonl_lines": lines, "jsonl_bytes": size}
+
+
+def c2_dead_section(dev, dead: int, steps: int = 3):
+    """The full length-2 enumeration at another dead-flag mask (Go
+    semantics): one warm-up and `steps` timed runs."""
+    dev.enumerate(None, dead=dead)
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        rules, st = dev.enumerate(None, dead=dead)
+        times.append(time.perf_counter() - t0)
+    dt = sum(times) / len(times)
+    print(f"bench: C2 dead 0x{dead:02X} {dt:.4f} s, {len(rules)} rules", file=sys.stderr, flush=True)
+    return {"workload": f"full length-2 enumeration, Go semantics, dead 0x{dead:02X}", "wall_s": dt,
+            "rules": int(len(rules)), "pairs_per_s": PAIRS / dt, "exh_states": int(st["exh_states"]),
+            "stage_ms": {"qc": st["ms_qc"], "mid": st["ms_mid"], "exh": st["ms_exh"]},
+            "ms_exh_kernel": st["ms_exh_kernel"], "exh_launches": int(st["exh_launches"])}
 
 
 def c2_cuda_section(dev):
@@ -225,7 +284,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--dead", type=lambda s: int(s, 0), default=0x00)
-    ap.add_argument("--cpu-sample", type=int, default=2048)
+    ap.add_argument("--cpu-sample", type=int, default=2048, help="targets of the CPU baseline at dead 0x00")
+    ap.add_argument("--cpu-sample-ff", type=int, default=768, help="targets of the CPU baseline at dead 0xFF")
+    ap.add_argument("--no-cli", action="store_true", help="skip the command-line wall times")
+    ap.add_argument("--no-ff", action="store_true", help="skip the dead 0xFF enumeration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-cuda", action="store_true", help="skip the CUDA-tool-semantics run")
@@ -302,7 +364,8 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": PAIRS / (ms * 1e-3) / PUBLISHED_PAIRS_PER_S,
         "vs_baseline_basis": "published 2.10e7 pairs/s: the CUDA tool's all-hits run of the same pairs "
-                             "(README.md:127-132); c2_cuda times that exact workload here",
+                             "(README.md:127-132), a different rule semantics from this line's Go one; "
+                             "vs_baseline_like_for_like = c2_cuda, that exact workload here",
         "dtype": "u8", "data": "synthetic (exhaustive target space)",
         "config": {"workload": "full length-2 enumeration: 7,633,612 pruned targets x 2,863 candidates, "
                                "QC->Mid->complete Exhaustive, Go semantics (first hit)",
@@ -322,15 +385,24 @@ def main():
     }
     if not args.no_cuda:
         out["c2_cuda"] = c2_cuda_section(dev)
+        out["vs_baseline_like_for_like"] = out["c2_cuda"]["speedup_vs_published"]
+    if not args.no_ff:
+        out["c2_deadFF"] = c2_dead_section(dev, 0xFF)
     if not args.no_c3:
         out["c3"] = c3_section(dev)
         out["target"] = target_section(dev)
     if not args.no_stoke:
         out["stoke"] = stoke_section(dev, args.stoke_steps, args.train_steps, args.harvest_targets)
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.dead)
-    print(json.dumps(out))
     dev.close()
+    if not args.no_cli:
+        out["l2_cli_wall_s"] = {f"0x{d:02X}": cli_wall(d) for d in (0x00, 0xFF)}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline({0x00: args.cpu_sample, 0xFF: args.cpu_sample_ff})
+        out["gpu_vs_cpu_wall"] = {
+            "0x00": out["cpu_baseline"]["runs"]["0x00"]["full_run_s"] / out["l2_wall_s"],
+            "0xFF": (out["cpu_baseline"]["runs"]["0xFF"]["full_run_s"] / out["c2_deadFF"]["wall_s"]
+                     if "c2_deadFF" in out else None)}
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":
