@@ -644,6 +644,24 @@ __device__ __forceinline__ uint32_t block_prefix(uint32_t cnt, uint32_t *s, uint
   return pre;
 }
 
+// dst[key] = min(dst[key], v) for every lane whose key is not kRankNone, with
+// one atomic per run of equal keys in the wave: a suffix minimum over lanes of
+// the same key (Hillis-Steele, 6 shuffle steps), then the first lane of each
+// run issues the atomic.  Survivor lists keep a wave's hits on one target
+// next to each other, and atomics on one address serialise, so at dense
+// targets this cuts the atomics by the run length.  Every lane of the wave
+// calls it.
+__device__ __forceinline__ void wave_run_min(uint32_t key, uint32_t v, uint32_t *dst) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t ok = __shfl_down(key, d), ov = __shfl_down(v, d);
+    if (lane + d < (uint32_t)kWave && ok == key) v = min(v, ov);
+  }
+  const uint32_t pk = __shfl_up(key, 1);
+  if (key != kRankNone && (lane == 0 || pk != key)) atomicMin(&dst[key], v);
+}
+
 // MidCheck and stage bookkeeping.  One lane per QuickCheck survivor.
 // hits_all = 0 (Go): pass 1 if the exact compare held, pass 2 if only the
 // masked one did (and dead != 0).  hits_all = 1 (CUDA tool): one masked pass.
@@ -651,7 +669,9 @@ __device__ __forceinline__ uint32_t block_prefix(uint32_t cnt, uint32_t *s, uint
 // masked P), which keeps every pair whose outputs agree.  Output entry:
 // (target, slot | pass << 30).  counters[0] += QuickCheck hits after the
 // byte/prune filters and counters[1] += passes, both with one 64-bit atomic
-// per block (counters[1] is also the output cursor).
+// per block (counters[1] is also the output cursor).  cur0 (first hit, else
+// null): the first round's minimum, cur0[target * 2 + pass - 1] = lowest rank
+// among the entry's passing survivors (cur0 holds kRankNone on entry).
 __global__ __launch_bounds__(kBlk) void k_midcheck(const uint2 *__restrict__ surv, const SurvRegions R, uint32_t ns,
                                                    const uint2 *__restrict__ tmh,
                                                    const uint8_t *__restrict__ tbytes,
@@ -660,7 +680,8 @@ __global__ __launch_bounds__(kBlk) void k_midcheck(const uint2 *__restrict__ sur
                                                    const uint8_t *__restrict__ cprune,
                                                    int hits_all, int use_mid,
                                                    uint2 *__restrict__ out, uint32_t out_cap,
-                                                   uint32_t *__restrict__ counters) {
+                                                   uint32_t *__restrict__ counters,
+                                                   const uint32_t *__restrict__ rank, uint32_t *cur0) {
   __shared__ uint32_t s_hits[kBlkWaves], s_pass[kBlkWaves];
   __shared__ unsigned long long s_base;
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
@@ -703,6 +724,7 @@ __global__ __launch_bounds__(kBlk) void k_midcheck(const uint2 *__restrict__ sur
     const uint32_t o = (uint32_t)(s_base >> 32) + pre + mbcnt(m);
     if (o < out_cap) out[o] = make_uint2(t, slot | (pass << 30));
   }
+  if (cur0) wave_run_min(pass_ok ? t * 2 + (pass - 1) : kRankNone, pass_ok ? rank[slot] : 0u, cur0);
 }
 
 // A segment's first-hit state, cleared in one launch: cur = none, rej = 0,
@@ -721,15 +743,15 @@ __global__ void k_seg_clear(uint32_t *cur, uint32_t *rej, uint8_t *done, uint32_
 __global__ __launch_bounds__(256) void k_round_min(const uint2 *mid, uint32_t nm,
                                                    const uint32_t *rank, uint32_t *cur,
                                                    const uint32_t *rej, const uint8_t *done) {
-  uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= nm) return;
-  uint2 e = mid[i];
-  uint32_t slot = e.y & 0x3FFFFFFFu, p = (e.y >> 30) - 1;
-  uint64_t k = (uint64_t)e.x * 2 + p;
-  if (done[k]) return;
-  uint32_t r = rank[slot];
-  if (r + 1 <= rej[k]) return;
-  atomicMin(&cur[k], r);
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  uint32_t key = kRankNone, r = 0;
+  if (i < nm) {
+    const uint2 e = mid[i];
+    const uint32_t slot = e.y & 0x3FFFFFFFu, k = e.x * 2 + (e.y >> 30) - 1;
+    r = rank[slot];
+    if (!done[k] && r + 1 > rej[k]) key = k;
+  }
+  wave_run_min(key, r, cur);
 }
 
 constexpr int kJobWords = 10;  // longest sequence a job holds (STOKE candidates)
@@ -902,6 +924,32 @@ __global__ __launch_bounds__(kBlk) void k_round_jobs(const uint2 *mid, uint32_t 
       }
       jobs[o] = j;
     }
+  }
+}
+
+// The first round's jobs (first hit), one lane per (target, pass) entry k
+// whose minimum the MidCheck kernel recorded (cur[k] != none) instead of one
+// per survivor; rank == slot (candidate slots are numbered in Go order), so
+// the minimum names the candidate.  The jobs are planned afterwards.
+__global__ __launch_bounds__(kBlk) void k_entry_jobs(const uint32_t *cur, uint32_t nk, const uint32_t *targets,
+                                                     uint32_t tl, uint32_t pm_masked, Job *jobs, uint32_t job_cap,
+                                                     uint32_t *njobs) {
+  __shared__ uint32_t s_take[kBlkWaves];
+  __shared__ uint32_t s_base;
+  const uint32_t k = blockIdx.x * kBlk + threadIdx.x;
+  const uint32_t slot = k < nk ? cur[k] : kRankNone;
+  const bool take = slot != kRankNone;
+  const uint64_t m = __ballot(take);
+  uint32_t tt = 0;
+  const uint32_t pre = block_prefix((uint32_t)__popcll(m), s_take, &tt);
+  if (threadIdx.x == 0 && tt) s_base = atomicAdd(njobs, tt);
+  __syncthreads();
+  if (take) {
+    const uint32_t pass = (k & 1) + 1;
+    Job j;
+    fill_job(j, targets, tl, nullptr, 0, k >> 1, slot, pass, pass == 2 ? pm_masked : kPMask);
+    const uint32_t o = s_base + pre + mbcnt(m);
+    if (o < job_cap) jobs[o] = j;
   }
 }
 
@@ -2544,12 +2592,13 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     for (;;) {
       ENSURE(B_MID, uint2, std::max<uint32_t>(nsurv, 1), mid);
       HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));  // counters[2..7] stay 0 for the rounds
+      hipLaunchKernelGGL(k_seg_clear, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, rej, done,
+                         cfg->hits ? nullptr : rule_flag, nt * 2);
+      // first hit: MidCheck also leaves the first round's minimum per (target, pass) in cur
       if (nsurv)
         hipLaunchKernelGGL(k_midcheck, dim3(blocks_for(nsurv, kBlk)), dim3(kBlk), 0, h->stream, surv, regions,
                            nsurv, tmh, tbytes, cmh, cbytes, cprune, cfg->hits, cfg->use_mid, mid,
-                           std::max<uint32_t>(nsurv, 1), counters);
-      hipLaunchKernelGGL(k_seg_clear, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, rej, done,
-                         cfg->hits ? nullptr : rule_flag, nt * 2);
+                           std::max<uint32_t>(nsurv, 1), counters, rank, cfg->hits ? nullptr : cur);
       HIPCHK(h, hipGetLastError());
       uint32_t cnt[2];
       HIPCHK(h, hipMemcpyAsync(cnt, counters, 8, hipMemcpyDeviceToHost, h->stream));
@@ -2589,6 +2638,9 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
       if ((uint64_t)round > nc + 1) return set_err(h, SX_EHIP, "first-hit rounds did not converge");
       if (!cfg->hits && h->bulk_after >= 0 && round >= h->bulk_after) go_bulk = true;
       if (!cfg->hits && go_bulk) {
+        if (round == 0)  // the batch's minimum goes to cur, which still holds MidCheck's minima
+          hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, nt * 2,
+                             kRankNone);
         HIPCHK(h, hipMemsetAsync(njobs_d, 0, 4, h->stream));
         hipLaunchKernelGGL(k_bulk_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, nmid, rank,
                            rej, done, tw, (uint32_t)tl, pm_masked, jobs, job_cap, njobs_d, cfg->exh_mode, ptab,
@@ -2616,23 +2668,33 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         break;
       }
       HIPCHK(h, hipMemsetAsync(njobs_d, 0, 8, h->stream));  // njobs and nrest
-      if (!cfg->hits)
-        hipLaunchKernelGGL(k_round_min, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
-                           cur, rej, done);
-      hipLaunchKernelGGL(k_round_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, nmid, rank,
-                         cur, rej, done, tw, (uint32_t)tl, pm_masked, cfg->hits, jobs, job_cap, njobs_d, nrest_d,
-                         cfg->hits ? 1 : 0, cfg->exh_mode, ptab, jchunks, jfail);
+      const bool first_round = !cfg->hits && round == 0;
+      if (first_round) {
+        // cur holds the minima (MidCheck); every survivor is untested
+        hipLaunchKernelGGL(k_entry_jobs, dim3(blocks_for(nt * 2, kBlk)), dim3(kBlk), 0, h->stream, cur,
+                           (uint32_t)(nt * 2), tw, (uint32_t)tl, pm_masked, jobs, job_cap, njobs_d);
+      } else {
+        if (!cfg->hits)
+          hipLaunchKernelGGL(k_round_min, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
+                             cur, rej, done);
+        hipLaunchKernelGGL(k_round_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, nmid, rank,
+                           cur, rej, done, tw, (uint32_t)tl, pm_masked, cfg->hits, jobs, job_cap, njobs_d, nrest_d,
+                           cfg->hits ? 1 : 0, cfg->exh_mode, ptab, jchunks, jfail);
+      }
       HIPCHK(h, hipGetLastError());
       uint32_t cnt2[2] = {0, 0};
       HIPCHK(h, hipMemcpyAsync(cnt2, njobs_d, 8, hipMemcpyDeviceToHost, h->stream));
       SYNC_WD("device sync");
+      if (first_round) cnt2[1] = nmid;
       uint32_t nj = cnt2[0];
       if (nj == 0) break;
       if (!cfg->hits && h->bulk_after < 0 &&
           ((uint64_t)cnt2[1] <= (uint64_t)kBulkRatio * nj || cnt2[1] <= kBulkJobs)) {
-        // drop this round's jobs: the batch covers them; cur back to "none" for the minimum
-        hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, nt * 2,
-                           kRankNone);
+        // drop this round's jobs: the batch covers them; cur back to "none" for the
+        // minimum (for the first round the batch branch does it)
+        if (round > 0)
+          hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, nt * 2,
+                             kRankNone);
         HIPCHK(h, hipGetLastError());
         go_bulk = true;
         round--;
