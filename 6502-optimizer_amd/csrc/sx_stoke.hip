@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <map>
 #include <set>
 #include <cmath>
 #include <cstring>
@@ -380,11 +381,50 @@ void uniform_weights(std::vector<uint32_t> &w) {
     }                                                                \
   } while (0)
 
-}  // namespace
+// FlagDiff (pkg/search/verifier.go:591-604) on the host: the P bits in which
+// the two sequences' outputs differ over the 8 QuickCheck vectors, 0 when
+// another byte differs.
+uint32_t host_flag_diff(const uint32_t *t, int tl, const uint32_t *c, int cl) {
+  static const uint8_t vec[32][8] = SX_MID_VECTORS;
+  uint32_t d = 0;
+  for (int v = 0; v < 8; v++) {
+    State a = unpack(vec[v][0] | vec[v][1] << 8 | vec[v][2] << 16 | (uint32_t)vec[v][3] << 24,
+                     vec[v][4] | vec[v][5] << 8 | vec[v][6] << 16 | (uint32_t)vec[v][7] << 24);
+    State b = a;
+    for (int k = 0; k < tl; k++) step(a, t[k] & 0xFFFFu, t[k] >> 16);
+    for (int k = 0; k < cl; k++) step(b, c[k] & 0xFFFFu, c[k] >> 16);
+    if (((pack_lo(a) ^ pack_lo(b)) & 0x00FFFFFFu) || pack_hi(a) != pack_hi(b)) return 0;
+    d |= ((a.p ^ b.p) & kPMask);
+  }
+  return d;
+}
 
-extern "C" int sx_stoke_run(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_t *targets, const uint8_t *lens,
-                            uint64_t n, sx_stoke_result *out, sx_stoke_chain *chains_out) {
-  if (!h || !cfg || (n && (!targets || !lens || !out))) return SX_EINVAL;
+// The chain state a chain starts (and, collecting, restarts) from: its
+// target at temperature t0 (pkg/stoke/mcmc.go:28-47); the step counter runs on.
+ChainState start_state(const uint32_t *targets, const uint8_t *lens, uint32_t ti, double t0, uint64_t step,
+                       uint64_t accepted) {
+  ChainState s;
+  memset(&s, 0, sizeof(s));
+  int bytes = 0;
+  for (int k = 0; k < kMaxSeq; k++) s.seq[k] = k < lens[ti] ? targets[ti * 8 + k] : 0u;
+  for (int k = 0; k < lens[ti]; k++) bytes += byte_size(targets[ti * 8 + k] & 0xFFFFu, 0);
+  s.len = lens[ti];
+  s.cost10 = bytes;
+  s.temp = t0;
+  s.step = step;
+  s.accepted = accepted;
+  return s;
+}
+
+// Both STOKE entry points.  first: out[n] gets each target's first verified
+// rewrite (smallest step, then chain, among the launch's verified posts).
+// found != null (collect, pkg/stoke/search.go:101-153 with Deduplicate,
+// :179-190): every verified rewrite is kept once per (target, replacement) in
+// the order found (launch, then target, step, chain), and the chain that
+// posted it restarts from its target at t0 at the next launch.
+int stoke_core(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_t *targets, const uint8_t *lens, uint64_t n,
+               sx_stoke_result *out, std::vector<sx_stoke_found> *found, sx_stoke_chain *chains_out) {
+  if (!h || !cfg || (n && (!targets || !lens)) || (n && !out && !found)) return SX_EINVAL;
   if (cfg->struct_size != sizeof(sx_stoke_cfg)) return sx_internal_error(h, SX_EINVAL, "sx_stoke_cfg size mismatch");
   if (cfg->chains_per_target == 0 || cfg->dead_flags > 0xFF || !(cfg->t0 > 0) || !(cfg->decay > 0) ||
       cfg->decay > 1.0)
@@ -394,11 +434,12 @@ extern "C" int sx_stoke_run(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_
     for (int k = 0; k < lens[i]; k++)
       if ((targets[i * 8 + k] & 0xFFFFu) >= OP_COUNT) return sx_internal_error(h, SX_EOPCODE, "opcode out of range");
   }
-  memset(out, 0, n * sizeof(sx_stoke_result));
+  if (out) memset(out, 0, n * sizeof(sx_stoke_result));
   if (n == 0) return SX_OK;
+  const bool collect = found != nullptr;
   const uint64_t nch64 = n * (uint64_t)cfg->chains_per_target;
   if (nch64 > (1u << 26)) return sx_internal_error(h, SX_EINVAL, "too many chains");
-  const uint32_t nch = (uint32_t)nch64;
+  const uint32_t nch = (uint32_t)nch64, cpt = cfg->chains_per_target;
   hipStream_t stream = sx_internal_stream(h);
   int rc = SX_OK;
   ChainState *d_ch = nullptr;
@@ -413,8 +454,10 @@ extern "C" int sx_stoke_run(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_
   std::vector<ChainState> hch(nch);
   std::vector<Post> hpost((size_t)nch * kPosts);
   std::vector<uint32_t> stoph(n, 0);
+  // verdicts of (target, candidate) pairs already verified: a repeat is not swept again
+  std::map<std::vector<uint32_t>, bool> verdict;
+  std::set<std::vector<uint32_t>> kept;  // collect: (target, replacement) pairs already reported
   auto t_start = std::chrono::steady_clock::now();
-  uint64_t done_steps = 0;
   if (per_target) {
     w.assign(cfg->weights, cfg->weights + ntab * kNumInsts);
     for (uint64_t t = 0; t < ntab; t++) {
@@ -445,98 +488,133 @@ extern "C" int sx_stoke_run(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_
   HCHK(hipMemcpyAsync(d_w, w.data(), w.size() * 4, hipMemcpyHostToDevice, stream));
   HCHK(hipMemcpyAsync(d_cum, cumh.data(), cumh.size() * 8, hipMemcpyHostToDevice, stream));
   HCHK(hipMemsetAsync(d_stop, 0, n * 4, stream));
-  hipLaunchKernelGGL(k_stoke_init, dim3((nch + 255) / 256), dim3(256), 0, stream, d_ch, nch,
-                     cfg->chains_per_target, d_tw, d_tl, cfg->t0);
+  hipLaunchKernelGGL(k_stoke_init, dim3((nch + 255) / 256), dim3(256), 0, stream, d_ch, nch, cpt, d_tw, d_tl,
+                     cfg->t0);
   HCHK(hipGetLastError());
   HCHK(hipStreamSynchronize(stream));
   t_start = std::chrono::steady_clock::now();
   {
-    const uint32_t per_launch = cfg->steps_per_launch ? cfg->steps_per_launch : 4096;
+    // launch lengths: the caller's, else 256 steps doubling up to 4,096 (posts
+    // are verified between launches, so a short first launch reports an early
+    // find early, and long later launches keep the launch overhead small)
+    uint32_t per_launch = cfg->steps_per_launch ? cfg->steps_per_launch : 256;
     const uint32_t pmask = kPMask & ~cfg->dead_flags;
     uint64_t remaining = cfg->steps;
     uint32_t open = (uint32_t)n;
     while (remaining > 0 && open > 0) {
       const uint32_t s = (uint32_t)std::min<uint64_t>(remaining, per_launch);
-      hipLaunchKernelGGL(k_stoke, dim3((nch + 3) / 4), dim3(256), 0, stream, d_ch, nch, cfg->chains_per_target,
-                         d_tw, d_tl, d_cum, d_w, per_target, s, (uint32_t)cfg->seed,
-                         (uint32_t)(cfg->seed >> 32), cfg->decay, pmask, d_stop, d_post);
+      hipLaunchKernelGGL(k_stoke, dim3((nch + 3) / 4), dim3(256), 0, stream, d_ch, nch, cpt, d_tw, d_tl, d_cum, d_w,
+                         per_target, s, (uint32_t)cfg->seed, (uint32_t)(cfg->seed >> 32), cfg->decay, pmask, d_stop,
+                         d_post);
       HCHK(hipGetLastError());
       HCHK(hipMemcpyAsync(hch.data(), d_ch, nch * sizeof(ChainState), hipMemcpyDeviceToHost, stream));
       HCHK(hipStreamSynchronize(stream));
       remaining -= s;
-      done_steps += s;
-      // verify posted candidates
+      if (!cfg->steps_per_launch) per_launch = std::min<uint32_t>(per_launch * 2, 4096);
       std::vector<uint32_t> cand;
       for (uint32_t c = 0; c < nch; c++)
         if (hch[c].posted) cand.push_back(c);
-      if (!cand.empty()) {
-        HCHK(hipMemcpyAsync(hpost.data(), d_post, (size_t)nch * kPosts * sizeof(Post), hipMemcpyDeviceToHost,
-                            stream));
-        HCHK(hipStreamSynchronize(stream));
+      if (cand.empty()) continue;
+      HCHK(hipMemcpyAsync(hpost.data(), d_post, (size_t)nch * kPosts * sizeof(Post), hipMemcpyDeviceToHost, stream));
+      HCHK(hipStreamSynchronize(stream));
+      // the launch's posts in (target, step, chain) order; pairs not yet verified go to one batch
+      std::vector<uint64_t> ids;  // post slot index c * kPosts + k
+      for (uint32_t c : cand) {
+        const uint32_t ti = c / cpt;
+        if (!collect && (stoph[ti] || out[ti].verified)) continue;  // only a target's first verified rewrite
+        for (uint32_t k = 0; k < hch[c].posted && k < (uint32_t)kPosts; k++) ids.push_back((uint64_t)c * kPosts + k);
+      }
+      std::sort(ids.begin(), ids.end(), [&](uint64_t a, uint64_t b) {
+        const uint32_t ta = (uint32_t)(a / kPosts) / cpt, tb = (uint32_t)(b / kPosts) / cpt;
+        if (ta != tb) return ta < tb;
+        if (hpost[a].step != hpost[b].step) return hpost[a].step < hpost[b].step;
+        return a < b;
+      });
+      auto key_of = [&](uint64_t id) {
+        const Post &pp = hpost[id];
+        std::vector<uint32_t> key(pp.seq, pp.seq + pp.len);
+        key.push_back(0xFFFFFFFFu);  // separator: a key is (replacement, target)
+        key.push_back((uint32_t)(id / kPosts) / cpt);
+        return key;
+      };
+      {
         std::vector<uint32_t> vt, vc;
         std::vector<uint8_t> vtl, vcl;
-        std::vector<uint64_t> ids;  // post slot index c * kPosts + k
-        std::set<std::vector<uint32_t>> seen;  // (target, candidate) pairs already queued this launch
-        for (uint32_t c : cand) {
-          uint32_t ti = c / cfg->chains_per_target;
-          if (stoph[ti] || out[ti].verified) continue;  // only a target's first verified rewrite is reported
-          for (uint32_t k = 0; k < hch[c].posted && k < (uint32_t)kPosts; k++) {
-            const Post &pp = hpost[(size_t)c * kPosts + k];
-            std::vector<uint32_t> key(pp.seq, pp.seq + pp.len);
-            key.push_back(ti);
-            if (!seen.insert(key).second) continue;
-            ids.push_back((uint64_t)c * kPosts + k);
-            for (int q = 0; q < kMaxSeq; q++) vt.push_back(q < lens[ti] ? targets[ti * 8 + q] : 0u);
-            for (int q = 0; q < kMaxSeq; q++) vc.push_back(q < pp.len ? pp.seq[q] : 0u);
-            vtl.push_back(lens[ti]);
-            vcl.push_back((uint8_t)pp.len);
-          }
+        std::vector<std::vector<uint32_t>> vkeys;
+        std::set<std::vector<uint32_t>> queued;
+        for (uint64_t id : ids) {
+          auto key = key_of(id);
+          if (verdict.count(key) || !queued.insert(key).second) continue;
+          const Post &pp = hpost[id];
+          const uint32_t ti = (uint32_t)(id / kPosts) / cpt;
+          for (int q = 0; q < kMaxSeq; q++) vt.push_back(q < lens[ti] ? targets[ti * 8 + q] : 0u);
+          for (int q = 0; q < kMaxSeq; q++) vc.push_back(q < pp.len ? pp.seq[q] : 0u);
+          vtl.push_back(lens[ti]);
+          vcl.push_back((uint8_t)pp.len);
+          vkeys.push_back(key);
         }
-        if (!ids.empty()) {
-          std::vector<uint8_t> res(ids.size());
-          rc = sx_verify_seqs(h, vt.data(), vtl.data(), vc.data(), vcl.data(), ids.size(), cfg->dead_flags,
+        if (!vkeys.empty()) {
+          std::vector<uint8_t> res(vkeys.size());
+          rc = sx_verify_seqs(h, vt.data(), vtl.data(), vc.data(), vcl.data(), vkeys.size(), cfg->dead_flags,
                               res.data());
           if (rc) goto done;
-          double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-          // per target, the verified post with the smallest step (then chain) wins
-          std::vector<int64_t> win(n, -1);
-          for (size_t i = 0; i < ids.size(); i++) {
-            if (!res[i]) continue;
-            const uint32_t c = (uint32_t)(ids[i] / kPosts), ti = c / cfg->chains_per_target;
-            if (out[ti].verified) continue;
-            const int64_t b = win[ti];
-            if (b < 0 || hpost[ids[i]].step < hpost[(size_t)b].step) win[ti] = (int64_t)ids[i];
-          }
-          for (uint64_t ti = 0; ti < n; ti++) {
-            if (win[ti] < 0) continue;
-            const Post &pp = hpost[(size_t)win[ti]];
-            sx_stoke_result &o = out[ti];
-            o.verified = 1;
-            o.rep_len = (uint8_t)pp.len;
-            for (int k = 0; k < kMaxSeq; k++) o.rep[k] = k < pp.len ? pp.seq[k] : 0u;
-            int tb = 0, tc = 0, cb = 0, cc = 0;
-            for (int k = 0; k < lens[ti]; k++)
-              tb += byte_size(targets[ti * 8 + k] & 0xFFFFu, 0), tc += cycles(targets[ti * 8 + k] & 0xFFFFu);
-            for (int k = 0; k < pp.len; k++)
-              cb += byte_size(pp.seq[k] & 0xFFFFu, 0), cc += cycles(pp.seq[k] & 0xFFFFu);
-            o.bytes_saved = (int16_t)(tb - cb);
-            o.cycles_saved = (int16_t)(tc - cc);
-            o.chain = (uint32_t)(win[ti] / kPosts);
-            o.step = pp.step;
-            o.seconds = secs;
-            if (cfg->stop_when_verified) {
-              stoph[ti] = 1;
-              open--;
-            }
-          }
+          for (size_t i = 0; i < vkeys.size(); i++) verdict[vkeys[i]] = res[i] != 0;
         }
-        // clear every posted flag so chains can post again
-        for (uint32_t c : cand) hch[c].posted = 0;
-        HCHK(hipMemcpyAsync(d_ch, hch.data(), nch * sizeof(ChainState), hipMemcpyHostToDevice, stream));
-        if (cfg->stop_when_verified)
-          HCHK(hipMemcpyAsync(d_stop, stoph.data(), n * 4, hipMemcpyHostToDevice, stream));
-        HCHK(hipStreamSynchronize(stream));
       }
+      const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+      std::vector<uint8_t> restart(collect ? nch : 0, 0);
+      for (uint64_t id : ids) {
+        if (!verdict[key_of(id)]) continue;
+        const Post &pp = hpost[id];
+        const uint32_t c = (uint32_t)(id / kPosts), ti = c / cpt;
+        int tb = 0, tc = 0, cb = 0, cc = 0;
+        for (int k = 0; k < lens[ti]; k++)
+          tb += byte_size(targets[ti * 8 + k] & 0xFFFFu, 0), tc += cycles(targets[ti * 8 + k] & 0xFFFFu);
+        for (int k = 0; k < pp.len; k++) cb += byte_size(pp.seq[k] & 0xFFFFu, 0), cc += cycles(pp.seq[k] & 0xFFFFu);
+        // search.go:106-110: with dead flags the rule's mask is FlagDiff(target, best)
+        const uint32_t fd = cfg->dead_flags ? host_flag_diff(targets + ti * 8, lens[ti], pp.seq, pp.len) : 0u;
+        if (collect) {
+          restart[c] = 1;
+          if (!kept.insert(key_of(id)).second) continue;
+          sx_stoke_found f;
+          memset(&f, 0, sizeof(f));
+          f.target = ti;
+          f.chain = c;
+          f.step = pp.step;
+          f.seconds = secs;
+          for (int k = 0; k < kMaxSeq; k++) f.rep[k] = k < pp.len ? pp.seq[k] : 0u;
+          f.rep_len = (uint8_t)pp.len;
+          f.dead_flags = (uint8_t)fd;
+          f.bytes_saved = (int16_t)(tb - cb);
+          f.cycles_saved = (int16_t)(tc - cc);
+          found->push_back(f);
+          continue;
+        }
+        sx_stoke_result &o = out[ti];
+        if (o.verified) continue;  // ids are in (target, step, chain) order: the first verified post wins
+        o.verified = 1;
+        o.rep_len = (uint8_t)pp.len;
+        for (int k = 0; k < kMaxSeq; k++) o.rep[k] = k < pp.len ? pp.seq[k] : 0u;
+        o.dead_flags = (uint8_t)fd;
+        o.bytes_saved = (int16_t)(tb - cb);
+        o.cycles_saved = (int16_t)(tc - cc);
+        o.chain = c;
+        o.step = pp.step;
+        o.seconds = secs;
+        if (cfg->stop_when_verified) {
+          stoph[ti] = 1;
+          open--;
+        }
+      }
+      // clear every posted flag so chains can post again; restart the chains that found a rewrite
+      for (uint32_t c : cand) {
+        hch[c].posted = 0;
+        if (collect && restart[c]) hch[c] = start_state(targets, lens, c / cpt, cfg->t0, hch[c].step, hch[c].accepted);
+      }
+      HCHK(hipMemcpyAsync(d_ch, hch.data(), nch * sizeof(ChainState), hipMemcpyHostToDevice, stream));
+      if (!collect && cfg->stop_when_verified)
+        HCHK(hipMemcpyAsync(d_stop, stoph.data(), n * 4, hipMemcpyHostToDevice, stream));
+      HCHK(hipStreamSynchronize(stream));
     }
   }
   if (chains_out) {
@@ -557,6 +635,26 @@ done:
   if (d_w) (void)hipFree(d_w);
   if (d_cum) (void)hipFree(d_cum);
   if (d_stop) (void)hipFree(d_stop);
-  (void)done_steps;
   return rc;
+}
+
+}  // namespace
+
+extern "C" int sx_stoke_run(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_t *targets, const uint8_t *lens,
+                            uint64_t n, sx_stoke_result *out, sx_stoke_chain *chains_out) {
+  if (n && !out) return SX_EINVAL;
+  return stoke_core(h, cfg, targets, lens, n, out, nullptr, chains_out);
+}
+
+extern "C" int sx_stoke_collect(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_t *targets, const uint8_t *lens,
+                                uint64_t n, sx_stoke_found *found, uint64_t cap, uint64_t *n_found,
+                                sx_stoke_chain *chains_out) {
+  if (!n_found || (cap && !found)) return SX_EINVAL;
+  *n_found = 0;
+  std::vector<sx_stoke_found> all;
+  int rc = stoke_core(h, cfg, targets, lens, n, nullptr, &all, chains_out);
+  if (rc) return rc;
+  *n_found = all.size();
+  if (cap) memcpy(found, all.data(), std::min<uint64_t>(cap, all.size()) * sizeof(sx_stoke_found));
+  return SX_OK;
 }

@@ -176,24 +176,29 @@ std::string dead_flag_desc(int mask) {
 std::string json_str(const std::string &s) { return "\"" + s + "\""; }
 
 // pkg/result/output.go:21-35 with table.go:38-51 ordering.
-void write_json(FILE *f, std::vector<sx_rule> rules) {
-  std::stable_sort(rules.begin(), rules.end(), [](const sx_rule &a, const sx_rule &b) {
-    if (a.bytes_saved != b.bytes_saved) return a.bytes_saved > b.bytes_saved;
-    return a.cycles_saved > b.cycles_saved;
-  });
+// A rule for the JSON writer: sequences of any length (STOKE's rewrites run
+// to 10 instructions; sx_rule holds 4).
+struct JsonRule {
+  std::vector<uint32_t> src, rep;
+  int bytes_saved, cycles_saved;
+  uint32_t dead_flags;
+};
+
+// result.WriteJSON (pkg/result/output.go:20-35): the rules in the order given.
+void write_json(FILE *f, const std::vector<JsonRule> &rules) {
   if (rules.empty()) {
     fputs("[]\n", f);
     return;
   }
   fputs("[\n", f);
   for (size_t i = 0; i < rules.size(); i++) {
-    const sx_rule &r = rules[i];
+    const JsonRule &r = rules[i];
     fputs("  {\n    \"source\": [\n", f);
-    for (int k = 0; k < r.src_len; k++)
-      fprintf(f, "      %s%s\n", json_str(disasm(r.src[k])).c_str(), k + 1 < r.src_len ? "," : "");
+    for (size_t k = 0; k < r.src.size(); k++)
+      fprintf(f, "      %s%s\n", json_str(disasm(r.src[k])).c_str(), k + 1 < r.src.size() ? "," : "");
     fputs("    ],\n    \"replacement\": [\n", f);
-    for (int k = 0; k < r.rep_len; k++)
-      fprintf(f, "      %s%s\n", json_str(disasm(r.rep[k])).c_str(), k + 1 < r.rep_len ? "," : "");
+    for (size_t k = 0; k < r.rep.size(); k++)
+      fprintf(f, "      %s%s\n", json_str(disasm(r.rep[k])).c_str(), k + 1 < r.rep.size() ? "," : "");
     fprintf(f, "    ],\n    \"bytes_saved\": %d,\n    \"cycles_saved\": %d", r.bytes_saved, r.cycles_saved);
     std::string d = dead_flag_desc(r.dead_flags);
     if (!d.empty()) fprintf(f, ",\n    \"dead_flags\": \"%s\"", d.c_str());
@@ -202,7 +207,20 @@ void write_json(FILE *f, std::vector<sx_rule> rules) {
   fputs("]\n", f);
 }
 
-// cuda/6502_search_v2.cu:912-930 schema, written through one buffer.
+// An enumeration's rules as table.Rules() hands them to WriteJSON
+// (pkg/result/table.go:38-52): by bytes saved, then cycles saved, descending.
+void write_json(FILE *f, const std::vector<sx_rule> &rules) {
+  std::vector<JsonRule> j;
+  j.reserve(rules.size());
+  for (const sx_rule &r : rules)
+    j.push_back({std::vector<uint32_t>(r.src, r.src + r.src_len), std::vector<uint32_t>(r.rep, r.rep + r.rep_len),
+                 r.bytes_saved, r.cycles_saved, r.dead_flags});
+  std::stable_sort(j.begin(), j.end(), [](const JsonRule &a, const JsonRule &b) {
+    if (a.bytes_saved != b.bytes_saved) return a.bytes_saved > b.bytes_saved;
+    return a.cycles_saved > b.cycles_saved;
+  });
+  write_json(f, j);
+}
 void write_jsonl(FILE *f, const std::vector<sx_rule> &rules, int model) {
   std::string buf;
   buf.reserve(rules.size() * 128);
@@ -565,45 +583,48 @@ int cmd_stoke(int argc, char **argv) {
   cfg.t0 = 1.0;
   cfg.seed = strtoull(a.get("seed", "0").c_str(), nullptr, 10);
   cfg.dead_flags = dead;
-  cfg.stop_when_verified = 1;
   uint32_t tw[8] = {0};
   for (size_t k = 0; k < seq.size(); k++) tw[k] = seq[k];
   uint8_t tl = (uint8_t)seq.size();
   sx_handle *h = nullptr;
   if (check(nullptr, sx_open(std::atoi(a.get("device", "0").c_str()), &h), "sx_open")) return 1;
-  sx_stoke_result res;
+  // stoke.Run then stoke.Deduplicate (main.go:231-232)
+  std::vector<sx_stoke_found> found(4096);
+  uint64_t nf = 0;
   auto t0 = std::chrono::steady_clock::now();
-  if (check(h, sx_stoke_run(h, &cfg, tw, &tl, 1, &res, nullptr), "sx_stoke_run")) return 1;
+  if (check(h, sx_stoke_collect(h, &cfg, tw, &tl, 1, found.data(), found.size(), &nf, nullptr), "sx_stoke_collect"))
+    return 1;
+  if (nf > found.size()) {
+    found.resize(nf);
+    if (check(h, sx_stoke_collect(h, &cfg, tw, &tl, 1, found.data(), found.size(), &nf, nullptr),
+              "sx_stoke_collect"))
+      return 1;
+  }
+  found.resize(std::min<uint64_t>(nf, found.size()));
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   sx_close(h);
-  std::vector<sx_rule> rules;
-  if (res.verified) {
-    sx_rule r;
-    memset(&r, 0, sizeof(r));
-    for (size_t k = 0; k < seq.size() && k < 4; k++) r.src[k] = seq[k];
-    for (int k = 0; k < res.rep_len && k < 4; k++) r.rep[k] = res.rep[k];
-    r.src_len = (uint8_t)std::min<size_t>(seq.size(), 4);
-    r.rep_len = (uint8_t)std::min<int>(res.rep_len, 4);
-    r.dead_flags = (uint8_t)dead;
-    r.bytes_saved = res.bytes_saved;
-    r.cycles_saved = res.cycles_saved;
-    rules.push_back(r);
+  if (a.has("verbose")) {
+    fprintf(stderr, "  %u chains x %llu iterations, %.3f s\n", cfg.chains_per_target,
+            (unsigned long long)cfg.steps, secs);
+    for (const sx_stoke_found &r : found)
+      fprintf(stderr, "  Chain %u @ iter %llu: %s VERIFIED\n", r.chain, (unsigned long long)r.step,
+              disasm_seq(r.rep, r.rep_len).c_str());
   }
-  if (a.has("verbose"))
-    fprintf(stderr, "  %u chains, %.3f s%s\n", cfg.chains_per_target, secs,
-            res.verified ? "" : ", nothing verified");
-  printf("\n%zu unique optimizations found\n", rules.size());
-  if (res.verified) {
-    printf("  1. %s", disasm_seq(res.rep, res.rep_len).c_str());
-    if (dead)
-      printf(" (-%d bytes, -%d cycles, dead flags: %s)\n", res.bytes_saved, res.cycles_saved,
-             dead_flag_desc((int)dead).c_str());
+  printf("\n%zu unique optimizations found\n", found.size());
+  std::vector<JsonRule> rules;
+  for (size_t i = 0; i < found.size(); i++) {
+    const sx_stoke_found &r = found[i];
+    printf("  %zu. %s", i + 1, disasm_seq(r.rep, r.rep_len).c_str());
+    if (r.dead_flags)
+      printf(" (-%d bytes, -%d cycles, dead flags: %s)\n", r.bytes_saved, r.cycles_saved,
+             dead_flag_desc(r.dead_flags).c_str());
     else
-      printf(" (-%d bytes, -%d cycles)\n", res.bytes_saved, res.cycles_saved);
+      printf(" (-%d bytes, -%d cycles)\n", r.bytes_saved, r.cycles_saved);
+    rules.push_back({seq, std::vector<uint32_t>(r.rep, r.rep + r.rep_len), r.bytes_saved, r.cycles_saved,
+                     r.dead_flags});
   }
   std::string outp = a.get("output", "");
   if (!outp.empty() && !rules.empty()) {
-    if (seq.size() > 4 || res.rep_len > 4) return die("--output holds sequences of up to 4 instructions");
     FILE *f = fopen(outp.c_str(), "w");
     if (!f) return die("cannot create " + outp);
     write_json(f, rules);
@@ -625,7 +646,14 @@ bool json_field(const std::string &line, const std::string &key, std::string *ou
   std::string v;
   for (size_t i = p + 1; i < line.size(); i++) {
     if (line[i] == '\\' && i + 1 < line.size()) {
-      v += line[++i];
+      const char e = line[++i];
+      if (e == 'u' && i + 4 < line.size()) {  // \uXXXX (Go's encoder writes <, > and & so)
+        const unsigned long c = strtoul(line.substr(i + 1, 4).c_str(), nullptr, 16);
+        v += c < 0x80 ? (char)c : '?';
+        i += 4;
+      } else {
+        v += e == 'n' ? '\n' : e == 't' ? '\t' : e == 'r' ? '\r' : e == 'b' ? '\b' : e == 'f' ? '\f' : e;
+      }
       continue;
     }
     if (line[i] == '"') {
@@ -656,11 +684,14 @@ int cmd_verify_jsonl(int argc, char **argv) {
     if (line.empty()) continue;
     total++;
     std::string s, r, err;
-    if (!json_field(line, "source_asm", &s) || !json_field(line, "replacement_asm", &r)) {
+    // an object without a field decodes it as "" (Go's zero value), which then fails to parse
+    if (line.front() != '{' || line.back() != '}') {
       fprintf(stderr, "  [%d] JSON parse error\n", total);
       skipped++;
       continue;
     }
+    if (!json_field(line, "source_asm", &s)) s.clear();
+    if (!json_field(line, "replacement_asm", &r)) r.clear();
     std::vector<uint32_t> src, rep;
     if (!parse_assembly(s, &src, &err)) {
       fprintf(stderr, "  [%d] Cannot parse source \"%s\": %s\n", total, s.c_str(), err.c_str());

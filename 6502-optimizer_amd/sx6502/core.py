@@ -23,7 +23,7 @@ EXH_PROJECTED, EXH_FULL = 0, 1
 # Every symbol include/sx6502.h declares.
 EXPORTS = ("sx_abi_version", "sx_open", "sx_close", "sx_last_error", "sx_enumerate", "sx_fetch_rules",
            "sx_pruned_targets", "sx_check_pairs", "sx_flag_diff", "sx_exec_states", "sx_exh_plan",
-           "sx_dep_table", "sx_stoke_run", "sx_mlp_param_count", "sx_mlp_create", "sx_mlp_destroy",
+           "sx_dep_table", "sx_stoke_run", "sx_stoke_collect", "sx_mlp_param_count", "sx_mlp_create", "sx_mlp_destroy",
            "sx_mlp_get_params", "sx_mlp_set_params", "sx_mlp_loss_grad", "sx_mlp_train", "sx_mlp_predict",
            "sx_proposal_weights", "sx_mlp_proposal", "sx_mfma_gemm_test", "sx_target_search",
            "sx_measure_int32_rate", "sx_check_pairs_noalloc", "sx_fetch_empty_rules")
@@ -77,9 +77,16 @@ class StokeCfg(C.Structure):
 
 
 class StokeResult(C.Structure):
-    _fields_ = [("rep", C.c_uint32 * 10), ("rep_len", C.c_uint8), ("verified", C.c_uint8), ("pad", C.c_uint8 * 2),
+    _fields_ = [("rep", C.c_uint32 * 10), ("rep_len", C.c_uint8), ("verified", C.c_uint8), ("dead_flags", C.c_uint8),
+                ("pad", C.c_uint8),
                 ("bytes_saved", C.c_int16), ("cycles_saved", C.c_int16), ("chain", C.c_uint32),
                 ("step", C.c_uint64), ("seconds", C.c_double)]
+
+
+class StokeFound(C.Structure):
+    _fields_ = [("target", C.c_uint32), ("chain", C.c_uint32), ("step", C.c_uint64), ("seconds", C.c_double),
+                ("rep", C.c_uint32 * 10), ("rep_len", C.c_uint8), ("dead_flags", C.c_uint8),
+                ("bytes_saved", C.c_int16), ("cycles_saved", C.c_int16), ("pad", C.c_uint16)]
 
 
 class StokeChain(C.Structure):
@@ -90,6 +97,8 @@ class StokeChain(C.Structure):
 # numpy views with the C layout (alignment included), checked against ctypes
 STOKE_RESULT_DTYPE = np.dtype(StokeResult)
 STOKE_CHAIN_DTYPE = np.dtype(StokeChain)
+STOKE_FOUND_DTYPE = np.dtype(StokeFound)
+assert STOKE_FOUND_DTYPE.itemsize == C.sizeof(StokeFound) == 72
 assert STOKE_RESULT_DTYPE.itemsize == C.sizeof(StokeResult) == 72
 assert STOKE_CHAIN_DTYPE.itemsize == C.sizeof(StokeChain) == 64
 
@@ -139,6 +148,8 @@ def lib() -> C.CDLL:
                               C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
     L.sx_dep_table.argtypes = [C.POINTER(C.c_uint16)]
     L.sx_stoke_run.argtypes = [vp, C.POINTER(StokeCfg), u32p, u8p, C.c_uint64, vp, vp]
+    L.sx_stoke_collect.argtypes = [vp, C.POINTER(StokeCfg), u32p, u8p, C.c_uint64, vp, C.c_uint64,
+                                   C.POINTER(C.c_uint64), vp]
     u16p, f32p = C.POINTER(C.c_uint16), C.POINTER(C.c_float)
     L.sx_mlp_create.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
     L.sx_mlp_destroy.argtypes = [vp]
@@ -354,12 +365,8 @@ class Device:
         return st
 
     # -- STOKE -----------------------------------------------------------
-    def stoke(self, targets, *, chains: int = 16, steps: int = 100000, steps_per_launch: int = 0,
-              t0: float = 1.0, decay: float = 0.9999, seed: int = 0, dead: int = 0, weights=None,
-              stop_when_verified: bool = True, return_chains: bool = False):
-        """STOKE over target sequences (lists of up to 8 words); returns the
-        per-target results (STOKE_RESULT_DTYPE) and, on request, the final
-        chain states."""
+    @staticmethod
+    def _stoke_cfg(targets, chains, steps, steps_per_launch, t0, decay, seed, dead, weights, stop_when_verified):
         n = len(targets)
         tw = np.zeros((n, 8), np.uint32)
         tl = np.zeros(n, np.uint8)
@@ -376,6 +383,17 @@ class Device:
             keep = np.ascontiguousarray(weights, dtype=np.uint32)
             assert keep.shape == (n, 2863)
             cfg.weights = _p32(keep)
+        return cfg, tw, tl, keep
+
+    def stoke(self, targets, *, chains: int = 16, steps: int = 100000, steps_per_launch: int = 0,
+              t0: float = 1.0, decay: float = 0.9999, seed: int = 0, dead: int = 0, weights=None,
+              stop_when_verified: bool = True, return_chains: bool = False):
+        """STOKE over target sequences (lists of up to 8 words); returns the
+        per-target results (STOKE_RESULT_DTYPE) and, on request, the final
+        chain states."""
+        n = len(targets)
+        cfg, tw, tl, keep = self._stoke_cfg(targets, chains, steps, steps_per_launch, t0, decay, seed, dead, weights,
+                                            stop_when_verified)
         out = np.zeros(n, STOKE_RESULT_DTYPE)
         ch = np.zeros(n * chains, STOKE_CHAIN_DTYPE) if return_chains else None
         rc = lib().sx_stoke_run(self._h, C.byref(cfg), _p32(tw), _p8(tl), n, out.ctypes.data_as(C.c_void_p),
@@ -383,6 +401,25 @@ class Device:
         self._check(rc, "sx_stoke_run")
         del keep
         return (out, ch) if return_chains else out
+
+    def stoke_collect(self, targets, *, chains: int = 16, steps: int = 100000, steps_per_launch: int = 0,
+                      t0: float = 1.0, decay: float = 0.9999, seed: int = 0, dead: int = 0, weights=None,
+                      cap: int = 1 << 16, return_chains: bool = False):
+        """sx_stoke_collect (stoke.Run + Deduplicate): every verified rewrite,
+        once per (target, replacement), as STOKE_FOUND_DTYPE, and the number
+        found (which exceeds the array's length when cap was too small)."""
+        n = len(targets)
+        cfg, tw, tl, keep = self._stoke_cfg(targets, chains, steps, steps_per_launch, t0, decay, seed, dead, weights,
+                                            False)
+        out = np.zeros(cap, STOKE_FOUND_DTYPE)
+        ch = np.zeros(n * chains, STOKE_CHAIN_DTYPE) if return_chains else None
+        nf = C.c_uint64(0)
+        rc = lib().sx_stoke_collect(self._h, C.byref(cfg), _p32(tw), _p8(tl), n, out.ctypes.data_as(C.c_void_p), cap,
+                                    C.byref(nf), ch.ctypes.data_as(C.c_void_p) if ch is not None else None)
+        self._check(rc, "sx_stoke_collect")
+        del keep
+        found = out[: min(cap, nf.value)]
+        return (found, nf.value, ch) if return_chains else (found, nf.value)
 
     # -- target command --------------------------------------------------
     def target_search(self, target: Sequence[int], max_cand_len: int = 4, dead: int = 0, bytes_model: int = 0):

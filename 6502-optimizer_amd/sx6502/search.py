@@ -45,13 +45,19 @@ class Config:
 
 
 def run(cfg: Config) -> List[Rule]:
-    """All rules for targets of length 2..max_target_len (only 2 and 3 are
-    served), as search.Run builds its Table."""
+    """search.Run (pkg/search/search.go:21-54): every target length from 2 to
+    max_target_len, each against candidates of length 1..max_cand_len
+    (max_target_len - 1 when 0, for every target length: search.go:26-28,33);
+    given targets (one length), those alone.  Rules in target order, as the
+    device emits them (Table.Rules ordering: result.sorted_rules)."""
     max_cand = cfg.max_cand_len or cfg.max_target_len - 1
+    if cfg.targets is not None:
+        lengths = [cfg.targets.shape[1]]
+    else:
+        lengths = list(range(2, cfg.max_target_len + 1))
     rules: List[Rule] = []
-    for tl in range(2, cfg.max_target_len + 1):
-        arr, st = device().enumerate(cfg.targets if cfg.targets is not None and cfg.targets.shape[1] == tl else None,
-                                     target_len=tl, max_cand_len=min(max_cand, tl - 1), dead=cfg.dead_flags,
+    for tl in lengths:
+        arr, st = device().enumerate(cfg.targets, target_len=tl, max_cand_len=max_cand, dead=cfg.dead_flags,
                                      bytes_model=cfg.bytes_model, hits=cfg.hits, exh_mode=cfg.exh_mode,
                                      use_mid=cfg.use_mid, first=cfg.first_target, end=cfg.end_target)
         cfg.stats[tl] = st

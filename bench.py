@@ -264,6 +264,16 @@ def stoke_section(dev, steps: int, train_steps: int = 20000, harvest_targets: in
     wall_l = time.perf_counter() - t0
     m.close()
     su, sl = _stoke_stats(ru), _stoke_stats(rl, tables_s)
+    # the population both methods verified: p90 is defined there whatever the budget
+    both = ru["verified"].astype(bool) & rl["verified"].astype(bool)
+    cu, cl = ru["seconds"][both], rl["seconds"][both] + tables_s
+    common = {"targets": int(both.sum())}
+    if both.any():
+        common.update({
+            "uniform_median_s": float(np.median(cu)), "uniform_p90_s": float(np.quantile(cu, 0.9, method="inverted_cdf")),
+            "learned_median_s": float(np.median(cl)), "learned_p90_s": float(np.quantile(cl, 0.9, method="inverted_cdf"))})
+        common["speedup_median"] = common["uniform_median_s"] / common["learned_median_s"]
+        common["speedup_p90"] = common["uniform_p90_s"] / common["learned_p90_s"]
     return {
         "workload": "256 length-4..8 targets (seed 2) x 16 chains, <= %d steps per chain, T0 1.0 x0.9999/step, "
                     "cost = Hamming over 32 Mid vectors + 0.1 bytes, exact-ratio MH, rewrites verified by "
@@ -272,6 +282,9 @@ def stoke_section(dev, steps: int, train_steps: int = 20000, harvest_targets: in
         "uniform": dict(su, wall_s=wall_u),
         "learned": dict(sl, wall_s=wall_l, tables_s=tables_s),
         "speedup_median": (su["median_s"] / sl["median_s"]) if su["median_s"] and sl["median_s"] else None,
+        "common": common,
+        "one_time_cost_s": {"harvest": harvest_s, "train": train_s,
+                            "note": "paid once per trained network, outside each target's time"},
         "training": {"harvest_s": harvest_s, "train_s": train_s, "adam_steps": train_steps, "batch": 1024,
                      "examples": int(len(le) + len(ls)), "enumerator_rewrites": len(pairs_e),
                      "stoke_rewrites": len(pairs_s), "loss_first100": float(losses[:100].mean()),

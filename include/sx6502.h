@@ -228,7 +228,7 @@ typedef struct sx_stoke_cfg {
   uint32_t struct_size;      /* sizeof(sx_stoke_cfg) */
   uint32_t chains_per_target;
   uint64_t steps;            /* steps per chain */
-  uint32_t steps_per_launch; /* 0 = 4096 */
+  uint32_t steps_per_launch; /* 0 = 256 steps, doubling per launch up to 4,096 */
   int32_t stop_when_verified;/* stop a target's chains at its first verified rewrite */
   double t0, decay;
   uint64_t seed;
@@ -239,7 +239,10 @@ typedef struct sx_stoke_cfg {
 
 typedef struct sx_stoke_result {
   uint32_t rep[10];
-  uint8_t rep_len, verified, pad[2];
+  uint8_t rep_len, verified;
+  uint8_t dead_flags;        /* FlagDiff(target, rep) when dead_flags != 0, else 0
+                              * (pkg/stoke/search.go:104-110) */
+  uint8_t pad;
   int16_t bytes_saved, cycles_saved;
   uint32_t chain;            /* chain that found it */
   uint64_t step;             /* its step index */
@@ -256,6 +259,30 @@ typedef struct sx_stoke_chain {
  * optional n * chains_per_target final chain states. */
 int sx_stoke_run(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_t *targets, const uint8_t *lens,
                  uint64_t n, sx_stoke_result *out, sx_stoke_chain *chains_out);
+
+/* One verified rewrite of a collecting run. */
+typedef struct sx_stoke_found {
+  uint32_t target;           /* index into the targets */
+  uint32_t chain;            /* chain that found it */
+  uint64_t step;             /* its step index */
+  double seconds;            /* wall time from the start of the search to its verification */
+  uint32_t rep[10];
+  uint8_t rep_len;
+  uint8_t dead_flags;        /* FlagDiff(target, rep) when dead_flags != 0, else 0 */
+  int16_t bytes_saved, cycles_saved;
+  uint16_t pad;
+} sx_stoke_found;
+
+/* stoke.Run followed by stoke.Deduplicate (pkg/stoke/search.go:32-176,
+ * 179-190): every chain runs cfg->steps steps; every verified rewrite is kept
+ * once per (target, replacement), in the order found (launch, then target,
+ * step, chain), and the chain that found it restarts from its target at t0
+ * (search.go:150-153; here at the next launch boundary, where the host
+ * verifies posts).  stop_when_verified is ignored.  The first min(cap,
+ * *n_found) rewrites are copied to found; *n_found is the number found. */
+int sx_stoke_collect(sx_handle *h, const sx_stoke_cfg *cfg, const uint32_t *targets, const uint8_t *lens,
+                     uint64_t n, sx_stoke_found *found, uint64_t cap, uint64_t *n_found,
+                     sx_stoke_chain *chains_out);
 
 /* The per-opcode data-dependency table behind SX_EXH_PROJECTED, at flag
  * granularity: inputs and outputs are numbered A=0, X=1, Y=2, S=4, M=5, S0=6,

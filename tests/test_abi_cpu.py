@@ -65,24 +65,27 @@ def _outputs(st):
 
 
 def test_dependency_table_is_sound():
-    """For every opcode, output o and input q outside dep[op][o], changing q
-    (all 256 values of a byte, both values of a flag bit) on 64 random base
-    states never changes output o."""
+    """For every opcode (immediate opcodes at all 256 immediates), output o and
+    input q outside dep[op][o], changing q (all 256 values of a byte, both
+    values of a flag bit) on random base states never changes output o."""
     dep = core.dep_table()
     rng = np.random.default_rng(5)
-    base = rng.integers(0, 256, (64, 8)).astype(np.uint8)
     inputs = [(q, 256) for q in BYTES] + [(8 + b, 2) for b in range(8)]
     for op in range(58):
-        imm = int(rng.integers(0, 256))
+        imms = np.arange(256, dtype=np.uint32) if "_IMM" in o.OPS[op] else np.zeros(1, np.uint32)
+        nb = 8 if len(imms) > 1 else 64
+        base = rng.integers(0, 256, (nb, 8)).astype(np.uint8)
         for q, nv in inputs:
-            st = np.repeat(base, nv, axis=0)
-            vals = np.tile(np.arange(nv, dtype=np.uint8), 64)
+            # axes (immediate, base state, value of q)
+            st = np.broadcast_to(base[None, :, None, :], (len(imms), nb, nv, 8)).copy()
+            vals = np.arange(nv, dtype=np.uint8)[None, None, :]
             if q < 8:
-                st[:, q] = vals
+                st[..., q] = vals
             else:
                 b = q - 8
-                st[:, 3] = (st[:, 3] & ~np.uint8(1 << b)) | (vals << b)
-            res = o.exec_many(st, np.uint32(op | (imm << 16))).reshape(64, nv, 8)
+                st[..., 3] = (st[..., 3] & ~np.uint8(1 << b)) | (vals << b)
+            words = np.broadcast_to((op | (imms << 16))[:, None, None], (len(imms), nb, nv)).ravel()
+            res = o.exec_many(st.reshape(-1, 8), words).reshape(len(imms) * nb, nv, 8)
             for out_i, col in _outputs(res).items():
                 d = int(dep[op, out_i])
                 deps = (1 << out_i) if d == 0xFFFF else d
@@ -151,7 +154,8 @@ def test_ctypes_mirrors_match_the_c_header(tmp_path):
     import subprocess
     structs = {"sx_rule": core.RuleC, "sx_enum_cfg": core.EnumCfg, "sx_stats": core.Stats,
                "sx_stoke_cfg": core.StokeCfg, "sx_stoke_result": core.StokeResult,
-               "sx_stoke_chain": core.StokeChain, "sx_target_result": core.TargetResult}
+               "sx_stoke_chain": core.StokeChain, "sx_target_result": core.TargetResult,
+               "sx_stoke_found": core.StokeFound}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sx6502.h"', 'int main(void) {']
     for name, py in structs.items():
         lines.append(f'  printf("{name} size %zu\\n", sizeof({name}));')

@@ -41,9 +41,19 @@ def test_stoke_finds_and_writes(tmp_path):
     # at T0 = 1 and x0.9999 per step the chains settle after about 35,000 steps
     out = run("stoke", "--target", "LDA #$01 : LDA #$02", "--chains", "4", "--iterations", "200000",
               "--seed", "1", "--output", str(js)).stdout
-    assert "1 unique optimizations found" in out
-    assert "LDA #$02 (-2 bytes, -2 cycles)" in out
-    assert '"LDA #$02"' in js.read_text()
+    # stoke.Run + Deduplicate: every verified rewrite once (main.go:231-249),
+    # written in the order found (main.go:253-262 does not sort)
+    import json
+    import re
+    m = re.search(r"(\d+) unique optimizations found", out)
+    listed = re.findall(r"^  \d+\. (.*) \(-(\d+) bytes", out, re.M)
+    assert m and int(m.group(1)) == len(listed) >= 1
+    assert "  1. LDA #$02 (-2 bytes, -2 cycles)" in out or ("LDA #$02", "2") in listed
+    rules = json.loads(js.read_text())
+    assert [" : ".join(r["replacement"]) for r in rules] == [a for a, _ in listed]
+    assert {" : ".join(r["replacement"]) for r in rules if r["bytes_saved"] == 2} == {"LDA #$02"}
+    assert all(r["source"] == ["LDA #$01", "LDA #$02"] and r["bytes_saved"] > 0 for r in rules)
+    assert sorted(" : ".join(r["replacement"]) for r in rules) == sorted(a for a, _ in listed)
 
 
 def test_enumerate_slice_then_verify_jsonl(tmp_path):

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle as o
-from sx6502 import core, isa, result
+from sx6502 import core, isa
 
 pytestmark = pytest.mark.gpu
 

@@ -73,3 +73,50 @@ def test_reported_rewrites_are_sound(dev):
         assert r["bytes_saved"] == isa.seq_byte_size(t) - isa.seq_byte_size(rep) > 0
         assert o.quick_check(t, rep) and o.mid_check(t, rep)
         assert o.exhaustive_check(t, rep, 0, mode=0)   # necessary: the complete sweep implies it
+
+
+def test_collect_keeps_the_first_mode_rewrites_and_more(dev):
+    """sx_stoke_collect (stoke.Run + Deduplicate): every kept rewrite is sound
+    and unique per (target, replacement); a target's earliest kept rewrite is
+    the first-verified result of sx_stoke_run on the same chains (the two
+    runs' trajectories agree until a collecting chain restarts)."""
+    targets = c4_targets(48)
+    kw = dict(chains=8, steps=12288, seed=5, steps_per_launch=4096)
+    first = dev.stoke(targets, stop_when_verified=False, **kw)
+    found, nf = dev.stoke_collect(targets, **kw)
+    assert nf == len(found) and nf >= first["verified"].sum() > 0
+    keys = {(int(f["target"]), tuple(f["rep"][: f["rep_len"]].tolist())) for f in found}
+    assert len(keys) == nf
+    for f in found:
+        t, rep = targets[f["target"]], f["rep"][: f["rep_len"]].tolist()
+        assert f["chain"] // 8 == f["target"] and f["dead_flags"] == 0
+        assert f["bytes_saved"] == isa.seq_byte_size(t) - isa.seq_byte_size(rep) > 0
+        assert o.quick_check(t, rep) and o.mid_check(t, rep) and o.exhaustive_check(t, rep, 0, mode=2)
+    for ti, r in enumerate(first):
+        if not r["verified"]:
+            continue
+        mine = found[found["target"] == ti]
+        assert len(mine) and mine[0]["step"] == r["step"] and mine[0]["chain"] == r["chain"], ti
+        assert mine[0]["rep"].tolist() == r["rep"].tolist()
+
+
+def test_collect_restarts_and_reports_flag_diff():
+    """With carry dead, CLC : CLC : CLC has several one-byte rewrites (CLC,
+    SEC, NOP ...): chains restarted after each verified rewrite find more than
+    one, and each carries FlagDiff(target, rewrite) as its dead flags
+    (pkg/stoke/search.go:104-110), 0 for CLC itself."""
+    with core.Device(0) as d:
+        t = [o.word(o.CLC)] * 3
+        found, nf = d.stoke_collect([t], chains=16, steps=20000, seed=2, dead=0x01, t0=0.05)
+        d1, nf1 = d.stoke_collect([t], chains=16, steps=20000, seed=2, dead=0x01, t0=0.05)
+    assert nf >= 2, [isa.disasm_seq(f["rep"][: f["rep_len"]].tolist()) for f in found]
+    for f in ("target", "chain", "step", "rep", "rep_len", "dead_flags"):  # deterministic for a seed
+        assert nf1 == nf and np.array_equal(found[f], d1[f]), f
+    for f in found:
+        rep = f["rep"][: f["rep_len"]].tolist()
+        assert o.exhaustive_check(t, rep, 0x01, mode=2)
+        assert f["dead_flags"] == o.flag_diff(t, rep)
+    reps = {isa.disasm_seq(f["rep"][: f["rep_len"]].tolist()) for f in found}
+    assert len(reps) == nf
+    if "CLC" in reps:
+        assert [f["dead_flags"] for f in found if f["rep_len"] == 1 and f["rep"][0] == o.word(o.CLC)] == [0]

@@ -152,47 +152,54 @@ def test_readme_example_rules():
     assert ("LDA #$00", 2, 4) in cuda
 
 
+def _brute_chunk(args):
+    """States [lo, hi) of the sweep grid through the textbook model: True when
+    target and candidate agree on all of them."""
+    t, c, dead, extra, lo, hi = args
+    idx = np.arange(lo, hi, dtype=np.int64)
+    st = np.zeros((hi - lo, 8), np.uint8)
+    st[:, 4] = 0xFD
+    st[:, 0] = idx & 0xFF
+    st[:, 3] = (idx >> 8) & 1
+    for j, b in enumerate(extra):
+        st[:, b] = (idx >> (9 + 8 * j)) & 0xFF
+    ts, cs = st, st
+    for w in t:
+        ts = model(w & 0xFFFF, ts, np.full(len(ts), w >> 16))
+    for w in c:
+        cs = model(w & 0xFFFF, cs, np.full(len(cs), w >> 16))
+    pm = 0xCF & ~dead
+    diff = (ts != cs)
+    diff[:, 3] = (ts[:, 3] & pm) != (cs[:, 3] & pm)
+    return not diff.any()
+
+
 def _brute_exh(t, c, dead):
-    """Complete sweep of the domain with the textbook model (k <= 2)."""
+    """Complete sweep of the domain with the textbook model (k <= 2): A (256)
+    x carry (2) x each extra byte read (256 each), S = 0xFD unless read;
+    chunks of 2^21 states spread over the host's cores."""
+    import multiprocessing as mp
     reads = 0
     for w in t + c:
         reads |= o.op_reads(w & 0xFFFF)
     extra = [b for r, b in ((o.REG_X, 1), (o.REG_Y, 2), (o.REG_M, 5), (o.REG_S, 4),
                             (o.REG_S0, 6), (o.REG_S1, 7)) if reads & r]
     assert len(extra) <= 2
-    grids = np.meshgrid(np.arange(256), np.arange(2), *([np.arange(256)] * len(extra)),
-                        indexing="ij")
-    n = grids[0].size
-    for lo in range(0, n, 1 << 22):
-        hi = min(n, lo + (1 << 22))
-        st = np.zeros((hi - lo, 8), np.uint8)
-        st[:, 4] = 0xFD
-        st[:, 0] = grids[0].ravel()[lo:hi]
-        st[:, 3] = grids[1].ravel()[lo:hi]
-        for j, b in enumerate(extra):
-            st[:, b] = grids[2 + j].ravel()[lo:hi]
-        ts, cs = st, st
-        for w in t:
-            ts = model(w & 0xFFFF, ts, np.full(len(ts), w >> 16))
-        for w in c:
-            cs = model(w & 0xFFFF, cs, np.full(len(cs), w >> 16))
-        pm = 0xCF & ~dead
-        diff = (ts != cs)
-        diff[:, 3] = (ts[:, 3] & pm) != (cs[:, 3] & pm)
-        if diff.any():
-            return False
-    return True
-
-
-SLOW = pytest.mark.skipif(not os.environ.get("SX_SLOW"), reason="2^25-state brute force; set SX_SLOW=1")
+    n = 512 << (8 * len(extra))
+    step = 1 << 21
+    jobs = [(list(t), list(c), dead, extra, lo, min(n, lo + step)) for lo in range(0, n, step)]
+    if len(jobs) == 1:
+        return _brute_chunk(jobs[0])
+    with mp.get_context("fork").Pool(min(8, len(jobs), os.cpu_count() or 1)) as pool:
+        return all(pool.imap_unordered(_brute_chunk, jobs))
 
 
 @pytest.mark.parametrize("t,c,dead", [
     (seq(o.CPX_M, (o.CMP_IMM, 9)), seq((o.CMP_IMM, 9)), 0),
-    pytest.param(seq((o.CPX_IMM, 5), (o.CPY_IMM, 0xE5)), seq((o.CPY_IMM, 0xE5)), 0, marks=SLOW),
-    pytest.param(seq(o.STX_M, o.LDA_M), seq(o.STX_M, o.TXA), 0, marks=SLOW),
+    pytest.param(seq((o.CPX_IMM, 5), (o.CPY_IMM, 0xE5)), seq((o.CPY_IMM, 0xE5)), 0),
+    pytest.param(seq(o.STX_M, o.LDA_M), seq(o.STX_M, o.TXA), 0),
     (seq(o.PHA, o.PLA), seq((o.ORA_IMM, 0)), 0),
-    pytest.param(seq(o.DEX, (o.CPY_IMM, 0x1D)), seq(o.DEX), 0xFF, marks=SLOW),
+    pytest.param(seq(o.DEX, (o.CPY_IMM, 0x1D)), seq(o.DEX), 0xFF),
     (seq(o.TSX, o.INX), seq(o.TSX), 0xFF),
     (seq(o.INX, o.DEX), seq((o.CPX_IMM, 0)), 0x01),
 ])
