@@ -1479,6 +1479,298 @@ __global__ __launch_bounds__(256) void k_empty_emit(const Job *jobs, uint32_t nj
   rules[atomicAdd(nrules, 1u)] = r;
 }
 
+// ---- Candidates of length 3: a hash join over all 2,863^3 of them.
+//
+// A (target, pass) entry that no candidate of length 1 or 2 resolves goes on
+// to the length-3 candidates (worker.go:122-123 loops candidate lengths in
+// turn).  There are 23.5e9 of them, too many to list per target, so one sweep
+// enumerates them on the fly for all open targets together: a wave holds 8
+// prefixes (i0, i1) x the 8 QuickCheck vectors (lane = prefix * 8 + vector),
+// runs each prefix once, then walks the 2,863 last instructions i2 in Go
+// order with a wave-uniform opcode.  Each group of 8 lanes XORs its vectors'
+// output hashes into the candidate's fingerprint hash, which is looked up in
+// an LDS Bloom filter of the open targets' hashes and, when it passes, in
+// their table in global memory.  Hits (the candidate, the table slot) are
+// verified exactly against every target of the slot, filtered by bytes and
+// the pruner and swept exhaustively; the first hit of an entry is the passing
+// candidate with the smallest Go index (i0 * 2863 + i1) * 2863 + i2.
+constexpr uint32_t kL3BloomLog = 18;                          // 2^18 bits = 32 KiB of LDS
+constexpr uint32_t kL3BloomWords = (1u << kL3BloomLog) / 32;
+constexpr uint32_t kL3MaxTargets = 8192;                      // open targets per sweep
+constexpr uint32_t kL3Queue = 64;                             // LDS hit queue per wave
+constexpr uint64_t kL3None = ~0ull;
+constexpr uint64_t kL3Prefixes = (uint64_t)kNumInsts * kNumInsts;
+
+// One vector's share of the fingerprint hash (P under the compare mask).
+__device__ __forceinline__ uint32_t l3_mix(uint32_t lo, uint32_t hi, uint32_t v) {
+  uint32_t x = (lo * 0x9E3779B1u) ^ hi ^ (v * 0x7FEB352Du);
+  x ^= x >> 15;
+  x *= 0x846CA68Bu;
+  x ^= x >> 16;
+  return x;
+}
+__device__ __forceinline__ uint32_t l3_bit1(uint32_t hsh) { return hsh & ((1u << kL3BloomLog) - 1u); }
+__device__ __forceinline__ uint32_t l3_bit2(uint32_t hsh) { return (hsh * 0x5BD1E995u) >> (32 - kL3BloomLog); }
+
+// Open targets: entry 2t (exact pass) without a rule, or with dead flags
+// entry 2t + 1 without one, and more bytes than the shortest length-3
+// candidate (3).  list <- t, hk <- its fingerprint hash (never 0);
+// cnt[0] = how many, cnt[1] = their largest byte size.
+__global__ __launch_bounds__(256) void k_l3_open(const uint32_t *rule_flag, uint32_t nt, int dead_any,
+                                                 const uint8_t *tbytes, const uint2 *tfp, uint32_t mlo,
+                                                 uint32_t *list, uint32_t *hk, uint32_t *cnt) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  bool open = false;
+  if (t < nt) open = tbytes[t] > 3 && (!rule_flag[2 * t] || (dead_any && !rule_flag[2 * t + 1]));
+  const uint64_t m = __ballot(open);
+  if (m == 0) return;
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(&cnt[0], (uint32_t)__popcll(m));
+  base = __shfl(base, 0);
+  if (!open) return;
+  uint32_t hsh = 0;
+  for (uint32_t v = 0; v < kFpVecs; v++) {
+    const uint2 f = tfp[(uint64_t)t * kFpVecs + v];
+    hsh ^= l3_mix(f.x & mlo, f.y, v);
+  }
+  const uint32_t o = base + mbcnt(m);
+  list[o] = t;
+  hk[o] = hsh | 1u;
+  atomicMax(&cnt[1], (uint32_t)tbytes[t]);
+}
+
+// The table of n open targets' hashes: open addressing over mask + 1 >= 4n
+// slots (keys: 0 = free), each slot heading a list of the targets with that
+// hash (head -1 = none, next); and the Bloom filter's two bits per hash.
+__global__ __launch_bounds__(256) void k_l3_insert(const uint32_t *hk, uint32_t n, uint32_t mask, uint32_t *keys,
+                                                   int32_t *head, int32_t *next, uint32_t *bloom) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t hsh = hk[i];
+  uint32_t sl = hsh & mask;
+  for (;;) {  // more than n slots are free, so a free or matching slot is reached
+    const uint32_t old = atomicCAS(&keys[sl], 0u, hsh);
+    if (old == 0u || old == hsh) break;
+    sl = (sl + 1) & mask;
+  }
+  next[i] = atomicExch(&head[sl], (int32_t)i);
+  atomicOr(&bloom[l3_bit1(hsh) >> 5], 1u << (l3_bit1(hsh) & 31u));
+  atomicOr(&bloom[l3_bit2(hsh) >> 5], 1u << (l3_bit2(hsh) & 31u));
+}
+
+// The sweep over prefixes [p0, p0 + np): raw hits (prefix, i2 | slot << 12).
+__global__ __launch_bounds__(256) void k_l3_join(uint64_t p0, uint64_t np, const uint32_t *__restrict__ keys,
+                                                 uint32_t mask, const uint32_t *__restrict__ bloom, int max_tbytes,
+                                                 int model, uint32_t mlo, uint2 *__restrict__ raw, uint32_t cap,
+                                                 uint32_t *__restrict__ count) {
+  __shared__ uint32_t s_bloom[kL3BloomWords];
+  __shared__ uint2 queue[4][kL3Queue];
+  for (uint32_t i = threadIdx.x; i < kL3BloomWords; i += 256) s_bloom[i] = bloom[i];
+  __syncthreads();
+  const uint32_t lane = lane_id(), wv = uni(threadIdx.x >> 6);
+  const uint32_t g = lane >> 3, v = lane & 7;
+  const uint64_t p = p0 + ((uint64_t)blockIdx.x * 4 + wv) * 8 + g;
+  bool active = p < p0 + np;
+  const uint32_t pi = active ? (uint32_t)p : 0u;
+  uint32_t w[2] = {go_inst(pi / kNumInsts), go_inst(pi % kNumInsts)};
+  const int pbytes = byte_size(w[0] & 0xFFFFu, model) + byte_size(w[1] & 0xFFFFu, model);
+  // a prunable prefix makes every extension prunable (the redundancy inside
+  // it stays in the candidate)
+  if (should_prune(w, 2, c_reads, c_writes)) active = false;
+  if (pbytes + 1 >= max_tbytes) active = false;
+  State s;
+  s.a = c_vec[v][0]; s.x = c_vec[v][1]; s.y = c_vec[v][2]; s.p = c_vec[v][3];
+  s.s = c_vec[v][4]; s.m = c_vec[v][5]; s.s0 = c_vec[v][6]; s.s1 = c_vec[v][7];
+  step(s, w[0] & 0xFFFFu, w[0] >> 16);
+  step(s, w[1] & 0xFFFFu, w[1] >> 16);
+  uint32_t qn = 0;
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count, qn);
+    base = uni(base);
+    for (uint32_t i = lane; i < qn; i += kWave)
+      if (base + i < cap) raw[base + i] = queue[wv][i];
+    __builtin_amdgcn_wave_barrier();
+    qn = 0;
+  };
+  if (__ballot(active) != 0) {
+    for (uint32_t i2 = 0; i2 < (uint32_t)kNumInsts; i2++) {
+      const uint32_t wi = go_inst(i2), op = wi & 0xFFFFu;
+      // Go order: the one-byte instructions first, then the two-byte ones
+      if (i2 == (uint32_t)kNumNonImm && __ballot(active && pbytes + 2 < max_tbytes) == 0) break;
+      const bool ok_b = active && pbytes + byte_size(op, model) < max_tbytes;
+      State s2 = s;
+      step(s2, op, wi >> 16);
+      uint32_t hsh = l3_mix(pack_lo(s2) & mlo, pack_hi(s2), v);
+      hsh ^= __shfl_xor(hsh, 1);
+      hsh ^= __shfl_xor(hsh, 2);
+      hsh ^= __shfl_xor(hsh, 4);
+      hsh |= 1u;
+      bool cand = false;
+      if (ok_b && v == 0) {
+        const uint32_t b1 = l3_bit1(hsh), b2 = l3_bit2(hsh);
+        cand = ((s_bloom[b1 >> 5] >> (b1 & 31u)) & (s_bloom[b2 >> 5] >> (b2 & 31u)) & 1u) != 0;
+      }
+      if (__ballot(cand) == 0) continue;
+      uint32_t slot = 0;
+      bool hit = false;
+      if (cand) {
+        uint32_t sl = hsh & mask;
+        for (;;) {  // the table has free slots, so the probe ends
+          const uint32_t k = keys[sl];
+          if (k == hsh) {
+            hit = true;
+            slot = sl;
+            break;
+          }
+          if (k == 0u) break;
+          sl = (sl + 1) & mask;
+        }
+      }
+      const uint64_t hm = __ballot(hit);
+      if (hm == 0) continue;
+      if (hit) queue[wv][qn + (uint32_t)__popcll(hm & ((1ull << lane) - 1))] = make_uint2(pi, i2 | (slot << 12));
+      qn += (uint32_t)__popcll(hm);
+      if (qn > kL3Queue - 8) flush();
+    }
+  }
+  if (qn) flush();
+}
+
+// Outputs of a length-3 candidate on the 8 QuickCheck vectors.
+__device__ __forceinline__ void l3_outputs(const uint32_t w[3], uint32_t lo[8], uint32_t hi[8]) {
+  for (uint32_t v = 0; v < kFpVecs; v++) {
+    State s;
+    s.a = c_vec[v][0]; s.x = c_vec[v][1]; s.y = c_vec[v][2]; s.p = c_vec[v][3];
+    s.s = c_vec[v][4]; s.m = c_vec[v][5]; s.s0 = c_vec[v][6]; s.s1 = c_vec[v][7];
+    for (int k = 0; k < 3; k++) step(s, w[k] & 0xFFFFu, w[k] >> 16);
+    lo[v] = pack_lo(s);
+    hi[v] = pack_hi(s);
+  }
+}
+__device__ __forceinline__ void l3_words(uint64_t c3, uint32_t w[3]) {
+  const uint32_t p = (uint32_t)(c3 / kNumInsts), i2 = (uint32_t)(c3 % kNumInsts);
+  w[0] = go_inst(p / kNumInsts);
+  w[1] = go_inst(p % kNumInsts);
+  w[2] = go_inst(i2);
+}
+
+// Raw hits -> exhaustive jobs: every target of the hit's slot (list[] maps
+// the sweep's entries to segment positions) whose QuickCheck the candidate
+// passes exactly (pass 1, entry open) or only under the mask (pass 2, dead
+// flags, entry open), with fewer bytes, the candidate not pruned.
+__global__ __launch_bounds__(256) void k_l3_expand(const uint2 *raw, uint32_t nraw, const int32_t *head,
+                                                   const int32_t *next, const uint32_t *list,
+                                                   const uint32_t *targets, uint32_t tl, const uint2 *tfp,
+                                                   const uint8_t *tbytes, const uint32_t *rule_flag, int dead_any,
+                                                   uint32_t pm_masked, int model, Job *jobs, uint64_t *c3out,
+                                                   uint32_t cap, uint32_t *njobs) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nraw) return;
+  const uint2 e = raw[i];
+  const uint64_t c3 = (uint64_t)e.x * kNumInsts + (e.y & 0xFFFu);
+  uint32_t w[3];
+  l3_words(c3, w);
+  if (should_prune(w, 3, c_reads, c_writes)) return;
+  int cb = 0;
+  for (int k = 0; k < 3; k++) cb += byte_size(w[k] & 0xFFFFu, model);
+  uint32_t lo[8], hi[8];
+  l3_outputs(w, lo, hi);
+  const uint32_t mlo = 0x00FFFFFFu | (pm_masked << 24), flo = 0x00FFFFFFu | (kPMask << 24);
+  for (int32_t q = head[e.y >> 12]; q >= 0; q = next[q]) {
+    const uint32_t t = list[q];
+    if (cb >= (int)tbytes[t]) continue;
+    uint32_t dl = 0, dh = 0;
+    for (uint32_t v = 0; v < kFpVecs; v++) {
+      const uint2 f = tfp[(uint64_t)t * kFpVecs + v];
+      dl |= f.x ^ lo[v];
+      dh |= f.y ^ hi[v];
+    }
+    const bool okf = ((dl & flo) | dh) == 0, okm = ((dl & mlo) | dh) == 0;
+    uint32_t pass = 0;
+    if (okf && !rule_flag[2 * t]) pass = 1;
+    else if (!okf && okm && dead_any && !rule_flag[2 * t + 1]) pass = 2;
+    if (!pass) continue;
+    const uint32_t o = atomicAdd(njobs, 1u);
+    if (o >= cap) continue;
+    Job j;
+    for (int k = 0; k < kJobWords; k++) {
+      j.tw[k] = k < (int)tl ? targets[(uint64_t)t * tl + k] : 0u;
+      j.cw[k] = k < 3 ? w[k] : 0u;
+    }
+    j.tl = (uint8_t)tl;
+    j.cl = 3;
+    j.pass = (uint8_t)pass;
+    j.pmask = (uint8_t)(pass == 2 ? pm_masked : kPMask);
+    j.target = t;
+    j.slot = 0;
+    jobs[o] = j;
+    c3out[o] = c3;
+  }
+}
+
+// FlagDiff (verifier.go:591-604) of a target and a length-3 candidate from
+// the target's fingerprint; 0 when a non-P byte differs.
+__device__ __forceinline__ uint32_t l3_flag_diff(const uint2 *tfp, uint32_t t, const uint32_t w[3]) {
+  uint32_t lo[8], hi[8];
+  l3_outputs(w, lo, hi);
+  uint32_t fd = 0;
+  for (uint32_t v = 0; v < kFpVecs; v++) {
+    const uint2 f = tfp[(uint64_t)t * kFpVecs + v];
+    if (((f.x ^ lo[v]) & 0x00FFFFFFu) || (f.y ^ hi[v])) return 0;
+    fd |= ((f.x ^ lo[v]) >> 24) & kPMask;
+  }
+  return fd;
+}
+
+// The passing jobs' smallest Go index per (target, pass); a flag-relaxed
+// rule needs FlagDiff != 0 (worker.go:211-213).
+__global__ __launch_bounds__(256) void k_l3_best(const Job *jobs, uint32_t nj, const uint32_t *fail,
+                                                 const uint64_t *c3, const uint2 *tfp, unsigned long long *best) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nj || fail[i]) return;
+  const Job &j = jobs[i];
+  if (j.pass == 2) {
+    uint32_t w[3] = {j.cw[0], j.cw[1], j.cw[2]};
+    if (l3_flag_diff(tfp, j.target, w) == 0) return;
+  }
+  atomicMin(&best[(uint64_t)j.target * 2 + (j.pass - 1)], (unsigned long long)c3[i]);
+}
+
+// Rules of the entries the length-3 candidates resolved, in their slots.
+__global__ __launch_bounds__(256) void k_l3_emit(const unsigned long long *best, uint32_t nt, const uint32_t *targets,
+                                                 uint32_t tl, const uint2 *tfp, const uint8_t *tbytes,
+                                                 const uint8_t *tcyc, int model, uint64_t tbase, sx_rule *rules,
+                                                 uint32_t *rule_flag) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= 2 * nt || best[k] == kL3None) return;
+  const uint32_t t = k >> 1, pass = (k & 1) + 1;
+  uint32_t w[3];
+  l3_words(best[k], w);
+  sx_rule r;
+  memset(&r, 0, sizeof(r));
+  int cb = 0, cc = 0;
+  for (int q = 0; q < 4; q++) {
+    r.src[q] = q < (int)tl ? targets[(uint64_t)t * tl + q] : 0u;
+    r.rep[q] = q < 3 ? w[q] : 0u;
+  }
+  for (int q = 0; q < 3; q++) {
+    cb += byte_size(w[q] & 0xFFFFu, model);
+    cc += cycles(w[q] & 0xFFFFu);
+  }
+  r.target_index = tbase + t;
+  r.src_len = (uint8_t)tl;
+  r.rep_len = 3;
+  r.pass = (uint8_t)pass;
+  r.dead_flags = (uint8_t)(pass == 2 ? l3_flag_diff(tfp, t, w) : 0u);
+  r.bytes_saved = (int16_t)((int)tbytes[t] - cb);
+  r.cycles_saved = (int16_t)((int)tcyc[t] - cc);
+  rules[k] = r;
+  rule_flag[k] = 1;
+}
+
 // Length-3 ranges: a rule's target_index (base + position in the segment)
 // becomes base + the target's offset in the raw triple index range.
 __global__ void k_fix_raw_index(sx_rule *rules, uint32_t n, const uint32_t *raw_off, uint64_t base) {
@@ -1712,7 +2004,8 @@ enum BufId {
   B_RANK, B_SURV, B_MID, B_COUNTERS, B_CUR, B_REJ, B_DONE, B_JOBS, B_CHUNKS, B_OFFS, B_SUMS,
   B_FAIL, B_RULES, B_SCRATCH, B_GEN_COUNTS, B_GEN_OFFS, B_BADOP, B_P_TW, B_P_TL, B_P_CW, B_P_CL,
   B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR, B_HEADS, B_RULEFLAG, B_RULEOFFS, B_RULES2,
-  B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_PLANTAB, B_EFP, B_ERULES, B_COUNT
+  B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_PLANTAB, B_EFP, B_ERULES, B_L3_LIST, B_L3_HK, B_L3_CNT,
+  B_L3_KEYS, B_L3_HEAD, B_L3_NEXT, B_L3_BLOOM, B_L3_RAW, B_L3_C3, B_L3_BEST, B_COUNT
 };
 static_assert(B_COUNT <= 64, "sx_handle::bufs holds every buffer id");
 
@@ -2109,6 +2402,112 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, bo
   return SX_OK;
 }
 
+// The length-3 stage of one segment (first hit): the entries still open after
+// the candidates of length 1 and 2 against all length-3 candidates, in
+// sweeps of up to kL3MaxTargets open targets; resolved entries get their rule
+// and flag in the segment's slots.
+int run_l3(sx_handle *h, uint32_t nt, const uint32_t *tw, uint32_t tl, const uint2 *tfp, const uint8_t *tbytes,
+           const uint8_t *tcyc, uint32_t *rule_flag, sx_rule *rules, uint32_t dead, int model, int exh_mode,
+           uint64_t tbase, sx_stats *st) {
+  const uint32_t pm_masked = kPMask & ~dead, mlo = 0x00FFFFFFu | (pm_masked << 24);
+  uint32_t *list, *hk, *cnt;
+  unsigned long long *best;
+  ENSURE(B_L3_LIST, uint32_t, nt, list);
+  ENSURE(B_L3_HK, uint32_t, nt, hk);
+  ENSURE(B_L3_CNT, uint32_t, 4, cnt);
+  ENSURE(B_L3_BEST, unsigned long long, 2 * (uint64_t)nt, best);
+  HIPCHK(h, hipMemsetAsync(cnt, 0, 16, h->stream));
+  HIPCHK(h, hipMemsetAsync(best, 0xFF, 2 * (uint64_t)nt * 8, h->stream));
+  hipLaunchKernelGGL(k_l3_open, dim3(blocks_for(nt, 256)), dim3(256), 0, h->stream, rule_flag, nt, dead ? 1 : 0,
+                     tbytes, tfp, mlo, list, hk, cnt);
+  HIPCHK(h, hipGetLastError());
+  uint32_t c[2] = {0, 0};
+  HIPCHK(h, hipMemcpyAsync(c, cnt, 8, hipMemcpyDeviceToHost, h->stream));
+  SYNC_WD("device sync");
+  const uint32_t nopen = c[0];
+  const int max_tbytes = (int)c[1];
+  trace(h, "  length-3 candidates: %u open targets (at most %d bytes)", nopen, max_tbytes);
+  uint32_t raw_cap = 1u << 20;
+  for (uint32_t b0 = 0; b0 < nopen; b0 += kL3MaxTargets) {
+    const uint32_t n = std::min(kL3MaxTargets, nopen - b0);
+    uint32_t lg = 6;
+    while ((1u << lg) < 4 * n) lg++;
+    const uint32_t mask = (1u << lg) - 1u;
+    uint32_t *keys, *bloom;
+    int32_t *head, *next;
+    ENSURE(B_L3_KEYS, uint32_t, mask + 1, keys);
+    ENSURE(B_L3_HEAD, int32_t, mask + 1, head);
+    ENSURE(B_L3_NEXT, int32_t, n, next);
+    ENSURE(B_L3_BLOOM, uint32_t, kL3BloomWords, bloom);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(mask + 1, 256)), dim3(256), 0, h->stream, keys, mask + 1, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(mask + 1, 256)), dim3(256), 0, h->stream, (uint32_t *)head,
+                       mask + 1, 0xFFFFFFFFu);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(kL3BloomWords, 256)), dim3(256), 0, h->stream, bloom,
+                       kL3BloomWords, 0u);
+    hipLaunchKernelGGL(k_l3_insert, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, hk + b0, n, mask, keys, head,
+                       next, bloom);
+    HIPCHK(h, hipGetLastError());
+    // the sweep, in launches of 2^20 prefixes (2^17 waves); repeated with a
+    // larger hit buffer if it overflowed
+    uint2 *raw;
+    uint32_t nraw = 0;
+    for (;;) {
+      ENSURE(B_L3_RAW, uint2, raw_cap, raw);
+      HIPCHK(h, hipMemsetAsync(cnt + 2, 0, 4, h->stream));
+      const uint64_t per = 1u << 20;
+      for (uint64_t p0 = 0; p0 < kL3Prefixes; p0 += per) {
+        const uint64_t np = std::min<uint64_t>(per, kL3Prefixes - p0);
+        hipLaunchKernelGGL(k_l3_join, dim3((unsigned)((np + 31) / 32)), dim3(256), 0, h->stream, p0, np, keys, mask,
+                           bloom, max_tbytes, model, mlo, raw, raw_cap, cnt + 2);
+        HIPCHK(h, hipGetLastError());
+      }
+      HIPCHK(h, hipMemcpyAsync(&nraw, cnt + 2, 4, hipMemcpyDeviceToHost, h->stream));
+      SYNC_WD("length-3 sweep");
+      if (nraw <= raw_cap) break;
+      if (nraw > (1u << 30)) return set_err(h, SX_ENOMEM, "length-3 hits (%u)", nraw);
+      raw_cap = nraw + nraw / 4;
+    }
+    st->qc_pairs += (uint64_t)n * kL3Prefixes * kNumInsts;
+    st->qc_raw_hits += nraw;
+    trace(h, "  length-3 sweep of %u targets: %u hits", n, nraw);
+    // hits -> jobs in chunks, the sweep, the smallest passing index per entry
+    const uint32_t chunk = 1u << 18;
+    for (uint32_t r0 = 0; r0 < nraw; r0 += chunk) {
+      const uint32_t nr = std::min(chunk, nraw - r0);
+      uint32_t job_cap = 4 * nr, nj = 0;
+      for (;;) {
+        Job *jobs;
+        uint64_t *c3;
+        ENSURE(B_JOBS, Job, job_cap, jobs);
+        ENSURE(B_L3_C3, uint64_t, job_cap, c3);
+        HIPCHK(h, hipMemsetAsync(cnt + 3, 0, 4, h->stream));
+        hipLaunchKernelGGL(k_l3_expand, dim3(blocks_for(nr, 256)), dim3(256), 0, h->stream, raw + r0, nr, head, next,
+                           list + b0, tw, tl, tfp, tbytes, rule_flag, dead ? 1 : 0, pm_masked, model, jobs, c3,
+                           job_cap, cnt + 3);
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipMemcpyAsync(&nj, cnt + 3, 4, hipMemcpyDeviceToHost, h->stream));
+        SYNC_WD("device sync");
+        if (nj <= job_cap) break;
+        job_cap = nj;
+      }
+      if (nj == 0) continue;
+      st->qc_hits += nj;
+      st->exh_pairs += nj;
+      st->exh_rounds++;
+      uint64_t chunks = 0;
+      int rc = run_exhaustive(h, nj, exh_mode, &chunks, false);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_l3_best, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, (const Job *)h->bufs[B_JOBS].p,
+                         nj, (const uint32_t *)h->bufs[B_FAIL].p, (const uint64_t *)h->bufs[B_L3_C3].p, tfp, best);
+      HIPCHK(h, hipGetLastError());
+    }
+  }
+  hipLaunchKernelGGL(k_l3_emit, dim3(blocks_for(2 * (uint64_t)nt, 256)), dim3(256), 0, h->stream, best, nt, tw, tl,
+                     tfp, tbytes, tcyc, model, tbase, rules, rule_flag);
+  HIPCHK(h, hipGetLastError());
+  return SX_OK;
+}
+
 // The evaluated-state counter (sharded): cleared, and summed on the host.
 int clear_evaluated(sx_handle *h) {
   unsigned long long *evald;
@@ -2335,11 +2734,15 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
   memset(&cfg_local, 0, sizeof(cfg_local));
   memcpy(&cfg_local, cfg_in, cfg_in->struct_size);
   const sx_enum_cfg *cfg = &cfg_local;
-  const int tl = cfg->target_len, ml = cfg->max_cand_len;
   // search.go:26-50: --max-target L runs every target length 2..L with
-  // candidates of length 1..L-1, so (2, 1), (2, 2) and (3, 2) (and (3, 1))
-  if (tl < 2 || tl > 3 || ml < 1 || ml > 2)
-    return set_err(h, SX_EINVAL, "target_len %d / max_cand_len %d unsupported", tl, ml);
+  // candidates of length 1..L-1.  Candidates of length 1 and 2 are slots of
+  // the pool (ml); length 3 (ml_req 3) is a stage of its own after them.
+  // Length-4 targets come as a list.
+  const int tl = cfg->target_len, ml_req = cfg->max_cand_len, ml = std::min(ml_req, 2);
+  if (tl < 2 || tl > 4 || ml_req < 1 || ml_req > 3)
+    return set_err(h, SX_EINVAL, "target_len %d / max_cand_len %d unsupported", tl, ml_req);
+  if (tl == 4 && !cfg->targets) return set_err(h, SX_EINVAL, "length-4 targets must be given as a list");
+  if (ml_req == 3 && cfg->hits) return set_err(h, SX_EINVAL, "length-3 candidates need first-hit mode");
   if (cfg->bytes_model < 0 || cfg->bytes_model > 1 || cfg->hits < 0 || cfg->hits > 1 ||
       cfg->exh_mode < 0 || cfg->exh_mode > 1 || cfg->dead_flags > 0xFF)
     return set_err(h, SX_EINVAL, "bad mode");
@@ -2713,6 +3116,11 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
                          rules, rule_cap, nrules_d, rule_flag);
       HIPCHK(h, hipGetLastError());
       if (cfg->hits) break;
+    }
+    if (ml_req == 3) {
+      int rc = run_l3(h, (uint32_t)nt, tw, (uint32_t)tl, tfp, tbytes, tcyc, rule_flag, rules, cfg->dead_flags,
+                      cfg->bytes_model, cfg->exh_mode, tbase + s0, &st);
+      if (rc) return rc;
     }
     uint32_t nr = 0;
     const sx_rule *rules_out = rules;

@@ -934,7 +934,7 @@ int64_t sxo_enumerate_pruned(int len, uint32_t *out, int64_t cap) {
 /* -------------------------------------------------------- Search driver -- */
 
 typedef struct cand_info {
-  uint32_t w[2];
+  uint32_t w[3];
   int len;
   int bytes;
   int cycles;
@@ -945,8 +945,11 @@ typedef struct worker_ctx {
   const sxo_search_cfg *cfg;
   const uint32_t *targets;
   int64_t ntargets;
-  const cand_info *cands[3]; /* by length */
+  const cand_info *cands[3]; /* by length (1, 2; length 3 is enumerated on the fly) */
   int64_t ncands[3];
+  uint32_t tab3[2863];       /* length 3: the instructions in Go order and their sizes */
+  int bytes3[2863];
+  int n3;
   int64_t next;              /* shared cursor, atomic */
   pthread_mutex_t mu;
   sxo_rule *rules;
@@ -989,6 +992,94 @@ static void make_rule(sxo_rule *r, const uint32_t *t, int tn, const cand_info *c
   r->bytes_saved = (int16_t)(tbytes - ci->bytes);
   r->cycles_saved = (int16_t)(tcycles - ci->cycles);
   r->target_index = (uint64_t)ti;
+}
+
+/* Length-3 candidates of one pass, in Go order (EnumerateSequences(3),
+ * enumerator.go: first instruction outermost), generated on the fly (there
+ * are 2,863^3): the first candidate that passes the byte filter, the pruner,
+ * the pass's QuickCheck, [MidCheck], the ExhaustiveCheck and, in the
+ * flag-relaxed pass, FlagDiff != 0 (worker.go:122-170, 180-240).  pass 1:
+ * the exact QuickCheck; pass 2: fails the exact QuickCheck and passes the
+ * masked one.  The 8 vectors' states after the first two instructions are
+ * computed once per prefix, and the QuickCheck looks at vector 0 first; the
+ * filters are conjunctive, so their order does not change the first hit.
+ * checked counts every candidate enumerated before the loop stops, as
+ * worker.go:126 does.  Returns 1 with the candidate in *out when found. */
+static int scan_len3(const worker_ctx *w, const uint32_t *t, int tn, int tbytes, const uint8_t tout[8][8],
+                     int pass, uint8_t dead, cand_info *out, sxo_stats *st) {
+  const sxo_search_cfg *cfg = w->cfg;
+  const uint32_t *tab = w->tab3;
+  const int *bm = w->bytes3;
+  const int n = w->n3;
+  const uint8_t d_qc = pass == 1 ? 0 : dead;
+  /* the byte filter in bulk: suf[i] = the fewest bytes of instructions i..n-1
+   * (Go order puts the one-byte instructions first), so once
+   * pb + suf[i2] >= tbytes no later last instruction fits either; skipped
+   * candidates still count as checked */
+  int suf[2864];
+  suf[n] = 1 << 20;
+  for (int i = n - 1; i >= 0; i--) suf[i] = bm[i] < suf[i + 1] ? bm[i] : suf[i + 1];
+  const uint64_t nn = (uint64_t)n * (uint64_t)n;
+  for (int i0 = 0; i0 < n; i0++) {
+    if (bm[i0] + suf[0] + suf[0] >= tbytes) {
+      st->checked += nn;
+      continue;
+    }
+    for (int i1 = 0; i1 < n; i1++) {
+      const int pb = bm[i0] + bm[i1];
+      uint8_t ps[8][8];
+      int have = 0;
+      if (pb + suf[0] >= tbytes) {
+        st->checked += (uint64_t)n;
+        continue;
+      }
+      for (int i2 = 0; i2 < n; i2++) {
+        if (pb + suf[i2] >= tbytes) {
+          st->checked += (uint64_t)(n - i2);
+          break;
+        }
+        st->checked++;
+        if (pb + bm[i2] >= tbytes) continue;
+        if (!have) {
+          const uint32_t pre[2] = {tab[i0], tab[i1]};
+          for (int v = 0; v < 8; v++) {
+            memcpy(ps[v], k_qc[v], 8);
+            sxo_exec_seq(ps[v], pre, 2);
+          }
+          have = 1;
+        }
+        uint8_t s0[8];
+        memcpy(s0, ps[0], 8);
+        exec_one(s0, SXO_OP(tab[i2]), SXO_IMM(tab[i2]));
+        if (!sxo_states_equal_masked(tout[0], s0, d_qc)) continue;
+        int ok = 1, exact = sxo_states_equal_masked(tout[0], s0, 0);
+        for (int v = 1; v < 8 && ok; v++) {
+          uint8_t sv[8];
+          memcpy(sv, ps[v], 8);
+          exec_one(sv, SXO_OP(tab[i2]), SXO_IMM(tab[i2]));
+          ok = sxo_states_equal_masked(tout[v], sv, d_qc);
+          exact = exact && sxo_states_equal_masked(tout[v], sv, 0);
+        }
+        if (!ok || (pass == 2 && exact)) continue;
+        const uint32_t c[3] = {tab[i0], tab[i1], tab[i2]};
+        if (sxo_should_prune(c, 3)) continue;
+        st->qc_hits++;
+        if (cfg->use_mid && !sxo_mid_check(t, tn, c, 3, d_qc)) continue;
+        st->exh_calls++;
+        if (!sxo_exhaustive_check(t, tn, c, 3, d_qc, cfg->exh_mode)) continue;
+        if (pass == 2 && sxo_flag_diff(t, tn, c, 3) == 0) continue;
+        memset(out, 0, sizeof(*out));
+        out->w[0] = c[0];
+        out->w[1] = c[1];
+        out->w[2] = c[2];
+        out->len = 3;
+        out->bytes = pb + bm[i2];
+        for (int k = 0; k < 3; k++) out->cycles += sxo_cycles(SXO_OP(c[k]));
+        return 1;
+      }
+    }
+  }
+  return 0;
 }
 
 static void process_target(worker_ctx *w, int64_t ti, sxo_stats *st) {
@@ -1053,6 +1144,15 @@ static void process_target(worker_ctx *w, int64_t ti, sxo_stats *st) {
   /* worker.go:119-178: first hit per target, candidate lengths ascending. */
   for (int len = 1; len <= cfg->max_cand_len; len++) {
     int found = 0;
+    if (len == 3) {
+      cand_info c3;
+      if (scan_len3(w, t, tn, tbytes, tout, 1, 0, &c3, st)) {
+        st->found++;
+        make_rule(&r, t, tn, &c3, ti, tbytes, tcycles, 0, 1);
+        push_rule(w, &r);
+      }
+      break;
+    }
     for (int64_t j = 0; j < w->ncands[len]; j++) {
       const cand_info *ci = &w->cands[len][j];
       st->checked++;
@@ -1076,6 +1176,15 @@ static void process_target(worker_ctx *w, int64_t ti, sxo_stats *st) {
    * exact QuickCheck. */
   for (int len = 1; len <= cfg->max_cand_len; len++) {
     int found = 0;
+    if (len == 3) {
+      cand_info c3;
+      if (scan_len3(w, t, tn, tbytes, tout, 2, cfg->dead, &c3, st)) {
+        st->found++;
+        make_rule(&r, t, tn, &c3, ti, tbytes, tcycles, sxo_flag_diff(t, tn, c3.w, 3), 2);
+        push_rule(w, &r);
+      }
+      break;
+    }
     for (int64_t j = 0; j < w->ncands[len]; j++) {
       const cand_info *ci = &w->cands[len][j];
       st->checked++;
@@ -1103,7 +1212,8 @@ static void *worker_main(void *arg) {
   worker_ctx *w = (worker_ctx *)arg;
   sxo_stats st;
   memset(&st, 0, sizeof(st));
-  const int64_t chunk = 64;
+  /* one target at a time when a target can take minutes (length-3 candidates) */
+  const int64_t chunk = w->cfg->max_cand_len == 3 ? 1 : 64;
   for (;;) {
     int64_t b = __atomic_fetch_add(&w->next, chunk, __ATOMIC_RELAXED);
     if (b >= w->ntargets) break;
@@ -1163,7 +1273,7 @@ static cand_info *build_cands(int len, int bytes_model, int64_t *count) {
 int sxo_search(const sxo_search_cfg *cfg, const uint32_t *targets, int64_t ntargets,
                sxo_rule *out, int64_t cap, int64_t *nout, sxo_stats *stats) {
   if (!cfg || cfg->target_len < 1 || cfg->target_len > 4 || cfg->max_cand_len < 1 ||
-      cfg->max_cand_len > 2)
+      cfg->max_cand_len > 3 || (cfg->max_cand_len == 3 && cfg->hits))
     return -1;
   worker_ctx w;
   memset(&w, 0, sizeof(w));
@@ -1171,14 +1281,18 @@ int sxo_search(const sxo_search_cfg *cfg, const uint32_t *targets, int64_t ntarg
   w.targets = targets;
   w.ntargets = ntargets;
   pthread_mutex_init(&w.mu, NULL);
-  for (int len = 1; len <= cfg->max_cand_len; len++)
+  for (int len = 1; len <= cfg->max_cand_len && len <= 2; len++)
     w.cands[len] = build_cands(len, cfg->bytes_model, &w.ncands[len]);
+  if (cfg->max_cand_len == 3) {
+    w.n3 = sxo_instruction_table(w.tab3, 0);
+    for (int i = 0; i < w.n3; i++) w.bytes3[i] = sxo_byte_size(SXO_OP(w.tab3[i]), cfg->bytes_model);
+  }
   int nt = cfg->nthreads > 0 ? cfg->nthreads : 1;
   pthread_t *th = (pthread_t *)calloc((size_t)nt, sizeof(pthread_t));
   for (int i = 0; i < nt; i++) pthread_create(&th[i], NULL, worker_main, &w);
   for (int i = 0; i < nt; i++) pthread_join(th[i], NULL);
   free(th);
-  for (int len = 1; len <= cfg->max_cand_len; len++) free((void *)w.cands[len]);
+  for (int len = 1; len <= cfg->max_cand_len && len <= 2; len++) free((void *)w.cands[len]);
   qsort(w.rules, (size_t)w.nrules, sizeof(sxo_rule), rule_cmp);
   if (nout) *nout = w.nrules;
   if (out) {

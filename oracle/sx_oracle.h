@@ -123,7 +123,7 @@ int64_t sxo_enumerate_pruned(int len, uint32_t *out, int64_t cap);
  *       1 = all hits, single masked pass, QC -> Mid -> Exh (the CUDA tool). */
 typedef struct sxo_search_cfg {
   int target_len;
-  int max_cand_len;   /* 1 or 2 */
+  int max_cand_len;   /* 1, 2 or 3 (3: first hit only; length 3 is enumerated on the fly) */
   int bytes_model;    /* 0 Go, 1 CUDA */
   int hits;           /* 0 first, 1 all */
   int exh_mode;       /* as sxo_exhaustive_check: 0 Go reduced, 1 complete (brute force), 2 complete (projection) */

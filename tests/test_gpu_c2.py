@@ -110,3 +110,21 @@ def test_full_run(dev, strata, l2, dead, n_rules):
             assert not (exact & core.STAGE_QC).any()
             fd = dev.flag_diff([t[:2].tolist() for t in ts[0]], [c[:1].tolist() for c in cs[0]])
             assert np.array_equal(fd, pick["dead_flags"][m])
+
+
+def test_disjoint_ranges_concatenate_to_the_whole_run(dev):
+    """Sharding by target range (first_target / end_target, one range per
+    process): three ranges with boundaries inside segments, concatenated,
+    are the whole run's rules, record for record, at 0xFF."""
+    print("c2 ranges, dead 0xFF", flush=True)
+    whole, st = dev.enumerate(None, dead=0xFF)
+    cuts = [0, 1_000_003, 4_200_000, N_L2]
+    parts = []
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        r, s = dev.enumerate(None, dead=0xFF, first=a, end=b)
+        assert s["targets"] == b - a
+        ti = r["target_index"].astype(np.int64)
+        assert ((ti >= a) & (ti < b)).all()
+        parts.append(r)
+    joined = np.concatenate(parts)
+    assert joined.tobytes() == whole.tobytes()
