@@ -684,35 +684,31 @@ __global__ __launch_bounds__(kBlk) void k_midcheck(const uint2 *__restrict__ sur
                                                    const uint32_t *__restrict__ rank, uint32_t *cur0) {
   __shared__ uint32_t s_hits[kBlkWaves], s_pass[kBlkWaves];
   __shared__ unsigned long long s_base;
+  // Straight-line on purpose: the lane masks are computed with selects, not
+  // with branches.  With uniform regions left unstructured (the build's
+  // -structurizecfg-skip-uniform-regions), a flag set inside a divergent
+  // branch and merged across the uniform use_mid branch came out as all
+  // lanes of the outer region, which counted pruned candidates as hits.
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
-  bool pass_ok = false;
-  uint32_t t = 0, slot = 0, pass = 0;
-  bool qc_hit = false;
-  if (i < ns) {
-    // list entry i -> its region and place there
-    uint32_t base = 0, r = 0;
+  const bool in = i < ns;
+  // list entry i -> its region and place there (lanes past the list read entry 0)
+  const uint32_t ii = in ? i : 0u;
+  uint32_t base = 0, r = 0;
 #pragma unroll
-    for (uint32_t k = 1; k < kQcRegions; k++)
-      if (i >= R.pre[k]) {
-        base = R.pre[k];
-        r = k;
-      }
-    const uint2 e = surv[(uint64_t)r * R.cap + (i - base)];
-    t = e.x;
-    slot = e.y & 0x3FFFFFFFu;
-    const bool full = (e.y >> 31) != 0;
-    if (cbytes[slot] < tbytes[t] && !cprune[slot]) {
-      const bool exact = !hits_all && full;  // pass 1 compares P under 0xCF
-      pass = (hits_all || full) ? 1 : 2;
-      qc_hit = true;
-      bool ok = true;
-      if (use_mid) {
-        const uint2 a = tmh[t], b = cmh[slot];
-        ok = exact ? a.x == b.x : a.y == b.y;
-      }
-      pass_ok = ok;
-    }
+  for (uint32_t k = 1; k < kQcRegions; k++) {
+    const bool after = ii >= R.pre[k];
+    base = after ? R.pre[k] : base;
+    r = after ? k : r;
   }
+  const uint2 e = surv[(uint64_t)r * R.cap + (ii - base)];
+  const uint32_t t = e.x, slot = e.y & 0x3FFFFFFFu;
+  const bool full = (e.y >> 31) != 0;
+  const bool qc_hit = in & (cbytes[slot] < tbytes[t]) & (cprune[slot] == 0);
+  const bool exact = !hits_all && full;  // pass 1 compares P under 0xCF
+  const uint32_t pass = (hits_all || full) ? 1u : 2u;
+  const uint2 a = tmh[t], b = cmh[slot];
+  const bool mid_ok = !use_mid || (exact ? a.x == b.x : a.y == b.y);
+  const bool pass_ok = qc_hit & mid_ok;
   const uint64_t hq = __ballot(qc_hit), m = __ballot(pass_ok);
   uint32_t th = 0, tp = 0;
   block_prefix((uint32_t)__popcll(hq), s_hits, &th);

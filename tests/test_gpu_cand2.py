@@ -35,3 +35,21 @@ def test_c1_targets_with_two_instruction_candidates(dev, dead):
     assert len(got) == len(gold) and (gold["rep_len"] == 2).any()
     for f in FIELDS:
         assert np.array_equal(got[f], gold[f]), f
+
+
+def test_all_hits_with_two_instruction_candidates_counts(dev):
+    """The CUDA tool's semantics (every hit, M ops 2 bytes, one masked pass)
+    with candidates of length 1 and 2 on 512 of the sampled targets: the
+    QuickCheck hits after the byte filter and the pruner, the MidCheck
+    passes and the rule set equal the oracle's all-hits search.  The hit
+    counter also stays the same from run to run (it once counted pruned
+    candidates, depending on which survivors shared a wave)."""
+    import oracle as o
+    targets = np.load(os.path.join(GOLD, "c1_targets.npy"))[:512]
+    want, ost = o.search(targets, max_cand_len=2, hits=1, bytes_model=1, exh_mode=o.EXH_COMPLETE, nthreads=8)
+    runs = [dev.enumerate(targets, target_len=2, max_cand_len=2, hits=core.HITS_ALL, bytes_model=1)
+            for _ in range(2)]
+    for got, st in runs:
+        assert st["qc_hits"] == ost.qc_hits and st["mid_pass"] == ost.mid_pass
+        key = lambda r: (int(r["target_index"]), tuple(r["rep"].tolist()))  # noqa: E731
+        assert sorted(map(key, got)) == sorted(map(key, want))

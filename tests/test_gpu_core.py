@@ -268,6 +268,12 @@ def test_error_paths(dev):
     with pytest.raises(core.SxError) as e:
         dev.enumerate(np.array([[o.word(60), 0]], np.uint32))
     assert e.value.code == core.SX_EOPCODE
-    with pytest.raises(core.SxError) as e:
-        dev.enumerate(np.zeros((1, 2), np.uint32), max_cand_len=3)
+    with pytest.raises(core.SxError) as e:  # candidates of length 4
+        dev.enumerate(np.zeros((1, 2), np.uint32), max_cand_len=4)
+    assert e.value.code == core.SX_EINVAL
+    with pytest.raises(core.SxError) as e:  # length-4 targets come as a list
+        dev.enumerate(None, target_len=4, max_cand_len=3)
+    assert e.value.code == core.SX_EINVAL
+    with pytest.raises(core.SxError) as e:  # the length-3 pool is first-hit only
+        dev.enumerate(np.zeros((1, 4), np.uint32), target_len=4, max_cand_len=3, hits=core.HITS_ALL)
     assert e.value.code == core.SX_EINVAL
