@@ -30,6 +30,7 @@
 //                    copied to a pinned host buffer per segment on a second
 //                    stream.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -1074,67 +1075,10 @@ __device__ __forceinline__ bool exh_pass(uint32_t jv, uint32_t tl, uint32_t cl, 
   return false;
 }
 
-// Packed pass: lane L evaluates states it + 128 j + L (low halves) and
-// it + 128 j + 64 + L (high halves), j < KP, below count, with the packed
-// interpreter (two states per register).  it is a multiple of 128, so bit 6
-// of a low state's index is 0 and the high state's index is that index | 64:
-// each of its fields is the low state's field plus hiadd[r] (the field
-// holding index bit 6 gets that bit).  start < 2^32 is a multiple of the
-// chunk size.
-template <int KP>
-__device__ __forceinline__ bool exh_pass_pk(uint32_t jv, uint32_t tl, uint32_t cl, uint32_t pm, const ExhFields &f,
-                                            const uint32_t (&hiadd)[8], uint32_t start, uint32_t it,
-                                            uint32_t count, uint32_t lane, uint32_t ji, uint32_t *fail) {
-  PState ts[KP], cs[KP];
-  uint32_t actm[KP];
-  auto pack = [&](uint32_t idx, int r) {
-    // v * 0x10001 + add as one 24-bit multiply-add (v < 256)
-    return __builtin_bit_cast(pk16, __umul24(__builtin_amdgcn_ubfe(idx, f.sh[r], f.wd[r]), 0x10001u) + hiadd[r]);
-  };
-#pragma unroll
-  for (int j = 0; j < KP; j++) {
-    const uint32_t k0 = it + j * 2 * kWave + lane;
-    const bool a0 = k0 < count, a1 = k0 + kWave < count;
-    actm[j] = (a0 ? 0x0000FFFFu : 0u) | (a1 ? 0xFFFF0000u : 0u);
-    const uint32_t idx = start | (a0 ? k0 : 0u);
-    PState s;
-    s.a = pack(idx, RA);
-    s.x = pack(idx, RX);
-    s.y = pack(idx, RY);
-    s.p = pack(idx, RP);
-    s.s = f.wd[RS] ? pack(idx, RS) : pk_splat(0xFD);
-    s.m = pack(idx, RM);
-    s.s0 = pack(idx, RS0);
-    s.s1 = pack(idx, RS1);
-    ts[j] = s;
-    cs[j] = s;
-  }
-  for (uint32_t q = 0; q < tl; q++) {
-    const uint32_t w = __builtin_amdgcn_readlane(jv, q);
-    step_pk<KP>(ts, w & 0xFFFFu, w >> 16);
-  }
-  for (uint32_t q = 0; q < cl; q++) {
-    const uint32_t w = __builtin_amdgcn_readlane(jv, kJobWords + q);
-    step_pk<KP>(cs, w & 0xFFFFu, w >> 16);
-  }
-  const pk16 pm2 = pk_splat(pm);
-  bool bad = false;
-#pragma unroll
-  for (int j = 0; j < KP; j++) {
-    const pk16 d = (ts[j].a ^ cs[j].a) | (ts[j].x ^ cs[j].x) | (ts[j].y ^ cs[j].y) | ((ts[j].p ^ cs[j].p) & pm2) |
-                   (ts[j].s ^ cs[j].s) | (ts[j].m ^ cs[j].m) | (ts[j].s0 ^ cs[j].s0) | (ts[j].s1 ^ cs[j].s1);
-    bad |= (__builtin_bit_cast(uint32_t, d) & actm[j]) != 0;
-  }
-  if (__ballot(bad)) {
-    if (lane == 0) ((volatile uint32_t *)fail)[ji] = 1u;
-    return true;
-  }
-  return false;
-}
-
-// One chunk of job ji (jv = the job spread over the lanes; single = the job
-// is this one chunk, c its global chunk number).  Returns the states swept.
-template <int K, bool Pk>
+// One chunk of job ji with the one-state interpreter (jv = the job spread over
+// the lanes; single = the job is this one chunk, c its global chunk number).
+// Returns the states swept.
+template <int K>
 __device__ __forceinline__ uint32_t exh_chunk(uint32_t ji, uint32_t jv, bool single, uint64_t c,
                                               const uint64_t *__restrict__ offs, uint32_t *fail, uint32_t lane) {
   uint64_t local = 0;
@@ -1184,22 +1128,6 @@ __device__ __forceinline__ uint32_t exh_chunk(uint32_t ji, uint32_t jv, bool sin
         failed = exh_pass<1, true>(jv, tl, cl, pm, f, start, it, count, lane, ji, fail);
         if (!failed && (it & (32 * kWave - 1)) == 31 * kWave && ((volatile uint32_t *)fail)[ji]) failed = true;
       }
-    } else if (Pk) {
-      // packed: the first pass covers 128 states, the later ones K x 128
-      uint32_t hiadd[8];
-#pragma unroll
-      for (int r = 0; r < 8; r++)
-        hiadd[r] = (f.wd[r] && f.sh[r] <= 6 && 6 < f.sh[r] + f.wd[r]) ? (1u << (6 - f.sh[r])) << 16 : 0u;
-      const uint32_t st32 = (uint32_t)start;
-      done_states = done_before + min(2u * kWave, count);
-      failed = exh_pass_pk<1>(jv, tl, cl, pm, f, hiadd, st32, 0, count, lane, ji, fail);
-      for (uint32_t it = 2 * kWave; it < count && !failed; it += K * 2 * kWave) {
-        done_states = done_before + min(it + K * 2 * kWave, count);
-        failed = exh_pass_pk<K>(jv, tl, cl, pm, f, hiadd, st32, it, count, lane, ji, fail);
-        if (!failed && !single && ((it - 2 * kWave) & (32 * kWave - 1)) == 32 * kWave - K * 2 * kWave &&
-            ((volatile uint32_t *)fail)[ji])
-          failed = true;
-      }
     } else {
       // the first pass covers 64 states (most failing pairs stop there), the
       // later ones K x 64 with one opcode dispatch per K states
@@ -1218,19 +1146,179 @@ __device__ __forceinline__ uint32_t exh_chunk(uint32_t ji, uint32_t jv, bool sin
   return done_states;
 }
 
+// Packed sweep of one chunk with every per-wave quantity on the scalar unit.
+// jw = the job's 28 words (a wave-uniform address: scalar loads).  For a
+// sub-sweep of 2^lg states the varying registers take consecutive bit fields
+// of the state index in the order A X Y S M S0 S1 P (the carry last), and the
+// index splits into lane bits (0-5), the half bit (6), bit 7, pass bits
+// (8-13) and chunk bits (14 and up): disjoint, so a field is the OR of its
+// parts.  Each lane builds its fields from its lane, half and chunk bits with
+// bit 7 clear and set once per sub-sweep (base[0], base[1]).  The first field
+// is 8 bits wide whenever the sweep has more than 256 states, so the pass bits
+// land in one field only, the second (R2, from bit 8): a pass ORs (it >> 8)
+// into R2 and leaves the others at their base.  Fields lie in bits [0, lg), so
+// for lg < 7 the lanes past 2^lg repeat states of lanes below it (no activity
+// mask is needed) and for lg >= 7 every pass is full.  KP packed pairs per lane
+// and opcode dispatch: KP = 1 takes base[(it >> 7) & 1] per pass of 128 states,
+// KP = 2 both bases per pass of 256.  (The instruction words by scalar loads
+// or by readlane from a register measured the same.)
+template <int KP>
+__device__ __forceinline__ uint32_t exh_chunk_pk(const uint32_t *__restrict__ jw, uint32_t ji, bool single,
+                                                 uint64_t c, const uint64_t *__restrict__ offs, uint32_t *fail,
+                                                 uint32_t lane) {
+  static_assert(KP == 1 || KP == 2, "pairs per lane");
+  uint64_t local = 0;
+  if (!single) {
+    if (uni(((volatile uint32_t *)fail)[ji])) return 0;
+    local = uni64(c - offs[ji]);
+  }
+  const uint32_t hdr = jw[20];  // tl, cl, pass, pmask
+  const uint32_t tl = hdr & 0xFFu, cl = (hdr >> 8) & 0xFFu, pm = hdr >> 24;
+  const uint32_t h23 = jw[23];  // nsets, pad[0] = packed
+  const uint32_t nsets = h23 & 0xFFu;
+  const bool packed = ((h23 >> 8) & 0xFFu) != 0;  // one wave sweeps every sub-sweep of the job
+  const uint64_t sets8 = (uint64_t)jw[24] | ((uint64_t)jw[25] << 32);
+  const uint64_t logs8 = (uint64_t)jw[26] | ((uint64_t)jw[27] << 32);
+  uint32_t s_first = 0;
+  if (!packed) {
+    for (uint32_t s = 0; s < nsets; s++) {
+      const uint32_t l = (uint32_t)(logs8 >> (8 * s)) & 0xFFu;
+      const uint64_t nch = l > kChunkLog ? (1ull << (l - kChunkLog)) : 1ull;
+      if (local < nch) {
+        s_first = s;
+        break;
+      }
+      local -= nch;
+    }
+  }
+  const uint32_t s_last = packed ? nsets : s_first + 1;
+  const pk16 pm2 = pk_splat(pm);
+  uint32_t done_states = 0;
+  for (uint32_t si = s_first; si < s_last; si++) {
+    const uint32_t set = (uint32_t)(sets8 >> (8 * si)) & 0xFFu, lg = (uint32_t)(logs8 >> (8 * si)) & 0xFFu;
+    const uint32_t count = lg < kChunkLog ? (1u << lg) : (1u << kChunkLog);
+    if (lg > 32) {
+      // index wider than 32 bits: one state per lane and pass (register order)
+      ExhFields f;
+      uint32_t pos = 0;
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        const uint32_t w = (set >> r) & 1u ? (r == RP ? 1u : 8u) : 0u;
+        f.sh[r] = pos;
+        f.wd[r] = w;
+        pos += w;
+      }
+      const uint64_t start = local << kChunkLog;
+      const uint32_t jw_l = lane < (uint32_t)kJobDwords ? jw[lane] : 0u;
+      for (uint32_t it = 0; it < count; it += kWave) {
+        done_states += min((uint32_t)kWave, count - it);
+        if (exh_pass<1, true>(jw_l, tl, cl, pm, f, start, it, count, lane, ji, fail)) return done_states;
+        if ((it & (32 * kWave - 1)) == 31 * kWave && uni(((volatile uint32_t *)fail)[ji])) return done_states;
+      }
+      continue;
+    }
+    // field positions in the order A X Y S M S0 S1 P; sel[r] = all ones for R2
+    uint32_t sh[8], wd[8], sel[8];
+    {
+      constexpr int kOrder[8] = {RA, RX, RY, RS, RM, RS0, RS1, RP};
+      uint32_t pos = 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int r = kOrder[k];
+        const uint32_t w = (set >> r) & 1u ? (r == RP ? 1u : 8u) : 0u;
+        sh[r] = pos;
+        wd[r] = w;
+        sel[r] = (w && pos == 8u) ? 0xFFFFFFFFu : 0u;
+        pos += w;
+      }
+    }
+    const uint32_t st32 = packed ? 0u : (uint32_t)(local << kChunkLog);
+    PState base[2];
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const uint32_t lo = st32 | lane | ((uint32_t)b << 7), hi = lo | (uint32_t)kWave;
+      auto fld = [&](int r) {
+        return __builtin_bit_cast(pk16, ((lo >> sh[r]) & ((1u << wd[r]) - 1u)) |
+                                            (((hi >> sh[r]) & ((1u << wd[r]) - 1u)) << 16));
+      };
+      base[b].a = fld(RA);
+      base[b].x = fld(RX);
+      base[b].y = fld(RY);
+      base[b].p = fld(RP);
+      base[b].s = wd[RS] ? fld(RS) : pk_splat(0xFD);
+      base[b].m = fld(RM);
+      base[b].s0 = fld(RS0);
+      base[b].s1 = fld(RS1);
+    }
+    for (uint32_t it = 0; it < count; it += 2 * KP * kWave) {
+      // the pass bits' value in R2, in both halves, on the scalar unit
+      const uint32_t inc = (it >> 8) * 0x10001u;
+      PState ts[KP], cs[KP];
+      auto init = [&](const PState &bs) {
+        PState s;
+        s.a = bs.a | __builtin_bit_cast(pk16, inc & sel[RA]);
+        s.x = bs.x | __builtin_bit_cast(pk16, inc & sel[RX]);
+        s.y = bs.y | __builtin_bit_cast(pk16, inc & sel[RY]);
+        s.p = bs.p | __builtin_bit_cast(pk16, inc & sel[RP]);
+        s.s = bs.s | __builtin_bit_cast(pk16, inc & sel[RS]);
+        s.m = bs.m | __builtin_bit_cast(pk16, inc & sel[RM]);
+        s.s0 = bs.s0 | __builtin_bit_cast(pk16, inc & sel[RS0]);
+        s.s1 = bs.s1 | __builtin_bit_cast(pk16, inc & sel[RS1]);
+        return s;
+      };
+      if (KP == 2) {
+        ts[0] = init(base[0]);
+        ts[KP - 1] = init(base[1]);
+      } else if ((it >> 7) & 1u) {
+        ts[0] = init(base[1]);
+      } else {
+        ts[0] = init(base[0]);
+      }
+#pragma unroll
+      for (int j = 0; j < KP; j++) cs[j] = ts[j];
+      for (uint32_t q = 0; q < tl; q++) {
+        const uint32_t w = jw[q];
+        step_pk<KP>(ts, w & 0xFFFFu, w >> 16);
+      }
+      for (uint32_t q = 0; q < cl; q++) {
+        const uint32_t w = jw[kJobWords + q];
+        step_pk<KP>(cs, w & 0xFFFFu, w >> 16);
+      }
+      bool bad = false;
+#pragma unroll
+      for (int j = 0; j < KP; j++) {
+        const pk16 d = (ts[j].a ^ cs[j].a) | (ts[j].x ^ cs[j].x) | (ts[j].y ^ cs[j].y) |
+                       ((ts[j].p ^ cs[j].p) & pm2) | (ts[j].s ^ cs[j].s) | (ts[j].m ^ cs[j].m) |
+                       (ts[j].s0 ^ cs[j].s0) | (ts[j].s1 ^ cs[j].s1);
+        bad |= __builtin_bit_cast(uint32_t, d) != 0;
+      }
+      done_states += min(2u * KP * kWave, count - it);
+      if (__ballot(bad)) {
+        if (lane == 0) ((volatile uint32_t *)fail)[ji] = 1u;
+        return done_states;
+      }
+      // other chunks of this job may have failed: look every 2,048 states
+      if (!single && (it & (32 * kWave - 1)) == 32 * kWave - 2 * KP * kWave && uni(((volatile uint32_t *)fail)[ji]))
+        return done_states;
+    }
+  }
+  return done_states;
+}
+
 __device__ __forceinline__ uint32_t job_dword(const Job *jobs, uint32_t ji, uint32_t lane) {
   return lane < (uint32_t)kJobDwords ? reinterpret_cast<const uint32_t *>(jobs + ji)[lane] : 0u;
 }
 
-// Exhaustive sweep: one wave per chunk of 2^14 states (launching a wave per
-// chunk lets the hardware balance the uneven chunks; persistent waves with
-// prefetch and two chunks per wave both measured slower).  The chunks of all
-// jobs are numbered consecutively (offs = exclusive scan of the per-job chunk
-// counts); a launch covers chunks [chunk0, chunk0 + nchunks).  The job is one
-// map load away, or found by a 64-ary search over the chunk offsets when
-// there is no map (each step the 64 lanes test 64 evenly spaced offsets and
-// the range shrinks 64-fold).  K = states per lane in the passes after the
-// first.
+// Exhaustive sweep, state-parallel: one wave per chunk of 2^14 states
+// (launching a wave per chunk lets the hardware balance the uneven chunks;
+// persistent waves with prefetch and two chunks per wave both measured
+// slower).  The chunks of all jobs are numbered consecutively (offs =
+// exclusive scan of the per-job chunk counts); a launch covers chunks
+// [chunk0, chunk0 + nchunks).  The job is one map load away, or found by a
+// 64-ary search over the chunk offsets when there is no map (each step the 64
+// lanes test 64 evenly spaced offsets and the range shrinks 64-fold).  Pk:
+// the packed interpreter with K pairs per lane and dispatch; else the
+// one-state interpreter with K states per lane in the passes after the first.
 template <int K, bool Pk>
 __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, uint32_t nj,
                                                  const uint64_t *__restrict__ offs,
@@ -1238,7 +1326,7 @@ __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, u
                                                  uint64_t nchunks, uint32_t *fail,
                                                  unsigned long long *evaluated) {
   const uint32_t lane = lane_id();
-  const uint64_t w = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint64_t w = (uint64_t)blockIdx.x * 4 + uni(threadIdx.x >> 6);
   if (w >= nchunks) return;  // whole wave
   const uint64_t c = chunk0 + w;
   uint32_t lo = 0, hi = nj;
@@ -1258,8 +1346,185 @@ __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, u
     hi = min(hi, lo + step);
   }
   const uint32_t ji = uni(lo);
-  const uint32_t done = exh_chunk<K, Pk>(ji, job_dword(jobs, ji, lane), single, c, offs, fail, lane);
+  uint32_t done;
+  if constexpr (Pk)
+    done = exh_chunk_pk<K>(reinterpret_cast<const uint32_t *>(jobs + ji), ji, single, c, offs, fail, lane);
+  else
+    done = exh_chunk<K>(ji, job_dword(jobs, ji, lane), single, c, offs, fail, lane);
   if (lane == 0 && done) atomicAdd(&evaluated[w & (kEvalShards - 1)], (unsigned long long)done);
+}
+
+// Sweep of small plans with one job per lane.
+//
+// Most jobs of length-2 targets against single-instruction candidates have
+// sub-sweeps of 256 or 512 states (one 8-bit register, and the carry when it
+// matters), so a wave per job spends most of its time on the job's set-up.
+// Such jobs are keyed by (opcode triple, compare mask), the plan-table index,
+// and sorted by key; wave w takes sorted positions [64 w, 64 w + 64).  Lanes
+// whose jobs share a key share the plan and every opcode dispatch (only the
+// immediates differ, per lane), so the wave sweeps up to 64 jobs at once:
+// every register value of a pass is wave-uniform, and lane L holds its job's
+// two states of the pass in the low and high halves.  A wave with lanes of
+// several keys (at the ends of the sorted runs) takes them one key after the
+// other.  A job fails at its first mismatching state; the wave leaves a key
+// once every one of its jobs has failed.
+constexpr uint32_t kGroupNone = 0xFFFFFFFFu;
+constexpr uint32_t kGroupKeyBits = 19;  // kPlanTab < 2^19; kGroupNone sorts last
+static_assert(kPlanTab < (1u << kGroupKeyBits), "group keys");
+constexpr uint32_t kGroupSort = (1u << kGroupKeyBits) - 1u;  // the sort key of ineligible jobs
+// Keyed jobs touch only A, X, Y and P (their opcodes and their sub-sweeps).
+constexpr uint32_t kRegsAXYP = (1u << RA) | (1u << RX) | (1u << RY) | (1u << RP);
+
+// Key of each job (kGroupSort when it stays with the state-parallel sweep);
+// a keyed job gets zero chunks there.
+__global__ __launch_bounds__(256) void k_group_keys(const Job *__restrict__ jobs, uint32_t nj, uint32_t *chunks,
+                                                    uint32_t *key, uint32_t *idx) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nj) return;
+  const uint32_t *jw = reinterpret_cast<const uint32_t *>(jobs + i);
+  const uint32_t hdr = jw[20], h23 = jw[23];
+  const uint32_t tl = hdr & 0xFFu, cl = (hdr >> 8) & 0xFFu, pm = hdr >> 24, nsets = h23 & 0xFFu;
+  const uint32_t t0 = jw[0] & 0xFFFFu, t1 = jw[1] & 0xFFFFu, c = jw[kJobWords] & 0xFFFFu;
+  const uint64_t logs8 = (uint64_t)jw[26] | ((uint64_t)jw[27] << 32);
+  bool ok = tl == 2 && cl == 1 && t0 < (uint32_t)OP_COUNT && t1 < (uint32_t)OP_COUNT && c < (uint32_t)OP_COUNT &&
+            nsets >= 1 && nsets <= 8;
+  const uint64_t sets8 = (uint64_t)jw[24] | ((uint64_t)jw[25] << 32);
+  uint32_t touch = ok ? pk_touch(t0) | pk_touch(t1) | pk_touch(c) : 0u;
+  for (uint32_t s = 0; ok && s < nsets; s++) {
+    ok = ((logs8 >> (8 * s)) & 0xFFu) <= 9u;
+    touch |= (uint32_t)(sets8 >> (8 * s)) & 0xFFu;
+  }
+  ok = ok && (touch & ~kRegsAXYP) == 0;
+  key[i] = ok ? ((t0 * OP_COUNT + t1) * OP_COUNT + c) * 2u + (pm != kPMask ? 1u : 0u) : kGroupSort;
+  idx[i] = i;
+  if (ok) chunks[i] = 0;
+}
+
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// The items of one key group (lanes gm of the wave) that wave u of the
+// block takes: item it of sub-sweep si when (items before si) + it = u (mod
+// 4).  Regs: the state registers the group's opcodes and sub-sweeps touch;
+// the others stay at their domain values and the compiler drops them.
+// Returns the lanes whose job failed; *evald counts the states swept for
+// jobs still open.
+template <int K, uint32_t Regs>
+__device__ __forceinline__ uint64_t group_items(uint64_t gm, uint32_t u, uint32_t op0, uint32_t op1, uint32_t opc,
+                                                pk16 imm0, pk16 imm1, pk16 immc, pk16 pm2, uint32_t nsets,
+                                                uint64_t sets8, uint64_t logs8, const uint32_t *fail, uint32_t ji,
+                                                uint64_t *evald) {
+  uint64_t failed = 0;
+  uint32_t item = 0;  // items of the sub-sweeps before si
+  for (uint32_t si = 0; si < nsets; si++) {
+    const uint32_t set = (uint32_t)(sets8 >> (8 * si)) & 0xFFu, lg = (uint32_t)(logs8 >> (8 * si)) & 0xFFu;
+    const uint32_t npairs = lg >= 1 ? 1u << (lg - 1) : 1u;  // two states per lane and pair
+    const uint32_t nitems = (npairs + 63u) / 64u;
+    for (uint32_t it = (u + 4u - (item & 3u)) & 3u; it < nitems; it += 4) {  // item + it = u (mod 4)
+      // the jobs still open (other waves of the block may have failed them)
+      const uint64_t live = gm & ~ballot(((volatile const uint32_t *)fail)[ji] != 0) & ~failed;
+      if (!live) continue;
+      // at most one 8-bit register r1 (index bits 0-7) and the carry (the next bit)
+      int r1 = -1;
+#pragma unroll
+      for (int r = 7; r >= 0; r--)
+        if (r != RP && ((set >> r) & 1u)) r1 = r;
+      const bool hasp = (set >> RP) & 1u;
+      const uint32_t q0 = it * 64u, q1 = min(npairs, q0 + 64u);
+      uint64_t gfail = 0;
+      for (uint32_t q = q0; q < q1 && (gfail | ~live) != ~0ull; q += K) {
+        PState ts[K], cs[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+          const uint32_t qj = q + j < q1 ? q + j : q;  // past the end: a pair already swept
+          const uint32_t lo = 2 * qj, hi = lg ? 2 * qj + 1 : 0;
+          PState st;
+          st.a = st.x = st.y = st.p = st.m = st.s0 = st.s1 = pk_splat(0);
+          st.s = pk_splat(0xFD);
+          if (hasp) st.p = __builtin_bit_cast(pk16, r1 >= 0 ? (lo >> 8) | ((hi >> 8) << 16) : lo | (hi << 16));
+          if (r1 >= 0) {
+            const pk16 v = __builtin_bit_cast(pk16, (lo & 0xFFu) | ((hi & 0xFFu) << 16));
+            if (r1 == RA) st.a = v;
+            if constexpr ((Regs >> RX) & 1u) if (r1 == RX) st.x = v;
+            if constexpr ((Regs >> RY) & 1u) if (r1 == RY) st.y = v;
+            if constexpr ((Regs >> RS) & 1u) if (r1 == RS) st.s = v;
+            if constexpr ((Regs >> RM) & 1u) if (r1 == RM) st.m = v;
+            if constexpr ((Regs >> RS0) & 1u) if (r1 == RS0) st.s0 = v;
+            if constexpr ((Regs >> RS1) & 1u) if (r1 == RS1) st.s1 = v;
+          }
+          ts[j] = st;
+          cs[j] = st;
+        }
+        step_pk_imm<K, Regs>(ts, op0, imm0);
+        step_pk_imm<K, Regs>(ts, op1, imm1);
+        step_pk_imm<K, Regs>(cs, opc, immc);
+        bool bad = false;
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+          const pk16 d = (ts[j].a ^ cs[j].a) | (ts[j].x ^ cs[j].x) | (ts[j].y ^ cs[j].y) |
+                         ((ts[j].p ^ cs[j].p) & pm2) | (ts[j].s ^ cs[j].s) | (ts[j].m ^ cs[j].m) |
+                         (ts[j].s0 ^ cs[j].s0) | (ts[j].s1 ^ cs[j].s1);
+          bad |= __builtin_bit_cast(uint32_t, d) != 0;
+        }
+        *evald += (uint64_t)__builtin_popcountll(live & ~gfail) * min((uint32_t)K, q1 - q) * (lg ? 2u : 1u);
+        gfail |= ballot(bad) & live;
+      }
+      failed |= gfail;
+    }
+    item += nitems;
+  }
+  return failed;
+}
+
+// A workgroup of 4 waves per 64 sorted positions: each wave takes items u,
+// u + 4, ... of the block's plan, an item being up to 128 states (64 pairs)
+// of one sub-sweep, so the waves stay short and many; every lane of every
+// wave reads the same 64 jobs.  K pairs per lane share one opcode dispatch.
+// Keyed groups touch only A, X, Y and P, so the interpreter runs without the
+// other registers.
+template <int K>
+__global__ __launch_bounds__(256) void k_exh_group(const Job *__restrict__ jobs, const uint32_t *__restrict__ skey,
+                                                   const uint32_t *__restrict__ sidx, uint32_t n, uint32_t *fail,
+                                                   unsigned long long *evaluated) {
+  static_assert(K == 1 || K == 2 || K == 4 || K == 8, "pairs per lane");
+  const uint32_t lane = lane_id();
+  const uint32_t u = uni(threadIdx.x >> 6);
+  const uint32_t p = blockIdx.x * kWave + lane;
+  const uint32_t key0 = p < n ? skey[p] : kGroupSort;
+  const bool mine = key0 != kGroupSort;
+  uint64_t todo = ballot(mine);
+  if (!todo) return;
+  const uint32_t ji = mine ? sidx[p] : 0u;
+  const uint32_t *jl = reinterpret_cast<const uint32_t *>(jobs + ji);
+  const uint32_t lw0 = jl[0], lw1 = jl[1], lc0 = jl[kJobWords], lhdr = jl[20];
+  const uint32_t key = key0 | ((lhdr >> 24) << 24);  // with the exact compare mask
+  const pk16 imm0 = pk_splat(lw0 >> 16), imm1 = pk_splat(lw1 >> 16), immc = pk_splat(lc0 >> 16);
+  uint64_t failed = 0, evald = 0;
+  while (todo) {
+    const uint32_t first = (uint32_t)__builtin_ctzll(todo);
+    const uint32_t k = __builtin_amdgcn_readlane(key, first);
+    const uint64_t gm = ballot(mine && key == k);
+    todo &= ~gm;
+    if (!(mine && key == k)) continue;
+    // the group's opcodes, mask and plan (the first lane's job)
+    const uint32_t *jw = reinterpret_cast<const uint32_t *>(jobs + __builtin_amdgcn_readlane(ji, first));
+    const uint32_t op0 = jw[0] & 0xFFFFu, op1 = jw[1] & 0xFFFFu, opc = jw[kJobWords] & 0xFFFFu;
+    const uint32_t pm = jw[20] >> 24, nsets = jw[23] & 0xFFu;
+    const uint64_t sets8 = (uint64_t)jw[24] | ((uint64_t)jw[25] << 32);
+    const uint64_t logs8 = (uint64_t)jw[26] | ((uint64_t)jw[27] << 32);
+    failed |= group_items<K, kRegsAXYP>(gm, u, op0, op1, opc, imm0, imm1, immc, pk_splat(pm), nsets, sets8, logs8,
+                                        fail, ji, &evald);
+  }
+  if ((failed >> lane) & 1u) fail[ji] = 1u;
+  // evaluated[0 .. kEvalShards) counts states, the next kEvalShards keyed jobs
+  if (lane == 0 && evald)
+    atomicAdd(&evaluated[(blockIdx.x * 4 + u) & (kEvalShards - 1)], (unsigned long long)evald);
+  const uint64_t keyed = ballot(mine);  // all lanes take part in the ballot
+  if (lane == 0 && u == 0)
+    atomicAdd(&evaluated[kEvalShards + (blockIdx.x & (kEvalShards - 1))], (unsigned long long)__popcll(keyed));
 }
 
 // Resolve a round: a passing job emits its rule and closes (target, pass); a
@@ -1939,8 +2204,10 @@ struct sx_handle {
   int num_cu = 256;
   bool poisoned = false;  // a device stage timed out: the stream is unusable
   int bulk_after = 1;     // first-hit rounds before the one-batch tail (SX_BULK_AFTER); -1 = by survivor count
-  int exh_k = 1;          // states (pairs when packed) per lane and opcode dispatch in the sweep (SX_EXH_K: 1 or 2)
+  int exh_k = 2;          // states (pairs when packed) per lane and opcode dispatch in the sweep (SX_EXH_K: 1 or 2)
   int exh_pk = 1;         // packed two-states-per-register interpreter in the sweep (SX_EXH_PK=0: one state)
+  int exh_group = 4;      // small plans swept one job per lane, this many pairs per lane and dispatch
+                          // (SX_EXH_GROUP: 1, 2, 4 or 8; 0: every job state-parallel)
   // the plan table in B_PLANTAB holds the plans of (sweep mode, relaxed mask)
   // = (ptab_mode, ptab_pm); -1: none yet
   int ptab_mode = -1;
@@ -1949,8 +2216,8 @@ struct sx_handle {
   uint32_t surv_cap_hint = 0;  // QuickCheck survivor buffer size that sufficed before
   // kernel-only timing of the QuickCheck and exhaustive-sweep launches
   hipEvent_t kev[2] = {nullptr, nullptr};
-  double k_qc_ms = 0, k_exh_ms = 0;
-  uint64_t k_qc_launches = 0, k_exh_launches = 0;
+  double k_qc_ms = 0, k_exh_ms = 0, k_grp_ms = 0;
+  uint64_t k_qc_launches = 0, k_exh_launches = 0, k_grp_launches = 0;
   // event pairs around QuickCheck and sweep launches, read at the next host
   // synchronisation instead of synchronising per launch
   struct Timed {
@@ -2001,7 +2268,8 @@ enum BufId {
   B_FAIL, B_RULES, B_SCRATCH, B_GEN_COUNTS, B_GEN_OFFS, B_BADOP, B_P_TW, B_P_TL, B_P_CW, B_P_CL,
   B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR, B_HEADS, B_RULEFLAG, B_RULEOFFS, B_RULES2,
   B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_PLANTAB, B_EFP, B_ERULES, B_L3_LIST, B_L3_HK, B_L3_CNT,
-  B_L3_KEYS, B_L3_HEAD, B_L3_NEXT, B_L3_BLOOM, B_L3_RAW, B_L3_C3, B_L3_BEST, B_COUNT
+  B_L3_KEYS, B_L3_HEAD, B_L3_NEXT, B_L3_BLOOM, B_L3_RAW, B_L3_C3, B_L3_BEST, B_GKEY, B_GIDX, B_GKEY_S, B_GIDX_S,
+  B_GSORT, B_COUNT
 };
 static_assert(B_COUNT <= 64, "sx_handle::bufs holds every buffer id");
 
@@ -2060,6 +2328,9 @@ void harvest_timed(sx_handle *h) {
     if (h->timed[i].kind == 0) {
       h->k_qc_ms += ms;
       h->k_qc_launches++;
+    } else if (h->timed[i].kind == 2) {
+      h->k_grp_ms += ms;
+      h->k_grp_launches++;
     } else {
       h->k_exh_ms += ms;
       h->k_exh_launches++;
@@ -2359,12 +2630,44 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, bo
   ENSURE(B_OFFS, uint64_t, nj + 1, offs);
   ENSURE(B_FAIL, uint32_t, nj + 1, fail);
   ENSURE(B_TOTAL, uint64_t, 1, total);
-  unsigned long long *evald;  // evaluated states, added up across calls
-  ENSURE(B_CURSOR, unsigned long long, kEvalShards, evald);
+  unsigned long long *evald;  // evaluated states, added up across calls (three banks)
+  ENSURE(B_CURSOR, unsigned long long, 3 * kEvalShards, evald);
   if (!planned)
     hipLaunchKernelGGL(k_exh_plan, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, jobs, nj, mode, ptab, pm_masked,
                        chunks, fail);
   HIPCHK(h, hipGetLastError());
+  if (h->exh_group && nj) {
+    // jobs with small plans: one job per lane, sorted by (opcodes, mask); they
+    // leave the state-parallel sweep below with zero chunks
+    uint32_t *key, *idx, *skey, *sidx;
+    ENSURE(B_GKEY, uint32_t, nj, key);
+    ENSURE(B_GIDX, uint32_t, nj, idx);
+    ENSURE(B_GKEY_S, uint32_t, nj, skey);
+    ENSURE(B_GIDX_S, uint32_t, nj, sidx);
+    hipLaunchKernelGGL(k_group_keys, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, jobs, nj, chunks, key, idx);
+    HIPCHK(h, hipGetLastError());
+    size_t tmp_bytes = 0;
+    HIPCHK(h, rocprim::radix_sort_pairs(nullptr, tmp_bytes, key, skey, idx, sidx, nj, 0, kGroupKeyBits, h->stream));
+    uint8_t *tmp;
+    ENSURE(B_GSORT, uint8_t, tmp_bytes + 1, tmp);
+    HIPCHK(h, rocprim::radix_sort_pairs(tmp, tmp_bytes, key, skey, idx, sidx, nj, 0, kGroupKeyBits, h->stream));
+    size_t ts = 0;
+    int rc = timed_begin(h, 2, &ts);
+    if (rc) return rc;
+    const dim3 gg(blocks_for(nj, kWave));
+    unsigned long long *gev = evald + kEvalShards;
+    if (h->exh_group == 8)
+      hipLaunchKernelGGL((k_exh_group<8>), gg, dim3(256), 0, h->stream, jobs, skey, sidx, nj, fail, gev);
+    else if (h->exh_group == 4)
+      hipLaunchKernelGGL((k_exh_group<4>), gg, dim3(256), 0, h->stream, jobs, skey, sidx, nj, fail, gev);
+    else if (h->exh_group == 2)
+      hipLaunchKernelGGL((k_exh_group<2>), gg, dim3(256), 0, h->stream, jobs, skey, sidx, nj, fail, gev);
+    else
+      hipLaunchKernelGGL((k_exh_group<1>), gg, dim3(256), 0, h->stream, jobs, skey, sidx, nj, fail, gev);
+    HIPCHK(h, hipGetLastError());
+    rc = timed_end(h, ts);
+    if (rc) return rc;
+  }
   int rc = scan_u32(h, chunks, nj, offs, total);
   if (rc) return rc;
   uint64_t tot = 0;
@@ -2504,23 +2807,29 @@ int run_l3(sx_handle *h, uint32_t nt, const uint32_t *tw, uint32_t tl, const uin
   return SX_OK;
 }
 
-// The evaluated-state counter (sharded): cleared, and summed on the host.
+// The evaluated-state counters (sharded, three banks: the state-parallel
+// sweep's states, the grouped sweep's states, the grouped sweep's jobs):
+// cleared, and summed on the host.
 int clear_evaluated(sx_handle *h) {
   unsigned long long *evald;
-  ENSURE(B_CURSOR, unsigned long long, kEvalShards, evald);
-  HIPCHK(h, hipMemsetAsync(evald, 0, kEvalShards * sizeof(unsigned long long), h->stream));
+  ENSURE(B_CURSOR, unsigned long long, 3 * kEvalShards, evald);
+  HIPCHK(h, hipMemsetAsync(evald, 0, 3 * kEvalShards * sizeof(unsigned long long), h->stream));
   return SX_OK;
 }
-int read_evaluated(sx_handle *h, uint64_t *sum) {
+int read_evaluated(sx_handle *h, uint64_t *sum, uint64_t *sum_group, uint64_t *group_jobs) {
   unsigned long long *evald;
-  ENSURE(B_CURSOR, unsigned long long, kEvalShards, evald);
-  std::vector<unsigned long long> shards(kEvalShards);
-  HIPCHK(h, hipMemcpyAsync(shards.data(), evald, kEvalShards * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                           h->stream));
+  ENSURE(B_CURSOR, unsigned long long, 3 * kEvalShards, evald);
+  std::vector<unsigned long long> shards(3 * kEvalShards);
+  HIPCHK(h, hipMemcpyAsync(shards.data(), evald, 3 * kEvalShards * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, h->stream));
   int rc = sync_wd(h, "exhaustive sweep");
   if (rc) return rc;
-  *sum = 0;
-  for (auto v : shards) *sum += v;
+  *sum = *sum_group = *group_jobs = 0;
+  for (int i = 0; i < kEvalShards; i++) {
+    *sum += shards[i] + shards[kEvalShards + i];
+    *sum_group += shards[kEvalShards + i];
+    *group_jobs += shards[2 * kEvalShards + i];
+  }
   return SX_OK;
 }
 
@@ -2658,6 +2967,7 @@ extern "C" int sx_open(int device, sx_handle **out) {
   if (const char *e = getenv("SX_BULK_AFTER")) h->bulk_after = std::max(-1, atoi(e));
   if (const char *e = getenv("SX_EXH_K")) h->exh_k = atoi(e) == 1 ? 1 : 2;
   if (const char *e = getenv("SX_EXH_PK")) h->exh_pk = atoi(e) != 0;
+  if (const char *e = getenv("SX_EXH_GROUP")) h->exh_group = atoi(e);
   if (const char *e = getenv("SX_SEG_LOG")) h->seg_log = std::min(24, std::max(16, atoi(e)));
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking) != hipSuccess ||
@@ -2748,8 +3058,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
   const uint64_t raw3 = (uint64_t)kNumInsts * kNumInsts * kNumInsts;
   HIPCHK(h, hipSetDevice(h->device));
   auto wall0 = std::chrono::steady_clock::now();
-  const double qc_ms0 = h->k_qc_ms, exh_ms0 = h->k_exh_ms;
-  const uint64_t qc_l0 = h->k_qc_launches, exh_l0 = h->k_exh_launches;
+  const double qc_ms0 = h->k_qc_ms, exh_ms0 = h->k_exh_ms, grp_ms0 = h->k_grp_ms;
+  const uint64_t qc_l0 = h->k_qc_launches, exh_l0 = h->k_exh_launches, grp_l0 = h->k_grp_launches;
   sx_stats st;
   memset(&st, 0, sizeof(st));
   h->rules.clear();
@@ -3259,7 +3569,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
   uint32_t bad = 0;
   HIPCHK(h, hipMemcpyAsync(&bad, badop, 4, hipMemcpyDeviceToHost, h->stream));
   {
-    int rc = read_evaluated(h, &st.exh_states);  // synchronises the stream
+    int rc = read_evaluated(h, &st.exh_states, &st.exh_group_states, &st.exh_group_jobs);  // synchronises
     if (rc) return rc;
   }
   HIPCHK(h, hipEventSynchronize(es));  // long complete unless there were no targets
@@ -3295,6 +3605,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
   st.ms_exh_kernel = h->k_exh_ms - exh_ms0;
   st.qc_launches = h->k_qc_launches - qc_l0;
   st.exh_launches = h->k_exh_launches - exh_l0;
+  st.ms_exh_group_kernel = h->k_grp_ms - grp_ms0;
+  st.exh_group_launches = h->k_grp_launches - grp_l0;
   st.ms_qc = ms_qc;
   st.ms_mid = ms_mid;
   st.ms_exh = ms_exh;

@@ -70,84 +70,129 @@ SX_HD pk16 pk_ror(PState &s, pk16 v) {
   return r;
 }
 
-// One instruction on K packed pairs (2K states), dispatched once.
-#define SX_PK_EACH(...)                   \
-  {                                       \
-    for (int j_ = 0; j_ < K; j_++) {      \
-      PState &s = st[j_];                 \
-      __VA_ARGS__;                        \
-    }                                     \
-  }                                       \
-  break;
-template <int K>
-SX_HD bool step_pk(PState (&st)[K], uint32_t op, uint32_t imm_word) {
-  const pk16 imm = pk_splat(imm_word & 0xFFu);
-  const pk16 one = pk_splat(1), ff = pk_splat(0xFF);
+// State registers an opcode's step below reads or writes (bit r = register
+// r of enum Reg), for interpreters compiled for a subset of the registers.
+constexpr uint32_t pk_touch(uint32_t op) {
+  constexpr uint32_t A = 1u << RA, X = 1u << RX, Y = 1u << RY, P = 1u << RP, S = 1u << RS, M = 1u << RM,
+                     S0 = 1u << RS0, S1 = 1u << RS1;
   switch (op) {
-  case TAX: SX_PK_EACH(s.x = s.a; pk_set_nz(s, s.x))
-  case TAY: SX_PK_EACH(s.y = s.a; pk_set_nz(s, s.y))
-  case TXA: SX_PK_EACH(s.a = s.x; pk_set_nz(s, s.a))
-  case TYA: SX_PK_EACH(s.a = s.y; pk_set_nz(s, s.a))
-  case TSX: SX_PK_EACH(s.x = s.s; pk_set_nz(s, s.x))
-  case TXS: SX_PK_EACH(s.s = s.x)
-  case INX: SX_PK_EACH(s.x = (s.x + one) & ff; pk_set_nz(s, s.x))
-  case INY: SX_PK_EACH(s.y = (s.y + one) & ff; pk_set_nz(s, s.y))
-  case DEX: SX_PK_EACH(s.x = (s.x - one) & ff; pk_set_nz(s, s.x))
-  case DEY: SX_PK_EACH(s.y = (s.y - one) & ff; pk_set_nz(s, s.y))
-  case ASL_A: SX_PK_EACH(s.a = pk_asl(s, s.a))
-  case LSR_A: SX_PK_EACH(s.a = pk_lsr(s, s.a))
-  case ROL_A: SX_PK_EACH(s.a = pk_rol(s, s.a))
-  case ROR_A: SX_PK_EACH(s.a = pk_ror(s, s.a))
-  case CLC: SX_PK_EACH(s.p &= pk_splat(0xFE))
-  case SEC: SX_PK_EACH(s.p |= pk_splat(0x01))
-  case CLD: SX_PK_EACH(s.p &= pk_splat(0xF7))
-  case SED: SX_PK_EACH(s.p |= pk_splat(0x08))
-  case CLI: SX_PK_EACH(s.p &= pk_splat(0xFB))
-  case SEI: SX_PK_EACH(s.p |= pk_splat(0x04))
-  case CLV: SX_PK_EACH(s.p &= pk_splat(0xBF))
-  case PHA: SX_PK_EACH(s.s1 = s.s0; s.s0 = s.a; s.s = (s.s - one) & ff)
-  case PLA: SX_PK_EACH(s.a = s.s0; s.s0 = s.s1; s.s1 = pk_splat(0); s.s = (s.s + one) & ff; pk_set_nz(s, s.a))
-  case PHP: SX_PK_EACH(s.s1 = s.s0; s.s0 = s.p | pk_splat(0x30); s.s = (s.s - one) & ff)
-  case PLP: SX_PK_EACH(s.p = s.s0; s.s0 = s.s1; s.s1 = pk_splat(0); s.s = (s.s + one) & ff)
-  case NOP: SX_PK_EACH()
-  case LDA_IMM: SX_PK_EACH(s.a = imm; pk_set_nz(s, s.a))
-  case LDX_IMM: SX_PK_EACH(s.x = imm; pk_set_nz(s, s.x))
-  case LDY_IMM: SX_PK_EACH(s.y = imm; pk_set_nz(s, s.y))
-  case ADC_IMM: SX_PK_EACH(pk_adc(s, imm))
-  case SBC_IMM: SX_PK_EACH(pk_sbc(s, imm))
-  case AND_IMM: SX_PK_EACH(s.a &= imm; pk_set_nz(s, s.a))
-  case ORA_IMM: SX_PK_EACH(s.a |= imm; pk_set_nz(s, s.a))
-  case EOR_IMM: SX_PK_EACH(s.a ^= imm; pk_set_nz(s, s.a))
-  case CMP_IMM: SX_PK_EACH(pk_cmp(s, s.a, imm))
-  case CPX_IMM: SX_PK_EACH(pk_cmp(s, s.x, imm))
-  case CPY_IMM: SX_PK_EACH(pk_cmp(s, s.y, imm))
-  case LDA_M: SX_PK_EACH(s.a = s.m; pk_set_nz(s, s.a))
-  case LDX_M: SX_PK_EACH(s.x = s.m; pk_set_nz(s, s.x))
-  case LDY_M: SX_PK_EACH(s.y = s.m; pk_set_nz(s, s.y))
-  case STA_M: SX_PK_EACH(s.m = s.a)
-  case STX_M: SX_PK_EACH(s.m = s.x)
-  case STY_M: SX_PK_EACH(s.m = s.y)
-  case ADC_M: SX_PK_EACH(pk_adc(s, s.m))
-  case SBC_M: SX_PK_EACH(pk_sbc(s, s.m))
-  case AND_M: SX_PK_EACH(s.a &= s.m; pk_set_nz(s, s.a))
-  case ORA_M: SX_PK_EACH(s.a |= s.m; pk_set_nz(s, s.a))
-  case EOR_M: SX_PK_EACH(s.a ^= s.m; pk_set_nz(s, s.a))
-  case CMP_M: SX_PK_EACH(pk_cmp(s, s.a, s.m))
-  case CPX_M: SX_PK_EACH(pk_cmp(s, s.x, s.m))
-  case CPY_M: SX_PK_EACH(pk_cmp(s, s.y, s.m))
-  case INC_M: SX_PK_EACH(s.m = (s.m + one) & ff; pk_set_nz(s, s.m))
-  case DEC_M: SX_PK_EACH(s.m = (s.m - one) & ff; pk_set_nz(s, s.m))
-  case ASL_M: SX_PK_EACH(s.m = pk_asl(s, s.m))
-  case LSR_M: SX_PK_EACH(s.m = pk_lsr(s, s.m))
-  case ROL_M: SX_PK_EACH(s.m = pk_rol(s, s.m))
-  case ROR_M: SX_PK_EACH(s.m = pk_ror(s, s.m))
-  case BIT_M:
-    SX_PK_EACH(s.p = (s.p & pk_splat(0x3D)) | (s.m & pk_splat(0xC0)) |
-                     ((((s.a & s.m) - one) >> 7) & pk_splat(2)))
+  case TAX: case TXA: return A | X | P;
+  case TAY: case TYA: return A | Y | P;
+  case TSX: return S | X | P;
+  case TXS: return X | S;
+  case INX: case DEX: return X | P;
+  case INY: case DEY: return Y | P;
+  case ASL_A: case LSR_A: case ROL_A: case ROR_A: return A | P;
+  case CLC: case SEC: case CLD: case SED: case CLI: case SEI: case CLV: return P;
+  case PHA: return A | S | S0 | S1;
+  case PLA: return A | P | S | S0 | S1;
+  case PHP: case PLP: return P | S | S0 | S1;
+  case NOP: return 0;
+  case LDA_IMM: case ADC_IMM: case SBC_IMM: case AND_IMM: case ORA_IMM: case EOR_IMM: case CMP_IMM: return A | P;
+  case LDX_IMM: case CPX_IMM: return X | P;
+  case LDY_IMM: case CPY_IMM: return Y | P;
+  case LDA_M: case ADC_M: case SBC_M: case AND_M: case ORA_M: case EOR_M: case CMP_M: case BIT_M: return A | M | P;
+  case LDX_M: case CPX_M: return X | M | P;
+  case LDY_M: case CPY_M: return Y | M | P;
+  case STA_M: return A | M;
+  case STX_M: return X | M;
+  case STY_M: return Y | M;
+  default: return M | P;  // INC_M DEC_M ASL_M LSR_M ROL_M ROR_M
+  }
+}
+constexpr uint32_t kPkAllRegs = 0xFFu;
+
+// One instruction on K packed pairs (2K states), dispatched once; imm is the
+// immediate in both halves (wave-uniform, or per lane when the lanes run
+// different instances of the same opcodes).  Regs: the registers the caller
+// keeps; opcodes touching others are compiled out (false, as for an unknown
+// opcode), so registers outside Regs stay at their initial values.
+#define SX_PK_EACH(OP, ...)                                   \
+  case OP:                                                    \
+    if constexpr ((pk_touch(OP) & ~Regs) == 0u) {             \
+      for (int j_ = 0; j_ < K; j_++) {                        \
+        PState &s = st[j_];                                   \
+        __VA_ARGS__;                                          \
+      }                                                       \
+    } else {                                                  \
+      return false;                                           \
+    }                                                         \
+    break;
+template <int K, uint32_t Regs = kPkAllRegs>
+SX_HD bool step_pk_imm(PState (&st)[K], uint32_t op, pk16 imm) {
+  const pk16 one = pk_splat(1), ff = pk_splat(0xFF);
+  (void)one;
+  (void)ff;
+  (void)imm;
+  switch (op) {
+  SX_PK_EACH(TAX, s.x = s.a; pk_set_nz(s, s.x))
+  SX_PK_EACH(TAY, s.y = s.a; pk_set_nz(s, s.y))
+  SX_PK_EACH(TXA, s.a = s.x; pk_set_nz(s, s.a))
+  SX_PK_EACH(TYA, s.a = s.y; pk_set_nz(s, s.a))
+  SX_PK_EACH(TSX, s.x = s.s; pk_set_nz(s, s.x))
+  SX_PK_EACH(TXS, s.s = s.x)
+  SX_PK_EACH(INX, s.x = (s.x + one) & ff; pk_set_nz(s, s.x))
+  SX_PK_EACH(INY, s.y = (s.y + one) & ff; pk_set_nz(s, s.y))
+  SX_PK_EACH(DEX, s.x = (s.x - one) & ff; pk_set_nz(s, s.x))
+  SX_PK_EACH(DEY, s.y = (s.y - one) & ff; pk_set_nz(s, s.y))
+  SX_PK_EACH(ASL_A, s.a = pk_asl(s, s.a))
+  SX_PK_EACH(LSR_A, s.a = pk_lsr(s, s.a))
+  SX_PK_EACH(ROL_A, s.a = pk_rol(s, s.a))
+  SX_PK_EACH(ROR_A, s.a = pk_ror(s, s.a))
+  SX_PK_EACH(CLC, s.p &= pk_splat(0xFE))
+  SX_PK_EACH(SEC, s.p |= pk_splat(0x01))
+  SX_PK_EACH(CLD, s.p &= pk_splat(0xF7))
+  SX_PK_EACH(SED, s.p |= pk_splat(0x08))
+  SX_PK_EACH(CLI, s.p &= pk_splat(0xFB))
+  SX_PK_EACH(SEI, s.p |= pk_splat(0x04))
+  SX_PK_EACH(CLV, s.p &= pk_splat(0xBF))
+  SX_PK_EACH(PHA, s.s1 = s.s0; s.s0 = s.a; s.s = (s.s - one) & ff)
+  SX_PK_EACH(PLA, s.a = s.s0; s.s0 = s.s1; s.s1 = pk_splat(0); s.s = (s.s + one) & ff; pk_set_nz(s, s.a))
+  SX_PK_EACH(PHP, s.s1 = s.s0; s.s0 = s.p | pk_splat(0x30); s.s = (s.s - one) & ff)
+  SX_PK_EACH(PLP, s.p = s.s0; s.s0 = s.s1; s.s1 = pk_splat(0); s.s = (s.s + one) & ff)
+  case NOP: break;
+  SX_PK_EACH(LDA_IMM, s.a = imm; pk_set_nz(s, s.a))
+  SX_PK_EACH(LDX_IMM, s.x = imm; pk_set_nz(s, s.x))
+  SX_PK_EACH(LDY_IMM, s.y = imm; pk_set_nz(s, s.y))
+  SX_PK_EACH(ADC_IMM, pk_adc(s, imm))
+  SX_PK_EACH(SBC_IMM, pk_sbc(s, imm))
+  SX_PK_EACH(AND_IMM, s.a &= imm; pk_set_nz(s, s.a))
+  SX_PK_EACH(ORA_IMM, s.a |= imm; pk_set_nz(s, s.a))
+  SX_PK_EACH(EOR_IMM, s.a ^= imm; pk_set_nz(s, s.a))
+  SX_PK_EACH(CMP_IMM, pk_cmp(s, s.a, imm))
+  SX_PK_EACH(CPX_IMM, pk_cmp(s, s.x, imm))
+  SX_PK_EACH(CPY_IMM, pk_cmp(s, s.y, imm))
+  SX_PK_EACH(LDA_M, s.a = s.m; pk_set_nz(s, s.a))
+  SX_PK_EACH(LDX_M, s.x = s.m; pk_set_nz(s, s.x))
+  SX_PK_EACH(LDY_M, s.y = s.m; pk_set_nz(s, s.y))
+  SX_PK_EACH(STA_M, s.m = s.a)
+  SX_PK_EACH(STX_M, s.m = s.x)
+  SX_PK_EACH(STY_M, s.m = s.y)
+  SX_PK_EACH(ADC_M, pk_adc(s, s.m))
+  SX_PK_EACH(SBC_M, pk_sbc(s, s.m))
+  SX_PK_EACH(AND_M, s.a &= s.m; pk_set_nz(s, s.a))
+  SX_PK_EACH(ORA_M, s.a |= s.m; pk_set_nz(s, s.a))
+  SX_PK_EACH(EOR_M, s.a ^= s.m; pk_set_nz(s, s.a))
+  SX_PK_EACH(CMP_M, pk_cmp(s, s.a, s.m))
+  SX_PK_EACH(CPX_M, pk_cmp(s, s.x, s.m))
+  SX_PK_EACH(CPY_M, pk_cmp(s, s.y, s.m))
+  SX_PK_EACH(INC_M, s.m = (s.m + one) & ff; pk_set_nz(s, s.m))
+  SX_PK_EACH(DEC_M, s.m = (s.m - one) & ff; pk_set_nz(s, s.m))
+  SX_PK_EACH(ASL_M, s.m = pk_asl(s, s.m))
+  SX_PK_EACH(LSR_M, s.m = pk_lsr(s, s.m))
+  SX_PK_EACH(ROL_M, s.m = pk_rol(s, s.m))
+  SX_PK_EACH(ROR_M, s.m = pk_ror(s, s.m))
+  SX_PK_EACH(BIT_M, s.p = (s.p & pk_splat(0x3D)) | (s.m & pk_splat(0xC0)) |
+                          ((((s.a & s.m) - one) >> 7) & pk_splat(2)))
   default: return false;
   }
   return true;
 }
 #undef SX_PK_EACH
+
+template <int K>
+SX_HD bool step_pk(PState (&st)[K], uint32_t op, uint32_t imm_word) {
+  return step_pk_imm<K>(st, op, pk_splat(imm_word & 0xFFu));
+}
 
 }  // namespace sx

@@ -63,7 +63,9 @@ class Stats(C.Structure):
                [(n, C.c_double) for n in ("ms_setup", "ms_qc", "ms_mid", "ms_exh", "ms_total",
                                           "ms_qc_kernel", "ms_exh_kernel")] + \
                [(n, C.c_uint64) for n in ("qc_launches", "exh_launches")] + \
-               [(n, C.c_uint32) for n in ("qc_tile", "qc_blocks_per_cu")] + [("empty_rules", C.c_uint64)]
+               [(n, C.c_uint32) for n in ("qc_tile", "qc_blocks_per_cu")] + [("empty_rules", C.c_uint64)] + \
+               [("ms_exh_group_kernel", C.c_double)] + \
+               [(n, C.c_uint64) for n in ("exh_group_launches", "exh_group_jobs", "exh_group_states")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SX_ABI_VERSION 2
+#define SX_ABI_VERSION 3
 
 /* Status codes. */
 #define SX_OK 0
@@ -75,6 +75,13 @@ typedef struct sx_stats {
   uint32_t qc_tile;          /* QuickCheck targets per work item (autotuned unless tile_targets is set) */
   uint32_t qc_blocks_per_cu; /* 0 = grid launch; else persistent launch with this many blocks per CU */
   uint64_t empty_rules;      /* rules of the empty-candidate stream (include_empty) */
+  /* the exhaustive sweep's one-job-per-lane kernel (jobs of length-2 targets
+     against single instructions with small plans); ms_exh_kernel,
+     exh_launches cover the state-parallel kernel, exh_states both */
+  double ms_exh_group_kernel;
+  uint64_t exh_group_launches;
+  uint64_t exh_group_jobs;   /* jobs it swept */
+  uint64_t exh_group_states; /* states it evaluated for jobs still open */
 } sx_stats;
 
 /* Enumeration semantics. */

@@ -23,7 +23,7 @@ def test_every_declared_symbol_is_exported():
     L = core.lib()
     for name in decl:
         assert hasattr(L, name), name
-    assert L.sx_abi_version() == 2
+    assert L.sx_abi_version() == 3
 
 
 def test_open_without_device_fails_cleanly():
