@@ -586,6 +586,123 @@ __global__ __launch_bounds__(256) void k_quickcheck(const uint32_t *__restrict__
   Q.flush_block(lane_id(), s_cnt, &s_base);
 }
 
+// QuickCheck as a hash join.  A pair can pass only if the target's and the
+// candidate's filter hashes (vectors 0-3 under the masked compare) are equal,
+// so instead of comparing every pair's hashes the candidates are sorted by
+// hash once per enumeration, with a directory of 2^dk buckets on the top
+// hash bits (dir[b] = first sorted position of bucket b, dir[2^dk] = nc),
+// and each target reads its bucket.  A wave takes 64 targets; the bucket
+// sizes are prefix-summed across the wave and the wave walks the flattened
+// (target, bucket entry) items 64 at a time, so a target with hundreds of
+// equal-hash candidates spreads over the lanes.  An entry with an equal hash
+// gets the exact 8-vector compare under both masks, as in qc_item, and the
+// survivors leave through the same queue into the same regions: the survivor
+// set is exactly the all-pairs kernel's.
+__global__ __launch_bounds__(256) void k_cand_filter_hash(const uint2 *__restrict__ cfp, uint64_t nc, uint32_t lo_mask,
+                                                          uint32_t *key, uint32_t *val) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nc) return;
+  uint32_t h = 0;
+#pragma unroll
+  for (uint32_t v = 0; v < 4; v++) {
+    const uint2 f = cfp[i * kFpVecs + v];
+    h ^= qc_mix(f.x & lo_mask, f.y, v);
+  }
+  key[i] = h;
+  val[i] = (uint32_t)i;
+}
+__global__ __launch_bounds__(256) void k_hash_dir(const uint32_t *__restrict__ shash, uint32_t nc, uint32_t dk,
+                                                  uint32_t *dir) {
+  const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+  if (b > (1u << dk)) return;
+  if (b == (1u << dk)) {
+    dir[b] = nc;
+    return;
+  }
+  const uint32_t k = b << (32 - dk);  // first hash of the bucket
+  uint32_t lo = 0, hi = nc;           // lower bound of k
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (shash[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  dir[b] = lo;
+}
+
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t x, uint32_t lane) {
+  uint32_t s = x;
+#pragma unroll
+  for (uint32_t o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)s, o);
+    if (lane >= o) s += y;
+  }
+  return s - x;
+}
+
+__global__ __launch_bounds__(256) void k_qc_join(const uint32_t *__restrict__ th, const uint2 *__restrict__ tfp,
+                                                 uint32_t nt, const uint32_t *__restrict__ shash,
+                                                 const uint32_t *__restrict__ sslot, const uint32_t *__restrict__ dir,
+                                                 uint32_t dk, const uint2 *__restrict__ cfp, uint32_t pm_masked,
+                                                 uint2 *__restrict__ surv, uint32_t region_cap,
+                                                 uint32_t *__restrict__ surv_count) {
+  __shared__ uint2 queue[4][kQcQueue];
+  __shared__ uint32_t s_cnt[4], s_base;
+  __shared__ uint32_t s_pre[4][kWave], s_lo[4][kWave], s_h[4][kWave];
+  const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+  QcQueue Q{queue[w], 0, surv + (uint64_t)(blockIdx.x & (kQcRegions - 1)) * region_cap, region_cap,
+            surv_count + (blockIdx.x & (kQcRegions - 1))};
+  const uint32_t t0 = (blockIdx.x * 4 + w) * kWave;  // the wave's first target
+  const uint32_t t = t0 + lane;
+  const bool have = t < nt;
+  const uint32_t h = have ? th[t] : 0u;
+  const uint32_t b = h >> (32 - dk);
+  const uint32_t lo = have ? dir[b] : 0u, c = have ? dir[b + 1] - lo : 0u;
+  const uint32_t pre = wave_excl_scan(c, lane);
+  const uint32_t total = __builtin_amdgcn_readlane(pre + c, kWave - 1);
+  s_pre[w][lane] = pre;
+  s_lo[w][lane] = lo;
+  s_h[w][lane] = h;
+  __builtin_amdgcn_wave_barrier();
+  const uint32_t mlo = 0x00FFFFFFu | (pm_masked << 24);
+  const uint32_t flo = 0x00FFFFFFu | (kPMask << 24);
+  for (uint32_t base = 0; base < total; base += kWave) {
+    const uint32_t i = base + lane;
+    bool okm = false, okf = false;
+    uint32_t tt = 0, slot = 0;
+    if (i < total) {
+      // the owning target: the last lane whose prefix is <= i (empty buckets
+      // share their prefix with the next lane, which then owns the item)
+      uint32_t o = 0;
+#pragma unroll
+      for (uint32_t step = kWave / 2; step >= 1; step >>= 1)
+        if (s_pre[w][o + step] <= i) o += step;
+      const uint32_t e = s_lo[w][o] + (i - s_pre[w][o]);
+      if (shash[e] == s_h[w][o]) {
+        tt = t0 + o;
+        slot = sslot[e];
+        const uint2 *tf = tfp + (uint64_t)tt * kFpVecs;
+        const uint2 *cf = cfp + (uint64_t)slot * kFpVecs;
+        uint32_t acc_lo = 0, acc_hi = 0;
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+          const uint2 x = tf[v], y = cf[v];
+          acc_lo |= x.x ^ y.x;
+          acc_hi |= x.y ^ y.y;
+        }
+        okm = ((acc_lo & mlo) | acc_hi) == 0;
+        okf = ((acc_lo & flo) | acc_hi) == 0;
+      }
+    }
+    const uint64_t hm = ballot(okm);
+    if (hm) {
+      if (okm) Q.q[Q.qn + mbcnt(hm)] = make_uint2(tt, slot | (okf ? 0x80000000u : 0u));
+      Q.qn += (uint32_t)__popcll(hm);
+      if (Q.qn > kQcQueue - kWave) Q.flush(lane);
+    }
+  }
+  Q.flush_block(lane, s_cnt, &s_base);
+}
+
 // Persistent form: a fixed grid pulls work items (candidate block, target
 // tile) from 8 device queues, one per XCD (workgroups are dispatched to the
 // XCDs round-robin, so block b runs on XCD b mod 8).  Queue q owns the target
@@ -2246,14 +2363,18 @@ struct sx_handle {
   // QuickCheck launch configurations chosen by the autotuner, per (target_len, max_cand_len)
   struct QcChoice {
     int tl, ml, big;  // big: segments of >= 65,536 targets
-    uint32_t tile, blocks_per_cu;  // blocks_per_cu 0 = grid form
+    uint32_t tile, blocks_per_cu;  // blocks_per_cu 0 = grid form; tile 0 too = the hash join
   };
+  // the hash join's candidate side for the current enumeration (sorted filter
+  // hashes, their slots, the bucket directory on the top qj_dk bits)
+  const uint32_t *qj_hash = nullptr, *qj_slot = nullptr, *qj_dir = nullptr;
+  uint32_t qj_dk = 1;
   std::vector<QcChoice> qc_tuned;
   // cached pruned length-2 target list (Go order)
   uint32_t *d_l2 = nullptr;
   uint64_t n_l2 = 0;
   // buffers reused across calls
-  DevBuf bufs[64];
+  DevBuf bufs[96];
   // device address of the no-alloc scratch array on this handle's device
   // (null when the lookup failed; scratch_rc says why)
   uint8_t *scratch = nullptr;
@@ -2269,9 +2390,9 @@ enum BufId {
   B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR, B_HEADS, B_RULEFLAG, B_RULEOFFS, B_RULES2,
   B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_PLANTAB, B_EFP, B_ERULES, B_L3_LIST, B_L3_HK, B_L3_CNT,
   B_L3_KEYS, B_L3_HEAD, B_L3_NEXT, B_L3_BLOOM, B_L3_RAW, B_L3_C3, B_L3_BEST, B_GKEY, B_GIDX, B_GKEY_S, B_GIDX_S,
-  B_GSORT, B_COUNT
+  B_GSORT, B_QJ_KEY, B_QJ_VAL, B_QJ_HASH, B_QJ_SLOT, B_QJ_DIR, B_COUNT
 };
-static_assert(B_COUNT <= 64, "sx_handle::bufs holds every buffer id");
+static_assert(B_COUNT <= 96, "sx_handle::bufs holds every buffer id");
 
 int set_err(sx_handle *h, int code, const char *fmt, ...) {
   if (h) {
@@ -2530,7 +2651,12 @@ int ensure_l2(sx_handle *h) {
 int launch_qc(sx_handle *h, uint32_t tile, uint32_t blocks_per_cu, const uint32_t *tv0, const uint2 *tfp, uint64_t nt,
               const uint2 *cfp, uint64_t nc, uint32_t pm_masked, uint2 *surv, uint32_t surv_cap,
               uint32_t *counters) {
-  if (blocks_per_cu == 0) {
+  if (blocks_per_cu == 0 && tile == 0) {
+    if (!h->qj_dir) return set_err(h, SX_EINVAL, "QuickCheck join without its candidate table");
+    hipLaunchKernelGGL(k_qc_join, dim3(blocks_for(nt, 256)), dim3(256), 0, h->stream, tv0, tfp, (uint32_t)nt,
+                       h->qj_hash, h->qj_slot, h->qj_dir, h->qj_dk, cfp, pm_masked, surv, surv_cap / kQcRegions,
+                       counters);
+  } else if (blocks_per_cu == 0) {
     dim3 grid(blocks_for(nc, 256), (unsigned)((nt + tile - 1) / tile));
     hipLaunchKernelGGL(k_quickcheck, grid, dim3(256), 0, h->stream, tv0, tfp, (uint32_t)nt, tile, cfp,
                        (uint32_t)nc, pm_masked, surv, surv_cap / kQcRegions, counters);
@@ -2565,21 +2691,28 @@ int tune_qc(sx_handle *h, int tl, int ml, const uint32_t *tv0, const uint2 *tfp,
   struct Cand {
     uint32_t tile, bpc;
   };
-  const Cand cands[] = {{1024, 0}, {2048, 0}, {4096, 0}, {8192, 0}, {2048, 2}, {2048, 4},
-                        {2048, 8}, {4096, 2}, {4096, 4}, {4096, 8}, {8192, 4}};
+  // {0, 0}: the hash join (over all nc candidates; the others take the first
+  // nc_t), compared per pair
+  const Cand cands[] = {{0, 0},    {1024, 0}, {2048, 0}, {4096, 0}, {8192, 0}, {2048, 2},
+                        {2048, 4}, {2048, 8}, {4096, 2}, {4096, 4}, {4096, 8}, {8192, 4}};
   float best = 1e30f;
   sx_handle::QcChoice choice{tl, ml, big, 2048, 0};
   for (int rep = 0; rep < 2; rep++)  // the first sweep warms up code and caches
     for (const Cand &c : cands) {
+      const bool join = c.tile == 0 && c.bpc == 0;
+      if (join && !h->qj_dir) continue;
       HIPCHK(h, hipEventRecord(h->kev[0], h->stream));
-      int rc = launch_qc(h, c.tile, c.bpc, tv0, tfp, nt_t, cfp, nc_t, pm_masked, surv, surv_cap, counters);
+      int rc = launch_qc(h, c.tile, c.bpc, tv0, tfp, nt_t, cfp, join ? nc : nc_t, pm_masked, surv, surv_cap,
+                         counters);
       if (rc) return rc;
       HIPCHK(h, hipEventRecord(h->kev[1], h->stream));
       rc = sync_wd(h, "QuickCheck autotune");
       if (rc) return rc;
       float ms = 0;
       HIPCHK(h, hipEventElapsedTime(&ms, h->kev[0], h->kev[1]));
-      trace(h, "  autotune QuickCheck tile %u blocks/CU %u: %.3f ms", c.tile, c.bpc, ms);
+      if (join) ms *= (float)nc_t / (float)nc;  // per pair, like the slices of the others
+      trace(h, "  autotune QuickCheck tile %u blocks/CU %u: %.3f ms (per %llu candidates)", c.tile, c.bpc, ms,
+            (unsigned long long)nc_t);
       if (rep == 1 && ms < best) {
         best = ms;
         choice.tile = c.tile;
@@ -3140,6 +3273,34 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
                          cbytes + kNumInsts, ccyc + kNumInsts, cprune + kNumInsts, badop);
     }
     HIPCHK(h, hipGetLastError());
+  }
+  // the QuickCheck hash join's candidate side: the filter hashes under the
+  // masked compare, sorted, with a directory on their top dk bits (about one
+  // candidate per bucket)
+  {
+    uint32_t *key, *val, *shs, *ssl, *dir;
+    uint32_t dk = 1;
+    while (dk < 24 && (1ull << dk) < nc) dk++;
+    ENSURE(B_QJ_KEY, uint32_t, nc, key);
+    ENSURE(B_QJ_VAL, uint32_t, nc, val);
+    ENSURE(B_QJ_HASH, uint32_t, nc, shs);
+    ENSURE(B_QJ_SLOT, uint32_t, nc, ssl);
+    ENSURE(B_QJ_DIR, uint32_t, (1ull << dk) + 1, dir);
+    hipLaunchKernelGGL(k_cand_filter_hash, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cfp, nc, lo_mask, key,
+                       val);
+    HIPCHK(h, hipGetLastError());
+    size_t tmp_bytes = 0;
+    HIPCHK(h, rocprim::radix_sort_pairs(nullptr, tmp_bytes, key, shs, val, ssl, (uint32_t)nc, 0, 32, h->stream));
+    uint8_t *tmp;
+    ENSURE(B_GSORT, uint8_t, tmp_bytes + 1, tmp);
+    HIPCHK(h, rocprim::radix_sort_pairs(tmp, tmp_bytes, key, shs, val, ssl, (uint32_t)nc, 0, 32, h->stream));
+    hipLaunchKernelGGL(k_hash_dir, dim3(blocks_for((1ull << dk) + 1, 256)), dim3(256), 0, h->stream, shs,
+                       (uint32_t)nc, dk, dir);
+    HIPCHK(h, hipGetLastError());
+    h->qj_hash = shs;
+    h->qj_slot = ssl;
+    h->qj_dir = dir;
+    h->qj_dk = dk;
   }
   // the empty candidate's fingerprint: a length-0 sequence's outputs are the
   // vectors themselves (no word is read)

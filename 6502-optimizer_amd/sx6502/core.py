@@ -274,9 +274,11 @@ class Device:
     def enumerate(self, targets: Optional[np.ndarray] = None, *, target_len: int = 2, max_cand_len: int = 1,
                   dead: int = 0, bytes_model: int = 0, hits: int = HITS_FIRST, exh_mode: int = EXH_PROJECTED,
                   use_mid: bool = True, first: int = 0, end: int = 0, tile: int = 0, survivor_cap: int = 0,
-                  include_empty: bool = False):
+                  include_empty: bool = False, out: Optional[np.ndarray] = None):
         """sx_enumerate: (rules, stats).  With include_empty the empty
-        candidate's stream is read with fetch_empty_rules()."""
+        candidate's stream is read with fetch_empty_rules().  out: a rule
+        array (RULE_DTYPE) to fetch into when it is large enough (the result
+        is then a view of its first records); else a new array."""
         cfg = EnumCfg()
         cfg.struct_size = C.sizeof(EnumCfg)
         cfg.target_len = target_len
@@ -300,7 +302,9 @@ class Device:
         st = Stats()
         rc = lib().sx_enumerate(self._h, C.byref(cfg), None, 0, C.byref(n), C.byref(st))
         self._check(rc, "sx_enumerate")
-        out = np.empty(n.value, RULE_DTYPE)  # sx_fetch_rules writes every byte of every record it returns
+        if out is None or out.dtype != RULE_DTYPE or out.ndim != 1 or len(out) < n.value or \
+                not out.flags.c_contiguous or not out.flags.writeable:
+            out = np.empty(n.value, RULE_DTYPE)  # sx_fetch_rules writes every byte of every record it returns
         got = C.c_uint64(0)
         rc = lib().sx_fetch_rules(self._h, out.ctypes.data_as(C.c_void_p), 0, n.value, C.byref(got))
         self._check(rc, "sx_fetch_rules")
