@@ -97,6 +97,12 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
 }
 
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 __device__ __forceinline__ void run_seq_uniform(State &st, const uint32_t *w, int n) {
   for (int i = 0; i < n; i++) {
     uint32_t word = uni(w[i]);
@@ -841,6 +847,255 @@ __global__ __launch_bounds__(kBlk) void k_midcheck(const uint2 *__restrict__ sur
   if (cur0) wave_run_min(pass_ok ? t * 2 + (pass - 1) : kRankNone, pass_ok ? rank[slot] : 0u, cur0);
 }
 
+// QuickCheck and MidCheck as one join, with QuickCheck's counters from
+// candidate classes (MidCheck on, either semantics).
+//
+// A pair survives MidCheck only if its outputs agree on the 8 QuickCheck
+// vectors under the masked compare and its masked Mid hashes agree (the exact
+// pass's tighter compares imply both).  So a join on jkey = f(hash of the 8
+// masked QuickCheck outputs, masked Mid hash) reaches every MidCheck
+// survivor, and each pair it reaches gets the exact QuickCheck compare and
+// MidCheck's test as k_midcheck applies them: the survivor list is the same
+// set, without the QuickCheck survivors (483M at --dead-flags 0xFF) ever
+// being written.  QuickCheck's counters count pairs that pass QuickCheck
+// whether or not they pass MidCheck; they come from classes of candidates
+// with equal masked 8-vector outputs (sorted by a 64-bit key of those
+// outputs; every member is compared with the class's first one, and a
+// mismatch sends the enumeration back to the separate kernels): class c
+// holds its size and its unpruned members per byte size, and a target whose
+// outputs equal c's adds c's size and its unpruned members below the
+// target's own size.
+constexpr uint32_t kClsBins = 16;  // byte sizes 0..15 per class, then the class size
+__device__ __forceinline__ void qc_keys(const uint2 (&o)[8], uint32_t lo_mask, uint32_t &k1, uint32_t &k2) {
+  k1 = 0;
+  k2 = 0;
+#pragma unroll
+  for (uint32_t v = 0; v < 8; v++) {
+    const uint32_t x = o[v].x & lo_mask, y = o[v].y;
+    k1 ^= qc_mix(x, y, v);
+    k2 ^= fmix32(x * 0x9E3779B1u + fmix32(y ^ (v * 0x632BE5ABu + 0x7F4A7C15u)));
+  }
+}
+__device__ __forceinline__ uint32_t join_key(uint32_t k1, uint32_t mh_masked) {
+  return fmix32(k1 ^ fmix32(mh_masked + 0x9E3779B9u));
+}
+__device__ __forceinline__ void load_fp8(const uint2 *__restrict__ fp, uint64_t i, uint2 (&o)[8]) {
+  const uint4 *f = reinterpret_cast<const uint4 *>(fp + i * kFpVecs);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint4 q = f[k];
+    o[2 * k] = make_uint2(q.x, q.y);
+    o[2 * k + 1] = make_uint2(q.z, q.w);
+  }
+}
+// differences of two outputs under the masked (okm) and exact (okf) compare
+__device__ __forceinline__ void fp_compare(const uint2 (&a)[8], const uint2 (&b)[8], uint32_t mlo, bool &okm,
+                                           bool &okf) {
+  uint32_t acc_lo = 0, acc_hi = 0;
+#pragma unroll
+  for (int v = 0; v < 8; v++) {
+    acc_lo |= a[v].x ^ b[v].x;
+    acc_hi |= a[v].y ^ b[v].y;
+  }
+  okm = ((acc_lo & mlo) | acc_hi) == 0;
+  okf = ((acc_lo & (0x00FFFFFFu | (kPMask << 24))) | acc_hi) == 0;
+}
+
+__global__ __launch_bounds__(256) void k_cand_keys(const uint2 *__restrict__ cfp, const uint2 *__restrict__ cmh,
+                                                   uint64_t nc, uint32_t lo_mask, unsigned long long *ckey,
+                                                   uint32_t *jkey, uint32_t *val1, uint32_t *val2) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nc) return;
+  uint2 o[8];
+  load_fp8(cfp, i, o);
+  uint32_t k1, k2;
+  qc_keys(o, lo_mask, k1, k2);
+  ckey[i] = (unsigned long long)k1 | ((unsigned long long)k2 << 32);
+  jkey[i] = join_key(k1, cmh[i].y);
+  val1[i] = (uint32_t)i;
+  val2[i] = (uint32_t)i;
+}
+__global__ __launch_bounds__(256) void k_class_heads(const unsigned long long *__restrict__ sk, uint32_t n,
+                                                     uint32_t *head) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) head[i] = (i == 0 || sk[i] != sk[i - 1]) ? 1u : 0u;
+}
+// class ids from the heads' exclusive scan; class tables zeroed beforehand
+__global__ __launch_bounds__(256) void k_class_fill(const unsigned long long *__restrict__ sk,
+                                                    const uint32_t *__restrict__ sslot, uint32_t n,
+                                                    const uint32_t *__restrict__ head, const uint64_t *__restrict__ hoff,
+                                                    const uint8_t *__restrict__ cbytes,
+                                                    const uint8_t *__restrict__ cprune, unsigned long long *ckey,
+                                                    uint32_t *crep, uint32_t *ccnt) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t cid = (uint32_t)hoff[i] + head[i] - 1u, slot = sslot[i];
+  if (head[i]) {
+    ckey[cid] = sk[i];
+    crep[cid] = slot;
+  }
+  atomicAdd(&ccnt[cid * (kClsBins + 1) + kClsBins], 1u);
+  if (!cprune[slot]) atomicAdd(&ccnt[cid * (kClsBins + 1) + min((uint32_t)cbytes[slot], kClsBins - 1)], 1u);
+}
+__global__ __launch_bounds__(256) void k_class_verify(const uint32_t *__restrict__ sslot, uint32_t n,
+                                                      const uint32_t *__restrict__ head,
+                                                      const uint64_t *__restrict__ hoff,
+                                                      const uint32_t *__restrict__ crep, const uint2 *__restrict__ cfp,
+                                                      uint32_t lo_mask, uint32_t *bad) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t cid = (uint32_t)hoff[i] + head[i] - 1u;
+  uint2 a[8], b[8];
+  load_fp8(cfp, sslot[i], a);
+  load_fp8(cfp, crep[cid], b);
+  bool okm, okf;
+  fp_compare(a, b, lo_mask, okm, okf);
+  if (!okm) atomicOr(bad, 1u);
+}
+
+constexpr uint32_t kQcCntShards = 64;
+// QuickCheck's counters of a segment from the classes: sums[2 s] += raw
+// survivors, sums[2 s + 1] += survivors after the byte filter and the pruner
+// (shard s = block mod 64).
+__global__ __launch_bounds__(256) void k_qc_count(const uint2 *__restrict__ tfp, const uint8_t *__restrict__ tbytes,
+                                                  uint32_t nt, uint32_t lo_mask,
+                                                  const unsigned long long *__restrict__ ckey,
+                                                  const uint64_t *__restrict__ ncls_p,
+                                                  const uint32_t *__restrict__ crep, const uint32_t *__restrict__ ccnt,
+                                                  const uint2 *__restrict__ cfp, unsigned long long *sums) {
+  __shared__ unsigned long long s_raw[4], s_hit[4];
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x, lane = lane_id(), w = threadIdx.x >> 6;
+  uint64_t raw = 0, hit = 0;
+  if (t < nt) {
+    uint2 o[8];
+    load_fp8(tfp, t, o);
+    uint32_t k1, k2;
+    qc_keys(o, lo_mask, k1, k2);
+    const unsigned long long k = (unsigned long long)k1 | ((unsigned long long)k2 << 32);
+    const uint32_t ncls = (uint32_t)*ncls_p;
+    uint32_t lo = 0, hi = ncls;  // lower bound of k
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (ckey[mid] < k) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo < ncls && ckey[lo] == k) {
+      uint2 r[8];
+      load_fp8(cfp, crep[lo], r);
+      bool okm, okf;
+      fp_compare(o, r, lo_mask, okm, okf);
+      if (okm) {
+        const uint32_t *c = ccnt + (uint64_t)lo * (kClsBins + 1);
+        raw = c[kClsBins];
+        const uint32_t tb = min((uint32_t)tbytes[t], kClsBins);
+        for (uint32_t b = 0; b < tb; b++) hit += c[b];
+      }
+    }
+  }
+  raw = wave_sum64(raw);
+  hit = wave_sum64(hit);
+  if (lane == 0) {
+    s_raw[w] = raw;
+    s_hit[w] = hit;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint64_t r = s_raw[0] + s_raw[1] + s_raw[2] + s_raw[3], hh = s_hit[0] + s_hit[1] + s_hit[2] + s_hit[3];
+    const uint32_t sh = blockIdx.x & (kQcCntShards - 1);
+    if (r) atomicAdd(&sums[2 * sh], (unsigned long long)r);
+    if (hh) atomicAdd(&sums[2 * sh + 1], (unsigned long long)hh);
+  }
+}
+
+// The join: a wave takes 64 targets, reads each one's bucket of the
+// candidates' sorted join keys and walks the prefix-summed items (as
+// k_qc_join); an entry with an equal key gets QuickCheck's exact compare and
+// MidCheck's test.  Survivors (target, slot | pass << 30) leave through a
+// per-wave LDS queue, reserved with one atomic on counters[1] per flush (the
+// MidCheck count); entries past out_cap are counted, not written, and the
+// caller re-runs the segment with a larger buffer.  cur0 (first hit): the
+// first round's minimum, as k_midcheck leaves it.
+__global__ __launch_bounds__(256) void k_qcm_join(const uint2 *__restrict__ tfp, const uint2 *__restrict__ tmh,
+                                                  const uint8_t *__restrict__ tbytes, uint32_t nt, uint32_t lo_mask,
+                                                  const uint32_t *__restrict__ sj, const uint32_t *__restrict__ sjslot,
+                                                  const uint32_t *__restrict__ dir, uint32_t dk,
+                                                  const uint2 *__restrict__ cfp, const uint2 *__restrict__ cmh,
+                                                  const uint8_t *__restrict__ cbytes,
+                                                  const uint8_t *__restrict__ cprune, int hits_all,
+                                                  uint2 *__restrict__ out, uint32_t out_cap,
+                                                  uint32_t *__restrict__ counters, const uint32_t *__restrict__ rank,
+                                                  uint32_t *cur0) {
+  __shared__ uint2 queue[4][kQcQueue];
+  __shared__ uint32_t s_pre[4][kWave], s_lo[4][kWave], s_key[4][kWave];
+  const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+  uint2 *q = queue[w];
+  uint32_t qn = 0;
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&counters[1], qn);
+    base = uni(base);
+    for (uint32_t i = lane; i < qn; i += kWave)
+      if (base + i < out_cap) out[base + i] = q[i];
+    __builtin_amdgcn_wave_barrier();
+    qn = 0;
+  };
+  const uint32_t t0 = (blockIdx.x * 4 + w) * kWave;
+  const uint32_t t = t0 + lane;
+  const bool have = t < nt;
+  uint32_t jk = 0;
+  if (have) {
+    uint2 o[8];
+    load_fp8(tfp, t, o);
+    uint32_t k1, k2;
+    qc_keys(o, lo_mask, k1, k2);
+    jk = join_key(k1, tmh[t].y);
+  }
+  const uint32_t b = jk >> (32 - dk);
+  const uint32_t lo = have ? dir[b] : 0u, c = have ? dir[b + 1] - lo : 0u;
+  const uint32_t pre = wave_excl_scan(c, lane);
+  const uint32_t total = __builtin_amdgcn_readlane(pre + c, kWave - 1);
+  s_pre[w][lane] = pre;
+  s_lo[w][lane] = lo;
+  s_key[w][lane] = jk;
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t base = 0; base < total; base += kWave) {
+    const uint32_t i = base + lane;
+    bool ok = false;
+    uint32_t tt = 0, slot = 0, pass = 1;
+    if (i < total) {
+      uint32_t o = 0;
+#pragma unroll
+      for (uint32_t step = kWave / 2; step >= 1; step >>= 1)
+        if (s_pre[w][o + step] <= i) o += step;
+      const uint32_t e = s_lo[w][o] + (i - s_pre[w][o]);
+      if (sj[e] == s_key[w][o]) {
+        tt = t0 + o;
+        slot = sjslot[e];
+        uint2 a[8], bb[8];
+        load_fp8(tfp, tt, a);
+        load_fp8(cfp, slot, bb);
+        bool okm, okf;
+        fp_compare(a, bb, lo_mask, okm, okf);
+        const bool qc_hit = okm && cbytes[slot] < tbytes[tt] && cprune[slot] == 0;
+        const bool exact = !hits_all && okf;  // pass 1 compares P under 0xCF
+        pass = (hits_all || okf) ? 1u : 2u;
+        const uint2 ma = tmh[tt], mb = cmh[slot];
+        ok = qc_hit && (exact ? ma.x == mb.x : ma.y == mb.y);
+      }
+    }
+    const uint64_t m = ballot(ok);
+    if (m) {
+      if (ok) q[qn + mbcnt(m)] = make_uint2(tt, slot | (pass << 30));
+      qn += (uint32_t)__popcll(m);
+      if (qn > kQcQueue - kWave) flush();
+    }
+    if (cur0) wave_run_min(ok ? tt * 2 + (pass - 1) : kRankNone, ok ? rank[slot] : 0u, cur0);
+  }
+  if (qn) flush();
+}
+
 // A segment's first-hit state, cleared in one launch: cur = none, rej = 0,
 // done = 0 and, when given, the rule slots' flags = 0.
 __global__ void k_seg_clear(uint32_t *cur, uint32_t *rej, uint8_t *done, uint32_t *rule_flag, uint64_t n) {
@@ -1515,12 +1770,6 @@ __global__ __launch_bounds__(256) void k_group_keys(const Job *__restrict__ jobs
   key[i] = ok ? ((t0 * OP_COUNT + t1) * OP_COUNT + c) * 2u + (pm != kPMask ? 1u : 0u) : kGroupSort;
   idx[i] = i;
   if (ok) chunks[i] = 0;
-}
-
-__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-  return v;
 }
 
 // The items of one key group (lanes gm of the wave) that wave u of the
@@ -2323,6 +2572,8 @@ struct sx_handle {
   int bulk_after = 1;     // first-hit rounds before the one-batch tail (SX_BULK_AFTER); -1 = by survivor count
   int exh_k = 2;          // states (pairs when packed) per lane and opcode dispatch in the sweep (SX_EXH_K: 1 or 2)
   int exh_pk = 1;         // packed two-states-per-register interpreter in the sweep (SX_EXH_PK=0: one state)
+  int qcm_join = 1;       // QuickCheck and MidCheck as one join when MidCheck is on (SX_QCM_JOIN=0: separate)
+  uint32_t mid_cap_hint = 0;  // MidCheck survivor buffer size that sufficed before
   int exh_group = 4;      // small plans swept one job per lane, this many pairs per lane and dispatch
                           // (SX_EXH_GROUP: 1, 2, 4 or 8; 0: every job state-parallel)
   // the plan table in B_PLANTAB holds the plans of (sweep mode, relaxed mask)
@@ -2390,7 +2641,8 @@ enum BufId {
   B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR, B_HEADS, B_RULEFLAG, B_RULEOFFS, B_RULES2,
   B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_PLANTAB, B_EFP, B_ERULES, B_L3_LIST, B_L3_HK, B_L3_CNT,
   B_L3_KEYS, B_L3_HEAD, B_L3_NEXT, B_L3_BLOOM, B_L3_RAW, B_L3_C3, B_L3_BEST, B_GKEY, B_GIDX, B_GKEY_S, B_GIDX_S,
-  B_GSORT, B_QJ_KEY, B_QJ_VAL, B_QJ_HASH, B_QJ_SLOT, B_QJ_DIR, B_COUNT
+  B_GSORT, B_QJ_KEY, B_QJ_VAL, B_QJ_HASH, B_QJ_SLOT, B_QJ_DIR, B_CK, B_CKS, B_CV1,
+  B_CV1S, B_JK, B_JV, B_JKS, B_JVS, B_JDIR, B_CHEAD, B_CHOFF, B_CLSKEY, B_CLSREP, B_CLSCNT, B_CLSN, B_QCSUM, B_COUNT
 };
 static_assert(B_COUNT <= 96, "sx_handle::bufs holds every buffer id");
 
@@ -3101,6 +3353,7 @@ extern "C" int sx_open(int device, sx_handle **out) {
   if (const char *e = getenv("SX_EXH_K")) h->exh_k = atoi(e) == 1 ? 1 : 2;
   if (const char *e = getenv("SX_EXH_PK")) h->exh_pk = atoi(e) != 0;
   if (const char *e = getenv("SX_EXH_GROUP")) h->exh_group = atoi(e);
+  if (const char *e = getenv("SX_QCM_JOIN")) h->qcm_join = atoi(e) != 0;
   if (const char *e = getenv("SX_SEG_LOG")) h->seg_log = std::min(24, std::max(16, atoi(e)));
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking) != hipSuccess ||
@@ -3302,6 +3555,70 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     h->qj_dir = dir;
     h->qj_dk = dk;
   }
+  // QuickCheck and MidCheck as one join (MidCheck on): the candidates' join
+  // keys sorted with a directory, and their classes of equal masked
+  // QuickCheck outputs (keys, first members, counts); a class whose members
+  // differ (a 64-bit key collision) sends this enumeration to the separate
+  // kernels
+  // (a caller's QuickCheck tile asks for the all-pairs grid kernel)
+  bool fused = cfg->use_mid && h->qcm_join && cfg->tile_targets == 0;
+  unsigned long long *cls_key = nullptr;
+  uint64_t *cls_n = nullptr;
+  uint32_t *cls_rep = nullptr, *cls_cnt = nullptr, *j_key = nullptr, *j_slot = nullptr, *j_dir = nullptr, j_dk = 1;
+  if (fused) {
+    while (j_dk < 24 && (1ull << j_dk) < nc) j_dk++;
+    unsigned long long *ck, *cks;
+    uint32_t *v1, *v1s, *jk, *v2, *head, *bad;
+    uint64_t *hoff;
+    ENSURE(B_CK, unsigned long long, nc, ck);
+    ENSURE(B_CKS, unsigned long long, nc, cks);
+    ENSURE(B_CV1, uint32_t, nc, v1);
+    ENSURE(B_CV1S, uint32_t, nc, v1s);
+    ENSURE(B_JK, uint32_t, nc, jk);
+    ENSURE(B_JV, uint32_t, nc, v2);
+    ENSURE(B_JKS, uint32_t, nc, j_key);
+    ENSURE(B_JVS, uint32_t, nc, j_slot);
+    ENSURE(B_JDIR, uint32_t, (1ull << j_dk) + 1, j_dir);
+    ENSURE(B_CHEAD, uint32_t, nc, head);
+    ENSURE(B_CHOFF, uint64_t, nc + 1, hoff);
+    ENSURE(B_CLSKEY, unsigned long long, nc, cls_key);
+    ENSURE(B_CLSREP, uint32_t, nc, cls_rep);
+    ENSURE(B_CLSCNT, uint32_t, nc * (kClsBins + 1), cls_cnt);
+    ENSURE(B_CLSN, uint64_t, 2, cls_n);
+    bad = reinterpret_cast<uint32_t *>(cls_n + 1);
+    const uint32_t n32 = (uint32_t)nc;
+    hipLaunchKernelGGL(k_cand_keys, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cfp, cmh, nc, lo_mask, ck, jk,
+                       v1, v2);
+    HIPCHK(h, hipGetLastError());
+    size_t tb1 = 0, tb2 = 0;
+    HIPCHK(h, rocprim::radix_sort_pairs(nullptr, tb1, ck, cks, v1, v1s, n32, 0, 64, h->stream));
+    HIPCHK(h, rocprim::radix_sort_pairs(nullptr, tb2, jk, j_key, v2, j_slot, n32, 0, 32, h->stream));
+    uint8_t *tmp;
+    ENSURE(B_GSORT, uint8_t, std::max(tb1, tb2) + 1, tmp);
+    HIPCHK(h, rocprim::radix_sort_pairs(tmp, tb1, ck, cks, v1, v1s, n32, 0, 64, h->stream));
+    HIPCHK(h, rocprim::radix_sort_pairs(tmp, tb2, jk, j_key, v2, j_slot, n32, 0, 32, h->stream));
+    hipLaunchKernelGGL(k_hash_dir, dim3(blocks_for((1ull << j_dk) + 1, 256)), dim3(256), 0, h->stream, j_key, n32, j_dk,
+                       j_dir);
+    hipLaunchKernelGGL(k_class_heads, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cks, n32, head);
+    HIPCHK(h, hipGetLastError());
+    int rc = scan_u32(h, head, nc, hoff, cls_n);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nc * (kClsBins + 1), 256)), dim3(256), 0, h->stream, cls_cnt,
+                       nc * (kClsBins + 1), 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, h->stream, bad, (uint64_t)1, 0u);
+    hipLaunchKernelGGL(k_class_fill, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cks, v1s, n32, head, hoff,
+                       cbytes, cprune, cls_key, cls_rep, cls_cnt);
+    hipLaunchKernelGGL(k_class_verify, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, v1s, n32, head, hoff,
+                       cls_rep, cfp, lo_mask, bad);
+    HIPCHK(h, hipGetLastError());
+    uint32_t bad_h = 0;
+    HIPCHK(h, hipMemcpyAsync(&bad_h, bad, 4, hipMemcpyDeviceToHost, h->stream));
+    SYNC_WD("device sync");
+    if (bad_h) {
+      trace(h, "  candidate classes: a 64-bit key holds two outputs; QuickCheck and MidCheck run separately");
+      fused = false;
+    }
+  }
   // the empty candidate's fingerprint: a length-0 sequence's outputs are the
   // vectors themselves (no word is read)
   uint2 *efp = nullptr;
@@ -3394,63 +3711,6 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
                        nt, cfg->bytes_model, tfp, tmh, tv0, lo_mask, tbytes, tcyc, nullptr, badop);
     HIPCHK(h, hipGetLastError());
 
-    // ---- QuickCheck, re-run with a larger buffer on overflow (no hit is dropped)
-    uint2 *surv;
-    uint32_t *counters;  // [0, 8): MidCheck and the rounds; [8, 16): QuickCheck's region counts
-    ENSURE(B_COUNTERS, uint32_t, 8 + kQcRegions, counters);
-    uint32_t *qc_counts = counters + 8;
-    uint32_t nsurv = 0;
-    SurvRegions regions;
-    for (;;) {
-      ENSURE(B_SURV, uint2, surv_cap, surv);
-      if (!qc_tuned) {
-        int rc = tune_qc(h, tl, ml, tv0, tfp, nt, cfp, nc, pm_masked, surv, surv_cap, qc_counts, &qcc);
-        if (rc) return rc;
-        qc_tuned = true;
-      }
-      HIPCHK(h, hipMemsetAsync(qc_counts, 0, kQcRegions * 4, h->stream));
-      {
-        size_t ts = 0;
-        int rc = timed_begin(h, 0, &ts);
-        if (rc) return rc;
-        rc = launch_qc(h, qcc.tile, qcc.blocks_per_cu, tv0, tfp, nt, cfp, nc, pm_masked, surv, surv_cap,
-                       qc_counts);
-        if (rc) return rc;
-        rc = timed_end(h, ts);
-        if (rc) return rc;
-      }
-      uint32_t cnt8[kQcRegions];
-      HIPCHK(h, hipMemcpyAsync(cnt8, qc_counts, sizeof(cnt8), hipMemcpyDeviceToHost, h->stream));
-      SYNC_WD("device sync");
-      const uint32_t region_cap = surv_cap / kQcRegions;
-      uint64_t total = 0;
-      uint32_t most = 0;
-      regions.cap = region_cap;
-      for (uint32_t r = 0; r < kQcRegions; r++) {
-        regions.pre[r] = (uint32_t)std::min<uint64_t>(total, 0xFFFFFFFFu);
-        total += cnt8[r];
-        most = std::max(most, cnt8[r]);
-      }
-      regions.pre[kQcRegions] = (uint32_t)std::min<uint64_t>(total, 0xFFFFFFFFu);
-      trace(h, "segment %llu: %llu targets, QuickCheck survivors %llu (largest region %u of %u)",
-            (unsigned long long)s0, (unsigned long long)nt, (unsigned long long)total, most, region_cap);
-      if (most <= region_cap) {
-        nsurv = (uint32_t)total;
-        break;
-      }
-      st.overflow_retries++;
-      if ((uint64_t)most * 5 / 4 + 1024 > 0xFFFFFFFFull / kQcRegions)
-        return set_err(h, SX_ENOMEM, "QuickCheck survivor buffer (%u in one region)", most);
-      surv_cap = kQcRegions * (most + most / 4 + 1024);
-      if (!cfg->survivor_cap) h->surv_cap_hint = std::max(h->surv_cap_hint, surv_cap);
-    }
-    HIPCHK(h, hipEventRecord(e2, h->stream));
-    st.qc_raw_hits += nsurv;
-    st.qc_pairs += nt * (uint64_t)(ml == 1 ? kNumInsts : nc);
-
-    // ---- MidCheck
-    uint2 *mid;
-    uint32_t nmid = 0;
     // the rounds' per-(target, pass) state, cleared while the host waits for
     // the MidCheck count
     uint32_t *cur, *rej, *rule_flag;  // rule_flag (first hit): slot k = target * 2 + pass - 1 holds a rule
@@ -3459,24 +3719,145 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     ENSURE(B_REJ, uint32_t, nt * 2, rej);
     ENSURE(B_DONE, uint8_t, nt * 2, done);
     ENSURE(B_RULEFLAG, uint32_t, nt * 2, rule_flag);
-    for (;;) {
-      ENSURE(B_MID, uint2, std::max<uint32_t>(nsurv, 1), mid);
-      HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));  // counters[2..7] stay 0 for the rounds
-      hipLaunchKernelGGL(k_seg_clear, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, rej, done,
-                         cfg->hits ? nullptr : rule_flag, nt * 2);
-      // first hit: MidCheck also leaves the first round's minimum per (target, pass) in cur
-      if (nsurv)
-        hipLaunchKernelGGL(k_midcheck, dim3(blocks_for(nsurv, kBlk)), dim3(kBlk), 0, h->stream, surv, regions,
-                           nsurv, tmh, tbytes, cmh, cbytes, cprune, cfg->hits, cfg->use_mid, mid,
-                           std::max<uint32_t>(nsurv, 1), counters, rank, cfg->hits ? nullptr : cur);
-      HIPCHK(h, hipGetLastError());
-      uint32_t cnt[2];
-      HIPCHK(h, hipMemcpyAsync(cnt, counters, 8, hipMemcpyDeviceToHost, h->stream));
-      SYNC_WD("device sync");
-      st.qc_hits += cnt[0];
-      nmid = cnt[1];
-      trace(h, "  MidCheck: %u QC hits after filters, %u pass", cnt[0], cnt[1]);
-      break;
+    uint2 *mid;
+    uint32_t nmid = 0;
+    uint32_t *counters;  // [0, 8): MidCheck and the rounds; [8, 16): QuickCheck's region counts
+    ENSURE(B_COUNTERS, uint32_t, 8 + kQcRegions, counters);
+    if (fused) {
+      // ---- QuickCheck's counters from the classes, then QuickCheck and
+      // MidCheck as one join (re-run with a larger buffer on overflow)
+      unsigned long long *qsums;
+      ENSURE(B_QCSUM, unsigned long long, 2 * kQcCntShards, qsums);
+      HIPCHK(h, hipMemsetAsync(qsums, 0, 2 * kQcCntShards * 8, h->stream));
+      {
+        size_t ts = 0;
+        int rc = timed_begin(h, 0, &ts);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_qc_count, dim3(blocks_for(nt, 256)), dim3(256), 0, h->stream, tfp, tbytes, (uint32_t)nt,
+                           lo_mask, cls_key, cls_n, cls_rep, cls_cnt, cfp, qsums);
+        HIPCHK(h, hipGetLastError());
+        rc = timed_end(h, ts);
+        if (rc) return rc;
+      }
+      HIPCHK(h, hipEventRecord(e2, h->stream));
+      // the caller's survivor buffer size, else what sufficed before, else 2^22
+      uint32_t mid_cap = cfg->survivor_cap ? cfg->survivor_cap : std::max<uint32_t>(h->mid_cap_hint, 1u << 22);
+      for (;;) {
+        ENSURE(B_MID, uint2, mid_cap, mid);
+        HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));  // counters[2..7] stay 0 for the rounds
+        hipLaunchKernelGGL(k_seg_clear, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, rej, done,
+                           cfg->hits ? nullptr : rule_flag, nt * 2);
+        size_t ts = 0;
+        int rc = timed_begin(h, 0, &ts);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_qcm_join, dim3(blocks_for(nt, 256)), dim3(256), 0, h->stream, tfp, tmh, tbytes,
+                           (uint32_t)nt, lo_mask, j_key, j_slot, j_dir, j_dk, cfp, cmh, cbytes, cprune, cfg->hits, mid,
+                           mid_cap, counters, rank, cfg->hits ? nullptr : cur);
+        HIPCHK(h, hipGetLastError());
+        rc = timed_end(h, ts);
+        if (rc) return rc;
+        struct {
+          uint32_t c[2];
+          unsigned long long q[2 * kQcCntShards];
+        } rb;
+        HIPCHK(h, hipMemcpyAsync(rb.c, counters, 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(rb.q, qsums, sizeof(rb.q), hipMemcpyDeviceToHost, h->stream));
+        SYNC_WD("device sync");
+        nmid = rb.c[1];
+        if (nmid > mid_cap) {
+          st.overflow_retries++;
+          if ((uint64_t)nmid * 5 / 4 + 1024 > 0xFFFFFFFFull)
+            return set_err(h, SX_ENOMEM, "MidCheck survivor buffer (%u)", nmid);
+          mid_cap = nmid + nmid / 4 + 1024;
+          if (!cfg->survivor_cap) h->mid_cap_hint = std::max(h->mid_cap_hint, mid_cap);
+          continue;
+        }
+        uint64_t raw = 0, hits = 0;
+        for (uint32_t k = 0; k < kQcCntShards; k++) {
+          raw += rb.q[2 * k];
+          hits += rb.q[2 * k + 1];
+        }
+        st.qc_raw_hits += raw;
+        st.qc_hits += hits;
+        trace(h, "segment %llu: %llu targets, QuickCheck survivors %llu (%llu after filters), MidCheck %u (joined)",
+              (unsigned long long)s0, (unsigned long long)nt, (unsigned long long)raw, (unsigned long long)hits,
+              nmid);
+        break;
+      }
+      st.qc_pairs += nt * (uint64_t)(ml == 1 ? kNumInsts : nc);
+    } else {
+      // ---- QuickCheck, re-run with a larger buffer on overflow (no hit is dropped)
+      uint2 *surv;
+      uint32_t *qc_counts = counters + 8;
+      uint32_t nsurv = 0;
+      SurvRegions regions;
+      for (;;) {
+        ENSURE(B_SURV, uint2, surv_cap, surv);
+        if (!qc_tuned) {
+          int rc = tune_qc(h, tl, ml, tv0, tfp, nt, cfp, nc, pm_masked, surv, surv_cap, qc_counts, &qcc);
+          if (rc) return rc;
+          qc_tuned = true;
+        }
+        HIPCHK(h, hipMemsetAsync(qc_counts, 0, kQcRegions * 4, h->stream));
+        {
+          size_t ts = 0;
+          int rc = timed_begin(h, 0, &ts);
+          if (rc) return rc;
+          rc = launch_qc(h, qcc.tile, qcc.blocks_per_cu, tv0, tfp, nt, cfp, nc, pm_masked, surv, surv_cap,
+                         qc_counts);
+          if (rc) return rc;
+          rc = timed_end(h, ts);
+          if (rc) return rc;
+        }
+        uint32_t cnt8[kQcRegions];
+        HIPCHK(h, hipMemcpyAsync(cnt8, qc_counts, sizeof(cnt8), hipMemcpyDeviceToHost, h->stream));
+        SYNC_WD("device sync");
+        const uint32_t region_cap = surv_cap / kQcRegions;
+        uint64_t total = 0;
+        uint32_t most = 0;
+        regions.cap = region_cap;
+        for (uint32_t r = 0; r < kQcRegions; r++) {
+          regions.pre[r] = (uint32_t)std::min<uint64_t>(total, 0xFFFFFFFFu);
+          total += cnt8[r];
+          most = std::max(most, cnt8[r]);
+        }
+        regions.pre[kQcRegions] = (uint32_t)std::min<uint64_t>(total, 0xFFFFFFFFu);
+        trace(h, "segment %llu: %llu targets, QuickCheck survivors %llu (largest region %u of %u)",
+              (unsigned long long)s0, (unsigned long long)nt, (unsigned long long)total, most, region_cap);
+        if (most <= region_cap) {
+          nsurv = (uint32_t)total;
+          break;
+        }
+        st.overflow_retries++;
+        if ((uint64_t)most * 5 / 4 + 1024 > 0xFFFFFFFFull / kQcRegions)
+          return set_err(h, SX_ENOMEM, "QuickCheck survivor buffer (%u in one region)", most);
+        surv_cap = kQcRegions * (most + most / 4 + 1024);
+        if (!cfg->survivor_cap) h->surv_cap_hint = std::max(h->surv_cap_hint, surv_cap);
+      }
+      HIPCHK(h, hipEventRecord(e2, h->stream));
+      st.qc_raw_hits += nsurv;
+      st.qc_pairs += nt * (uint64_t)(ml == 1 ? kNumInsts : nc);
+
+      // ---- MidCheck
+      for (;;) {
+        ENSURE(B_MID, uint2, std::max<uint32_t>(nsurv, 1), mid);
+        HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));  // counters[2..7] stay 0 for the rounds
+        hipLaunchKernelGGL(k_seg_clear, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, rej, done,
+                           cfg->hits ? nullptr : rule_flag, nt * 2);
+        // first hit: MidCheck also leaves the first round's minimum per (target, pass) in cur
+        if (nsurv)
+          hipLaunchKernelGGL(k_midcheck, dim3(blocks_for(nsurv, kBlk)), dim3(kBlk), 0, h->stream, surv, regions,
+                             nsurv, tmh, tbytes, cmh, cbytes, cprune, cfg->hits, cfg->use_mid, mid,
+                             std::max<uint32_t>(nsurv, 1), counters, rank, cfg->hits ? nullptr : cur);
+        HIPCHK(h, hipGetLastError());
+        uint32_t cnt[2];
+        HIPCHK(h, hipMemcpyAsync(cnt, counters, 8, hipMemcpyDeviceToHost, h->stream));
+        SYNC_WD("device sync");
+        st.qc_hits += cnt[0];
+        nmid = cnt[1];
+        trace(h, "  MidCheck: %u QC hits after filters, %u pass", cnt[0], cnt[1]);
+        break;
+      }
     }
     st.mid_pass += nmid;
     HIPCHK(h, hipEventRecord(e3, h->stream));

@@ -70,7 +70,8 @@ typedef struct sx_stats {
   uint64_t rules;        /* rules emitted */
   uint64_t overflow_retries; /* compaction-buffer overflows recovered by re-running a tile */
   double ms_setup, ms_qc, ms_mid, ms_exh, ms_total; /* device times (HIP events) and wall */
-  double ms_qc_kernel, ms_exh_kernel; /* HIP events around the QuickCheck / exhaustive-sweep launches only */
+  double ms_qc_kernel, ms_exh_kernel; /* HIP events around the QuickCheck / exhaustive-sweep launches only
+                                         (QuickCheck's: with MidCheck when the two run as one join) */
   uint64_t qc_launches, exh_launches;
   uint32_t qc_tile;          /* QuickCheck targets per work item (autotuned unless tile_targets is set) */
   uint32_t qc_blocks_per_cu; /* 0 = grid launch; else persistent launch with this many blocks per CU */
@@ -106,10 +107,12 @@ typedef struct sx_enum_cfg {
   uint64_t n_targets;
   uint64_t first_target;  /* with targets == NULL: half-open range of the */
   uint64_t end_target;    /* pruned target index space (0, 0 = everything) */
-  uint32_t tile_targets;  /* QuickCheck targets per tile, grid launch, rounded up to a multiple of 32
-                             (0 = autotuned shape) */
-  uint32_t survivor_cap;  /* QuickCheck survivor buffer entries per segment, split over 8 regions
-                             (0 = default); a full region re-runs the segment with a larger buffer */
+  uint32_t tile_targets;  /* QuickCheck targets per tile, all-pairs grid launch, rounded up to a multiple
+                             of 32 (0 = with use_mid, QuickCheck and MidCheck as one hash join; without,
+                             the autotuned shape) */
+  uint32_t survivor_cap;  /* survivor buffer entries per segment: QuickCheck's, split over 8 regions, or
+                             with the join MidCheck's (0 = default); a full buffer re-runs the segment
+                             with a larger one */
   int32_t include_empty;  /* 1: also check every target against the empty sequence (0 bytes, never
                              pruned).  Go never tries it (worker.go:123 starts at length 1), so its
                              rules (rep_len 0) form a stream of their own, fetched with
