@@ -923,7 +923,8 @@ __global__ __launch_bounds__(256) void k_class_heads(const unsigned long long *_
 // class ids from the heads' exclusive scan; class tables zeroed beforehand
 __global__ __launch_bounds__(256) void k_class_fill(const unsigned long long *__restrict__ sk,
                                                     const uint32_t *__restrict__ sslot, uint32_t n,
-                                                    const uint32_t *__restrict__ head, const uint64_t *__restrict__ hoff,
+                                                    const uint32_t *__restrict__ head,
+                                                    const uint64_t *__restrict__ hoff,
                                                     const uint8_t *__restrict__ cbytes,
                                                     const uint8_t *__restrict__ cprune, unsigned long long *ckey,
                                                     uint32_t *crep, uint32_t *ccnt) {
@@ -2573,6 +2574,14 @@ struct sx_handle {
   int exh_k = 2;          // states (pairs when packed) per lane and opcode dispatch in the sweep (SX_EXH_K: 1 or 2)
   int exh_pk = 1;         // packed two-states-per-register interpreter in the sweep (SX_EXH_PK=0: one state)
   int qcm_join = 1;       // QuickCheck and MidCheck as one join when MidCheck is on (SX_QCM_JOIN=0: separate)
+  // the candidate side (words, fingerprints, hash tables, classes) is kept
+  // between enumerations while the candidate length, byte model and compare
+  // mask stay the same
+  struct {
+    bool valid = false, fused_built = false, fused_ok = false;
+    int ml = 0, bytes_model = -1;
+    uint32_t lo_mask = 0;
+  } cand;
   uint32_t mid_cap_hint = 0;  // MidCheck survivor buffer size that sufficed before
   int exh_group = 4;      // small plans swept one job per lane, this many pairs per lane and dispatch
                           // (SX_EXH_GROUP: 1, 2, 4 or 8; 0: every job state-parallel)
@@ -3508,10 +3517,15 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
   HIPCHK(h, hipEventCreate(&es));
   HIPCHK(h, hipEventRecord(e0, h->stream));
   HIPCHK(h, hipMemsetAsync(badop, 0, 4, h->stream));
-  hipLaunchKernelGGL(k_cand_words, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cwords, nc);
-  hipLaunchKernelGGL(k_iota, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, rank, nc);
+  const bool cand_kept = h->cand.valid && h->cand.ml == ml && h->cand.bytes_model == cfg->bytes_model &&
+                         h->cand.lo_mask == lo_mask;
+  if (!cand_kept) {
+    h->cand.valid = h->cand.fused_built = false;
+    hipLaunchKernelGGL(k_cand_words, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cwords, nc);
+    hipLaunchKernelGGL(k_iota, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, rank, nc);
+  }
   // fingerprints of single-instruction slots, then of pairs
-  {
+  if (!cand_kept) {
     uint32_t *w1;
     ENSURE(B_SCRATCH, uint32_t, kNumInsts, w1);
     hipLaunchKernelGGL(k_inst_words, dim3(blocks_for(kNumInsts, 256)), dim3(256), 0, h->stream, w1);
@@ -3539,6 +3553,15 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     ENSURE(B_QJ_HASH, uint32_t, nc, shs);
     ENSURE(B_QJ_SLOT, uint32_t, nc, ssl);
     ENSURE(B_QJ_DIR, uint32_t, (1ull << dk) + 1, dir);
+    h->qj_hash = shs;
+    h->qj_slot = ssl;
+    h->qj_dir = dir;
+    h->qj_dk = dk;
+  }
+  if (!cand_kept) {
+    uint32_t *key = (uint32_t *)h->bufs[B_QJ_KEY].p, *val = (uint32_t *)h->bufs[B_QJ_VAL].p;
+    uint32_t *shs = (uint32_t *)h->qj_hash, *ssl = (uint32_t *)h->qj_slot, *dir = (uint32_t *)h->qj_dir;
+    const uint32_t dk = h->qj_dk;
     hipLaunchKernelGGL(k_cand_filter_hash, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cfp, nc, lo_mask, key,
                        val);
     HIPCHK(h, hipGetLastError());
@@ -3550,10 +3573,10 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     hipLaunchKernelGGL(k_hash_dir, dim3(blocks_for((1ull << dk) + 1, 256)), dim3(256), 0, h->stream, shs,
                        (uint32_t)nc, dk, dir);
     HIPCHK(h, hipGetLastError());
-    h->qj_hash = shs;
-    h->qj_slot = ssl;
-    h->qj_dir = dir;
-    h->qj_dk = dk;
+    h->cand.valid = true;
+    h->cand.ml = ml;
+    h->cand.bytes_model = cfg->bytes_model;
+    h->cand.lo_mask = lo_mask;
   }
   // QuickCheck and MidCheck as one join (MidCheck on): the candidates' join
   // keys sorted with a directory, and their classes of equal masked
@@ -3587,36 +3610,43 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     ENSURE(B_CLSN, uint64_t, 2, cls_n);
     bad = reinterpret_cast<uint32_t *>(cls_n + 1);
     const uint32_t n32 = (uint32_t)nc;
-    hipLaunchKernelGGL(k_cand_keys, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cfp, cmh, nc, lo_mask, ck, jk,
-                       v1, v2);
-    HIPCHK(h, hipGetLastError());
-    size_t tb1 = 0, tb2 = 0;
-    HIPCHK(h, rocprim::radix_sort_pairs(nullptr, tb1, ck, cks, v1, v1s, n32, 0, 64, h->stream));
-    HIPCHK(h, rocprim::radix_sort_pairs(nullptr, tb2, jk, j_key, v2, j_slot, n32, 0, 32, h->stream));
-    uint8_t *tmp;
-    ENSURE(B_GSORT, uint8_t, std::max(tb1, tb2) + 1, tmp);
-    HIPCHK(h, rocprim::radix_sort_pairs(tmp, tb1, ck, cks, v1, v1s, n32, 0, 64, h->stream));
-    HIPCHK(h, rocprim::radix_sort_pairs(tmp, tb2, jk, j_key, v2, j_slot, n32, 0, 32, h->stream));
-    hipLaunchKernelGGL(k_hash_dir, dim3(blocks_for((1ull << j_dk) + 1, 256)), dim3(256), 0, h->stream, j_key, n32, j_dk,
-                       j_dir);
-    hipLaunchKernelGGL(k_class_heads, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cks, n32, head);
-    HIPCHK(h, hipGetLastError());
-    int rc = scan_u32(h, head, nc, hoff, cls_n);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nc * (kClsBins + 1), 256)), dim3(256), 0, h->stream, cls_cnt,
-                       nc * (kClsBins + 1), 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, h->stream, bad, (uint64_t)1, 0u);
-    hipLaunchKernelGGL(k_class_fill, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cks, v1s, n32, head, hoff,
-                       cbytes, cprune, cls_key, cls_rep, cls_cnt);
-    hipLaunchKernelGGL(k_class_verify, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, v1s, n32, head, hoff,
-                       cls_rep, cfp, lo_mask, bad);
-    HIPCHK(h, hipGetLastError());
-    uint32_t bad_h = 0;
-    HIPCHK(h, hipMemcpyAsync(&bad_h, bad, 4, hipMemcpyDeviceToHost, h->stream));
-    SYNC_WD("device sync");
-    if (bad_h) {
-      trace(h, "  candidate classes: a 64-bit key holds two outputs; QuickCheck and MidCheck run separately");
-      fused = false;
+    if (h->cand.fused_built) {
+      fused = h->cand.fused_ok;
+    } else {
+      hipLaunchKernelGGL(k_cand_keys, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cfp, cmh, nc, lo_mask, ck, jk,
+                         v1, v2);
+      HIPCHK(h, hipGetLastError());
+      size_t tb1 = 0, tb2 = 0;
+      HIPCHK(h, rocprim::radix_sort_pairs(nullptr, tb1, ck, cks, v1, v1s, n32, 0, 64, h->stream));
+      HIPCHK(h, rocprim::radix_sort_pairs(nullptr, tb2, jk, j_key, v2, j_slot, n32, 0, 32, h->stream));
+      uint8_t *tmp;
+      ENSURE(B_GSORT, uint8_t, std::max(tb1, tb2) + 1, tmp);
+      HIPCHK(h, rocprim::radix_sort_pairs(tmp, tb1, ck, cks, v1, v1s, n32, 0, 64, h->stream));
+      HIPCHK(h, rocprim::radix_sort_pairs(tmp, tb2, jk, j_key, v2, j_slot, n32, 0, 32, h->stream));
+      hipLaunchKernelGGL(k_hash_dir, dim3(blocks_for((1ull << j_dk) + 1, 256)), dim3(256), 0, h->stream, j_key, n32,
+                         j_dk,
+                         j_dir);
+      hipLaunchKernelGGL(k_class_heads, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cks, n32, head);
+      HIPCHK(h, hipGetLastError());
+      int rc = scan_u32(h, head, nc, hoff, cls_n);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nc * (kClsBins + 1), 256)), dim3(256), 0, h->stream, cls_cnt,
+                         nc * (kClsBins + 1), 0u);
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, h->stream, bad, (uint64_t)1, 0u);
+      hipLaunchKernelGGL(k_class_fill, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cks, v1s, n32, head, hoff,
+                         cbytes, cprune, cls_key, cls_rep, cls_cnt);
+      hipLaunchKernelGGL(k_class_verify, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, v1s, n32, head, hoff,
+                         cls_rep, cfp, lo_mask, bad);
+      HIPCHK(h, hipGetLastError());
+      uint32_t bad_h = 0;
+      HIPCHK(h, hipMemcpyAsync(&bad_h, bad, 4, hipMemcpyDeviceToHost, h->stream));
+      SYNC_WD("device sync");
+      if (bad_h) {
+        trace(h, "  candidate classes: a 64-bit key holds two outputs; QuickCheck and MidCheck run separately");
+        fused = false;
+      }
+      h->cand.fused_built = true;
+      h->cand.fused_ok = fused;
     }
   }
   // the empty candidate's fingerprint: a length-0 sequence's outputs are the
