@@ -10,9 +10,9 @@
 // position (sx_mlp_proposal).
 //
 // Weights are used in bf16 with fp32 master copies for Adam.  All products
-// run on MFMA (v_mfma_f32_16x16x16bf16_1k: bf16 inputs, fp32 accumulation)
-// in one tiled kernel with generic strides, so the transposed products of the
-// backward pass need no copies.  The training step (batch draw, forward,
+// run on MFMA (gfx950's v_mfma_f32_16x16x32_bf16: bf16 inputs, fp32
+// accumulation, 32 of K per instruction) in one tiled kernel with generic
+// strides, so the transposed products of the backward pass need no copies.  The training step (batch draw, forward,
 // loss, backward, Adam) is captured once in a hipGraph and replayed; its
 // step counter lives in device memory.
 #include <hip/hip_runtime.h>
@@ -45,7 +45,7 @@ constexpr size_t kOffW2 = kOffB1 + kHid;
 constexpr size_t kOffB2 = kOffW2 + (size_t)kHid * kOut;
 constexpr size_t kParams = kOffB2 + kOut;
 
-typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -58,7 +58,9 @@ __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint
 // C[M][N] (fp32, ldc) = sum_k A(m,k) B(k,n) with A(m,k) = A[m*sam + k*sak] and
 // B(k,n) = B[k*sbk + n*sbn] (bf16).  64x64 tile per 256-thread block, each
 // wave a 32x32 quarter as 2x2 MFMA 16x16 blocks; K in steps of 32 staged in
-// LDS (two MFMA k-steps of 16 per stage).  gridDim.z > 1 splits K: block z
+// LDS, one MFMA k-step of 32 per stage (lane l holds A rows l % 16 and B
+// columns l % 16 at k = 8 (l / 16) + j, j < 8: the same k for both operands,
+// so the products pair up whatever order the unit sums them in).  gridDim.z > 1 splits K: block z
 // takes K range [z * kchunk, ...) and adds its partial product to C with
 // atomics (C zeroed by the caller; no epilogue).  Otherwise the epilogue
 // applies + bias[n] (if bias), relu (if relu), + C (if accumulate) and
@@ -103,22 +105,20 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, lo
       }
     }
     __syncthreads();
-    const int l16 = lane & 15;
+    const int l16 = lane & 15, kq = (lane >> 4) * 8;
+    v8bf a[2], b[2];
 #pragma unroll
-    for (int kk = 0; kk < 32; kk += 16) {
-      const int kq = kk + (lane >> 4) * 4;
-      v4s a[2], b[2];
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-      for (int i = 0; i < 2; i++)
-        for (int q = 0; q < 4; q++) a[i][q] = (short)As[wm + i * 16 + l16][kq + q];
+      for (int q = 0; q < 8; q++) a[i][q] = __builtin_bit_cast(__bf16, As[wm + i * 16 + l16][kq + q]);
 #pragma unroll
-      for (int j = 0; j < 2; j++)
-        for (int q = 0; q < 4; q++) b[j][q] = (short)Bs[kq + q][wn + j * 16 + l16];
+    for (int j = 0; j < 2; j++)
 #pragma unroll
-      for (int i = 0; i < 2; i++)
+      for (int q = 0; q < 8; q++) b[j][q] = __builtin_bit_cast(__bf16, Bs[kq + q][wn + j * 16 + l16]);
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     __syncthreads();
   }
   const bool split = gridDim.z > 1;
@@ -221,18 +221,22 @@ __global__ void k_softmax_ce(const float *Z, long ldz, const uint16_t *lab, int 
   if (threadIdx.x == 0 && loss_rows && y >= 0) loss_rows[b] = -(z[y] - mx - logf(sum));
 }
 
-// column sums: out[n] = sum_b in[b][n]; a block takes 64 columns, its 4
-// waves a quarter of the rows each, reduced through LDS.
+// column sums: out[n] += sum_b in[b][n] (out zeroed by the caller); block
+// (x, y) takes 64 columns and rows [y kColRows, (y + 1) kColRows), its 4 waves
+// a quarter of those each, reduced through LDS and added with one atomic per
+// column (a block per 64 columns alone left the chip mostly idle).
+constexpr int kColRows = 128;
 __global__ __launch_bounds__(256) void k_colsum(const float *in, long ld, int B, int N, float *out) {
   __shared__ float part[4][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int n = blockIdx.x * 64 + lane;
+  const int b1 = min(B, (int)(blockIdx.y + 1) * kColRows);
   float acc = 0.f;
   if (n < N)
-    for (int b = wv; b < B; b += 4) acc += in[(long)b * ld + n];
+    for (int b = (int)blockIdx.y * kColRows + wv; b < b1; b += 4) acc += in[(long)b * ld + n];
   part[wv][lane] = acc;
   __syncthreads();
-  if (wv == 0 && n < N) out[n] = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+  if (wv == 0 && n < N) atomicAdd(&out[n], part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
 }
 
 // dH *= relu'(H), and a bf16 copy
@@ -244,13 +248,81 @@ __global__ void k_relu_back(float *dH, const float *H, uint16_t *dHb, int n) {
   dHb[i] = f2bf(g);
 }
 
-// dE[tok] += dX rows
-__global__ void k_scatter_emb(const float *dX, const uint16_t *tok, int B, float *dE) {
-  int b = blockIdx.x;
-  for (int e = threadIdx.x; e < kIn; e += blockDim.x) {
-    int s = e / kEmb, d = e % kEmb;
-    atomicAdd(&dE[(size_t)tok[b * kCtx + s] * kEmb + d], dX[(size_t)b * kIn + e]);
+// The embedding gradient dE[t] = sum of the dX rows (example b, context
+// position s) whose token is t, as segmented sums instead of one atomic per
+// element (the padding token alone fills most contexts, and its atomics on
+// one row of dE serialised).  k_emb_sort (one block) lists the batch's
+// positions b * 8 + s grouped by token: off[t] .. off[t + 1] in pos.  (The
+// padding token kTok - 1 is a token like the others.)
+__global__ __launch_bounds__(1024) void k_emb_sort(const uint16_t *tok, int B, uint32_t *off, uint32_t *pos) {
+  __shared__ uint32_t cnt[kTok + 1];
+  __shared__ uint32_t wsum[16];
+  const int n = B * kCtx, tid = threadIdx.x;
+  for (int t = tid; t <= kTok; t += 1024) cnt[t] = 0;
+  __syncthreads();
+  for (int i = tid; i < n; i += 1024) atomicAdd(&cnt[tok[i]], 1u);
+  __syncthreads();
+  // exclusive scan of cnt[0 .. kTok] in place: 3 entries per thread
+  constexpr int kPer = (kTok + 1 + 1023) / 1024;
+  uint32_t loc[kPer], run = 0;
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    const int t = tid * kPer + k;
+    loc[k] = t <= kTok ? cnt[t] : 0u;
+    run += loc[k];
   }
+  // inclusive scan of the per-thread totals across the block
+  uint32_t x = run;
+  const int lane = tid & 63, w = tid >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  uint32_t before = 0;
+  for (int k = 0; k < w; k++) before += wsum[k];
+  uint32_t acc = before + x - run;  // exclusive prefix of this thread's first entry
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    const int t = tid * kPer + k;
+    if (t <= kTok) {
+      cnt[t] = acc;
+      off[t] = acc;
+    }
+    acc += loc[k];
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += 1024) pos[atomicAdd(&cnt[tok[i]], 1u)] = (uint32_t)i;
+}
+// The sums: a wave takes 16 consecutive listed positions (positions of one
+// token are consecutive), lane d the embedding dimension d, and adds its
+// running sum to dE (zeroed with the other gradients) when the token changes
+// and at the end: a token's rows spread over many waves, with one atomic per
+// wave and dimension where a block per token left the padding token's
+// thousand rows to one block.
+constexpr int kEmbPerWave = 16;
+__global__ __launch_bounds__(256) void k_emb_grad(const float *dX, const uint16_t *tok, const uint32_t *pos, int n,
+                                                  float *dE) {
+  const int lane = threadIdx.x & 63;
+  const int i0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * kEmbPerWave;
+  if (i0 >= n) return;
+  const int i1 = min(n, i0 + kEmbPerWave);
+  uint32_t cur = kTok;
+  float acc = 0.f;
+  for (int i = i0; i < i1; i++) {
+    const uint32_t q = pos[i];  // b * kCtx + s
+    const uint32_t t = tok[q];
+    if (t != cur) {
+      if (cur != kTok) atomicAdd(&dE[(size_t)cur * kEmb + lane], acc);
+      cur = t;
+      acc = 0.f;
+    }
+    acc += dX[(size_t)(q / kCtx) * kIn + (q % kCtx) * kEmb + lane];
+  }
+  atomicAdd(&dE[(size_t)cur * kEmb + lane], acc);
 }
 
 __global__ void k_fill_f32(float *p, size_t n, float v) {
@@ -285,12 +357,18 @@ __global__ void k_to_bf16(const float *p, uint16_t *pb, size_t n) {
   if (i < n) pb[i] = f2bf(p[i]);
 }
 
-__global__ void k_sum_loss(const float *rows, int B, float *out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    float s = 0.f;
-    for (int b = 0; b < B; b++) s += rows[b];
-    *out = s / (float)B;
+// mean of the B row losses, one 256-thread block
+__global__ __launch_bounds__(256) void k_sum_loss(const float *rows, int B, float *out) {
+  __shared__ float red[256];
+  float s = 0.f;
+  for (int b = threadIdx.x; b < B; b += 256) s += rows[b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) *out = red[0] / (float)B;
 }
 
 }  // namespace
@@ -305,6 +383,7 @@ struct sx_mlp {
   float *H = nullptr, *Z = nullptr, *dZ = nullptr, *dH = nullptr, *dX = nullptr, *lossr = nullptr,
         *loss = nullptr;
   uint32_t *step = nullptr;
+  uint32_t *eoff = nullptr, *epos = nullptr;  // the batch's positions grouped by token (embedding gradient)
   uint16_t *ctx_d = nullptr, *lab_d = nullptr;
   size_t n_data = 0;
   hipGraph_t graph = nullptr;
@@ -321,10 +400,12 @@ inline unsigned nb(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs)
 int alloc_work(sx_mlp *m, int B) {
   if (m->B >= B) return SX_OK;
   for (void *q : {(void *)m->X, (void *)m->tok, (void *)m->lab, (void *)m->Hb, (void *)m->dZb, (void *)m->dHb,
-                  (void *)m->H, (void *)m->Z, (void *)m->dZ, (void *)m->dH, (void *)m->dX, (void *)m->lossr})
+                  (void *)m->H, (void *)m->Z, (void *)m->dZ, (void *)m->dH, (void *)m->dX, (void *)m->lossr,
+                  (void *)m->eoff, (void *)m->epos})
     if (q) (void)hipFree(q);
   m->X = m->tok = m->lab = m->Hb = m->dZb = m->dHb = nullptr;
   m->H = m->Z = m->dZ = m->dH = m->dX = m->lossr = nullptr;
+  m->eoff = m->epos = nullptr;
   m->B = 0;
   if (m->gexec) { (void)hipGraphExecDestroy(m->gexec); m->gexec = nullptr; }
   if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
@@ -334,7 +415,9 @@ int alloc_work(sx_mlp *m, int B) {
             hipMalloc(&m->dHb, (size_t)B * kHid * 2) == hipSuccess && hipMalloc(&m->H, (size_t)B * kHid * 4) == hipSuccess &&
             hipMalloc(&m->Z, (size_t)B * kOutPad * 4) == hipSuccess && hipMalloc(&m->dZ, (size_t)B * kOutPad * 4) == hipSuccess &&
             hipMalloc(&m->dH, (size_t)B * kHid * 4) == hipSuccess && hipMalloc(&m->dX, (size_t)B * kIn * 4) == hipSuccess &&
-            hipMalloc(&m->lossr, (size_t)B * 4) == hipSuccess;
+            hipMalloc(&m->lossr, (size_t)B * 4) == hipSuccess &&
+            hipMalloc(&m->eoff, (size_t)(kTok + 1) * 4) == hipSuccess &&
+            hipMalloc(&m->epos, (size_t)B * kCtx * 4) == hipSuccess;
   if (!ok) return sx_internal_error(m->h, SX_ENOMEM, "mlp work buffers");
   m->B = B;
   return SX_OK;
@@ -352,7 +435,7 @@ void enqueue_fwd_bwd(sx_mlp *m, int B, bool backward, float *probs, hipStream_t 
   hipLaunchKernelGGL(k_softmax_ce, dim3(B), dim3(256), 0, s, m->Z, (long)kOutPad, backward ? m->lab : nullptr, B,
                      backward ? m->dZ : nullptr, backward ? m->dZb : nullptr, backward ? m->lossr : nullptr, probs);
   if (!backward) return;
-  hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(64), 0, s, m->lossr, B, m->loss);
+  hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(256), 0, s, m->lossr, B, m->loss);
   hipLaunchKernelGGL(k_fill_f32, dim3(nb(kParams, 256)), dim3(256), 0, s, m->g, kParams, 0.f);
   // products over the batch (K = B) and over the 2,871 outputs are split in K
   // and accumulated with atomics into zeroed outputs: the output tiles alone
@@ -362,13 +445,15 @@ void enqueue_fwd_bwd(sx_mlp *m, int B, bool backward, float *probs, hipStream_t 
   hipLaunchKernelGGL(k_gemm, dim3(nb(kOut, 64), nb(kHid, 64), split_b), dim3(256), 0, s, m->Hb, 1L, (long)kHid, m->dZb,
                      (long)kOutPad, 1L, m->g + kOffW2, (long)kOut, (uint16_t *)nullptr, kHid, kOut, B,
                      (const float *)nullptr, 0, 0);
-  hipLaunchKernelGGL(k_colsum, dim3(nb(kOut, 64)), dim3(256), 0, s, m->dZ, (long)kOutPad, B, kOut, m->g + kOffB2);
+  hipLaunchKernelGGL(k_colsum, dim3(nb(kOut, 64), nb(B, kColRows)), dim3(256), 0, s, m->dZ, (long)kOutPad, B, kOut,
+                     m->g + kOffB2);
   // dH = dZ W2^T  (B(k=c, n=j) = W2[j*kOut + c])
   hipLaunchKernelGGL(k_fill_f32, dim3(nb((size_t)B * kHid, 256)), dim3(256), 0, s, m->dH, (size_t)B * kHid, 0.f);
   hipLaunchKernelGGL(k_gemm, dim3(nb(kHid, 64), nb(B, 64), 8), dim3(256), 0, s, m->dZb, (long)kOutPad, 1L, W2b, 1L,
                      (long)kOut, m->dH, (long)kHid, (uint16_t *)nullptr, B, kHid, kOut, (const float *)nullptr, 0, 0);
   hipLaunchKernelGGL(k_relu_back, dim3(nb((size_t)B * kHid, 256)), dim3(256), 0, s, m->dH, m->H, m->dHb, B * kHid);
-  hipLaunchKernelGGL(k_colsum, dim3(nb(kHid, 64)), dim3(256), 0, s, m->dH, (long)kHid, B, kHid, m->g + kOffB1);
+  hipLaunchKernelGGL(k_colsum, dim3(nb(kHid, 64), nb(B, kColRows)), dim3(256), 0, s, m->dH, (long)kHid, B, kHid,
+                     m->g + kOffB1);
   // dW1 = X^T dH
   hipLaunchKernelGGL(k_gemm, dim3(nb(kHid, 64), nb(kIn, 64), (unsigned)std::max(1, B / 128)), dim3(256), 0, s, m->X, 1L,
                      (long)kIn, m->dHb, (long)kHid,
@@ -376,7 +461,9 @@ void enqueue_fwd_bwd(sx_mlp *m, int B, bool backward, float *probs, hipStream_t 
   // dX = dH W1^T, scattered into dE
   hipLaunchKernelGGL(k_gemm, dim3(nb(kIn, 64), nb(B, 64)), dim3(256), 0, s, m->dHb, (long)kHid, 1L, W1b, 1L,
                      (long)kHid, m->dX, (long)kIn, (uint16_t *)nullptr, B, kIn, kHid, (const float *)nullptr, 0, 0);
-  hipLaunchKernelGGL(k_scatter_emb, dim3(B), dim3(256), 0, s, m->dX, m->tok, B, m->g + kOffE);
+  hipLaunchKernelGGL(k_emb_sort, dim3(1), dim3(1024), 0, s, m->tok, B, m->eoff, m->epos);
+  hipLaunchKernelGGL(k_emb_grad, dim3(nb((size_t)B * kCtx, 4 * kEmbPerWave)), dim3(256), 0, s, m->dX, m->tok,
+                     m->epos, B * kCtx, m->g + kOffE);
 }
 
 int upload_set(sx_mlp *m, const uint16_t *ctx, const uint16_t *labels, size_t n) {
@@ -444,7 +531,7 @@ extern "C" int sx_mlp_destroy(sx_mlp *m) {
   for (void *q : {(void *)m->p, (void *)m->g, (void *)m->m, (void *)m->v, (void *)m->pb, (void *)m->step,
                   (void *)m->loss, (void *)m->X, (void *)m->tok, (void *)m->lab, (void *)m->Hb, (void *)m->dZb,
                   (void *)m->dHb, (void *)m->H, (void *)m->Z, (void *)m->dZ, (void *)m->dH, (void *)m->dX,
-                  (void *)m->lossr, (void *)m->ctx_d, (void *)m->lab_d})
+                  (void *)m->lossr, (void *)m->ctx_d, (void *)m->lab_d, (void *)m->eoff, (void *)m->epos})
     if (q) (void)hipFree(q);
   delete m;
   return SX_OK;
