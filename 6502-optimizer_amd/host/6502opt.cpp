@@ -25,9 +25,11 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <unistd.h>
@@ -221,27 +223,102 @@ void write_json(FILE *f, const std::vector<sx_rule> &rules) {
   });
   write_json(f, j);
 }
-void write_jsonl(FILE *f, const std::vector<sx_rule> &rules, int model) {
-  std::string buf;
-  buf.reserve(rules.size() * 128);
-  char line[512];
-  for (const sx_rule &r : rules) {
+// cu:912-930: one JSON object per line.  A million-line file is formatted
+// on up to 16 threads into per-thread buffers (disassembly from a table of
+// every opcode and immediate, integers by hand), written in order.
+struct JsonlFormatter {
+  char dis[58][256][8];  // NUL-terminated disassembly of (opcode, immediate)
+  uint8_t dlen[58][256];
+  JsonlFormatter() {
+    for (int op = 0; op < 58; op++)
+      for (int imm = 0; imm < 256; imm++) {
+        const std::string d = disasm((uint32_t)op | ((uint32_t)imm << 16));
+        memcpy(dis[op][imm], d.c_str(), d.size() + 1);
+        dlen[op][imm] = (uint8_t)d.size();
+      }
+  }
+  static char *put_int(char *p, int v) {
+    if (v < 0) {
+      *p++ = '-';
+      v = -v;
+    }
+    char t[12];
+    int n = 0;
+    do {
+      t[n++] = (char)('0' + v % 10);
+      v /= 10;
+    } while (v);
+    while (n) *p++ = t[--n];
+    return p;
+  }
+  static char *put(char *p, const char *s) {
+    while (*s) *p++ = *s++;
+    return p;
+  }
+  char *put_seq(char *p, const uint32_t *w, int n) const {
+    for (int i = 0; i < n; i++) {
+      if (i) p = put(p, " : ");
+      const uint32_t op = w[i] & 0xFFFFu;
+      if (op >= 58) {
+        const std::string d = disasm(w[i]);
+        memcpy(p, d.data(), d.size());
+        p += d.size();
+        continue;
+      }
+      const uint32_t imm = has_imm((int)op) ? (w[i] >> 16) & 0xFFu : 0u;
+      memcpy(p, dis[op][imm], dlen[op][imm]);
+      p += dlen[op][imm];
+    }
+    return p;
+  }
+  // one line, at most 8 x 11 + 120 characters
+  char *line(char *p, const sx_rule &r, int model) const {
     int sb = 0, rb = 0;
     for (int k = 0; k < r.src_len; k++) sb += byte_size(r.src[k] & 0xFFFF, model);
     for (int k = 0; k < r.rep_len; k++) rb += byte_size(r.rep[k] & 0xFFFF, model);
-    int n = snprintf(line, sizeof(line),
-                     "{\"source_asm\":\"%s\",\"replacement_asm\":\"%s\",\"source_bytes\":%d,"
-                     "\"replacement_bytes\":%d,\"bytes_saved\":%d,\"cycles_saved\":%d",
-                     disasm_seq(r.src, r.src_len).c_str(), disasm_seq(r.rep, r.rep_len).c_str(), sb, rb,
-                     r.bytes_saved, r.cycles_saved);
-    buf.append(line, n);
+    p = put(p, "{\"source_asm\":\"");
+    p = put_seq(p, r.src, r.src_len);
+    p = put(p, "\",\"replacement_asm\":\"");
+    p = put_seq(p, r.rep, r.rep_len);
+    p = put(p, "\",\"source_bytes\":");
+    p = put_int(p, sb);
+    p = put(p, ",\"replacement_bytes\":");
+    p = put_int(p, rb);
+    p = put(p, ",\"bytes_saved\":");
+    p = put_int(p, r.bytes_saved);
+    p = put(p, ",\"cycles_saved\":");
+    p = put_int(p, r.cycles_saved);
     if (r.dead_flags) {
-      n = snprintf(line, sizeof(line), ",\"dead_flags\":\"0x%02X\"", r.dead_flags);
-      buf.append(line, n);
+      static const char hx[] = "0123456789ABCDEF";
+      p = put(p, ",\"dead_flags\":\"0x");
+      *p++ = hx[(r.dead_flags >> 4) & 15];
+      *p++ = hx[r.dead_flags & 15];
+      *p++ = '"';
     }
-    buf += "}\n";
+    p = put(p, "}\n");
+    return p;
   }
-  fwrite(buf.data(), 1, buf.size(), f);
+};
+
+bool write_jsonl(FILE *f, const sx_rule *rules, size_t n, int model) {
+  static const JsonlFormatter fmt;
+  constexpr size_t kLineMax = 256;
+  const unsigned nth = n < 65536 ? 1u : std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  std::vector<std::vector<char>> bufs(nth);
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nth; t++)
+    pool.emplace_back([&, t] {
+      const size_t a = n * t / nth, b = n * (t + 1) / nth;
+      std::vector<char> &buf = bufs[t];
+      buf.resize((b - a) * kLineMax);
+      char *p = buf.data();
+      for (size_t i = a; i < b; i++) p = fmt.line(p, rules[i], model);
+      buf.resize((size_t)(p - buf.data()));
+    });
+  for (auto &th : pool) th.join();
+  for (const auto &buf : bufs)
+    if (!buf.empty() && fwrite(buf.data(), 1, buf.size(), f) != buf.size()) return false;
+  return true;
 }
 
 struct Args {
@@ -459,17 +536,17 @@ int cmd_enumerate(int argc, char **argv) {
     uint64_t n = 0;
     sx_stats st;
     if (check(h, sx_enumerate(h, &cfg, nullptr, 0, &n, &st), "sx_enumerate")) return 1;
-    std::vector<sx_rule> rules(n);
+    // (left uninitialised: sx_fetch_rules writes every byte of every record)
+    std::unique_ptr<sx_rule[]> rules(new sx_rule[std::max<uint64_t>(n, 1)]);
     uint64_t got = 0;
-    if (check(h, sx_fetch_rules(h, rules.data(), 0, n, &got), "sx_fetch_rules")) return 1;
+    if (check(h, sx_fetch_rules(h, rules.get(), 0, n, &got), "sx_fetch_rules")) return 1;
     tot.targets += st.targets;
     tot.qc_pairs += st.qc_pairs;
     tot.qc_hits += st.qc_hits;
     tot.mid_pass += st.mid_pass;
     tot.exh_pairs += st.exh_pairs;
     if (jf) {
-      write_jsonl(jf, rules, model);
-      if (fflush(jf) != 0) return die("cannot write " + jl);
+      if (!write_jsonl(jf, rules.get(), got, model) || fflush(jf) != 0) return die("cannot write " + jl);
     }
     ck.rules += n;
     ck.next = stop;
@@ -483,7 +560,7 @@ int cmd_enumerate(int argc, char **argv) {
               (unsigned long long)ck.end, 100.0 * (double)ck.next / (double)std::max<uint64_t>(ck.end, 1),
               (unsigned long long)ck.rules);
     }
-    if (out.size() || a.has("verbose")) all.insert(all.end(), rules.begin(), rules.end());
+    if (out.size() || a.has("verbose")) all.insert(all.end(), rules.get(), rules.get() + got);
   }
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   printf("  %llu targets | %llu QC pairs | QC hits %llu | Mid %llu | Exh jobs %llu | %.2f s (%.3g pairs/s)\n",
