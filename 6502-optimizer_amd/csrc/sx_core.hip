@@ -37,6 +37,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -76,6 +77,7 @@ struct SurvRegions {
   uint32_t cap;                  // entries per region
 };
 constexpr int kEvalShards = 1024;       // shards of the evaluated-states counter
+constexpr size_t kPinBytes = 64 << 10;  // pinned read-back staging per handle
 
 
 // ------------------------------------------------------ device helpers --
@@ -2589,7 +2591,7 @@ struct sx_handle {
   // = (ptab_mode, ptab_pm); -1: none yet
   int ptab_mode = -1;
   uint32_t ptab_pm = 0;
-  int seg_log = 22;       // targets per segment = 2^seg_log (SX_SEG_LOG, 16..24)
+  int seg_log = 23;       // targets per segment = 2^seg_log (SX_SEG_LOG, 16..24)
   uint32_t surv_cap_hint = 0;  // QuickCheck survivor buffer size that sufficed before
   // kernel-only timing of the QuickCheck and exhaustive-sweep launches
   hipEvent_t kev[2] = {nullptr, nullptr};
@@ -2620,6 +2622,10 @@ struct sx_handle {
   hipEvent_t seg_ev = nullptr;
   uint32_t *h_rules = nullptr;
   uint64_t h_rules_capw = 0;
+  // pinned staging block for the counters the host reads back to size the
+  // next stage (a copy to pageable memory waits by itself, so two such reads
+  // cost two round trips)
+  uint8_t *pin = nullptr;
   // QuickCheck launch configurations chosen by the autotuner, per (target_len, max_cand_len)
   struct QcChoice {
     int tl, ml, big;  // big: segments of >= 65,536 targets
@@ -2746,6 +2752,30 @@ int sync_wd(sx_handle *h, const char *stage) {
     if (dt < 0.002) std::this_thread::yield();
     else std::this_thread::sleep_for(std::chrono::microseconds(dt < 0.01 ? 50 : 500));
   }
+}
+
+// Device-to-host reads of small counters through the handle's pinned staging
+// block: the copies are queued together and waited for once.
+struct ReadBack {
+  void *dst;
+  const void *src;
+  size_t bytes;
+};
+int read_back(sx_handle *h, std::initializer_list<ReadBack> rs, const char *stage) {
+  size_t off = 0;
+  for (const ReadBack &r : rs) {
+    if (off + r.bytes > kPinBytes) return set_err(h, SX_EINVAL, "%s: read-back of %zu bytes", stage, off + r.bytes);
+    HIPCHK(h, hipMemcpyAsync(h->pin + off, r.src, r.bytes, hipMemcpyDeviceToHost, h->stream));
+    off += (r.bytes + 63) & ~size_t(63);
+  }
+  int rc = sync_wd(h, stage);
+  if (rc) return rc;
+  off = 0;
+  for (const ReadBack &r : rs) {
+    memcpy(r.dst, h->pin + off, r.bytes);
+    off += (r.bytes + 63) & ~size_t(63);
+  }
+  return SX_OK;
 }
 
 // SX_TRACE=1 prints stage progress to stderr.
@@ -3065,8 +3095,7 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, bo
   int rc = scan_u32(h, chunks, nj, offs, total);
   if (rc) return rc;
   uint64_t tot = 0;
-  HIPCHK(h, hipMemcpyAsync(&tot, total, sizeof(tot), hipMemcpyDeviceToHost, h->stream));
-  rc = sync_wd(h, "exhaustive plan");
+  rc = read_back(h, {{&tot, total, sizeof(tot)}}, "exhaustive plan");
   if (rc) return rc;
   *chunks_out = tot;
   trace(h, "  exhaustive: %u jobs, %llu chunks", nj, (unsigned long long)tot);
@@ -3210,13 +3239,15 @@ int clear_evaluated(sx_handle *h) {
   HIPCHK(h, hipMemsetAsync(evald, 0, 3 * kEvalShards * sizeof(unsigned long long), h->stream));
   return SX_OK;
 }
-int read_evaluated(sx_handle *h, uint64_t *sum, uint64_t *sum_group, uint64_t *group_jobs) {
+// bad / bad_d: a 32-bit device flag read in the same wait
+int read_evaluated(sx_handle *h, uint64_t *sum, uint64_t *sum_group, uint64_t *group_jobs, uint32_t *bad,
+                   const uint32_t *bad_d) {
   unsigned long long *evald;
   ENSURE(B_CURSOR, unsigned long long, 3 * kEvalShards, evald);
   std::vector<unsigned long long> shards(3 * kEvalShards);
-  HIPCHK(h, hipMemcpyAsync(shards.data(), evald, 3 * kEvalShards * sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, h->stream));
-  int rc = sync_wd(h, "exhaustive sweep");
+  const size_t sb = 3 * kEvalShards * sizeof(unsigned long long);
+  int rc = bad ? read_back(h, {{shards.data(), evald, sb}, {bad, bad_d, 4}}, "exhaustive sweep")
+               : read_back(h, {{shards.data(), evald, sb}}, "exhaustive sweep");
   if (rc) return rc;
   *sum = *sum_group = *group_jobs = 0;
   for (int i = 0; i < kEvalShards; i++) {
@@ -3367,7 +3398,8 @@ extern "C" int sx_open(int device, sx_handle **out) {
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->kev[0]) != hipSuccess || hipEventCreate(&h->kev[1]) != hipSuccess ||
-      hipEventCreateWithFlags(&h->seg_ev, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&h->seg_ev, hipEventDisableTiming) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&h->pin), kPinBytes, hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
     sx_close(h);
     return SX_EHIP;
@@ -3386,6 +3418,7 @@ extern "C" int sx_close(sx_handle *h) {
   if (h->d_l2) (void)hipFree(h->d_l2);
   if (h->d_rules) (void)hipFree(h->d_rules);
   if (h->h_rules) (void)hipHostFree(h->h_rules);
+  if (h->pin) (void)hipHostFree(h->pin);
   for (hipEvent_t e : h->kev)
     if (e) (void)hipEventDestroy(e);
   for (auto &t : h->timed) {
@@ -3508,13 +3541,15 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
   ENSURE(B_CCYC, uint8_t, nc, ccyc);
   ENSURE(B_CPRUNE, uint8_t, nc, cprune);
   ENSURE(B_BADOP, uint32_t, 1, badop);
-  hipEvent_t e0, e1, e2, e3, e4, es;
+  // stage events: two sets for the segments, used in turn, so that a
+  // segment's stage times are read once a later wait has passed them (no
+  // wait of their own)
+  hipEvent_t e0, es, seg_evs[2][4];
+  bool seg_evs_pending[2] = {false, false};
   HIPCHK(h, hipEventCreate(&e0));
-  HIPCHK(h, hipEventCreate(&e1));
-  HIPCHK(h, hipEventCreate(&e2));
-  HIPCHK(h, hipEventCreate(&e3));
-  HIPCHK(h, hipEventCreate(&e4));
   HIPCHK(h, hipEventCreate(&es));
+  for (auto &set : seg_evs)
+    for (auto &ev : set) HIPCHK(h, hipEventCreate(&ev));
   HIPCHK(h, hipEventRecord(e0, h->stream));
   HIPCHK(h, hipMemsetAsync(badop, 0, 4, h->stream));
   const bool cand_kept = h->cand.valid && h->cand.ml == ml && h->cand.bytes_model == cfg->bytes_model &&
@@ -3694,9 +3729,29 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
   // pm_masked (the masked compare's P mask) is defined with the candidates
   float ms_qc = 0, ms_mid = 0, ms_exh = 0, ms_setup = 0;
   HIPCHK(h, hipEventRecord(es, h->stream));  // end of the setup; read after the last sync
+  auto harvest_stage_times = [&](int set) -> int {
+    if (!seg_evs_pending[set]) return SX_OK;
+    float a = 0, b = 0, c = 0;
+    HIPCHK(h, hipEventSynchronize(seg_evs[set][3]));
+    HIPCHK(h, hipEventElapsedTime(&a, seg_evs[set][0], seg_evs[set][1]));
+    HIPCHK(h, hipEventElapsedTime(&b, seg_evs[set][1], seg_evs[set][2]));
+    HIPCHK(h, hipEventElapsedTime(&c, seg_evs[set][2], seg_evs[set][3]));
+    ms_qc += a;
+    ms_mid += b;
+    ms_exh += c;
+    seg_evs_pending[set] = false;
+    return SX_OK;
+  };
 
-  for (uint64_t s0 = 0; s0 < nt_all; s0 += seg) {
+  uint64_t seg_no = 0;
+  for (uint64_t s0 = 0; s0 < nt_all; s0 += seg, seg_no++) {
     uint64_t nt = std::min<uint64_t>(seg, nt_all - s0);
+    const int evset = (int)(seg_no & 1);
+    {
+      int rc = harvest_stage_times(evset);
+      if (rc) return rc;
+    }
+    hipEvent_t e1 = seg_evs[evset][0], e2 = seg_evs[evset][1], e3 = seg_evs[evset][2], e4 = seg_evs[evset][3];
     uint32_t *traw = nullptr;  // implicit length 3: offset of each target in the raw range
     if (implicit3) {
       uint32_t *tw3, *counts;
@@ -3790,9 +3845,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
           uint32_t c[2];
           unsigned long long q[2 * kQcCntShards];
         } rb;
-        HIPCHK(h, hipMemcpyAsync(rb.c, counters, 8, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(h, hipMemcpyAsync(rb.q, qsums, sizeof(rb.q), hipMemcpyDeviceToHost, h->stream));
-        SYNC_WD("device sync");
+        rc = read_back(h, {{rb.c, counters, 8}, {rb.q, qsums, sizeof(rb.q)}}, "QuickCheck and MidCheck");
+        if (rc) return rc;
         nmid = rb.c[1];
         if (nmid > mid_cap) {
           st.overflow_retries++;
@@ -3840,8 +3894,10 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
           if (rc) return rc;
         }
         uint32_t cnt8[kQcRegions];
-        HIPCHK(h, hipMemcpyAsync(cnt8, qc_counts, sizeof(cnt8), hipMemcpyDeviceToHost, h->stream));
-        SYNC_WD("device sync");
+        {
+          int rc = read_back(h, {{cnt8, qc_counts, sizeof(cnt8)}}, "QuickCheck");
+          if (rc) return rc;
+        }
         const uint32_t region_cap = surv_cap / kQcRegions;
         uint64_t total = 0;
         uint32_t most = 0;
@@ -3881,8 +3937,10 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
                              std::max<uint32_t>(nsurv, 1), counters, rank, cfg->hits ? nullptr : cur);
         HIPCHK(h, hipGetLastError());
         uint32_t cnt[2];
-        HIPCHK(h, hipMemcpyAsync(cnt, counters, 8, hipMemcpyDeviceToHost, h->stream));
-        SYNC_WD("device sync");
+        {
+          int rc = read_back(h, {{cnt, counters, 8}}, "MidCheck");
+          if (rc) return rc;
+        }
         st.qc_hits += cnt[0];
         nmid = cnt[1];
         trace(h, "  MidCheck: %u QC hits after filters, %u pass", cnt[0], cnt[1]);
@@ -3895,7 +3953,17 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     // ---- ExhaustiveCheck (rounds for first-hit)
     Job *jobs;
     sx_rule *rules;
-    const uint32_t job_cap = std::max<uint32_t>(nmid, 1);  // the first-hit tail may queue every survivor
+    // every round queues at most one job per open (target, pass) entry; the
+    // first-hit tail queues the untested survivors of the open entries, and
+    // is only chosen (below) while they number at most kBulkRatio x a round's
+    // jobs or kBulkJobs; all hits, or a tail forced after a fixed round
+    // (SX_BULK_AFTER), may queue every survivor
+    constexpr uint32_t kBulkRatio = 4;
+    constexpr uint32_t kBulkJobs = 1u << 22;
+    uint64_t job_bound = nmid;
+    if (!cfg->hits && h->bulk_after < 0)
+      job_bound = std::min<uint64_t>(nmid, std::max<uint64_t>((uint64_t)kBulkRatio * 2 * nt, kBulkJobs));
+    const uint32_t job_cap = (uint32_t)std::max<uint64_t>(job_bound, 1);
     ENSURE(B_JOBS, Job, job_cap, jobs);
     uint32_t *jchunks, *jfail;  // written by the job kernels, read by the sweep
     ENSURE(B_CHUNKS, uint32_t, job_cap + 1, jchunks);
@@ -3910,8 +3978,6 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     // SX_BULK_AFTER is set.  Rounds stay in use while a batch would sweep far
     // more pairs than the first hits need (at --dead-flags 0xFF the first
     // round resolves most targets out of ~470M survivors).
-    constexpr uint32_t kBulkRatio = 4;
-    constexpr uint32_t kBulkJobs = 1u << 22;
     uint32_t *nrest_d = counters + 4;
     bool go_bulk = false;
     for (int round = 0; nmid > 0; round++) {
@@ -3928,8 +3994,10 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
                            jchunks, jfail);
         HIPCHK(h, hipGetLastError());
         uint32_t nj = 0;
-        HIPCHK(h, hipMemcpyAsync(&nj, njobs_d, 4, hipMemcpyDeviceToHost, h->stream));
-        SYNC_WD("device sync");
+        {
+          int rc = read_back(h, {{&nj, njobs_d, 4}}, "first-hit tail jobs");
+          if (rc) return rc;
+        }
         if (nj == 0) break;
         if (nj > job_cap) return set_err(h, SX_ENOMEM, "job buffer overflow (%u > %u)", nj, job_cap);
         trace(h, "  first-hit tail: %u jobs", nj);
@@ -3964,8 +4032,10 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
       }
       HIPCHK(h, hipGetLastError());
       uint32_t cnt2[2] = {0, 0};
-      HIPCHK(h, hipMemcpyAsync(cnt2, njobs_d, 8, hipMemcpyDeviceToHost, h->stream));
-      SYNC_WD("device sync");
+      {
+        int rc = read_back(h, {{cnt2, njobs_d, 8}}, "round jobs");
+        if (rc) return rc;
+      }
       if (first_round) cnt2[1] = nmid;
       uint32_t nj = cnt2[0];
       if (nj == 0) break;
@@ -4013,8 +4083,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
       int rc = scan_u32(h, rule_flag, nt * 2, roffs, rtot);
       if (rc) return rc;
       uint64_t tot = 0;
-      HIPCHK(h, hipMemcpyAsync(&tot, rtot, 8, hipMemcpyDeviceToHost, h->stream));
-      SYNC_WD("device sync");
+      rc = read_back(h, {{&tot, rtot, 8}}, "rule count");
+      if (rc) return rc;
       nr = (uint32_t)tot;
       ENSURE(B_RULES2, sx_rule, std::max<uint32_t>(nr, 1), packed);
       hipLaunchKernelGGL(k_compact_rules, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, rules, rule_flag,
@@ -4023,7 +4093,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
       rules_out = packed;
     }
     HIPCHK(h, hipEventRecord(e4, h->stream));
-    SYNC_WD("device sync");
+    seg_evs_pending[evset] = true;
+    if (cfg->hits) SYNC_WD("device sync");  // nr
     if (nr > rule_cap) return set_err(h, SX_ENOMEM, "rule buffer overflow (%u rules)", nr);
     if (implicit3 && nr) {
       // rules name their target by its raw Go-order triple index
@@ -4129,29 +4200,23 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         }
       }
     }
-    float a = 0, b = 0, c = 0;
-    HIPCHK(h, hipEventElapsedTime(&a, e1, e2));
-    HIPCHK(h, hipEventElapsedTime(&b, e2, e3));
-    HIPCHK(h, hipEventElapsedTime(&c, e3, e4));
-    ms_qc += a;
-    ms_mid += b;
-    ms_exh += c;
     st.targets += nt;
   }
   uint32_t bad = 0;
-  HIPCHK(h, hipMemcpyAsync(&bad, badop, 4, hipMemcpyDeviceToHost, h->stream));
   {
-    int rc = read_evaluated(h, &st.exh_states, &st.exh_group_states, &st.exh_group_jobs);  // synchronises
+    int rc = read_evaluated(h, &st.exh_states, &st.exh_group_states, &st.exh_group_jobs, &bad, badop);  // synchronises
     if (rc) return rc;
   }
   HIPCHK(h, hipEventSynchronize(es));  // long complete unless there were no targets
   HIPCHK(h, hipEventElapsedTime(&ms_setup, e0, es));
+  for (int set = 0; set < 2; set++) {
+    int rc = harvest_stage_times(set);
+    if (rc) return rc;
+  }
   (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipEventDestroy(e2);
-  (void)hipEventDestroy(e3);
-  (void)hipEventDestroy(e4);
   (void)hipEventDestroy(es);
+  for (auto &set : seg_evs)
+    for (auto &ev : set) (void)hipEventDestroy(ev);
   if (bad) return set_err(h, SX_EOPCODE, "opcode out of range in a target");
   // first hit: segments in order, slots compacted in (target, pass) order, so
   // the rules are already sorted; all hits: sort on the host

@@ -337,6 +337,7 @@ def main():
     ap.add_argument("--no-ff", action="store_true", help="skip the dead 0xFF enumeration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="skip the batched length-4 section")
     ap.add_argument("--no-cuda", action="store_true", help="skip the CUDA-tool-semantics run")
     ap.add_argument("--no-stoke", action="store_true")
     ap.add_argument("--stoke-steps", type=int, default=1_000_000)
@@ -453,7 +454,8 @@ def main():
         out["c2_deadFF"] = c2_dead_section(dev, 0xFF)
     if not args.no_c3:
         out["c3"] = c3_section(dev)
-        out["c4_batch"] = c4_section(dev)
+        if not args.no_c4:
+            out["c4_batch"] = c4_section(dev)
         out["target"] = target_section(dev)
     if not args.no_stoke:
         out["stoke"] = stoke_section(dev, args.stoke_steps, args.train_steps, args.harvest_targets)
