@@ -320,16 +320,25 @@ constexpr uint32_t kFpPerWave = 16;
 constexpr uint32_t kFpVecs = 8;  // vectors kept per sequence: the QuickCheck ones
 static_assert(kFpPerWave * 4 <= kWave, "sequence words (len <= 4) must fit one per lane");
 static_assert(kFpPerWave % 2 == 0 && kFpPerWave <= 32, "sequences go two per pass; masks are 32-bit");
-__global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uint32_t len,
-                                                      uint64_t n, int bytes_model, uint2 *fp, uint2 *mh,
-                                                      uint32_t *th, uint32_t lo_mask, uint8_t *bytes, uint8_t *cyc,
-                                                      uint8_t *prune,
-                                                      uint32_t *bad_op) {
-  const uint64_t first = ((uint64_t)blockIdx.x * 4 + uni(threadIdx.x >> 6)) * kFpPerWave;  // wave-uniform
-  if (first >= n) return;
+// With a list (fp_batch's `list`), the wave's sequences are list[first ..
+// first + cnt) instead of first .. first + cnt; only mh is written (fp, th,
+// bytes, cycles and the pruner flag are null).
+__device__ __forceinline__ void fp_batch(const uint32_t *words, const uint32_t *list, uint32_t len, uint64_t first,
+                                         uint64_t n, int bytes_model, uint2 *fp, uint2 *mh, uint32_t *th,
+                                         uint32_t lo_mask, uint8_t *bytes, uint8_t *cyc, uint8_t *prune,
+                                         uint32_t *bad_op) {
   const uint32_t lane = lane_id(), v = lane & 31, half = lane >> 5;
   const uint32_t cnt = (uint32_t)(first + kFpPerWave < n ? kFpPerWave : n - first);
-  const uint32_t wv = lane < cnt * len ? words[first * len + lane] : 0u;  // len <= 4
+  uint32_t wv = 0;  // len <= 4
+  uint64_t my_seq = 0;  // lane q < cnt: the index of sequence q
+  if (lane < cnt) my_seq = list ? list[first + lane] : first + lane;
+  if (list && len) {
+    const uint32_t j = lane / len;
+    const uint64_t sj = (uint64_t)__shfl((int)(uint32_t)my_seq, (int)j);
+    if (lane < cnt * len) wv = words[sj * len + (lane - j * len)];
+  } else if (lane < cnt * len) {
+    wv = words[first * len + lane];
+  }
   // lane q < cnt: flags of sequence q (the other lanes read sequence 0 and
   // are masked out of the ballots)
   const uint32_t cmask = cnt >= 32 ? 0xFFFFFFFFu : (1u << cnt) - 1u;
@@ -379,7 +388,7 @@ __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uin
         }
         st.p &= kPMask;
         const uint2 o = make_uint2(pack_lo(st), pack_hi(st));
-        if (v < kFpVecs) fp[(first + sq) * kFpVecs + v] = o;
+        if (fp && v < kFpVecs) fp[(first + sq) * kFpVecs + v] = o;
         // one mix per lane: QuickCheck vectors 0..3 under the compare mask,
         // Mid vectors 8..31 exact (the masked Mid hash is a second mix, only
         // when the masks differ)
@@ -415,11 +424,111 @@ __global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uin
     }
   }
   if (lane < cnt && !((bad_seqs >> lane) & 1u)) {
-    mh[first + lane] = make_uint2(g_hx, g_hm);
+    mh[my_seq] = make_uint2(g_hx, g_hm);
     if (th) th[first + lane] = g_th;
   }
   const bool bad_any = bad_seqs != 0;
   if (bad_any && lane == 0) atomicOr(bad_op, 1u);
+  const uint64_t seq = first + lane;
+  if (bytes && lane < cnt) {
+    uint32_t w[4] = {0, 0, 0, 0};
+    bool bad = false;
+    for (uint32_t k = 0; k < len; k++) {
+      w[k] = words[seq * len + k];
+      bad |= (w[k] & 0xFFFFu) >= OP_COUNT;
+    }
+    if (!bad) {
+      int b = 0, c = 0;
+      for (uint32_t k = 0; k < len; k++) {
+        b += byte_size(w[k] & 0xFFFFu, bytes_model);
+        c += cycles(w[k] & 0xFFFFu);
+      }
+      bytes[seq] = (uint8_t)b;
+      cyc[seq] = (uint8_t)c;
+      if (prune) prune[seq] = should_prune(w, (int)len, c_reads, c_writes) ? 1 : 0;
+    }
+  }
+}
+__global__ __launch_bounds__(256) void k_fingerprints(const uint32_t *words, uint32_t len,
+                                                      uint64_t n, int bytes_model, uint2 *fp, uint2 *mh,
+                                                      uint32_t *th, uint32_t lo_mask, uint8_t *bytes, uint8_t *cyc,
+                                                      uint8_t *prune,
+                                                      uint32_t *bad_op) {
+  const uint64_t first = ((uint64_t)blockIdx.x * 4 + uni(threadIdx.x >> 6)) * kFpPerWave;  // wave-uniform
+  if (first >= n) return;
+  fp_batch(words, nullptr, len, first, n, bytes_model, fp, mh, th, lo_mask, bytes, cyc, prune, bad_op);
+}
+// The Mid hashes of the sequences list[0 .. *list_n), waves striding over the
+// list (its length is known on the device only).
+__global__ __launch_bounds__(256) void k_fp_mid_list(const uint32_t *words, const uint32_t *list,
+                                                     const uint32_t *list_n, uint32_t len, uint2 *mh, uint32_t lo_mask,
+                                                     uint32_t *bad_op) {
+  const uint64_t n = uni(*list_n);
+  const uint64_t stride = (uint64_t)gridDim.x * 4 * kFpPerWave;
+  for (uint64_t first = ((uint64_t)blockIdx.x * 4 + uni(threadIdx.x >> 6)) * kFpPerWave; first < n;
+       first += stride)
+    fp_batch(words, list, len, first, n, 0, nullptr, mh, nullptr, lo_mask, nullptr, nullptr, nullptr, bad_op);
+}
+
+// Outputs on the 8 QuickCheck vectors only (fp as k_fingerprints writes it),
+// eight sequences per pass: lane group g = lane / 8 runs sequence q + g of a
+// run of sequences with equal opcodes (per-lane immediates), lane v = lane %
+// 8 its vector v.  Runs of one first instruction share their prefix as in
+// k_fingerprints.  bytes, cycles, the pruner flag and bad_op as there.
+constexpr uint32_t kFpqPerWave = 16;
+static_assert(kFpqPerWave * 4 <= kWave, "sequence words (len <= 4) must fit one per lane");
+__global__ __launch_bounds__(256) void k_fp_qc(const uint32_t *__restrict__ words, uint32_t len, uint64_t n,
+                                               int bytes_model, uint2 *__restrict__ fp, uint8_t *bytes, uint8_t *cyc,
+                                               uint8_t *prune, uint32_t *bad_op) {
+  const uint64_t first = ((uint64_t)blockIdx.x * 4 + uni(threadIdx.x >> 6)) * kFpqPerWave;  // wave-uniform
+  if (first >= n) return;
+  const uint32_t lane = lane_id(), v = lane & 7, g = lane >> 3;
+  const uint32_t cnt = (uint32_t)(first + kFpqPerWave < n ? kFpqPerWave : n - first);
+  const uint32_t wv = lane < cnt * len ? words[first * len + lane] : 0u;  // len <= 4
+  const uint32_t cmask = (1u << cnt) - 1u;
+  uint32_t bad_seqs, same_next;
+  bool share;
+  {
+    const uint32_t q = lane < cnt ? lane : 0u, qn = q + 1 < cnt ? q + 1 : q;
+    bool l_bad = false, l_same = q + 1 < cnt, l_pre = true;
+    for (uint32_t k = 0; k < len; k++) {
+      const uint32_t w = (uint32_t)__shfl((int)wv, (int)(q * len + k));
+      const uint32_t wn = (uint32_t)__shfl((int)wv, (int)(qn * len + k));
+      l_bad |= (w & 0xFFFFu) >= OP_COUNT;
+      l_same &= (w & 0xFFFFu) == (wn & 0xFFFFu);
+      if (k + 1 < len) l_pre &= w == __builtin_amdgcn_readlane(wv, k);
+    }
+    bad_seqs = (uint32_t)ballot(l_bad) & cmask;
+    same_next = (uint32_t)ballot(l_same) & cmask;
+    share = len > 1 && ((uint32_t)ballot(l_pre) & cmask) == cmask;
+  }
+  State base = load_vec(v);
+  uint32_t k0 = 0;
+  if (share) {
+    for (uint32_t k = 0; k + 1 < len; k++) {
+      const uint32_t w = __builtin_amdgcn_readlane(wv, k);
+      step(base, w & 0xFFFFu, w >> 16);
+    }
+    k0 = len - 1;
+  }
+  for (uint32_t q = 0; q < cnt;) {
+    // sequences q .. q + r - 1 have the opcodes of q (so all or none is bad)
+    uint32_t r = (uint32_t)__builtin_ctz(~(same_next >> q)) + 1u;
+    r = min(min(r, 8u), cnt - q);
+    if (!((bad_seqs >> q) & 1u)) {
+      const uint32_t sq = q + min(g, r - 1u);
+      State st = base;
+      for (uint32_t k = k0; k < len; k++) {
+        const uint32_t op = __builtin_amdgcn_readlane(wv, q * len + k) & 0xFFFFu;
+        const uint32_t w = (uint32_t)__shfl((int)wv, (int)(sq * len + k));
+        step(st, op, w >> 16);
+      }
+      st.p &= kPMask;
+      if (g < r) fp[(first + q + g) * kFpVecs + v] = make_uint2(pack_lo(st), pack_hi(st));
+    }
+    q += r;
+  }
+  if (bad_seqs && lane == 0) atomicOr(bad_op, 1u);
   const uint64_t seq = first + lane;
   if (lane < cnt) {
     uint32_t w[4] = {0, 0, 0, 0};
@@ -959,16 +1068,21 @@ __global__ __launch_bounds__(256) void k_class_verify(const uint32_t *__restrict
 constexpr uint32_t kQcCntShards = 64;
 // QuickCheck's counters of a segment from the classes: sums[2 s] += raw
 // survivors, sums[2 s + 1] += survivors after the byte filter and the pruner
-// (shard s = block mod 64).
+// (shard s = block mod 64).  list (when given): the targets whose masked
+// QuickCheck outputs equal some candidate's (every other target has no
+// QuickCheck survivor), appended per wave in target order at
+// list[atomicAdd(list_n)].
 __global__ __launch_bounds__(256) void k_qc_count(const uint2 *__restrict__ tfp, const uint8_t *__restrict__ tbytes,
                                                   uint32_t nt, uint32_t lo_mask,
                                                   const unsigned long long *__restrict__ ckey,
                                                   const uint64_t *__restrict__ ncls_p,
                                                   const uint32_t *__restrict__ crep, const uint32_t *__restrict__ ccnt,
-                                                  const uint2 *__restrict__ cfp, unsigned long long *sums) {
+                                                  const uint2 *__restrict__ cfp, unsigned long long *sums,
+                                                  uint32_t *list, uint32_t *list_n) {
   __shared__ unsigned long long s_raw[4], s_hit[4];
   const uint32_t t = blockIdx.x * 256 + threadIdx.x, lane = lane_id(), w = threadIdx.x >> 6;
   uint64_t raw = 0, hit = 0;
+  bool in_class = false;
   if (t < nt) {
     uint2 o[8];
     load_fp8(tfp, t, o);
@@ -987,6 +1101,7 @@ __global__ __launch_bounds__(256) void k_qc_count(const uint2 *__restrict__ tfp,
       load_fp8(cfp, crep[lo], r);
       bool okm, okf;
       fp_compare(o, r, lo_mask, okm, okf);
+      in_class = okm;
       if (okm) {
         const uint32_t *c = ccnt + (uint64_t)lo * (kClsBins + 1);
         raw = c[kClsBins];
@@ -994,6 +1109,13 @@ __global__ __launch_bounds__(256) void k_qc_count(const uint2 *__restrict__ tfp,
         for (uint32_t b = 0; b < tb; b++) hit += c[b];
       }
     }
+  }
+  if (list) {
+    const uint64_t m = ballot(in_class);
+    uint32_t base = 0;
+    if (m && lane == 0) base = atomicAdd(list_n, (uint32_t)__popcll(m));
+    base = uni(base);
+    if (in_class) list[base + mbcnt(m)] = t;
   }
   raw = wave_sum64(raw);
   hit = wave_sum64(hit);
@@ -1027,9 +1149,10 @@ __global__ __launch_bounds__(256) void k_qcm_join(const uint2 *__restrict__ tfp,
                                                   const uint8_t *__restrict__ cprune, int hits_all,
                                                   uint2 *__restrict__ out, uint32_t out_cap,
                                                   uint32_t *__restrict__ counters, const uint32_t *__restrict__ rank,
-                                                  uint32_t *cur0) {
+                                                  uint32_t *cur0, const uint32_t *__restrict__ list,
+                                                  const uint32_t *__restrict__ list_n) {
   __shared__ uint2 queue[4][kQcQueue];
-  __shared__ uint32_t s_pre[4][kWave], s_lo[4][kWave], s_key[4][kWave];
+  __shared__ uint32_t s_pre[4][kWave], s_lo[4][kWave], s_key[4][kWave], s_tid[4][kWave];
   const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
   uint2 *q = queue[w];
   uint32_t qn = 0;
@@ -1044,9 +1167,13 @@ __global__ __launch_bounds__(256) void k_qcm_join(const uint2 *__restrict__ tfp,
     __builtin_amdgcn_wave_barrier();
     qn = 0;
   };
-  const uint32_t t0 = (blockIdx.x * 4 + w) * kWave;
-  const uint32_t t = t0 + lane;
-  const bool have = t < nt;
+  // without a list: targets 0 .. nt, 64 per wave; with one: list[0 ..
+  // *list_n), 64 entries per wave, the waves striding over the list
+  const uint32_t n = list ? uni(*list_n) : nt;
+  const uint32_t stride = list ? gridDim.x * 4 * kWave : 0u;
+  for (uint32_t i0 = (blockIdx.x * 4 + w) * kWave; i0 < n; i0 += stride) {
+  const bool have = i0 + lane < n;
+  const uint32_t t = !have ? 0u : list ? list[i0 + lane] : i0 + lane;
   uint32_t jk = 0;
   if (have) {
     uint2 o[8];
@@ -1059,9 +1186,11 @@ __global__ __launch_bounds__(256) void k_qcm_join(const uint2 *__restrict__ tfp,
   const uint32_t lo = have ? dir[b] : 0u, c = have ? dir[b + 1] - lo : 0u;
   const uint32_t pre = wave_excl_scan(c, lane);
   const uint32_t total = __builtin_amdgcn_readlane(pre + c, kWave - 1);
+  __builtin_amdgcn_wave_barrier();
   s_pre[w][lane] = pre;
   s_lo[w][lane] = lo;
   s_key[w][lane] = jk;
+  s_tid[w][lane] = t;
   __builtin_amdgcn_wave_barrier();
   for (uint32_t base = 0; base < total; base += kWave) {
     const uint32_t i = base + lane;
@@ -1074,7 +1203,7 @@ __global__ __launch_bounds__(256) void k_qcm_join(const uint2 *__restrict__ tfp,
         if (s_pre[w][o + step] <= i) o += step;
       const uint32_t e = s_lo[w][o] + (i - s_pre[w][o]);
       if (sj[e] == s_key[w][o]) {
-        tt = t0 + o;
+        tt = s_tid[w][o];
         slot = sjslot[e];
         uint2 a[8], bb[8];
         load_fp8(tfp, tt, a);
@@ -1095,6 +1224,8 @@ __global__ __launch_bounds__(256) void k_qcm_join(const uint2 *__restrict__ tfp,
       if (qn > kQcQueue - kWave) flush();
     }
     if (cur0) wave_run_min(ok ? tt * 2 + (pass - 1) : kRankNone, ok ? rank[slot] : 0u, cur0);
+  }
+  if (!list) break;
   }
   if (qn) flush();
 }
@@ -2576,6 +2707,7 @@ struct sx_handle {
   int exh_k = 2;          // states (pairs when packed) per lane and opcode dispatch in the sweep (SX_EXH_K: 1 or 2)
   int exh_pk = 1;         // packed two-states-per-register interpreter in the sweep (SX_EXH_PK=0: one state)
   int qcm_join = 1;       // QuickCheck and MidCheck as one join when MidCheck is on (SX_QCM_JOIN=0: separate)
+  int fp_split = 1;       // the join's targets: QuickCheck outputs first, Mid hashes of class members (SX_FP_SPLIT)
   // the candidate side (words, fingerprints, hash tables, classes) is kept
   // between enumerations while the candidate length, byte model and compare
   // mask stay the same
@@ -2657,7 +2789,7 @@ enum BufId {
   B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_PLANTAB, B_EFP, B_ERULES, B_L3_LIST, B_L3_HK, B_L3_CNT,
   B_L3_KEYS, B_L3_HEAD, B_L3_NEXT, B_L3_BLOOM, B_L3_RAW, B_L3_C3, B_L3_BEST, B_GKEY, B_GIDX, B_GKEY_S, B_GIDX_S,
   B_GSORT, B_QJ_KEY, B_QJ_VAL, B_QJ_HASH, B_QJ_SLOT, B_QJ_DIR, B_CK, B_CKS, B_CV1,
-  B_CV1S, B_JK, B_JV, B_JKS, B_JVS, B_JDIR, B_CHEAD, B_CHOFF, B_CLSKEY, B_CLSREP, B_CLSCNT, B_CLSN, B_QCSUM, B_COUNT
+  B_CV1S, B_JK, B_JV, B_JKS, B_JVS, B_JDIR, B_CHEAD, B_CHOFF, B_CLSKEY, B_CLSREP, B_CLSCNT, B_CLSN, B_QCSUM, B_TLIST, B_COUNT
 };
 static_assert(B_COUNT <= 96, "sx_handle::bufs holds every buffer id");
 
@@ -3394,6 +3526,7 @@ extern "C" int sx_open(int device, sx_handle **out) {
   if (const char *e = getenv("SX_EXH_PK")) h->exh_pk = atoi(e) != 0;
   if (const char *e = getenv("SX_EXH_GROUP")) h->exh_group = atoi(e);
   if (const char *e = getenv("SX_QCM_JOIN")) h->qcm_join = atoi(e) != 0;
+  if (const char *e = getenv("SX_FP_SPLIT")) h->fp_split = atoi(e) != 0;
   if (const char *e = getenv("SX_SEG_LOG")) h->seg_log = std::min(24, std::max(16, atoi(e)));
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking) != hipSuccess ||
@@ -3792,8 +3925,18 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     } else {
       HIPCHK(h, hipMemcpyAsync(tw, d_tsrc + s0 * 2, nt * 8, hipMemcpyDeviceToDevice, h->stream));
     }
-    hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(nt, 4 * kFpPerWave)), dim3(256), 0, h->stream, tw, (uint32_t)tl,
-                       nt, cfg->bytes_model, tfp, tmh, tv0, lo_mask, tbytes, tcyc, nullptr, badop);
+    // the joined QuickCheck + MidCheck needs the Mid hashes only of targets
+    // whose masked QuickCheck outputs equal some candidate's (about 15% of
+    // the length-2 targets at --dead-flags 0x00): their QuickCheck outputs
+    // first, eight targets per pass, the Mid hashes after the class lookup;
+    // the empty candidate reads every target's Mid hashes
+    const bool fp_split = fused && h->fp_split && !cfg->include_empty;
+    if (fp_split)
+      hipLaunchKernelGGL(k_fp_qc, dim3(blocks_for(nt, 4 * kFpqPerWave)), dim3(256), 0, h->stream, tw, (uint32_t)tl,
+                         nt, cfg->bytes_model, tfp, tbytes, tcyc, nullptr, badop);
+    else
+      hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(nt, 4 * kFpPerWave)), dim3(256), 0, h->stream, tw,
+                         (uint32_t)tl, nt, cfg->bytes_model, tfp, tmh, tv0, lo_mask, tbytes, tcyc, nullptr, badop);
     HIPCHK(h, hipGetLastError());
 
     // the rounds' per-(target, pass) state, cleared while the host waits for
@@ -3812,14 +3955,23 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
       // ---- QuickCheck's counters from the classes, then QuickCheck and
       // MidCheck as one join (re-run with a larger buffer on overflow)
       unsigned long long *qsums;
-      ENSURE(B_QCSUM, unsigned long long, 2 * kQcCntShards, qsums);
-      HIPCHK(h, hipMemsetAsync(qsums, 0, 2 * kQcCntShards * 8, h->stream));
+      uint32_t *tlist = nullptr, *tlist_n = nullptr;  // fp_split: the targets with a QuickCheck class
+      ENSURE(B_QCSUM, unsigned long long, 2 * kQcCntShards + 1, qsums);
+      if (fp_split) {
+        ENSURE(B_TLIST, uint32_t, nt, tlist);
+        tlist_n = reinterpret_cast<uint32_t *>(qsums + 2 * kQcCntShards);
+      }
+      HIPCHK(h, hipMemsetAsync(qsums, 0, (2 * kQcCntShards + 1) * 8, h->stream));
       {
         size_t ts = 0;
         int rc = timed_begin(h, 0, &ts);
         if (rc) return rc;
         hipLaunchKernelGGL(k_qc_count, dim3(blocks_for(nt, 256)), dim3(256), 0, h->stream, tfp, tbytes, (uint32_t)nt,
-                           lo_mask, cls_key, cls_n, cls_rep, cls_cnt, cfp, qsums);
+                           lo_mask, cls_key, cls_n, cls_rep, cls_cnt, cfp, qsums, tlist, tlist_n);
+        HIPCHK(h, hipGetLastError());
+        if (fp_split)
+          hipLaunchKernelGGL(k_fp_mid_list, dim3(std::min(blocks_for(nt, 4 * kFpPerWave), 8u * h->num_cu)), dim3(256), 0,
+                             h->stream, tw, tlist, tlist_n, (uint32_t)tl, tmh, lo_mask, badop);
         HIPCHK(h, hipGetLastError());
         rc = timed_end(h, ts);
         if (rc) return rc;
@@ -3835,9 +3987,11 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         size_t ts = 0;
         int rc = timed_begin(h, 0, &ts);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_qcm_join, dim3(blocks_for(nt, 256)), dim3(256), 0, h->stream, tfp, tmh, tbytes,
-                           (uint32_t)nt, lo_mask, j_key, j_slot, j_dir, j_dk, cfp, cmh, cbytes, cprune, cfg->hits, mid,
-                           mid_cap, counters, rank, cfg->hits ? nullptr : cur);
+        hipLaunchKernelGGL(k_qcm_join,
+                           dim3(fp_split ? std::min(blocks_for(nt, 256), 16u * h->num_cu) : blocks_for(nt, 256)),
+                           dim3(256), 0, h->stream, tfp, tmh, tbytes, (uint32_t)nt, lo_mask, j_key, j_slot, j_dir,
+                           j_dk, cfp, cmh, cbytes, cprune, cfg->hits, mid, mid_cap, counters, rank,
+                           cfg->hits ? nullptr : cur, tlist, tlist_n);
         HIPCHK(h, hipGetLastError());
         rc = timed_end(h, ts);
         if (rc) return rc;
