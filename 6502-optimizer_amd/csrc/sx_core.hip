@@ -475,17 +475,18 @@ __global__ __launch_bounds__(256) void k_fp_mid_list(const uint32_t *words, cons
 // run of sequences with equal opcodes (per-lane immediates), lane v = lane %
 // 8 its vector v.  Runs of one first instruction share their prefix as in
 // k_fingerprints.  bytes, cycles, the pruner flag and bad_op as there.
-constexpr uint32_t kFpqPerWave = 16;
-static_assert(kFpqPerWave * 4 <= kWave, "sequence words (len <= 4) must fit one per lane");
+// A wave takes fpq_per_wave(len) sequences (their words one per lane).
+__host__ __device__ constexpr uint32_t fpq_per_wave(uint32_t len) { return len <= 2 ? 32u : 16u; }
 __global__ __launch_bounds__(256) void k_fp_qc(const uint32_t *__restrict__ words, uint32_t len, uint64_t n,
                                                int bytes_model, uint2 *__restrict__ fp, uint8_t *bytes, uint8_t *cyc,
                                                uint8_t *prune, uint32_t *bad_op) {
-  const uint64_t first = ((uint64_t)blockIdx.x * 4 + uni(threadIdx.x >> 6)) * kFpqPerWave;  // wave-uniform
+  const uint32_t per = fpq_per_wave(len);  // len <= 4
+  const uint64_t first = ((uint64_t)blockIdx.x * 4 + uni(threadIdx.x >> 6)) * per;  // wave-uniform
   if (first >= n) return;
   const uint32_t lane = lane_id(), v = lane & 7, g = lane >> 3;
-  const uint32_t cnt = (uint32_t)(first + kFpqPerWave < n ? kFpqPerWave : n - first);
-  const uint32_t wv = lane < cnt * len ? words[first * len + lane] : 0u;  // len <= 4
-  const uint32_t cmask = (1u << cnt) - 1u;
+  const uint32_t cnt = (uint32_t)(first + per < n ? per : n - first);
+  const uint32_t wv = lane < cnt * len ? words[first * len + lane] : 0u;
+  const uint32_t cmask = cnt >= 32 ? 0xFFFFFFFFu : (1u << cnt) - 1u;
   uint32_t bad_seqs, same_next;
   bool share;
   {
@@ -1070,8 +1071,10 @@ constexpr uint32_t kQcCntShards = 64;
 // survivors, sums[2 s + 1] += survivors after the byte filter and the pruner
 // (shard s = block mod 64).  list (when given): the targets whose masked
 // QuickCheck outputs equal some candidate's (every other target has no
-// QuickCheck survivor), appended per wave in target order at
-// list[atomicAdd(list_n)].
+// QuickCheck survivor), appended per block in target order at
+// list[atomicAdd(list_n)] (one atomic per block: per-wave atomics on the one
+// counter serialise).
+constexpr uint32_t kQcCntChunks = 4;  // 256-target chunks per k_qc_count block
 __global__ __launch_bounds__(256) void k_qc_count(const uint2 *__restrict__ tfp, const uint8_t *__restrict__ tbytes,
                                                   uint32_t nt, uint32_t lo_mask,
                                                   const unsigned long long *__restrict__ ckey,
@@ -1080,42 +1083,43 @@ __global__ __launch_bounds__(256) void k_qc_count(const uint2 *__restrict__ tfp,
                                                   const uint2 *__restrict__ cfp, unsigned long long *sums,
                                                   uint32_t *list, uint32_t *list_n) {
   __shared__ unsigned long long s_raw[4], s_hit[4];
-  const uint32_t t = blockIdx.x * 256 + threadIdx.x, lane = lane_id(), w = threadIdx.x >> 6;
+  __shared__ uint32_t s_in[kQcCntChunks][4], s_base;
+  const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+  const uint32_t ncls = (uint32_t)*ncls_p;
   uint64_t raw = 0, hit = 0;
-  bool in_class = false;
-  if (t < nt) {
-    uint2 o[8];
-    load_fp8(tfp, t, o);
-    uint32_t k1, k2;
-    qc_keys(o, lo_mask, k1, k2);
-    const unsigned long long k = (unsigned long long)k1 | ((unsigned long long)k2 << 32);
-    const uint32_t ncls = (uint32_t)*ncls_p;
-    uint32_t lo = 0, hi = ncls;  // lower bound of k
-    while (lo < hi) {
-      const uint32_t mid = lo + (hi - lo) / 2;
-      if (ckey[mid] < k) lo = mid + 1;
-      else hi = mid;
-    }
-    if (lo < ncls && ckey[lo] == k) {
-      uint2 r[8];
-      load_fp8(cfp, crep[lo], r);
-      bool okm, okf;
-      fp_compare(o, r, lo_mask, okm, okf);
-      in_class = okm;
-      if (okm) {
-        const uint32_t *c = ccnt + (uint64_t)lo * (kClsBins + 1);
-        raw = c[kClsBins];
-        const uint32_t tb = min((uint32_t)tbytes[t], kClsBins);
-        for (uint32_t b = 0; b < tb; b++) hit += c[b];
+  uint64_t m[kQcCntChunks];  // per chunk: the wave's targets with a class
+#pragma unroll
+  for (uint32_t ch = 0; ch < kQcCntChunks; ch++) {
+    const uint32_t t = (blockIdx.x * kQcCntChunks + ch) * 256 + threadIdx.x;
+    bool in_class = false;
+    if (t < nt) {
+      uint2 o[8];
+      load_fp8(tfp, t, o);
+      uint32_t k1, k2;
+      qc_keys(o, lo_mask, k1, k2);
+      const unsigned long long k = (unsigned long long)k1 | ((unsigned long long)k2 << 32);
+      uint32_t lo = 0, hi = ncls;  // lower bound of k
+      while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (ckey[mid] < k) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < ncls && ckey[lo] == k) {
+        uint2 r[8];
+        load_fp8(cfp, crep[lo], r);
+        bool okm, okf;
+        fp_compare(o, r, lo_mask, okm, okf);
+        in_class = okm;
+        if (okm) {
+          const uint32_t *c = ccnt + (uint64_t)lo * (kClsBins + 1);
+          raw += c[kClsBins];
+          const uint32_t tb = min((uint32_t)tbytes[t], kClsBins);
+          for (uint32_t b = 0; b < tb; b++) hit += c[b];
+        }
       }
     }
-  }
-  if (list) {
-    const uint64_t m = ballot(in_class);
-    uint32_t base = 0;
-    if (m && lane == 0) base = atomicAdd(list_n, (uint32_t)__popcll(m));
-    base = uni(base);
-    if (in_class) list[base + mbcnt(m)] = t;
+    m[ch] = ballot(in_class);
+    if (lane == 0) s_in[ch][w] = (uint32_t)__popcll(m[ch]);
   }
   raw = wave_sum64(raw);
   hit = wave_sum64(hit);
@@ -1124,6 +1128,22 @@ __global__ __launch_bounds__(256) void k_qc_count(const uint2 *__restrict__ tfp,
     s_hit[w] = hit;
   }
   __syncthreads();
+  if (list) {
+    if (threadIdx.x == 0) {
+      uint32_t tot = 0;
+      for (uint32_t ch = 0; ch < kQcCntChunks; ch++) tot += s_in[ch][0] + s_in[ch][1] + s_in[ch][2] + s_in[ch][3];
+      s_base = tot ? atomicAdd(list_n, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t pre = s_base;
+#pragma unroll
+    for (uint32_t ch = 0; ch < kQcCntChunks; ch++) {
+      uint32_t p = pre;
+      for (uint32_t k = 0; k < w; k++) p += s_in[ch][k];
+      if ((m[ch] >> lane) & 1u) list[p + mbcnt(m[ch])] = (blockIdx.x * kQcCntChunks + ch) * 256 + threadIdx.x;
+      pre += s_in[ch][0] + s_in[ch][1] + s_in[ch][2] + s_in[ch][3];
+    }
+  }
   if (threadIdx.x == 0) {
     const uint64_t r = s_raw[0] + s_raw[1] + s_raw[2] + s_raw[3], hh = s_hit[0] + s_hit[1] + s_hit[2] + s_hit[3];
     const uint32_t sh = blockIdx.x & (kQcCntShards - 1);
@@ -3932,7 +3952,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     // the empty candidate reads every target's Mid hashes
     const bool fp_split = fused && h->fp_split && !cfg->include_empty;
     if (fp_split)
-      hipLaunchKernelGGL(k_fp_qc, dim3(blocks_for(nt, 4 * kFpqPerWave)), dim3(256), 0, h->stream, tw, (uint32_t)tl,
+      hipLaunchKernelGGL(k_fp_qc, dim3(blocks_for(nt, 4 * fpq_per_wave((uint32_t)tl))), dim3(256), 0, h->stream, tw, (uint32_t)tl,
                          nt, cfg->bytes_model, tfp, tbytes, tcyc, nullptr, badop);
     else
       hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(nt, 4 * kFpPerWave)), dim3(256), 0, h->stream, tw,
@@ -3966,7 +3986,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         size_t ts = 0;
         int rc = timed_begin(h, 0, &ts);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_qc_count, dim3(blocks_for(nt, 256)), dim3(256), 0, h->stream, tfp, tbytes, (uint32_t)nt,
+        hipLaunchKernelGGL(k_qc_count, dim3(blocks_for(nt, 256 * kQcCntChunks)), dim3(256), 0, h->stream, tfp, tbytes, (uint32_t)nt,
                            lo_mask, cls_key, cls_n, cls_rep, cls_cnt, cfp, qsums, tlist, tlist_n);
         HIPCHK(h, hipGetLastError());
         if (fp_split)
