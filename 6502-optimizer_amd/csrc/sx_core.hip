@@ -33,8 +33,10 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <initializer_list>
@@ -78,6 +80,7 @@ struct SurvRegions {
 };
 constexpr int kEvalShards = 1024;       // shards of the evaluated-states counter
 constexpr size_t kPinBytes = 64 << 10;  // pinned read-back staging per handle
+constexpr uint64_t kCopyPiece = 1u << 16;  // rules per piece of the copy to the host
 
 
 // ------------------------------------------------------ device helpers --
@@ -2728,6 +2731,7 @@ struct sx_handle {
   int exh_pk = 1;         // packed two-states-per-register interpreter in the sweep (SX_EXH_PK=0: one state)
   int qcm_join = 1;       // QuickCheck and MidCheck as one join when MidCheck is on (SX_QCM_JOIN=0: separate)
   int fp_split = 1;       // the join's targets: QuickCheck outputs first, Mid hashes of class members (SX_FP_SPLIT)
+  int fetch_threads = 16;  // host threads of sx_fetch_rules (SX_FETCH_THREADS, 1..64)
   // the candidate side (words, fingerprints, hash tables, classes) is kept
   // between enumerations while the candidate length, byte model and compare
   // mask stay the same
@@ -2772,6 +2776,11 @@ struct sx_handle {
   // copy_stream while the later segments compute
   hipStream_t copy_stream = nullptr;
   hipEvent_t seg_ev = nullptr;
+  // the host copy arrives in pieces of kCopyPiece rules, each with an event
+  // on copy_stream, so that sx_fetch_rules unpacks the first pieces while the
+  // later ones are still on the link; pieces[i] = (first rule, count)
+  std::vector<hipEvent_t> piece_ev;
+  std::vector<std::pair<uint64_t, uint64_t>> pieces;
   uint32_t *h_rules = nullptr;
   uint64_t h_rules_capw = 0;
   // pinned staging block for the counters the host reads back to size the
@@ -2809,7 +2818,7 @@ enum BufId {
   B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_PLANTAB, B_EFP, B_ERULES, B_L3_LIST, B_L3_HK, B_L3_CNT,
   B_L3_KEYS, B_L3_HEAD, B_L3_NEXT, B_L3_BLOOM, B_L3_RAW, B_L3_C3, B_L3_BEST, B_GKEY, B_GIDX, B_GKEY_S, B_GIDX_S,
   B_GSORT, B_QJ_KEY, B_QJ_VAL, B_QJ_HASH, B_QJ_SLOT, B_QJ_DIR, B_CK, B_CKS, B_CV1,
-  B_CV1S, B_JK, B_JV, B_JKS, B_JVS, B_JDIR, B_CHEAD, B_CHOFF, B_CLSKEY, B_CLSREP, B_CLSCNT, B_CLSN, B_QCSUM, B_TLIST, B_COUNT
+  B_CV1S, B_JK, B_JV, B_JKS, B_JVS, B_JDIR, B_CHEAD, B_CHOFF, B_CLSKEY, B_CLSREP, B_CLSCNT, B_CLSN, B_QCSUM, B_TLIST, B_EVSUM, B_COUNT
 };
 static_assert(B_COUNT <= 96, "sx_handle::bufs holds every buffer id");
 
@@ -2981,71 +2990,42 @@ int ensure(sx_handle *h, BufId id, size_t count, T **out) {
 
 inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// Host copy of a large block with up to 16 threads: the destination is
-// usually fresh memory, whose page faults then spread over the threads too.
-void parallel_memcpy(void *dst, const void *src, size_t bytes) {
-  const size_t kPiece = 4u << 20;
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const unsigned nt = (unsigned)std::min<size_t>(std::min(16u, hw), bytes / kPiece);
-  if (nt < 2) {
-    memcpy(dst, src, bytes);
-    return;
-  }
-  const size_t per = ((bytes + nt - 1) / nt + 4095) & ~(size_t)4095;
-  auto piece = [=](unsigned i) {
-    const size_t o = i * per;
-    if (o < bytes) memcpy(static_cast<char *>(dst) + o, static_cast<const char *>(src) + o, std::min(per, bytes - o));
-  };
-  std::vector<std::thread> pool;
-  unsigned i = 1;
-  try {
-    for (; i < nt; i++) pool.emplace_back(piece, i);
-  } catch (...) {  // no thread to be had: the caller copies the rest
-    for (; i < nt; i++) piece(i);
-  }
-  piece(0);
-  for (auto &t : pool) t.join();
-}
-
-// The packed rules of k_pack_rules back into sx_rule records, in pieces on
-// up to 16 threads like parallel_memcpy.
-void parallel_unpack_rules(sx_rule *out, const uint32_t *w, uint64_t n) {
-  auto unpack = [=](uint64_t a, uint64_t b) {
-    for (uint64_t i = a; i < b; i++) {
+// The packed rules of k_pack_rules back into sx_rule records: words 0-1
+// src[0..1], 4 rep[0], 8 the target index, 10 the lengths, dead flags and
+// pass, 11 the savings, the rest 0.  Streaming (non-temporal) 16-byte stores
+// when out is 16-byte aligned: the records are written once and not read
+// back here, and such stores skip reading the destination lines first.
+void unpack_rules(sx_rule *out, const uint32_t *w, uint64_t n) {
+  static_assert(sizeof(sx_rule) == 48 && offsetof(sx_rule, target_index) == 32 && offsetof(sx_rule, src_len) == 40,
+                "sx_rule layout");
+  using v4u = uint32_t __attribute__((ext_vector_type(4)));
+  if ((reinterpret_cast<uintptr_t>(out) & 15) == 0) {
+    v4u *o = reinterpret_cast<v4u *>(out);
+    for (uint64_t i = 0; i < n; i++) {
       const uint32_t *p = w + i * kPackedWords;
-      sx_rule r;
-      memset(&r, 0, sizeof(r));
-      r.src[0] = p[0];
-      r.src[1] = p[1];
-      r.rep[0] = p[2];
-      r.target_index = p[3];
-      r.src_len = (uint8_t)p[4];
-      r.rep_len = (uint8_t)(p[4] >> 8);
-      r.dead_flags = (uint8_t)(p[4] >> 16);
-      r.pass = (uint8_t)(p[4] >> 24);
-      r.bytes_saved = (int16_t)(uint16_t)p[5];
-      r.cycles_saved = (int16_t)(uint16_t)(p[5] >> 16);
-      out[i] = r;
+      __builtin_nontemporal_store(v4u{p[0], p[1], 0u, 0u}, o + 3 * i);
+      __builtin_nontemporal_store(v4u{p[2], 0u, 0u, 0u}, o + 3 * i + 1);
+      __builtin_nontemporal_store(v4u{p[3], 0u, p[4], p[5]}, o + 3 * i + 2);
     }
-  };
-  const uint64_t kPiece = 1u << 16;
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const unsigned nt = (unsigned)std::min<uint64_t>(std::min(16u, hw), n / kPiece);
-  if (nt < 2) {
-    unpack(0, n);
+    std::atomic_thread_fence(std::memory_order_seq_cst);  // the streaming stores before any later use
     return;
   }
-  const uint64_t per = (n + nt - 1) / nt;
-  auto piece = [=](unsigned i) { unpack(std::min<uint64_t>(n, i * per), std::min<uint64_t>(n, (i + 1) * per)); };
-  std::vector<std::thread> pool;
-  unsigned i = 1;
-  try {
-    for (; i < nt; i++) pool.emplace_back(piece, i);
-  } catch (...) {  // no thread to be had: the caller unpacks the rest
-    for (; i < nt; i++) piece(i);
+  for (uint64_t i = 0; i < n; i++) {
+    const uint32_t *p = w + i * kPackedWords;
+    sx_rule r;
+    memset(&r, 0, sizeof(r));
+    r.src[0] = p[0];
+    r.src[1] = p[1];
+    r.rep[0] = p[2];
+    r.target_index = p[3];
+    r.src_len = (uint8_t)p[4];
+    r.rep_len = (uint8_t)(p[4] >> 8);
+    r.dead_flags = (uint8_t)(p[4] >> 16);
+    r.pass = (uint8_t)(p[4] >> 24);
+    r.bytes_saved = (int16_t)(uint16_t)p[5];
+    r.cycles_saved = (int16_t)(uint16_t)(p[5] >> 16);
+    out[i] = r;
   }
-  piece(0);
-  for (auto &t : pool) t.join();
 }
 
 // Exclusive scan of n uint32 counts into uint64 offsets; *total on device.
@@ -3382,6 +3362,23 @@ int run_l3(sx_handle *h, uint32_t nt, const uint32_t *tw, uint32_t tl, const uin
   return SX_OK;
 }
 
+// Sums of the three banks of evaluated-state shards: out[b] = sum of bank b.
+static_assert(kEvalShards == 1024, "one shard per thread of one block");
+__global__ __launch_bounds__(1024) void k_sum_shards(const unsigned long long *ev, unsigned long long *out) {
+  __shared__ unsigned long long s[3][16];
+  const uint32_t t = threadIdx.x, w = t >> 6;
+  for (int b = 0; b < 3; b++) {
+    const uint64_t x = wave_sum64(ev[b * kEvalShards + t]);
+    if (lane_id() == 0) s[b][w] = x;
+  }
+  __syncthreads();
+  if (t < 3) {
+    uint64_t x = 0;
+    for (int k = 0; k < 16; k++) x += s[t][k];
+    out[t] = x;
+  }
+}
+
 // The evaluated-state counters (sharded, three banks: the state-parallel
 // sweep's states, the grouped sweep's states, the grouped sweep's jobs):
 // cleared, and summed on the host.
@@ -3391,22 +3388,24 @@ int clear_evaluated(sx_handle *h) {
   HIPCHK(h, hipMemsetAsync(evald, 0, 3 * kEvalShards * sizeof(unsigned long long), h->stream));
   return SX_OK;
 }
-// bad / bad_d: a 32-bit device flag read in the same wait
+// The three banks summed on the device first: a read-back of a few bytes is
+// a kernel copy on the library's stream, while one of some kilobytes goes to
+// the copy engine, behind the rules' copy to the host.  bad / bad_d: a 32-bit
+// device flag read in the same wait.
 int read_evaluated(sx_handle *h, uint64_t *sum, uint64_t *sum_group, uint64_t *group_jobs, uint32_t *bad,
                    const uint32_t *bad_d) {
-  unsigned long long *evald;
+  unsigned long long *evald, *sums;
   ENSURE(B_CURSOR, unsigned long long, 3 * kEvalShards, evald);
-  std::vector<unsigned long long> shards(3 * kEvalShards);
-  const size_t sb = 3 * kEvalShards * sizeof(unsigned long long);
-  int rc = bad ? read_back(h, {{shards.data(), evald, sb}, {bad, bad_d, 4}}, "exhaustive sweep")
-               : read_back(h, {{shards.data(), evald, sb}}, "exhaustive sweep");
+  ENSURE(B_EVSUM, unsigned long long, 3, sums);
+  hipLaunchKernelGGL(k_sum_shards, dim3(1), dim3(kEvalShards), 0, h->stream, evald, sums);
+  HIPCHK(h, hipGetLastError());
+  unsigned long long tot[3] = {0, 0, 0};
+  int rc = bad ? read_back(h, {{tot, sums, sizeof(tot)}, {bad, bad_d, 4}}, "exhaustive sweep")
+               : read_back(h, {{tot, sums, sizeof(tot)}}, "exhaustive sweep");
   if (rc) return rc;
-  *sum = *sum_group = *group_jobs = 0;
-  for (int i = 0; i < kEvalShards; i++) {
-    *sum += shards[i] + shards[kEvalShards + i];
-    *sum_group += shards[kEvalShards + i];
-    *group_jobs += shards[2 * kEvalShards + i];
-  }
+  *sum = tot[0] + tot[1];
+  *sum_group = tot[1];
+  *group_jobs = tot[2];
   return SX_OK;
 }
 
@@ -3547,6 +3546,7 @@ extern "C" int sx_open(int device, sx_handle **out) {
   if (const char *e = getenv("SX_EXH_GROUP")) h->exh_group = atoi(e);
   if (const char *e = getenv("SX_QCM_JOIN")) h->qcm_join = atoi(e) != 0;
   if (const char *e = getenv("SX_FP_SPLIT")) h->fp_split = atoi(e) != 0;
+  if (const char *e = getenv("SX_FETCH_THREADS")) h->fetch_threads = std::min(64, std::max(1, atoi(e)));
   if (const char *e = getenv("SX_SEG_LOG")) h->seg_log = std::min(24, std::max(16, atoi(e)));
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking) != hipSuccess ||
@@ -3579,6 +3579,7 @@ extern "C" int sx_close(sx_handle *h) {
     (void)hipEventDestroy(t.b);
   }
   if (h->seg_ev) (void)hipEventDestroy(h->seg_ev);
+  for (hipEvent_t e : h->piece_ev) (void)hipEventDestroy(e);
   if (h->copy_stream) (void)hipStreamDestroy(h->copy_stream);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -3647,6 +3648,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
   h->empty_rules.clear();
   HIPCHK(h, hipStreamSynchronize(h->copy_stream));  // the last call's rule copies read d_rules
   h->d_rules_n = 0;
+  h->pieces.clear();
   h->rules_on_device = !cfg->hits;
   h->rec_words = 12;
   {
@@ -4319,8 +4321,19 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         // copy the segment's rules to the host while the next segments run
         HIPCHK(h, hipEventRecord(h->seg_ev, h->stream));
         HIPCHK(h, hipStreamWaitEvent(h->copy_stream, h->seg_ev, 0));
-        HIPCHK(h, hipMemcpyAsync(h->h_rules + used, h->d_rules + used, (size_t)nr * rw * 4, hipMemcpyDeviceToHost,
-                                 h->copy_stream));
+        for (uint64_t r0 = 0; r0 < nr; r0 += kCopyPiece) {
+          const uint64_t pn = std::min<uint64_t>(kCopyPiece, nr - r0);
+          const size_t pi = h->pieces.size();
+          if (pi == h->piece_ev.size()) {
+            hipEvent_t ev;
+            HIPCHK(h, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            h->piece_ev.push_back(ev);
+          }
+          HIPCHK(h, hipMemcpyAsync(h->h_rules + used + r0 * rw, h->d_rules + used + r0 * rw, (size_t)pn * rw * 4,
+                                   hipMemcpyDeviceToHost, h->copy_stream));
+          HIPCHK(h, hipEventRecord(h->piece_ev[pi], h->copy_stream));
+          h->pieces.emplace_back(h->d_rules_n + r0, pn);
+        }
       }
       h->d_rules_n += nr;
     } else {
@@ -4432,19 +4445,60 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
   return SX_OK;
 }
 
+namespace {
+// Rules [offset, offset + m) from the pinned host copy into out, piece by
+// piece as the copy's events complete, on up to 16 threads (thread k takes
+// pieces k, k + T, ...); a piece's part is unpacked (packed records) or
+// copied.
+int fetch_pieces(sx_handle *h, sx_rule *out, uint64_t offset, uint64_t m) {
+  const uint64_t end = offset + m;
+  std::vector<size_t> idx;  // the pieces that overlap the range
+  for (size_t i = 0; i < h->pieces.size(); i++)
+    if (h->pieces[i].first < end && h->pieces[i].first + h->pieces[i].second > offset) idx.push_back(i);
+  const uint32_t rw = h->rec_words;
+  std::atomic<int> err{0};
+  auto work = [&](size_t k, size_t stride) {
+    for (size_t j = k; j < idx.size(); j += stride) {
+      const auto &pc = h->pieces[idx[j]];
+      // poll: a blocking wait sleeps until an interrupt, far longer than a piece
+      hipError_t e;
+      while ((e = hipEventQuery(h->piece_ev[idx[j]])) == hipErrorNotReady) std::this_thread::yield();
+      if (e != hipSuccess) {
+        err = 1;
+        return;
+      }
+      const uint64_t a = std::max(pc.first, offset), b = std::min(pc.first + pc.second, end);
+      if (rw == kPackedWords)
+        unpack_rules(out + (a - offset), h->h_rules + a * kPackedWords, b - a);
+      else
+        memcpy(out + (a - offset), h->h_rules + a * 12, (b - a) * sizeof(sx_rule));
+    }
+  };
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nth = std::min<size_t>(std::min<unsigned>((unsigned)h->fetch_threads, hw), idx.size());
+  std::vector<std::thread> pool;
+  size_t started = 1;
+  try {
+    for (; started < nth; started++) pool.emplace_back(work, started, nth);
+  } catch (...) {  // no thread to be had: this thread takes the rest
+    for (size_t k = started; k < nth; k++) work(k, nth);
+  }
+  if (nth) work(0, nth);
+  for (auto &t : pool) t.join();
+  if (err) return set_err(h, SX_EHIP, "rule copy to the host failed");
+  return SX_OK;
+}
+}  // namespace
+
 extern "C" int sx_fetch_rules(sx_handle *h, sx_rule *out, uint64_t offset, uint64_t cap, uint64_t *n_copied) {
   if (!h || (!out && cap)) return SX_EINVAL;
   const uint64_t n = h->rules_on_device ? h->d_rules_n : h->rules.size();
   uint64_t m = offset >= n ? 0 : std::min<uint64_t>(cap, n - offset);
   if (m) {
     if (h->rules_on_device) {
-      // the pinned copy is complete once the copy stream drains
       HIPCHK(h, hipSetDevice(h->device));
-      HIPCHK(h, hipStreamSynchronize(h->copy_stream));
-      if (h->rec_words == kPackedWords)
-        parallel_unpack_rules(out, h->h_rules + offset * kPackedWords, m);
-      else
-        parallel_memcpy(out, h->h_rules + offset * 12, m * sizeof(sx_rule));
+      int rc = fetch_pieces(h, out, offset, m);
+      if (rc) return rc;
     } else {
       memcpy(out, h->rules.data() + offset, m * sizeof(sx_rule));
     }

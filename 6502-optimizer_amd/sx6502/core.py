@@ -311,6 +311,15 @@ class Device:
         del keep
         return out[: got.value], st.as_dict()
 
+    def fetch_rules(self, offset: int, n: int) -> np.ndarray:
+        """sx_fetch_rules: the last enumeration's rules [offset, offset + n)
+        (fewer at the end of the list)."""
+        out = np.empty(n, RULE_DTYPE)
+        got = C.c_uint64(0)
+        rc = lib().sx_fetch_rules(self._h, out.ctypes.data_as(C.c_void_p), offset, n, C.byref(got))
+        self._check(rc, "sx_fetch_rules")
+        return out[: got.value]
+
     def fetch_empty_rules(self, n: int) -> np.ndarray:
         """The empty-candidate stream of the last enumeration (its size is
         stats["empty_rules"])."""
