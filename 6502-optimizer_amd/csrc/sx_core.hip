@@ -1162,7 +1162,10 @@ __global__ __launch_bounds__(256) void k_qc_count(const uint2 *__restrict__ tfp,
 // per-wave LDS queue, reserved with one atomic on counters[1] per flush (the
 // MidCheck count); entries past out_cap are counted, not written, and the
 // caller re-runs the segment with a larger buffer.  cur0 (first hit): the
-// first round's minimum, as k_midcheck leaves it.
+// first round's minimum, as k_midcheck leaves it.  gridDim.y waves share each
+// 64-target chunk, taking its items in turn (blockIdx.y = the wave's turn):
+// few targets with many items each (length-3 targets against 8.2M
+// candidates) then still fill the device.
 __global__ __launch_bounds__(256) void k_qcm_join(const uint2 *__restrict__ tfp, const uint2 *__restrict__ tmh,
                                                   const uint8_t *__restrict__ tbytes, uint32_t nt, uint32_t lo_mask,
                                                   const uint32_t *__restrict__ sj, const uint32_t *__restrict__ sjslot,
@@ -1215,7 +1218,7 @@ __global__ __launch_bounds__(256) void k_qcm_join(const uint2 *__restrict__ tfp,
   s_key[w][lane] = jk;
   s_tid[w][lane] = t;
   __builtin_amdgcn_wave_barrier();
-  for (uint32_t base = 0; base < total; base += kWave) {
+  for (uint32_t base = blockIdx.y * kWave; base < total; base += kWave * gridDim.y) {
     const uint32_t i = base + lane;
     bool ok = false;
     uint32_t tt = 0, slot = 0, pass = 1;
@@ -3949,10 +3952,12 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     }
     // the joined QuickCheck + MidCheck needs the Mid hashes only of targets
     // whose masked QuickCheck outputs equal some candidate's (about 15% of
-    // the length-2 targets at --dead-flags 0x00): their QuickCheck outputs
-    // first, eight targets per pass, the Mid hashes after the class lookup;
-    // the empty candidate reads every target's Mid hashes
-    const bool fp_split = fused && h->fp_split && !cfg->include_empty;
+    // the length-2 targets against single instructions at --dead-flags
+    // 0x00): their QuickCheck outputs first, eight targets per pass, the Mid
+    // hashes after the class lookup.  Against the 8.2M candidates of length
+    // 1 and 2 nearly every target has a class, and the list only costs; the
+    // empty candidate reads every target's Mid hashes.
+    const bool fp_split = fused && h->fp_split && ml == 1 && !cfg->include_empty;
     if (fp_split)
       hipLaunchKernelGGL(k_fp_qc, dim3(blocks_for(nt, 4 * fpq_per_wave((uint32_t)tl))), dim3(256), 0, h->stream, tw, (uint32_t)tl,
                          nt, cfg->bytes_model, tfp, tbytes, tcyc, nullptr, badop);
@@ -4009,8 +4014,11 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         size_t ts = 0;
         int rc = timed_begin(h, 0, &ts);
         if (rc) return rc;
+        // waves per 64-target chunk: about 32 waves per CU in all
+        const uint32_t chunks = blocks_for(nt, kWave);
+        const uint32_t parts = std::min(16u, std::max(1u, 32u * (uint32_t)h->num_cu / std::max(chunks, 1u)));
         hipLaunchKernelGGL(k_qcm_join,
-                           dim3(fp_split ? std::min(blocks_for(nt, 256), 16u * h->num_cu) : blocks_for(nt, 256)),
+                           dim3(fp_split ? std::min(blocks_for(nt, 256), 16u * h->num_cu) : blocks_for(nt, 256), parts),
                            dim3(256), 0, h->stream, tfp, tmh, tbytes, (uint32_t)nt, lo_mask, j_key, j_slot, j_dir,
                            j_dk, cfp, cmh, cbytes, cprune, cfg->hits, mid, mid_cap, counters, rank,
                            cfg->hits ? nullptr : cur, tlist, tlist_n);

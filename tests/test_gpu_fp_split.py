@@ -1,10 +1,10 @@
 """The joined QuickCheck + MidCheck with split fingerprints agrees with the
 whole-target fingerprints.
 
-With SX_FP_SPLIT=1 (the default) the targets' outputs on the 8 QuickCheck
-vectors come first (k_fp_qc), the QuickCheck class lookup lists the targets
-whose masked outputs equal some candidate's, and only those get Mid hashes
-and enter the join.  With SX_FP_SPLIT=0 every target gets all 32 vectors and
+With SX_FP_SPLIT=1 (the default) and single-instruction candidates the
+targets' outputs on the 8 QuickCheck vectors come first (k_fp_qc), the
+QuickCheck class lookup lists the targets whose masked outputs equal some
+candidate's, and only those get Mid hashes and enter the join.  With SX_FP_SPLIT=0 every target gets all 32 vectors and
 the join reads every target.  Rules and every stage counter must be the same.
 """
 import os
@@ -60,8 +60,12 @@ def test_cuda_semantics(devs):
     assert b[1]["qc_hits"] == 20_654_396 and b[1]["mid_pass"] == 5_373_297
 
 
-def test_length3_sample(devs):
-    print("split fingerprints, length-3 sample", flush=True)
+@pytest.mark.parametrize("ml", [1, 2])
+def test_length3_sample(devs, ml):
+    # against single instructions the QuickCheck outputs of length-3 targets
+    # go 16 per wave; against 8.2M candidates (max_cand_len 2) the split is
+    # not used and both handles take the whole fingerprints
+    print(f"split fingerprints, length-3 sample, max_cand_len {ml}", flush=True)
     t = np.load(os.path.join(GOLD, "c3_targets.npy"))[:8192]
-    _same(devs[0].enumerate(t, target_len=3, max_cand_len=2, dead=0x01),
-          devs[1].enumerate(t, target_len=3, max_cand_len=2, dead=0x01))
+    _same(devs[0].enumerate(t, target_len=3, max_cand_len=ml, dead=0x01),
+          devs[1].enumerate(t, target_len=3, max_cand_len=ml, dead=0x01))
