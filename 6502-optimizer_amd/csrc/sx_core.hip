@@ -2574,23 +2574,20 @@ __global__ __launch_bounds__(256) void k_compact_rules(const sx_rule *slots, con
 }
 
 // Rules of length-2 targets against single-instruction candidates packed
-// for the trip to the host: 6 words instead of the 12 of an sx_rule (the
-// unused src[2..3] and rep[1..3] are 0 and the target index fits 32 bits):
-// src[0], src[1], rep[0], target index, src_len | rep_len << 8 | dead << 16 |
-// pass << 24, bytes_saved | cycles_saved << 16.  The copy over the host link
-// is the one part of an enumeration that its own compute cannot hide.
-constexpr uint32_t kPackedWords = 6;
+// for the trip to the host: 4 words instead of the 12 of an sx_rule: src[0],
+// src[1], rep[0] | pass << 14 | dead flags << 24 (a candidate's opcode is
+// below 64 and its immediate below 256, so those bits are free), target
+// index (below 2^32).  The lengths are 2 and 1 and the savings follow from
+// the words, so the host fills them in.  The copy over the host link is the
+// one part of an enumeration that its own compute cannot hide.
+constexpr uint32_t kPackedWords = 4;
 static_assert(sizeof(sx_rule) == 48, "sx_rule is 12 words");
-__global__ __launch_bounds__(256) void k_pack_rules(const sx_rule *r, uint32_t n, uint2 *w) {
+__global__ __launch_bounds__(256) void k_pack_rules(const sx_rule *r, uint32_t n, uint4 *w) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const sx_rule x = r[i];
-  uint2 *o = w + (uint64_t)i * 3;
-  o[0] = make_uint2(x.src[0], x.src[1]);
-  o[1] = make_uint2(x.rep[0], (uint32_t)x.target_index);
-  o[2] = make_uint2((uint32_t)x.src_len | (uint32_t)x.rep_len << 8 | (uint32_t)x.dead_flags << 16 |
-                        (uint32_t)x.pass << 24,
-                    (uint32_t)(uint16_t)x.bytes_saved | (uint32_t)(uint16_t)x.cycles_saved << 16);
+  w[i] = make_uint4(x.src[0], x.src[1], x.rep[0] | (uint32_t)x.pass << 14 | (uint32_t)x.dead_flags << 24,
+                    (uint32_t)x.target_index);
 }
 
 // Bulk QuickCheck / MidCheck / FlagDiff for sx_check_pairs: one wave per pair.
@@ -2786,6 +2783,7 @@ struct sx_handle {
   std::vector<std::pair<uint64_t, uint64_t>> pieces;
   uint32_t *h_rules = nullptr;
   uint64_t h_rules_capw = 0;
+  int fetch_model = 0;  // the byte model of the enumeration whose rules are held
   // pinned staging block for the counters the host reads back to size the
   // next stage (a copy to pageable memory waits by itself, so two such reads
   // cost two round trips)
@@ -2993,41 +2991,54 @@ int ensure(sx_handle *h, BufId id, size_t count, T **out) {
 
 inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// The packed rules of k_pack_rules back into sx_rule records: words 0-1
-// src[0..1], 4 rep[0], 8 the target index, 10 the lengths, dead flags and
-// pass, 11 the savings, the rest 0.  Streaming (non-temporal) 16-byte stores
-// when out is 16-byte aligned: the records are written once and not read
-// back here, and such stores skip reading the destination lines first.
-void unpack_rules(sx_rule *out, const uint32_t *w, uint64_t n) {
+// The packed rules of k_pack_rules back into sx_rule records (model: the
+// byte model of the enumeration): words 0-1 src[0..1], 4 rep[0], 8 the target
+// index, 10 the lengths (2, 1), dead flags and pass, 11 the bytes and cycles
+// saved, from per-opcode tables; the rest 0.  Streaming (non-temporal)
+// 16-byte stores when out is 16-byte aligned: the records are written once
+// and not read back here, and such stores skip reading the destination lines
+// first.
+void unpack_rules(sx_rule *out, const uint32_t *w, uint64_t n, int model) {
   static_assert(sizeof(sx_rule) == 48 && offsetof(sx_rule, target_index) == 32 && offsetof(sx_rule, src_len) == 40,
                 "sx_rule layout");
+  struct Tables {
+    int8_t bytes[2][64], cyc[64];
+    Tables() {
+      for (uint32_t op = 0; op < 64; op++) {
+        for (int m = 0; m < 2; m++) bytes[m][op] = op < (uint32_t)OP_COUNT ? (int8_t)byte_size(op, m) : 0;
+        cyc[op] = op < (uint32_t)OP_COUNT ? (int8_t)cycles(op) : 0;
+      }
+    }
+  };
+  static const Tables tab;
+  const int8_t *tb = tab.bytes[model ? 1 : 0], *tc = tab.cyc;
   using v4u = uint32_t __attribute__((ext_vector_type(4)));
+  auto rec = [&](uint64_t i, v4u &a, v4u &b, v4u &c) {
+    const uint32_t *p = w + i * kPackedWords;
+    const uint32_t o0 = p[0] & 63u, o1 = p[1] & 63u, r = p[2], oc = r & 63u;
+    const int bs = tb[o0] + tb[o1] - tb[oc], cs = tc[o0] + tc[o1] - tc[oc];
+    a = v4u{p[0], p[1], 0u, 0u};
+    b = v4u{r & 0x00FF003Fu, 0u, 0u, 0u};
+    c = v4u{p[3], 0u, 2u | 1u << 8 | (r >> 24) << 16 | ((r >> 14) & 3u) << 24,
+            (uint32_t)(uint16_t)bs | (uint32_t)(uint16_t)cs << 16};
+  };
+  v4u a, b, c;
   if ((reinterpret_cast<uintptr_t>(out) & 15) == 0) {
     v4u *o = reinterpret_cast<v4u *>(out);
     for (uint64_t i = 0; i < n; i++) {
-      const uint32_t *p = w + i * kPackedWords;
-      __builtin_nontemporal_store(v4u{p[0], p[1], 0u, 0u}, o + 3 * i);
-      __builtin_nontemporal_store(v4u{p[2], 0u, 0u, 0u}, o + 3 * i + 1);
-      __builtin_nontemporal_store(v4u{p[3], 0u, p[4], p[5]}, o + 3 * i + 2);
+      rec(i, a, b, c);
+      __builtin_nontemporal_store(a, o + 3 * i);
+      __builtin_nontemporal_store(b, o + 3 * i + 1);
+      __builtin_nontemporal_store(c, o + 3 * i + 2);
     }
     std::atomic_thread_fence(std::memory_order_seq_cst);  // the streaming stores before any later use
     return;
   }
   for (uint64_t i = 0; i < n; i++) {
-    const uint32_t *p = w + i * kPackedWords;
-    sx_rule r;
-    memset(&r, 0, sizeof(r));
-    r.src[0] = p[0];
-    r.src[1] = p[1];
-    r.rep[0] = p[2];
-    r.target_index = p[3];
-    r.src_len = (uint8_t)p[4];
-    r.rep_len = (uint8_t)(p[4] >> 8);
-    r.dead_flags = (uint8_t)(p[4] >> 16);
-    r.pass = (uint8_t)(p[4] >> 24);
-    r.bytes_saved = (int16_t)(uint16_t)p[5];
-    r.cycles_saved = (int16_t)(uint16_t)(p[5] >> 16);
-    out[i] = r;
+    rec(i, a, b, c);
+    memcpy(reinterpret_cast<char *>(out + i), &a, 16);
+    memcpy(reinterpret_cast<char *>(out + i) + 16, &b, 16);
+    memcpy(reinterpret_cast<char *>(out + i) + 32, &c, 16);
   }
 }
 
@@ -3683,6 +3694,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     d_tsrc = h->d_l2 + 2 * a;
   }
   if (h->rules_on_device && tl == 2 && ml == 1 && tbase + nt_all <= 0xFFFFFFFFull) h->rec_words = kPackedWords;
+  h->fetch_model = cfg->bytes_model;
 
   // ---- candidates: slots in Go order (rank = slot)
   const uint64_t nc = ml == 1 ? (uint64_t)kNumInsts : (uint64_t)kNumInsts + (uint64_t)kNumInsts * kNumInsts;
@@ -4320,7 +4332,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
       if (nr) {
         if (rw == kPackedWords) {
           hipLaunchKernelGGL(k_pack_rules, dim3(blocks_for(nr, 256)), dim3(256), 0, h->stream, rules_out, nr,
-                             reinterpret_cast<uint2 *>(h->d_rules + used));
+                             reinterpret_cast<uint4 *>(h->d_rules + used));
           HIPCHK(h, hipGetLastError());
         } else {
           HIPCHK(h, hipMemcpyAsync(h->d_rules + used, rules_out, (size_t)nr * sizeof(sx_rule),
@@ -4477,7 +4489,7 @@ int fetch_pieces(sx_handle *h, sx_rule *out, uint64_t offset, uint64_t m) {
       }
       const uint64_t a = std::max(pc.first, offset), b = std::min(pc.first + pc.second, end);
       if (rw == kPackedWords)
-        unpack_rules(out + (a - offset), h->h_rules + a * kPackedWords, b - a);
+        unpack_rules(out + (a - offset), h->h_rules + a * kPackedWords, b - a, h->fetch_model);
       else
         memcpy(out + (a - offset), h->h_rules + a * 12, (b - a) * sizeof(sx_rule));
     }
