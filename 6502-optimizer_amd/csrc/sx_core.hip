@@ -1403,6 +1403,47 @@ __device__ __forceinline__ uint32_t plan_job_tab(Job &j, int mode, const PlanEnt
   return plan_job(j, mode);
 }
 
+constexpr uint32_t kGroupNone = 0xFFFFFFFFu;
+constexpr uint32_t kGroupKeyBits = 19;  // kPlanTab < 2^19; kGroupNone sorts last
+static_assert(kPlanTab < (1u << kGroupKeyBits), "group keys");
+constexpr uint32_t kGroupSort = (1u << kGroupKeyBits) - 1u;  // the sort key of ineligible jobs
+// Keyed jobs touch only A, X, Y and P (their opcodes and their sub-sweeps).
+constexpr uint32_t kRegsAXYP = (1u << RA) | (1u << RX) | (1u << RY) | (1u << RP);
+
+// The group key of a planned job from its words (Job layout: tw 0-9, cw
+// 10-19, word 20 = tl | cl << 8 | pass << 16 | pmask << 24, 23 = nsets, 24-25
+// the sets, 26-27 their log sizes).
+__device__ __forceinline__ uint32_t group_key_words(uint32_t tw0, uint32_t tw1, uint32_t cw0, uint32_t hdr,
+                                                    uint32_t nsets, uint64_t sets8, uint64_t logs8) {
+  const uint32_t tl = hdr & 0xFFu, cl = (hdr >> 8) & 0xFFu, pm = hdr >> 24;
+  const uint32_t t0 = tw0 & 0xFFFFu, t1 = tw1 & 0xFFFFu, c = cw0 & 0xFFFFu;
+  bool ok = tl == 2 && cl == 1 && t0 < (uint32_t)OP_COUNT && t1 < (uint32_t)OP_COUNT && c < (uint32_t)OP_COUNT &&
+            nsets >= 1 && nsets <= 8;
+  uint32_t touch = ok ? pk_touch(t0) | pk_touch(t1) | pk_touch(c) : 0u;
+  for (uint32_t s = 0; ok && s < nsets; s++) {
+    ok = ((logs8 >> (8 * s)) & 0xFFu) <= 9u;
+    touch |= (uint32_t)(sets8 >> (8 * s)) & 0xFFu;
+  }
+  ok = ok && (touch & ~kRegsAXYP) == 0;
+  return ok ? ((t0 * OP_COUNT + t1) * OP_COUNT + c) * 2u + (pm != kPMask ? 1u : 0u) : kGroupSort;
+}
+__device__ __forceinline__ uint32_t group_key(const Job &j) {
+  uint64_t sets8, logs8;
+  __builtin_memcpy(&sets8, j.sets, 8);
+  __builtin_memcpy(&logs8, j.logsz, 8);
+  const uint32_t hdr = (uint32_t)j.tl | (uint32_t)j.cl << 8 | (uint32_t)j.pass << 16 | (uint32_t)j.pmask << 24;
+  return group_key_words(j.tw[0], j.tw[1], j.cw[0], hdr, j.nsets, sets8, logs8);
+}
+// A planned job's group key into key/idx (when the grouped sweep is on): a
+// keyed job leaves the state-parallel sweep (zero chunks).
+__device__ __forceinline__ void write_group_key(const Job &j, uint32_t i, uint32_t *chunks, uint32_t *key,
+                                                uint32_t *idx) {
+  const uint32_t k = group_key(j);
+  key[i] = k;
+  idx[i] = i;
+  if (k != kGroupSort) chunks[i] = 0;
+}
+
 // Jobs of one round: the current candidate of every open (target, pass).
 // In first-hit mode it also counts the survivors not yet tested (*nrest),
 // which decides when the tail runs as one batch.  plan: plan the jobs here
@@ -1414,7 +1455,8 @@ __global__ __launch_bounds__(kBlk) void k_round_jobs(const uint2 *mid, uint32_t 
                                                      const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
                                                      int hits_all, Job *jobs, uint32_t job_cap, uint32_t *njobs,
                                                      uint32_t *nrest, int plan, int plan_mode,
-                                                     const PlanEntry *ptab, uint32_t *chunks, uint32_t *fail) {
+                                                     const PlanEntry *ptab, uint32_t *chunks, uint32_t *fail,
+                                                     uint32_t *gkey, uint32_t *gidx) {
   __shared__ uint32_t s_rest[kBlkWaves], s_take[kBlkWaves];
   __shared__ uint32_t s_base;
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
@@ -1450,6 +1492,7 @@ __global__ __launch_bounds__(kBlk) void k_round_jobs(const uint2 *mid, uint32_t 
       if (plan) {
         chunks[o] = plan_job_tab(j, plan_mode, ptab, pm_masked);
         fail[o] = 0;
+        if (gkey) write_group_key(j, o, chunks, gkey, gidx);
       }
       jobs[o] = j;
     }
@@ -1482,15 +1525,19 @@ __global__ __launch_bounds__(kBlk) void k_entry_jobs(const uint32_t *cur, uint32
   }
 }
 
-// Plan each job's sweep and count its chunks.
+// Plan each job's sweep and count its chunks; with gkey/gidx, also the
+// job's group key for the grouped sweep (k_group_keys' work, while the job is
+// in registers).
 __global__ __launch_bounds__(256) void k_exh_plan(Job *jobs, uint32_t nj, int mode, const PlanEntry *ptab,
-                                                  uint32_t pm_masked, uint32_t *chunks, uint32_t *fail) {
+                                                  uint32_t pm_masked, uint32_t *chunks, uint32_t *fail,
+                                                  uint32_t *gkey, uint32_t *gidx) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nj) return;
   Job j = jobs[i];
   chunks[i] = plan_job_tab(j, mode, ptab, pm_masked);
   jobs[i] = j;
   fail[i] = 0;
+  if (gkey) write_group_key(j, i, chunks, gkey, gidx);
 }
 
 // Exhaustive sweep: one wave per chunk of 2^14 states.  The chunks of all
@@ -1899,37 +1946,22 @@ __global__ __launch_bounds__(256) void k_exh_run(const Job *__restrict__ jobs, u
 // two states of the pass in the low and high halves.  A wave with lanes of
 // several keys (at the ends of the sorted runs) takes them one key after the
 // other.  A job fails at its first mismatching state; the wave leaves a key
-// once every one of its jobs has failed.
-constexpr uint32_t kGroupNone = 0xFFFFFFFFu;
-constexpr uint32_t kGroupKeyBits = 19;  // kPlanTab < 2^19; kGroupNone sorts last
-static_assert(kPlanTab < (1u << kGroupKeyBits), "group keys");
-constexpr uint32_t kGroupSort = (1u << kGroupKeyBits) - 1u;  // the sort key of ineligible jobs
-// Keyed jobs touch only A, X, Y and P (their opcodes and their sub-sweeps).
-constexpr uint32_t kRegsAXYP = (1u << RA) | (1u << RX) | (1u << RY) | (1u << RP);
+// once every one of its jobs has failed.  The keys and constants are above
+// k_round_jobs, whose planning writes them.
 
-// Key of each job (kGroupSort when it stays with the state-parallel sweep);
-// a keyed job gets zero chunks there.
+// Key of each job (kGroupSort when it stays with the state-parallel sweep),
+// for jobs whose planner did not write it; a keyed job gets zero chunks there.
 __global__ __launch_bounds__(256) void k_group_keys(const Job *__restrict__ jobs, uint32_t nj, uint32_t *chunks,
                                                     uint32_t *key, uint32_t *idx) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nj) return;
   const uint32_t *jw = reinterpret_cast<const uint32_t *>(jobs + i);
-  const uint32_t hdr = jw[20], h23 = jw[23];
-  const uint32_t tl = hdr & 0xFFu, cl = (hdr >> 8) & 0xFFu, pm = hdr >> 24, nsets = h23 & 0xFFu;
-  const uint32_t t0 = jw[0] & 0xFFFFu, t1 = jw[1] & 0xFFFFu, c = jw[kJobWords] & 0xFFFFu;
-  const uint64_t logs8 = (uint64_t)jw[26] | ((uint64_t)jw[27] << 32);
-  bool ok = tl == 2 && cl == 1 && t0 < (uint32_t)OP_COUNT && t1 < (uint32_t)OP_COUNT && c < (uint32_t)OP_COUNT &&
-            nsets >= 1 && nsets <= 8;
-  const uint64_t sets8 = (uint64_t)jw[24] | ((uint64_t)jw[25] << 32);
-  uint32_t touch = ok ? pk_touch(t0) | pk_touch(t1) | pk_touch(c) : 0u;
-  for (uint32_t s = 0; ok && s < nsets; s++) {
-    ok = ((logs8 >> (8 * s)) & 0xFFu) <= 9u;
-    touch |= (uint32_t)(sets8 >> (8 * s)) & 0xFFu;
-  }
-  ok = ok && (touch & ~kRegsAXYP) == 0;
-  key[i] = ok ? ((t0 * OP_COUNT + t1) * OP_COUNT + c) * 2u + (pm != kPMask ? 1u : 0u) : kGroupSort;
+  const uint32_t k = group_key_words(jw[0], jw[1], jw[kJobWords], jw[20], jw[23] & 0xFFu,
+                                     (uint64_t)jw[24] | ((uint64_t)jw[25] << 32),
+                                     (uint64_t)jw[26] | ((uint64_t)jw[27] << 32));
+  key[i] = k;
   idx[i] = i;
-  if (ok) chunks[i] = 0;
+  if (k != kGroupSort) chunks[i] = 0;
 }
 
 // The items of one key group (lanes gm of the wave) that wave u of the
@@ -2114,7 +2146,8 @@ __global__ __launch_bounds__(kBlk) void k_bulk_jobs(const uint2 *mid, uint32_t n
                                                     const uint32_t *rej, const uint8_t *done,
                                                     const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
                                                     Job *jobs, uint32_t job_cap, uint32_t *njobs, int plan_mode,
-                                                    const PlanEntry *ptab, uint32_t *chunks, uint32_t *fail) {
+                                                    const PlanEntry *ptab, uint32_t *chunks, uint32_t *fail,
+                                                    uint32_t *gkey, uint32_t *gidx) {
   __shared__ uint32_t s_take[kBlkWaves];
   __shared__ uint32_t s_base;
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
@@ -2139,6 +2172,7 @@ __global__ __launch_bounds__(kBlk) void k_bulk_jobs(const uint2 *mid, uint32_t n
     if (o < job_cap) {
       chunks[o] = plan_job_tab(j, plan_mode, ptab, pm_masked);
       fail[o] = 0;
+      if (gkey) write_group_key(j, o, chunks, gkey, gidx);
       jobs[o] = j;
     }
   }
@@ -3191,8 +3225,10 @@ int launch_exh_run(sx_handle *h, const Job *jobs, uint32_t nj, const uint64_t *o
 // planned: the jobs' plans, chunk counts and fail flags are already written
 // (by the kernel that made the jobs).
 // ptab / pm_masked: the plan table of the call (or null) and its relaxed mask.
+// keyed: the kernel that planned the jobs also wrote their group keys into
+// B_GKEY / B_GIDX (with the grouped sweep on).
 int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, bool planned = false,
-                   const PlanEntry *ptab = nullptr, uint32_t pm_masked = 0) {
+                   const PlanEntry *ptab = nullptr, uint32_t pm_masked = 0, bool keyed = false) {
   Job *jobs = (Job *)h->bufs[B_JOBS].p;
   uint32_t *chunks, *fail;
   uint64_t *offs, *total;
@@ -3202,19 +3238,25 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, bo
   ENSURE(B_TOTAL, uint64_t, 1, total);
   unsigned long long *evald;  // evaluated states, added up across calls (three banks)
   ENSURE(B_CURSOR, unsigned long long, 3 * kEvalShards, evald);
-  if (!planned)
+  uint32_t *key = nullptr, *idx = nullptr;  // group keys of the jobs (grouped sweep)
+  if (h->exh_group && nj) {
+    ENSURE(B_GKEY, uint32_t, nj, key);
+    ENSURE(B_GIDX, uint32_t, nj, idx);
+  }
+  if (!planned) {
     hipLaunchKernelGGL(k_exh_plan, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, jobs, nj, mode, ptab, pm_masked,
-                       chunks, fail);
+                       chunks, fail, key, idx);
+    keyed = key != nullptr;
+  }
   HIPCHK(h, hipGetLastError());
   if (h->exh_group && nj) {
     // jobs with small plans: one job per lane, sorted by (opcodes, mask); they
     // leave the state-parallel sweep below with zero chunks
-    uint32_t *key, *idx, *skey, *sidx;
-    ENSURE(B_GKEY, uint32_t, nj, key);
-    ENSURE(B_GIDX, uint32_t, nj, idx);
+    uint32_t *skey, *sidx;
     ENSURE(B_GKEY_S, uint32_t, nj, skey);
     ENSURE(B_GIDX_S, uint32_t, nj, sidx);
-    hipLaunchKernelGGL(k_group_keys, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, jobs, nj, chunks, key, idx);
+    if (!keyed)
+      hipLaunchKernelGGL(k_group_keys, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, jobs, nj, chunks, key, idx);
     HIPCHK(h, hipGetLastError());
     size_t tmp_bytes = 0;
     HIPCHK(h, rocprim::radix_sort_pairs(nullptr, tmp_bytes, key, skey, idx, sidx, nj, 0, kGroupKeyBits, h->stream));
@@ -4185,9 +4227,14 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
           hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, nt * 2,
                              kRankNone);
         HIPCHK(h, hipMemsetAsync(njobs_d, 0, 4, h->stream));
+        uint32_t *bkey = nullptr, *bidx = nullptr;  // the jobs' group keys, written as they are planned
+        if (h->exh_group) {
+          ENSURE(B_GKEY, uint32_t, job_cap, bkey);
+          ENSURE(B_GIDX, uint32_t, job_cap, bidx);
+        }
         hipLaunchKernelGGL(k_bulk_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, nmid, rank,
                            rej, done, tw, (uint32_t)tl, pm_masked, jobs, job_cap, njobs_d, cfg->exh_mode, ptab,
-                           jchunks, jfail);
+                           jchunks, jfail, bkey, bidx);
         HIPCHK(h, hipGetLastError());
         uint32_t nj = 0;
         {
@@ -4200,7 +4247,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         st.exh_rounds++;
         st.exh_pairs += nj;
         uint64_t chunks = 0;
-        int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, true);
+        int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, true, nullptr, 0, bkey != nullptr);
         if (rc) return rc;
         // cur holds kRankNone for every entry after a resolve: reuse it for the minimum
         hipLaunchKernelGGL(k_bulk_min, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream,
@@ -4214,6 +4261,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
       }
       HIPCHK(h, hipMemsetAsync(njobs_d, 0, 8, h->stream));  // njobs and nrest
       const bool first_round = !cfg->hits && round == 0;
+      uint32_t *rkey = nullptr, *ridx = nullptr;
       if (first_round) {
         // cur holds the minima (MidCheck); every survivor is untested
         hipLaunchKernelGGL(k_entry_jobs, dim3(blocks_for(nt * 2, kBlk)), dim3(kBlk), 0, h->stream, cur,
@@ -4222,9 +4270,14 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         if (!cfg->hits)
           hipLaunchKernelGGL(k_round_min, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
                              cur, rej, done);
+        // all hits: the jobs are planned (and keyed) here
+        if (cfg->hits && h->exh_group) {
+          ENSURE(B_GKEY, uint32_t, job_cap, rkey);
+          ENSURE(B_GIDX, uint32_t, job_cap, ridx);
+        }
         hipLaunchKernelGGL(k_round_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, nmid, rank,
                            cur, rej, done, tw, (uint32_t)tl, pm_masked, cfg->hits, jobs, job_cap, njobs_d, nrest_d,
-                           cfg->hits ? 1 : 0, cfg->exh_mode, ptab, jchunks, jfail);
+                           cfg->hits ? 1 : 0, cfg->exh_mode, ptab, jchunks, jfail, rkey, ridx);
       }
       HIPCHK(h, hipGetLastError());
       uint32_t cnt2[2] = {0, 0};
@@ -4252,7 +4305,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
       st.exh_rounds++;
       st.exh_pairs += nj;
       uint64_t chunks = 0;
-      int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, cfg->hits != 0, ptab, pm_masked);  // first hit: planned there
+      int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, cfg->hits != 0, ptab, pm_masked,
+                              rkey != nullptr);  // first hit: planned (and keyed) there
       if (rc) return rc;
       hipLaunchKernelGGL(k_round_resolve, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream,
                          (const Job *)h->bufs[B_JOBS].p, nj, (const uint32_t *)h->bufs[B_FAIL].p, rank, cur, rej,
@@ -4674,7 +4728,7 @@ extern "C" int sx_check_pairs_noalloc(sx_handle *h, const uint32_t *t_words, con
     hipLaunchKernelGGL(k_pair_jobs, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, dtw, dtl, dcw, dcl, nn, pm,
                        jobs);
     hipLaunchKernelGGL(k_exh_plan, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, jobs, nn, (int)exh_mode,
-                       (const PlanEntry *)nullptr, pm, chunks, fail);
+                       (const PlanEntry *)nullptr, pm, chunks, fail, (uint32_t *)nullptr, (uint32_t *)nullptr);
     const uint64_t nb = (n + 1023) / 1024;
     hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(1024), 0, h->stream, chunks, n, offs, sums);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, h->stream, sums, nb, total);
