@@ -750,6 +750,28 @@ __global__ __launch_bounds__(256) void k_hash_dir(const uint32_t *__restrict__ s
   dir[b] = lo;
 }
 
+// Bucket directory of the sorted 64-bit class keys on their top dk bits:
+// dir[b] = the first class whose key is in bucket b or later, dir[2^dk] =
+// the class count (known on the device only).
+__global__ __launch_bounds__(256) void k_class_dir(const unsigned long long *__restrict__ ckey,
+                                                   const uint64_t *__restrict__ ncls_p, uint32_t dk, uint32_t *dir) {
+  const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+  if (b > (1u << dk)) return;
+  const uint32_t ncls = (uint32_t)*ncls_p;
+  if (b == (1u << dk)) {
+    dir[b] = ncls;
+    return;
+  }
+  const unsigned long long k = (unsigned long long)b << (64 - dk);  // first key of the bucket
+  uint32_t lo = 0, hi = ncls;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (ckey[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  dir[b] = lo;
+}
+
 __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t x, uint32_t lane) {
   uint32_t s = x;
 #pragma unroll
@@ -1081,14 +1103,13 @@ constexpr uint32_t kQcCntChunks = 4;  // 256-target chunks per k_qc_count block
 __global__ __launch_bounds__(256) void k_qc_count(const uint2 *__restrict__ tfp, const uint8_t *__restrict__ tbytes,
                                                   uint32_t nt, uint32_t lo_mask,
                                                   const unsigned long long *__restrict__ ckey,
-                                                  const uint64_t *__restrict__ ncls_p,
+                                                  const uint32_t *__restrict__ cdir, uint32_t cdk,
                                                   const uint32_t *__restrict__ crep, const uint32_t *__restrict__ ccnt,
                                                   const uint2 *__restrict__ cfp, unsigned long long *sums,
                                                   uint32_t *list, uint32_t *list_n) {
   __shared__ unsigned long long s_raw[4], s_hit[4];
   __shared__ uint32_t s_in[kQcCntChunks][4], s_base;
   const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-  const uint32_t ncls = (uint32_t)*ncls_p;
   uint64_t raw = 0, hit = 0;
   uint64_t m[kQcCntChunks];  // per chunk: the wave's targets with a class
 #pragma unroll
@@ -1101,13 +1122,12 @@ __global__ __launch_bounds__(256) void k_qc_count(const uint2 *__restrict__ tfp,
       uint32_t k1, k2;
       qc_keys(o, lo_mask, k1, k2);
       const unsigned long long k = (unsigned long long)k1 | ((unsigned long long)k2 << 32);
-      uint32_t lo = 0, hi = ncls;  // lower bound of k
-      while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (ckey[mid] < k) lo = mid + 1;
-        else hi = mid;
-      }
-      if (lo < ncls && ckey[lo] == k) {
+      // the class of key k: its bucket, then the bucket's few keys in order
+      const uint32_t b = (uint32_t)(k >> (64 - cdk));
+      uint32_t lo = cdir[b];
+      const uint32_t hi = cdir[b + 1];
+      while (lo < hi && ckey[lo] < k) lo++;
+      if (lo < hi && ckey[lo] == k) {
         uint2 r[8];
         load_fp8(cfp, crep[lo], r);
         bool okm, okf;
@@ -2853,7 +2873,7 @@ enum BufId {
   B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_PLANTAB, B_EFP, B_ERULES, B_L3_LIST, B_L3_HK, B_L3_CNT,
   B_L3_KEYS, B_L3_HEAD, B_L3_NEXT, B_L3_BLOOM, B_L3_RAW, B_L3_C3, B_L3_BEST, B_GKEY, B_GIDX, B_GKEY_S, B_GIDX_S,
   B_GSORT, B_QJ_KEY, B_QJ_VAL, B_QJ_HASH, B_QJ_SLOT, B_QJ_DIR, B_CK, B_CKS, B_CV1,
-  B_CV1S, B_JK, B_JV, B_JKS, B_JVS, B_JDIR, B_CHEAD, B_CHOFF, B_CLSKEY, B_CLSREP, B_CLSCNT, B_CLSN, B_QCSUM, B_TLIST, B_EVSUM, B_COUNT
+  B_CV1S, B_JK, B_JV, B_JKS, B_JVS, B_JDIR, B_CHEAD, B_CHOFF, B_CLSKEY, B_CLSREP, B_CLSCNT, B_CLSN, B_QCSUM, B_TLIST, B_EVSUM, B_CLSDIR, B_COUNT
 };
 static_assert(B_COUNT <= 96, "sx_handle::bufs holds every buffer id");
 
@@ -3835,6 +3855,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
   unsigned long long *cls_key = nullptr;
   uint64_t *cls_n = nullptr;
   uint32_t *cls_rep = nullptr, *cls_cnt = nullptr, *j_key = nullptr, *j_slot = nullptr, *j_dir = nullptr, j_dk = 1;
+  uint32_t *cls_dir = nullptr, cls_dk = 1;  // the classes' bucket directory on the top cls_dk key bits
   if (fused) {
     while (j_dk < 24 && (1ull << j_dk) < nc) j_dk++;
     unsigned long long *ck, *cks;
@@ -3855,6 +3876,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     ENSURE(B_CLSREP, uint32_t, nc, cls_rep);
     ENSURE(B_CLSCNT, uint32_t, nc * (kClsBins + 1), cls_cnt);
     ENSURE(B_CLSN, uint64_t, 2, cls_n);
+    while (cls_dk < 24 && (1ull << cls_dk) < nc) cls_dk++;
+    ENSURE(B_CLSDIR, uint32_t, (1ull << cls_dk) + 1, cls_dir);
     bad = reinterpret_cast<uint32_t *>(cls_n + 1);
     const uint32_t n32 = (uint32_t)nc;
     if (h->cand.fused_built) {
@@ -3884,6 +3907,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
                          cbytes, cprune, cls_key, cls_rep, cls_cnt);
       hipLaunchKernelGGL(k_class_verify, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, v1s, n32, head, hoff,
                          cls_rep, cfp, lo_mask, bad);
+      hipLaunchKernelGGL(k_class_dir, dim3(blocks_for((1ull << cls_dk) + 1, 256)), dim3(256), 0, h->stream, cls_key,
+                         cls_n, cls_dk, cls_dir);
       HIPCHK(h, hipGetLastError());
       uint32_t bad_h = 0;
       HIPCHK(h, hipMemcpyAsync(&bad_h, bad, 4, hipMemcpyDeviceToHost, h->stream));
@@ -4048,7 +4073,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         int rc = timed_begin(h, 0, &ts);
         if (rc) return rc;
         hipLaunchKernelGGL(k_qc_count, dim3(blocks_for(nt, 256 * kQcCntChunks)), dim3(256), 0, h->stream, tfp, tbytes, (uint32_t)nt,
-                           lo_mask, cls_key, cls_n, cls_rep, cls_cnt, cfp, qsums, tlist, tlist_n);
+                           lo_mask, cls_key, cls_dir, cls_dk, cls_rep, cls_cnt, cfp, qsums, tlist, tlist_n);
         HIPCHK(h, hipGetLastError());
         if (fp_split)
           hipLaunchKernelGGL(k_fp_mid_list, dim3(std::min(blocks_for(nt, 4 * kFpPerWave), 8u * h->num_cu)), dim3(256), 0,
