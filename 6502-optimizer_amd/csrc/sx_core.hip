@@ -473,87 +473,6 @@ __global__ __launch_bounds__(256) void k_fp_mid_list(const uint32_t *words, cons
     fp_batch(words, list, len, first, n, 0, nullptr, mh, nullptr, lo_mask, nullptr, nullptr, nullptr, bad_op);
 }
 
-// Outputs on the 8 QuickCheck vectors only (fp as k_fingerprints writes it),
-// eight sequences per pass: lane group g = lane / 8 runs sequence q + g of a
-// run of sequences with equal opcodes (per-lane immediates), lane v = lane %
-// 8 its vector v.  Runs of one first instruction share their prefix as in
-// k_fingerprints.  bytes, cycles, the pruner flag and bad_op as there.
-// A wave takes fpq_per_wave(len) sequences (their words one per lane).
-__host__ __device__ constexpr uint32_t fpq_per_wave(uint32_t len) { return len <= 2 ? 32u : 16u; }
-__global__ __launch_bounds__(256) void k_fp_qc(const uint32_t *__restrict__ words, uint32_t len, uint64_t n,
-                                               int bytes_model, uint2 *__restrict__ fp, uint8_t *bytes, uint8_t *cyc,
-                                               uint8_t *prune, uint32_t *bad_op) {
-  const uint32_t per = fpq_per_wave(len);  // len <= 4
-  const uint64_t first = ((uint64_t)blockIdx.x * 4 + uni(threadIdx.x >> 6)) * per;  // wave-uniform
-  if (first >= n) return;
-  const uint32_t lane = lane_id(), v = lane & 7, g = lane >> 3;
-  const uint32_t cnt = (uint32_t)(first + per < n ? per : n - first);
-  const uint32_t wv = lane < cnt * len ? words[first * len + lane] : 0u;
-  const uint32_t cmask = cnt >= 32 ? 0xFFFFFFFFu : (1u << cnt) - 1u;
-  uint32_t bad_seqs, same_next;
-  bool share;
-  {
-    const uint32_t q = lane < cnt ? lane : 0u, qn = q + 1 < cnt ? q + 1 : q;
-    bool l_bad = false, l_same = q + 1 < cnt, l_pre = true;
-    for (uint32_t k = 0; k < len; k++) {
-      const uint32_t w = (uint32_t)__shfl((int)wv, (int)(q * len + k));
-      const uint32_t wn = (uint32_t)__shfl((int)wv, (int)(qn * len + k));
-      l_bad |= (w & 0xFFFFu) >= OP_COUNT;
-      l_same &= (w & 0xFFFFu) == (wn & 0xFFFFu);
-      if (k + 1 < len) l_pre &= w == __builtin_amdgcn_readlane(wv, k);
-    }
-    bad_seqs = (uint32_t)ballot(l_bad) & cmask;
-    same_next = (uint32_t)ballot(l_same) & cmask;
-    share = len > 1 && ((uint32_t)ballot(l_pre) & cmask) == cmask;
-  }
-  State base = load_vec(v);
-  uint32_t k0 = 0;
-  if (share) {
-    for (uint32_t k = 0; k + 1 < len; k++) {
-      const uint32_t w = __builtin_amdgcn_readlane(wv, k);
-      step(base, w & 0xFFFFu, w >> 16);
-    }
-    k0 = len - 1;
-  }
-  for (uint32_t q = 0; q < cnt;) {
-    // sequences q .. q + r - 1 have the opcodes of q (so all or none is bad)
-    uint32_t r = (uint32_t)__builtin_ctz(~(same_next >> q)) + 1u;
-    r = min(min(r, 8u), cnt - q);
-    if (!((bad_seqs >> q) & 1u)) {
-      const uint32_t sq = q + min(g, r - 1u);
-      State st = base;
-      for (uint32_t k = k0; k < len; k++) {
-        const uint32_t op = __builtin_amdgcn_readlane(wv, q * len + k) & 0xFFFFu;
-        const uint32_t w = (uint32_t)__shfl((int)wv, (int)(sq * len + k));
-        step(st, op, w >> 16);
-      }
-      st.p &= kPMask;
-      if (g < r) fp[(first + q + g) * kFpVecs + v] = make_uint2(pack_lo(st), pack_hi(st));
-    }
-    q += r;
-  }
-  if (bad_seqs && lane == 0) atomicOr(bad_op, 1u);
-  const uint64_t seq = first + lane;
-  if (lane < cnt) {
-    uint32_t w[4] = {0, 0, 0, 0};
-    bool bad = false;
-    for (uint32_t k = 0; k < len; k++) {
-      w[k] = words[seq * len + k];
-      bad |= (w[k] & 0xFFFFu) >= OP_COUNT;
-    }
-    if (!bad) {
-      int b = 0, c = 0;
-      for (uint32_t k = 0; k < len; k++) {
-        b += byte_size(w[k] & 0xFFFFu, bytes_model);
-        c += cycles(w[k] & 0xFFFFu);
-      }
-      bytes[seq] = (uint8_t)b;
-      cyc[seq] = (uint8_t)c;
-      if (prune) prune[seq] = should_prune(w, (int)len, c_reads, c_writes) ? 1 : 0;
-    }
-  }
-}
-
 // QuickCheck.  Grid: x = 256-candidate group, y = target tile.
 // Survivor entry: (target, slot | full << 31), where full = the exact
 // (0xCF) compare holds; every entry satisfies the compare under pm_masked.
@@ -1172,6 +1091,171 @@ __global__ __launch_bounds__(256) void k_qc_count(const uint2 *__restrict__ tfp,
     const uint32_t sh = blockIdx.x & (kQcCntShards - 1);
     if (r) atomicAdd(&sums[2 * sh], (unsigned long long)r);
     if (hh) atomicAdd(&sums[2 * sh + 1], (unsigned long long)hh);
+  }
+}
+
+// Outputs on the 8 QuickCheck vectors only (fp as k_fingerprints writes it),
+// eight sequences per pass: lane group g = lane / 8 runs sequence q + g of a
+// run of sequences with equal opcodes (per-lane immediates), lane v = lane %
+// 8 its vector v.  Runs of one first instruction share their prefix as in
+// k_fingerprints.  bytes, cycles, the pruner flag and bad_op as there.
+// Cls: the outputs stay in LDS and lane q looks up sequence q's QuickCheck
+// class as k_qc_count does (counters into ca.sums, class members appended
+// to ca.list with one atomic per block); only class members' outputs are
+// written to fp, the only ones read later, so the other 85% of the length-2
+// targets cost no fingerprint traffic at all.
+struct QcClassArgs {
+  const unsigned long long *ckey;
+  const uint32_t *cdir;
+  uint32_t cdk;
+  const uint32_t *crep, *ccnt;
+  const uint2 *cfp;
+  uint32_t lo_mask;
+  unsigned long long *sums;
+  uint32_t *list, *list_n;
+};
+// A wave takes fpq_per_wave(len) sequences (their words one per lane).
+__host__ __device__ constexpr uint32_t fpq_per_wave(uint32_t len) { return len <= 2 ? 32u : 16u; }
+template <bool Cls>
+__global__ __launch_bounds__(256) void k_fp_qc(const uint32_t *__restrict__ words, uint32_t len, uint64_t n,
+                                               int bytes_model, uint2 *__restrict__ fp, uint8_t *bytes, uint8_t *cyc,
+                                               uint8_t *prune, uint32_t *bad_op, QcClassArgs ca) {
+  __shared__ uint2 s_fp[Cls ? 4 : 1][Cls ? 32 : 1][kFpVecs];
+  __shared__ unsigned long long s_raw[4], s_hit[4];
+  __shared__ uint32_t s_in[4], s_base;
+  const uint32_t per = fpq_per_wave(len);  // len <= 4
+  const uint32_t wid = uni(threadIdx.x >> 6);
+  const uint64_t first = ((uint64_t)blockIdx.x * 4 + wid) * per;  // wave-uniform
+  if (!Cls && first >= n) return;
+  const uint32_t lane = lane_id(), v = lane & 7, g = lane >> 3;
+  const uint32_t cnt = first >= n ? 0u : (uint32_t)(first + per < n ? per : n - first);
+  const uint32_t wv = lane < cnt * len ? words[first * len + lane] : 0u;
+  const uint32_t cmask = cnt >= 32 ? 0xFFFFFFFFu : (1u << cnt) - 1u;
+  uint32_t bad_seqs, same_next;
+  bool share;
+  {
+    const uint32_t q = lane < cnt ? lane : 0u, qn = q + 1 < cnt ? q + 1 : q;
+    bool l_bad = false, l_same = q + 1 < cnt, l_pre = true;
+    for (uint32_t k = 0; k < len; k++) {
+      const uint32_t w = (uint32_t)__shfl((int)wv, (int)(q * len + k));
+      const uint32_t wn = (uint32_t)__shfl((int)wv, (int)(qn * len + k));
+      l_bad |= (w & 0xFFFFu) >= OP_COUNT;
+      l_same &= (w & 0xFFFFu) == (wn & 0xFFFFu);
+      if (k + 1 < len) l_pre &= w == __builtin_amdgcn_readlane(wv, k);
+    }
+    bad_seqs = (uint32_t)ballot(l_bad) & cmask;
+    same_next = (uint32_t)ballot(l_same) & cmask;
+    share = len > 1 && ((uint32_t)ballot(l_pre) & cmask) == cmask;
+  }
+  State base = load_vec(v);
+  uint32_t k0 = 0;
+  if (share) {
+    for (uint32_t k = 0; k + 1 < len; k++) {
+      const uint32_t w = __builtin_amdgcn_readlane(wv, k);
+      step(base, w & 0xFFFFu, w >> 16);
+    }
+    k0 = len - 1;
+  }
+  for (uint32_t q = 0; q < cnt;) {
+    // sequences q .. q + r - 1 have the opcodes of q (so all or none is bad)
+    uint32_t r = (uint32_t)__builtin_ctz(~(same_next >> q)) + 1u;
+    r = min(min(r, 8u), cnt - q);
+    if (!((bad_seqs >> q) & 1u)) {
+      const uint32_t sq = q + min(g, r - 1u);
+      State st = base;
+      for (uint32_t k = k0; k < len; k++) {
+        const uint32_t op = __builtin_amdgcn_readlane(wv, q * len + k) & 0xFFFFu;
+        const uint32_t w = (uint32_t)__shfl((int)wv, (int)(sq * len + k));
+        step(st, op, w >> 16);
+      }
+      st.p &= kPMask;
+      if (g < r) {
+        const uint2 o = make_uint2(pack_lo(st), pack_hi(st));
+        if constexpr (Cls) s_fp[wid][q + g][v] = o;
+        else fp[(first + q + g) * kFpVecs + v] = o;
+      }
+    }
+    q += r;
+  }
+  if (bad_seqs && lane == 0) atomicOr(bad_op, 1u);
+  const uint64_t seq = first + lane;
+  bool good = false;  // lane q: sequence q has known opcodes
+  uint32_t my_bytes = 0;
+  if (lane < cnt) {
+    uint32_t w[4] = {0, 0, 0, 0};
+    bool bad = false;
+    for (uint32_t k = 0; k < len; k++) {
+      w[k] = words[seq * len + k];
+      bad |= (w[k] & 0xFFFFu) >= OP_COUNT;
+    }
+    if (!bad) {
+      int b = 0, c = 0;
+      for (uint32_t k = 0; k < len; k++) {
+        b += byte_size(w[k] & 0xFFFFu, bytes_model);
+        c += cycles(w[k] & 0xFFFFu);
+      }
+      bytes[seq] = (uint8_t)b;
+      cyc[seq] = (uint8_t)c;
+      if (prune) prune[seq] = should_prune(w, (int)len, c_reads, c_writes) ? 1 : 0;
+      good = true;
+      my_bytes = (uint32_t)b;
+    }
+  }
+  if constexpr (Cls) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint64_t raw = 0, hit = 0;
+    bool in_class = false;
+    uint2 o[8];
+    if (good) {
+#pragma unroll
+      for (int k = 0; k < 8; k++) o[k] = s_fp[wid][lane][k];
+      uint32_t k1, k2;
+      qc_keys(o, ca.lo_mask, k1, k2);
+      const unsigned long long key = (unsigned long long)k1 | ((unsigned long long)k2 << 32);
+      const uint32_t b = (uint32_t)(key >> (64 - ca.cdk));
+      uint32_t lo = ca.cdir[b];
+      const uint32_t hi = ca.cdir[b + 1];
+      while (lo < hi && ca.ckey[lo] < key) lo++;
+      if (lo < hi && ca.ckey[lo] == key) {
+        uint2 r[8];
+        load_fp8(ca.cfp, ca.crep[lo], r);
+        bool okm, okf;
+        fp_compare(o, r, ca.lo_mask, okm, okf);
+        in_class = okm;
+        if (okm) {
+          const uint32_t *c = ca.ccnt + (uint64_t)lo * (kClsBins + 1);
+          raw = c[kClsBins];
+          const uint32_t tb = min(my_bytes, kClsBins);
+          for (uint32_t bb = 0; bb < tb; bb++) hit += c[bb];
+          uint4 *dst = reinterpret_cast<uint4 *>(fp + seq * kFpVecs);
+#pragma unroll
+          for (int k = 0; k < 4; k++) dst[k] = make_uint4(o[2 * k].x, o[2 * k].y, o[2 * k + 1].x, o[2 * k + 1].y);
+        }
+      }
+    }
+    const uint64_t m = ballot(in_class);
+    raw = wave_sum64(raw);
+    hit = wave_sum64(hit);
+    if (lane == 0) {
+      s_raw[wid] = raw;
+      s_hit[wid] = hit;
+      s_in[wid] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t tot = s_in[0] + s_in[1] + s_in[2] + s_in[3];
+      s_base = tot ? atomicAdd(ca.list_n, tot) : 0u;
+      const uint64_t rr = s_raw[0] + s_raw[1] + s_raw[2] + s_raw[3], hh = s_hit[0] + s_hit[1] + s_hit[2] + s_hit[3];
+      const uint32_t sh = blockIdx.x & (kQcCntShards - 1);
+      if (rr) atomicAdd(&ca.sums[2 * sh], (unsigned long long)rr);
+      if (hh) atomicAdd(&ca.sums[2 * sh + 1], (unsigned long long)hh);
+    }
+    __syncthreads();
+    uint32_t pre = s_base;
+    for (uint32_t k = 0; k < wid; k++) pre += s_in[k];
+    if (in_class) ca.list[pre + mbcnt(m)] = (uint32_t)seq;
   }
 }
 
@@ -4037,12 +4121,30 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     // 1 and 2 nearly every target has a class, and the list only costs; the
     // empty candidate reads every target's Mid hashes.
     const bool fp_split = fused && h->fp_split && ml == 1 && !cfg->include_empty;
-    if (fp_split)
-      hipLaunchKernelGGL(k_fp_qc, dim3(blocks_for(nt, 4 * fpq_per_wave((uint32_t)tl))), dim3(256), 0, h->stream, tw, (uint32_t)tl,
-                         nt, cfg->bytes_model, tfp, tbytes, tcyc, nullptr, badop);
-    else
+    // QuickCheck's counters (2 per shard) and, with fp_split, the class
+    // members' list and its length (the last slot)
+    unsigned long long *qsums = nullptr;
+    uint32_t *tlist = nullptr, *tlist_n = nullptr;
+    if (fused) {
+      ENSURE(B_QCSUM, unsigned long long, 2 * kQcCntShards + 1, qsums);
+      HIPCHK(h, hipMemsetAsync(qsums, 0, (2 * kQcCntShards + 1) * 8, h->stream));
+    }
+    if (fp_split) {
+      ENSURE(B_TLIST, uint32_t, nt, tlist);
+      tlist_n = reinterpret_cast<uint32_t *>(qsums + 2 * kQcCntShards);
+      const QcClassArgs ca{cls_key, cls_dir, cls_dk, cls_rep, cls_cnt, cfp, lo_mask, qsums, tlist, tlist_n};
+      size_t ts = 0;
+      int rc = timed_begin(h, 0, &ts);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_fp_qc<true>, dim3(blocks_for(nt, 4 * fpq_per_wave((uint32_t)tl))), dim3(256), 0, h->stream,
+                         tw, (uint32_t)tl, nt, cfg->bytes_model, tfp, tbytes, tcyc, nullptr, badop, ca);
+      HIPCHK(h, hipGetLastError());
+      rc = timed_end(h, ts);
+      if (rc) return rc;
+    } else {
       hipLaunchKernelGGL(k_fingerprints, dim3(blocks_for(nt, 4 * kFpPerWave)), dim3(256), 0, h->stream, tw,
                          (uint32_t)tl, nt, cfg->bytes_model, tfp, tmh, tv0, lo_mask, tbytes, tcyc, nullptr, badop);
+    }
     HIPCHK(h, hipGetLastError());
 
     // the rounds' per-(target, pass) state, cleared while the host waits for
@@ -4060,20 +4162,14 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     if (fused) {
       // ---- QuickCheck's counters from the classes, then QuickCheck and
       // MidCheck as one join (re-run with a larger buffer on overflow)
-      unsigned long long *qsums;
-      uint32_t *tlist = nullptr, *tlist_n = nullptr;  // fp_split: the targets with a QuickCheck class
-      ENSURE(B_QCSUM, unsigned long long, 2 * kQcCntShards + 1, qsums);
-      if (fp_split) {
-        ENSURE(B_TLIST, uint32_t, nt, tlist);
-        tlist_n = reinterpret_cast<uint32_t *>(qsums + 2 * kQcCntShards);
-      }
-      HIPCHK(h, hipMemsetAsync(qsums, 0, (2 * kQcCntShards + 1) * 8, h->stream));
       {
         size_t ts = 0;
         int rc = timed_begin(h, 0, &ts);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_qc_count, dim3(blocks_for(nt, 256 * kQcCntChunks)), dim3(256), 0, h->stream, tfp, tbytes, (uint32_t)nt,
-                           lo_mask, cls_key, cls_dir, cls_dk, cls_rep, cls_cnt, cfp, qsums, tlist, tlist_n);
+        if (!fp_split)  // (with fp_split, k_fp_qc looked the classes up)
+          hipLaunchKernelGGL(k_qc_count, dim3(blocks_for(nt, 256 * kQcCntChunks)), dim3(256), 0, h->stream, tfp,
+                             tbytes, (uint32_t)nt, lo_mask, cls_key, cls_dir, cls_dk, cls_rep, cls_cnt, cfp, qsums,
+                             (uint32_t *)nullptr, (uint32_t *)nullptr);
         HIPCHK(h, hipGetLastError());
         if (fp_split)
           hipLaunchKernelGGL(k_fp_mid_list, dim3(std::min(blocks_for(nt, 4 * kFpPerWave), 8u * h->num_cu)), dim3(256), 0,
