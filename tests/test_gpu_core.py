@@ -236,6 +236,21 @@ def test_survivor_overflow_is_recovered(dev):
     same_rules(got, gold)
 
 
+@pytest.mark.parametrize("dead,name,cap", [(0xFF, "c1_go_deadFF", 1000), (0x00, "c1_go_dead00", 64),
+                                           (0x01, "c1_go_dead01", 1)])
+def test_first_hit_with_a_short_survivor_buffer(dev, dead, name, cap):
+    # the joined QuickCheck + MidCheck (no tile given) with a survivor buffer
+    # far too small: the join is re-run with a larger one, and the rules and
+    # counters are those of a run that never overflowed
+    print(f"short survivor buffer, dead 0x{dead:02X}, cap {cap}", flush=True)
+    targets = load("c1_targets")
+    got, st = dev.enumerate(targets, dead=dead, survivor_cap=cap)
+    assert st["overflow_retries"] >= 1
+    same_rules(got, load(name))
+    full, sf = dev.enumerate(targets, dead=dead)
+    assert st["mid_pass"] == sf["mid_pass"] and st["qc_hits"] == sf["qc_hits"]
+
+
 def test_odd_caller_tile_is_rounded(dev):
     # a tile that is not a whole number of QuickCheck scan steps is rounded
     # up; the rules do not depend on the tile
