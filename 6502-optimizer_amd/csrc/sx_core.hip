@@ -956,14 +956,22 @@ __device__ __forceinline__ void fp_compare(const uint2 (&a)[8], const uint2 (&b)
   okm = ((acc_lo & mlo) | acc_hi) == 0;
   okf = ((acc_lo & (0x00FFFFFFu | (kPMask << 24))) | acc_hi) == 0;
 }
+// The P bytes (under 0xCF) of the 8 QuickCheck outputs: two sequences that
+// agree under the masked compare agree under the exact one iff these do.
+__device__ __forceinline__ uint2 p_sig(const uint2 (&o)[8]) {
+  return make_uint2((o[0].x >> 24) | ((o[1].x >> 24) << 8) | ((o[2].x >> 24) << 16) | (o[3].x & 0xFF000000u),
+                    (o[4].x >> 24) | ((o[5].x >> 24) << 8) | ((o[6].x >> 24) << 16) | (o[7].x & 0xFF000000u));
+}
+constexpr uint32_t kNoClass = 0xFFFFFFFFu;
 
 __global__ __launch_bounds__(256) void k_cand_keys(const uint2 *__restrict__ cfp, const uint2 *__restrict__ cmh,
                                                    uint64_t nc, uint32_t lo_mask, unsigned long long *ckey,
-                                                   uint32_t *jkey, uint32_t *val1, uint32_t *val2) {
+                                                   uint32_t *jkey, uint32_t *val1, uint32_t *val2, uint2 *psig) {
   const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= nc) return;
   uint2 o[8];
   load_fp8(cfp, i, o);
+  psig[i] = p_sig(o);
   uint32_t k1, k2;
   qc_keys(o, lo_mask, k1, k2);
   ckey[i] = (unsigned long long)k1 | ((unsigned long long)k2 << 32);
@@ -976,17 +984,19 @@ __global__ __launch_bounds__(256) void k_class_heads(const unsigned long long *_
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) head[i] = (i == 0 || sk[i] != sk[i - 1]) ? 1u : 0u;
 }
-// class ids from the heads' exclusive scan; class tables zeroed beforehand
+// class ids from the heads' exclusive scan (ccls[slot] = its class); class
+// tables zeroed beforehand
 __global__ __launch_bounds__(256) void k_class_fill(const unsigned long long *__restrict__ sk,
                                                     const uint32_t *__restrict__ sslot, uint32_t n,
                                                     const uint32_t *__restrict__ head,
                                                     const uint64_t *__restrict__ hoff,
                                                     const uint8_t *__restrict__ cbytes,
                                                     const uint8_t *__restrict__ cprune, unsigned long long *ckey,
-                                                    uint32_t *crep, uint32_t *ccnt) {
+                                                    uint32_t *crep, uint32_t *ccnt, uint32_t *ccls) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const uint32_t cid = (uint32_t)hoff[i] + head[i] - 1u, slot = sslot[i];
+  ccls[slot] = cid;
   if (head[i]) {
     ckey[cid] = sk[i];
     crep[cid] = slot;
@@ -1025,7 +1035,8 @@ __global__ __launch_bounds__(256) void k_qc_count(const uint2 *__restrict__ tfp,
                                                   const uint32_t *__restrict__ cdir, uint32_t cdk,
                                                   const uint32_t *__restrict__ crep, const uint32_t *__restrict__ ccnt,
                                                   const uint2 *__restrict__ cfp, unsigned long long *sums,
-                                                  uint32_t *list, uint32_t *list_n) {
+                                                  uint32_t *list, uint32_t *list_n, uint32_t *tcls) {
+  // tcls[t] = the target's class (kNoClass without one)
   __shared__ unsigned long long s_raw[4], s_hit[4];
   __shared__ uint32_t s_in[kQcCntChunks][4], s_base;
   const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
@@ -1059,6 +1070,7 @@ __global__ __launch_bounds__(256) void k_qc_count(const uint2 *__restrict__ tfp,
           for (uint32_t b = 0; b < tb; b++) hit += c[b];
         }
       }
+      tcls[t] = in_class ? lo : kNoClass;
     }
     m[ch] = ballot(in_class);
     if (lane == 0) s_in[ch][w] = (uint32_t)__popcll(m[ch]);
@@ -1113,6 +1125,7 @@ struct QcClassArgs {
   uint32_t lo_mask;
   unsigned long long *sums;
   uint32_t *list, *list_n;
+  uint32_t *tcls;  // the class of each class member (others are not written)
 };
 // A wave takes fpq_per_wave(len) sequences (their words one per lane).
 __host__ __device__ constexpr uint32_t fpq_per_wave(uint32_t len) { return len <= 2 ? 32u : 16u; }
@@ -1232,6 +1245,7 @@ __global__ __launch_bounds__(256) void k_fp_qc(const uint32_t *__restrict__ word
           uint4 *dst = reinterpret_cast<uint4 *>(fp + seq * kFpVecs);
 #pragma unroll
           for (int k = 0; k < 4; k++) dst[k] = make_uint4(o[2 * k].x, o[2 * k].y, o[2 * k + 1].x, o[2 * k + 1].y);
+          ca.tcls[seq] = lo;
         }
       }
     }
@@ -1280,9 +1294,12 @@ __global__ __launch_bounds__(256) void k_qcm_join(const uint2 *__restrict__ tfp,
                                                   uint2 *__restrict__ out, uint32_t out_cap,
                                                   uint32_t *__restrict__ counters, const uint32_t *__restrict__ rank,
                                                   uint32_t *cur0, const uint32_t *__restrict__ list,
-                                                  const uint32_t *__restrict__ list_n) {
+                                                  const uint32_t *__restrict__ list_n, const uint32_t *__restrict__ tcls,
+                                                  const uint32_t *__restrict__ ccls, const uint2 *__restrict__ cpsig) {
   __shared__ uint2 queue[4][kQcQueue];
   __shared__ uint32_t s_pre[4][kWave], s_lo[4][kWave], s_key[4][kWave], s_tid[4][kWave];
+  __shared__ uint32_t s_cls[4][kWave];
+  __shared__ uint2 s_ps[4][kWave];
   const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
   uint2 *q = queue[w];
   uint32_t qn = 0;
@@ -1304,13 +1321,16 @@ __global__ __launch_bounds__(256) void k_qcm_join(const uint2 *__restrict__ tfp,
   for (uint32_t i0 = (blockIdx.x * 4 + w) * kWave; i0 < n; i0 += stride) {
   const bool have = i0 + lane < n;
   const uint32_t t = !have ? 0u : list ? list[i0 + lane] : i0 + lane;
-  uint32_t jk = 0;
+  uint32_t jk = 0, cls = kNoClass;
+  uint2 ps = make_uint2(0, 0);
   if (have) {
     uint2 o[8];
     load_fp8(tfp, t, o);
     uint32_t k1, k2;
     qc_keys(o, lo_mask, k1, k2);
     jk = join_key(k1, tmh[t].y);
+    cls = tcls[t];
+    ps = p_sig(o);
   }
   const uint32_t b = jk >> (32 - dk);
   const uint32_t lo = have ? dir[b] : 0u, c = have ? dir[b + 1] - lo : 0u;
@@ -1321,6 +1341,8 @@ __global__ __launch_bounds__(256) void k_qcm_join(const uint2 *__restrict__ tfp,
   s_lo[w][lane] = lo;
   s_key[w][lane] = jk;
   s_tid[w][lane] = t;
+  s_cls[w][lane] = cls;
+  s_ps[w][lane] = ps;
   __builtin_amdgcn_wave_barrier();
   for (uint32_t base = blockIdx.y * kWave; base < total; base += kWave * gridDim.y) {
     const uint32_t i = base + lane;
@@ -1335,11 +1357,13 @@ __global__ __launch_bounds__(256) void k_qcm_join(const uint2 *__restrict__ tfp,
       if (sj[e] == s_key[w][o]) {
         tt = s_tid[w][o];
         slot = sjslot[e];
-        uint2 a[8], bb[8];
-        load_fp8(tfp, tt, a);
-        load_fp8(cfp, slot, bb);
-        bool okm, okf;
-        fp_compare(a, bb, lo_mask, okm, okf);
+        // QuickCheck by class: the masked compare holds iff target and
+        // candidate are in one class (every member of a class equals its
+        // first under the mask); then the exact compare holds iff their P
+        // bytes agree too
+        const bool okm = ccls[slot] == s_cls[w][o];
+        const uint2 cps = cpsig[slot], tps = s_ps[w][o];
+        const bool okf = okm && cps.x == tps.x && cps.y == tps.y;
         const bool qc_hit = okm && cbytes[slot] < tbytes[tt] && cprune[slot] == 0;
         const bool exact = !hits_all && okf;  // pass 1 compares P under 0xCF
         pass = (hits_all || okf) ? 1u : 2u;
@@ -2957,7 +2981,7 @@ enum BufId {
   B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_PLANTAB, B_EFP, B_ERULES, B_L3_LIST, B_L3_HK, B_L3_CNT,
   B_L3_KEYS, B_L3_HEAD, B_L3_NEXT, B_L3_BLOOM, B_L3_RAW, B_L3_C3, B_L3_BEST, B_GKEY, B_GIDX, B_GKEY_S, B_GIDX_S,
   B_GSORT, B_QJ_KEY, B_QJ_VAL, B_QJ_HASH, B_QJ_SLOT, B_QJ_DIR, B_CK, B_CKS, B_CV1,
-  B_CV1S, B_JK, B_JV, B_JKS, B_JVS, B_JDIR, B_CHEAD, B_CHOFF, B_CLSKEY, B_CLSREP, B_CLSCNT, B_CLSN, B_QCSUM, B_TLIST, B_EVSUM, B_CLSDIR, B_COUNT
+  B_CV1S, B_JK, B_JV, B_JKS, B_JVS, B_JDIR, B_CHEAD, B_CHOFF, B_CLSKEY, B_CLSREP, B_CLSCNT, B_CLSN, B_QCSUM, B_TLIST, B_EVSUM, B_CLSDIR, B_TCLS, B_CCLS, B_CPSIG, B_COUNT
 };
 static_assert(B_COUNT <= 96, "sx_handle::bufs holds every buffer id");
 
@@ -3940,6 +3964,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
   uint64_t *cls_n = nullptr;
   uint32_t *cls_rep = nullptr, *cls_cnt = nullptr, *j_key = nullptr, *j_slot = nullptr, *j_dir = nullptr, j_dk = 1;
   uint32_t *cls_dir = nullptr, cls_dk = 1;  // the classes' bucket directory on the top cls_dk key bits
+  uint32_t *ccls_id = nullptr;              // each candidate's class
+  uint2 *cps_sig = nullptr;                 // each candidate's P bytes on the QuickCheck vectors
   if (fused) {
     while (j_dk < 24 && (1ull << j_dk) < nc) j_dk++;
     unsigned long long *ck, *cks;
@@ -3960,6 +3986,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     ENSURE(B_CLSREP, uint32_t, nc, cls_rep);
     ENSURE(B_CLSCNT, uint32_t, nc * (kClsBins + 1), cls_cnt);
     ENSURE(B_CLSN, uint64_t, 2, cls_n);
+    ENSURE(B_CCLS, uint32_t, nc, ccls_id);
+    ENSURE(B_CPSIG, uint2, nc, cps_sig);
     while (cls_dk < 24 && (1ull << cls_dk) < nc) cls_dk++;
     ENSURE(B_CLSDIR, uint32_t, (1ull << cls_dk) + 1, cls_dir);
     bad = reinterpret_cast<uint32_t *>(cls_n + 1);
@@ -3968,7 +3996,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
       fused = h->cand.fused_ok;
     } else {
       hipLaunchKernelGGL(k_cand_keys, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cfp, cmh, nc, lo_mask, ck, jk,
-                         v1, v2);
+                         v1, v2, cps_sig);
       HIPCHK(h, hipGetLastError());
       size_t tb1 = 0, tb2 = 0;
       HIPCHK(h, rocprim::radix_sort_pairs(nullptr, tb1, ck, cks, v1, v1s, n32, 0, 64, h->stream));
@@ -3988,7 +4016,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
                          nc * (kClsBins + 1), 0u);
       hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, h->stream, bad, (uint64_t)1, 0u);
       hipLaunchKernelGGL(k_class_fill, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, cks, v1s, n32, head, hoff,
-                         cbytes, cprune, cls_key, cls_rep, cls_cnt);
+                         cbytes, cprune, cls_key, cls_rep, cls_cnt, ccls_id);
       hipLaunchKernelGGL(k_class_verify, dim3(blocks_for(nc, 256)), dim3(256), 0, h->stream, v1s, n32, head, hoff,
                          cls_rep, cfp, lo_mask, bad);
       hipLaunchKernelGGL(k_class_dir, dim3(blocks_for((1ull << cls_dk) + 1, 256)), dim3(256), 0, h->stream, cls_key,
@@ -4124,15 +4152,16 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     // QuickCheck's counters (2 per shard) and, with fp_split, the class
     // members' list and its length (the last slot)
     unsigned long long *qsums = nullptr;
-    uint32_t *tlist = nullptr, *tlist_n = nullptr;
+    uint32_t *tlist = nullptr, *tlist_n = nullptr, *tcls = nullptr;
     if (fused) {
+      ENSURE(B_TCLS, uint32_t, nt, tcls);
       ENSURE(B_QCSUM, unsigned long long, 2 * kQcCntShards + 1, qsums);
       HIPCHK(h, hipMemsetAsync(qsums, 0, (2 * kQcCntShards + 1) * 8, h->stream));
     }
     if (fp_split) {
       ENSURE(B_TLIST, uint32_t, nt, tlist);
       tlist_n = reinterpret_cast<uint32_t *>(qsums + 2 * kQcCntShards);
-      const QcClassArgs ca{cls_key, cls_dir, cls_dk, cls_rep, cls_cnt, cfp, lo_mask, qsums, tlist, tlist_n};
+      const QcClassArgs ca{cls_key, cls_dir, cls_dk, cls_rep, cls_cnt, cfp, lo_mask, qsums, tlist, tlist_n, tcls};
       size_t ts = 0;
       int rc = timed_begin(h, 0, &ts);
       if (rc) return rc;
@@ -4169,7 +4198,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         if (!fp_split)  // (with fp_split, k_fp_qc looked the classes up)
           hipLaunchKernelGGL(k_qc_count, dim3(blocks_for(nt, 256 * kQcCntChunks)), dim3(256), 0, h->stream, tfp,
                              tbytes, (uint32_t)nt, lo_mask, cls_key, cls_dir, cls_dk, cls_rep, cls_cnt, cfp, qsums,
-                             (uint32_t *)nullptr, (uint32_t *)nullptr);
+                             (uint32_t *)nullptr, (uint32_t *)nullptr, tcls);
         HIPCHK(h, hipGetLastError());
         if (fp_split)
           hipLaunchKernelGGL(k_fp_mid_list, dim3(std::min(blocks_for(nt, 4 * kFpPerWave), 8u * h->num_cu)), dim3(256), 0,
@@ -4196,7 +4225,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
                            dim3(fp_split ? std::min(blocks_for(nt, 256), 16u * h->num_cu) : blocks_for(nt, 256), parts),
                            dim3(256), 0, h->stream, tfp, tmh, tbytes, (uint32_t)nt, lo_mask, j_key, j_slot, j_dir,
                            j_dk, cfp, cmh, cbytes, cprune, cfg->hits, mid, mid_cap, counters, rank,
-                           cfg->hits ? nullptr : cur, tlist, tlist_n);
+                           cfg->hits ? nullptr : cur, tlist, tlist_n, tcls, ccls_id, cps_sig);
         HIPCHK(h, hipGetLastError());
         rc = timed_end(h, ts);
         if (rc) return rc;
