@@ -78,6 +78,23 @@ struct SurvRegions {
   uint32_t pre[kQcRegions + 1];  // region r holds list entries [pre[r], pre[r + 1])
   uint32_t cap;                  // entries per region
 };
+// The Mid survivor list in regions with a counter each, 256 bytes apart (a
+// wave flushes into region wave mod kMidRegions: one counter for all flushes
+// serialised the joined QuickCheck + MidCheck at --dead-flags 0xFF, 470M
+// survivors); list entry i is in region r = the last with pre[r] <= i, at
+// r * cap + i - pre[r].  A single list is region 0 with pre[1..] = its length.
+constexpr uint32_t kMidRegions = 16, kMidCntStride = 64;
+struct MidList {
+  uint32_t pre[kMidRegions + 1];
+  uint32_t cap;
+  __device__ __forceinline__ uint64_t at(uint32_t i) const {
+    uint32_t r = 0;
+#pragma unroll
+    for (uint32_t k = 1; k < kMidRegions; k++) r += i >= pre[k] ? 1u : 0u;
+    return (uint64_t)r * cap + (i - pre[r]);
+  }
+  __device__ __forceinline__ uint32_t size() const { return pre[kMidRegions]; }
+};
 constexpr int kEvalShards = 1024;       // shards of the evaluated-states counter
 constexpr size_t kPinBytes = 64 << 10;  // pinned read-back staging per handle
 constexpr uint64_t kCopyPiece = 1u << 16;  // rules per piece of the copy to the host
@@ -1292,7 +1309,7 @@ __global__ __launch_bounds__(256) void k_qcm_join(const uint2 *__restrict__ tfp,
                                                   const uint8_t *__restrict__ cbytes,
                                                   const uint8_t *__restrict__ cprune, int hits_all,
                                                   uint2 *__restrict__ out, uint32_t out_cap,
-                                                  uint32_t *__restrict__ counters, const uint32_t *__restrict__ rank,
+                                                  uint32_t *__restrict__ mcnt, const uint32_t *__restrict__ rank,
                                                   uint32_t *cur0, const uint32_t *__restrict__ list,
                                                   const uint32_t *__restrict__ list_n, const uint32_t *__restrict__ tcls,
                                                   const uint32_t *__restrict__ ccls, const uint2 *__restrict__ cpsig) {
@@ -1303,14 +1320,17 @@ __global__ __launch_bounds__(256) void k_qcm_join(const uint2 *__restrict__ tfp,
   const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
   uint2 *q = queue[w];
   uint32_t qn = 0;
+  // this wave's survivor region: counter mcnt[region * kMidCntStride], entries from region * out_cap
+  const uint32_t region = ((blockIdx.y * gridDim.x + blockIdx.x) * 4 + w) % kMidRegions;
+  uint2 *rout = out + (uint64_t)region * out_cap;
   auto flush = [&]() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&counters[1], qn);
+    if (lane == 0) base = atomicAdd(&mcnt[region * kMidCntStride], qn);
     base = uni(base);
     for (uint32_t i = lane; i < qn; i += kWave)
-      if (base + i < out_cap) out[base + i] = q[i];
+      if (base + i < out_cap) rout[base + i] = q[i];
     __builtin_amdgcn_wave_barrier();
     qn = 0;
   };
@@ -1397,13 +1417,13 @@ __global__ void k_seg_clear(uint32_t *cur, uint32_t *rej, uint8_t *done, uint32_
 
 // First-hit rounds.  cur/rej/done are per (target, pass-1): cur = lowest rank
 // this round, rej = 1 + highest rank rejected so far, done = resolved.
-__global__ __launch_bounds__(256) void k_round_min(const uint2 *mid, uint32_t nm,
+__global__ __launch_bounds__(256) void k_round_min(const uint2 *mid, MidList ml,
                                                    const uint32_t *rank, uint32_t *cur,
                                                    const uint32_t *rej, const uint8_t *done) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   uint32_t key = kRankNone, r = 0;
-  if (i < nm) {
-    const uint2 e = mid[i];
+  if (i < ml.size()) {
+    const uint2 e = mid[ml.at(i)];
     const uint32_t slot = e.y & 0x3FFFFFFFu, k = e.x * 2 + (e.y >> 30) - 1;
     r = rank[slot];
     if (!done[k] && r + 1 > rej[k]) key = k;
@@ -1578,7 +1598,7 @@ __device__ __forceinline__ void write_group_key(const Job &j, uint32_t i, uint32
 // (all hits, where every lane takes a job); in first-hit mode one survivor in
 // several takes one, so the planner would run on mostly idle waves and the
 // jobs are planned afterwards, densely, by k_exh_plan.
-__global__ __launch_bounds__(kBlk) void k_round_jobs(const uint2 *mid, uint32_t nm, const uint32_t *rank,
+__global__ __launch_bounds__(kBlk) void k_round_jobs(const uint2 *mid, MidList ml, const uint32_t *rank,
                                                      const uint32_t *cur, const uint32_t *rej, const uint8_t *done,
                                                      const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
                                                      int hits_all, Job *jobs, uint32_t job_cap, uint32_t *njobs,
@@ -1590,8 +1610,8 @@ __global__ __launch_bounds__(kBlk) void k_round_jobs(const uint2 *mid, uint32_t 
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
   bool take = false, rest = false;
   uint2 e = make_uint2(0, 0);
-  if (i < nm) {
-    e = mid[i];
+  if (i < ml.size()) {
+    e = mid[ml.at(i)];
     if (hits_all) {
       take = true;
     } else {
@@ -2270,7 +2290,7 @@ __global__ __launch_bounds__(256) void k_round_resolve(const Job *jobs, uint32_t
 // (rank >= rej) becomes a job at once, and the smallest passing Go rank is
 // the first hit (the rounds would reach the same candidate one rank at a
 // time).
-__global__ __launch_bounds__(kBlk) void k_bulk_jobs(const uint2 *mid, uint32_t nm, const uint32_t *rank,
+__global__ __launch_bounds__(kBlk) void k_bulk_jobs(const uint2 *mid, MidList ml, const uint32_t *rank,
                                                     const uint32_t *rej, const uint8_t *done,
                                                     const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
                                                     Job *jobs, uint32_t job_cap, uint32_t *njobs, int plan_mode,
@@ -2281,8 +2301,8 @@ __global__ __launch_bounds__(kBlk) void k_bulk_jobs(const uint2 *mid, uint32_t n
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
   bool take = false;
   uint2 e = make_uint2(0, 0);
-  if (i < nm) {
-    e = mid[i];
+  if (i < ml.size()) {
+    e = mid[ml.at(i)];
     uint32_t slot = e.y & 0x3FFFFFFFu, p = (e.y >> 30) - 1;
     uint64_t k = (uint64_t)e.x * 2 + p;
     take = !done[k] && rank[slot] >= rej[k];
@@ -2981,7 +3001,7 @@ enum BufId {
   B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_PLANTAB, B_EFP, B_ERULES, B_L3_LIST, B_L3_HK, B_L3_CNT,
   B_L3_KEYS, B_L3_HEAD, B_L3_NEXT, B_L3_BLOOM, B_L3_RAW, B_L3_C3, B_L3_BEST, B_GKEY, B_GIDX, B_GKEY_S, B_GIDX_S,
   B_GSORT, B_QJ_KEY, B_QJ_VAL, B_QJ_HASH, B_QJ_SLOT, B_QJ_DIR, B_CK, B_CKS, B_CV1,
-  B_CV1S, B_JK, B_JV, B_JKS, B_JVS, B_JDIR, B_CHEAD, B_CHOFF, B_CLSKEY, B_CLSREP, B_CLSCNT, B_CLSN, B_QCSUM, B_TLIST, B_EVSUM, B_CLSDIR, B_TCLS, B_CCLS, B_CPSIG, B_COUNT
+  B_CV1S, B_JK, B_JV, B_JKS, B_JVS, B_JDIR, B_CHEAD, B_CHOFF, B_CLSKEY, B_CLSREP, B_CLSCNT, B_CLSN, B_QCSUM, B_TLIST, B_EVSUM, B_CLSDIR, B_TCLS, B_CCLS, B_CPSIG, B_MIDCNT, B_COUNT
 };
 static_assert(B_COUNT <= 96, "sx_handle::bufs holds every buffer id");
 
@@ -4186,6 +4206,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     ENSURE(B_RULEFLAG, uint32_t, nt * 2, rule_flag);
     uint2 *mid;
     uint32_t nmid = 0;
+    MidList mlist{};  // where the nmid survivors are in mid
     uint32_t *counters;  // [0, 8): MidCheck and the rounds; [8, 16): QuickCheck's region counts
     ENSURE(B_COUNTERS, uint32_t, 8 + kQcRegions, counters);
     if (fused) {
@@ -4208,11 +4229,16 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         if (rc) return rc;
       }
       HIPCHK(h, hipEventRecord(e2, h->stream));
-      // the caller's survivor buffer size, else what sufficed before, else 2^22
-      uint32_t mid_cap = cfg->survivor_cap ? cfg->survivor_cap : std::max<uint32_t>(h->mid_cap_hint, 1u << 22);
+      // the caller's survivor buffer size, else what sufficed before, else
+      // 2^22, over kMidRegions regions
+      const uint32_t mid_total = cfg->survivor_cap ? cfg->survivor_cap : std::max<uint32_t>(h->mid_cap_hint, 1u << 22);
+      uint32_t mid_cap = (mid_total + kMidRegions - 1) / kMidRegions;  // per region
+      uint32_t *mcnt;
+      ENSURE(B_MIDCNT, uint32_t, kMidRegions * kMidCntStride, mcnt);
       for (;;) {
-        ENSURE(B_MID, uint2, mid_cap, mid);
+        ENSURE(B_MID, uint2, (uint64_t)mid_cap * kMidRegions, mid);
         HIPCHK(h, hipMemsetAsync(counters, 0, 8 * 4, h->stream));  // counters[2..7] stay 0 for the rounds
+        HIPCHK(h, hipMemsetAsync(mcnt, 0, kMidRegions * kMidCntStride * 4, h->stream));
         hipLaunchKernelGGL(k_seg_clear, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, rej, done,
                            cfg->hits ? nullptr : rule_flag, nt * 2);
         size_t ts = 0;
@@ -4224,24 +4250,35 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         hipLaunchKernelGGL(k_qcm_join,
                            dim3(fp_split ? std::min(blocks_for(nt, 256), 16u * h->num_cu) : blocks_for(nt, 256), parts),
                            dim3(256), 0, h->stream, tfp, tmh, tbytes, (uint32_t)nt, lo_mask, j_key, j_slot, j_dir,
-                           j_dk, cfp, cmh, cbytes, cprune, cfg->hits, mid, mid_cap, counters, rank,
+                           j_dk, cfp, cmh, cbytes, cprune, cfg->hits, mid, mid_cap, mcnt, rank,
                            cfg->hits ? nullptr : cur, tlist, tlist_n, tcls, ccls_id, cps_sig);
         HIPCHK(h, hipGetLastError());
         rc = timed_end(h, ts);
         if (rc) return rc;
         struct {
-          uint32_t c[2];
+          uint32_t c[kMidRegions * kMidCntStride];
           unsigned long long q[2 * kQcCntShards];
         } rb;
-        rc = read_back(h, {{rb.c, counters, 8}, {rb.q, qsums, sizeof(rb.q)}}, "QuickCheck and MidCheck");
+        rc = read_back(h, {{rb.c, mcnt, sizeof(rb.c)}, {rb.q, qsums, sizeof(rb.q)}}, "QuickCheck and MidCheck");
         if (rc) return rc;
-        nmid = rb.c[1];
-        if (nmid > mid_cap) {
+        uint64_t total = 0;
+        uint32_t most = 0;
+        for (uint32_t r = 0; r < kMidRegions; r++) {
+          const uint32_t nr = rb.c[r * kMidCntStride];
+          mlist.pre[r] = (uint32_t)std::min<uint64_t>(total, 0xFFFFFFFFull);
+          total += nr;
+          most = std::max(most, nr);
+        }
+        if (total > 0xFFFFFFFFull) return set_err(h, SX_ENOMEM, "MidCheck survivors (%llu)", (unsigned long long)total);
+        mlist.pre[kMidRegions] = (uint32_t)total;
+        mlist.cap = mid_cap;
+        nmid = (uint32_t)total;
+        if (most > mid_cap) {
           st.overflow_retries++;
-          if ((uint64_t)nmid * 5 / 4 + 1024 > 0xFFFFFFFFull)
-            return set_err(h, SX_ENOMEM, "MidCheck survivor buffer (%u)", nmid);
-          mid_cap = nmid + nmid / 4 + 1024;
-          if (!cfg->survivor_cap) h->mid_cap_hint = std::max(h->mid_cap_hint, mid_cap);
+          if ((uint64_t)most * 5 / 4 + 1024 > 0xFFFFFFFFull / kMidRegions)
+            return set_err(h, SX_ENOMEM, "MidCheck survivor buffer (%u in one region)", most);
+          mid_cap = most + most / 4 + 1024;
+          if (!cfg->survivor_cap) h->mid_cap_hint = std::max(h->mid_cap_hint, mid_cap * kMidRegions);
           continue;
         }
         uint64_t raw = 0, hits = 0;
@@ -4331,6 +4368,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         }
         st.qc_hits += cnt[0];
         nmid = cnt[1];
+        for (uint32_t r = 0; r <= kMidRegions; r++) mlist.pre[r] = r ? nmid : 0u;  // one list
+        mlist.cap = std::max<uint32_t>(nmid, 1);
         trace(h, "  MidCheck: %u QC hits after filters, %u pass", cnt[0], cnt[1]);
         break;
       }
@@ -4382,7 +4421,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
           ENSURE(B_GKEY, uint32_t, job_cap, bkey);
           ENSURE(B_GIDX, uint32_t, job_cap, bidx);
         }
-        hipLaunchKernelGGL(k_bulk_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, nmid, rank,
+        hipLaunchKernelGGL(k_bulk_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, mlist, rank,
                            rej, done, tw, (uint32_t)tl, pm_masked, jobs, job_cap, njobs_d, cfg->exh_mode, ptab,
                            jchunks, jfail, bkey, bidx);
         HIPCHK(h, hipGetLastError());
@@ -4418,14 +4457,14 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
                            (uint32_t)(nt * 2), tw, (uint32_t)tl, pm_masked, jobs, job_cap, njobs_d);
       } else {
         if (!cfg->hits)
-          hipLaunchKernelGGL(k_round_min, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, nmid, rank,
+          hipLaunchKernelGGL(k_round_min, dim3(blocks_for(nmid, 256)), dim3(256), 0, h->stream, mid, mlist, rank,
                              cur, rej, done);
         // all hits: the jobs are planned (and keyed) here
         if (cfg->hits && h->exh_group) {
           ENSURE(B_GKEY, uint32_t, job_cap, rkey);
           ENSURE(B_GIDX, uint32_t, job_cap, ridx);
         }
-        hipLaunchKernelGGL(k_round_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, nmid, rank,
+        hipLaunchKernelGGL(k_round_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, mlist, rank,
                            cur, rej, done, tw, (uint32_t)tl, pm_masked, cfg->hits, jobs, job_cap, njobs_d, nrest_d,
                            cfg->hits ? 1 : 0, cfg->exh_mode, ptab, jchunks, jfail, rkey, ridx);
       }
