@@ -1404,6 +1404,12 @@ __global__ __launch_bounds__(256) void k_qcm_join(const uint2 *__restrict__ tfp,
   if (qn) flush();
 }
 
+// The survivor regions' counters (kMidCntStride apart) gathered into one
+// block, so that the host reads them in one short copy.
+__global__ void k_gather_counts(const uint32_t *mcnt, uint32_t *out) {
+  if (threadIdx.x < kMidRegions) out[threadIdx.x] = mcnt[threadIdx.x * kMidCntStride];
+}
+
 // A segment's first-hit state, cleared in one launch: cur = none, rej = 0,
 // done = 0 and, when given, the rule slots' flags = 0.
 __global__ void k_seg_clear(uint32_t *cur, uint32_t *rej, uint8_t *done, uint32_t *rule_flag, uint64_t n) {
@@ -4175,7 +4181,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     uint32_t *tlist = nullptr, *tlist_n = nullptr, *tcls = nullptr;
     if (fused) {
       ENSURE(B_TCLS, uint32_t, nt, tcls);
-      ENSURE(B_QCSUM, unsigned long long, 2 * kQcCntShards + 1, qsums);
+      // [0, 2 * shards): counters; then the list length; then the gathered region counts
+      ENSURE(B_QCSUM, unsigned long long, 2 * kQcCntShards + 1 + kMidRegions / 2, qsums);
       HIPCHK(h, hipMemsetAsync(qsums, 0, (2 * kQcCntShards + 1) * 8, h->stream));
     }
     if (fp_split) {
@@ -4255,16 +4262,19 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         HIPCHK(h, hipGetLastError());
         rc = timed_end(h, ts);
         if (rc) return rc;
+        uint32_t *gathered = reinterpret_cast<uint32_t *>(qsums + 2 * kQcCntShards + 1);
+        hipLaunchKernelGGL(k_gather_counts, dim3(1), dim3(64), 0, h->stream, mcnt, gathered);
+        HIPCHK(h, hipGetLastError());
         struct {
-          uint32_t c[kMidRegions * kMidCntStride];
-          unsigned long long q[2 * kQcCntShards];
+          unsigned long long q[2 * kQcCntShards + 1 + kMidRegions / 2];
         } rb;
-        rc = read_back(h, {{rb.c, mcnt, sizeof(rb.c)}, {rb.q, qsums, sizeof(rb.q)}}, "QuickCheck and MidCheck");
+        rc = read_back(h, {{rb.q, qsums, sizeof(rb.q)}}, "QuickCheck and MidCheck");
         if (rc) return rc;
+        const uint32_t *rbc = reinterpret_cast<const uint32_t *>(rb.q + 2 * kQcCntShards + 1);
         uint64_t total = 0;
         uint32_t most = 0;
         for (uint32_t r = 0; r < kMidRegions; r++) {
-          const uint32_t nr = rb.c[r * kMidCntStride];
+          const uint32_t nr = rbc[r];
           mlist.pre[r] = (uint32_t)std::min<uint64_t>(total, 0xFFFFFFFFull);
           total += nr;
           most = std::max(most, nr);
