@@ -386,14 +386,17 @@ def main():
     states_grp = sum(s["exh_group_states"] for s in stats)
     states = sum(s["exh_states"] for s in stats) - states_grp
     kernels = {
-        # QuickCheck's timed launches: the class lookup, the class members' Mid
-        # hashes and the joined QuickCheck + MidCheck (k_qc_count,
-        # k_fp_mid_list, k_qcm_join) on the default path; k_quickcheck on the
-        # separate path (a caller's tile)
+        # QuickCheck's timed launches: the targets' QuickCheck outputs with the
+        # class lookup, the class members' Mid hashes and the joined
+        # QuickCheck + MidCheck (k_fp_qc, k_fp_mid_list, k_qcm_join) on the
+        # default path; k_qc_count + k_qcm_join with SX_FP_SPLIT=0, whose
+        # fingerprints are not timed; k_quickcheck on the separate path (a
+        # caller's tile)
         "quickcheck": {"ops_per_unit": OPS_QC_PAIR, "unit": "QC pair", "units": PAIRS * len(stats),
                        "launches": n_qc, "kernel_ms": k_qc, "avg_launch_ms": k_qc / max(n_qc, 1),
-                       "kernels": "k_qc_count + k_fp_mid_list + k_qcm_join" if st["qc_tile"] == 0
-                       and st["qc_blocks_per_cu"] == 0 else "k_quickcheck"},
+                       "kernels": ("k_quickcheck" if st["qc_tile"] or st["qc_blocks_per_cu"]
+                                   else "k_qc_count + k_qcm_join" if os.environ.get("SX_FP_SPLIT", "1") == "0"
+                                   else "k_fp_qc + k_fp_mid_list + k_qcm_join")},
         # the exhaustive sweep: jobs with small plans one per lane (k_exh_group),
         # the others one chunk of states per wave (k_exh_run)
         "k_exh_group": {"ops_per_unit": OPS_EXH_STATE, "unit": "exhaustive state", "units": states_grp,
