@@ -234,22 +234,91 @@ __global__ __launch_bounds__(256) void k_gen3_write(uint64_t r0, uint32_t n, con
   }
 }
 
-// Per-block exclusive scan of uint32 counts into uint64 offsets (3 kernels).
-__global__ __launch_bounds__(1024) void k_scan_block(const uint32_t *in, uint64_t n, uint64_t *out,
-                                                     uint64_t *block_sums) {
-  __shared__ uint64_t s[1024];
-  uint64_t i = (uint64_t)blockIdx.x * 1024 + threadIdx.x;
-  uint64_t v = i < n ? in[i] : 0;
-  s[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    uint64_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
-    __syncthreads();
-    s[threadIdx.x] += t;
-    __syncthreads();
+// Exclusive scan of uint32 counts into uint64 offsets, reduce-then-scan over
+// tiles of 4096 counts (256 threads x 16 consecutive counts, read as four
+// 16-byte loads): k_tile_sums writes each tile's sum, k_scan_sums scans the
+// sums in one block, k_tile_scan re-reads the tile and writes its offsets.
+// The counts are read twice and the offsets written once.
+constexpr uint32_t kScanItems = 16, kScanTile = 256 * kScanItems;
+
+__device__ __forceinline__ void scan_load(const uint32_t *in, uint64_t n, uint64_t base, uint32_t (&v)[kScanItems]) {
+  if (base + kScanItems <= n) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(in + base);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint4 x = p[q];
+      v[4 * q] = x.x;
+      v[4 * q + 1] = x.y;
+      v[4 * q + 2] = x.z;
+      v[4 * q + 3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < (int)kScanItems; i++) v[i] = base + i < n ? in[base + i] : 0u;
   }
-  if (i < n) out[i] = s[threadIdx.x] - v;
-  if (threadIdx.x == 1023) block_sums[blockIdx.x] = s[1023];
+}
+
+// Exclusive prefix of x over the 256 threads of the block, and the block's
+// total; it synchronises once (s_w: 4 entries, not reused by the caller).
+__device__ __forceinline__ uint64_t block_scan256(uint64_t x, uint64_t *s_w, uint64_t *total) {
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint64_t inc = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t y = __shfl_up(inc, d);
+    if (lane >= (uint32_t)d) inc += y;
+  }
+  if (lane == 63) s_w[w] = inc;
+  __syncthreads();
+  uint64_t pre = 0, tot = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; k++) {
+    pre += k < w ? s_w[k] : 0ull;
+    tot += s_w[k];
+  }
+  *total = tot;
+  return pre + inc - x;
+}
+
+// Flags: count the nonzero entries instead of adding them.
+__global__ __launch_bounds__(256) void k_tile_sums(const uint32_t *in, uint64_t n, uint64_t *sums, int flags) {
+  __shared__ uint64_t s_w[4];
+  uint32_t v[kScanItems];
+  scan_load(in, n, (uint64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems, v);
+  uint64_t t = 0;
+#pragma unroll
+  for (int i = 0; i < (int)kScanItems; i++) t += flags ? (uint64_t)(v[i] != 0u) : (uint64_t)v[i];
+  uint64_t tot = 0;
+  (void)block_scan256(t, s_w, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_tile_scan(const uint32_t *in, uint64_t n, const uint64_t *sums,
+                                                   uint64_t *out) {
+  __shared__ uint64_t s_w[4];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  scan_load(in, n, base, v);
+  uint64_t t = 0;
+#pragma unroll
+  for (int i = 0; i < (int)kScanItems; i++) t += v[i];
+  uint64_t tot = 0;
+  uint64_t run = block_scan256(t, s_w, &tot) + sums[blockIdx.x];
+  uint64_t o[kScanItems];
+#pragma unroll
+  for (int i = 0; i < (int)kScanItems; i++) {
+    o[i] = run;
+    run += v[i];
+  }
+  if (base + kScanItems <= n) {
+    ulonglong2 *q = reinterpret_cast<ulonglong2 *>(out + base);
+#pragma unroll
+    for (int i = 0; i < (int)kScanItems / 2; i++) q[i] = make_ulonglong2(o[2 * i], o[2 * i + 1]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < (int)kScanItems; i++)
+      if (base + i < n) out[base + i] = o[i];
+  }
 }
 
 __global__ __launch_bounds__(1024) void k_scan_sums(uint64_t *sums, uint64_t nb, uint64_t *total) {
@@ -274,11 +343,6 @@ __global__ __launch_bounds__(1024) void k_scan_sums(uint64_t *sums, uint64_t nb,
     __syncthreads();
   }
   if (threadIdx.x == 0) *total = carry;
-}
-
-__global__ __launch_bounds__(1024) void k_scan_add(uint64_t *out, uint64_t n, const uint64_t *sums) {
-  uint64_t i = (uint64_t)blockIdx.x * 1024 + threadIdx.x;
-  if (i < n) out[i] += sums[blockIdx.x];
 }
 
 // Filter hashes.  qc_mix(lo, hi, v) hashes one vector's output (64 bits and
@@ -1656,7 +1720,9 @@ __global__ __launch_bounds__(kBlk) void k_round_jobs(const uint2 *mid, MidList m
 // The first round's jobs (first hit), one lane per (target, pass) entry k
 // whose minimum the MidCheck kernel recorded (cur[k] != none) instead of one
 // per survivor; rank == slot (candidate slots are numbered in Go order), so
-// the minimum names the candidate.  The jobs are planned afterwards.
+// the minimum names the candidate.  The jobs are planned afterwards, densely,
+// by k_exh_plan (planning here, even by the table, made this kernel slower
+// than the two together: most of its lanes take no job).
 __global__ __launch_bounds__(kBlk) void k_entry_jobs(const uint32_t *cur, uint32_t nk, const uint32_t *targets,
                                                      uint32_t tl, uint32_t pm_masked, Job *jobs, uint32_t job_cap,
                                                      uint32_t *njobs) {
@@ -2753,12 +2819,31 @@ __global__ void k_fix_raw_index(sx_rule *rules, uint32_t n, const uint32_t *raw_
   if (i < n) rules[i].target_index = base + raw_off[rules[i].target_index - base];
 }
 
-// Ordered compaction of the first-hit rule slots (flag[k] = 1 where slot k
-// holds a rule; offs = exclusive scan of the flags).
-__global__ __launch_bounds__(256) void k_compact_rules(const sx_rule *slots, const uint32_t *flag,
-                                                       const uint64_t *offs, uint64_t n, sx_rule *out) {
-  uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (k < n && flag[k]) out[offs[k]] = slots[k];
+// Ordered compaction of the first-hit rule slots (flag[k] != 0 where slot k
+// holds a rule), tile by tile as in the scan: sums = the exclusive scan of
+// the tiles' flag counts (k_tile_sums with flags, then k_scan_sums).  The
+// tile's filled slots are listed in LDS in order, then the block copies the
+// listed records with consecutive threads (one record per thread and pass),
+// so the copies of a tile are all in flight at once.
+__global__ __launch_bounds__(256) void k_compact_rules(const sx_rule *slots, const uint32_t *flag, uint64_t n,
+                                                       const uint64_t *sums, sx_rule *out) {
+  __shared__ uint64_t s_w[4];
+  __shared__ uint16_t s_idx[kScanTile];
+  const uint32_t t0 = threadIdx.x * kScanItems;
+  const uint64_t tile = (uint64_t)blockIdx.x * kScanTile;
+  uint32_t v[kScanItems];
+  scan_load(flag, n, tile + t0, v);
+  uint32_t t = 0;
+#pragma unroll
+  for (int i = 0; i < (int)kScanItems; i++) t += v[i] != 0u;
+  uint64_t tot = 0;
+  uint32_t o = (uint32_t)block_scan256(t, s_w, &tot);
+#pragma unroll
+  for (int i = 0; i < (int)kScanItems; i++)
+    if (v[i]) s_idx[o++] = (uint16_t)(t0 + i);
+  __syncthreads();
+  const uint64_t ob = sums[blockIdx.x];
+  for (uint32_t j = threadIdx.x; j < (uint32_t)tot; j += 256) out[ob + j] = slots[tile + s_idx[j]];
 }
 
 // Rules of length-2 targets against single-instruction candidates packed
@@ -2914,7 +2999,7 @@ struct sx_handle {
   int device = 0;
   int num_cu = 256;
   bool poisoned = false;  // a device stage timed out: the stream is unusable
-  int bulk_after = 1;     // first-hit rounds before the one-batch tail (SX_BULK_AFTER); -1 = by survivor count
+  int bulk_after = -2;    // first-hit rounds before the one-batch tail (SX_BULK_AFTER); -1 = by survivor count, -2 = 0 or 1 by survivor count
   int exh_k = 2;          // states (pairs when packed) per lane and opcode dispatch in the sweep (SX_EXH_K: 1 or 2)
   int exh_pk = 1;         // packed two-states-per-register interpreter in the sweep (SX_EXH_PK=0: one state)
   int qcm_join = 1;       // QuickCheck and MidCheck as one join when MidCheck is on (SX_QCM_JOIN=0: separate)
@@ -3003,7 +3088,7 @@ enum BufId {
   B_TARGETS, B_TFP, B_TV0, B_TBYTES, B_TCYC, B_CWORDS, B_CFP, B_CBYTES, B_CCYC, B_CPRUNE,
   B_RANK, B_SURV, B_MID, B_COUNTERS, B_CUR, B_REJ, B_DONE, B_JOBS, B_CHUNKS, B_OFFS, B_SUMS,
   B_FAIL, B_RULES, B_SCRATCH, B_GEN_COUNTS, B_GEN_OFFS, B_BADOP, B_P_TW, B_P_TL, B_P_CW, B_P_CL,
-  B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR, B_HEADS, B_RULEFLAG, B_RULEOFFS, B_RULES2,
+  B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR, B_HEADS, B_RULEFLAG, B_RULES2,
   B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_PLANTAB, B_EFP, B_ERULES, B_L3_LIST, B_L3_HK, B_L3_CNT,
   B_L3_KEYS, B_L3_HEAD, B_L3_NEXT, B_L3_BLOOM, B_L3_RAW, B_L3_C3, B_L3_BEST, B_GKEY, B_GIDX, B_GKEY_S, B_GIDX_S,
   B_GSORT, B_QJ_KEY, B_QJ_VAL, B_QJ_HASH, B_QJ_SLOT, B_QJ_DIR, B_CK, B_CKS, B_CV1,
@@ -3231,17 +3316,19 @@ void unpack_rules(sx_rule *out, const uint32_t *w, uint64_t n, int model) {
 }
 
 // Exclusive scan of n uint32 counts into uint64 offsets; *total on device.
+// (in and out 16-byte aligned: the tiles move 16 bytes per access)
 int scan_u32(sx_handle *h, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *d_total) {
-  uint64_t nb = (n + 1023) / 1024;
+  const uint64_t nb = (n + kScanTile - 1) / kScanTile;
   uint64_t *sums;
   ENSURE(B_SUMS, uint64_t, nb + 1, sums);
   if (n == 0) {
     HIPCHK(h, hipMemsetAsync(d_total, 0, sizeof(uint64_t), h->stream));
     return SX_OK;
   }
-  hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(1024), 0, h->stream, in, n, out, sums);
+  if (((uintptr_t)in | (uintptr_t)out) & 15u) return set_err(h, SX_EINVAL, "scan buffers not 16-byte aligned");
+  hipLaunchKernelGGL(k_tile_sums, dim3((unsigned)nb), dim3(256), 0, h->stream, in, n, sums, 0);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, h->stream, sums, nb, d_total);
-  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(1024), 0, h->stream, out, n, sums);
+  hipLaunchKernelGGL(k_tile_scan, dim3((unsigned)nb), dim3(256), 0, h->stream, in, n, (const uint64_t *)sums, out);
   HIPCHK(h, hipGetLastError());
   return SX_OK;
 }
@@ -3412,11 +3499,17 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, bo
     if (!keyed)
       hipLaunchKernelGGL(k_group_keys, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, jobs, nj, chunks, key, idx);
     HIPCHK(h, hipGetLastError());
+    // onesweep at every size above one block: up to 2^20 keys rocPRIM's
+    // default is a merge sort, about 20 dependent launches
+    using SortCfg = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                               rocprim::default_config, 0>;
     size_t tmp_bytes = 0;
-    HIPCHK(h, rocprim::radix_sort_pairs(nullptr, tmp_bytes, key, skey, idx, sidx, nj, 0, kGroupKeyBits, h->stream));
+    HIPCHK(h, rocprim::radix_sort_pairs<SortCfg>(nullptr, tmp_bytes, key, skey, idx, sidx, nj, 0, kGroupKeyBits,
+                                                 h->stream));
     uint8_t *tmp;
     ENSURE(B_GSORT, uint8_t, tmp_bytes + 1, tmp);
-    HIPCHK(h, rocprim::radix_sort_pairs(tmp, tmp_bytes, key, skey, idx, sidx, nj, 0, kGroupKeyBits, h->stream));
+    HIPCHK(h, rocprim::radix_sort_pairs<SortCfg>(tmp, tmp_bytes, key, skey, idx, sidx, nj, 0, kGroupKeyBits,
+                                                 h->stream));
     size_t ts = 0;
     int rc = timed_begin(h, 2, &ts);
     if (rc) return rc;
@@ -3698,7 +3791,7 @@ constexpr NaLayout na_layout() {
   L.chunks = o; o += na_align(4 * (n + 1));
   L.offs = o;   o += na_align(8 * (n + 1));
   L.fail = o;   o += na_align(4 * (n + 1));
-  L.sums = o;   o += na_align(8 * (n / 1024 + 2));
+  L.sums = o;   o += na_align(8 * (n / kScanTile + 2));
   L.total = o;  o += na_align(8);
   L.evald = o;  o += na_align(8 * kEvalShards);
   L.bytes = o;
@@ -3750,7 +3843,7 @@ extern "C" int sx_open(int device, sx_handle **out) {
   h->err.reserve(512);  // set_err's messages then fit without a heap allocation
   h->scratch_rc = scratch_init(device);
   h->scratch = h->scratch_rc == hipSuccess ? static_cast<uint8_t *>(g_scratch_ptr[device]) : nullptr;
-  if (const char *e = getenv("SX_BULK_AFTER")) h->bulk_after = std::max(-1, atoi(e));
+  if (const char *e = getenv("SX_BULK_AFTER")) h->bulk_after = std::max(-2, atoi(e));
   if (const char *e = getenv("SX_EXH_K")) h->exh_k = atoi(e) == 1 ? 1 : 2;
   if (const char *e = getenv("SX_EXH_PK")) h->exh_pk = atoi(e) != 0;
   if (const char *e = getenv("SX_EXH_GROUP")) h->exh_group = atoi(e);
@@ -4397,8 +4490,14 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     // (SX_BULK_AFTER), may queue every survivor
     constexpr uint32_t kBulkRatio = 4;
     constexpr uint32_t kBulkJobs = 1u << 22;
+    // The tail's round: by default at once while the survivors number at most
+    // the (target, pass) entries (C2 at 0x00: 5.4M survivors for 15.3M
+    // entries; the batch sweeps 10% more pairs than a round and a tail, and
+    // saves the round's launches and read-backs), else after one round (at
+    // 0xFF the first round resolves most entries out of ~470M survivors).
+    const int bulk_at = h->bulk_after == -2 ? (nmid <= 2 * nt ? 0 : 1) : h->bulk_after;
     uint64_t job_bound = nmid;
-    if (!cfg->hits && h->bulk_after < 0)
+    if (!cfg->hits && bulk_at < 0)
       job_bound = std::min<uint64_t>(nmid, std::max<uint64_t>((uint64_t)kBulkRatio * 2 * nt, kBulkJobs));
     const uint32_t job_cap = (uint32_t)std::max<uint64_t>(job_bound, 1);
     ENSURE(B_JOBS, Job, job_cap, jobs);
@@ -4420,7 +4519,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
     for (int round = 0; nmid > 0; round++) {
       // every round rejects or resolves one candidate per open (target, pass)
       if ((uint64_t)round > nc + 1) return set_err(h, SX_EHIP, "first-hit rounds did not converge");
-      if (!cfg->hits && h->bulk_after >= 0 && round >= h->bulk_after) go_bulk = true;
+      if (!cfg->hits && bulk_at >= 0 && round >= bulk_at) go_bulk = true;
       if (!cfg->hits && go_bulk) {
         if (round == 0)  // the batch's minimum goes to cur, which still holds MidCheck's minima
           hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, nt * 2,
@@ -4487,7 +4586,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
       if (first_round) cnt2[1] = nmid;
       uint32_t nj = cnt2[0];
       if (nj == 0) break;
-      if (!cfg->hits && h->bulk_after < 0 &&
+      if (!cfg->hits && bulk_at < 0 &&
           ((uint64_t)cnt2[1] <= (uint64_t)kBulkRatio * nj || cnt2[1] <= kBulkJobs)) {
         // drop this round's jobs: the batch covers them; cur back to "none" for the
         // minimum (for the first round the batch branch does it)
@@ -4525,19 +4624,21 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
       HIPCHK(h, hipMemcpyAsync(&nr, nrules_d, 4, hipMemcpyDeviceToHost, h->stream));
     } else {
       // the filled slots in (target, pass) order
-      uint64_t *roffs, *rtot;
+      uint64_t *rsums, *rtot;
       sx_rule *packed;
-      ENSURE(B_RULEOFFS, uint64_t, nt * 2 + 1, roffs);
+      const uint64_t nk = nt * 2, nb = (nk + kScanTile - 1) / kScanTile;
+      ENSURE(B_SUMS, uint64_t, nb + 1, rsums);
       ENSURE(B_RULETOT, uint64_t, 1, rtot);
-      int rc = scan_u32(h, rule_flag, nt * 2, roffs, rtot);
-      if (rc) return rc;
+      hipLaunchKernelGGL(k_tile_sums, dim3((unsigned)nb), dim3(256), 0, h->stream, rule_flag, nk, rsums, 1);
+      hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, h->stream, rsums, nb, rtot);
+      HIPCHK(h, hipGetLastError());
       uint64_t tot = 0;
-      rc = read_back(h, {{&tot, rtot, 8}}, "rule count");
+      int rc = read_back(h, {{&tot, rtot, 8}}, "rule count");
       if (rc) return rc;
       nr = (uint32_t)tot;
       ENSURE(B_RULES2, sx_rule, std::max<uint32_t>(nr, 1), packed);
-      hipLaunchKernelGGL(k_compact_rules, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, rules, rule_flag,
-                         roffs, nt * 2, packed);
+      hipLaunchKernelGGL(k_compact_rules, dim3((unsigned)nb), dim3(256), 0, h->stream, rules, rule_flag, nk,
+                         (const uint64_t *)rsums, packed);
       HIPCHK(h, hipGetLastError());
       rules_out = packed;
     }
@@ -4928,10 +5029,11 @@ extern "C" int sx_check_pairs_noalloc(sx_handle *h, const uint32_t *t_words, con
                        jobs);
     hipLaunchKernelGGL(k_exh_plan, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, jobs, nn, (int)exh_mode,
                        (const PlanEntry *)nullptr, pm, chunks, fail, (uint32_t *)nullptr, (uint32_t *)nullptr);
-    const uint64_t nb = (n + 1023) / 1024;
-    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(1024), 0, h->stream, chunks, n, offs, sums);
+    const uint64_t nb = (n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_tile_sums, dim3((unsigned)nb), dim3(256), 0, h->stream, chunks, n, sums, 0);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, h->stream, sums, nb, total);
-    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(1024), 0, h->stream, offs, n, sums);
+    hipLaunchKernelGGL(k_tile_scan, dim3((unsigned)nb), dim3(256), 0, h->stream, chunks, n, (const uint64_t *)sums,
+                       offs);
     HIPCHK(h, hipGetLastError());
     uint64_t tot = 0;
     HIPCHK(h, hipMemcpyAsync(&tot, total, sizeof(tot), hipMemcpyDeviceToHost, h->stream));
