@@ -2263,8 +2263,14 @@ __device__ __forceinline__ uint64_t group_items(uint64_t gm, uint32_t u, uint32_
 // wave reads the same 64 jobs.  K pairs per lane share one opcode dispatch.
 // Keyed groups touch only A, X, Y and P, so the interpreter runs without the
 // other registers.
-template <int K>
-__global__ __launch_bounds__(256) void k_exh_group(const Job *__restrict__ jobs, const uint32_t *__restrict__ skey,
+// Ent: the jobs are compact records {target word 0, target word 1, slot |
+// pass << 30, target} (jent) instead of Jobs; a keyed job's plan is the plan
+// table's entry at its group key, which is the table's index, and its mask
+// follows from the pass.
+template <int K, bool Ent>
+__global__ __launch_bounds__(256) void k_exh_group(const Job *__restrict__ jobs, const uint4 *__restrict__ jent,
+                                                   const PlanEntry *__restrict__ ptab, uint32_t pm_masked,
+                                                   const uint32_t *__restrict__ skey,
                                                    const uint32_t *__restrict__ sidx, uint32_t n, uint32_t *fail,
                                                    unsigned long long *evaluated) {
   static_assert(K == 1 || K == 2 || K == 4 || K == 8, "pairs per lane");
@@ -2276,9 +2282,21 @@ __global__ __launch_bounds__(256) void k_exh_group(const Job *__restrict__ jobs,
   uint64_t todo = ballot(mine);
   if (!todo) return;
   const uint32_t ji = mine ? sidx[p] : 0u;
-  const uint32_t *jl = reinterpret_cast<const uint32_t *>(jobs + ji);
-  const uint32_t lw0 = jl[0], lw1 = jl[1], lc0 = jl[kJobWords], lhdr = jl[20];
-  const uint32_t key = key0 | ((lhdr >> 24) << 24);  // with the exact compare mask
+  uint32_t lw0, lw1, lc0, lpm;
+  if constexpr (Ent) {
+    const uint4 r = jent[ji];
+    lw0 = r.x;
+    lw1 = r.y;
+    lc0 = go_inst(r.z & 0x3FFFFFFFu);
+    lpm = (r.z >> 30) == 2 ? pm_masked : kPMask;
+  } else {
+    const uint32_t *jl = reinterpret_cast<const uint32_t *>(jobs + ji);
+    lw0 = jl[0];
+    lw1 = jl[1];
+    lc0 = jl[kJobWords];
+    lpm = jl[20] >> 24;
+  }
+  const uint32_t key = key0 | (lpm << 24);  // with the exact compare mask
   const pk16 imm0 = pk_splat(lw0 >> 16), imm1 = pk_splat(lw1 >> 16), immc = pk_splat(lc0 >> 16);
   uint64_t failed = 0, evald = 0;
   while (todo) {
@@ -2287,12 +2305,31 @@ __global__ __launch_bounds__(256) void k_exh_group(const Job *__restrict__ jobs,
     const uint64_t gm = ballot(mine && key == k);
     todo &= ~gm;
     if (!(mine && key == k)) continue;
-    // the group's opcodes, mask and plan (the first lane's job)
-    const uint32_t *jw = reinterpret_cast<const uint32_t *>(jobs + __builtin_amdgcn_readlane(ji, first));
-    const uint32_t op0 = jw[0] & 0xFFFFu, op1 = jw[1] & 0xFFFFu, opc = jw[kJobWords] & 0xFFFFu;
-    const uint32_t pm = jw[20] >> 24, nsets = jw[23] & 0xFFu;
-    const uint64_t sets8 = (uint64_t)jw[24] | ((uint64_t)jw[25] << 32);
-    const uint64_t logs8 = (uint64_t)jw[26] | ((uint64_t)jw[27] << 32);
+    // the group's opcodes, mask and plan
+    uint32_t op0, op1, opc, pm, nsets;
+    uint64_t sets8, logs8;
+    if constexpr (Ent) {
+      op0 = __builtin_amdgcn_readlane(lw0, first) & 0xFFFFu;
+      op1 = __builtin_amdgcn_readlane(lw1, first) & 0xFFFFu;
+      opc = __builtin_amdgcn_readlane(lc0, first) & 0xFFFFu;
+      pm = k >> 24;
+      // (the compiler loads the entry through the vector unit: the words are
+      // made uniform so that the sweep's loop control stays scalar)
+      const PlanEntry e = ptab[k & 0xFFFFFFu];
+      nsets = uni(e.meta) & 0xFFu;
+      sets8 = (uint64_t)uni(e.sets_lo) | ((uint64_t)uni(e.sets_hi) << 32);
+      logs8 = (uint64_t)uni(e.log_lo) | ((uint64_t)uni(e.log_hi) << 32);
+    } else {
+      // the first lane's job
+      const uint32_t *jw = reinterpret_cast<const uint32_t *>(jobs + __builtin_amdgcn_readlane(ji, first));
+      op0 = jw[0] & 0xFFFFu;
+      op1 = jw[1] & 0xFFFFu;
+      opc = jw[kJobWords] & 0xFFFFu;
+      pm = jw[20] >> 24;
+      nsets = jw[23] & 0xFFu;
+      sets8 = (uint64_t)jw[24] | ((uint64_t)jw[25] << 32);
+      logs8 = (uint64_t)jw[26] | ((uint64_t)jw[27] << 32);
+    }
     failed |= group_items<K, kRegsAXYP>(gm, u, op0, op1, opc, imm0, imm1, immc, pk_splat(pm), nsets, sets8, logs8,
                                         fail, ji, &evald);
   }
@@ -2361,13 +2398,16 @@ __global__ __launch_bounds__(256) void k_round_resolve(const Job *jobs, uint32_t
 // entries are resolved; for the rest, every Mid survivor not yet tested
 // (rank >= rej) becomes a job at once, and the smallest passing Go rank is
 // the first hit (the rounds would reach the same candidate one rank at a
-// time).
+// time).  Every job gets its compact record in jent (target words 0-1, slot
+// | pass << 30, target: 16 bytes); with a plan table a full Job (112 bytes)
+// is written only for a job the grouped sweep does not take (no group key,
+// or no grouped sweep), which the state-parallel sweep reads.
 __global__ __launch_bounds__(kBlk) void k_bulk_jobs(const uint2 *mid, MidList ml, const uint32_t *rank,
                                                     const uint32_t *rej, const uint8_t *done,
                                                     const uint32_t *targets, uint32_t tl, uint32_t pm_masked,
-                                                    Job *jobs, uint32_t job_cap, uint32_t *njobs, int plan_mode,
-                                                    const PlanEntry *ptab, uint32_t *chunks, uint32_t *fail,
-                                                    uint32_t *gkey, uint32_t *gidx) {
+                                                    Job *jobs, uint4 *jent, uint32_t job_cap, uint32_t *njobs,
+                                                    int plan_mode, const PlanEntry *ptab, uint32_t *chunks,
+                                                    uint32_t *fail, uint32_t *gkey, uint32_t *gidx) {
   __shared__ uint32_t s_take[kBlkWaves];
   __shared__ uint32_t s_base;
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
@@ -2390,10 +2430,16 @@ __global__ __launch_bounds__(kBlk) void k_bulk_jobs(const uint2 *mid, MidList ml
     fill_job(j, targets, tl, nullptr, 0, e.x, slot, pass, pass == 2 ? pm_masked : kPMask);
     uint32_t o = s_base + pre + mbcnt(m);
     if (o < job_cap) {
-      chunks[o] = plan_job_tab(j, plan_mode, ptab, pm_masked);
+      const uint32_t c = plan_job_tab(j, plan_mode, ptab, pm_masked);
+      const uint32_t k = gkey ? group_key(j) : kGroupSort;
+      if (gkey) {
+        gkey[o] = k;
+        gidx[o] = o;
+      }
+      chunks[o] = k != kGroupSort ? 0u : c;  // a keyed job leaves the state-parallel sweep
       fail[o] = 0;
-      if (gkey) write_group_key(j, o, chunks, gkey, gidx);
-      jobs[o] = j;
+      jent[o] = make_uint4(j.tw[0], j.tw[1], e.y, e.x);
+      if (k == kGroupSort || !ptab) jobs[o] = j;  // without the table the grouped sweep reads Jobs
     }
   }
 }
@@ -2407,14 +2453,17 @@ __device__ __forceinline__ uint32_t fp_flag_diff(const uint2 *tfp, const uint2 *
   return fd;
 }
 
-__global__ __launch_bounds__(256) void k_bulk_min(const Job *jobs, uint32_t nj, const uint32_t *fail,
+// The tail's minimum passing rank per (target, pass), from the jobs' compact
+// records.
+__global__ __launch_bounds__(256) void k_bulk_min(const uint4 *jent, uint32_t nj, const uint32_t *fail,
                                                   const uint32_t *rank, const uint2 *tfp, const uint2 *cfp,
                                                   uint32_t *best) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nj || fail[i]) return;
-  const Job &j = jobs[i];
-  if (j.pass == 2 && fp_flag_diff(tfp, cfp, j.target, j.slot) == 0) return;  // worker.go:211-213
-  atomicMin(&best[(uint64_t)j.target * 2 + (j.pass - 1)], rank[j.slot]);
+  const uint4 r = jent[i];
+  const uint32_t target = r.w, slot = r.z & 0x3FFFFFFFu, pass = r.z >> 30;
+  if (pass == 2 && fp_flag_diff(tfp, cfp, target, slot) == 0) return;  // worker.go:211-213
+  atomicMin(&best[(uint64_t)target * 2 + (pass - 1)], rank[slot]);
 }
 
 // Rules of the (target, pass) entries the tail resolved; rank == slot.
@@ -2999,6 +3048,7 @@ struct sx_handle {
   int device = 0;
   int num_cu = 256;
   bool poisoned = false;  // a device stage timed out: the stream is unusable
+  int sort_bits = 10;     // key bits per pass of the group sort (SX_SORT_BITS: 8 or 10)
   int bulk_after = -2;    // first-hit rounds before the one-batch tail (SX_BULK_AFTER); -1 = by survivor count, -2 = 0 or 1 by survivor count
   int exh_k = 2;          // states (pairs when packed) per lane and opcode dispatch in the sweep (SX_EXH_K: 1 or 2)
   int exh_pk = 1;         // packed two-states-per-register interpreter in the sweep (SX_EXH_PK=0: one state)
@@ -3086,7 +3136,7 @@ namespace {
 
 enum BufId {
   B_TARGETS, B_TFP, B_TV0, B_TBYTES, B_TCYC, B_CWORDS, B_CFP, B_CBYTES, B_CCYC, B_CPRUNE,
-  B_RANK, B_SURV, B_MID, B_COUNTERS, B_CUR, B_REJ, B_DONE, B_JOBS, B_CHUNKS, B_OFFS, B_SUMS,
+  B_RANK, B_SURV, B_MID, B_COUNTERS, B_CUR, B_REJ, B_DONE, B_JOBS, B_JENT, B_CHUNKS, B_OFFS, B_SUMS,
   B_FAIL, B_RULES, B_SCRATCH, B_GEN_COUNTS, B_GEN_OFFS, B_BADOP, B_P_TW, B_P_TL, B_P_CW, B_P_CL,
   B_P_OUT1, B_P_OUT2, B_P_OUT3, B_STATES, B_TOTAL, B_CURSOR, B_HEADS, B_RULEFLAG, B_RULES2,
   B_RULETOT, B_TRAW, B_CHUNKMAP, B_TMH, B_CMH, B_PLANTAB, B_EFP, B_ERULES, B_L3_LIST, B_L3_HK, B_L3_CNT,
@@ -3181,11 +3231,12 @@ int sync_wd(sx_handle *h, const char *stage) {
       h->poisoned = true;
       return set_err(h, SX_EHIP, "%s: device work did not finish within %.0f s", stage, limit);
     }
-    // Most waits here are a kernel or two (tens to hundreds of microseconds):
-    // spin for the first 2 ms (a timed sleep oversleeps by the timer slack,
-    // about 50 us, on every wait), then poll at 50 us and later at 0.5 ms.
-    if (dt < 0.002) std::this_thread::yield();
-    else std::this_thread::sleep_for(std::chrono::microseconds(dt < 0.01 ? 50 : 500));
+    // Most waits here are a few kernels (tens of microseconds to a few
+    // milliseconds: the length-2 sweep is one 2 ms launch, a 0xFF join 7 ms):
+    // spin for the first 20 ms (a timed sleep oversleeps by the timer slack,
+    // 50-100 us, on every wait), then poll at 50 us and after 0.1 s at 0.5 ms.
+    if (dt < 0.02) std::this_thread::yield();
+    else std::this_thread::sleep_for(std::chrono::microseconds(dt < 0.1 ? 50 : 500));
   }
 }
 
@@ -3316,6 +3367,22 @@ void unpack_rules(sx_rule *out, const uint32_t *w, uint64_t n, int model) {
 }
 
 // Exclusive scan of n uint32 counts into uint64 offsets; *total on device.
+// The grouped sweep's sort of its jobs by group key: onesweep at every size
+// above one block (up to 2^20 keys rocPRIM's default is a merge sort, about
+// 20 dependent launches), Bits key bits per pass.
+template <unsigned Bits>
+using GroupSortCfg = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 16>, Bits,
+                                        rocprim::block_radix_rank_algorithm::match>,
+    0>;
+hipError_t group_sort(int bits, void *tmp, size_t &bytes, const uint32_t *key, uint32_t *skey, const uint32_t *idx,
+                      uint32_t *sidx, uint32_t n, hipStream_t s) {
+  if (bits == 10)
+    return rocprim::radix_sort_pairs<GroupSortCfg<10>>(tmp, bytes, key, skey, idx, sidx, n, 0, kGroupKeyBits, s);
+  return rocprim::radix_sort_pairs<GroupSortCfg<8>>(tmp, bytes, key, skey, idx, sidx, n, 0, kGroupKeyBits, s);
+}
+
 // (in and out 16-byte aligned: the tiles move 16 bytes per access)
 int scan_u32(sx_handle *h, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *d_total) {
   const uint64_t nb = (n + kScanTile - 1) / kScanTile;
@@ -3468,8 +3535,35 @@ int launch_exh_run(sx_handle *h, const Job *jobs, uint32_t nj, const uint64_t *o
 // ptab / pm_masked: the plan table of the call (or null) and its relaxed mask.
 // keyed: the kernel that planned the jobs also wrote their group keys into
 // B_GKEY / B_GIDX (with the grouped sweep on).
+template <int K>
+void launch_exh_group_k(bool ent, dim3 gg, hipStream_t s, const Job *jobs, const uint4 *jent, const PlanEntry *ptab,
+                        uint32_t pm_masked, const uint32_t *skey, const uint32_t *sidx, uint32_t nj, uint32_t *fail,
+                        unsigned long long *gev) {
+  if (ent)
+    hipLaunchKernelGGL((k_exh_group<K, true>), gg, dim3(256), 0, s, jobs, jent, ptab, pm_masked, skey, sidx, nj, fail,
+                       gev);
+  else
+    hipLaunchKernelGGL((k_exh_group<K, false>), gg, dim3(256), 0, s, jobs, jent, ptab, pm_masked, skey, sidx, nj,
+                       fail, gev);
+}
+void launch_exh_group(int k, bool ent, dim3 gg, hipStream_t s, const Job *jobs, const uint4 *jent,
+                      const PlanEntry *ptab, uint32_t pm_masked, const uint32_t *skey, const uint32_t *sidx,
+                      uint32_t nj, uint32_t *fail, unsigned long long *gev) {
+  if (k == 8)
+    launch_exh_group_k<8>(ent, gg, s, jobs, jent, ptab, pm_masked, skey, sidx, nj, fail, gev);
+  else if (k == 4)
+    launch_exh_group_k<4>(ent, gg, s, jobs, jent, ptab, pm_masked, skey, sidx, nj, fail, gev);
+  else if (k == 2)
+    launch_exh_group_k<2>(ent, gg, s, jobs, jent, ptab, pm_masked, skey, sidx, nj, fail, gev);
+  else
+    launch_exh_group_k<1>(ent, gg, s, jobs, jent, ptab, pm_masked, skey, sidx, nj, fail, gev);
+}
+
+// jent (with ptab): the keyed jobs are compact records there (k_bulk_jobs);
+// B_JOBS holds only the others.
 int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, bool planned = false,
-                   const PlanEntry *ptab = nullptr, uint32_t pm_masked = 0, bool keyed = false) {
+                   const PlanEntry *ptab = nullptr, uint32_t pm_masked = 0, bool keyed = false,
+                   const uint4 *jent = nullptr) {
   Job *jobs = (Job *)h->bufs[B_JOBS].p;
   uint32_t *chunks, *fail;
   uint64_t *offs, *total;
@@ -3499,30 +3593,18 @@ int run_exhaustive(sx_handle *h, uint32_t nj, int mode, uint64_t *chunks_out, bo
     if (!keyed)
       hipLaunchKernelGGL(k_group_keys, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, jobs, nj, chunks, key, idx);
     HIPCHK(h, hipGetLastError());
-    // onesweep at every size above one block: up to 2^20 keys rocPRIM's
-    // default is a merge sort, about 20 dependent launches
-    using SortCfg = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
-                                               rocprim::default_config, 0>;
     size_t tmp_bytes = 0;
-    HIPCHK(h, rocprim::radix_sort_pairs<SortCfg>(nullptr, tmp_bytes, key, skey, idx, sidx, nj, 0, kGroupKeyBits,
-                                                 h->stream));
+    HIPCHK(h, group_sort(h->sort_bits, nullptr, tmp_bytes, key, skey, idx, sidx, nj, h->stream));
     uint8_t *tmp;
     ENSURE(B_GSORT, uint8_t, tmp_bytes + 1, tmp);
-    HIPCHK(h, rocprim::radix_sort_pairs<SortCfg>(tmp, tmp_bytes, key, skey, idx, sidx, nj, 0, kGroupKeyBits,
-                                                 h->stream));
+    HIPCHK(h, group_sort(h->sort_bits, tmp, tmp_bytes, key, skey, idx, sidx, nj, h->stream));
     size_t ts = 0;
     int rc = timed_begin(h, 2, &ts);
     if (rc) return rc;
     const dim3 gg(blocks_for(nj, kWave));
     unsigned long long *gev = evald + kEvalShards;
-    if (h->exh_group == 8)
-      hipLaunchKernelGGL((k_exh_group<8>), gg, dim3(256), 0, h->stream, jobs, skey, sidx, nj, fail, gev);
-    else if (h->exh_group == 4)
-      hipLaunchKernelGGL((k_exh_group<4>), gg, dim3(256), 0, h->stream, jobs, skey, sidx, nj, fail, gev);
-    else if (h->exh_group == 2)
-      hipLaunchKernelGGL((k_exh_group<2>), gg, dim3(256), 0, h->stream, jobs, skey, sidx, nj, fail, gev);
-    else
-      hipLaunchKernelGGL((k_exh_group<1>), gg, dim3(256), 0, h->stream, jobs, skey, sidx, nj, fail, gev);
+    launch_exh_group(h->exh_group, jent != nullptr, gg, h->stream, jobs, jent, ptab, pm_masked, skey, sidx, nj, fail,
+                     gev);
     HIPCHK(h, hipGetLastError());
     rc = timed_end(h, ts);
     if (rc) return rc;
@@ -3843,6 +3925,7 @@ extern "C" int sx_open(int device, sx_handle **out) {
   h->err.reserve(512);  // set_err's messages then fit without a heap allocation
   h->scratch_rc = scratch_init(device);
   h->scratch = h->scratch_rc == hipSuccess ? static_cast<uint8_t *>(g_scratch_ptr[device]) : nullptr;
+  if (const char *e = getenv("SX_SORT_BITS")) h->sort_bits = atoi(e) == 10 ? 10 : 8;
   if (const char *e = getenv("SX_BULK_AFTER")) h->bulk_after = std::max(-2, atoi(e));
   if (const char *e = getenv("SX_EXH_K")) h->exh_k = atoi(e) == 1 ? 1 : 2;
   if (const char *e = getenv("SX_EXH_PK")) h->exh_pk = atoi(e) != 0;
@@ -4530,8 +4613,10 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
           ENSURE(B_GKEY, uint32_t, job_cap, bkey);
           ENSURE(B_GIDX, uint32_t, job_cap, bidx);
         }
+        uint4 *jent;
+        ENSURE(B_JENT, uint4, job_cap, jent);
         hipLaunchKernelGGL(k_bulk_jobs, dim3(blocks_for(nmid, kBlk)), dim3(kBlk), 0, h->stream, mid, mlist, rank,
-                           rej, done, tw, (uint32_t)tl, pm_masked, jobs, job_cap, njobs_d, cfg->exh_mode, ptab,
+                           rej, done, tw, (uint32_t)tl, pm_masked, jobs, jent, job_cap, njobs_d, cfg->exh_mode, ptab,
                            jchunks, jfail, bkey, bidx);
         HIPCHK(h, hipGetLastError());
         uint32_t nj = 0;
@@ -4545,12 +4630,13 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         st.exh_rounds++;
         st.exh_pairs += nj;
         uint64_t chunks = 0;
-        int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, true, nullptr, 0, bkey != nullptr);
+        // with the plan table the keyed jobs are compact records only
+        int rc = run_exhaustive(h, nj, cfg->exh_mode, &chunks, true, ptab, pm_masked, bkey != nullptr,
+                                ptab ? jent : nullptr);
         if (rc) return rc;
         // cur holds kRankNone for every entry after a resolve: reuse it for the minimum
-        hipLaunchKernelGGL(k_bulk_min, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream,
-                           (const Job *)h->bufs[B_JOBS].p, nj, (const uint32_t *)h->bufs[B_FAIL].p, rank, tfp,
-                           cfp, cur);
+        hipLaunchKernelGGL(k_bulk_min, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, (const uint4 *)jent, nj,
+                           (const uint32_t *)h->bufs[B_FAIL].p, rank, tfp, cfp, cur);
         hipLaunchKernelGGL(k_bulk_emit, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur,
                            (uint32_t)nt, tw, (uint32_t)tl, tfp, cfp, tbytes, tcyc, cbytes, ccyc, tbase + s0, rules,
                            rule_flag);
