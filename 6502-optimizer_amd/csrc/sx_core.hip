@@ -3240,6 +3240,33 @@ int sync_wd(sx_handle *h, const char *stage) {
   }
 }
 
+// A segment's packed rules whose copy to the host is yet to be queued: the
+// words at offset used of the device and host lists, n rules from rule
+// first on, rw words each.
+struct PendingRules {
+  uint64_t used, n, first, rw;
+};
+// Queue the copy of p's rules in pieces of kCopyPiece rules, each with an
+// event that sx_fetch_rules polls; the copy stream already waits for the
+// packing.
+int queue_pieces(sx_handle *h, PendingRules &p) {
+  for (uint64_t r0 = 0; r0 < p.n; r0 += kCopyPiece) {
+    const uint64_t pn = std::min<uint64_t>(kCopyPiece, p.n - r0);
+    const size_t pi = h->pieces.size();
+    if (pi == h->piece_ev.size()) {
+      hipEvent_t ev;
+      HIPCHK(h, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      h->piece_ev.push_back(ev);
+    }
+    HIPCHK(h, hipMemcpyAsync(h->h_rules + p.used + r0 * p.rw, h->d_rules + p.used + r0 * p.rw,
+                             (size_t)pn * p.rw * 4, hipMemcpyDeviceToHost, h->copy_stream));
+    HIPCHK(h, hipEventRecord(h->piece_ev[pi], h->copy_stream));
+    h->pieces.emplace_back(p.first + r0, pn);
+  }
+  p.n = 0;
+  return SX_OK;
+}
+
 // Device-to-host reads of small counters through the handle's pinned staging
 // block: the copies are queued together and waited for once.
 struct ReadBack {
@@ -3247,12 +3274,17 @@ struct ReadBack {
   const void *src;
   size_t bytes;
 };
-int read_back(sx_handle *h, std::initializer_list<ReadBack> rs, const char *stage) {
+// pend: rule copies queued between the read-back's copies and the wait.
+int read_back(sx_handle *h, std::initializer_list<ReadBack> rs, const char *stage, PendingRules *pend = nullptr) {
   size_t off = 0;
   for (const ReadBack &r : rs) {
     if (off + r.bytes > kPinBytes) return set_err(h, SX_EINVAL, "%s: read-back of %zu bytes", stage, off + r.bytes);
     HIPCHK(h, hipMemcpyAsync(h->pin + off, r.src, r.bytes, hipMemcpyDeviceToHost, h->stream));
     off += (r.bytes + 63) & ~size_t(63);
+  }
+  if (pend && pend->n) {
+    int rcq = queue_pieces(h, *pend);
+    if (rcq) return rcq;
   }
   int rc = sync_wd(h, stage);
   if (rc) return rc;
@@ -3778,15 +3810,15 @@ int clear_evaluated(sx_handle *h) {
 // the copy engine, behind the rules' copy to the host.  bad / bad_d: a 32-bit
 // device flag read in the same wait.
 int read_evaluated(sx_handle *h, uint64_t *sum, uint64_t *sum_group, uint64_t *group_jobs, uint32_t *bad,
-                   const uint32_t *bad_d) {
+                   const uint32_t *bad_d, PendingRules *pend) {
   unsigned long long *evald, *sums;
   ENSURE(B_CURSOR, unsigned long long, 3 * kEvalShards, evald);
   ENSURE(B_EVSUM, unsigned long long, 3, sums);
   hipLaunchKernelGGL(k_sum_shards, dim3(1), dim3(kEvalShards), 0, h->stream, evald, sums);
   HIPCHK(h, hipGetLastError());
   unsigned long long tot[3] = {0, 0, 0};
-  int rc = bad ? read_back(h, {{tot, sums, sizeof(tot)}, {bad, bad_d, 4}}, "exhaustive sweep")
-               : read_back(h, {{tot, sums, sizeof(tot)}}, "exhaustive sweep");
+  int rc = bad ? read_back(h, {{tot, sums, sizeof(tot)}, {bad, bad_d, 4}}, "exhaustive sweep", pend)
+               : read_back(h, {{tot, sums, sizeof(tot)}}, "exhaustive sweep", pend);
   if (rc) return rc;
   *sum = tot[0] + tot[1];
   *sum_group = tot[1];
@@ -4295,9 +4327,14 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
   };
 
   uint64_t seg_no = 0;
+  PendingRules pend{0, 0, 0, 0};  // the last segment's rules, not yet queued for the host
   for (uint64_t s0 = 0; s0 < nt_all; s0 += seg, seg_no++) {
     uint64_t nt = std::min<uint64_t>(seg, nt_all - s0);
     const int evset = (int)(seg_no & 1);
+    if (pend.n) {  // the previous segment's (its last kernels are still running)
+      int rc = queue_pieces(h, pend);
+      if (rc) return rc;
+    }
     {
       int rc = harvest_stage_times(evset);
       if (rc) return rc;
@@ -4778,22 +4815,14 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
           HIPCHK(h, hipMemcpyAsync(h->d_rules + used, rules_out, (size_t)nr * sizeof(sx_rule),
                                    hipMemcpyDeviceToDevice, h->stream));
         }
-        // copy the segment's rules to the host while the next segments run
+        // copy the segment's rules to the host while the next segments run:
+        // the copy stream waits for the packing here; its copies are queued
+        // after the next launches on the library's stream (the next
+        // segment's first ones, or the final read-back's), so the device does
+        // not idle while the host queues them
         HIPCHK(h, hipEventRecord(h->seg_ev, h->stream));
         HIPCHK(h, hipStreamWaitEvent(h->copy_stream, h->seg_ev, 0));
-        for (uint64_t r0 = 0; r0 < nr; r0 += kCopyPiece) {
-          const uint64_t pn = std::min<uint64_t>(kCopyPiece, nr - r0);
-          const size_t pi = h->pieces.size();
-          if (pi == h->piece_ev.size()) {
-            hipEvent_t ev;
-            HIPCHK(h, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-            h->piece_ev.push_back(ev);
-          }
-          HIPCHK(h, hipMemcpyAsync(h->h_rules + used + r0 * rw, h->d_rules + used + r0 * rw, (size_t)pn * rw * 4,
-                                   hipMemcpyDeviceToHost, h->copy_stream));
-          HIPCHK(h, hipEventRecord(h->piece_ev[pi], h->copy_stream));
-          h->pieces.emplace_back(h->d_rules_n + r0, pn);
-        }
+        pend = {used, nr, h->d_rules_n, rw};
       }
       h->d_rules_n += nr;
     } else {
@@ -4851,7 +4880,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
   }
   uint32_t bad = 0;
   {
-    int rc = read_evaluated(h, &st.exh_states, &st.exh_group_states, &st.exh_group_jobs, &bad, badop);  // synchronises
+    // synchronises; the last segment's rule copies are queued before the wait
+    int rc = read_evaluated(h, &st.exh_states, &st.exh_group_states, &st.exh_group_jobs, &bad, badop, &pend);
     if (rc) return rc;
   }
   HIPCHK(h, hipEventSynchronize(es));  // long complete unless there were no targets

@@ -7,6 +7,8 @@ sx_fetch_rules unpacks as they land; the targets are the handle's own list
 The records must be identical byte for byte, at both byte models and both
 dead-flag masks, and fetches of sub-ranges must return the same records.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -46,3 +48,28 @@ def test_fetch_of_sub_ranges(dev):
     for off, cnt in ((0, 1), (65_535, 3), (n // 2, 100_000), (n - 7, 7), (n - 1, 10)):
         part = dev.fetch_rules(off, cnt)
         assert part.tobytes() == a[off: off + cnt].tobytes(), (off, cnt)
+
+
+def test_segments(dev):
+    # 2^20-target segments: eight segments, each one's rule copies queued
+    # while the next one runs, the last one's before the final wait
+    print("rule forms, 2^20-target segments", flush=True)
+    old = os.environ.get("SX_SEG_LOG")
+    os.environ["SX_SEG_LOG"] = "20"
+    try:
+        d = core.Device(0)
+    finally:
+        if old is None:
+            del os.environ["SX_SEG_LOG"]
+        else:
+            os.environ["SX_SEG_LOG"] = old
+    try:
+        for dead in (0x00, 0xFF):
+            a, sa = dev.enumerate(None, dead=dead)
+            b, sb = d.enumerate(None, dead=dead)
+            assert sa["rules"] == sb["rules"] and a.tobytes() == b.tobytes(), dead
+        n = len(b)
+        for off, cnt in ((0, 70_000), (n // 3, 200_000), (n - 5, 5)):
+            assert d.fetch_rules(off, cnt).tobytes() == b[off: off + cnt].tobytes(), (off, cnt)
+    finally:
+        d.close()
