@@ -2896,20 +2896,24 @@ __global__ __launch_bounds__(256) void k_compact_rules(const sx_rule *slots, con
 }
 
 // Rules of length-2 targets against single-instruction candidates packed
-// for the trip to the host: 4 words instead of the 12 of an sx_rule: src[0],
-// src[1], rep[0] | pass << 14 | dead flags << 24 (a candidate's opcode is
-// below 64 and its immediate below 256, so those bits are free), target
-// index (below 2^32).  The lengths are 2 and 1 and the savings follow from
-// the words, so the host fills them in.  The copy over the host link is the
-// one part of an enumeration that its own compute cannot hide.
-constexpr uint32_t kPackedWords = 4;
+// for the trip to the host: 3 words instead of the 12 of an sx_rule.  An
+// instruction word (opcode below 64, immediate below 256 at bit 16) takes 14
+// bits as opcode | immediate << 6: word 0 = src[0] | src[1] << 14 | pass <<
+// 28, word 1 = rep[0] | dead flags << 14, word 2 = the target index (below
+// 2^32).  The lengths are 2 and 1 and the savings follow from the words, so
+// the host fills them in.  The copy over the host link is the one part of an
+// enumeration that its own compute cannot hide.
+constexpr uint32_t kPackedWords = 3;
 static_assert(sizeof(sx_rule) == 48, "sx_rule is 12 words");
-__global__ __launch_bounds__(256) void k_pack_rules(const sx_rule *r, uint32_t n, uint4 *w) {
+__host__ __device__ inline uint32_t inst14(uint32_t w) { return (w & 63u) | ((w >> 16) & 0xFFu) << 6; }
+__host__ __device__ inline uint32_t inst_of14(uint32_t v) { return (v & 63u) | ((v >> 6) & 0xFFu) << 16; }
+__global__ __launch_bounds__(256) void k_pack_rules(const sx_rule *r, uint32_t n, uint32_t *w) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const sx_rule x = r[i];
-  w[i] = make_uint4(x.src[0], x.src[1], x.rep[0] | (uint32_t)x.pass << 14 | (uint32_t)x.dead_flags << 24,
-                    (uint32_t)x.target_index);
+  w[3 * i] = inst14(x.src[0]) | inst14(x.src[1]) << 14 | (uint32_t)x.pass << 28;
+  w[3 * i + 1] = inst14(x.rep[0]) | (uint32_t)x.dead_flags << 14;
+  w[3 * i + 2] = (uint32_t)x.target_index;
 }
 
 // Bulk QuickCheck / MidCheck / FlagDiff for sx_check_pairs: one wave per pair.
@@ -3089,7 +3093,7 @@ struct sx_handle {
   std::vector<sx_rule> rules;  // all-hits mode: sorted on the host
   std::vector<sx_rule> empty_rules;  // the empty-candidate stream (include_empty), sorted on the host
   // first-hit mode: the rules stay on the device, in order, until fetched,
-  // rec_words 32-bit words per rule: 12 (an sx_rule) or 6 (the packed form of
+  // rec_words 32-bit words per rule: 12 (an sx_rule) or 3 (the packed form of
   // k_pack_rules, which sx_fetch_rules expands)
   uint32_t *d_rules = nullptr;
   uint64_t d_rules_capw = 0, d_rules_n = 0;
@@ -3371,11 +3375,13 @@ void unpack_rules(sx_rule *out, const uint32_t *w, uint64_t n, int model) {
   using v4u = uint32_t __attribute__((ext_vector_type(4)));
   auto rec = [&](uint64_t i, v4u &a, v4u &b, v4u &c) {
     const uint32_t *p = w + i * kPackedWords;
-    const uint32_t o0 = p[0] & 63u, o1 = p[1] & 63u, r = p[2], oc = r & 63u;
+    const uint32_t w0 = p[0], w1 = p[1];
+    const uint32_t s0 = inst_of14(w0), s1 = inst_of14(w0 >> 14), rp = inst_of14(w1);
+    const uint32_t o0 = s0 & 63u, o1 = s1 & 63u, oc = rp & 63u;
     const int bs = tb[o0] + tb[o1] - tb[oc], cs = tc[o0] + tc[o1] - tc[oc];
-    a = v4u{p[0], p[1], 0u, 0u};
-    b = v4u{r & 0x00FF003Fu, 0u, 0u, 0u};
-    c = v4u{p[3], 0u, 2u | 1u << 8 | (r >> 24) << 16 | ((r >> 14) & 3u) << 24,
+    a = v4u{s0, s1, 0u, 0u};
+    b = v4u{rp, 0u, 0u, 0u};
+    c = v4u{p[2], 0u, 2u | 1u << 8 | ((w1 >> 14) & 0xFFu) << 16 | ((w0 >> 28) & 3u) << 24,
             (uint32_t)(uint16_t)bs | (uint32_t)(uint16_t)cs << 16};
   };
   v4u a, b, c;
@@ -3398,7 +3404,6 @@ void unpack_rules(sx_rule *out, const uint32_t *w, uint64_t n, int model) {
   }
 }
 
-// Exclusive scan of n uint32 counts into uint64 offsets; *total on device.
 // The grouped sweep's sort of its jobs by group key: onesweep at every size
 // above one block (up to 2^20 keys rocPRIM's default is a merge sort, about
 // 20 dependent launches), Bits key bits per pass.
@@ -3415,7 +3420,8 @@ hipError_t group_sort(int bits, void *tmp, size_t &bytes, const uint32_t *key, u
   return rocprim::radix_sort_pairs<GroupSortCfg<8>>(tmp, bytes, key, skey, idx, sidx, n, 0, kGroupKeyBits, s);
 }
 
-// (in and out 16-byte aligned: the tiles move 16 bytes per access)
+// Exclusive scan of n uint32 counts into uint64 offsets; *total on device
+// (in and out 16-byte aligned: the tiles move 16 bytes per access).
 int scan_u32(sx_handle *h, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *d_total) {
   const uint64_t nb = (n + kScanTile - 1) / kScanTile;
   uint64_t *sums;
@@ -4809,7 +4815,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
       if (nr) {
         if (rw == kPackedWords) {
           hipLaunchKernelGGL(k_pack_rules, dim3(blocks_for(nr, 256)), dim3(256), 0, h->stream, rules_out, nr,
-                             reinterpret_cast<uint4 *>(h->d_rules + used));
+                             h->d_rules + used);
           HIPCHK(h, hipGetLastError());
         } else {
           HIPCHK(h, hipMemcpyAsync(h->d_rules + used, rules_out, (size_t)nr * sizeof(sx_rule),

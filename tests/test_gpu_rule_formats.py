@@ -1,7 +1,7 @@
 """Rule records reach the host the same whichever way the targets came.
 
 A first-hit enumeration of length-2 targets against single instructions
-sends each rule to the host as 4 words, 16 bytes (k_pack_rules), in pieces whose copies
+sends each rule to the host as 3 words, 12 bytes (k_pack_rules), in pieces whose copies
 sx_fetch_rules unpacks as they land; the targets are the handle's own list
 (copied on the device) or the same list passed by the caller (uploaded).
 The records must be identical byte for byte, at both byte models and both
@@ -36,7 +36,7 @@ def test_device_list_equals_caller_list(dev, targets, dead, model):
     b, sb = dev.enumerate(targets, dead=dead, bytes_model=model)
     assert sa["rules"] == sb["rules"] == len(a) == len(b)
     assert a.tobytes() == b.tobytes()
-    # the fields the 4-word form leaves out
+    # the fields the 3-word form leaves out
     assert np.all(a["src_len"] == 2) and np.all(a["rep_len"] == 1)
     assert np.all(a["src"][:, 2:] == 0) and np.all(a["rep"][:, 1:] == 0)
 
