@@ -4948,38 +4948,49 @@ namespace {
 // copied.
 int fetch_pieces(sx_handle *h, sx_rule *out, uint64_t offset, uint64_t m) {
   const uint64_t end = offset + m;
-  std::vector<size_t> idx;  // the pieces that overlap the range
-  for (size_t i = 0; i < h->pieces.size(); i++)
-    if (h->pieces[i].first < end && h->pieces[i].first + h->pieces[i].second > offset) idx.push_back(i);
+  // work items: slices of kFetchSlice rules of the pieces that overlap the
+  // range, in piece order, taken by the threads one at a time, so the pieces
+  // that land last are unpacked by all threads at once
+  constexpr uint64_t kFetchSlice = 8192;
+  struct Item {
+    size_t piece;
+    uint64_t a, b;
+  };
+  std::vector<Item> items;
+  for (size_t i = 0; i < h->pieces.size(); i++) {
+    const auto &pc = h->pieces[i];
+    const uint64_t a = std::max(pc.first, offset), b = std::min(pc.first + pc.second, end);
+    for (uint64_t x = a; x < b; x += kFetchSlice) items.push_back({i, x, std::min(b, x + kFetchSlice)});
+  }
   const uint32_t rw = h->rec_words;
   std::atomic<int> err{0};
-  auto work = [&](size_t k, size_t stride) {
-    for (size_t j = k; j < idx.size(); j += stride) {
-      const auto &pc = h->pieces[idx[j]];
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    for (;;) {
+      const size_t j = next.fetch_add(1);
+      if (j >= items.size() || err) return;
+      const Item &it = items[j];
       // poll: a blocking wait sleeps until an interrupt, far longer than a piece
       hipError_t e;
-      while ((e = hipEventQuery(h->piece_ev[idx[j]])) == hipErrorNotReady) std::this_thread::yield();
+      while ((e = hipEventQuery(h->piece_ev[it.piece])) == hipErrorNotReady) std::this_thread::yield();
       if (e != hipSuccess) {
         err = 1;
         return;
       }
-      const uint64_t a = std::max(pc.first, offset), b = std::min(pc.first + pc.second, end);
       if (rw == kPackedWords)
-        unpack_rules(out + (a - offset), h->h_rules + a * kPackedWords, b - a, h->fetch_model);
+        unpack_rules(out + (it.a - offset), h->h_rules + it.a * kPackedWords, it.b - it.a, h->fetch_model);
       else
-        memcpy(out + (a - offset), h->h_rules + a * 12, (b - a) * sizeof(sx_rule));
+        memcpy(out + (it.a - offset), h->h_rules + it.a * 12, (it.b - it.a) * sizeof(sx_rule));
     }
   };
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t nth = std::min<size_t>(std::min<unsigned>((unsigned)h->fetch_threads, hw), idx.size());
+  const size_t nth = std::min<size_t>(std::min<unsigned>((unsigned)h->fetch_threads, hw), items.size());
   std::vector<std::thread> pool;
-  size_t started = 1;
   try {
-    for (; started < nth; started++) pool.emplace_back(work, started, nth);
-  } catch (...) {  // no thread to be had: this thread takes the rest
-    for (size_t k = started; k < nth; k++) work(k, nth);
+    for (size_t k = 1; k < nth; k++) pool.emplace_back(work);
+  } catch (...) {  // no thread to be had: the ones started and this one take every item
   }
-  if (nth) work(0, nth);
+  if (nth) work();
   for (auto &t : pool) t.join();
   if (err) return set_err(h, SX_EHIP, "rule copy to the host failed");
   return SX_OK;
