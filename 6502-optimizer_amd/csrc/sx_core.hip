@@ -4662,8 +4662,8 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
                            rej, done, tw, (uint32_t)tl, pm_masked, jobs, jent, job_cap, njobs_d, cfg->exh_mode, ptab,
                            jchunks, jfail, bkey, bidx);
         HIPCHK(h, hipGetLastError());
-        uint32_t nj = 0;
-        {
+        uint32_t nj = nmid;  // at round 0 nothing is resolved or rejected: every survivor is a job
+        if (round > 0) {
           int rc = read_back(h, {{&nj, njobs_d, 4}}, "first-hit tail jobs");
           if (rc) return rc;
         }
