@@ -40,6 +40,8 @@
 #include <cstdio>
 #include <cstring>
 #include <initializer_list>
+#include <condition_variable>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -3059,6 +3061,19 @@ struct sx_handle {
   int qcm_join = 1;       // QuickCheck and MidCheck as one join when MidCheck is on (SX_QCM_JOIN=0: separate)
   int fp_split = 1;       // the join's targets: QuickCheck outputs first, Mid hashes of class members (SX_FP_SPLIT)
   int fetch_threads = 16;  // host threads of sx_fetch_rules (SX_FETCH_THREADS, 1..64)
+  // sx_fetch_rules' helper threads, started on its first call and kept (a
+  // thread costs tens of microseconds to start, 15 of them more than the
+  // unpacking): each call posts its work and runs it on the caller's thread
+  // too, then waits until every helper has left it
+  struct FetchPool {
+    std::vector<std::thread> th;
+    std::mutex mu;
+    std::condition_variable cv, cv_done;
+    uint64_t gen = 0;
+    const std::function<void()> *job = nullptr;
+    size_t busy = 0;
+    bool stop = false;
+  } fetch_pool;
   // the candidate side (words, fingerprints, hash tables, classes) is kept
   // between enumerations while the candidate length, byte model and compare
   // mask stay the same
@@ -3987,6 +4002,12 @@ extern "C" int sx_open(int device, sx_handle **out) {
 
 extern "C" int sx_close(sx_handle *h) {
   if (!h) return SX_EINVAL;
+  {
+    std::lock_guard<std::mutex> lk(h->fetch_pool.mu);
+    h->fetch_pool.stop = true;
+    h->fetch_pool.cv.notify_all();
+  }
+  for (auto &t : h->fetch_pool.th) t.join();
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->copy_stream) (void)hipStreamSynchronize(h->copy_stream);
@@ -4984,14 +5005,43 @@ int fetch_pieces(sx_handle *h, sx_rule *out, uint64_t offset, uint64_t m) {
     }
   };
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t nth = std::min<size_t>(std::min<unsigned>((unsigned)h->fetch_threads, hw), items.size());
-  std::vector<std::thread> pool;
-  try {
-    for (size_t k = 1; k < nth; k++) pool.emplace_back(work);
-  } catch (...) {  // no thread to be had: the ones started and this one take every item
+  const size_t nth = std::min<size_t>((unsigned)h->fetch_threads, hw);
+  auto &P = h->fetch_pool;
+  if (items.size() > 1 && P.th.size() + 1 < nth) {
+    try {
+      while (P.th.size() + 1 < nth)
+        P.th.emplace_back([&P] {
+          uint64_t seen = 0;
+          std::unique_lock<std::mutex> lk(P.mu);
+          for (;;) {
+            P.cv.wait(lk, [&] { return P.stop || P.gen != seen; });
+            if (P.stop) return;
+            seen = P.gen;
+            const std::function<void()> *job = P.job;
+            lk.unlock();
+            (*job)();
+            lk.lock();
+            if (--P.busy == 0) P.cv_done.notify_all();
+          }
+        });
+    } catch (...) {  // no more threads to be had: the ones started and this one take every item
+    }
   }
-  if (nth) work();
-  for (auto &t : pool) t.join();
+  const std::function<void()> job = work;
+  const bool helpers = items.size() > 1 && !P.th.empty();
+  if (helpers) {
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.job = &job;
+    P.busy = P.th.size();
+    P.gen++;
+    P.cv.notify_all();
+  }
+  work();
+  if (helpers) {
+    std::unique_lock<std::mutex> lk(P.mu);
+    P.cv_done.wait(lk, [&] { return P.busy == 0; });
+    P.job = nullptr;
+  }
   if (err) return set_err(h, SX_EHIP, "rule copy to the host failed");
   return SX_OK;
 }
