@@ -3117,6 +3117,8 @@ struct sx_handle {
   // ... and a pinned host copy of them, filled segment by segment on
   // copy_stream while the later segments compute
   hipStream_t copy_stream = nullptr;
+  hipStream_t copy_stream2 = nullptr;  // the odd pieces' (SX_COPY_STREAMS=2, the default)
+  int copy_streams = 2;
   hipEvent_t seg_ev = nullptr;
   // the host copy arrives in pieces of kCopyPiece rules, each with an event
   // on copy_stream, so that sx_fetch_rules unpacks the first pieces while the
@@ -3272,14 +3274,16 @@ int queue_pieces(sx_handle *h, PendingRules &p) {
   for (uint64_t r0 = 0; r0 < p.n; r0 += kCopyPiece) {
     const uint64_t pn = std::min<uint64_t>(kCopyPiece, p.n - r0);
     const size_t pi = h->pieces.size();
+    // alternate pieces over two copy streams (two copy engines)
+    hipStream_t cs = (h->copy_streams == 2 && (pi & 1)) ? h->copy_stream2 : h->copy_stream;
     if (pi == h->piece_ev.size()) {
       hipEvent_t ev;
       HIPCHK(h, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
       h->piece_ev.push_back(ev);
     }
     HIPCHK(h, hipMemcpyAsync(h->h_rules + p.used + r0 * p.rw, h->d_rules + p.used + r0 * p.rw,
-                             (size_t)pn * p.rw * 4, hipMemcpyDeviceToHost, h->copy_stream));
-    HIPCHK(h, hipEventRecord(h->piece_ev[pi], h->copy_stream));
+                             (size_t)pn * p.rw * 4, hipMemcpyDeviceToHost, cs));
+    HIPCHK(h, hipEventRecord(h->piece_ev[pi], cs));
     h->pieces.emplace_back(p.first + r0, pn);
   }
   p.n = 0;
@@ -3985,10 +3989,12 @@ extern "C" int sx_open(int device, sx_handle **out) {
   if (const char *e = getenv("SX_EXH_GROUP")) h->exh_group = atoi(e);
   if (const char *e = getenv("SX_QCM_JOIN")) h->qcm_join = atoi(e) != 0;
   if (const char *e = getenv("SX_FP_SPLIT")) h->fp_split = atoi(e) != 0;
+  if (const char *e = getenv("SX_COPY_STREAMS")) h->copy_streams = atoi(e) == 1 ? 1 : 2;
   if (const char *e = getenv("SX_FETCH_THREADS")) h->fetch_threads = std::min(64, std::max(1, atoi(e)));
   if (const char *e = getenv("SX_SEG_LOG")) h->seg_log = std::min(24, std::max(16, atoi(e)));
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->copy_stream2, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->kev[0]) != hipSuccess || hipEventCreate(&h->kev[1]) != hipSuccess ||
       hipEventCreateWithFlags(&h->seg_ev, hipEventDisableTiming) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void **>(&h->pin), kPinBytes, hipHostMallocDefault) != hipSuccess) {
@@ -4011,6 +4017,7 @@ extern "C" int sx_close(sx_handle *h) {
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->copy_stream) (void)hipStreamSynchronize(h->copy_stream);
+  if (h->copy_stream2) (void)hipStreamSynchronize(h->copy_stream2);
   for (auto &b : h->bufs)
     if (b.p) (void)hipFree(b.p);
   if (h->d_l2) (void)hipFree(h->d_l2);
@@ -4026,6 +4033,7 @@ extern "C" int sx_close(sx_handle *h) {
   if (h->seg_ev) (void)hipEventDestroy(h->seg_ev);
   for (hipEvent_t e : h->piece_ev) (void)hipEventDestroy(e);
   if (h->copy_stream) (void)hipStreamDestroy(h->copy_stream);
+  if (h->copy_stream2) (void)hipStreamDestroy(h->copy_stream2);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return SX_OK;
@@ -4092,6 +4100,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
   h->rules.clear();
   h->empty_rules.clear();
   HIPCHK(h, hipStreamSynchronize(h->copy_stream));  // the last call's rule copies read d_rules
+  HIPCHK(h, hipStreamSynchronize(h->copy_stream2));
   h->d_rules_n = 0;
   h->pieces.clear();
   h->rules_on_device = !cfg->hits;
@@ -4814,6 +4823,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
           HIPCHK(h, hipMemcpyAsync(grown, h->d_rules, used * 4, hipMemcpyDeviceToDevice, h->stream));
         SYNC_WD("device sync");
         HIPCHK(h, hipStreamSynchronize(h->copy_stream));  // copies in flight read the old list
+        HIPCHK(h, hipStreamSynchronize(h->copy_stream2));
         if (h->d_rules) HIPCHK(h, hipFree(h->d_rules));
         h->d_rules = grown;
         h->d_rules_capw = cap2;
@@ -4822,6 +4832,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         // grow the pinned host copy (kept across calls, so this happens once
         // per size class)
         HIPCHK(h, hipStreamSynchronize(h->copy_stream));
+        HIPCHK(h, hipStreamSynchronize(h->copy_stream2));
         uint64_t cap2 = std::max<uint64_t>(std::max<uint64_t>(h->h_rules_capw * 2, need), 12u << 20);
         uint32_t *grown = nullptr;
         if (hipHostMalloc(&grown, cap2 * 4, hipHostMallocDefault) != hipSuccess) {
@@ -4849,6 +4860,7 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         // not idle while the host queues them
         HIPCHK(h, hipEventRecord(h->seg_ev, h->stream));
         HIPCHK(h, hipStreamWaitEvent(h->copy_stream, h->seg_ev, 0));
+        HIPCHK(h, hipStreamWaitEvent(h->copy_stream2, h->seg_ev, 0));
         pend = {used, nr, h->d_rules_n, rw};
       }
       h->d_rules_n += nr;
