@@ -2456,16 +2456,21 @@ __device__ __forceinline__ uint32_t fp_flag_diff(const uint2 *tfp, const uint2 *
 }
 
 // The tail's minimum passing rank per (target, pass), from the jobs' compact
-// records.
+// records.  comp: best holds the complement of the minimum (it starts at 0,
+// as rej does at round 0, and takes atomicMax of ~rank); else best starts
+// at kRankNone and takes atomicMin.
 __global__ __launch_bounds__(256) void k_bulk_min(const uint4 *jent, uint32_t nj, const uint32_t *fail,
                                                   const uint32_t *rank, const uint2 *tfp, const uint2 *cfp,
-                                                  uint32_t *best) {
+                                                  uint32_t *best, int comp) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nj || fail[i]) return;
   const uint4 r = jent[i];
   const uint32_t target = r.w, slot = r.z & 0x3FFFFFFFu, pass = r.z >> 30;
   if (pass == 2 && fp_flag_diff(tfp, cfp, target, slot) == 0) return;  // worker.go:211-213
-  atomicMin(&best[(uint64_t)target * 2 + (pass - 1)], rank[slot]);
+  if (comp)
+    atomicMax(&best[(uint64_t)target * 2 + (pass - 1)], ~rank[slot]);
+  else
+    atomicMin(&best[(uint64_t)target * 2 + (pass - 1)], rank[slot]);
 }
 
 // Rules of the (target, pass) entries the tail resolved; rank == slot.
@@ -2473,10 +2478,12 @@ __global__ __launch_bounds__(256) void k_bulk_emit(const uint32_t *best, uint32_
                                                    uint32_t tl, const uint2 *tfp, const uint2 *cfp,
                                                    const uint8_t *tbytes, const uint8_t *tcyc,
                                                    const uint8_t *cbytes, const uint8_t *ccyc, uint64_t tbase,
-                                                   sx_rule *rules, uint32_t *rule_flag) {
+                                                   sx_rule *rules, uint32_t *rule_flag, int comp) {
   uint32_t k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= 2 * nt || best[k] == kRankNone) return;
-  const uint32_t t = k >> 1, pass = (k & 1) + 1, slot = best[k];
+  if (k >= 2 * nt) return;
+  const uint32_t b = comp ? ~best[k] : best[k];  // comp: see k_bulk_min
+  if (b == kRankNone) return;
+  const uint32_t t = k >> 1, pass = (k & 1) + 1, slot = b;
   Job j;
   fill_job(j, targets, tl, nullptr, 0, t, slot, pass, 0);
   const uint32_t o = k;
@@ -4677,9 +4684,10 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
       if ((uint64_t)round > nc + 1) return set_err(h, SX_EHIP, "first-hit rounds did not converge");
       if (!cfg->hits && bulk_at >= 0 && round >= bulk_at) go_bulk = true;
       if (!cfg->hits && go_bulk) {
-        if (round == 0)  // the batch's minimum goes to cur, which still holds MidCheck's minima
-          hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur, nt * 2,
-                             kRankNone);
+        // the batch's minimum: in cur (kRankNone after a round's resolve), or at
+        // round 0, where cur still holds MidCheck's minima, as complements in
+        // rej (all 0 until a round rejects)
+        uint32_t *best = round == 0 ? rej : cur;
         HIPCHK(h, hipMemsetAsync(njobs_d, 0, 4, h->stream));
         uint32_t *bkey = nullptr, *bidx = nullptr;  // the jobs' group keys, written as they are planned
         if (h->exh_group) {
@@ -4709,10 +4717,10 @@ extern "C" int sx_enumerate(sx_handle *h, const sx_enum_cfg *cfg_in, sx_rule *ou
         if (rc) return rc;
         // cur holds kRankNone for every entry after a resolve: reuse it for the minimum
         hipLaunchKernelGGL(k_bulk_min, dim3(blocks_for(nj, 256)), dim3(256), 0, h->stream, (const uint4 *)jent, nj,
-                           (const uint32_t *)h->bufs[B_FAIL].p, rank, tfp, cfp, cur);
-        hipLaunchKernelGGL(k_bulk_emit, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, cur,
+                           (const uint32_t *)h->bufs[B_FAIL].p, rank, tfp, cfp, best, round == 0 ? 1 : 0);
+        hipLaunchKernelGGL(k_bulk_emit, dim3(blocks_for(nt * 2, 256)), dim3(256), 0, h->stream, best,
                            (uint32_t)nt, tw, (uint32_t)tl, tfp, cfp, tbytes, tcyc, cbytes, ccyc, tbase + s0, rules,
-                           rule_flag);
+                           rule_flag, round == 0 ? 1 : 0);
         HIPCHK(h, hipGetLastError());
         break;
       }
